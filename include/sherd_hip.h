@@ -1,0 +1,194 @@
+/*
+ * sherd_hip: the MI355X backend of sherd-refit's `Executor` boundary, as a C ABI.
+ *
+ * The reference's drop-in point is the Rust trait `Executor`
+ * (crates/sherd-core/src/executor/mod.rs:105-137) with its batch structs
+ * (crates/sherd-core/src/executor/batch.rs:42-371). Its implementations are CpuExecutor
+ * (executor/cpu.rs) and the wgpu GpuExecutor (crates/sherd-gpu/src/executor.rs:219-487), selected
+ * by sherd_backend::resolve (crates/sherd-backend/src/lib.rs:203-310). Every entry point below
+ * names the reference interface it replaces.
+ *
+ * Conventions:
+ *   - every function returns an int status: SHERD_OK (0) or a negative SHERD_E_* code; the
+ *     message of the last failure on the calling thread is sherd_last_error();
+ *   - all buffers are caller-owned host memory in the reference's own layouts: points and normals
+ *     are n x 3 f64 row-major (`&[[f64; 3]]`), poses are 4 x 4 f64 row-major (`Matrix4<f64>`
+ *     read row by row), meshes are f32 vertices and u32 faces (`RayScene` is f32);
+ *   - nothing computes on the CPU: a call that cannot run on the device fails with a status, and
+ *     the caller (the Rust shim) answers it with CpuExecutor, exactly as GpuExecutor delegates
+ *     (crates/sherd-gpu/src/executor.rs:19-33);
+ *   - a context serialises the calls made on it, so one context may be shared by threads (the
+ *     trait is Send + Sync, executor/mod.rs:103-104); results never depend on the caller's
+ *     thread or grouping.
+ */
+#ifndef SHERD_HIP_H
+#define SHERD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SHERD_OK = 0,
+  SHERD_E_ARG = -1,     /* a malformed argument (null pointer, size mismatch, bad enum) */
+  SHERD_E_HIP = -2,     /* the HIP runtime reported an error */
+  SHERD_E_NOMEM = -3,   /* a device allocation failed */
+  SHERD_E_LIMIT = -4,   /* the input exceeds a documented device limit */
+  SHERD_E_NODEV = -5    /* no usable gfx950 device */
+};
+
+typedef struct sherd_ctx sherd_ctx;
+typedef struct sherd_scene sherd_scene;
+
+/* One point's answer to R §6.4 (batch.rs:342-352 `InsideOutcome`, 8 bytes). */
+typedef struct {
+  float depth;     /* distance to the surface when inside, 0 otherwise */
+  uint32_t inside; /* 1 when inside */
+} sherd_inside_outcome;
+
+/* Per-method device counters (crates/sherd-gpu/src/executor.rs:50-354 `MethodStats`). */
+typedef struct {
+  uint64_t calls;      /* entry-point calls */
+  uint64_t items;      /* points, rays, poses or candidates processed */
+  uint64_t device_ns;  /* device time measured with HIP events around the kernels */
+} sherd_method_stats;
+
+enum {
+  SHERD_M_FIRST_HIT, SHERD_M_BOUNDED_DISTANCE, SHERD_M_INSIDE, SHERD_M_COARSE, SHERD_M_ICP,
+  SHERD_M_NEAREST, SHERD_M_VERIFY, SHERD_M_SCENE_BUILD, SHERD_M_CONE, SHERD_M_COUNT
+};
+
+/* ---- the executor ------------------------------------------------------------------------- */
+
+/* GpuExecutor::new + Gpu::open (crates/sherd-gpu/src/executor.rs:356, device.rs:528): binds the
+ * device and creates the context's stream. */
+int sherd_open(int device, sherd_ctx **out);
+void sherd_close(sherd_ctx *ctx);
+/* Executor::name (executor/mod.rs:107): "hip". */
+const char *sherd_name(void);
+const char *sherd_last_error(void);
+/* The device marketing name and gfx target, for `info` lines (sherd-backend/src/lib.rs:30). */
+int sherd_device_name(sherd_ctx *ctx, char *buf, size_t len);
+/* Counters of one method (SHERD_M_*). */
+int sherd_stats(sherd_ctx *ctx, int method, sherd_method_stats *out);
+
+/* ---- scenes (RayScene, crates/sherd-core/src/spatial/bvh.rs:36-82) -------------------------- */
+
+/* RayScene::of_mesh / of_subset: a device LBVH over f32 vertices V (nv x 3) and faces F (nf x 3).
+ * mask (nf bytes, nonzero keeps the face) may be NULL; a mask that keeps nothing falls back to the
+ * first face (bvh.rs:71-76). Face indices reported by queries are positions in the kept list. */
+int sherd_scene_build(sherd_ctx *ctx, const float *V, size_t nv, const uint32_t *F, size_t nf,
+                      const uint8_t *mask, sherd_scene **out);
+void sherd_scene_free(sherd_scene *scene);
+/* RayScene::n_faces and RayScene::aabb (bvh.rs:182-194). */
+int sherd_scene_info(const sherd_scene *scene, size_t *faces, float lo[3], float hi[3]);
+
+/* RayScene::first_hit (bvh.rs:84-91): n rays (origins and directions n x 3 f32); smallest t with
+ * ties to the lowest face; a miss is t = +inf, prim = 0xFFFFFFFF. retest != 0 retests
+ * near-degenerate hits in FP64. */
+int sherd_first_hit(sherd_ctx *ctx, const sherd_scene *scene, const float *orig, const float *dir,
+                    size_t n, int retest, float *t, uint32_t *prim);
+
+/* Executor::bounded_distance (executor/mod.rs:121; CpuExecutor at cpu.rs:84-113; DistBatch at
+ * batch.rs:285-310): points moved by T in f64, narrowed to f32, distance to the scene when at most
+ * max_dist. reduce = 0 (DistReduce::All): n values, +inf beyond the window. reduce = 1
+ * (DistReduce::Min): one value, the minimum of the window and every distance. */
+int sherd_bounded_distance(sherd_ctx *ctx, const sherd_scene *scene, const double *pts, size_t n,
+                           const double T[16], double max_dist, int reduce, double *out);
+
+/* Executor::inside (executor/mod.rs:124; cpu.rs:115-128, 182-188; InsideBatch at batch.rs:316):
+ * the AABB reject, three axis rays with a majority of odd crossing counts, the depth of an inside
+ * point. retest = 0 is the reference's arithmetic; 1 adds the FP64 retest of degenerate hits. */
+int sherd_inside(sherd_ctx *ctx, const sherd_scene *scene, const double *pts, size_t n,
+                 const double T[16], int retest, sherd_inside_outcome *out);
+
+/* ---- nearest neighbours (PointTree, crates/sherd-core/src/spatial/kdtree.rs) ---------------- */
+
+/* PointTree::nearest_below (kdtree.rs:182-192) over an f64 cloud (n x 3): for each of m queries
+ * the nearest point with ties to the lowest index, when sqrt(d^2) < bound; idx = -1 otherwise.
+ * d^2 is kiddo's ((dx*dx + dy*dy) + dz*dz) in f64. */
+int sherd_nearest_below(sherd_ctx *ctx, const double *cloud, size_t n, const double *q, size_t m,
+                        double bound, int64_t *idx, double *dist);
+
+/* ---- the coarse score (Executor::coarse_scores) --------------------------------------------- */
+
+/* Executor::coarse_scores (executor/mod.rs:115; cpu.rs:46-68, 136-169; CoarseBatch at
+ * batch.rs:148-176): poses are k x 12 f64, the rotation row-major then the translation (the
+ * `Poses::get` view, batch.rs:110-126, of both Split and Homogeneous poses). counts (may be NULL)
+ * gets the agree count per pose; scores gets f64(count) / n_probe, and 0 when the probe is empty. */
+int sherd_coarse_scores(sherd_ctx *ctx, const double *target_p, const double *target_n,
+                        size_t n_target, const double *probe_p, const double *probe_n,
+                        size_t n_probe, const double *poses, size_t k, double radius,
+                        double normal_agree, uint32_t *counts, double *scores);
+
+/* ---- one ICP rung (Executor::icp_rung) -------------------------------------------------------- */
+
+/* `Registration` (crates/sherd-core/src/matching/icp.rs:182-196). */
+typedef struct {
+  double transform[16]; /* row-major 4x4 */
+  double fitness;
+  double inlier_rmse;
+  uint64_t correspondences;
+  uint32_t iterations;
+  uint32_t converged;
+} sherd_registration;
+
+enum { SHERD_POINT_TO_POINT = 0, SHERD_POINT_TO_PLANE = 1 };
+
+/* Executor::icp_rung (executor/mod.rs:118; cpu.rs:70-82 -> icp::register, icp.rs:308-356;
+ * IcpBatch at batch.rs:188-238): k candidates from `inits` (k x 16), each up to max_iteration
+ * iterations of Open3D's ICP at `distance`, in f64. tgt_n may be NULL (a point-to-plane rung then
+ * makes identity updates, icp.rs:460-462). */
+int sherd_icp_rung(sherd_ctx *ctx, const double *src, size_t n_src, const double *tgt_p,
+                   const double *tgt_n, size_t n_tgt, const double *inits, size_t k,
+                   int estimation, double distance, int max_iteration, sherd_registration *out);
+
+/* ---- R §6 verification (crates/sherd-core/src/matching/verify.rs) ------------------------------ */
+
+typedef struct sherd_fragment sherd_fragment;
+
+/* One fragment as R §6 measures it (verify.rs:223-302 `Surfaces`): its fracture scene and whole
+ * mesh scene (borrowed handles; mesh may be NULL, which makes the penetration test unavailable),
+ * R §3.3.2's watertight verdict, R §3.4's fracture area, and its sample sets, all n x 3 f64:
+ * pf (fracture samples), s (surface samples), brk_p / brk_ns (breakline and shell normals),
+ * margin_p / margin_n (shell margin and normals). The arrays are copied to the device. */
+int sherd_fragment_create(sherd_ctx *ctx, const sherd_scene *fracture, const sherd_scene *mesh,
+                          int watertight, double frac_area, const double *pf, size_t n_pf,
+                          const double *s, size_t n_s, const double *brk_p, const double *brk_ns,
+                          size_t n_brk, const double *margin_p, const double *margin_n,
+                          size_t n_margin, sherd_fragment **out);
+void sherd_fragment_free(sherd_fragment *frag);
+
+/* A pair of fragments and its Scales (scales.rs:29-54, scales.json order: t, res, coarse, stage1,
+ * tight, facing, gap, seam, near, pen, nms, icp). */
+typedef struct {
+  const sherd_fragment *a;
+  const sherd_fragment *b;
+  double scales[12];
+} sherd_verify_pair;
+
+enum {
+  SHERD_S_TIGHT_A, SHERD_S_TIGHT_B, SHERD_S_TIGHT, SHERD_S_GAP_A, SHERD_S_GAP_B, SHERD_S_GAP,
+  SHERD_S_CONTACT_A, SHERD_S_CONTACT_B, SHERD_S_CONTACT, SHERD_S_SEAM, SHERD_S_GAP_LIMIT,
+  SHERD_S_TIGHT_DELTA, SHERD_S_CONT, SHERD_S_CONT_N, SHERD_S_PEN, SHERD_S_PEN_DEPTH,
+  SHERD_S_PEN_UNAVAILABLE, SHERD_S_PARTIAL, SHERD_S_BRK, SHERD_S_BRK_BEST, SHERD_S_COUNT
+};
+
+/* verify::verify (verify.rs:590-626) and verify::accept (verify.rs:632-638, Params::default) for
+ * n_cand candidates over many pairs in one batch: candidate i is pose cand_T[16 i ..] of pair
+ * cand_pair[i]. scores: n_cand x SHERD_S_COUNT doubles (the Scores fields in order; brk and
+ * brk_best are left 0 for the caller); accepted: n_cand bytes. full = 0 is R §5.6's cheap half.
+ * retest = 1 adds the FP64 retest of degenerate parity hits. This replaces the per-candidate
+ * `par_iter` over verify in Pair::stage2_batch (crates/sherd-core/src/matching/pair.rs:350-359). */
+int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_pairs,
+                       const uint32_t *cand_pair, const double *cand_T, size_t n_cand, int full,
+                       int retest, double *scores, uint8_t *accepted);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,514 @@
+// The C ABI (include/sherd_hip.h): contexts, handles, argument checks, transfers, counters.
+//
+// Host code only: every computation is a device kernel launched from the subsystem translation
+// units; this file moves caller buffers to and from the device and reports failures as statuses
+// (a failed call computes nothing on the CPU; the caller delegates, as GpuExecutor does in
+// crates/sherd-gpu/src/executor.rs:425-487).
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sherd_hip.h"
+#include "api_internal.hpp"
+#include "kernels.hpp"
+
+using namespace sherd;
+
+namespace {
+thread_local std::string g_error;
+}
+
+namespace sherd {
+
+int fail(int code, const std::string &msg) {
+  g_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  std::string m = std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+  g_error = m;
+  return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? SHERD_E_NOMEM : SHERD_E_HIP;
+}
+
+void *Ctx::scratch(int slot, size_t bytes, hipError_t *err) {
+  *err = hipSuccess;
+  Buf &b = bufs[slot];
+  if (b.n >= bytes && b.p) return b.p;
+  if (b.p) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+  }
+  size_t want = bytes < 256 ? 256 : bytes;
+  want += want / 4;
+  *err = hipMalloc(&b.p, want);
+  if (*err != hipSuccess) {
+    b.p = nullptr;
+    return nullptr;
+  }
+  b.n = want;
+  return b.p;
+}
+
+Timer::Timer(Ctx *c, int method, uint64_t items) : c_(c), m_(method) {
+  c_->stats[m_].calls += 1;
+  c_->stats[m_].items += items;
+  (void)hipEventRecord(c_->ev0, c_->stream);
+}
+
+void Timer::stop() {
+  if (done_) return;
+  done_ = true;
+  (void)hipEventRecord(c_->ev1, c_->stream);
+  if (hipEventSynchronize(c_->ev1) == hipSuccess) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, c_->ev0, c_->ev1) == hipSuccess) c_->stats[m_].device_ns += (uint64_t)(ms * 1e6);
+  }
+}
+
+}  // namespace sherd
+
+#define LOCK(ctx)                                                    \
+  if (!(ctx)) return fail(SHERD_E_ARG, "null context");             \
+  std::lock_guard<std::mutex> lock_((ctx)->mu);                      \
+  if (hipSetDevice((ctx)->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice")
+
+#define HIPCK(call, what)                                            \
+  do {                                                               \
+    hipError_t e_ = (call);                                          \
+    if (e_ != hipSuccess) return hip_fail(e_, what);                 \
+  } while (0)
+
+extern "C" {
+
+const char *sherd_name(void) { return "hip"; }
+const char *sherd_last_error(void) { return g_error.c_str(); }
+
+int sherd_open(int device, sherd_ctx **out) {
+  if (!out) return fail(SHERD_E_ARG, "null output");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(SHERD_E_NODEV, "no HIP device");
+  if (device < 0 || device >= count) return fail(SHERD_E_NODEV, "device index out of range");
+  hipDeviceProp_t prop;
+  HIPCK(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(SHERD_E_NODEV, std::string("built for gfx950, found ") + prop.gcnArchName);
+  HIPCK(hipSetDevice(device), "hipSetDevice");
+  Ctx *c = new Ctx();
+  c->device = device;
+  c->name = std::string(prop.name) + " (" + prop.gcnArchName + ", " + std::to_string(prop.multiProcessorCount) + " CUs)";
+  c->cus = prop.multiProcessorCount;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "stream/event creation");
+  }
+  *out = reinterpret_cast<sherd_ctx *>(c);
+  return SHERD_OK;
+}
+
+void sherd_close(sherd_ctx *ctx) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &b : c->bufs)
+    if (b.p) (void)hipFree(b.p);
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int sherd_device_name(sherd_ctx *ctx, char *buf, size_t len) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || !buf || len == 0) return fail(SHERD_E_ARG, "null argument");
+  std::snprintf(buf, len, "%s", c->name.c_str());
+  return SHERD_OK;
+}
+
+int sherd_stats(sherd_ctx *ctx, int method, sherd_method_stats *out) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || !out || method < 0 || method >= SHERD_M_COUNT) return fail(SHERD_E_ARG, "bad argument");
+  std::lock_guard<std::mutex> lock(c->mu);
+  *out = c->stats[method];
+  return SHERD_OK;
+}
+
+// ---- scenes --------------------------------------------------------------------------------------
+
+int sherd_scene_build(sherd_ctx *ctx, const float *V, size_t nv, const uint32_t *F, size_t nf,
+                      const uint8_t *mask, sherd_scene **out) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!out) return fail(SHERD_E_ARG, "null output");
+  *out = nullptr;
+  LOCK(c);
+  if (!V || !F || nf == 0 || nv == 0) return fail(SHERD_E_ARG, "a scene needs vertices and at least one face");
+  if (nf > (size_t)(INT_MAX / 2) || nv > (size_t)INT_MAX) return fail(SHERD_E_LIMIT, "mesh larger than 2^30 faces");
+  std::vector<uint32_t> kept;
+  kept.reserve(3 * nf);
+  for (size_t i = 0; i < nf; ++i) {
+    if (mask && !mask[i]) continue;
+    for (int k = 0; k < 3; ++k) {
+      if (F[3 * i + k] >= nv) return fail(SHERD_E_ARG, "face index out of range");
+      kept.push_back(F[3 * i + k]);
+    }
+  }
+  if (kept.empty()) {  // bvh.rs:71-76: the first face
+    for (int k = 0; k < 3; ++k) {
+      if (F[k] >= nv) return fail(SHERD_E_ARG, "face index out of range");
+      kept.push_back(F[k]);
+    }
+  }
+  int m = (int)(kept.size() / 3);
+  float *dV = nullptr;
+  uint32_t *dF = nullptr;
+  HIPCK(hipMalloc(&dV, sizeof(float) * 3 * nv), "hipMalloc vertices");
+  hipError_t e = hipMalloc(&dF, sizeof(uint32_t) * kept.size());
+  if (e != hipSuccess) {
+    (void)hipFree(dV);
+    return hip_fail(e, "hipMalloc faces");
+  }
+  sherd_scene *s = new sherd_scene();
+  s->ctx = c;
+  e = hipMemcpyAsync(dV, V, sizeof(float) * 3 * nv, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dF, kept.data(), sizeof(uint32_t) * kept.size(), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    Timer tm(c, SHERD_M_SCENE_BUILD, (uint64_t)m);
+    e = scene_build(dV, dF, m, &s->d, c->stream);
+    tm.stop();
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(dV);
+  (void)hipFree(dF);
+  if (e != hipSuccess) {
+    delete s;
+    return hip_fail(e, "scene build");
+  }
+  *out = s;
+  return SHERD_OK;
+}
+
+void sherd_scene_free(sherd_scene *scene) {
+  if (!scene) return;
+  std::lock_guard<std::mutex> lock(scene->ctx->mu);
+  (void)hipSetDevice(scene->ctx->device);
+  (void)hipStreamSynchronize(scene->ctx->stream);
+  scene_free(&scene->d);
+  delete scene;
+}
+
+int sherd_scene_info(const sherd_scene *scene, size_t *faces, float lo[3], float hi[3]) {
+  if (!scene) return fail(SHERD_E_ARG, "null scene");
+  if (faces) *faces = (size_t)scene->d.n;
+  for (int k = 0; k < 3; ++k) {
+    if (lo) lo[k] = scene->d.lo[k];
+    if (hi) hi[k] = scene->d.hi[k];
+  }
+  return SHERD_OK;
+}
+
+// ---- executor methods over a scene ---------------------------------------------------------------
+
+int sherd_first_hit(sherd_ctx *ctx, const sherd_scene *scene, const float *orig, const float *dir, size_t n,
+                    int retest, float *t, uint32_t *prim) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  LOCK(c);
+  if (!scene || scene->ctx != c) return fail(SHERD_E_ARG, "scene of another context");
+  if (n == 0) return SHERD_OK;
+  if (!orig || !dir || !t || !prim) return fail(SHERD_E_ARG, "null buffer");
+  if (n > (size_t)INT_MAX) return fail(SHERD_E_LIMIT, "more than 2^31 rays");
+  hipError_t e;
+  float *d_o = (float *)c->scratch(0, sizeof(float) * 3 * n, &e);
+  if (!d_o) return hip_fail(e, "scratch");
+  float *d_d = (float *)c->scratch(1, sizeof(float) * 3 * n, &e);
+  if (!d_d) return hip_fail(e, "scratch");
+  float *d_t = (float *)c->scratch(2, sizeof(float) * n, &e);
+  if (!d_t) return hip_fail(e, "scratch");
+  uint32_t *d_p = (uint32_t *)c->scratch(3, sizeof(uint32_t) * n, &e);
+  if (!d_p) return hip_fail(e, "scratch");
+  HIPCK(hipMemcpyAsync(d_o, orig, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->stream), "upload");
+  HIPCK(hipMemcpyAsync(d_d, dir, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->stream), "upload");
+  Timer tm(c, SHERD_M_FIRST_HIT, n);
+  HIPCK(launch_first_hit(view_of(scene->d), d_o, d_d, (int)n, retest, d_t, d_p, c->stream), "first_hit kernel");
+  tm.stop();
+  HIPCK(hipMemcpyAsync(t, d_t, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipMemcpyAsync(prim, d_p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  return SHERD_OK;
+}
+
+static bool pose_ok(const double *T, Pose *p) {
+  if (!T) return false;
+  for (int i = 0; i < 16; ++i) p->m[i] = T[i];
+  return true;
+}
+
+int sherd_bounded_distance(sherd_ctx *ctx, const sherd_scene *scene, const double *pts, size_t n,
+                           const double T[16], double max_dist, int reduce, double *out) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  LOCK(c);
+  Pose P;
+  if (!scene || scene->ctx != c) return fail(SHERD_E_ARG, "scene of another context");
+  if (!pose_ok(T, &P) || !out || (n && !pts)) return fail(SHERD_E_ARG, "null buffer");
+  if (reduce != 0 && reduce != 1) return fail(SHERD_E_ARG, "reduce must be 0 (All) or 1 (Min)");
+  if (n > (size_t)INT_MAX) return fail(SHERD_E_LIMIT, "more than 2^31 points");
+  float window = (float)max_dist;
+  if (reduce == 1) {
+    if (n == 0) {
+      out[0] = (double)window;
+      return SHERD_OK;
+    }
+  } else if (n == 0) {
+    return SHERD_OK;
+  }
+  hipError_t e;
+  double *d_p = (double *)c->scratch(0, sizeof(double) * 3 * n, &e);
+  if (!d_p) return hip_fail(e, "scratch");
+  HIPCK(hipMemcpyAsync(d_p, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream), "upload");
+  if (reduce == 0) {
+    float *d_o = (float *)c->scratch(1, sizeof(float) * n, &e);
+    if (!d_o) return hip_fail(e, "scratch");
+    Timer tm(c, SHERD_M_BOUNDED_DISTANCE, n);
+    HIPCK(launch_bounded(view_of(scene->d), d_p, (int)n, P, window, d_o, c->stream), "bounded kernel");
+    tm.stop();
+    std::vector<float> h(n);
+    HIPCK(hipMemcpyAsync(h.data(), d_o, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream), "download");
+    HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+    for (size_t i = 0; i < n; ++i) out[i] = (double)h[i];
+    return SHERD_OK;
+  }
+  unsigned int *d_b = (unsigned int *)c->scratch(1, sizeof(unsigned int), &e);
+  if (!d_b) return hip_fail(e, "scratch");
+  // The window starts at max_dist (f32); a negative or NaN window answers itself (bvh.rs:124).
+  unsigned int init = __builtin_bit_cast(unsigned int, window);
+  if (!(window > 0.0f)) {
+    out[0] = (double)window;
+    return SHERD_OK;
+  }
+  HIPCK(hipMemcpyAsync(d_b, &init, sizeof init, hipMemcpyHostToDevice, c->stream), "upload");
+  Timer tm(c, SHERD_M_BOUNDED_DISTANCE, n);
+  HIPCK(launch_bounded_min(view_of(scene->d), d_p, (int)n, P, d_b, c->stream), "bounded min kernel");
+  tm.stop();
+  unsigned int bits = 0;
+  HIPCK(hipMemcpyAsync(&bits, d_b, sizeof bits, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  out[0] = (double)__builtin_bit_cast(float, bits);
+  return SHERD_OK;
+}
+
+int sherd_inside(sherd_ctx *ctx, const sherd_scene *scene, const double *pts, size_t n, const double T[16],
+                 int retest, sherd_inside_outcome *out) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  LOCK(c);
+  Pose P;
+  if (!scene || scene->ctx != c) return fail(SHERD_E_ARG, "scene of another context");
+  if (n == 0) return SHERD_OK;
+  if (!pose_ok(T, &P) || !out || !pts) return fail(SHERD_E_ARG, "null buffer");
+  if (n > (size_t)INT_MAX) return fail(SHERD_E_LIMIT, "more than 2^31 points");
+  hipError_t e;
+  double *d_p = (double *)c->scratch(0, sizeof(double) * 3 * n, &e);
+  if (!d_p) return hip_fail(e, "scratch");
+  float *d_depth = (float *)c->scratch(1, sizeof(float) * n, &e);
+  if (!d_depth) return hip_fail(e, "scratch");
+  uint32_t *d_in = (uint32_t *)c->scratch(2, sizeof(uint32_t) * n, &e);
+  if (!d_in) return hip_fail(e, "scratch");
+  HIPCK(hipMemcpyAsync(d_p, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream), "upload");
+  Timer tm(c, SHERD_M_INSIDE, n);
+  HIPCK(launch_inside(view_of(scene->d), d_p, (int)n, P, retest, d_depth, d_in, c->stream), "inside kernel");
+  tm.stop();
+  std::vector<float> hd(n);
+  std::vector<uint32_t> hi(n);
+  HIPCK(hipMemcpyAsync(hd.data(), d_depth, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipMemcpyAsync(hi.data(), d_in, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  for (size_t i = 0; i < n; ++i) {
+    out[i].depth = hd[i];
+    out[i].inside = hi[i];
+  }
+  return SHERD_OK;
+}
+
+}  // extern "C"
+
+// ---- nearest neighbours, coarse score, ICP rung ---------------------------------------------------
+
+namespace {
+
+// Uploads n x 3 f64 into scratch slot `slot`.
+double *upload3(Ctx *c, int slot, const double *h, size_t n, hipError_t *e) {
+  double *d = (double *)c->scratch(slot, sizeof(double) * 3 * (n ? n : 1), e);
+  if (!d) return nullptr;
+  if (n) *e = hipMemcpyAsync(d, h, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream);
+  return *e == hipSuccess ? d : nullptr;
+}
+
+struct GridGuard {
+  DevGrid g;
+  ~GridGuard() { grid_free(&g); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int sherd_nearest_below(sherd_ctx *ctx, const double *cloud, size_t n, const double *q, size_t m, double bound,
+                        int64_t *idx, double *dist) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  LOCK(c);
+  if (m == 0) return SHERD_OK;
+  if (!q || !idx || !dist || (n && !cloud)) return fail(SHERD_E_ARG, "null buffer");
+  if (n > (size_t)INT_MAX / 2 || m > (size_t)INT_MAX) return fail(SHERD_E_LIMIT, "cloud or query set too large");
+  if (n == 0 || !(bound > 0.0)) {
+    for (size_t i = 0; i < m; ++i) {
+      idx[i] = -1;
+      dist[i] = __builtin_inf();
+    }
+    return SHERD_OK;
+  }
+  hipError_t e = hipSuccess;
+  double *dc = upload3(c, 0, cloud, n, &e);
+  if (!dc) return hip_fail(e, "upload cloud");
+  double *dq = upload3(c, 1, q, m, &e);
+  if (!dq) return hip_fail(e, "upload queries");
+  int64_t *di = (int64_t *)c->scratch(2, sizeof(int64_t) * m, &e);
+  if (!di) return hip_fail(e, "scratch");
+  double *dd = (double *)c->scratch(3, sizeof(double) * m, &e);
+  if (!dd) return hip_fail(e, "scratch");
+  GridGuard gg;
+  HIPCK(grid_build(dc, (int)n, bound, &gg.g, c->stream), "grid build");
+  Timer tm(c, SHERD_M_NEAREST, m);
+  HIPCK(launch_nearest(view_of(gg.g), dq, (int)m, bound, di, dd, c->stream), "nearest kernel");
+  tm.stop();
+  HIPCK(hipMemcpyAsync(idx, di, sizeof(int64_t) * m, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipMemcpyAsync(dist, dd, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  return SHERD_OK;
+}
+
+int sherd_coarse_scores(sherd_ctx *ctx, const double *target_p, const double *target_n, size_t n_target,
+                        const double *probe_p, const double *probe_n, size_t n_probe, const double *poses,
+                        size_t k, double radius, double normal_agree, uint32_t *counts, double *scores) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  LOCK(c);
+  if (k == 0) return SHERD_OK;
+  if (!poses || !scores) return fail(SHERD_E_ARG, "null buffer");
+  if (k > (size_t)INT_MAX / 64 || n_probe > (size_t)INT_MAX || n_target > (size_t)INT_MAX / 2)
+    return fail(SHERD_E_LIMIT, "batch too large");
+  if (n_probe == 0 || n_target == 0 || !(radius > 0.0)) {
+    // cpu.rs:47-55: an empty probe scores zero; no target or no radius finds no neighbour.
+    for (size_t h = 0; h < k; ++h) {
+      scores[h] = 0.0;
+      if (counts) counts[h] = 0;
+    }
+    return SHERD_OK;
+  }
+  if (!target_p || !target_n || !probe_p || !probe_n) return fail(SHERD_E_ARG, "null buffer");
+  hipError_t e = hipSuccess;
+  double *dtp = upload3(c, 0, target_p, n_target, &e);
+  if (!dtp) return hip_fail(e, "upload target");
+  double *dtn = upload3(c, 1, target_n, n_target, &e);
+  if (!dtn) return hip_fail(e, "upload target normals");
+  double *dpp = upload3(c, 2, probe_p, n_probe, &e);
+  if (!dpp) return hip_fail(e, "upload probe");
+  double *dpn = upload3(c, 3, probe_n, n_probe, &e);
+  if (!dpn) return hip_fail(e, "upload probe normals");
+  double *dposes = (double *)c->scratch(4, sizeof(double) * 12 * k, &e);
+  if (!dposes) return hip_fail(e, "scratch");
+  HIPCK(hipMemcpyAsync(dposes, poses, sizeof(double) * 12 * k, hipMemcpyHostToDevice, c->stream), "upload poses");
+  uint32_t *dcnt = (uint32_t *)c->scratch(5, sizeof(uint32_t) * k, &e);
+  if (!dcnt) return hip_fail(e, "scratch");
+  double *dsc = (double *)c->scratch(6, sizeof(double) * k, &e);
+  if (!dsc) return hip_fail(e, "scratch");
+  GridGuard gg;
+  HIPCK(grid_build(dtp, (int)n_target, radius, &gg.g, c->stream), "grid build");
+  Timer tm(c, SHERD_M_COARSE, k);
+  HIPCK(launch_coarse(view_of(gg.g), dtn, dpp, dpn, (int)n_probe, dposes, (int)k, radius, normal_agree, dcnt, dsc,
+                      c->stream),
+        "coarse kernel");
+  tm.stop();
+  if (counts) HIPCK(hipMemcpyAsync(counts, dcnt, sizeof(uint32_t) * k, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipMemcpyAsync(scores, dsc, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  return SHERD_OK;
+}
+
+int sherd_icp_rung(sherd_ctx *ctx, const double *src, size_t n_src, const double *tgt_p, const double *tgt_n,
+                   size_t n_tgt, const double *inits, size_t k, int estimation, double distance, int max_iteration,
+                   sherd_registration *out) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  LOCK(c);
+  if (k == 0) return SHERD_OK;
+  if (!inits || !out) return fail(SHERD_E_ARG, "null buffer");
+  if (estimation != SHERD_POINT_TO_POINT && estimation != SHERD_POINT_TO_PLANE)
+    return fail(SHERD_E_ARG, "unknown estimation");
+  if (max_iteration < 0) return fail(SHERD_E_ARG, "negative iteration cap");
+  if (n_src == 0 || n_tgt == 0) {
+    // Registration::unmoved (icp.rs:198-209).
+    for (size_t i = 0; i < k; ++i) {
+      std::memcpy(out[i].transform, inits + 16 * i, sizeof(double) * 16);
+      out[i].fitness = out[i].inlier_rmse = 0.0;
+      out[i].correspondences = 0;
+      out[i].iterations = 0;
+      out[i].converged = 0;
+    }
+    return SHERD_OK;
+  }
+  if (!src || !tgt_p) return fail(SHERD_E_ARG, "null buffer");
+  if (n_src > (size_t)INT_MAX / 4 || n_tgt > (size_t)INT_MAX / 2 || k > (size_t)INT_MAX)
+    return fail(SHERD_E_LIMIT, "batch too large");
+  int est = estimation;
+  if (est == SHERD_POINT_TO_PLANE && !tgt_n) est = 2;  // identity updates (icp.rs:460-462)
+  hipError_t e = hipSuccess;
+  double *dsrc = upload3(c, 0, src, n_src, &e);
+  if (!dsrc) return hip_fail(e, "upload source");
+  double *dtp = upload3(c, 1, tgt_p, n_tgt, &e);
+  if (!dtp) return hip_fail(e, "upload target");
+  double *dtn = nullptr;
+  if (tgt_n) {
+    dtn = upload3(c, 2, tgt_n, n_tgt, &e);
+    if (!dtn) return hip_fail(e, "upload normals");
+  }
+  double *dinit = (double *)c->scratch(3, sizeof(double) * 16 * k, &e);
+  if (!dinit) return hip_fail(e, "scratch");
+  HIPCK(hipMemcpyAsync(dinit, inits, sizeof(double) * 16 * k, hipMemcpyHostToDevice, c->stream), "upload inits");
+  double *dmoved = (double *)c->scratch(4, sizeof(double) * 3 * n_src * k, &e);
+  if (!dmoved) return hip_fail(e, "scratch moved points");
+  uint32_t *dcorr = (uint32_t *)c->scratch(5, sizeof(uint32_t) * n_src * k, &e);
+  if (!dcorr) return hip_fail(e, "scratch correspondences");
+  double *dout = (double *)c->scratch(6, sizeof(double) * 21 * k, &e);
+  if (!dout) return hip_fail(e, "scratch");
+  GridGuard gg;
+  double reach = distance > 0.0 ? distance : 1.0;
+  HIPCK(grid_build(dtp, (int)n_tgt, reach, &gg.g, c->stream), "grid build");
+  Timer tm(c, SHERD_M_ICP, k);
+  HIPCK(launch_icp_rung(view_of(gg.g), dtp, dtn, dsrc, (int)n_src, dinit, (int)k, est, distance, max_iteration,
+                        dmoved, dcorr, dout, c->stream),
+        "icp kernel");
+  tm.stop();
+  std::vector<double> h(21 * k);
+  HIPCK(hipMemcpyAsync(h.data(), dout, sizeof(double) * 21 * k, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  for (size_t i = 0; i < k; ++i) {
+    const double *o = &h[21 * i];
+    std::memcpy(out[i].transform, o, sizeof(double) * 16);
+    out[i].fitness = o[16];
+    out[i].inlier_rmse = o[17];
+    out[i].correspondences = (uint64_t)o[18];
+    out[i].iterations = (uint32_t)o[19];
+    out[i].converged = (uint32_t)o[20];
+  }
+  return SHERD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// Internal host structures behind the C ABI handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+
+#include "../../include/sherd_hip.h"
+#include "kernels.hpp"
+
+namespace sherd {
+
+struct Ctx {
+  int device = 0;
+  int cus = 0;
+  std::string name;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::mutex mu;
+  sherd_method_stats stats[SHERD_M_COUNT] = {};
+  struct Buf {
+    void *p = nullptr;
+    size_t n = 0;
+  };
+  Buf bufs[24];
+  // A grow-only device buffer per slot, reused across calls (the context is serialised).
+  void *scratch(int slot, size_t bytes, hipError_t *err);
+};
+
+// Accumulates one method's call count, item count and device time (HIP events on the stream).
+class Timer {
+ public:
+  Timer(Ctx *c, int method, uint64_t items);
+  ~Timer() { stop(); }
+  void stop();
+
+ private:
+  Ctx *c_;
+  int m_;
+  bool done_ = false;
+};
+
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+}  // namespace sherd
+
+struct sherd_scene {
+  sherd::Ctx *ctx = nullptr;
+  sherd::DevScene d;
+};
+
+struct sherd_fragment {
+  sherd::Ctx *ctx = nullptr;
+  const sherd_scene *frac = nullptr;
+  const sherd_scene *mesh = nullptr;
+  int watertight = 0;
+  double frac_area = 0.0;
+  double *pf = nullptr, *s = nullptr, *brk_p = nullptr, *brk_ns = nullptr, *margin_p = nullptr, *margin_n = nullptr;
+  int n_pf = 0, n_s = 0, n_brk = 0, n_margin = 0;
+  sherd::DevGrid margin_grid;  // rebuilt when a batch needs a larger reach
+  double margin_reach = 0.0;
+};

@@ -1,0 +1,255 @@
+// C ABI for the R §6 verification: fragment handles and the batched verify entry point.
+// Host code only: descriptors, uploads, chunking and argument checks; the statistics run in
+// verify.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/sherd_hip.h"
+#include "api_internal.hpp"
+#include "kernels.hpp"
+
+using namespace sherd;
+
+#define HIPCK(call, what)                            \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+namespace {
+
+// Copies n x 3 f64 to a fresh device allocation (nullptr for n == 0).
+hipError_t upload_new(const double *h, size_t n, double **d, hipStream_t s) {
+  *d = nullptr;
+  if (n == 0) return hipSuccess;
+  hipError_t e = hipMalloc((void **)d, sizeof(double) * 3 * n);
+  if (e != hipSuccess) return e;
+  return hipMemcpyAsync(*d, h, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s);
+}
+
+void free_fragment(sherd_fragment *f) {
+  double *ptrs[] = {f->pf, f->s, f->brk_p, f->brk_ns, f->margin_p, f->margin_n};
+  for (double *p : ptrs)
+    if (p) (void)hipFree(p);
+  grid_free(&f->margin_grid);
+  delete f;
+}
+
+uint32_t pow2_at_least(uint64_t x) {
+  uint32_t c = 2;
+  while (c < x) c <<= 1;
+  return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sherd_fragment_create(sherd_ctx *ctx, const sherd_scene *fracture, const sherd_scene *mesh, int watertight,
+                          double frac_area, const double *pf, size_t n_pf, const double *s, size_t n_s,
+                          const double *brk_p, const double *brk_ns, size_t n_brk, const double *margin_p,
+                          const double *margin_n, size_t n_margin, sherd_fragment **out) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || !out) return fail(SHERD_E_ARG, "null argument");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lock(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
+  if (!fracture || fracture->ctx != c || (mesh && mesh->ctx != c)) return fail(SHERD_E_ARG, "scene of another context");
+  if ((n_pf && !pf) || (n_s && !s) || (n_brk && (!brk_p || !brk_ns)) || (n_margin && (!margin_p || !margin_n)))
+    return fail(SHERD_E_ARG, "null sample array");
+  const size_t LIM = (size_t)1 << 28;
+  if (n_pf > LIM || n_s > LIM || n_brk > LIM || n_margin > LIM) return fail(SHERD_E_LIMIT, "sample set too large");
+  sherd_fragment *f = new sherd_fragment();
+  f->ctx = c;
+  f->frac = fracture;
+  f->mesh = mesh;
+  f->watertight = watertight ? 1 : 0;
+  f->frac_area = frac_area;
+  f->n_pf = (int)n_pf;
+  f->n_s = (int)n_s;
+  f->n_brk = (int)n_brk;
+  f->n_margin = (int)n_margin;
+  hipError_t e = upload_new(pf, n_pf, &f->pf, c->stream);
+  if (e == hipSuccess) e = upload_new(s, n_s, &f->s, c->stream);
+  if (e == hipSuccess) e = upload_new(brk_p, n_brk, &f->brk_p, c->stream);
+  if (e == hipSuccess) e = upload_new(brk_ns, n_brk, &f->brk_ns, c->stream);
+  if (e == hipSuccess) e = upload_new(margin_p, n_margin, &f->margin_p, c->stream);
+  if (e == hipSuccess) e = upload_new(margin_n, n_margin, &f->margin_n, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    free_fragment(f);
+    return hip_fail(e, "fragment upload");
+  }
+  *out = f;
+  return SHERD_OK;
+}
+
+void sherd_fragment_free(sherd_fragment *frag) {
+  if (!frag) return;
+  std::lock_guard<std::mutex> lock(frag->ctx->mu);
+  (void)hipSetDevice(frag->ctx->device);
+  (void)hipStreamSynchronize(frag->ctx->stream);
+  free_fragment(frag);
+}
+
+int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_pairs, const uint32_t *cand_pair,
+                       const double *cand_T, size_t n_cand, int full, int retest, double *scores,
+                       uint8_t *accepted) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c) return fail(SHERD_E_ARG, "null context");
+  std::lock_guard<std::mutex> lock(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
+  if (n_cand == 0) return SHERD_OK;
+  if (!pairs || !cand_pair || !cand_T || !scores || !accepted) return fail(SHERD_E_ARG, "null buffer");
+  if (n_pairs > (size_t)INT_MAX || n_cand > (size_t)INT_MAX / 4) return fail(SHERD_E_LIMIT, "batch too large");
+  // Fragments of the batch, and the reach each one's margin grid must cover (its largest `near`
+  // as side A, verify.rs:456-462).
+  std::map<const sherd_fragment *, int> fidx;
+  std::vector<sherd_fragment *> frags;
+  std::vector<double> need;
+  for (size_t p = 0; p < n_pairs; ++p) {
+    const sherd_fragment *ab[2] = {pairs[p].a, pairs[p].b};
+    for (int k = 0; k < 2; ++k) {
+      if (!ab[k] || ab[k]->ctx != c) return fail(SHERD_E_ARG, "fragment missing or of another context");
+      if (!fidx.count(ab[k])) {
+        fidx[ab[k]] = (int)frags.size();
+        frags.push_back(const_cast<sherd_fragment *>(ab[k]));
+        need.push_back(0.0);
+      }
+    }
+    double nr = pairs[p].scales[8];
+    int ia = fidx[pairs[p].a];
+    if (nr > need[ia]) need[ia] = nr;
+  }
+  for (size_t i = 0; i < n_cand; ++i)
+    if (cand_pair[i] >= n_pairs) return fail(SHERD_E_ARG, "candidate names a pair out of range");
+  int max_pf = 0, max_s = 0, max_margin = 0;
+  std::vector<FragDesc> fd(frags.size());
+  for (size_t i = 0; i < frags.size(); ++i) {
+    sherd_fragment *f = frags[i];
+    if (f->n_margin > 0 && need[i] > 0.0 && (f->margin_grid.pts == nullptr || f->margin_reach < need[i])) {
+      grid_free(&f->margin_grid);
+      HIPCK(grid_build(f->margin_p, f->n_margin, need[i], &f->margin_grid, c->stream), "margin grid");
+      f->margin_reach = need[i];
+    }
+    FragDesc &d = fd[i];
+    std::memset(&d, 0, sizeof d);
+    d.frac = view_of(f->frac->d);
+    d.has_mesh = f->mesh != nullptr;
+    if (f->mesh) d.mesh = view_of(f->mesh->d);
+    d.watertight = f->watertight;
+    d.frac_area = f->frac_area;
+    d.pf = f->pf;
+    d.s = f->s;
+    d.brk_p = f->brk_p;
+    d.brk_ns = f->brk_ns;
+    d.margin_p = f->margin_p;
+    d.margin_n = f->margin_n;
+    d.n_pf = f->n_pf;
+    d.n_s = f->n_s;
+    d.n_brk = f->n_brk;
+    d.n_margin = f->n_margin;
+    d.margin = view_of(f->margin_grid);
+    max_pf = std::max(max_pf, f->n_pf);
+    max_s = std::max(max_s, f->n_s);
+    max_margin = std::max(max_margin, f->n_margin);
+  }
+  std::vector<PairDesc> pd(n_pairs);
+  for (size_t p = 0; p < n_pairs; ++p) {
+    const sherd_fragment *a = pairs[p].a, *b = pairs[p].b;
+    pd[p].a = fidx[a];
+    pd[p].b = fidx[b];
+    pd[p].pen_ok = (a->mesh && b->mesh && a->watertight && b->watertight) ? 1 : 0;
+    for (int k = 0; k < 12; ++k) pd[p].sc[k] = pairs[p].scales[k];
+  }
+  hipError_t e;
+  FragDesc *dfd = (FragDesc *)c->scratch(10, sizeof(FragDesc) * fd.size(), &e);
+  if (!dfd) return hip_fail(e, "scratch");
+  PairDesc *dpd = (PairDesc *)c->scratch(11, sizeof(PairDesc) * pd.size(), &e);
+  if (!dpd) return hip_fail(e, "scratch");
+  HIPCK(hipMemcpyAsync(dfd, fd.data(), sizeof(FragDesc) * fd.size(), hipMemcpyHostToDevice, c->stream), "upload");
+  HIPCK(hipMemcpyAsync(dpd, pd.data(), sizeof(PairDesc) * pd.size(), hipMemcpyHostToDevice, c->stream), "upload");
+  Timer tm(c, SHERD_M_VERIFY, n_cand);
+  // Chunks bounded by the key and voxel-table scratch they need (256 MiB of keys per chunk).
+  const size_t KEY_BUDGET = (size_t)1 << 25;
+  size_t begin = 0;
+  std::vector<CandDesc> cd;
+  std::vector<double> hs;
+  std::vector<uint8_t> ha;
+  std::vector<CandAcc> hacc;
+  while (begin < n_cand) {
+    cd.clear();
+    size_t keys = 0, table = 0;
+    size_t end = begin;
+    while (end < n_cand) {
+      const PairDesc &p = pd[cand_pair[end]];
+      const FragDesc &A = fd[p.a], &B = fd[p.b];
+      size_t need_keys = (size_t)A.n_pf + B.n_pf + 2 * (size_t)B.n_margin;
+      if (end > begin && keys + need_keys > KEY_BUDGET) break;
+      CandDesc x;
+      std::memset(&x, 0, sizeof x);
+      x.pair = (int)cand_pair[end];
+      x.seg[0] = keys;
+      x.seg[1] = keys + A.n_pf;
+      x.seg[2] = keys + A.n_pf + B.n_pf;
+      x.seg[3] = x.seg[2] + B.n_margin;
+      keys += need_keys;
+      uint32_t cap = pow2_at_least(2 * (uint64_t)(A.n_brk > 0 ? A.n_brk : 1));
+      x.seam_mask = cap - 1;
+      x.seam_off = table;
+      table += cap;
+      std::memcpy(x.T, cand_T + 16 * end, sizeof x.T);
+      cd.push_back(x);
+      ++end;
+    }
+    size_t nc = cd.size();
+    CandDesc *dcd = (CandDesc *)c->scratch(12, sizeof(CandDesc) * nc, &e);
+    if (!dcd) return hip_fail(e, "scratch");
+    CandAcc *dacc = (CandAcc *)c->scratch(13, sizeof(CandAcc) * nc, &e);
+    if (!dacc) return hip_fail(e, "scratch");
+    uint64_t *dkeys = (uint64_t *)c->scratch(14, sizeof(uint64_t) * (keys ? keys : 1), &e);
+    if (!dkeys) return hip_fail(e, "scratch keys");
+    uint64_t *dtab = (uint64_t *)c->scratch(15, sizeof(uint64_t) * table, &e);
+    if (!dtab) return hip_fail(e, "scratch seam tables");
+    double *dsc = (double *)c->scratch(16, sizeof(double) * VS_COUNT * nc, &e);
+    if (!dsc) return hip_fail(e, "scratch");
+    uint8_t *dac = (uint8_t *)c->scratch(17, nc, &e);
+    if (!dac) return hip_fail(e, "scratch");
+    HIPCK(hipMemcpyAsync(dcd, cd.data(), sizeof(CandDesc) * nc, hipMemcpyHostToDevice, c->stream), "upload");
+    VerifyBatch vb;
+    vb.frag = dfd;
+    vb.pair = dpd;
+    vb.cand = dcd;
+    vb.acc = dacc;
+    vb.keys = dkeys;
+    vb.seam_table = dtab;
+    vb.scores = dsc;
+    vb.accepted = dac;
+    vb.n_cand = (int)nc;
+    vb.full = full ? 1 : 0;
+    vb.retest = retest ? 1 : 0;
+    HIPCK(launch_verify(vb, max_pf, max_s, max_margin, c->stream), "verify kernels");
+    hs.resize(VS_COUNT * nc);
+    ha.resize(nc);
+    hacc.resize(nc);
+    HIPCK(hipMemcpyAsync(hs.data(), dsc, sizeof(double) * VS_COUNT * nc, hipMemcpyDeviceToHost, c->stream), "download");
+    HIPCK(hipMemcpyAsync(ha.data(), dac, nc, hipMemcpyDeviceToHost, c->stream), "download");
+    HIPCK(hipMemcpyAsync(hacc.data(), dacc, sizeof(CandAcc) * nc, hipMemcpyDeviceToHost, c->stream), "download");
+    HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+    for (size_t i = 0; i < nc; ++i)
+      if (hacc[i].flags & FLAG_VOXEL_RANGE) return fail(SHERD_E_LIMIT, "a seam voxel coordinate exceeds 2^20");
+    std::memcpy(scores + VS_COUNT * begin, hs.data(), sizeof(double) * VS_COUNT * nc);
+    std::memcpy(accepted + begin, ha.data(), nc);
+    begin = end;
+  }
+  tm.stop();
+  return SHERD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,224 @@
+// Shared device arithmetic and layouts for the MI355X backend.
+//
+// Every parity-relevant expression here is written in one fixed order and the library is built
+// with -ffp-contract=off, so no multiply-add is fused: the f32 ray and distance arithmetic, the f64
+// pose application and the f64 neighbour distances are the same IEEE operations as the CPU
+// executor's (crates/sherd-core/src/spatial/bvh.rs:199-221, executor/cpu.rs:192-200,
+// spatial/kdtree.rs), one operation at a time.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SH_HD __host__ __device__ __forceinline__
+
+namespace sherd {
+
+constexpr int WAVE = 64;
+
+// ---- f32 triples with glam's association (parry3d's `Vector`) ---------------------------------
+struct V3 {
+  float x, y, z;
+};
+SH_HD V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+SH_HD V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+SH_HD V3 scale(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+SH_HD float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+SH_HD V3 cross(V3 a, V3 b) { return V3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+SH_HD float len2(V3 a) { return (a.x * a.x + a.y * a.y) + a.z * a.z; }
+// std::min's definition: the first operand unless the second is strictly smaller.
+SH_HD float mnf(float a, float b) { return (b < a) ? b : a; }
+SH_HD float absf(float a) { return __builtin_fabsf(a); }
+
+constexpr float MT_EPS = 1e-9f;       // bvh.rs:204
+constexpr float BARY_TOL = 1e-6f;     // barycentric margin that triggers the FP64 retest
+constexpr float DET_TOL = 1e-6f;      // relative determinant that triggers the FP64 retest
+
+// Moller-Trumbore exactly as bvh.rs:199-221, with the optional FP64 retest of a near-degenerate
+// case (|det| < 1e-6 |e1|_1 |e2|_1 |d|_1, or min(u, v, 1-u-v) within 1e-6 of zero).
+// Returns 1 and writes *t for an accepted hit.
+SH_HD int mt_hit(V3 o, V3 d, V3 v0, V3 v1, V3 v2, int retest, float *t) {
+  V3 e1 = sub(v1, v0);
+  V3 e2 = sub(v2, v0);
+  V3 pvec = cross(d, e2);
+  float det = dot(e1, pvec);
+  bool near = false;
+  if (retest) {
+    float sc = (absf(e1.x) + absf(e1.y) + absf(e1.z)) * (absf(e2.x) + absf(e2.y) + absf(e2.z)) *
+               (absf(d.x) + absf(d.y) + absf(d.z));
+    near = absf(det) < DET_TOL * sc;
+  }
+  if (!near) {
+    if (absf(det) < MT_EPS) return 0;
+    float inv = 1.0f / det;
+    V3 tvec = sub(o, v0);
+    float u = inv * dot(tvec, pvec);
+    V3 qvec = cross(tvec, e1);
+    float v = inv * dot(d, qvec);
+    if (retest) {
+      float w = (1.0f - u) - v;
+      float m = mnf(u, mnf(v, w));
+      if (!(absf(m) < BARY_TOL)) {
+        if (!(m >= BARY_TOL)) return 0;
+        float tt = inv * dot(e2, qvec);
+        if (!(tt > MT_EPS)) return 0;
+        *t = tt;
+        return 1;
+      }
+      near = true;
+    } else {
+      if (!(u >= 0.0f && u <= 1.0f)) return 0;
+      if (v < 0.0f || u + v > 1.0f) return 0;
+      float tt = inv * dot(e2, qvec);
+      if (!(tt > MT_EPS)) return 0;
+      *t = tt;
+      return 1;
+    }
+  }
+  // FP64 retest from the same f32 data.
+  double ox = o.x, oy = o.y, oz = o.z, dx = d.x, dy = d.y, dz = d.z;
+  double ax = v0.x, ay = v0.y, az = v0.z;
+  double e1x = (double)v1.x - ax, e1y = (double)v1.y - ay, e1z = (double)v1.z - az;
+  double e2x = (double)v2.x - ax, e2y = (double)v2.y - ay, e2z = (double)v2.z - az;
+  double px = dy * e2z - e2y * dz, py = dz * e2x - e2z * dx, pz = dx * e2y - e2x * dy;
+  double dt = e1x * px + e1y * py + e1z * pz;
+  if (dt == 0.0 || !(dt - dt == 0.0)) return 0;
+  double inv = 1.0 / dt;
+  double tx = ox - ax, ty = oy - ay, tz = oz - az;
+  double u = inv * (tx * px + ty * py + tz * pz);
+  if (!(u >= 0.0 && u <= 1.0)) return 0;
+  double qx = ty * e1z - e1y * tz, qy = tz * e1x - e1z * tx, qz = tx * e1y - e1x * ty;
+  double v = inv * (dx * qx + dy * qy + dz * qz);
+  if (v < 0.0 || u + v > 1.0) return 0;
+  double tt = inv * (e2x * qx + e2y * qy + e2z * qz);
+  if (!(tt > 1e-9)) return 0;
+  *t = (float)tt;
+  return 1;
+}
+
+// Squared distance from p to the segment a + s (b - a), s in [0, 1], given ap = p - a, ab = b - a.
+SH_HD float seg_d2(V3 ap, V3 ab) {
+  float t = dot(ab, ap);
+  if (t <= 0.0f) return len2(ap);
+  float l = dot(ab, ab);
+  if (t >= l) return len2(sub(ap, ab));
+  float s = t / l;
+  return len2(sub(ap, scale(ab, s)));
+}
+
+// Squared point-triangle distance in f32: the Voronoi regions of the triangle (after Ericson,
+// Real-Time Collision Detection 5.1.5), offsets from the nearest vertex, the interior through the
+// plane equation s^2 / |n|^2.
+SH_HD float tri_d2(V3 p, V3 a, V3 b, V3 c) {
+  V3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+  float d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) return len2(ap);
+  V3 bp = sub(p, b);
+  float d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0.0f && d4 <= d3) return len2(bp);
+  float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    float v = d1 / (d1 - d3);
+    return len2(sub(ap, scale(ab, v)));
+  }
+  V3 cp = sub(p, c);
+  float d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) return len2(cp);
+  float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    float w = d2 / (d2 - d6);
+    return len2(sub(ap, scale(ac, w)));
+  }
+  float va = d3 * d6 - d5 * d4;
+  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+    float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    return len2(sub(bp, scale(sub(c, b), w)));
+  }
+  V3 n = cross(ab, ac);
+  float nn = dot(n, n);
+  if (!(nn > 0.0f)) {
+    float e = mnf(seg_d2(ap, ab), seg_d2(ap, ac));
+    return mnf(e, seg_d2(bp, sub(c, b)));
+  }
+  float s = dot(ap, n);
+  return (s * s) / nn;
+}
+
+// executor/cpu.rs:192-200: T.p in f64, left to right, then narrowed once to f32.
+SH_HD V3 moved_f32(const double *T, double px, double py, double pz) {
+  float x = (float)(T[0] * px + T[1] * py + T[2] * pz + T[3]);
+  float y = (float)(T[4] * px + T[5] * py + T[6] * pz + T[7]);
+  float z = (float)(T[8] * px + T[9] * py + T[10] * pz + T[11]);
+  return V3{x, y, z};
+}
+
+// ---- BVH layout --------------------------------------------------------------------------------
+// Nodes 0..n-2 are internal (node i splits sorted leaves i | i+1), nodes n-1..2n-2 are leaves.
+// `a` is the left child (internal) or the triangle slot (leaf); `esc` is the next node in
+// pre-order after this node's subtree, or -1. 32 bytes, one dwordx4 pair per node.
+struct Node {
+  float lo[3];
+  int a;
+  float hi[3];
+  int esc;
+};
+
+// A triangle in leaf order: the three f32 vertices and the scene-local face index.
+struct Tri {
+  float v[9];
+  uint32_t face;
+  uint32_t pad[2];
+};
+
+struct SceneView {
+  const Node *nodes;
+  const Tri *tris;
+  int n;         // triangles
+  int root;      // root node index
+  float lo[3];   // aabb of the scene's vertices (RayScene::aabb)
+  float hi[3];
+  float absmax;  // largest |coordinate| of the scene
+};
+
+SH_HD V3 tri_v(const Tri &t, int k) { return V3{t.v[3 * k], t.v[3 * k + 1], t.v[3 * k + 2]}; }
+
+// The pruning slack: ~80 ulps of the largest coordinate in play, more than the f32 rounding of
+// any distance or hit parameter computed above, so a box can never hide the triangle that attains
+// the computed minimum (the oracle prunes with the same bound).
+SH_HD float slack_of(float absmax, V3 p) {
+  float m = absmax;
+  float ax = absf(p.x), ay = absf(p.y), az = absf(p.z);
+  m = m < ax ? ax : m;
+  m = m < ay ? ay : m;
+  m = m < az ? az : m;
+  return 1e-5f * m + 1e-30f;
+}
+
+// Squared distance from p to the node's box grown by pad (conservative).
+SH_HD float box_d2(const Node &nd, V3 p, float pad) {
+  float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
+  float l0 = nd.lo[0] - pad, l1 = nd.lo[1] - pad, l2 = nd.lo[2] - pad;
+  float h0 = nd.hi[0] + pad, h1 = nd.hi[1] + pad, h2 = nd.hi[2] + pad;
+  if (p.x < l0) g0 = l0 - p.x; else if (p.x > h0) g0 = p.x - h0;
+  if (p.y < l1) g1 = l1 - p.y; else if (p.y > h1) g1 = p.y - h1;
+  if (p.z < l2) g2 = l2 - p.z; else if (p.z > h2) g2 = p.z - h2;
+  return (g0 * g0 + g1 * g1) + g2 * g2;
+}
+
+// ---- f64 neighbour distance (kiddo's SquaredEuclidean fold: (dx^2 + dy^2) + dz^2) --------------
+SH_HD double d2_f64(double qx, double qy, double qz, double px, double py, double pz) {
+  double ex = qx - px, ey = qy - py, ez = qz - pz;
+  return (ex * ex + ey * ey) + ez * ez;
+}
+
+// f64::total_cmp order as an unsigned key, and back.
+SH_HD uint64_t total_key(double v) {
+  uint64_t u = __builtin_bit_cast(uint64_t, v);
+  return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+SH_HD double from_total_key(uint64_t k) {
+  uint64_t u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __builtin_bit_cast(double, u);
+}
+
+}  // namespace sherd

@@ -1,0 +1,79 @@
+// Host-side declarations of the device launchers (one translation unit per subsystem).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+#include "grid.hpp"
+#include "verify.hpp"
+
+namespace sherd {
+
+// A 4x4 pose passed by value to a kernel (row-major f64).
+struct Pose {
+  double m[16];
+};
+
+// ---- LBVH (bvh.hip) ----------------------------------------------------------------------------
+struct DevScene {
+  Node *nodes = nullptr;
+  Tri *tris = nullptr;
+  int n = 0;
+  int root = -1;
+  float lo[3] = {0, 0, 0};
+  float hi[3] = {0, 0, 0};
+  float absmax = 0;
+};
+
+// Builds the tree over m device faces (dF, 3 u32 each, indexing dV's f32 triples). Synchronises
+// the stream before returning (the root and the bounds come back to the host).
+hipError_t scene_build(const float *dV, const uint32_t *dF, int m, DevScene *out, hipStream_t s);
+void scene_free(DevScene *s);
+SceneView view_of(const DevScene &d);
+
+hipError_t launch_first_hit(const SceneView &sv, const float *orig, const float *dir, int n, int retest, float *t,
+                            uint32_t *prim, hipStream_t s);
+hipError_t launch_inside(const SceneView &sv, const double *pts, int n, const Pose &T, int retest, float *depth,
+                         uint32_t *inside, hipStream_t s);
+hipError_t launch_bounded(const SceneView &sv, const double *pts, int n, const Pose &T, float window, float *out,
+                          hipStream_t s);
+hipError_t launch_bounded_min(const SceneView &sv, const double *pts, int n, const Pose &T, unsigned int *best,
+                              hipStream_t s);
+
+// ---- f64 neighbour grid (grid.hip) ------------------------------------------------------------
+struct DevGrid {
+  double *pts = nullptr;
+  uint32_t *idx = nullptr;
+  uint64_t *keys = nullptr;
+  uint2 *cells = nullptr;
+  uint32_t mask = 0;
+  int n = 0;
+  double origin[3] = {0, 0, 0};
+  double cell = 1.0;
+};
+
+// Builds the grid over n device points (3 f64 each) for queries of reach <= `reach`.
+// Synchronises the stream (the bounding box comes back to size the cells).
+hipError_t grid_build(const double *dp, int n, double reach, DevGrid *g, hipStream_t s);
+void grid_free(DevGrid *g);
+GridView view_of(const DevGrid &d);
+
+hipError_t launch_nearest(const GridView &g, const double *q, int m, double bound, int64_t *idx, double *dist,
+                          hipStream_t s);
+hipError_t launch_coarse(const GridView &g, const double *tn, const double *pp, const double *pn, int np,
+                         const double *poses, int k, double radius, double agree, uint32_t *counts, double *scores,
+                         hipStream_t s);
+
+// ---- ICP rung (icp.hip) --------------------------------------------------------------------------
+// One workgroup per candidate runs every iteration of the rung (icp::register, icp.rs:308-356).
+// moved: k * ns * 3 f64 scratch; corr: k * ns u32 scratch; out: k * 21 doubles.
+hipError_t launch_icp_rung(const GridView &g, const double *tp, const double *tn, const double *src, int ns,
+                           const double *inits, int k, int estimation, double distance, int max_iter,
+                           double *moved, uint32_t *corr, double *out, hipStream_t s);
+
+// ---- verification (verify.hip) -------------------------------------------------------------------
+// max_pf / max_s / max_margin: the largest per-fragment sample counts in the batch (grid sizes).
+hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_margin, hipStream_t s);
+
+}  // namespace sherd

@@ -1,0 +1,423 @@
+// R §6 verification on the device: verify::verify (crates/sherd-core/src/matching/verify.rs:590-626)
+// and verify::accept (:632-638) for a batch of candidates over many pairs at once.
+//
+// The statistics are counts, medians, a maximum and a minimum, so none depends on the order the
+// per-point results arrive in: counts are atomic increments, medians are exact selections (radix
+// select on the f64 total order), the deepest excursion and the closest approach are atomic
+// max/min on the bits of non-negative floats. The per-point values themselves are the executor's
+// arithmetic (bvh_query.hpp, grid.hpp), so every score equals the CPU restatement's bit for bit.
+//
+// Kernels, each over all candidates of the batch:
+//   k_prep     pose inverse per candidate (verify.rs:739-798), counters cleared
+//   k_frac     R §6.1: both sides' fracture samples against the other's fracture scene
+//   k_seam     R §6.2: A's breakline against the moved B breakline, voxel set of the seam points
+//   k_cont     R §6.3: B's margin against A's margin grid, steps and normal agreements
+//   k_pen      R §6.4: surface samples inside the other mesh, deeper-than-pen count, deepest
+//   k_pen_min  R §6.4's fallback when nothing is inside: the closest approach
+//   k_median   the four medians (numpy semantics, verify.rs:693-700)
+//   k_final    Scores and the accept verdict
+#include <hip/hip_runtime.h>
+
+#include "bvh_query.hpp"
+#include "grid.hpp"
+#include "kernels.hpp"
+#include "linalg.hpp"
+#include "verify.hpp"
+
+namespace sherd {
+namespace {
+
+enum { S_T, S_RES, S_COARSE, S_STAGE1, S_TIGHT, S_FACING, S_GAP, S_SEAM, S_NEAR, S_PEN, S_NMS, S_ICP };
+
+constexpr int VT = 256;
+
+__device__ __forceinline__ void apply_f64(const double *T, const double *p, double *o) {
+  o[0] = T[0] * p[0] + T[1] * p[1] + T[2] * p[2] + T[3];
+  o[1] = T[4] * p[0] + T[5] * p[1] + T[6] * p[2] + T[7];
+  o[2] = T[8] * p[0] + T[9] * p[1] + T[10] * p[2] + T[11];
+}
+__device__ __forceinline__ void rotate_f64(const double *T, const double *p, double *o) {
+  o[0] = T[0] * p[0] + T[1] * p[1] + T[2] * p[2];
+  o[1] = T[4] * p[0] + T[5] * p[1] + T[6] * p[2];
+  o[2] = T[8] * p[0] + T[9] * p[1] + T[10] * p[2];
+}
+__device__ __forceinline__ double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+__global__ void k_prep(VerifyBatch vb) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= vb.n_cand) return;
+  CandDesc &cd = vb.cand[c];
+  pose_inverse(cd.T, cd.Ti);
+  CandAcc &a = vb.acc[c];
+  for (int s = 0; s < 2; ++s) {
+    a.facing[s] = a.tight[s] = a.touching[s] = 0;
+    a.inside[s] = a.deeper[s] = 0;
+    a.deepest[s] = 0u;
+    a.closest[s] = 0x7f7fffffu;  // f32::MAX, the window of the fallback (verify.rs:570-577)
+  }
+  a.n_cont = 0;
+  a.seam_cells = 0;
+  a.flags = 0;
+}
+
+// R §6.1 one_side inputs (verify.rs:318-349): side 0 is A's samples moved by T^-1 against B's
+// fracture, side 1 is B's samples moved by T against A's fracture.
+__global__ void __launch_bounds__(VT) k_frac(VerifyBatch vb) {
+  int cs = blockIdx.y;
+  int c = cs >> 1, side = cs & 1;
+  const CandDesc &cd = vb.cand[c];
+  const PairDesc &pd = vb.pair[cd.pair];
+  const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
+  const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
+  int i = blockIdx.x * VT + threadIdx.x;
+  bool valid = i < own.n_pf;
+  bool facing = false, tight = false, touching = false;
+  double d64 = 0.0;
+  if (valid) {
+    const double *T = side == 0 ? cd.Ti : cd.T;
+    const double *p = own.pf + 3 * (size_t)i;
+    V3 q = moved_f32(T, p[0], p[1], p[2]);
+    float d;
+    float window = (float)pd.sc[S_FACING];
+    d64 = bounded_distance(other.frac, q, window, &d) ? (double)d : __builtin_inf();
+    facing = d64 < pd.sc[S_FACING];
+    tight = facing && d64 < pd.sc[S_TIGHT];
+    touching = d64 < 2.0 * pd.sc[S_TIGHT];
+  }
+  CandAcc &acc = vb.acc[c];
+  if (facing) {
+    uint32_t slot = atomicAdd(&acc.facing[side], 1u);
+    vb.keys[cd.seg[side] + slot] = total_key(d64);
+  }
+  uint32_t nt = (uint32_t)__popcll(__ballot(tight)), nh = (uint32_t)__popcll(__ballot(touching));
+  if ((threadIdx.x & 63) == 0) {
+    if (nt) atomicAdd(&acc.tight[side], nt);
+    if (nh) atomicAdd(&acc.touching[side], nh);
+  }
+}
+
+// R §6.2 seam_points / seam_cells (verify.rs:378-442): for each point of A's breakline, the nearest
+// moved point of B's breakline (f64, ties to the lowest index); a seam point when it is strictly
+// inside sc.seam and the two shell normals agree; the score counts distinct t/3 voxels of them.
+__global__ void __launch_bounds__(VT) k_seam(VerifyBatch vb) {
+  int c = blockIdx.x;
+  const CandDesc &cd = vb.cand[c];
+  const PairDesc &pd = vb.pair[cd.pair];
+  const FragDesc &A = vb.frag[pd.a];
+  const FragDesc &B = vb.frag[pd.b];
+  __shared__ double sp[SEAM_TILE * 3];
+  __shared__ double sn[SEAM_TILE * 3];
+  __shared__ uint32_t s_cells;
+  unsigned long long *table = (unsigned long long *)(vb.seam_table + cd.seam_off);
+  uint32_t mask = cd.seam_mask;
+  for (uint32_t s = threadIdx.x; s <= mask; s += VT) table[s] = ~0ull;
+  if (threadIdx.x == 0) s_cells = 0;
+  __syncthreads();
+  double voxel = pd.sc[S_T] / 3.0;
+  double seam = pd.sc[S_SEAM];
+  // Each thread owns A's points t, t + VT, ...; its running best survives across the tiles.
+  constexpr int PER = 16;
+  for (int base = 0; base < A.n_brk; base += VT * PER) {
+    double best[PER];
+    uint32_t bj[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      best[r] = __builtin_inf();
+      bj[r] = 0xFFFFFFFFu;
+    }
+    for (int tile = 0; tile < B.n_brk; tile += SEAM_TILE) {
+      int m = B.n_brk - tile < SEAM_TILE ? B.n_brk - tile : SEAM_TILE;
+      __syncthreads();
+      for (int j = threadIdx.x; j < m; j += VT) {
+        apply_f64(cd.T, B.brk_p + 3 * (size_t)(tile + j), sp + 3 * j);
+        rotate_f64(cd.T, B.brk_ns + 3 * (size_t)(tile + j), sn + 3 * j);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < PER; ++r) {
+        int i = base + r * VT + threadIdx.x;
+        if (i >= A.n_brk) continue;
+        const double *q = A.brk_p + 3 * (size_t)i;
+        double q0 = q[0], q1 = q[1], q2 = q[2];
+        for (int j = 0; j < m; ++j) {
+          double d2 = d2_f64(q0, q1, q2, sp[3 * j], sp[3 * j + 1], sp[3 * j + 2]);
+          if (d2 < best[r]) {  // ascending j: the first minimum is the lowest index
+            best[r] = d2;
+            bj[r] = (uint32_t)(tile + j);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      int i = base + r * VT + threadIdx.x;
+      if (i >= A.n_brk || bj[r] == 0xFFFFFFFFu) continue;
+      if (!(seam > 0.0) || !(sqrt(best[r]) < seam)) continue;
+      double nb[3];
+      rotate_f64(cd.T, B.brk_ns + 3 * (size_t)bj[r], nb);
+      if (!(dot3(A.brk_ns + 3 * (size_t)i, nb) > 0.7)) continue;
+      const double *p = A.brk_p + 3 * (size_t)i;
+      int64_t x = (int64_t)floor(p[0] / voxel), y = (int64_t)floor(p[1] / voxel), z = (int64_t)floor(p[2] / voxel);
+      const int64_t H = 1ll << 20;
+      if (x < -H || x >= H || y < -H || y >= H || z < -H || z >= H) {
+        atomicOr(&vb.acc[c].flags, FLAG_VOXEL_RANGE);
+        continue;
+      }
+      uint64_t key = (uint64_t)(x + H) | ((uint64_t)(y + H) << 21) | ((uint64_t)(z + H) << 42);
+      uint32_t slot = grid_hash(key, mask);
+      for (uint32_t probe = 0; probe <= mask; ++probe) {
+        unsigned long long prev = atomicCAS(&table[slot], ~0ull, (unsigned long long)key);
+        if (prev == ~0ull) {
+          atomicAdd(&s_cells, 1u);
+          break;
+        }
+        if (prev == key) break;
+        slot = (slot + 1) & mask;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) vb.acc[c].seam_cells = s_cells;
+}
+
+// R §6.3 continuity_scores (verify.rs:450-474).
+__global__ void __launch_bounds__(VT) k_cont(VerifyBatch vb) {
+  int c = blockIdx.y;
+  const CandDesc &cd = vb.cand[c];
+  const PairDesc &pd = vb.pair[cd.pair];
+  const FragDesc &A = vb.frag[pd.a];
+  const FragDesc &B = vb.frag[pd.b];
+  int i = blockIdx.x * VT + threadIdx.x;
+  if (i >= B.n_margin || A.n_margin == 0) return;
+  double moved[3];
+  apply_f64(cd.T, B.margin_p + 3 * (size_t)i, moved);
+  uint32_t j;
+  double dist;
+  if (!grid_nearest_below(A.margin, moved[0], moved[1], moved[2], pd.sc[S_NEAR], &j, &dist)) return;
+  const double *np = A.margin_p + 3 * (size_t)j, *nn = A.margin_n + 3 * (size_t)j;
+  double delta[3] = {moved[0] - np[0], moved[1] - np[1], moved[2] - np[2]};
+  double step = fabs(dot3(delta, nn));
+  double turned[3];
+  rotate_f64(cd.T, B.margin_n + 3 * (size_t)i, turned);
+  double agree = dot3(turned, nn);
+  uint32_t slot = atomicAdd(&vb.acc[c].n_cont, 1u);
+  vb.keys[cd.seg[2] + slot] = total_key(step);
+  vb.keys[cd.seg[3] + slot] = total_key(agree);
+}
+
+// R §6.4 one_penetration's first pass (verify.rs:541-566): side 0 is A's surface samples moved by
+// T^-1 against B's mesh, side 1 is B's moved by T against A's mesh.
+__global__ void __launch_bounds__(VT) k_pen(VerifyBatch vb) {
+  int cs = blockIdx.y;
+  int c = cs >> 1, side = cs & 1;
+  const CandDesc &cd = vb.cand[c];
+  const PairDesc &pd = vb.pair[cd.pair];
+  const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
+  const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
+  if (!pd.pen_ok) return;
+  int i = blockIdx.x * VT + threadIdx.x;
+  bool in = false, deeper = false;
+  float depth = 0.0f;
+  if (i < own.n_s) {
+    const double *T = side == 0 ? cd.Ti : cd.T;
+    const double *p = own.s + 3 * (size_t)i;
+    V3 q = moved_f32(T, p[0], p[1], p[2]);
+    in = inside_depth(other.mesh, q, vb.retest, &depth);
+    deeper = in && (double)depth > pd.sc[S_PEN];
+  }
+  CandAcc &acc = vb.acc[c];
+  if (in) atomicMax(&acc.deepest[side], __float_as_uint(depth));
+  uint32_t ni = (uint32_t)__popcll(__ballot(in)), nd = (uint32_t)__popcll(__ballot(deeper));
+  if ((threadIdx.x & 63) == 0) {
+    if (ni) atomicAdd(&acc.inside[side], ni);
+    if (nd) atomicAdd(&acc.deeper[side], nd);
+  }
+}
+
+// verify.rs:567-578: nothing inside, so min(sd) is the closest approach, a running minimum whose
+// window shrinks to the best so far (DistReduce::Min, cpu.rs:100-111), from f32::MAX.
+__global__ void __launch_bounds__(VT) k_pen_min(VerifyBatch vb) {
+  int cs = blockIdx.y;
+  int c = cs >> 1, side = cs & 1;
+  const CandDesc &cd = vb.cand[c];
+  const PairDesc &pd = vb.pair[cd.pair];
+  if (!pd.pen_ok) return;
+  CandAcc &acc = vb.acc[c];
+  if (acc.inside[side] != 0) return;
+  const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
+  const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
+  int i = blockIdx.x * VT + threadIdx.x;
+  if (i >= own.n_s) return;
+  const double *T = side == 0 ? cd.Ti : cd.T;
+  const double *p = own.s + 3 * (size_t)i;
+  V3 q = moved_f32(T, p[0], p[1], p[2]);
+  float window = __uint_as_float(__hip_atomic_load(&acc.closest[side], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  float d;
+  if (bounded_distance(other.mesh, q, window, &d)) atomicMin(&acc.closest[side], __float_as_uint(d));
+}
+
+// The k-th smallest key (0-based) of n by an 8-pass MSD radix select.
+__device__ uint64_t select_kth(const uint64_t *keys, int n, int k, uint32_t *hist, uint64_t *s_prefix, int *s_k) {
+  uint64_t prefix = 0, pmask = 0;
+  if (threadIdx.x == 0) *s_k = k;
+  for (int pass = 0; pass < 8; ++pass) {
+    int shift = 56 - 8 * pass;
+    for (int b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      uint64_t key = keys[i];
+      if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int kk = *s_k;
+      int d = 0;
+      for (; d < 255; ++d) {
+        if (kk < (int)hist[d]) break;
+        kk -= (int)hist[d];
+      }
+      *s_k = kk;
+      *s_prefix = prefix | ((uint64_t)d << shift);
+    }
+    __syncthreads();
+    prefix = *s_prefix;
+    pmask |= (255ull << shift);
+    __syncthreads();
+  }
+  return prefix;
+}
+
+// Rust's f64::midpoint.
+__device__ double midpoint(double a, double b) {
+  const double LO = 2.2250738585072014e-308 * 2.0;
+  const double HI = 1.7976931348623157e308 / 2.0;
+  double aa = fabs(a), ab = fabs(b);
+  if (aa <= HI && ab <= HI) return (a + b) / 2.0;
+  if (aa < LO) return a + (b / 2.0);
+  if (ab < LO) return (a / 2.0) + b;
+  return (a / 2.0) + (b / 2.0);
+}
+
+// verify.rs:693-700 numpy's median over each candidate's four segments.
+__global__ void __launch_bounds__(VT) k_median(VerifyBatch vb) {
+  int c = blockIdx.x >> 2, seg = blockIdx.x & 3;
+  CandAcc &acc = vb.acc[c];
+  int n = seg < 2 ? (int)acc.facing[seg] : (int)acc.n_cont;
+  __shared__ uint32_t hist[256];
+  __shared__ uint64_t s_prefix;
+  __shared__ int s_k;
+  __shared__ uint64_t lds_keys[MEDIAN_LDS];
+  if (n == 0) {
+    if (threadIdx.x == 0) acc.med[seg] = __builtin_nan("");
+    return;
+  }
+  const uint64_t *keys = vb.keys + vb.cand[c].seg[seg];
+  if (n <= MEDIAN_LDS) {
+    for (int i = threadIdx.x; i < n; i += VT) lds_keys[i] = keys[i];
+    __syncthreads();
+    keys = lds_keys;
+  }
+  double med;
+  if (n % 2 == 1) {
+    med = from_total_key(select_kth(keys, n, n / 2, hist, &s_prefix, &s_k));
+  } else {
+    double lo = from_total_key(select_kth(keys, n, n / 2 - 1, hist, &s_prefix, &s_k));
+    double hi = from_total_key(select_kth(keys, n, n / 2, hist, &s_prefix, &s_k));
+    med = midpoint(lo, hi);
+  }
+  if (threadIdx.x == 0) acc.med[seg] = med;
+}
+
+// Scores (verify.rs:73-160, 590-626) and accept (verify.rs:632-638) per candidate.
+__global__ void k_final(VerifyBatch vb) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= vb.n_cand) return;
+  const CandDesc &cd = vb.cand[c];
+  const PairDesc &pd = vb.pair[cd.pair];
+  const FragDesc &A = vb.frag[pd.a];
+  const FragDesc &B = vb.frag[pd.b];
+  const CandAcc &acc = vb.acc[c];
+  const double *sc = pd.sc;
+  double side[2][3];
+  const FragDesc *own[2] = {&A, &B};
+  for (int s = 0; s < 2; ++s) {
+    if (acc.facing[s] < 20) {  // MIN_FACING (verify.rs:61)
+      side[s][0] = 0.0;
+      side[s][1] = 1.0;
+      side[s][2] = 0.0;
+      continue;
+    }
+    side[s][0] = (double)acc.tight[s] / (double)acc.facing[s];
+    side[s][1] = acc.med[s] / sc[S_T];
+    double share = (double)acc.touching[s] / (double)own[s]->n_pf;
+    side[s][2] = share * own[s]->frac_area / (sc[S_T] * sc[S_T]);
+  }
+  double *o = vb.scores + (size_t)VS_COUNT * c;
+  for (int k = 0; k < VS_COUNT; ++k) o[k] = 0.0;
+  o[VS_TIGHT_A] = side[0][0];
+  o[VS_TIGHT_B] = side[1][0];
+  o[VS_TIGHT] = fmin(side[0][0], side[1][0]);
+  o[VS_GAP_A] = side[0][1];
+  o[VS_GAP_B] = side[1][1];
+  o[VS_GAP] = fmax(side[0][1], side[1][1]);
+  o[VS_CONTACT_A] = side[0][2];
+  o[VS_CONTACT_B] = side[1][2];
+  o[VS_CONTACT] = fmin(side[0][2], side[1][2]);
+  o[VS_SEAM] = acc.seam_cells == 0 ? 0.0 : (double)acc.seam_cells / 3.0;
+  o[VS_GAP_LIMIT] = sc[S_GAP] / sc[S_T];
+  o[VS_TIGHT_DELTA] = sc[S_TIGHT] / sc[S_T];
+  o[VS_CONT] = 1.0;
+  o[VS_CONT_N] = -1.0;
+  if (!vb.full) {
+    o[VS_PARTIAL] = 1.0;
+  } else {
+    if (A.n_margin > 0 && B.n_margin > 0 && acc.n_cont > 20) {  // MIN_NEAR (verify.rs:63)
+      o[VS_CONT] = acc.med[2] / sc[S_T];
+      o[VS_CONT_N] = acc.med[3];
+    }
+    if (!pd.pen_ok) {
+      o[VS_PEN_UNAVAILABLE] = 1.0;
+    } else {
+      double pen[2], mn[2];
+      for (int s = 0; s < 2; ++s) {
+        int n = own[s]->n_s;
+        if (n == 0) {
+          pen[s] = 0.0;
+          mn[s] = 0.0;
+          continue;
+        }
+        pen[s] = (double)acc.deeper[s] / (double)n;
+        mn[s] = acc.inside[s] > 0 ? -(double)__uint_as_float(acc.deepest[s]) : (double)__uint_as_float(acc.closest[s]);
+      }
+      o[VS_PEN] = fmax(pen[0], pen[1]);
+      o[VS_PEN_DEPTH] = fmax(-mn[0], -mn[1]) / sc[S_T];
+    }
+  }
+  bool ok = o[VS_TIGHT] >= 0.25 && o[VS_GAP] * sc[S_T] <= sc[S_GAP] && o[VS_PEN] <= 0.005 && o[VS_SEAM] >= 3.0 &&
+            o[VS_CONT_N] >= 0.8;
+  vb.accepted[c] = ok ? 1 : 0;
+}
+
+inline int blocks_for(long n, int t) { return (int)((n + t - 1) / t); }
+
+}  // namespace
+
+hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_margin, hipStream_t s) {
+  if (vb.n_cand <= 0) return hipSuccess;
+  int nc = vb.n_cand;
+  hipLaunchKernelGGL(k_prep, dim3(blocks_for(nc, 64)), dim3(64), 0, s, vb);
+  if (max_pf > 0) hipLaunchKernelGGL(k_frac, dim3(blocks_for(max_pf, VT), 2 * nc), dim3(VT), 0, s, vb);
+  hipLaunchKernelGGL(k_seam, dim3(nc), dim3(VT), 0, s, vb);
+  if (vb.full) {
+    if (max_margin > 0) hipLaunchKernelGGL(k_cont, dim3(blocks_for(max_margin, VT), nc), dim3(VT), 0, s, vb);
+    if (max_s > 0) {
+      hipLaunchKernelGGL(k_pen, dim3(blocks_for(max_s, VT), 2 * nc), dim3(VT), 0, s, vb);
+      hipLaunchKernelGGL(k_pen_min, dim3(blocks_for(max_s, VT), 2 * nc), dim3(VT), 0, s, vb);
+    }
+  }
+  hipLaunchKernelGGL(k_median, dim3(4 * nc), dim3(VT), 0, s, vb);
+  hipLaunchKernelGGL(k_final, dim3(blocks_for(nc, 64)), dim3(64), 0, s, vb);
+  return hipGetLastError();
+}
+
+}  // namespace sherd

@@ -1,0 +1,79 @@
+// Device-side descriptors of a verification batch: fragments, pairs, candidates, per-candidate
+// accumulators and the Scores layout (crates/sherd-core/src/matching/verify.rs:73-160).
+#pragma once
+
+#include "common.hpp"
+#include "grid.hpp"
+
+namespace sherd {
+
+// Scores as 20 doubles, in the reference's field order.
+enum {
+  VS_TIGHT_A, VS_TIGHT_B, VS_TIGHT, VS_GAP_A, VS_GAP_B, VS_GAP, VS_CONTACT_A, VS_CONTACT_B, VS_CONTACT, VS_SEAM,
+  VS_GAP_LIMIT, VS_TIGHT_DELTA, VS_CONT, VS_CONT_N, VS_PEN, VS_PEN_DEPTH, VS_PEN_UNAVAILABLE, VS_PARTIAL, VS_BRK,
+  VS_BRK_BEST, VS_COUNT
+};
+
+constexpr int SEAM_TILE = 1024;     // moved B breakline points staged in LDS per pass (48 KiB)
+constexpr int MEDIAN_LDS = 12288;   // a median segment up to this many keys is selected from LDS
+constexpr uint32_t FLAG_VOXEL_RANGE = 1u;  // a seam voxel coordinate beyond +-2^20
+
+// One fragment as R §6 reads it (verify.rs:223-255 `Surfaces`).
+struct FragDesc {
+  SceneView frac;
+  SceneView mesh;
+  int has_mesh;
+  int watertight;
+  double frac_area;
+  const double *pf;
+  const double *s;
+  const double *brk_p;
+  const double *brk_ns;
+  const double *margin_p;
+  const double *margin_n;
+  int n_pf, n_s, n_brk, n_margin;
+  GridView margin;  // over margin_p, cell at least the largest `near` of the batch
+};
+
+struct PairDesc {
+  int a, b;
+  int pen_ok;     // both meshes present and watertight (verify.rs:490-493)
+  double sc[12];  // Scales in scales.json order
+};
+
+struct CandDesc {
+  int pair;
+  uint32_t seam_mask;  // seam voxel table capacity - 1
+  size_t seam_off;     // into VerifyBatch::seam_table
+  size_t seg[4];       // key segments: facing A, facing B, steps, agreements
+  double T[16];
+  double Ti[16];
+};
+
+struct CandAcc {
+  uint32_t facing[2], tight[2], touching[2];
+  uint32_t inside[2], deeper[2];
+  uint32_t deepest[2];  // bits of the largest inside depth (non-negative f32)
+  uint32_t closest[2];  // bits of the closest approach when nothing is inside
+  uint32_t n_cont;
+  uint32_t seam_cells;
+  uint32_t flags;
+  uint32_t pad;
+  double med[4];
+};
+
+struct VerifyBatch {
+  const FragDesc *frag;
+  const PairDesc *pair;
+  CandDesc *cand;
+  CandAcc *acc;
+  uint64_t *keys;        // median segments (f64 total-order keys)
+  uint64_t *seam_table;  // per-candidate voxel hash sets
+  double *scores;        // VS_COUNT per candidate
+  uint8_t *accepted;
+  int n_cand;
+  int full;
+  int retest;
+};
+
+}  // namespace sherd

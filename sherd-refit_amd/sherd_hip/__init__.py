@@ -1,0 +1,317 @@
+"""Python face of the MI355X backend: ctypes over the C ABI in include/sherd_hip.h.
+
+`HipExecutor` mirrors the reference's `Executor` trait (crates/sherd-core/src/executor/mod.rs:105-137)
+method for method: `name`, `coarse_scores`, `icp_rung`, `bounded_distance`, `inside`, plus the
+scene and ray entry points the trait leaves to preprocessing (`first_hit`, `cone_votes`) and the
+batched forms. Arrays are numpy arrays in the reference's layouts (n x 3 f64 points, 4 x 4 f64
+poses, f32 meshes).
+
+There is no CPU fallback: if libsherd_hip.so is missing or no gfx950 device is present, opening an
+executor raises.
+"""
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsherd_hip.so")
+
+SHERD_M = dict(first_hit=0, bounded_distance=1, inside=2, coarse=3, icp=4, nearest=5, verify=6,
+               scene_build=7, cone=8)
+
+_lib = None
+
+
+class HipError(RuntimeError):
+    """A nonzero status from the C ABI, with sherd_last_error()'s message."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"sherd_hip status {status}: {msg}")
+        self.status = status
+
+
+class InsideOutcome(C.Structure):
+    _fields_ = [("depth", C.c_float), ("inside", C.c_uint32)]
+
+
+class Registration(C.Structure):
+    """`Registration` (crates/sherd-core/src/matching/icp.rs:182-196)."""
+
+    _fields_ = [("transform", C.c_double * 16), ("fitness", C.c_double), ("inlier_rmse", C.c_double),
+                ("correspondences", C.c_uint64), ("iterations", C.c_uint32), ("converged", C.c_uint32)]
+
+
+class VerifyPair(C.Structure):
+    _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("scales", C.c_double * 12)]
+
+
+SCORE_KEYS = [
+    "tightA", "tightB", "tight", "gapA", "gapB", "gap", "contactA", "contactB", "contact", "seam",
+    "gap_limit", "tight_delta", "cont", "cont_n", "pen", "pen_depth", "pen_unavailable", "partial",
+    "brk", "brk_best",
+]
+
+
+class MethodStats(C.Structure):
+    _fields_ = [("calls", C.c_uint64), ("items", C.c_uint64), ("device_ns", C.c_uint64)]
+
+
+def lib():
+    """The loaded library; raises if it was never built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HipError(-5, f"{LIB_PATH} is missing: build it with sherd-refit_amd/build.py")
+        _lib = _bind(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def _p(a, ct):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+def _bind(L):
+    d, f, u32, u8, sz, i64 = C.c_double, C.c_float, C.c_uint32, C.c_uint8, C.c_size_t, C.c_int64
+    P, vp, ci = C.POINTER, C.c_void_p, C.c_int
+    sig = {
+        "sherd_open": (ci, [ci, P(vp)]),
+        "sherd_close": (None, [vp]),
+        "sherd_name": (C.c_char_p, []),
+        "sherd_last_error": (C.c_char_p, []),
+        "sherd_device_name": (ci, [vp, C.c_char_p, sz]),
+        "sherd_stats": (ci, [vp, ci, P(MethodStats)]),
+        "sherd_scene_build": (ci, [vp, P(f), sz, P(u32), sz, P(u8), P(vp)]),
+        "sherd_scene_free": (None, [vp]),
+        "sherd_scene_info": (ci, [vp, P(sz), P(f), P(f)]),
+        "sherd_first_hit": (ci, [vp, vp, P(f), P(f), sz, ci, P(f), P(u32)]),
+        "sherd_bounded_distance": (ci, [vp, vp, P(d), sz, P(d), d, ci, P(d)]),
+        "sherd_inside": (ci, [vp, vp, P(d), sz, P(d), ci, P(InsideOutcome)]),
+        "sherd_nearest_below": (ci, [vp, P(d), sz, P(d), sz, d, P(i64), P(d)]),
+        "sherd_coarse_scores": (ci, [vp, P(d), P(d), sz, P(d), P(d), sz, P(d), sz, d, d, P(u32), P(d)]),
+        "sherd_icp_rung": (ci, [vp, P(d), sz, P(d), P(d), sz, P(d), sz, ci, d, ci, P(Registration)]),
+        "sherd_fragment_create": (ci, [vp, vp, vp, ci, d, P(d), sz, P(d), sz, P(d), P(d), sz, P(d), P(d), sz,
+                                       P(vp)]),
+        "sherd_fragment_free": (None, [vp]),
+        "sherd_verify_batch": (ci, [vp, P(VerifyPair), sz, P(u32), P(d), sz, ci, ci, P(d), P(u8)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+def exported_symbols():
+    return [n for n in dir(lib()) if n.startswith("sherd_")]
+
+
+def _check(status):
+    if status != 0:
+        raise HipError(status, lib().sherd_last_error().decode())
+
+
+def f64(a, cols=3):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1, cols)
+
+
+def f32(a, cols=3):
+    return np.ascontiguousarray(a, dtype=np.float32).reshape(-1, cols)
+
+
+def pose16(T):
+    T = np.ascontiguousarray(T, dtype=np.float64).reshape(16)
+    return T
+
+
+class Scene:
+    """A device LBVH over an f32 mesh (RayScene, crates/sherd-core/src/spatial/bvh.rs:36-82)."""
+
+    def __init__(self, ex, V, F, mask=None):
+        self.ex = ex
+        V = f32(V)
+        F = np.ascontiguousarray(F, dtype=np.uint32).reshape(-1, 3)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        out = C.c_void_p()
+        _check(lib().sherd_scene_build(ex.ptr, _p(V, C.c_float), len(V), _p(F, C.c_uint32), len(F),
+                                       _p(m, C.c_uint8), C.byref(out)))
+        self.ptr = out
+
+    def close(self):
+        if self.ptr:
+            lib().sherd_scene_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        n = C.c_size_t()
+        lo = np.zeros(3, np.float32)
+        hi = np.zeros(3, np.float32)
+        _check(lib().sherd_scene_info(self.ptr, C.byref(n), _p(lo, C.c_float), _p(hi, C.c_float)))
+        return n.value, lo, hi
+
+
+class Fragment:
+    """One fragment as R §6 measures it (verify.rs:223-302 `Surfaces`), resident on the device."""
+
+    def __init__(self, ex, fracture, mesh, watertight, frac_area, pf, s, brk_p, brk_ns, margin_p, margin_n):
+        self.ex, self.fracture, self.mesh = ex, fracture, mesh
+        arrs = [f64(x) for x in (pf, s, brk_p, brk_ns, margin_p, margin_n)]
+        pf, s, bp, bn, mp, mn = arrs
+        out = C.c_void_p()
+        _check(lib().sherd_fragment_create(
+            ex.ptr, fracture.ptr, mesh.ptr if mesh is not None else None, int(bool(watertight)), float(frac_area),
+            _p(pf, C.c_double), len(pf), _p(s, C.c_double), len(s), _p(bp, C.c_double), _p(bn, C.c_double), len(bp),
+            _p(mp, C.c_double), _p(mn, C.c_double), len(mp), C.byref(out)))
+        self.ptr = out
+
+    def close(self):
+        if self.ptr:
+            lib().sherd_fragment_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HipExecutor:
+    """The `Executor` trait over one MI355X context."""
+
+    def __init__(self, device=0):
+        out = C.c_void_p()
+        _check(lib().sherd_open(int(device), C.byref(out)))
+        self.ptr = out
+
+    def close(self):
+        if self.ptr:
+            lib().sherd_close(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def name():
+        return lib().sherd_name().decode()
+
+    def device_name(self):
+        buf = C.create_string_buffer(256)
+        _check(lib().sherd_device_name(self.ptr, buf, 256))
+        return buf.value.decode()
+
+    def stats(self, method):
+        s = MethodStats()
+        _check(lib().sherd_stats(self.ptr, SHERD_M[method], C.byref(s)))
+        return dict(calls=s.calls, items=s.items, device_ns=s.device_ns)
+
+    def scene(self, V, F, mask=None):
+        return Scene(self, V, F, mask)
+
+    # ---- Executor methods -------------------------------------------------------------------
+
+    def first_hit(self, scene, orig, dirs, retest=False):
+        o, d = f32(orig), f32(dirs)
+        n = len(o)
+        t = np.empty(n, np.float32)
+        prim = np.empty(n, np.uint32)
+        _check(lib().sherd_first_hit(self.ptr, scene.ptr, _p(o, C.c_float), _p(d, C.c_float), n, int(retest),
+                                     _p(t, C.c_float), _p(prim, C.c_uint32)))
+        return t, prim
+
+    def bounded_distance(self, scene, points, T, max_dist, reduce="all"):
+        pts = f64(points)
+        T = pose16(T)
+        rd = {"all": 0, "min": 1}[reduce]
+        out = np.empty(1 if rd else len(pts), np.float64)
+        _check(lib().sherd_bounded_distance(self.ptr, scene.ptr, _p(pts, C.c_double), len(pts), _p(T, C.c_double),
+                                            float(max_dist), rd, _p(out, C.c_double)))
+        return out
+
+    def inside(self, scene, points, T, retest=False):
+        pts = f64(points)
+        T = pose16(T)
+        out = (InsideOutcome * max(len(pts), 1))()
+        _check(lib().sherd_inside(self.ptr, scene.ptr, _p(pts, C.c_double), len(pts), _p(T, C.c_double),
+                                  int(retest), out))
+        arr = np.frombuffer(out, dtype=np.dtype([("depth", np.float32), ("inside", np.uint32)]))[: len(pts)]
+        return arr["depth"].copy(), arr["inside"].copy()
+
+    def nearest_below(self, cloud, queries, bound):
+        """PointTree::nearest_below (kdtree.rs:182-192): (idx or -1, distance or inf)."""
+        c, q = f64(cloud), f64(queries)
+        idx = np.empty(len(q), np.int64)
+        dist = np.empty(len(q), np.float64)
+        _check(lib().sherd_nearest_below(self.ptr, _p(c, C.c_double), len(c), _p(q, C.c_double), len(q),
+                                         float(bound), _p(idx, C.c_int64), _p(dist, C.c_double)))
+        return idx, dist
+
+    def coarse_scores(self, target_p, target_n, probe_p, probe_n, poses, radius, normal_agree=0.7):
+        """Executor::coarse_scores; poses are (k, 12) rotation-then-translation rows or (k, 4, 4)."""
+        poses = np.asarray(poses, dtype=np.float64)
+        if poses.ndim == 3:
+            poses = np.concatenate([poses[:, :3, :3].reshape(-1, 9), poses[:, :3, 3]], axis=1)
+        poses = np.ascontiguousarray(poses)
+        tp, tn, pp, pn = (f64(x) for x in (target_p, target_n, probe_p, probe_n))
+        k = len(poses)
+        counts = np.empty(k, np.uint32)
+        scores = np.empty(k, np.float64)
+        _check(lib().sherd_coarse_scores(self.ptr, _p(tp, C.c_double), _p(tn, C.c_double), len(tp),
+                                         _p(pp, C.c_double), _p(pn, C.c_double), len(pp), _p(poses, C.c_double), k,
+                                         float(radius), float(normal_agree), _p(counts, C.c_uint32),
+                                         _p(scores, C.c_double)))
+        return scores, counts
+
+    def icp_rung(self, source, target_p, target_n, inits, estimation, distance, max_iteration):
+        """Executor::icp_rung: one Registration per init, as a (k, 21) array
+        [T(16), fitness, rmse, correspondences, iterations, converged] (the oracle's layout)."""
+        src, tp = f64(source), f64(target_p)
+        tn = None if target_n is None else f64(target_n)
+        inits = np.ascontiguousarray(inits, dtype=np.float64).reshape(-1, 16)
+        est = {"point_to_point": 0, "point_to_plane": 1}.get(estimation, estimation)
+        k = len(inits)
+        out = (Registration * max(k, 1))()
+        _check(lib().sherd_icp_rung(self.ptr, _p(src, C.c_double), len(src), _p(tp, C.c_double),
+                                    _p(tn, C.c_double), len(tp), _p(inits, C.c_double), k, int(est), float(distance),
+                                    int(max_iteration), out))
+        res = np.empty((k, 21))
+        for i in range(k):
+            r = out[i]
+            res[i, :16] = np.ctypeslib.as_array(r.transform)
+            res[i, 16:] = [r.fitness, r.inlier_rmse, r.correspondences, r.iterations, r.converged]
+        return res
+
+    def fragment(self, fracture, mesh, watertight, frac_area, pf, s, brk_p, brk_ns, margin_p, margin_n):
+        return Fragment(self, fracture, mesh, watertight, frac_area, pf, s, brk_p, brk_ns, margin_p, margin_n)
+
+    def verify_batch(self, pairs, cand_pair, cand_T, full=True, retest=False):
+        """verify::verify + accept over candidates of many pairs. pairs: [(frag_a, frag_b, scales12)].
+        Returns (scores (n, 20), accepted (n,) bool)."""
+        pv = (VerifyPair * max(len(pairs), 1))()
+        for i, (a, b, sc) in enumerate(pairs):
+            pv[i].a = a.ptr
+            pv[i].b = b.ptr
+            for k in range(12):
+                pv[i].scales[k] = float(sc[k])
+        cp = np.ascontiguousarray(cand_pair, dtype=np.uint32)
+        T = np.ascontiguousarray(cand_T, dtype=np.float64).reshape(-1, 16)
+        n = len(T)
+        scores = np.empty((n, 20), np.float64)
+        acc = np.empty(n, np.uint8)
+        _check(lib().sherd_verify_batch(self.ptr, pv, len(pairs), _p(cp, C.c_uint32), _p(T, C.c_double), n,
+                                        int(full), int(retest), _p(scores, C.c_double), _p(acc, C.c_uint8)))
+        return scores, acc.astype(bool)
+

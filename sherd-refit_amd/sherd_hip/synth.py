@@ -1,0 +1,286 @@
+"""Synthetic fragment collections for the benchmark workloads (host-side input generation).
+
+`vessel(...)` is the thin-wall vessel of the benchmark configurations: a surface of revolution
+(height 200 t, outer radius a seeded cubic spline in [40 t, 80 t], wall t, rim thickened to 1.4 t)
+cut into fragments by seeded Voronoi seeds on the mid-surface whose distances carry seeded smooth
+noise of amplitude 0.25 t, so the fracture lines wander. Each fragment is a closed, consistently
+oriented triangle mesh: its outer and inner shell, the end caps it owns, and the fracture walls it
+shares with its neighbours. Neighbouring fragments are built on one shared vertex lattice, so
+their fracture surfaces are exactly complementary and the relative pose that reassembles them is
+known to machine precision. Every fragment is then moved by its own seeded rigid pose.
+
+Per fragment the generator also produces what R §6 reads (verify.rs:223-302 `Surfaces`), drawn the
+way R §3.6 draws them (params.rs:179-187 defaults): fracture samples Pf (150 per t^2, clamped to
+[5 000, 12 000]), 20 000 surface samples S, the breakline (shell/fracture junction) thinned to
+0.5 t with shell macro-normals, and 6 000 shell-margin points between 0.15 t and 0.6 t of it.
+The meshes are f32; the samples are f64.
+"""
+
+import numpy as np
+from scipy.interpolate import CubicSpline
+from scipy.spatial import cKDTree
+
+# Params::default (crates/sherd-core/src/params.rs:143-188) values R §3.6 draws with.
+FRAC_PER_T2, MIN_FRAC, MAX_FRAC = 150.0, 5000, 12000
+SURFACE_POINTS, MARGIN_POINTS = 20000, 6000
+BRK_VOXEL, MACRO_INNER, MACRO_OUTER = 0.5, 0.15, 0.60
+
+
+def scales_for_pair(t, res):
+    """Scales::for_pair (crates/sherd-core/src/matching/scales.rs:72-88) under Params::default, in
+    scales.json order: t, res, coarse, stage1, tight, facing, gap, seam, near, pen, nms, icp."""
+    f = lambda k, m: max(k * t, m * res)
+    return np.array([t, res, f(0.15, 2.3), f(0.06, 0.9), f(0.01, 0.15), f(0.3, 1.0), f(0.03, 0.45),
+                     f(0.12, 1.8), f(0.5, 4.0), f(0.06, 0.9), 0.5 * t, f(0.04, 0.6) / (0.04 * t)])
+
+
+def _value_noise(p, wavelength, seed):
+    """Smooth seeded noise in [-1, 1]: trilinear interpolation of hashed lattice values."""
+    q = p / wavelength
+    i0 = np.floor(q).astype(np.int64)
+    f = q - i0
+    f = f * f * (3.0 - 2.0 * f)
+    out = np.zeros(len(p))
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                h = (i0[:, 0] + dx) * 73856093 ^ (i0[:, 1] + dy) * 19349663 ^ (i0[:, 2] + dz) * 83492791 ^ (seed * 2654435761)
+                h = (h ^ (h >> 13)) * 1274126177
+                v = ((h ^ (h >> 16)) & 0xFFFF) / 32767.5 - 1.0
+                w = (f[:, 0] if dx else 1 - f[:, 0]) * (f[:, 1] if dy else 1 - f[:, 1]) * (f[:, 2] if dz else 1 - f[:, 2])
+                out += w * v
+    return out
+
+
+def _random_rotation(rng, max_angle=np.pi):
+    axis = rng.normal(size=3)
+    axis /= np.linalg.norm(axis)
+    ang = rng.uniform(0.0, max_angle)
+    k = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + np.sin(ang) * k + (1 - np.cos(ang)) * (k @ k)
+
+
+def _sample_faces(v, f, faces, n, rng):
+    """n area-weighted uniform points on the listed faces: (points, face ids)."""
+    a, b, c = v[f[faces, 0]], v[f[faces, 1]], v[f[faces, 2]]
+    area = 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1)
+    pick = rng.choice(len(faces), size=n, p=area / area.sum())
+    r1 = np.sqrt(rng.uniform(size=n))[:, None]
+    r2 = rng.uniform(size=n)[:, None]
+    pts = (1 - r1) * a[pick] + r1 * (1 - r2) * b[pick] + r1 * r2 * c[pick]
+    return pts, faces[pick], area.sum()
+
+
+def vessel(t=1.0, edges_per_t=5.5, n_frag=8, seed=10, height_t=200.0, rmin_t=40.0, rmax_t=80.0,
+           noise_t=0.25, pose_seed=None):
+    """The vessel collection. Returns a dict with `fragments` (list of dicts), `poses` (4x4 each,
+    mapping the unposed vessel frame into the fragment's own frame), `adjacent` (set of index
+    pairs sharing a fracture wall), `t` and `res`."""
+    rng = np.random.default_rng(seed)
+    e = t / edges_per_t
+    H = height_t * t
+    zc = np.linspace(0.0, H, 6)
+    rc = rng.uniform(rmin_t * t, rmax_t * t, size=6)
+    spline = CubicSpline(zc, rc)
+    nz = max(4, int(round(H / e)))
+    zs = np.linspace(0.0, H, nz + 1)
+    r_out = np.clip(spline(zs), rmin_t * t, rmax_t * t)
+    thick = np.full(nz + 1, t)
+    rim = zs > H - 3.0 * t
+    thick[rim] = t + 0.4 * t * (zs[rim] - (H - 3.0 * t)) / (3.0 * t)
+    r_in = r_out - thick
+    nth = max(8, int(round(2 * np.pi * np.mean(0.5 * (r_out + r_in)) / e)))
+    th = np.linspace(0.0, 2 * np.pi, nth, endpoint=False)
+    K = max(1, int(round(t / e)))  # lattice levels through the wall
+
+    # Cell labels: nearest seed on the mid-surface, distances perturbed by smooth noise.
+    rng_seeds = np.random.default_rng(seed + 1)
+    ci, cj = np.meshgrid(np.arange(nth), np.arange(nz), indexing="ij")
+    cth = (ci + 0.5) * (2 * np.pi / nth)
+    czz = (cj + 0.5) * (H / nz)
+    crm = 0.5 * (np.interp(czz, zs, r_out) + np.interp(czz, zs, r_in))
+    cell_pts = np.stack([crm * np.cos(cth), crm * np.sin(cth), czz], axis=-1).reshape(-1, 3)
+    seeds = cell_pts[rng_seeds.choice(len(cell_pts), size=n_frag, replace=False)]
+    dist = np.empty((n_frag, len(cell_pts)))
+    for k in range(n_frag):
+        dist[k] = np.linalg.norm(cell_pts - seeds[k], axis=1) + noise_t * t * _value_noise(cell_pts, 2.0 * t, seed + 2 + k)
+    label = np.argmin(dist, axis=0).reshape(nth, nz).astype(np.int32)
+    del dist
+
+    # The shared lattice: node (i, j, l) -> index i*(nz+1)*(K+1) + j*(K+1) + l.
+    def node(i, j, l):
+        return ((i % nth) * (nz + 1) + j) * (K + 1) + l
+
+    L = (np.arange(K + 1) / K)[None, None, :]
+    rad = r_in[None, :, None] + L * thick[None, :, None]
+    X = rad * np.cos(th)[:, None, None]
+    Y = rad * np.sin(th)[:, None, None]
+    Z = np.broadcast_to(zs[None, :, None], X.shape)
+    verts = np.stack([X, Y, Z], axis=-1).reshape(-1, 3).astype(np.float32).astype(np.float64)
+
+    tris, owner, is_frac = [], [], []
+
+    def add(a, b, c, lab, frac):
+        tris.append(np.stack([a, b, c], axis=-1))
+        owner.append(np.broadcast_to(lab, np.shape(a)).ravel() if np.ndim(lab) else np.full(np.size(a), lab))
+        is_frac.append(np.full(np.size(a), frac, dtype=bool))
+
+    I, J = ci, cj
+    lab = label
+    # Outer shell (level K) outward, inner shell (level 0) inward.
+    a, b, c, d = node(I, J, K), node(I + 1, J, K), node(I + 1, J + 1, K), node(I, J + 1, K)
+    add(a, b, c, lab, False)
+    add(a, c, d, lab, False)
+    a, b, c, d = node(I, J, 0), node(I + 1, J, 0), node(I + 1, J + 1, 0), node(I, J + 1, 0)
+    add(a, c, b, lab, False)
+    add(a, d, c, lab, False)
+    for l in range(K):
+        # Walls at theta_{i+1} between cell (i, j) and (i+1, j).
+        nb = np.roll(label, -1, axis=0)
+        m = nb != label
+        ii, jj = I[m], J[m]
+        p0, p1, p2, p3 = node(ii + 1, jj, l), node(ii + 1, jj + 1, l), node(ii + 1, jj + 1, l + 1), node(ii + 1, jj, l + 1)
+        add(p0, p1, p2, label[m], True)
+        add(p0, p2, p3, label[m], True)
+        add(p0, p2, p1, nb[m], True)
+        add(p0, p3, p2, nb[m], True)
+        # Walls at z_{j+1} between cell (i, j) and (i, j+1).
+        up = label[:, 1:]
+        m = up != label[:, :-1]
+        ii, jj = I[:, :-1][m], J[:, :-1][m]
+        p0, p1, p2, p3 = node(ii, jj + 1, l), node(ii, jj + 1, l + 1), node(ii + 1, jj + 1, l + 1), node(ii + 1, jj + 1, l)
+        add(p0, p1, p2, label[:, :-1][m], True)
+        add(p0, p2, p3, label[:, :-1][m], True)
+        add(p0, p2, p1, up[m], True)
+        add(p0, p3, p2, up[m], True)
+        # End caps (shell): bottom faces -z, top faces +z.
+        ib = np.arange(nth)
+        p0, p1, p2, p3 = node(ib, 0, l), node(ib, 0, l + 1), node(ib + 1, 0, l + 1), node(ib + 1, 0, l)
+        add(p0, p2, p1, label[:, 0], False)
+        add(p0, p3, p2, label[:, 0], False)
+        p0, p1, p2, p3 = node(ib, nz, l), node(ib, nz, l + 1), node(ib + 1, nz, l + 1), node(ib + 1, nz, l)
+        add(p0, p1, p2, label[:, -1], False)
+        add(p0, p2, p3, label[:, -1], False)
+
+    tris = np.concatenate([x.reshape(-1, 3) for x in tris])
+    owner = np.concatenate(owner)
+    is_frac = np.concatenate(is_frac)
+
+    prng = np.random.default_rng(seed + 3 if pose_seed is None else pose_seed)
+    frags, poses, adjacent = [], [], set()
+    wall = (np.roll(label, -1, axis=0) != label)
+    for k, l2 in zip(label[wall], np.roll(label, -1, axis=0)[wall]):
+        adjacent.add((min(k, l2), max(k, l2)))
+    upm = label[:, 1:] != label[:, :-1]
+    for k, l2 in zip(label[:, :-1][upm], label[:, 1:][upm]):
+        adjacent.add((min(k, l2), max(k, l2)))
+    res = e
+    for k in range(n_frag):
+        sel = owner == k
+        ft = tris[sel]
+        used, inv = np.unique(ft, return_inverse=True)
+        v = verts[used]
+        f = inv.reshape(-1, 3).astype(np.uint32)
+        frac = is_frac[sel]
+        R = _random_rotation(prng)
+        tau = prng.uniform(-50 * t, 50 * t, size=3)
+        P = np.eye(4)
+        P[:3, :3] = R
+        P[:3, 3] = tau
+        frag = _fragment_data(v, f, frac, t, np.random.default_rng(seed + 100 + k))
+        frag = _posed(frag, P)
+        frag["name"] = f"frag{k}"
+        frags.append(frag)
+        poses.append(P)
+    return dict(fragments=frags, poses=poses, adjacent=adjacent, t=t, res=res)
+
+
+def _fragment_data(v, f, frac, t, rng):
+    fi = np.arange(len(f))
+    a, b, c = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    n = np.cross(b - a, c - a)
+    ln = np.linalg.norm(n, axis=1)
+    fn = n / np.maximum(ln, 1e-12)[:, None]
+    frac_area = 0.5 * ln[frac].sum()
+    n_pf = int(np.clip(round(FRAC_PER_T2 * frac_area / (t * t)), MIN_FRAC, MAX_FRAC))
+    pf, pf_face, _ = _sample_faces(v, f, fi[frac], n_pf, rng)
+    s, s_face, _ = _sample_faces(v, f, fi, SURFACE_POINTS, rng)
+    # Breakline: shell-face vertices that also belong to a fracture face, thinned on a 0.5 t voxel
+    # grid; the shell macro-normal is the mean shell face normal around the vertex.
+    shell = ~frac
+    on_frac = np.zeros(len(v), bool)
+    on_frac[f[frac].ravel()] = True
+    on_shell = np.zeros(len(v), bool)
+    on_shell[f[shell].ravel()] = True
+    vn = np.zeros_like(v)
+    np.add.at(vn, f[shell].ravel(), np.repeat(fn[shell], 3, axis=0))
+    brk = np.nonzero(on_frac & on_shell & (np.linalg.norm(vn, axis=1) > 0))[0]
+    key = np.floor(v[brk] / (BRK_VOXEL * t)).astype(np.int64)
+    _, first = np.unique(key, axis=0, return_index=True)
+    brk = brk[np.sort(first)]
+    brk_p = v[brk]
+    brk_ns = vn[brk] / np.linalg.norm(vn[brk], axis=1, keepdims=True)
+    # Shell margin: shell faces whose centroid lies 0.15 t .. 0.6 t from the breakline.
+    cen = (a + b + c) / 3.0
+    shell_faces = fi[shell]
+    dist, _ = cKDTree(v[np.nonzero(on_frac & on_shell)[0]]).query(cen[shell_faces])
+    band = shell_faces[(dist >= MACRO_INNER * t) & (dist <= MACRO_OUTER * t)]
+    margin_p, margin_face, _ = _sample_faces(v, f, band, MARGIN_POINTS, rng)
+    return dict(V=v.astype(np.float32), F=f, frac=frac, frac_area=float(frac_area), watertight=True,
+                Pf=pf, S=s, brk_P=brk_p, brk_ns=brk_ns, margin_p=margin_p, margin_n=fn[margin_face],
+                face_normals=fn)
+
+
+def _posed(frag, P):
+    R, tau = P[:3, :3], P[:3, 3]
+    out = dict(frag)
+    out["V"] = (frag["V"].astype(np.float64) @ R.T + tau).astype(np.float32)
+    for key in ("Pf", "S", "brk_P", "margin_p"):
+        out[key] = frag[key] @ R.T + tau
+    for key in ("brk_ns", "margin_n", "face_normals"):
+        out[key] = frag[key] @ R.T
+    return out
+
+
+def remove_patch(frag, fraction, rng):
+    """A non-watertight copy: drop `fraction` of the faces around a random shell face."""
+    v, f = frag["V"].astype(np.float64), frag["F"]
+    cen = v[f].mean(axis=1)
+    start = cen[rng.integers(len(f))]
+    order = np.argsort(np.linalg.norm(cen - start, axis=1))
+    keep = np.ones(len(f), bool)
+    keep[order[: int(fraction * len(f))]] = False
+    out = dict(frag)
+    out["F"] = f[keep]
+    out["frac"] = frag["frac"][keep]
+    out["watertight"] = False
+    return out
+
+
+def candidates(collection, per_pair=10, seed=20):
+    """Candidate poses for every pair (a < b): the true relative pose (B's frame into A's,
+    P_A . P_B^-1) and per_pair - 1 perturbations of it, from a tenth of a degree and a hundredth
+    of t up to 90 degrees and 10 t. Returns (pairs [(a, b)], cand_pair, cand_T)."""
+    rng = np.random.default_rng(seed)
+    P = collection["poses"]
+    t = collection["t"]
+    n = len(P)
+    angles = np.radians([0.1, 0.3, 1.0, 2.0, 5.0, 10.0, 20.0, 45.0, 90.0])
+    shifts = t * np.array([0.01, 0.03, 0.1, 0.2, 0.5, 1.0, 2.0, 5.0, 10.0])
+    pairs, cp, cT = [], [], []
+    for a in range(n):
+        for b in range(a + 1, n):
+            truth = P[a] @ np.linalg.inv(P[b])
+            pairs.append((a, b))
+            for k in range(per_pair):
+                T = truth.copy()
+                if k > 0:
+                    j = (k - 1) % len(angles)
+                    d = np.eye(4)
+                    d[:3, :3] = _random_rotation(rng, angles[j])
+                    v = rng.normal(size=3)
+                    d[:3, 3] = shifts[j] * v / np.linalg.norm(v)
+                    T = d @ truth
+                cp.append(len(pairs) - 1)
+                cT.append(T)
+    return pairs, np.array(cp, np.uint32), np.array(cT)

@@ -1,0 +1,155 @@
+"""HIP R §6 verification against the oracle (bit for bit) and the reference's slab dump (its own
+tolerances, `accepted` identical) (GPU).
+
+Cases: the slab's ten stage-2 candidates one at a time and as one batch, the same candidates
+replicated over several pairs in one batch, the cheap half (full = 0), the FP64 retest, and the
+reference's wall/box known answers (verify.rs:889-1120).
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import slab_data as S
+
+pytestmark = pytest.mark.gpu
+
+TOL = dict(tight=0.01, gap=0.002, seam=0.34, cont=0.005, cont_n=0.01, pen=0.0005)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ex():
+    import sherd_hip
+
+    e = sherd_hip.HipExecutor(0)
+    yield e
+    e.close()
+
+
+def sides(ex):
+    dev, ora, keep = [], [], []
+    for name in ("pieceA", "pieceB"):
+        pc = S.piece(name)
+        v32 = pc["V"].astype(np.float32)
+        fr, me = ex.scene(v32, pc["F"], pc["frac"]), ex.scene(v32, pc["F"])
+        ofr, ome = O.Scene(v32, pc["F"], pc["frac"]), O.Scene(v32, pc["F"])
+        keep += [fr, me, ofr, ome]
+        args = (pc["watertight"], pc["frac_area"], pc["Pf"], pc["S"], pc["brk_P"], pc["brk_ns"], pc["margin_p"],
+                pc["margin_n"])
+        dev.append(ex.fragment(fr, me, *args))
+        ora.append(O.SideData(ofr, ome, *args))
+    return dev, ora, keep
+
+
+@pytest.fixture(scope="module")
+def slab(ex):
+    return sides(ex)
+
+
+def test_slab_candidates_bit_exact_and_within_the_reference(ex, slab):
+    dev, ora, _ = slab
+    p = S.pair()
+    sc = p["scales"]
+    T = p["s2_T_frac2"]
+    say("case: slab verify batch of 10")
+    got, acc = ex.verify_batch([(dev[0], dev[1], sc)], np.zeros(len(T)), T)
+    for i in range(len(T)):
+        want = O.verify(ora[0], ora[1], T[i], sc)
+        assert np.array_equal(got[i].view(np.uint64), want.view(np.uint64)), (i, got[i] - want)
+        assert acc[i] == O.accept(want, sc) == bool(p["s2_accepted"][i])
+        theirs = p["s2_scores"][i]
+        for k, tl in TOL.items():
+            assert abs(got[i][O.OS_KEYS.index(k)] - theirs[k]) <= tl, (i, k)
+    say("case: slab verify one candidate per call")
+    for i in range(len(T)):
+        one, a1 = ex.verify_batch([(dev[0], dev[1], sc)], [0], T[i:i + 1])
+        assert np.array_equal(one[0].view(np.uint64), got[i].view(np.uint64)) and a1[0] == acc[i]
+
+
+def test_many_pairs_in_one_batch(ex, slab):
+    """The same candidates under several pairs (both orders, other scales) in one batch answer
+    what each answers alone: a batch is grouping, not arithmetic."""
+    dev, ora, _ = slab
+    p = S.pair()
+    sc = p["scales"]
+    sc2 = O.scales_for_pair(28.0, 2.5)
+    T = p["s2_T_frac2"]
+    Tinv = np.array([O.pose_inverse(t) for t in T])
+    pairs = [(dev[0], dev[1], sc), (dev[1], dev[0], sc), (dev[0], dev[1], sc2)]
+    cp = np.repeat([0, 1, 2], len(T))
+    cT = np.concatenate([T, Tinv, T])
+    say("case: 30 candidates over 3 pairs")
+    got, acc = ex.verify_batch(pairs, cp, cT)
+    for i in range(len(cT)):
+        a, b, s_ = [(ora[0], ora[1], sc), (ora[1], ora[0], sc), (ora[0], ora[1], sc2)][cp[i]]
+        want = O.verify(a, b, cT[i], s_)
+        assert np.array_equal(got[i].view(np.uint64), want.view(np.uint64)), i
+        assert acc[i] == O.accept(want, s_)
+
+
+@pytest.mark.parametrize("full,retest", [(False, False), (True, True)])
+def test_cheap_half_and_retest(ex, slab, full, retest):
+    dev, ora, _ = slab
+    p = S.pair()
+    sc = p["scales"]
+    T = p["s2_T_frac2"]
+    say(f"case: full={full} retest={retest}")
+    got, acc = ex.verify_batch([(dev[0], dev[1], sc)], np.zeros(len(T)), T, full=full, retest=retest)
+    for i in range(len(T)):
+        want = O.verify(ora[0], ora[1], T[i], sc, full=full, retest=retest)
+        assert np.array_equal(got[i].view(np.uint64), want.view(np.uint64)), i
+    if not full:
+        assert not acc.any()
+
+
+def box_mesh(lo, hi):
+    v = np.array([
+        [lo[0], lo[1], lo[2]], [hi[0], lo[1], lo[2]], [hi[0], hi[1], lo[2]], [lo[0], hi[1], lo[2]],
+        [lo[0], lo[1], hi[2]], [hi[0], lo[1], hi[2]], [hi[0], hi[1], hi[2]], [lo[0], hi[1], hi[2]],
+    ], dtype=np.float32)
+    f = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4], [3, 7, 6], [3, 6, 2],
+                  [0, 4, 7], [0, 7, 3], [1, 2, 6], [1, 6, 5]], dtype=np.uint32)
+    return v, f
+
+
+def test_known_answers_through_the_device(ex):
+    """verify.rs:889-1060: walls, seams, continuity and penetration with exact answers."""
+    say("case: wall/box known answers")
+    wall_v = np.array([[0, 0, 0], [0, 2, 0], [0, 2, 2], [0, 0, 2]], np.float32)
+    wall_f = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+    wall = ex.scene(wall_v, wall_f)
+    samples = np.array([[0.0, 0.2 + i * 0.2, 0.2 + j * 0.2] for i in range(10) for j in range(10)])
+    curve = np.array([[0.0, k * 0.1, 0.0] for k in range(30)])
+    up = np.tile([0.0, 0.0, 1.0], (30, 1))
+    grid = np.array([[i * 0.1, j * 0.1, 0.0] for i in range(5) for j in range(5)])
+    gup = np.tile([0.0, 0.0, 1.0], (25, 1))
+    va, fa = box_mesh([0, 0, 0], [10, 10, 10])
+    vb, fb = box_mesh([20, 20, 20], [30, 30, 30])
+    ma, mb = ex.scene(va, fa), ex.scene(vb, fb)
+    e = np.zeros((0, 3))
+    a = ex.fragment(wall, ma, True, 4.0, samples, [[4, 5, 6], [1, 2, 3]], curve, up, grid, gup)
+    b = ex.fragment(wall, mb, True, 4.0, samples, [[4, 5, 6], [1, 5.5, 6.5], [50, 5, 5], [-5, 5, 5]], curve, up,
+                    grid, gup)
+    sc1 = O.scales_for_pair(1.0, 0.0)
+    sc3 = O.scales_for_pair(3.0, 0.0)
+    shifted = lambda dx: np.array([[1, 0, 0, dx], [0, 1, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1.0]])
+    got, _ = ex.verify_batch([(a, b, sc1), (a, b, sc3)], [0, 0, 0, 1],
+                             np.stack([np.eye(4), shifted(0.005), shifted(0.5), np.eye(4)]))
+    k = O.OS_KEYS.index
+    assert abs(got[0][k("tight")] - 1.0) < 1e-12 and abs(got[0][k("contact")] - 4.0) < 1e-9
+    assert abs(got[1][k("gap")] - 0.005) < 1e-6
+    assert (got[2][k("tight")], got[2][k("gap")], got[2][k("contact")]) == (0.0, 1.0, 0.0)
+    assert abs(got[3][k("seam")] - 1.0) < 1e-12
+    assert abs(got[0][k("cont")]) < 1e-12 and abs(got[0][k("cont_n")] - 1.0) < 1e-12
+    assert abs(got[0][k("pen")] - 0.5) < 1e-12 and abs(got[0][k("pen_depth")] - 4.0) < 1e-5
+    holed = ex.fragment(wall, mb, False, 4.0, samples, [[5, 5, 5]], curve, up, e, e)
+    got, _ = ex.verify_batch([(a, holed, sc1)], [0], np.eye(4)[None])
+    assert (got[0][k("pen")], got[0][k("pen_unavailable")]) == (0.0, 1.0)
+    assert (got[0][k("cont")], got[0][k("cont_n")]) == (1.0, -1.0)
