@@ -1,0 +1,231 @@
+"""Benchmark of the MI355X sherd-refit backend on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload ("plain method on GPU vs CPU"): the synthetic thin-wall vessel
+(height 200 t, spline radius 40-80 t, wall t = 1, rim 1.4 t) broken into 8 noisy Voronoi fragments at
+5.5 edges per t (0.9-1.6 M triangles each; the 17 edges/t of the configuration text would be ~87 M
+triangles), each under its own seeded rigid pose. All 28 pairs x 10 candidate poses
+(the true relative pose and nine perturbations of it) go through R §6's verification.
+
+One step = one batched verification of all 280 candidates (sherd_verify_batch), inputs resident on
+the device. `value` is pair-verifications per second over the timed steps. Beside it:
+  * the plain (unbatched, one candidate per call) HIP backend on the same candidates, and the
+    batched/plain ratio; accepted seams are checked identical between the two;
+  * rays/s of R §3.4.3's seven-ray cone vote over every face of every fragment;
+  * LBVH build rate, kNN queries/s;
+  * the roofline of the dominant verification kernel, its duration measured with HIP events on the
+    library's own stream around every launch;
+  * the CPU baseline: the oracle (the C++ restatement of the CPU executor) on a bounded sample of
+    the same candidates, on this host's cores.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "sherd-refit_amd"))
+
+METRIC = "pair-verifications/s + rays/s, batched vs plain HIP, 1 MI355X, same accepted seams"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(kernel, frags, pairs, cand_pair):
+    """Compulsory bytes of one launch of a verification kernel over the batch: every input element
+    read once, every output written once, every resident structure used counted once."""
+    nodes = lambda f: 32 * (2 * f - 1)
+    used = {}
+    total = 0
+    for c in cand_pair:
+        a, b = pairs[c]
+        fa, fb = frags[a], frags[b]
+        if kernel == "k_pen" or kernel == "k_pen_min":
+            total += 24 * (len(fa["S"]) + len(fb["S"]))
+            used[("mesh", a)] = used[("mesh", b)] = 0
+        elif kernel == "k_frac":
+            total += 24 * (len(fa["Pf"]) + len(fb["Pf"])) + 8 * (len(fa["Pf"]) + len(fb["Pf"]))
+            used[("frac", a)] = used[("frac", b)] = 0
+        elif kernel == "k_seam":
+            total += 48 * (len(fa["brk_P"]) + len(fb["brk_P"]))
+        elif kernel == "k_cont":
+            total += 48 * len(fb["margin_p"]) + 16 * len(fb["margin_p"])
+            used[("margin", a)] = 0
+    for (kind, i) in used:
+        fr = frags[i]
+        if kind == "mesh":
+            n = len(fr["F"])
+            total += nodes(n) + 48 * n
+        elif kind == "frac":
+            n = int(fr["frac"].sum())
+            total += nodes(n) + 48 * n
+        else:
+            total += 48 * len(fr["margin_p"])
+    return total
+
+
+def cpu_baseline(col, pairs, scales, cand_pair, cand_T, limit_s=20.0):
+    """The oracle on a bounded sample of the candidates: pair-verifications/s on this host."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import oracle_lib as O
+
+    cores = min(os.cpu_count() or 1, 16)
+    O.set_threads(cores)
+    frags = col["fragments"]
+    sample_pairs = sorted(set(int(c) for c in cand_pair[:30]))
+    need = sorted(set(i for p in sample_pairs for i in pairs[p]))
+    sides = {}
+    keep = []
+    for i in need:
+        fr = frags[i]
+        fs, ms = O.Scene(fr["V"], fr["F"], fr["frac"]), O.Scene(fr["V"], fr["F"])
+        keep += [fs, ms]
+        sides[i] = O.SideData(fs, ms, fr["watertight"], fr["frac_area"], fr["Pf"], fr["S"], fr["brk_P"],
+                              fr["brk_ns"], fr["margin_p"], fr["margin_n"])
+    t0 = time.perf_counter()
+    done = 0
+    out = []
+    for k in range(min(30, len(cand_pair))):
+        a, b = pairs[int(cand_pair[k])]
+        out.append(O.verify(sides[a], sides[b], cand_T[k], scales))
+        done += 1
+        if time.perf_counter() - t0 > limit_s:
+            break
+    el = time.perf_counter() - t0
+    return dict(value=done / el, unit="pair-verifications/s", cores=cores, kind="port",
+                sample=f"oracle verify of the first {done} candidates (pairs {sample_pairs[0]}-{sample_pairs[-1]}) "
+                       f"of the same workload, {cores} OpenMP threads, scene builds excluded"), out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--edges-per-t", type=float, default=5.5)
+    ap.add_argument("--plain-candidates", type=int, default=280)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the pair list would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import sherd_hip
+    from sherd_hip import synth
+
+    t0 = time.perf_counter()
+    col = synth.vessel(edges_per_t=args.edges_per_t)
+    frags = col["fragments"]
+    log(f"generated {len(frags)} fragments, {sum(len(f['F']) for f in frags)} triangles in "
+        f"{time.perf_counter() - t0:.1f}s")
+    scales = synth.scales_for_pair(col["t"], col["res"])
+    pairs, cand_pair, cand_T = synth.candidates(col)
+
+    ex = sherd_hip.HipExecutor(0)
+    log("device:", ex.device_name())
+    scenes, handles = [], []
+    for fr in frags:
+        fs = ex.scene(fr["V"], fr["F"], fr["frac"])
+        ms = ex.scene(fr["V"], fr["F"])
+        scenes.append((fs, ms))
+        handles.append(ex.fragment(fs, ms, fr["watertight"], fr["frac_area"], fr["Pf"], fr["S"], fr["brk_P"],
+                                   fr["brk_ns"], fr["margin_p"], fr["margin_n"]))
+    st = ex.stats("scene_build")
+    lbvh_mtri = st["items"] / (st["device_ns"] * 1e-9) / 1e6
+    vpairs = [(handles[a], handles[b], scales) for (a, b) in pairs]
+
+    # ---- batched verification (the timed steps) -----------------------------------------------
+    for _ in range(args.warmup):
+        ex.verify_batch(vpairs, cand_pair, cand_T)
+    ex.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        scores, accepted = ex.verify_batch(vpairs, cand_pair, cand_T)
+    el = time.perf_counter() - t0
+    prof = ex.profile_read()
+    ex.profile(False)
+    n_cand = len(cand_T)
+    value = n_cand * args.steps / el
+    log(f"batched: {n_cand} candidates x {args.steps} steps in {el:.3f}s; accepted {int(accepted.sum())}")
+
+    # ---- plain: one candidate per call ------------------------------------------------------------
+    n_plain = min(args.plain_candidates, n_cand)
+    t0 = time.perf_counter()
+    plain_acc = np.empty(n_plain, bool)
+    plain_scores = np.empty((n_plain, 20))
+    for i in range(n_plain):
+        s1, a1 = ex.verify_batch(vpairs, cand_pair[i:i + 1], cand_T[i:i + 1])
+        plain_scores[i], plain_acc[i] = s1[0], a1[0]
+    el_plain = time.perf_counter() - t0
+    plain = n_plain / el_plain
+    same = bool(np.array_equal(plain_acc, accepted[:n_plain]) and
+                np.array_equal(plain_scores.view(np.uint64), scores[:n_plain].view(np.uint64)))
+    log(f"plain: {n_plain} candidates in {el_plain:.3f}s; identical to batched: {same}")
+
+    # ---- rays: the seven-ray cone vote over every face ----------------------------------------
+    before = ex.stats("cone")
+    for fr, (fs, ms) in zip(frags, scenes):
+        ns = fr["face_normals"]
+        ex.cone_votes(ms, fr["V"].astype(np.float64), fr["F"], None, ns, col["t"])
+    after = ex.stats("cone")
+    rays = after["items"] - before["items"]
+    rays_per_s = rays / ((after["device_ns"] - before["device_ns"]) * 1e-9)
+
+    # ---- kNN queries/s: R §6.3's margin query shape, all margins against one -----------------------
+    q = np.concatenate([fr["margin_p"] for fr in frags])
+    cloud = frags[0]["margin_p"]
+    b0 = ex.stats("nearest")
+    for _ in range(3):
+        ex.nearest_below(cloud, q, scales[8])
+    b1 = ex.stats("nearest")
+    knn = (b1["items"] - b0["items"]) / ((b1["device_ns"] - b0["device_ns"]) * 1e-9)
+
+    # ---- roofline of the dominant verification kernel -------------------------------------------
+    dom = max(prof, key=lambda k: prof[k]["ms"])
+    avg_ms = prof[dom]["ms"] / prof[dom]["launches"]
+    nbytes = algorithmic_bytes(dom, frags, pairs, cand_pair)
+    achieved = nbytes / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    pmc = os.path.join(HERE, "profiles", "verify_vessel_pmc.json")
+    if os.path.exists(pmc):
+        with open(pmc) as fh:
+            traffic = json.load(fh).get(dom, {}).get("bytes_per_launch")
+    roof = dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 6), traffic=traffic, kernel=dom,
+                algorithmic_bytes_per_launch=nbytes, avg_launch_ms=round(avg_ms, 4))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu, oracle_scores = cpu_baseline(col, pairs, scales, cand_pair, cand_T)
+        k = len(oracle_scores)
+        parity = bool(np.array_equal(np.array(oracle_scores).view(np.uint64), scores[:k].view(np.uint64)))
+        cpu["scores_identical_to_hip"] = parity
+        log(f"cpu baseline: {cpu['value']:.2f} pair-verifications/s; oracle == HIP on the sample: {parity}")
+
+    line = dict(
+        metric=METRIC, value=round(value, 3), unit="pair-verifications/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=round(1e3 * el / args.steps, 3), higher_is_better=True, scaling="weak",
+        vs_baseline=None, dtype="f32+f64", data="synthetic",
+        config=dict(workload=f"vessel (configs[1]): 8 fragments, {args.edges_per_t} edges/t, "
+                             f"{sum(len(f['F']) for f in frags)} triangles, 28 pairs x 10 candidates",
+                    candidates_per_step=n_cand, parallelism="single GPU"),
+        plain_pair_verifications_per_s=round(plain, 3), batched_vs_plain=round(value / plain, 3),
+        accepted=int(accepted.sum()), accepted_identical_plain_batched=same,
+        rays_per_s=round(rays_per_s, 1), rays=int(rays), knn_queries_per_s=round(knn, 1),
+        lbvh_mtri_per_s=round(lbvh_mtri, 2), kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
+    )
+    print(json.dumps(line), flush=True)
+    ex.close()
+
+
+if __name__ == "__main__":
+    main()

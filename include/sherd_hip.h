@@ -187,6 +187,25 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
                        const uint32_t *cand_pair, const double *cand_T, size_t n_cand, int full,
                        int retest, double *scores, uint8_t *accepted);
 
+/* ---- preprocessing rays (R §3.4.3; the trait's fifth method, executor/mod.rs:9-11) ------------ */
+
+/* segment::classify_faces (crates/sherd-core/src/fragment/segment.rs:318-381): the seven-ray cone
+ * vote of each listed face. scene is the mesh's own LBVH (built from V narrowed to f32, no mask);
+ * V (nv x 3 f64) and F (nf x 3) are the same mesh; faces lists n_faces face indices (NULL: all
+ * faces, n_faces = nf); ns holds one f64 shell normal per listed face; thick is the wall t.
+ * good receives the vote count 0..7 per listed face. */
+int sherd_cone_votes(sherd_ctx *ctx, const sherd_scene *scene, const double *V, size_t nv,
+                     const uint32_t *F, size_t nf, const int64_t *faces, const double *ns,
+                     size_t n_faces, double thick, int retest, uint8_t *good);
+
+/* ---- per-kernel timing ------------------------------------------------------------------------ */
+
+/* Turns per-kernel HIP-event timing on the context's stream on (1) or off (0), clearing the
+ * totals. Counterpart of the reference's per-method device counters (sherd-gpu/src/executor.rs). */
+int sherd_profile(sherd_ctx *ctx, int on);
+/* The totals as a JSON object {"kernel": {"ms": total, "launches": n}, ...}. */
+int sherd_profile_read(sherd_ctx *ctx, char *buf, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -71,6 +72,42 @@ void Timer::stop() {
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, c_->ev0, c_->ev1) == hipSuccess) c_->stats[m_].device_ns += (uint64_t)(ms * 1e6);
   }
+}
+
+int KernelProfiler::slot(const char *k) {
+  for (int i = 0; i < n; ++i)
+    if (std::strcmp(name[i], k) == 0) return i;
+  if (n >= MAXK) return MAXK - 1;
+  name[n] = k;
+  return n++;
+}
+
+void KernelProfiler::begin(const char *k, hipStream_t s) {
+  if (npending >= MAXK) return;
+  int i = slot(k);
+  if (!ev[2 * npending]) {
+    (void)hipEventCreate(&ev[2 * npending]);
+    (void)hipEventCreate(&ev[2 * npending + 1]);
+  }
+  pending[npending] = i;
+  (void)hipEventRecord(ev[2 * npending], s);
+}
+
+void KernelProfiler::end(hipStream_t s) {
+  if (npending >= MAXK) return;
+  (void)hipEventRecord(ev[2 * npending + 1], s);
+  ++npending;
+}
+
+void KernelProfiler::collect() {
+  for (int p = 0; p < npending; ++p) {
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, ev[2 * p], ev[2 * p + 1]) == hipSuccess) {
+      ms[pending[p]] += t;
+      launches[pending[p]] += 1;
+    }
+  }
+  npending = 0;
 }
 
 }  // namespace sherd
@@ -512,3 +549,94 @@ int sherd_icp_rung(sherd_ctx *ctx, const double *src, size_t n_src, const double
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int sherd_cone_votes(sherd_ctx *ctx, const sherd_scene *scene, const double *V, size_t nv, const uint32_t *F,
+                     size_t nf, const int64_t *faces, const double *ns, size_t n_faces, double thick, int retest,
+                     uint8_t *good) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  LOCK(c);
+  if (!scene || scene->ctx != c) return fail(SHERD_E_ARG, "scene of another context");
+  if (n_faces == 0) return SHERD_OK;
+  if (!V || !F || !ns || !good) return fail(SHERD_E_ARG, "null buffer");
+  if (!faces && n_faces != nf) return fail(SHERD_E_ARG, "without a face list, n_faces must be nf");
+  if (nf > (size_t)INT_MAX / 3 || n_faces > (size_t)INT_MAX || nv > (size_t)INT_MAX / 3)
+    return fail(SHERD_E_LIMIT, "mesh too large");
+  if ((size_t)scene->d.n != nf) return fail(SHERD_E_ARG, "the scene must be built over every face of F");
+  for (size_t i = 0; i < 3 * nf; ++i)
+    if (F[i] >= nv) return fail(SHERD_E_ARG, "face index out of range");
+  if (faces)
+    for (size_t i = 0; i < n_faces; ++i)
+      if (faces[i] < 0 || (size_t)faces[i] >= nf) return fail(SHERD_E_ARG, "listed face out of range");
+  hipError_t e = hipSuccess;
+  double *dV = upload3(c, 0, V, nv, &e);
+  if (!dV) return hip_fail(e, "upload vertices");
+  uint32_t *dF = (uint32_t *)c->scratch(1, sizeof(uint32_t) * 3 * nf, &e);
+  if (!dF) return hip_fail(e, "scratch");
+  HIPCK(hipMemcpyAsync(dF, F, sizeof(uint32_t) * 3 * nf, hipMemcpyHostToDevice, c->stream), "upload faces");
+  int64_t *dfaces = nullptr;
+  if (faces) {
+    dfaces = (int64_t *)c->scratch(2, sizeof(int64_t) * n_faces, &e);
+    if (!dfaces) return hip_fail(e, "scratch");
+    HIPCK(hipMemcpyAsync(dfaces, faces, sizeof(int64_t) * n_faces, hipMemcpyHostToDevice, c->stream), "upload");
+  }
+  double *dns = upload3(c, 3, ns, n_faces, &e);
+  if (!dns) return hip_fail(e, "upload normals");
+  uint8_t *dg = (uint8_t *)c->scratch(4, n_faces, &e);
+  if (!dg) return hip_fail(e, "scratch");
+  ConeConst cc;
+  const double PI = 3.14159265358979323846;
+  double angle = 15.0 * (PI / 180.0);
+  cc.cos_a = std::cos(angle);
+  cc.sin_a = std::sin(angle);
+  for (int k = 1; k < 7; ++k) {
+    double phi = 2.0 * PI * (double)(k - 1) / 6.0;
+    cc.cp[k - 1] = std::cos(phi);
+    cc.sp[k - 1] = std::sin(phi);
+  }
+  cc.min_hit = (float)(0.1 * thick);
+  cc.thick = (float)thick;
+  Timer tm(c, SHERD_M_CONE, 7 * (uint64_t)n_faces);
+  HIPCK(launch_cone(view_of(scene->d), dV, dF, (int)nf, dfaces, dns, (int)n_faces, cc, retest, dg, c->stream),
+        "cone kernel");
+  tm.stop();
+  HIPCK(hipMemcpyAsync(good, dg, n_faces, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  return SHERD_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int sherd_profile(sherd_ctx *ctx, int on) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  LOCK(c);
+  c->prof.on = on != 0;
+  for (int i = 0; i < KernelProfiler::MAXK; ++i) {
+    c->prof.ms[i] = 0.0;
+    c->prof.launches[i] = 0;
+  }
+  return SHERD_OK;
+}
+
+int sherd_profile_read(sherd_ctx *ctx, char *buf, size_t len) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  LOCK(c);
+  if (!buf || len == 0) return fail(SHERD_E_ARG, "null buffer");
+  std::string out = "{";
+  for (int i = 0; i < c->prof.n; ++i) {
+    char item[160];
+    std::snprintf(item, sizeof item, "%s\"%s\": {\"ms\": %.6f, \"launches\": %llu}", i ? ", " : "", c->prof.name[i],
+                  c->prof.ms[i], (unsigned long long)c->prof.launches[i]);
+    out += item;
+  }
+  out += "}";
+  if (out.size() + 1 > len) return fail(SHERD_E_LIMIT, "buffer too small");
+  std::memcpy(buf, out.c_str(), out.size() + 1);
+  return SHERD_OK;
+}
+
+}  // extern "C"
+

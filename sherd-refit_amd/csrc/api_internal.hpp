@@ -24,6 +24,7 @@ struct Ctx {
     size_t n = 0;
   };
   Buf bufs[24];
+  KernelProfiler prof;
   // A grow-only device buffer per slot, reused across calls (the context is serialised).
   void *scratch(int slot, size_t bytes, hipError_t *err);
 };

@@ -234,7 +234,7 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     vb.n_cand = (int)nc;
     vb.full = full ? 1 : 0;
     vb.retest = retest ? 1 : 0;
-    HIPCK(launch_verify(vb, max_pf, max_s, max_margin, c->stream), "verify kernels");
+    HIPCK(launch_verify(vb, max_pf, max_s, max_margin, c->stream, &c->prof), "verify kernels");
     hs.resize(VS_COUNT * nc);
     ha.resize(nc);
     hacc.resize(nc);
@@ -242,6 +242,7 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     HIPCK(hipMemcpyAsync(ha.data(), dac, nc, hipMemcpyDeviceToHost, c->stream), "download");
     HIPCK(hipMemcpyAsync(hacc.data(), dacc, sizeof(CandAcc) * nc, hipMemcpyDeviceToHost, c->stream), "download");
     HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+    c->prof.collect();
     for (size_t i = 0; i < nc; ++i)
       if (hacc[i].flags & FLAG_VOXEL_RANGE) return fail(SHERD_E_LIMIT, "a seam voxel coordinate exceeds 2^20");
     std::memcpy(scores + VS_COUNT * begin, hs.data(), sizeof(double) * VS_COUNT * nc);

@@ -1,0 +1,91 @@
+// R §3.4.3's cone vote on the device: segment::classify_faces
+// (crates/sherd-core/src/fragment/segment.rs:318-381, cone_basis :383-396, constants :53-62).
+//
+// One thread per face: the face normal and centroid in f64 (mesh/geometry.rs:70-96), the origin
+// pushed 1e-3 along the raw normal and narrowed to f32, seven rays (straight down -NS, then six on
+// a 15 degree cone), each a first hit through the LBVH; a hit votes when 0.1 t < dh, 0.5 < dh/t < 1.8
+// in f32, and the hit face looks back along the ray (h . d > 0.7 in f64). The cone's sines and
+// cosines are computed once on the host and passed in, so every lane uses the same doubles.
+#include <hip/hip_runtime.h>
+
+#include "bvh_query.hpp"
+#include "kernels.hpp"
+
+namespace sherd {
+namespace {
+
+__device__ __forceinline__ void face_normal(const double *V, const uint32_t *F, uint32_t f, double *n) {
+  const double *a = V + 3 * (size_t)F[3 * f], *b = V + 3 * (size_t)F[3 * f + 1], *c = V + 3 * (size_t)F[3 * f + 2];
+  double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  double x = e1[1] * e2[2] - e1[2] * e2[1], y = e1[2] * e2[0] - e1[0] * e2[2], z = e1[0] * e2[1] - e1[1] * e2[0];
+  double len = sqrt((x * x + y * y) + z * z);
+  double den = len < 1e-12 ? 1e-12 : len;  // f64::max(len, 1e-12)
+  n[0] = x / den;
+  n[1] = y / den;
+  n[2] = z / den;
+}
+
+__global__ void __launch_bounds__(256) k_cone(SceneView sv, const double *V, const uint32_t *F, int nf,
+                                              const int64_t *faces, const double *ns, int n, ConeConst cc,
+                                              int retest, uint8_t *good) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  uint32_t i = faces ? (uint32_t)faces[q] : (uint32_t)q;
+  const double *nv = ns + 3 * (size_t)q;
+  double n0 = nv[0], n1 = nv[1], n2 = nv[2];
+  uint8_t votes = 0;
+  if (n0 - n0 == 0.0 && n1 - n1 == 0.0 && n2 - n2 == 0.0) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    if (fabs(n0) < 0.9) a0 = 1.0; else a1 = 1.0;
+    double e0 = n1 * a2 - n2 * a1, e1 = n2 * a0 - n0 * a2, e2 = n0 * a1 - n1 * a0;
+    double len = sqrt((e0 * e0 + e1 * e1) + e2 * e2);
+    if (len > 0.0) {
+      e0 /= len;
+      e1 /= len;
+      e2 /= len;
+      double f0 = n1 * e2 - n2 * e1, f1 = n2 * e0 - n0 * e2, f2 = n0 * e1 - n1 * e0;
+      double fn[3];
+      face_normal(V, F, i, fn);
+      const double *pa = V + 3 * (size_t)F[3 * i], *pb = V + 3 * (size_t)F[3 * i + 1], *pc = V + 3 * (size_t)F[3 * i + 2];
+      double c0 = ((pa[0] + pb[0]) + pc[0]) / 3.0, c1 = ((pa[1] + pb[1]) + pc[1]) / 3.0, c2 = ((pa[2] + pb[2]) + pc[2]) / 3.0;
+      V3 o = V3{(float)(c0 - fn[0] * 1e-3), (float)(c1 - fn[1] * 1e-3), (float)(c2 - fn[2] * 1e-3)};
+      for (int k = 0; k < 7; ++k) {
+        double d0, d1, d2;
+        if (k == 0) {
+          d0 = -n0;
+          d1 = -n1;
+          d2 = -n2;
+        } else {
+          double cp = cc.cp[k - 1], sp = cc.sp[k - 1];
+          d0 = -(cc.cos_a * n0) + cc.sin_a * (cp * e0 + sp * f0);
+          d1 = -(cc.cos_a * n1) + cc.sin_a * (cp * e1 + sp * f1);
+          d2 = -(cc.cos_a * n2) + cc.sin_a * (cp * e2 + sp * f2);
+        }
+        V3 dir = V3{(float)d0, (float)d1, (float)d2};
+        float t;
+        uint32_t prim;
+        first_hit(sv, o, dir, retest, &t, &prim);
+        if (prim == 0xFFFFFFFFu || !(t - t == 0.0f) || prim >= (uint32_t)nf || t <= cc.min_hit) continue;
+        float ratio = t / cc.thick;
+        if (ratio <= 0.5f || ratio >= 1.8f) continue;
+        double h[3];
+        face_normal(V, F, prim, h);
+        double al = (h[0] * d0 + h[1] * d1) + h[2] * d2;
+        if (al > 0.7) votes += 1;
+      }
+    }
+  }
+  good[q] = votes;
+}
+
+}  // namespace
+
+hipError_t launch_cone(const SceneView &sv, const double *V, const uint32_t *F, int nf, const int64_t *faces,
+                       const double *ns, int n, const ConeConst &cc, int retest, uint8_t *good, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cone, dim3((n + 255) / 256), dim3(256), 0, s, sv, V, F, nf, faces, ns, n, cc, retest, good);
+  return hipGetLastError();
+}
+
+}  // namespace sherd

@@ -10,6 +10,24 @@
 
 namespace sherd {
 
+// Optional per-kernel timing: HIP events recorded on the launching stream around each kernel,
+// read back once the stream has synchronised.
+struct KernelProfiler {
+  bool on = false;
+  static constexpr int MAXK = 16;
+  const char *name[MAXK] = {};
+  double ms[MAXK] = {};
+  uint64_t launches[MAXK] = {};
+  int n = 0;
+  hipEvent_t ev[2 * MAXK] = {};
+  int pending[MAXK] = {};
+  int npending = 0;
+  int slot(const char *k);
+  void begin(const char *k, hipStream_t s);  // record the start event of kernel k
+  void end(hipStream_t s);                    // record the end event of the last begun kernel
+  void collect();                             // after a stream synchronisation
+};
+
 // A 4x4 pose passed by value to a kernel (row-major f64).
 struct Pose {
   double m[16];
@@ -74,6 +92,18 @@ hipError_t launch_icp_rung(const GridView &g, const double *tp, const double *tn
 
 // ---- verification (verify.hip) -------------------------------------------------------------------
 // max_pf / max_s / max_margin: the largest per-fragment sample counts in the batch (grid sizes).
-hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_margin, hipStream_t s);
+hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_margin, hipStream_t s,
+                         KernelProfiler *prof);
+
+// ---- cone vote (cone.hip) -------------------------------------------------------------------------
+struct ConeConst {
+  double cos_a, sin_a;  // of the 15 degree half-angle
+  double cp[6], sp[6];  // of the six cone azimuths 2 pi k / 6
+  float min_hit;        // (0.1 t) as f32
+  float thick;          // t as f32
+};
+
+hipError_t launch_cone(const SceneView &sv, const double *V, const uint32_t *F, int nf, const int64_t *faces,
+                       const double *ns, int n, const ConeConst &cc, int retest, uint8_t *good, hipStream_t s);
 
 }  // namespace sherd

@@ -106,7 +106,6 @@ __global__ void __launch_bounds__(VT) k_seam(VerifyBatch vb) {
   const FragDesc &A = vb.frag[pd.a];
   const FragDesc &B = vb.frag[pd.b];
   __shared__ double sp[SEAM_TILE * 3];
-  __shared__ double sn[SEAM_TILE * 3];
   __shared__ uint32_t s_cells;
   unsigned long long *table = (unsigned long long *)(vb.seam_table + cd.seam_off);
   uint32_t mask = cd.seam_mask;
@@ -130,7 +129,6 @@ __global__ void __launch_bounds__(VT) k_seam(VerifyBatch vb) {
       __syncthreads();
       for (int j = threadIdx.x; j < m; j += VT) {
         apply_f64(cd.T, B.brk_p + 3 * (size_t)(tile + j), sp + 3 * j);
-        rotate_f64(cd.T, B.brk_ns + 3 * (size_t)(tile + j), sn + 3 * j);
       }
       __syncthreads();
 #pragma unroll
@@ -402,21 +400,48 @@ inline int blocks_for(long n, int t) { return (int)((n + t - 1) / t); }
 
 }  // namespace
 
-hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_margin, hipStream_t s) {
+hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_margin, hipStream_t s,
+                         KernelProfiler *prof) {
   if (vb.n_cand <= 0) return hipSuccess;
   int nc = vb.n_cand;
+  auto B = [&](const char *k) {
+    if (prof && prof->on) prof->begin(k, s);
+  };
+  auto E = [&]() {
+    if (prof && prof->on) prof->end(s);
+  };
+  B("k_prep");
   hipLaunchKernelGGL(k_prep, dim3(blocks_for(nc, 64)), dim3(64), 0, s, vb);
-  if (max_pf > 0) hipLaunchKernelGGL(k_frac, dim3(blocks_for(max_pf, VT), 2 * nc), dim3(VT), 0, s, vb);
+  E();
+  if (max_pf > 0) {
+    B("k_frac");
+    hipLaunchKernelGGL(k_frac, dim3(blocks_for(max_pf, VT), 2 * nc), dim3(VT), 0, s, vb);
+    E();
+  }
+  B("k_seam");
   hipLaunchKernelGGL(k_seam, dim3(nc), dim3(VT), 0, s, vb);
+  E();
   if (vb.full) {
-    if (max_margin > 0) hipLaunchKernelGGL(k_cont, dim3(blocks_for(max_margin, VT), nc), dim3(VT), 0, s, vb);
+    if (max_margin > 0) {
+      B("k_cont");
+      hipLaunchKernelGGL(k_cont, dim3(blocks_for(max_margin, VT), nc), dim3(VT), 0, s, vb);
+      E();
+    }
     if (max_s > 0) {
+      B("k_pen");
       hipLaunchKernelGGL(k_pen, dim3(blocks_for(max_s, VT), 2 * nc), dim3(VT), 0, s, vb);
+      E();
+      B("k_pen_min");
       hipLaunchKernelGGL(k_pen_min, dim3(blocks_for(max_s, VT), 2 * nc), dim3(VT), 0, s, vb);
+      E();
     }
   }
+  B("k_median");
   hipLaunchKernelGGL(k_median, dim3(4 * nc), dim3(VT), 0, s, vb);
+  E();
+  B("k_final");
   hipLaunchKernelGGL(k_final, dim3(blocks_for(nc, 64)), dim3(64), 0, s, vb);
+  E();
   return hipGetLastError();
 }
 

@@ -14,7 +14,7 @@ enum {
   VS_BRK_BEST, VS_COUNT
 };
 
-constexpr int SEAM_TILE = 1024;     // moved B breakline points staged in LDS per pass (48 KiB)
+constexpr int SEAM_TILE = 2048;     // moved B breakline points staged in LDS per pass (48 KiB)
 constexpr int MEDIAN_LDS = 12288;   // a median segment up to this many keys is selected from LDS
 constexpr uint32_t FLAG_VOXEL_RANGE = 1u;  // a seam voxel coordinate beyond +-2^20
 

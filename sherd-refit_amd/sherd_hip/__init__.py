@@ -94,6 +94,9 @@ def _bind(L):
         "sherd_fragment_create": (ci, [vp, vp, vp, ci, d, P(d), sz, P(d), sz, P(d), P(d), sz, P(d), P(d), sz,
                                        P(vp)]),
         "sherd_fragment_free": (None, [vp]),
+        "sherd_profile": (ci, [vp, ci]),
+        "sherd_profile_read": (ci, [vp, C.c_char_p, sz]),
+        "sherd_cone_votes": (ci, [vp, vp, P(d), sz, P(u32), sz, P(i64), P(d), sz, d, ci, P(u8)]),
         "sherd_verify_batch": (ci, [vp, P(VerifyPair), sz, P(u32), P(d), sz, ci, ci, P(d), P(u8)]),
     }
     for name, (res, args) in sig.items():
@@ -218,6 +221,16 @@ class HipExecutor:
         _check(lib().sherd_stats(self.ptr, SHERD_M[method], C.byref(s)))
         return dict(calls=s.calls, items=s.items, device_ns=s.device_ns)
 
+    def profile(self, on=True):
+        _check(lib().sherd_profile(self.ptr, int(on)))
+
+    def profile_read(self):
+        import json
+
+        buf = C.create_string_buffer(8192)
+        _check(lib().sherd_profile_read(self.ptr, buf, 8192))
+        return json.loads(buf.value.decode())
+
     def scene(self, V, F, mask=None):
         return Scene(self, V, F, mask)
 
@@ -314,4 +327,17 @@ class HipExecutor:
         _check(lib().sherd_verify_batch(self.ptr, pv, len(pairs), _p(cp, C.c_uint32), _p(T, C.c_double), n,
                                         int(full), int(retest), _p(scores, C.c_double), _p(acc, C.c_uint8)))
         return scores, acc.astype(bool)
+
+    def cone_votes(self, scene, V64, F, faces, ns, thick, retest=False):
+        """segment::classify_faces' seven-ray vote (segment.rs:318-381) for the listed faces."""
+        V64 = f64(V64)
+        F = np.ascontiguousarray(F, dtype=np.uint32).reshape(-1, 3)
+        fl = None if faces is None else np.ascontiguousarray(faces, dtype=np.int64)
+        ns = f64(ns)
+        n = len(F) if fl is None else len(fl)
+        good = np.empty(n, np.uint8)
+        _check(lib().sherd_cone_votes(self.ptr, scene.ptr, _p(V64, C.c_double), len(V64), _p(F, C.c_uint32), len(F),
+                                      _p(fl, C.c_int64), _p(ns, C.c_double), n, float(thick), int(retest),
+                                      _p(good, C.c_uint8)))
+        return good
 

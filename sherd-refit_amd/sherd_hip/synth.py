@@ -223,7 +223,7 @@ def _fragment_data(v, f, frac, t, rng):
     # Shell margin: shell faces whose centroid lies 0.15 t .. 0.6 t from the breakline.
     cen = (a + b + c) / 3.0
     shell_faces = fi[shell]
-    dist, _ = cKDTree(v[np.nonzero(on_frac & on_shell)[0]]).query(cen[shell_faces])
+    dist, _ = cKDTree(v[np.nonzero(on_frac & on_shell)[0]]).query(cen[shell_faces], distance_upper_bound=1.01 * MACRO_OUTER * t, workers=-1)
     band = shell_faces[(dist >= MACRO_INNER * t) & (dist <= MACRO_OUTER * t)]
     margin_p, margin_face, _ = _sample_faces(v, f, band, MARGIN_POINTS, rng)
     return dict(V=v.astype(np.float32), F=f, frac=frac, frac_area=float(frac_area), watertight=True,

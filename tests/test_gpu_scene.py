@@ -158,3 +158,15 @@ def test_random_soups_against_the_oracle(ex, seed):
         g = ex.bounded_distance(dev, pts, T, w)
         o_ = ora.bounded_distance(pts, T, w)
         assert np.array_equal(g, o_)
+
+
+@pytest.mark.parametrize("name", ["pieceA", "pieceB"])
+def test_cone_votes_bit_exact_and_against_the_dump(ex, name):
+    say(f"case: cone votes {name}")
+    pc = S.piece(name)
+    v32 = pc["V"].astype(np.float32)
+    dev = ex.scene(v32, pc["F"])
+    good = ex.cone_votes(dev, pc["V"], pc["F"], pc["seg_faces"], pc["seg_NS"], pc["thick_t"])
+    want = O.Scene(v32, pc["F"]).cone_votes(pc["V"], pc["F"], pc["seg_faces"], pc["seg_NS"], pc["thick_t"])
+    assert np.array_equal(good, want)
+    assert np.mean(good == pc["seg_good"]) > 0.999

@@ -29,9 +29,11 @@ def sides():
 
 
 def pose_gap(a, b, t):
-    dr = a[:, :3, :3] @ np.transpose(b[:, :3, :3], (0, 2, 1))
-    c = np.clip((np.trace(dr, axis1=1, axis2=2) - 1) / 2, -1, 1)
-    return np.degrees(np.arccos(c)), np.linalg.norm(a[:, :3, 3] - b[:, :3, 3], axis=1) / t
+    """Rotation gap in degrees (from |Ra Rb^T - I|_F / sqrt 2, exact to rounding for small angles,
+    where an arccos of the trace bottoms out near 5e-6 degrees) and translation gap in t."""
+    dr = a[:, :3, :3] @ np.transpose(b[:, :3, :3], (0, 2, 1)) - np.eye(3)
+    ang = 2.0 * np.arcsin(np.clip(np.linalg.norm(dr, axis=(1, 2)) / (2.0 * np.sqrt(2.0)), 0, 1))
+    return np.degrees(ang), np.linalg.norm(a[:, :3, 3] - b[:, :3, 3], axis=1) / t
 
 
 def test_coarse_scores_are_bit_exact():
