@@ -48,7 +48,7 @@ def algorithmic_bytes(kernel, frags, pairs, cand_pair):
     for c in cand_pair:
         a, b = pairs[c]
         fa, fb = frags[a], frags[b]
-        if kernel == "k_pen" or kernel == "k_pen_min":
+        if kernel in ("k_pen", "k_pen_seed", "k_pen_min"):
             total += 24 * (len(fa["S"]) + len(fb["S"]))
             used[("mesh", a)] = used[("mesh", b)] = 0
         elif kernel == "k_frac":

@@ -255,16 +255,18 @@ __global__ void __launch_bounds__(VT) k_pen_min(VerifyBatch vb) {
 }
 
 // The k-th smallest key (0-based) of n by an 8-pass MSD radix select.
-__device__ uint64_t select_kth(const uint64_t *keys, int n, int k, uint32_t *hist, uint64_t *s_prefix, int *s_k) {
-  uint64_t prefix = 0, pmask = 0;
+template <class K>
+__device__ K select_kth(const K *keys, int n, int k, uint32_t *hist, K *s_prefix, int *s_k) {
+  constexpr int PASSES = (int)sizeof(K);
+  K prefix = 0, pmask = 0;
   if (threadIdx.x == 0) *s_k = k;
-  for (int pass = 0; pass < 8; ++pass) {
-    int shift = 56 - 8 * pass;
+  for (int pass = 0; pass < PASSES; ++pass) {
+    int shift = 8 * (PASSES - 1) - 8 * pass;
     for (int b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      uint64_t key = keys[i];
-      if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+      K key = keys[i];
+      if ((key & pmask) == prefix) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -275,14 +277,56 @@ __device__ uint64_t select_kth(const uint64_t *keys, int n, int k, uint32_t *his
         kk -= (int)hist[d];
       }
       *s_k = kk;
-      *s_prefix = prefix | ((uint64_t)d << shift);
+      *s_prefix = prefix | ((K)d << shift);
     }
     __syncthreads();
     prefix = *s_prefix;
-    pmask |= (255ull << shift);
+    pmask |= ((K)255 << shift);
     __syncthreads();
   }
   return prefix;
+}
+
+// The closest approach is a minimum over every sample, and a search is cheap once a small
+// window is known. Before the exact pass, every sample walks once from the root to a single leaf,
+// always into the nearer child, and measures that one triangle: a distance the sample really
+// attains, so the smallest of them is at least the true minimum and makes a window the exact pass
+// can prune against. The minimum that comes out is the same; only the work changes.
+__global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb) {
+  int cs = blockIdx.y;
+  int c = cs >> 1, side = cs & 1;
+  const CandDesc &cd = vb.cand[c];
+  const PairDesc &pd = vb.pair[cd.pair];
+  if (!pd.pen_ok) return;
+  CandAcc &acc = vb.acc[c];
+  if (acc.inside[side] != 0) return;
+  const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
+  const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
+  int i = blockIdx.x * VT + threadIdx.x;
+  if (i >= own.n_s) return;
+  const double *T = side == 0 ? cd.Ti : cd.T;
+  const double *p = own.s + 3 * (size_t)i;
+  V3 q = moved_f32(T, p[0], p[1], p[2]);
+  const SceneView &sv = other.mesh;
+  int leaf0 = sv.n - 1, nodes = 2 * sv.n - 1;
+  int cur = sv.root;
+  for (int depth = 0; depth < nodes && cur >= 0 && cur < leaf0; ++depth) {
+    Node nd = load_node(sv.nodes, cur);
+    int l = nd.a;
+    Node ln = load_node(sv.nodes, l);
+    int r = ln.esc;
+    if (r < 0 || r >= nodes) {
+      cur = l;
+      continue;
+    }
+    cur = box_d2(ln, q, 0.0f) <= box_d2(load_node(sv.nodes, r), q, 0.0f) ? l : r;
+  }
+  if (cur < leaf0 || cur >= nodes) return;
+  V3 a, b, cc;
+  uint32_t face;
+  load_tri(sv.tris, load_node(sv.nodes, cur).a, &a, &b, &cc, &face);
+  float d = sqrtf(tri_d2(q, a, b, cc));
+  if (d - d == 0.0f) atomicMin(&acc.closest[side], __float_as_uint(d));
 }
 
 // Rust's f64::midpoint.
@@ -430,6 +474,9 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
     if (max_s > 0) {
       B("k_pen");
       hipLaunchKernelGGL(k_pen, dim3(blocks_for(max_s, VT), 2 * nc), dim3(VT), 0, s, vb);
+      E();
+      B("k_pen_seed");
+      hipLaunchKernelGGL(k_pen_seed, dim3(blocks_for(max_s, VT), 2 * nc), dim3(VT), 0, s, vb);
       E();
       B("k_pen_min");
       hipLaunchKernelGGL(k_pen_min, dim3(blocks_for(max_s, VT), 2 * nc), dim3(VT), 0, s, vb);
