@@ -113,6 +113,11 @@ def main():
     ap.add_argument("--edges-per-t", type=float, default=5.5)
     ap.add_argument("--plain-candidates", type=int, default=280)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched steps (for counter collection); prints no bench line")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="launch one device-to-device copy of this many MiB and exit: the known-size "
+                         "kernel that scales FETCH_SIZE / WRITE_SIZE readings to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -121,6 +126,13 @@ def main():
 
     import sherd_hip
     from sherd_hip import synth
+
+    if args.copy_probe_mib:
+        ex = sherd_hip.HipExecutor(0)
+        nbytes = ex.copy_probe(args.copy_probe_mib << 20)
+        log(f"copy probe: {nbytes} bytes read and {nbytes} bytes written by one launch")
+        ex.close()
+        return
 
     t0 = time.perf_counter()
     col = synth.vessel(edges_per_t=args.edges_per_t)
@@ -156,6 +168,11 @@ def main():
     n_cand = len(cand_T)
     value = n_cand * args.steps / el
     log(f"batched: {n_cand} candidates x {args.steps} steps in {el:.3f}s; accepted {int(accepted.sum())}")
+    if args.batched_only:
+        for k in sorted(prof, key=lambda k: -prof[k]["ms"]):
+            log(f"  {k}: {prof[k]['ms'] / prof[k]['launches']:.4f} ms/launch over {prof[k]['launches']} launches")
+        ex.close()
+        return
 
     # ---- plain: one candidate per call ------------------------------------------------------------
     n_plain = min(args.plain_candidates, n_cand)
@@ -198,7 +215,7 @@ def main():
     pmc = os.path.join(HERE, "profiles", "verify_vessel_pmc.json")
     if os.path.exists(pmc):
         with open(pmc) as fh:
-            traffic = json.load(fh).get(dom, {}).get("bytes_per_launch")
+            traffic = json.load(fh).get("kernels", {}).get(dom, {}).get("bytes_per_launch")
     roof = dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 6), traffic=traffic, kernel=dom,
                 algorithmic_bytes_per_launch=nbytes, avg_launch_ms=round(avg_ms, 4))

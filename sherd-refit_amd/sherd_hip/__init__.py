@@ -96,6 +96,7 @@ def _bind(L):
         "sherd_fragment_free": (None, [vp]),
         "sherd_profile": (ci, [vp, ci]),
         "sherd_profile_read": (ci, [vp, C.c_char_p, sz]),
+        "sherd_copy_probe": (ci, [vp, sz]),
         "sherd_cone_votes": (ci, [vp, vp, P(d), sz, P(u32), sz, P(i64), P(d), sz, d, ci, P(u8)]),
         "sherd_verify_batch": (ci, [vp, P(VerifyPair), sz, P(u32), P(d), sz, ci, ci, P(d), P(u8)]),
     }
@@ -230,6 +231,12 @@ class HipExecutor:
         buf = C.create_string_buffer(8192)
         _check(lib().sherd_profile_read(self.ptr, buf, 8192))
         return json.loads(buf.value.decode())
+
+    def copy_probe(self, nbytes):
+        """One launch of the plain copy kernel k_copy_probe over `nbytes` (rounded down to 16 B);
+        returns the bytes it read (and wrote)."""
+        _check(lib().sherd_copy_probe(self.ptr, int(nbytes)))
+        return int(nbytes) // 16 * 16
 
     def scene(self, V, F, mask=None):
         return Scene(self, V, F, mask)
