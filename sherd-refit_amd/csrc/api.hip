@@ -353,7 +353,9 @@ int sherd_inside(sherd_ctx *ctx, const sherd_scene *scene, const double *pts, si
   if (n == 0) return SHERD_OK;
   if (!pose_ok(T, &P) || !out || !pts) return fail(SHERD_E_ARG, "null buffer");
   if (n > (size_t)INT_MAX) return fail(SHERD_E_LIMIT, "more than 2^31 points");
-  hipError_t e;
+  hipError_t e = scene_parity_grids(&const_cast<sherd_scene *>(scene)->d, c->stream);
+  if (e == hipErrorInvalidValue) return fail(SHERD_E_LIMIT, "parity grid beyond 2^31 entries");
+  if (e != hipSuccess) return hip_fail(e, "parity grids");
   double *d_p = (double *)c->scratch(0, sizeof(double) * 3 * n, &e);
   if (!d_p) return hip_fail(e, "scratch");
   float *d_depth = (float *)c->scratch(1, sizeof(float) * n, &e);

@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <limits>
+#include <utility>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -41,6 +43,47 @@ void free_fragment(sherd_fragment *f) {
   delete f;
 }
 
+// The points of a sample set in Morton order of their own bounding box (10 bits per axis), so
+// that the lanes of a wave query neighbouring places: their traversals share nodes, and the
+// wave-coherent parity walk visits little more than one ray's nodes. Only sets whose statistics
+// are order-free (counts, extremes, key multisets: Pf and S) are reordered.
+std::vector<double> morton_ordered(const double *h, size_t n) {
+  double lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = std::numeric_limits<double>::infinity();
+    hi[k] = -std::numeric_limits<double>::infinity();
+  }
+  for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      double v = h[3 * i + k];
+      if (v < lo[k]) lo[k] = v;
+      if (v > hi[k]) hi[k] = v;
+    }
+  double scale[3];
+  for (int k = 0; k < 3; ++k) scale[k] = hi[k] > lo[k] ? 1023.0 / (hi[k] - lo[k]) : 0.0;
+  auto spread = [](uint32_t v) {
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+  };
+  std::vector<std::pair<uint32_t, uint32_t>> order(n);
+  for (size_t i = 0; i < n; ++i) {
+    uint32_t code = 0;
+    for (int k = 0; k < 3; ++k) {
+      double q = (h[3 * i + k] - lo[k]) * scale[k];
+      uint32_t b = q >= 1023.0 ? 1023u : (q > 0.0 ? (uint32_t)q : 0u);  // NaN lands on 0
+      code |= spread(b) << (2 - k);
+    }
+    order[i] = {code, (uint32_t)i};
+  }
+  std::sort(order.begin(), order.end());
+  std::vector<double> out(3 * n);
+  for (size_t j = 0; j < n; ++j) std::memcpy(&out[3 * j], h + 3 * (size_t)order[j].second, 3 * sizeof(double));
+  return out;
+}
+
 uint32_t pow2_at_least(uint64_t x) {
   uint32_t c = 2;
   while (c < x) c <<= 1;
@@ -65,6 +108,15 @@ int sherd_fragment_create(sherd_ctx *ctx, const sherd_scene *fracture, const she
     return fail(SHERD_E_ARG, "null sample array");
   const size_t LIM = (size_t)1 << 28;
   if (n_pf > LIM || n_s > LIM || n_brk > LIM || n_margin > LIM) return fail(SHERD_E_LIMIT, "sample set too large");
+  if (mesh) {
+    // R §6.4 against this mesh: its parity rays read the parity grids, its closest approach
+    // prunes with the distance bounds.
+    DevScene *md = &const_cast<sherd_scene *>(mesh)->d;
+    hipError_t e = scene_parity_grids(md, c->stream);
+    if (e == hipErrorInvalidValue) return fail(SHERD_E_LIMIT, "parity grid beyond 2^31 entries");
+    if (e != hipSuccess) return hip_fail(e, "parity grids");
+    if ((e = scene_distance_bounds(md, c->stream)) != hipSuccess) return hip_fail(e, "distance bounds");
+  }
   sherd_fragment *f = new sherd_fragment();
   f->ctx = c;
   f->frac = fracture;
@@ -75,8 +127,9 @@ int sherd_fragment_create(sherd_ctx *ctx, const sherd_scene *fracture, const she
   f->n_s = (int)n_s;
   f->n_brk = (int)n_brk;
   f->n_margin = (int)n_margin;
-  hipError_t e = upload_new(pf, n_pf, &f->pf, c->stream);
-  if (e == hipSuccess) e = upload_new(s, n_s, &f->s, c->stream);
+  std::vector<double> pf_sorted = morton_ordered(pf, n_pf), s_sorted = morton_ordered(s, n_s);
+  hipError_t e = upload_new(pf_sorted.data(), n_pf, &f->pf, c->stream);
+  if (e == hipSuccess) e = upload_new(s_sorted.data(), n_s, &f->s, c->stream);
   if (e == hipSuccess) e = upload_new(brk_p, n_brk, &f->brk_p, c->stream);
   if (e == hipSuccess) e = upload_new(brk_ns, n_brk, &f->brk_ns, c->stream);
   if (e == hipSuccess) e = upload_new(margin_p, n_margin, &f->margin_p, c->stream);
@@ -178,7 +231,12 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
   Timer tm(c, SHERD_M_VERIFY, n_cand);
   // Chunks bounded by the key and voxel-table scratch they need (256 MiB of keys per chunk).
   const size_t KEY_BUDGET = (size_t)1 << 25;
+  const size_t HARD_BUDGET = (size_t)1 << 27;  // closest-approach queue entries (1 GiB)
+  // Workgroups per (candidate, side) task of the per-point kernels; a chunk keeps its grid in int.
+  const size_t per_task = (size_t)std::max({max_pf, max_s, max_margin, 1}) / 256 + 1;
+  const size_t MAX_TASK_CAND = (size_t)INT_MAX / (2 * per_task);
   size_t begin = 0;
+  std::vector<uint32_t> task;
   std::vector<CandDesc> cd;
   std::vector<double> hs;
   std::vector<uint8_t> ha;
@@ -191,7 +249,9 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
       const PairDesc &p = pd[cand_pair[end]];
       const FragDesc &A = fd[p.a], &B = fd[p.b];
       size_t need_keys = (size_t)A.n_pf + B.n_pf + 2 * (size_t)B.n_margin;
-      if (end > begin && keys + need_keys > KEY_BUDGET) break;
+      if (end > begin && (keys + need_keys > KEY_BUDGET || end - begin >= MAX_TASK_CAND ||
+                          (end - begin + 1) * 2 * (size_t)max_s > HARD_BUDGET))
+        break;
       CandDesc x;
       std::memset(&x, 0, sizeof x);
       x.pair = (int)cand_pair[end];
@@ -221,7 +281,23 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     if (!dsc) return hip_fail(e, "scratch");
     uint8_t *dac = (uint8_t *)c->scratch(17, nc, &e);
     if (!dac) return hip_fail(e, "scratch");
+    // (candidate, side) tasks grouped by the fragment they query: side 0 queries B, side 1 A.
+    task.resize(2 * nc);
+    for (size_t i = 0; i < 2 * nc; ++i) task[i] = (uint32_t)i;
+    auto queried = [&](uint32_t t) {
+      const PairDesc &p = pd[cd[t >> 1].pair];
+      return (t & 1u) ? p.a : p.b;
+    };
+    std::stable_sort(task.begin(), task.end(), [&](uint32_t x, uint32_t y) { return queried(x) < queried(y); });
+    uint32_t *dtask = (uint32_t *)c->scratch(18, sizeof(uint32_t) * 2 * nc, &e);
+    if (!dtask) return hip_fail(e, "scratch");
+    uint64_t *dhard = (uint64_t *)c->scratch(19, sizeof(uint64_t) * 2 * nc * (size_t)std::max(max_s, 1), &e);
+    if (!dhard) return hip_fail(e, "scratch");
+    uint32_t *dhard_n = (uint32_t *)c->scratch(20, sizeof(uint32_t) * 2, &e);
+    if (!dhard_n) return hip_fail(e, "scratch");
     HIPCK(hipMemcpyAsync(dcd, cd.data(), sizeof(CandDesc) * nc, hipMemcpyHostToDevice, c->stream), "upload");
+    HIPCK(hipMemcpyAsync(dtask, task.data(), sizeof(uint32_t) * 2 * nc, hipMemcpyHostToDevice, c->stream),
+          "upload");
     VerifyBatch vb;
     vb.frag = dfd;
     vb.pair = dpd;
@@ -231,6 +307,9 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     vb.seam_table = dtab;
     vb.scores = dsc;
     vb.accepted = dac;
+    vb.task = dtask;
+    vb.hard = dhard;
+    vb.hard_n = dhard_n;
     vb.n_cand = (int)nc;
     vb.full = full ? 1 : 0;
     vb.retest = retest ? 1 : 0;

@@ -8,9 +8,14 @@
 //   4. leaves in sorted order, then the hierarchy bottom-up in one pass (agglomerative LBVH: each
 //      node joins the neighbour with the longer common key prefix, the second child to arrive
 //      unions the boxes and climbs), min/max unions being order-free, so the tree is deterministic;
-//   5. escape links: the node after subtree [l, r] in pre-order is the right child of split r.
+//   5. escape links: the node after subtree [l, r] in pre-order is the right child of split r;
+//   6. the nodes moved to pre-order (a node's left child is the next node).
+// Parity grids (scene_parity_grids), built on first use by an inside query: per ray axis, a uniform
+// grid over the other two coordinates listing the triangles each cell's rays can cross.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "bvh_query.hpp"
@@ -135,8 +140,8 @@ __device__ __forceinline__ uint64_t delta(const uint32_t *code, int i) {
   return a ^ b;
 }
 
-__global__ void k_hierarchy(const uint32_t *code, int m, Node *nodes, int *other, int *childb, int *range_r,
-                            int *root) {
+__global__ void k_hierarchy(const uint32_t *code, int m, Node *nodes, int *other, int *childb, int *range_l,
+                            int *range_r, int *parent_of, int *root) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
   int cur = m - 1 + j;
@@ -149,6 +154,7 @@ __global__ void k_hierarchy(const uint32_t *code, int m, Node *nodes, int *other
     } else {
       __hip_atomic_store(&childb[parent], cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    parent_of[cur] = parent;
     __threadfence();
     int prev = __hip_atomic_exchange(&other[parent], left ? l : r, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (prev < 0) return;
@@ -169,6 +175,7 @@ __global__ void k_hierarchy(const uint32_t *code, int m, Node *nodes, int *other
       __hip_atomic_store(&nodes[parent].lo[k], lo[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&nodes[parent].hi[k], hi[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    range_l[parent] = l;
     range_r[parent] = r;
     cur = parent;
     if (l == 0 && r == m - 1) {
@@ -183,6 +190,34 @@ __global__ void k_ropes(int m, Node *nodes, const int *childb, const int *range_
   if (i >= 2 * m - 1) return;
   int r = i < m - 1 ? range_r[i] : i - (m - 1);
   nodes[i].esc = (r >= m - 1) ? -1 : childb[r];
+}
+
+// Pre-order position of every node. A node whose leaves start at L is preceded in pre-order by its
+// ancestors and by every node of the subtrees wholly left of it: L leaves and L minus (the number
+// of ancestors it lies right of) internal nodes. That totals 2L + (the number of ancestors whose
+// left subtree holds it), counted here by walking up the parent links.
+__global__ void k_preorder(int m, const Node *nodes, const int *parent_of, const int *range_l, int root, int *pre) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= 2 * m - 1) return;
+  int L = x >= m - 1 ? x - (m - 1) : range_l[x];
+  int lt = 0, c = x;
+  for (int steps = 0; c != root && c >= 0 && steps < 2 * m; ++steps) {
+    int p = parent_of[c];
+    lt += nodes[p].a == c ? 1 : 0;
+    c = p;
+  }
+  pre[x] = 2 * L + lt;
+}
+
+// Moves every node to its pre-order position: an internal node's left child is then the next
+// node, so a descent reads consecutive memory. Leaves get a = ~(triangle slot).
+__global__ void k_relayout(int m, const Node *in, const int *pre, Node *out) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= 2 * m - 1) return;
+  Node nd = in[x];
+  nd.a = x < m - 1 ? pre[nd.a] : ~nd.a;
+  nd.esc = nd.esc < 0 ? -1 : pre[nd.esc];
+  out[pre[x]] = nd;
 }
 
 #define CK(x)                                 \
@@ -208,12 +243,18 @@ hipError_t scene_build(const float *dV, const uint32_t *dF, int m, DevScene *out
   float *cent = nullptr, *part = nullptr, *bounds = nullptr;
   Tri *raw = nullptr;
   uint32_t *code = nullptr, *val = nullptr, *tk = nullptr, *tv = nullptr, *hist = nullptr;
-  int *other = nullptr, *childb = nullptr, *range_r = nullptr, *root = nullptr;
+  int *other = nullptr, *childb = nullptr, *range_l = nullptr, *range_r = nullptr, *root = nullptr;
+  int *parent_of = nullptr, *pre = nullptr;
+  Node *tree = nullptr;  // nodes in build order, before the pre-order relayout
   hipError_t err = hipSuccess;
   int pblocks = blocks_for(m, BT);
   if (pblocks > 1024) pblocks = 1024;
   do {
     if ((err = dalloc(&out->nodes, 2 * (size_t)m - 1)) != hipSuccess) break;
+    if ((err = dalloc(&tree, 2 * (size_t)m - 1)) != hipSuccess) break;
+    if ((err = dalloc(&parent_of, 2 * (size_t)m - 1)) != hipSuccess) break;
+    if ((err = dalloc(&pre, 2 * (size_t)m - 1)) != hipSuccess) break;
+    if ((err = dalloc(&range_l, m)) != hipSuccess) break;
     if ((err = dalloc(&out->tris, m)) != hipSuccess) break;
     if ((err = dalloc(&raw, m)) != hipSuccess) break;
     if ((err = dalloc(&cent, 3 * (size_t)m)) != hipSuccess) break;
@@ -234,28 +275,40 @@ hipError_t scene_build(const float *dV, const uint32_t *dF, int m, DevScene *out
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(m, BT)), dim3(BT), 0, s, cent, m, bounds, code, val);
     if ((err = hipGetLastError()) != hipSuccess) break;
     if ((err = rsort::sort_pairs<uint32_t>(code, val, tk, tv, hist, m, 0, 30, s)) != hipSuccess) break;
-    hipLaunchKernelGGL(k_leaves, dim3(blocks_for(m, BT)), dim3(BT), 0, s, raw, val, m, out->tris, out->nodes, other);
+    hipLaunchKernelGGL(k_leaves, dim3(blocks_for(m, BT)), dim3(BT), 0, s, raw, val, m, out->tris, tree, other);
     int zero = m == 1 ? 0 : -1;
     if ((err = hipMemcpyAsync(root, &zero, sizeof(int), hipMemcpyHostToDevice, s)) != hipSuccess) break;
     if (m > 1) {
-      hipLaunchKernelGGL(k_hierarchy, dim3(blocks_for(m, BT)), dim3(BT), 0, s, code, m, out->nodes, other, childb,
-                         range_r, root);
+      hipLaunchKernelGGL(k_hierarchy, dim3(blocks_for(m, BT)), dim3(BT), 0, s, code, m, tree, other, childb,
+                         range_l, range_r, parent_of, root);
     }
-    hipLaunchKernelGGL(k_ropes, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, out->nodes, childb, range_r);
+    hipLaunchKernelGGL(k_ropes, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, tree, childb, range_r);
     if ((err = hipGetLastError()) != hipSuccess) break;
     float hb[13];
     if ((err = hipMemcpyAsync(hb, bounds, sizeof hb, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
     int r = -1;
     if ((err = hipMemcpyAsync(&r, root, sizeof r, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
     if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
-    out->root = r;
+    if (r < 0 || r >= 2 * m - 1) {
+      err = hipErrorUnknown;
+      break;
+    }
+    hipLaunchKernelGGL(k_preorder, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, tree, parent_of, range_l, r,
+                       pre);
+    hipLaunchKernelGGL(k_relayout, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, tree, pre, out->nodes);
+    if ((err = hipGetLastError()) != hipSuccess) break;
+    if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
+    out->root = 0;
     for (int k = 0; k < 3; ++k) {
       out->lo[k] = hb[6 + k];
       out->hi[k] = hb[9 + k];
     }
     out->absmax = hb[12];
-    if (r < 0) err = hipErrorUnknown;
   } while (0);
+  (void)hipFree(tree);
+  (void)hipFree(parent_of);
+  (void)hipFree(pre);
+  (void)hipFree(range_l);
   (void)hipFree(raw);
   (void)hipFree(cent);
   (void)hipFree(part);
@@ -278,6 +331,273 @@ void scene_free(DevScene *s) {
   if (s->tris) (void)hipFree(s->tris);
   s->nodes = nullptr;
   s->tris = nullptr;
+  for (AxisGrid &g : s->pg) {
+    if (g.start) (void)hipFree(g.start);
+    if (g.item) (void)hipFree(g.item);
+    g = AxisGrid{};
+  }
+  if (s->db.lb) (void)hipFree(s->db.lb);
+  s->db = DistBound{};
+}
+
+// ---- distance bounds ---------------------------------------------------------------------------
+//
+// Cells a triangle's (slack-grown) box overlaps are marked; an exact Euclidean distance transform
+// (Felzenszwalb and Huttenlocher's lower envelope of parabolas, one axis at a time) gives every
+// cell the squared distance d, in cells, from its centre to the nearest marked centre. A point p of
+// the cell and a triangle point q in marked cell c' are each within sqrt(3)/2 cells of their
+// centres, so |p - q| >= (sqrt(d) - sqrt(3)) h: the stored bound, rounded down.
+
+namespace {
+
+constexpr int DB_MAX_CELLS = 1 << 22;
+constexpr float DB_FAR = 1e30f;
+
+__global__ void k_db_fill(float *d, size_t n, float v) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) d[i] = v;
+}
+
+__global__ void k_db_mark(const Tri *tris, int m, DistBound g, float pad, float *d) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const Tri &t = tris[j];
+  int c0[3], c1[3];
+  float inv = 1.0f / g.h;
+  for (int k = 0; k < 3; ++k) {
+    float lo = fminf(t.v[k], fminf(t.v[3 + k], t.v[6 + k])) - pad;
+    float hi = fmaxf(t.v[k], fmaxf(t.v[3 + k], t.v[6 + k])) + pad;
+    c0[k] = grid_cell(lo, g.org[k], inv, g.n[k]);
+    c1[k] = grid_cell(hi, g.org[k], inv, g.n[k]);
+  }
+  for (int z = c0[2]; z <= c1[2]; ++z)
+    for (int y = c0[1]; y <= c1[1]; ++y)
+      for (int x = c0[0]; x <= c1[0]; ++x) d[((size_t)z * g.n[1] + y) * g.n[0] + x] = 0.0f;
+}
+
+// One row along `axis` per thread: d(q) <- min_r (f(r) + (q - r)^2), f being the row's values.
+// v, fv and zb are this row's work arrays (n, n and n + 1 entries): the lower envelope's
+// parabolas (position, value) and the boundaries between them.
+__global__ void k_db_edt(float *d, int nx, int ny, int nz, int axis, int *vwork, float *fwork, double *zwork) {
+  int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+  int rows = axis == 0 ? ny * nz : (axis == 1 ? nx * nz : nx * ny);
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  size_t base, stride;
+  if (axis == 0) {
+    base = (size_t)r * nx;
+    stride = 1;
+  } else if (axis == 1) {
+    int x = r % nx, z = r / nx;
+    base = (size_t)z * ny * nx + x;
+    stride = nx;
+  } else {
+    base = r;
+    stride = (size_t)nx * ny;
+  }
+  int *v = vwork + (size_t)r * n;
+  float *fv = fwork + (size_t)r * n;
+  double *zb = zwork + (size_t)r * (n + 1);
+  int k = -1;
+  for (int q = 0; q < n; ++q) {
+    float fq = d[base + q * stride];
+    if (!(fq < DB_FAR)) continue;
+    // Values are integers below 2^24 (exact in f32); the intersections are formed in f64.
+    double s = -1e300;
+    while (k >= 0) {
+      int p = v[k];
+      s = (((double)fq + (double)q * q) - ((double)fv[k] + (double)p * p)) / (2.0 * (q - p));
+      if (s > zb[k]) break;
+      --k;
+    }
+    ++k;
+    v[k] = q;
+    fv[k] = fq;
+    zb[k] = k == 0 ? -1e300 : s;
+    zb[k + 1] = 1e300;
+  }
+  if (k < 0) return;  // nothing finite in the row: it stays far
+  int j = 0;
+  for (int q = 0; q < n; ++q) {
+    while (zb[j + 1] < (double)q) ++j;
+    float dq = (float)(q - v[j]);
+    d[base + q * stride] = dq * dq + fv[j];
+  }
+}
+
+// Squared cell distances to the stored bounds.
+__global__ void k_db_final(float *d, size_t n, float h) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float v = d[i];
+  if (!(v < DB_FAR)) {
+    d[i] = DB_FAR;
+    return;
+  }
+  float b = (sqrtf(v) - 1.7320509f) * h * 0.999f;
+  d[i] = b > 0.0f ? b : 0.0f;
+}
+
+}  // namespace
+
+// ---- parity grids ------------------------------------------------------------------------------
+
+namespace {
+
+constexpr double PG_CELLS_PER_TRI = 0.5;  // cells per triangle of each parity grid
+constexpr int PG_MAX_CELLS = 1 << 26;
+
+__device__ __forceinline__ void pg_range(const Tri &t, int axis, const AxisGrid &g, float pad, int *u0, int *u1,
+                                         int *v0, int *v1) {
+  int U = axis == 0 ? 1 : 0, W = axis == 2 ? 1 : 2;
+  float lu = fminf(t.v[U], fminf(t.v[3 + U], t.v[6 + U])) - pad;
+  float hu = fmaxf(t.v[U], fmaxf(t.v[3 + U], t.v[6 + U])) + pad;
+  float lv = fminf(t.v[W], fminf(t.v[3 + W], t.v[6 + W])) - pad;
+  float hv = fmaxf(t.v[W], fmaxf(t.v[3 + W], t.v[6 + W])) + pad;
+  *u0 = grid_cell(lu, g.org_u, g.inv, g.nu);
+  *u1 = grid_cell(hu, g.org_u, g.inv, g.nu);
+  *v0 = grid_cell(lv, g.org_v, g.inv, g.nv);
+  *v1 = grid_cell(hv, g.org_v, g.inv, g.nv);
+}
+
+__global__ void k_pg_count(const Tri *tris, int m, AxisGrid g, int axis, float pad, uint32_t *count) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  int u0, u1, v0, v1;
+  pg_range(tris[j], axis, g, pad, &u0, &u1, &v0, &v1);
+  for (int cu = u0; cu <= u1; ++cu)
+    for (int cv = v0; cv <= v1; ++cv) atomicAdd(&count[(size_t)cu * g.nv + cv], 1u);
+}
+
+__global__ void k_pg_fill(const Tri *tris, int m, AxisGrid g, int axis, float pad, uint32_t *cursor) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  int u0, u1, v0, v1;
+  pg_range(tris[j], axis, g, pad, &u0, &u1, &v0, &v1);
+  for (int cu = u0; cu <= u1; ++cu)
+    for (int cv = v0; cv <= v1; ++cv) g.item[atomicAdd(&cursor[(size_t)cu * g.nv + cv], 1u)] = (uint32_t)j;
+}
+
+}  // namespace
+
+hipError_t scene_distance_bounds(DevScene *sc, hipStream_t s) {
+  if (sc->db.lb) return hipSuccess;
+  DistBound g{};
+  double ext[3], vol = 1.0;
+  for (int k = 0; k < 3; ++k) {
+    ext[k] = std::max((double)sc->hi[k] - sc->lo[k], 1e-6 * std::max(1.0, (double)sc->absmax));
+    vol *= ext[k];
+  }
+  double h = std::cbrt(vol / DB_MAX_CELLS);
+  for (int k = 0; k < 3; ++k) h = std::max(h, ext[k] / 1000.0);
+  for (int tries = 0; tries < 8; ++tries) {
+    double cells = 1.0;
+    for (int k = 0; k < 3; ++k) cells *= std::ceil(ext[k] / h) + 2.0;
+    if (cells <= 1.2 * DB_MAX_CELLS) break;
+    h *= 1.05;
+  }
+  g.h = (float)h;
+  size_t ncell = 1;
+  for (int k = 0; k < 3; ++k) {
+    g.n[k] = (int)std::ceil(ext[k] / h) + 2;
+    g.org[k] = (float)((double)sc->lo[k] - h);
+    ncell *= (size_t)g.n[k];
+  }
+  float pad = 1e-5f * sc->absmax + 1e-30f;
+  int *vwork = nullptr;
+  float *fwork = nullptr;
+  double *zwork = nullptr;
+  hipError_t err = hipSuccess;
+  do {
+    if ((err = dalloc(&g.lb, ncell)) != hipSuccess) break;
+    // Row r of a pass uses entries [r n, r n + n) and [r (n + 1), r (n + 1) + n + 1).
+    if ((err = dalloc(&vwork, ncell)) != hipSuccess) break;
+    if ((err = dalloc(&fwork, ncell)) != hipSuccess) break;
+    if ((err = dalloc(&zwork, 2 * ncell + 1)) != hipSuccess) break;
+    hipLaunchKernelGGL(k_db_fill, dim3(blocks_for((long)ncell, BT)), dim3(BT), 0, s, g.lb, ncell, DB_FAR);
+    hipLaunchKernelGGL(k_db_mark, dim3(blocks_for(sc->n, BT)), dim3(BT), 0, s, sc->tris, sc->n, g, pad, g.lb);
+    for (int axis = 0; axis < 3; ++axis) {
+      long nrows = (long)(ncell / (size_t)g.n[axis]);
+      hipLaunchKernelGGL(k_db_edt, dim3(blocks_for(nrows, 64)), dim3(64), 0, s, g.lb, g.n[0], g.n[1], g.n[2], axis,
+                         vwork, fwork, zwork);
+    }
+    hipLaunchKernelGGL(k_db_final, dim3(blocks_for((long)ncell, BT)), dim3(BT), 0, s, g.lb, ncell, g.h);
+    if ((err = hipGetLastError()) != hipSuccess) break;
+    err = hipStreamSynchronize(s);
+  } while (0);
+  if (vwork) (void)hipFree(vwork);
+  if (fwork) (void)hipFree(fwork);
+  if (zwork) (void)hipFree(zwork);
+  if (err != hipSuccess) {
+    if (g.lb) (void)hipFree(g.lb);
+    return err;
+  }
+  sc->db = g;
+  return hipSuccess;
+}
+
+hipError_t scene_parity_grids(DevScene *sc, hipStream_t s) {
+  if (sc->pg[0].start) return hipSuccess;
+  int m = sc->n;
+  // The slack every parity ray prunes with: its origin is inside the scene's box (the AABB reject
+  // comes first), so slack_of gives 1e-5 * absmax.
+  float pad = 1e-5f * sc->absmax + 1e-30f;
+  hipError_t err = hipSuccess;
+  uint32_t *cursor = nullptr;
+  for (int axis = 0; axis < 3 && err == hipSuccess; ++axis) {
+    int U = axis == 0 ? 1 : 0, W = axis == 2 ? 1 : 2;
+    double eu = (double)sc->hi[U] - sc->lo[U], ev = (double)sc->hi[W] - sc->lo[W];
+    double ext = std::max(eu, ev);
+    double fl = ext > 0.0 ? ext * 1e-6 : 1.0;
+    eu = std::max(eu, fl);
+    ev = std::max(ev, fl);
+    double cell = std::sqrt(eu * ev / (PG_CELLS_PER_TRI * m));
+    double nu = std::ceil(eu / cell), nv = std::ceil(ev / cell);
+    if (nu * nv > PG_MAX_CELLS) {
+      cell *= std::sqrt(nu * nv / PG_MAX_CELLS) * 1.01;
+      nu = std::ceil(eu / cell);
+      nv = std::ceil(ev / cell);
+    }
+    AxisGrid g{};
+    g.nu = (int)std::max(1.0, nu);
+    g.nv = (int)std::max(1.0, nv);
+    g.org_u = sc->lo[U];
+    g.org_v = sc->lo[W];
+    g.inv = (float)(1.0 / cell);
+    size_t ncell = (size_t)g.nu * g.nv;
+    do {
+      if ((err = dalloc(&g.start, ncell + 1)) != hipSuccess) break;
+      if ((err = hipMemsetAsync(g.start, 0, sizeof(uint32_t) * (ncell + 1), s)) != hipSuccess) break;
+      hipLaunchKernelGGL(k_pg_count, dim3(blocks_for(m, BT)), dim3(BT), 0, s, sc->tris, m, g, axis, pad, g.start);
+      hipLaunchKernelGGL(rsort::k_scan, dim3(1), dim3(1024), 0, s, g.start, (int)(ncell + 1));
+      if ((err = hipGetLastError()) != hipSuccess) break;
+      uint32_t total = 0;
+      if ((err = hipMemcpyAsync(&total, g.start + ncell, sizeof total, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
+      if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
+      if (total > (1u << 31)) {
+        err = hipErrorInvalidValue;
+        break;
+      }
+      if ((err = dalloc(&g.item, total)) != hipSuccess) break;
+      if ((err = dalloc(&cursor, ncell)) != hipSuccess) break;
+      if ((err = hipMemcpyAsync(cursor, g.start, sizeof(uint32_t) * ncell, hipMemcpyDeviceToDevice, s)) != hipSuccess)
+        break;
+      hipLaunchKernelGGL(k_pg_fill, dim3(blocks_for(m, BT)), dim3(BT), 0, s, sc->tris, m, g, axis, pad, cursor);
+      if ((err = hipGetLastError()) != hipSuccess) break;
+      if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
+    } while (0);
+    if (cursor) (void)hipFree(cursor);
+    cursor = nullptr;
+    sc->pg[axis] = g;
+  }
+  if (err != hipSuccess) {
+    for (AxisGrid &g : sc->pg) {
+      if (g.start) (void)hipFree(g.start);
+      if (g.item) (void)hipFree(g.item);
+      g = AxisGrid{};
+    }
+  }
+  return err;
 }
 
 SceneView view_of(const DevScene &d) {
@@ -291,6 +611,8 @@ SceneView view_of(const DevScene &d) {
     v.hi[k] = d.hi[k];
   }
   v.absmax = d.absmax;
+  for (int k = 0; k < 3; ++k) v.pg[k] = d.pg[k];
+  v.db = d.db;
   return v;
 }
 

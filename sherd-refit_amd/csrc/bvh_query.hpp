@@ -61,19 +61,18 @@ __device__ inline void first_hit_stackless(const SceneView &sv, V3 o, V3 d, int 
   dm = dm < absf(d.z) ? absf(d.z) : dm;
   float tpad = pad / (dm > 0.0f ? dm : 1.0f);
   V3 inv = V3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-  int leaf0 = sv.n - 1;
   int nodes = 2 * sv.n - 1, steps = 0;
   int i = sv.root;
   // Pre-order visits each node at most once; the cap only bounds a corrupted tree.
   while (i >= 0 && i < nodes && steps++ < nodes) {
     Node nd = load_node(sv.nodes, i);
-    if (i < leaf0) {
+    if (nd.a >= 0) {
       float te = ray_enter(nd, o, d, inv, pad, best + tpad);
       i = te >= 0.0f ? nd.a : nd.esc;
     } else {
       V3 a, b, c;
       uint32_t face;
-      load_tri(sv.tris, nd.a, &a, &b, &c, &face);
+      load_tri(sv.tris, ~nd.a, &a, &b, &c, &face);
       float t;
       if (mt_hit(o, d, a, b, c, retest, &t)) {
         if (t < best || (t == best && face < bface)) {
@@ -100,7 +99,6 @@ __device__ inline void first_hit(const SceneView &sv, V3 o, V3 d, int retest, fl
   dm = dm < absf(d.z) ? absf(d.z) : dm;
   float tpad = pad / (dm > 0.0f ? dm : 1.0f);
   V3 inv = V3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-  int leaf0 = sv.n - 1;
   int nodes = 2 * sv.n - 1;
   int cur = sv.root;
   *t_out = best;
@@ -114,10 +112,10 @@ __device__ inline void first_hit(const SceneView &sv, V3 o, V3 d, int retest, fl
   for (int iter = 0; iter <= nodes; ++iter) {
     Node nd = load_node(sv.nodes, cur);
     int next = -1;
-    if (cur >= leaf0) {
+    if (nd.a < 0) {
       V3 a, b, c;
       uint32_t face;
-      load_tri(sv.tris, nd.a, &a, &b, &c, &face);
+      load_tri(sv.tris, ~nd.a, &a, &b, &c, &face);
       float t;
       if (mt_hit(o, d, a, b, c, retest, &t)) {
         if (t < best || (t == best && face < bface)) {
@@ -165,32 +163,37 @@ __device__ inline void first_hit(const SceneView &sv, V3 o, V3 d, int retest, fl
   *face_out = bface;
 }
 
-// Crossings of the ray from o along +axis (RayScene::crossings, bvh.rs:160-178): every hit.
-__device__ inline uint32_t crossings_axis(const SceneView &sv, V3 o, int axis, int retest) {
-  float pad = slack_of(sv.absmax, o);
+// Crossings of the ray from o along +AXIS (RayScene::crossings, bvh.rs:160-178): every hit, from
+// the triangles of the ray's cell in the scene's parity grid for that axis. The cell lists every
+// triangle whose slack-grown box the ray's line meets, so this counts what an exhaustive scan
+// would. Triangles wholly behind the origin (by more than the slack) cannot be hit and are skipped
+// before the intersection test. *nearest gets the smallest hit parameter (+inf without a hit).
+template <int AXIS>
+__device__ inline uint32_t crossings_axis(const SceneView &sv, V3 o, int retest, float *nearest) {
+  constexpr int U = AXIS == 0 ? 1 : 0, W = AXIS == 2 ? 1 : 2;
+  const AxisGrid &g = sv.pg[AXIS];
   float oo[3] = {o.x, o.y, o.z};
-  int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
-  V3 d = V3{axis == 0 ? 1.0f : 0.0f, axis == 1 ? 1.0f : 0.0f, axis == 2 ? 1.0f : 0.0f};
+  float pad = slack_of(sv.absmax, o);
+  int cell = grid_cell(oo[U], g.org_u, g.inv, g.nu) * g.nv + grid_cell(oo[W], g.org_v, g.inv, g.nv);
+  uint32_t b = g.start[cell], e = g.start[cell + 1];
+  V3 d = V3{AXIS == 0 ? 1.0f : 0.0f, AXIS == 1 ? 1.0f : 0.0f, AXIS == 2 ? 1.0f : 0.0f};
   uint32_t hits = 0;
-  int leaf0 = sv.n - 1;
-  int nodes = 2 * sv.n - 1, steps = 0;
-  int i = sv.root;
-  // Pre-order visits each node at most once; the cap only bounds a corrupted tree.
-  while (i >= 0 && i < nodes && steps++ < nodes) {
-    Node nd = load_node(sv.nodes, i);
-    if (i < leaf0) {
-      bool in = oo[a1] >= nd.lo[a1] - pad && oo[a1] <= nd.hi[a1] + pad && oo[a2] >= nd.lo[a2] - pad &&
-                oo[a2] <= nd.hi[a2] + pad && nd.hi[axis] + pad >= oo[axis];
-      i = in ? nd.a : nd.esc;
-    } else {
-      V3 a, b, c;
-      uint32_t face;
-      load_tri(sv.tris, nd.a, &a, &b, &c, &face);
-      float t;
-      hits += (uint32_t)mt_hit(o, d, a, b, c, retest, &t);
-      i = nd.esc;
+  float tmin = __builtin_inff();
+  for (uint32_t k = b; k < e; ++k) {
+    V3 v0, v1, v2;
+    uint32_t face;
+    load_tri(sv.tris, (int)g.item[k], &v0, &v1, &v2, &face);
+    float x0 = AXIS == 0 ? v0.x : (AXIS == 1 ? v0.y : v0.z);
+    float x1 = AXIS == 0 ? v1.x : (AXIS == 1 ? v1.y : v1.z);
+    float x2 = AXIS == 0 ? v2.x : (AXIS == 1 ? v2.y : v2.z);
+    if (fmaxf(x0, fmaxf(x1, x2)) + pad < oo[AXIS]) continue;
+    float t;
+    if (mt_hit(o, d, v0, v1, v2, retest, &t)) {
+      ++hits;
+      tmin = t < tmin ? t : tmin;
     }
   }
+  *nearest = tmin;
   return hits;
 }
 
@@ -201,18 +204,17 @@ __device__ inline float min_d2_stackless(const SceneView &sv, V3 p, float window
   float slack = slack_of(sv.absmax, p);
   float lim = window + slack;
   float lim2 = lim * lim;
-  int leaf0 = sv.n - 1;
   int nodes = 2 * sv.n - 1, steps = 0;
   int i = sv.root;
   // Pre-order visits each node at most once; the cap only bounds a corrupted tree.
   while (i >= 0 && i < nodes && steps++ < nodes) {
     Node nd = load_node(sv.nodes, i);
-    if (i < leaf0) {
+    if (nd.a >= 0) {
       i = box_d2(nd, p, slack) <= lim2 ? nd.a : nd.esc;
     } else {
       V3 a, b, c;
       uint32_t face;
-      load_tri(sv.tris, nd.a, &a, &b, &c, &face);
+      load_tri(sv.tris, ~nd.a, &a, &b, &c, &face);
       float d2 = tri_d2(p, a, b, c);
       if (d2 < best) {
         best = d2;
@@ -230,12 +232,15 @@ __device__ inline float min_d2_stackless(const SceneView &sv, V3 p, float window
 // minimum found since has moved past it. In this layout the right child of an internal node is the
 // escape link of its left child, so no child pointer beyond the tree's own is needed. Should the
 // stack ever fill, the exhaustive walk above finishes the query, so the answer is the same.
-__device__ inline float min_d2(const SceneView &sv, V3 p, float window) {
+//
+// With a step budget (steps > 0) the walk gives up after that many nodes: *complete is then false
+// and the result is only the smallest distance met so far, one the point really attains.
+__device__ inline float min_d2(const SceneView &sv, V3 p, float window, int steps = 0, bool *complete = nullptr) {
+  if (complete) *complete = true;
   float best = __builtin_inff();
   float slack = slack_of(sv.absmax, p);
   float lim = window + slack;
   float lim2 = lim * lim;
-  int leaf0 = sv.n - 1;
   int nodes = 2 * sv.n - 1;
   int cur = sv.root;
   if (cur < 0 || cur >= nodes) return best;
@@ -245,12 +250,16 @@ __device__ inline float min_d2(const SceneView &sv, V3 p, float window) {
   int sp = 0;
   bool overflow = false;
   for (int iter = 0; iter <= nodes; ++iter) {
+    if (steps > 0 && iter >= steps) {
+      *complete = false;
+      return best;
+    }
     Node nd = load_node(sv.nodes, cur);
     int next = -1;
-    if (cur >= leaf0) {
+    if (nd.a < 0) {
       V3 a, b, c;
       uint32_t face;
-      load_tri(sv.tris, nd.a, &a, &b, &c, &face);
+      load_tri(sv.tris, ~nd.a, &a, &b, &c, &face);
       float d2 = tri_d2(p, a, b, c);
       if (d2 < best) {
         best = d2;
@@ -296,6 +305,81 @@ __device__ inline float min_d2(const SceneView &sv, V3 p, float window) {
   return best;
 }
 
+// min_d2 for one point by a whole wave: p and window are wave-uniform, and the wave keeps its
+// pending nodes in a stack in LDS (stk, cap entries, this wave's own), expanding up to 64 of them
+// per step, one per lane. Children are pushed when their box is within the current bound, leaves
+// are measured, and the wave's minimum tightens the bound. The result is the same minimum; a query
+// whose frontier would overflow the stack has the overflowing lanes finish their subtrees alone.
+// Every lane of the wave calls it.
+__device__ inline float min_d2_wave(const SceneView &sv, V3 p, float window, int *stk, int cap) {
+  int lane = threadIdx.x & 63;
+  float slack = slack_of(sv.absmax, p);
+  float best = __builtin_inff();
+  int nodes = 2 * sv.n - 1;
+  if (sv.root < 0 || sv.root >= nodes) return best;
+  {
+    float lim = window + slack;
+    if (box_d2(load_node(sv.nodes, sv.root), p, slack) > lim * lim) return best;
+  }
+  if (lane == 0) stk[0] = sv.root;
+  int sp = 1;
+  for (int guard = 0; sp > 0 && guard <= nodes; ++guard) {
+    float lim = mnf(sqrtf(best), window) + slack;
+    float lim2 = lim * lim;
+    int take = sp < 64 ? sp : 64;
+    int node = lane < take ? stk[sp - take + lane] : -1;
+    sp -= take;
+    __builtin_amdgcn_wave_barrier();
+    float mine = __builtin_inff();
+    int c0 = -1, c1 = -1;
+    if (node >= 0 && node < nodes) {
+      Node nd = load_node(sv.nodes, node);
+      if (nd.a < 0) {
+        V3 a, b, c;
+        uint32_t face;
+        load_tri(sv.tris, ~nd.a, &a, &b, &c, &face);
+        mine = tri_d2(p, a, b, c);
+      } else if (nd.a < nodes) {
+        int l = nd.a;
+        Node ln = load_node(sv.nodes, l);
+        int r = ln.esc;
+        if (box_d2(ln, p, slack) <= lim2) c0 = l;
+        if (r >= 0 && r < nodes && box_d2(load_node(sv.nodes, r), p, slack) <= lim2) c1 = r;
+      }
+    }
+    uint32_t wb = wave_min(__float_as_uint(mine));
+    float wm = __uint_as_float(wb);
+    best = wm < best ? wm : best;
+    uint64_t m0 = __ballot(c0 >= 0), m1 = __ballot(c1 >= 0);
+    int n0 = (int)__popcll(m0), n1 = (int)__popcll(m1);
+    if (sp + n0 + n1 <= cap) {
+      if (c0 >= 0) stk[sp + (int)lanes_below(m0)] = c0;
+      if (c1 >= 0) stk[sp + n0 + (int)lanes_below(m1)] = c1;
+      sp += n0 + n1;
+    } else {
+      // No room: each lane searches its own children's subtrees (the same bound).
+      float w = mnf(sqrtf(best), window);
+      float own = __builtin_inff();
+      if (c0 >= 0) {
+        SceneView sub = sv;
+        sub.root = c0;
+        own = min_d2(sub, p, w);
+      }
+      if (c1 >= 0) {
+        SceneView sub = sv;
+        sub.root = c1;
+        float e = min_d2(sub, p, w);
+        own = e < own ? e : own;
+      }
+      uint32_t ob = wave_min(__float_as_uint(own));
+      float om = __uint_as_float(ob);
+      best = om < best ? om : best;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  return best;
+}
+
 // RayScene::bounded_distance (bvh.rs:123-135): the distance when it is at most max_dist.
 __device__ __forceinline__ bool bounded_distance(const SceneView &sv, V3 p, float max_dist, float *d) {
   if (!(max_dist > 0.0f)) return false;
@@ -313,12 +397,25 @@ __device__ __forceinline__ bool inside_depth(const SceneView &sv, V3 p, int rete
     return false;
   // Majority of three odd counts: two agreeing rays decide it, and the third is cast only when
   // the first two disagree.
-  int v0 = (int)(crossings_axis(sv, p, 0, retest) & 1u);
-  int v1 = (int)(crossings_axis(sv, p, 1, retest) & 1u);
+  float t0, t1, t2 = __builtin_inff();
+  int v0 = (int)(crossings_axis<0>(sv, p, retest, &t0) & 1u);
+  int v1 = (int)(crossings_axis<1>(sv, p, retest, &t1) & 1u);
   int votes = v0 + v1;
-  if (v0 != v1) votes += (int)(crossings_axis(sv, p, 2, retest) & 1u);
+  if (v0 != v1) votes += (int)(crossings_axis<2>(sv, p, retest, &t2) & 1u);
   if (votes < 2) return false;
-  *depth = sqrtf(min_d2(sv, p, __builtin_inff()));
+  // The depth is the distance to the nearest triangle. A ray's nearest crossing is a point of the
+  // surface t away, so a search bounded a little beyond the smallest t finds the same minimum
+  // (any triangle within the bound is measured); only if that bound somehow missed does the
+  // unbounded search run.
+  float w = fminf(t0, fminf(t1, t2));
+  float d2 = __builtin_inff();
+  if (w < __builtin_inff()) {
+    w = w * 1.0001f + slack_of(sv.absmax, p);
+    d2 = min_d2(sv, p, w);
+    if (!(d2 <= w * w)) d2 = __builtin_inff();
+  }
+  if (d2 == __builtin_inff()) d2 = min_d2(sv, p, __builtin_inff());
+  *depth = sqrtf(d2);
   return true;
 }
 

@@ -153,9 +153,10 @@ SH_HD V3 moved_f32(const double *T, double px, double py, double pz) {
 }
 
 // ---- BVH layout --------------------------------------------------------------------------------
-// Nodes 0..n-2 are internal (node i splits sorted leaves i | i+1), nodes n-1..2n-2 are leaves.
-// `a` is the left child (internal) or the triangle slot (leaf); `esc` is the next node in
-// pre-order after this node's subtree, or -1. 32 bytes, one dwordx4 pair per node.
+// 2n-1 nodes in pre-order, the root at 0. `a` >= 0 marks an internal node and is its left child
+// (always the next node); a leaf holds a = ~(its triangle slot). `esc` is the next node in
+// pre-order after this node's subtree, or -1, so the right child of an internal node is the escape
+// of its left child. 32 bytes, one dwordx4 pair per node.
 struct Node {
   float lo[3];
   int a;
@@ -170,9 +171,30 @@ struct Tri {
   uint32_t pad[2];
 };
 
+// The triangles an axis-parallel ray can cross, by the cell of the ray's two other coordinates:
+// cell (cu, cv) of a uniform grid over the scene's box lists every triangle whose box, grown by
+// the pruning slack, overlaps it. A +axis ray's crossings are the hits among its cell's triangles.
+struct AxisGrid {
+  uint32_t *start;  // nu * nv + 1 offsets into item
+  uint32_t *item;   // triangle slots
+  float org_u, org_v, inv;
+  int nu, nv;
+};
+
+// A lower bound of the distance to the scene from anywhere: a uniform grid over the scene's box
+// (one cell of margin) holding, per cell, a bound no point of the cell can beat; outside the grid
+// the distance to the grid's box bounds it.
+struct DistBound {
+  float *lb;  // n[0] * n[1] * n[2] bounds, x fastest
+  float org[3], h;
+  int n[3];
+};
+
 struct SceneView {
   const Node *nodes;
   const Tri *tris;
+  AxisGrid pg[3];  // parity grids, one per ray axis (null start until built)
+  DistBound db;    // distance bounds (null lb until built)
   int n;         // triangles
   int root;      // root node index
   float lo[3];   // aabb of the scene's vertices (RayScene::aabb)
@@ -181,6 +203,35 @@ struct SceneView {
 };
 
 SH_HD V3 tri_v(const Tri &t, int k) { return V3{t.v[3 * k], t.v[3 * k + 1], t.v[3 * k + 2]}; }
+
+// Cell of coordinate x on a grid axis: floor((x - org) * inv) clamped to [0, n - 1], computed the
+// same way when a grid is filled and when it is queried, and monotone in x.
+SH_HD int grid_cell(float x, float org, float inv, int n) {
+  float f = (x - org) * inv;
+  if (!(f >= 0.0f)) return 0;
+  if (f >= (float)n) return n - 1;
+  int c = (int)f;
+  return c < n ? c : n - 1;
+}
+
+// A lower bound of the distance from p to the scene (see DistBound); 0 where nothing is known.
+// Outside the grid's box, p' (p clamped to the box) has the bound of its cell, and for any scene
+// point q inside the box |p - q|^2 >= |p - p'|^2 + |p' - q|^2 (projection onto a convex set).
+SH_HD float dist_bound(const DistBound &g, V3 p) {
+  if (!g.lb) return 0.0f;
+  float q[3] = {p.x, p.y, p.z};
+  float out2 = 0.0f;
+  int c[3];
+  for (int k = 0; k < 3; ++k) {
+    float lo = g.org[k], hi = g.org[k] + g.h * (float)g.n[k];
+    float e = q[k] < lo ? lo - q[k] : (q[k] > hi ? q[k] - hi : 0.0f);
+    out2 += e * e;
+    c[k] = grid_cell(q[k], g.org[k], 1.0f / g.h, g.n[k]);
+  }
+  float in = g.lb[((size_t)c[2] * g.n[1] + c[1]) * g.n[0] + c[0]];
+  if (out2 == 0.0f) return in;
+  return sqrtf(out2 + in * in) * 0.999f;
+}
 
 // The pruning slack: ~80 ulps of the largest coordinate in play, more than the f32 rounding of
 // any distance or hit parameter computed above, so a box can never hide the triangle that attains
@@ -219,6 +270,49 @@ SH_HD uint64_t total_key(double v) {
 SH_HD double from_total_key(uint64_t k) {
   uint64_t u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
   return __builtin_bit_cast(double, u);
+}
+
+// ---- wave64 helpers ----------------------------------------------------------------------------
+// How many lanes below this one are set in m.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Consecutive slots of *counter for the lanes that take one, with one atomic per wave. Every
+// active lane of the wave calls it.
+__device__ __forceinline__ uint32_t wave_slot(uint32_t *counter, bool take) {
+  uint64_t m = __ballot(take);
+  if (m == 0) return 0;
+  uint32_t rank = lanes_below(m);
+  uint32_t base = 0;
+  if (take && rank == 0) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, __ffsll((unsigned long long)m) - 1);
+  return base + rank;
+}
+
+// Maximum and minimum over the 64 lanes; every lane of the wave must be active.
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t w = (uint32_t)__shfl_xor((int)v, o);
+    v = v > w ? v : w;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t w = (uint32_t)__shfl_xor((int)v, o);
+    v = v < w ? v : w;
+  }
+  return v;
+}
+
+// MI355X dispatches consecutive workgroups of a launch to its 8 XCDs in turn, and each XCD has
+// its own L2. This maps the physical workgroup id to a logical one so that each XCD runs one
+// contiguous run of the logical order: work laid out by shared data then shares one L2.
+constexpr uint32_t N_XCD = 8;
+__device__ __forceinline__ uint32_t xcd_logical(uint32_t phys, uint32_t n) {
+  uint32_t q = n / N_XCD, r = n % N_XCD, x = phys % N_XCD, k = phys / N_XCD;
+  return x * q + (x < r ? x : r) + k;
 }
 
 }  // namespace sherd

@@ -37,6 +37,8 @@ struct Pose {
 struct DevScene {
   Node *nodes = nullptr;
   Tri *tris = nullptr;
+  AxisGrid pg[3] = {};  // built on demand by scene_parity_grids
+  DistBound db = {};    // built on demand by scene_distance_bounds
   int n = 0;
   int root = -1;
   float lo[3] = {0, 0, 0};
@@ -49,6 +51,10 @@ struct DevScene {
 hipError_t scene_build(const float *dV, const uint32_t *dF, int m, DevScene *out, hipStream_t s);
 void scene_free(DevScene *s);
 SceneView view_of(const DevScene &d);
+// Builds the scene's three parity grids unless built; synchronises the stream.
+hipError_t scene_parity_grids(DevScene *sc, hipStream_t s);
+// Builds the scene's distance bounds unless built; synchronises the stream.
+hipError_t scene_distance_bounds(DevScene *sc, hipStream_t s);
 
 hipError_t launch_first_hit(const SceneView &sv, const float *orig, const float *dir, int n, int retest, float *t,
                             uint32_t *prim, hipStream_t s);

@@ -14,6 +14,7 @@
 //   k_cont     R §6.3: B's margin against A's margin grid, steps and normal agreements
 //   k_pen      R §6.4: surface samples inside the other mesh, deeper-than-pen count, deepest
 //   k_pen_min  R §6.4's fallback when nothing is inside: the closest approach
+//   k_pen_hard the closest-approach queries too long for one lane, one wave each
 //   k_median   the four medians (numpy semantics, verify.rs:693-700)
 //   k_final    Scores and the accept verdict
 #include <hip/hip_runtime.h>
@@ -30,6 +31,9 @@ namespace {
 enum { S_T, S_RES, S_COARSE, S_STAGE1, S_TIGHT, S_FACING, S_GAP, S_SEAM, S_NEAR, S_PEN, S_NMS, S_ICP };
 
 constexpr int VT = 256;
+constexpr int PEN_STEPS = 192;     // nodes one lane walks for a closest-approach query before a wave takes over
+constexpr int HARD_STACK = 2048;   // pending nodes of a wave's cooperative query (LDS, 8 KiB per wave)
+constexpr int HARD_BLOCKS = 1024;  // workgroups of the cooperative pass
 
 __device__ __forceinline__ void apply_f64(const double *T, const double *p, double *o) {
   o[0] = T[0] * p[0] + T[1] * p[1] + T[2] * p[2] + T[3];
@@ -42,6 +46,18 @@ __device__ __forceinline__ void rotate_f64(const double *T, const double *p, dou
   o[2] = T[8] * p[0] + T[9] * p[1] + T[10] * p[2];
 }
 __device__ __forceinline__ double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// The (candidate, side) task and the point of this thread in a launch of per_task workgroups per
+// task. The tasks are ordered by the fragment they query (VerifyBatch::task), and xcd_logical keeps
+// a run of them on one XCD, so the workgroups that walk one mesh share that XCD's L2.
+struct TaskPos {
+  int c, side, i;
+};
+__device__ __forceinline__ TaskPos task_pos(const VerifyBatch &vb, int per_task) {
+  uint32_t lb = xcd_logical(blockIdx.x, gridDim.x);
+  uint32_t cs = vb.task[lb / (uint32_t)per_task];
+  return TaskPos{(int)(cs >> 1), (int)(cs & 1u), (int)((lb % (uint32_t)per_task) * VT + threadIdx.x)};
+}
 
 __global__ void k_prep(VerifyBatch vb) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -56,20 +72,20 @@ __global__ void k_prep(VerifyBatch vb) {
     a.closest[s] = 0x7f7fffffu;  // f32::MAX, the window of the fallback (verify.rs:570-577)
   }
   a.n_cont = 0;
+  if (c == 0) vb.hard_n[0] = vb.hard_n[1] = 0;
   a.seam_cells = 0;
   a.flags = 0;
 }
 
 // R §6.1 one_side inputs (verify.rs:318-349): side 0 is A's samples moved by T^-1 against B's
 // fracture, side 1 is B's samples moved by T against A's fracture.
-__global__ void __launch_bounds__(VT) k_frac(VerifyBatch vb) {
-  int cs = blockIdx.y;
-  int c = cs >> 1, side = cs & 1;
+__global__ void __launch_bounds__(VT) k_frac(VerifyBatch vb, int per_task) {
+  TaskPos tp = task_pos(vb, per_task);
+  int c = tp.c, side = tp.side, i = tp.i;
   const CandDesc &cd = vb.cand[c];
   const PairDesc &pd = vb.pair[cd.pair];
   const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
   const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
-  int i = blockIdx.x * VT + threadIdx.x;
   bool valid = i < own.n_pf;
   bool facing = false, tight = false, touching = false;
   double d64 = 0.0;
@@ -85,10 +101,8 @@ __global__ void __launch_bounds__(VT) k_frac(VerifyBatch vb) {
     touching = d64 < 2.0 * pd.sc[S_TIGHT];
   }
   CandAcc &acc = vb.acc[c];
-  if (facing) {
-    uint32_t slot = atomicAdd(&acc.facing[side], 1u);
-    vb.keys[cd.seg[side] + slot] = total_key(d64);
-  }
+  uint32_t slot = wave_slot(&acc.facing[side], facing);
+  if (facing) vb.keys[cd.seg[side] + slot] = total_key(d64);
   uint32_t nt = (uint32_t)__popcll(__ballot(tight)), nh = (uint32_t)__popcll(__ballot(touching));
   if ((threadIdx.x & 63) == 0) {
     if (nt) atomicAdd(&acc.tight[side], nt);
@@ -198,22 +212,21 @@ __global__ void __launch_bounds__(VT) k_cont(VerifyBatch vb) {
   double turned[3];
   rotate_f64(cd.T, B.margin_n + 3 * (size_t)i, turned);
   double agree = dot3(turned, nn);
-  uint32_t slot = atomicAdd(&vb.acc[c].n_cont, 1u);
+  uint32_t slot = wave_slot(&vb.acc[c].n_cont, true);
   vb.keys[cd.seg[2] + slot] = total_key(step);
   vb.keys[cd.seg[3] + slot] = total_key(agree);
 }
 
 // R §6.4 one_penetration's first pass (verify.rs:541-566): side 0 is A's surface samples moved by
 // T^-1 against B's mesh, side 1 is B's moved by T against A's mesh.
-__global__ void __launch_bounds__(VT) k_pen(VerifyBatch vb) {
-  int cs = blockIdx.y;
-  int c = cs >> 1, side = cs & 1;
+__global__ void __launch_bounds__(VT) k_pen(VerifyBatch vb, int per_task) {
+  TaskPos tp = task_pos(vb, per_task);
+  int c = tp.c, side = tp.side, i = tp.i;
   const CandDesc &cd = vb.cand[c];
   const PairDesc &pd = vb.pair[cd.pair];
   const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
   const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
   if (!pd.pen_ok) return;
-  int i = blockIdx.x * VT + threadIdx.x;
   bool in = false, deeper = false;
   float depth = 0.0f;
   if (i < own.n_s) {
@@ -224,19 +237,20 @@ __global__ void __launch_bounds__(VT) k_pen(VerifyBatch vb) {
     deeper = in && (double)depth > pd.sc[S_PEN];
   }
   CandAcc &acc = vb.acc[c];
-  if (in) atomicMax(&acc.deepest[side], __float_as_uint(depth));
+  uint32_t deepest = wave_max(in ? __float_as_uint(depth) : 0u);
   uint32_t ni = (uint32_t)__popcll(__ballot(in)), nd = (uint32_t)__popcll(__ballot(deeper));
   if ((threadIdx.x & 63) == 0) {
     if (ni) atomicAdd(&acc.inside[side], ni);
     if (nd) atomicAdd(&acc.deeper[side], nd);
+    if (deepest) atomicMax(&acc.deepest[side], deepest);
   }
 }
 
 // verify.rs:567-578: nothing inside, so min(sd) is the closest approach, a running minimum whose
 // window shrinks to the best so far (DistReduce::Min, cpu.rs:100-111), from f32::MAX.
-__global__ void __launch_bounds__(VT) k_pen_min(VerifyBatch vb) {
-  int cs = blockIdx.y;
-  int c = cs >> 1, side = cs & 1;
+__global__ void __launch_bounds__(VT) k_pen_min(VerifyBatch vb, int per_task) {
+  TaskPos tp = task_pos(vb, per_task);
+  int c = tp.c, side = tp.side, i = tp.i;
   const CandDesc &cd = vb.cand[c];
   const PairDesc &pd = vb.pair[cd.pair];
   if (!pd.pen_ok) return;
@@ -244,14 +258,29 @@ __global__ void __launch_bounds__(VT) k_pen_min(VerifyBatch vb) {
   if (acc.inside[side] != 0) return;
   const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
   const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
-  int i = blockIdx.x * VT + threadIdx.x;
-  if (i >= own.n_s) return;
-  const double *T = side == 0 ? cd.Ti : cd.T;
-  const double *p = own.s + 3 * (size_t)i;
-  V3 q = moved_f32(T, p[0], p[1], p[2]);
-  float window = __uint_as_float(__hip_atomic_load(&acc.closest[side], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  float d;
-  if (bounded_distance(other.mesh, q, window, &d)) atomicMin(&acc.closest[side], __float_as_uint(d));
+  uint32_t best = 0xffffffffu;
+  bool hard = false;
+  if (i < own.n_s) {
+    const double *T = side == 0 ? cd.Ti : cd.T;
+    const double *p = own.s + 3 * (size_t)i;
+    V3 q = moved_f32(T, p[0], p[1], p[2]);
+    float window =
+        __uint_as_float(__hip_atomic_load(&acc.closest[side], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    // A sample whose distance bound already exceeds the window (by more than the slack, which
+    // covers the rounding of the moved point) cannot lower the minimum.
+    if (dist_bound(other.mesh.db, q) <= window + slack_of(other.mesh.absmax, q) && window > 0.0f) {
+      // A query that needs more than PEN_STEPS nodes is finished by a whole wave (k_pen_hard);
+      // what it met so far is a distance it attains, so it may still lower the window.
+      bool complete;
+      float dist = sqrtf(min_d2(other.mesh, q, window, PEN_STEPS, &complete));
+      if (dist <= window) best = __float_as_uint(dist);
+      hard = !complete;
+    }
+  }
+  uint32_t slot = wave_slot(&vb.hard_n[0], hard);
+  if (hard) vb.hard[slot] = ((uint64_t)c << 33) | ((uint64_t)side << 32) | (uint32_t)i;
+  best = wave_min(best);
+  if ((threadIdx.x & 63) == 0 && best != 0xffffffffu) atomicMin(&acc.closest[side], best);
 }
 
 // The k-th smallest key (0-based) of n by an 8-pass MSD radix select.
@@ -292,9 +321,9 @@ __device__ K select_kth(const K *keys, int n, int k, uint32_t *hist, K *s_prefix
 // always into the nearer child, and measures that one triangle: a distance the sample really
 // attains, so the smallest of them is at least the true minimum and makes a window the exact pass
 // can prune against. The minimum that comes out is the same; only the work changes.
-__global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb) {
-  int cs = blockIdx.y;
-  int c = cs >> 1, side = cs & 1;
+__global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb, int per_task) {
+  TaskPos tp = task_pos(vb, per_task);
+  int c = tp.c, side = tp.side, i = tp.i;
   const CandDesc &cd = vb.cand[c];
   const PairDesc &pd = vb.pair[cd.pair];
   if (!pd.pen_ok) return;
@@ -302,31 +331,75 @@ __global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb) {
   if (acc.inside[side] != 0) return;
   const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
   const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
-  int i = blockIdx.x * VT + threadIdx.x;
-  if (i >= own.n_s) return;
-  const double *T = side == 0 ? cd.Ti : cd.T;
-  const double *p = own.s + 3 * (size_t)i;
-  V3 q = moved_f32(T, p[0], p[1], p[2]);
+  uint32_t best = 0xffffffffu;
   const SceneView &sv = other.mesh;
-  int leaf0 = sv.n - 1, nodes = 2 * sv.n - 1;
-  int cur = sv.root;
-  for (int depth = 0; depth < nodes && cur >= 0 && cur < leaf0; ++depth) {
-    Node nd = load_node(sv.nodes, cur);
-    int l = nd.a;
-    Node ln = load_node(sv.nodes, l);
-    int r = ln.esc;
-    if (r < 0 || r >= nodes) {
-      cur = l;
-      continue;
-    }
-    cur = box_d2(ln, q, 0.0f) <= box_d2(load_node(sv.nodes, r), q, 0.0f) ? l : r;
+  V3 q = V3{0.0f, 0.0f, 0.0f};
+  bool go = false;
+  if (i < own.n_s) {
+    const double *T = side == 0 ? cd.Ti : cd.T;
+    const double *p = own.s + 3 * (size_t)i;
+    q = moved_f32(T, p[0], p[1], p[2]);
+    // No need for a bound from a sample that cannot beat the best bound so far.
+    float window =
+        __uint_as_float(__hip_atomic_load(&acc.closest[side], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    go = dist_bound(sv.db, q) <= window;
   }
-  if (cur < leaf0 || cur >= nodes) return;
-  V3 a, b, cc;
-  uint32_t face;
-  load_tri(sv.tris, load_node(sv.nodes, cur).a, &a, &b, &cc, &face);
-  float d = sqrtf(tri_d2(q, a, b, cc));
-  if (d - d == 0.0f) atomicMin(&acc.closest[side], __float_as_uint(d));
+  if (go) {
+    int nodes = 2 * sv.n - 1;
+    int cur = sv.root;
+    Node nd = load_node(sv.nodes, cur);
+    for (int depth = 0; depth < nodes && nd.a >= 0 && nd.a < nodes; ++depth) {
+      int l = nd.a;
+      Node ln = load_node(sv.nodes, l);
+      int r = ln.esc;
+      if (r < 0 || r >= nodes) {
+        cur = l;
+        nd = ln;
+        continue;
+      }
+      Node rn = load_node(sv.nodes, r);
+      bool go_left = box_d2(ln, q, 0.0f) <= box_d2(rn, q, 0.0f);
+      cur = go_left ? l : r;
+      nd = go_left ? ln : rn;
+    }
+    if (nd.a < 0) {
+      V3 a, b, cc;
+      uint32_t face;
+      load_tri(sv.tris, ~nd.a, &a, &b, &cc, &face);
+      float d = sqrtf(tri_d2(q, a, b, cc));
+      if (d - d == 0.0f) best = __float_as_uint(d);
+    }
+  }
+  best = wave_min(best);
+  if ((threadIdx.x & 63) == 0 && best != 0xffffffffu) atomicMin(&acc.closest[side], best);
+}
+
+// The closest-approach queries k_pen_min left unfinished, one per wave (min_d2_wave), taken from
+// the queue until it is empty.
+__global__ void __launch_bounds__(VT) k_pen_hard(VerifyBatch vb) {
+  __shared__ int stk[VT / 64][HARD_STACK];
+  int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint32_t n = vb.hard_n[0];
+  for (;;) {
+    uint32_t idx = 0;
+    if (lane == 0) idx = atomicAdd(&vb.hard_n[1], 1u);
+    idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
+    if (idx >= n) break;
+    uint64_t e = vb.hard[idx];
+    int c = (int)(e >> 33), side = (int)((e >> 32) & 1u), i = (int)(uint32_t)e;
+    const CandDesc &cd = vb.cand[c];
+    const PairDesc &pd = vb.pair[cd.pair];
+    CandAcc &acc = vb.acc[c];
+    const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
+    const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
+    const double *T = side == 0 ? cd.Ti : cd.T;
+    const double *p = own.s + 3 * (size_t)i;
+    V3 q = moved_f32(T, p[0], p[1], p[2]);
+    float window = __uint_as_float(__builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(&acc.closest[side], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+    float dist = sqrtf(min_d2_wave(other.mesh, q, window, stk[w], HARD_STACK));
+    if (lane == 0 && dist <= window) atomicMin(&acc.closest[side], __float_as_uint(dist));
+  }
 }
 
 // Rust's f64::midpoint.
@@ -458,8 +531,9 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
   hipLaunchKernelGGL(k_prep, dim3(blocks_for(nc, 64)), dim3(64), 0, s, vb);
   E();
   if (max_pf > 0) {
+    int per = blocks_for(max_pf, VT);
     B("k_frac");
-    hipLaunchKernelGGL(k_frac, dim3(blocks_for(max_pf, VT), 2 * nc), dim3(VT), 0, s, vb);
+    hipLaunchKernelGGL(k_frac, dim3(2 * nc * per), dim3(VT), 0, s, vb, per);
     E();
   }
   B("k_seam");
@@ -472,14 +546,18 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
       E();
     }
     if (max_s > 0) {
+      int per = blocks_for(max_s, VT);
       B("k_pen");
-      hipLaunchKernelGGL(k_pen, dim3(blocks_for(max_s, VT), 2 * nc), dim3(VT), 0, s, vb);
+      hipLaunchKernelGGL(k_pen, dim3(2 * nc * per), dim3(VT), 0, s, vb, per);
       E();
       B("k_pen_seed");
-      hipLaunchKernelGGL(k_pen_seed, dim3(blocks_for(max_s, VT), 2 * nc), dim3(VT), 0, s, vb);
+      hipLaunchKernelGGL(k_pen_seed, dim3(2 * nc * per), dim3(VT), 0, s, vb, per);
       E();
       B("k_pen_min");
-      hipLaunchKernelGGL(k_pen_min, dim3(blocks_for(max_s, VT), 2 * nc), dim3(VT), 0, s, vb);
+      hipLaunchKernelGGL(k_pen_min, dim3(2 * nc * per), dim3(VT), 0, s, vb, per);
+      E();
+      B("k_pen_hard");
+      hipLaunchKernelGGL(k_pen_hard, dim3(HARD_BLOCKS), dim3(VT), 0, s, vb);
       E();
     }
   }

@@ -71,6 +71,9 @@ struct VerifyBatch {
   uint64_t *seam_table;  // per-candidate voxel hash sets
   double *scores;        // VS_COUNT per candidate
   uint8_t *accepted;
+  const uint32_t *task;  // 2 * n_cand (candidate << 1 | side), grouped by the fragment queried
+  uint64_t *hard;        // closest-approach queries left to whole waves: candidate << 33 | side << 32 | sample
+  uint32_t *hard_n;      // [0] queued, [1] taken
   int n_cand;
   int full;
   int retest;

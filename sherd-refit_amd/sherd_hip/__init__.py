@@ -16,7 +16,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libsherd_hip.so")
+# SHERD_HIP_LIB names another build of the library (for instance a variant being compared).
+LIB_PATH = os.environ.get("SHERD_HIP_LIB") or os.path.join(HERE, "lib", "libsherd_hip.so")
 
 SHERD_M = dict(first_hit=0, bounded_distance=1, inside=2, coarse=3, icp=4, nearest=5, verify=6,
                scene_build=7, cone=8)
