@@ -62,4 +62,6 @@ struct sherd_fragment {
   int n_pf = 0, n_s = 0, n_brk = 0, n_margin = 0;
   sherd::DevGrid margin_grid;  // rebuilt when a batch needs a larger reach
   double margin_reach = 0.0;
+  sherd::DevGrid brk_grid;     // over brk_p in the fragment's frame, likewise
+  double brk_reach = 0.0;
 };

@@ -40,6 +40,7 @@ void free_fragment(sherd_fragment *f) {
   for (double *p : ptrs)
     if (p) (void)hipFree(p);
   grid_free(&f->margin_grid);
+  grid_free(&f->brk_grid);
   delete f;
 }
 
@@ -162,10 +163,10 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
   if (!pairs || !cand_pair || !cand_T || !scores || !accepted) return fail(SHERD_E_ARG, "null buffer");
   if (n_pairs > (size_t)INT_MAX || n_cand > (size_t)INT_MAX / 4) return fail(SHERD_E_LIMIT, "batch too large");
   // Fragments of the batch, and the reach each one's margin grid must cover (its largest `near`
-  // as side A, verify.rs:456-462).
+  // as side A, verify.rs:456-462) and its breakline grid (its largest `seam` as side B).
   std::map<const sherd_fragment *, int> fidx;
   std::vector<sherd_fragment *> frags;
-  std::vector<double> need;
+  std::vector<double> need, need_brk;
   for (size_t p = 0; p < n_pairs; ++p) {
     const sherd_fragment *ab[2] = {pairs[p].a, pairs[p].b};
     for (int k = 0; k < 2; ++k) {
@@ -174,15 +175,19 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
         fidx[ab[k]] = (int)frags.size();
         frags.push_back(const_cast<sherd_fragment *>(ab[k]));
         need.push_back(0.0);
+        need_brk.push_back(0.0);
       }
     }
     double nr = pairs[p].scales[8];
     int ia = fidx[pairs[p].a];
     if (nr > need[ia]) need[ia] = nr;
+    double sr = pairs[p].scales[7];
+    int ib = fidx[pairs[p].b];
+    if (sr > need_brk[ib]) need_brk[ib] = sr;
   }
   for (size_t i = 0; i < n_cand; ++i)
     if (cand_pair[i] >= n_pairs) return fail(SHERD_E_ARG, "candidate names a pair out of range");
-  int max_pf = 0, max_s = 0, max_margin = 0;
+  int max_pf = 0, max_s = 0, max_margin = 0, max_brk = 0;
   std::vector<FragDesc> fd(frags.size());
   for (size_t i = 0; i < frags.size(); ++i) {
     sherd_fragment *f = frags[i];
@@ -190,6 +195,14 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
       grid_free(&f->margin_grid);
       HIPCK(grid_build(f->margin_p, f->n_margin, need[i], &f->margin_grid, c->stream), "margin grid");
       f->margin_reach = need[i];
+    }
+    // The breakline grid is queried at A's points taken into B's frame (T^-1 p), which differ from
+    // the exact preimage by rounding: the reach carries a relative margin far above it.
+    double brk_need = need_brk[i] * (1.0 + 1e-6);
+    if (f->n_brk > 0 && need_brk[i] > 0.0 && (f->brk_grid.pts == nullptr || f->brk_reach < brk_need)) {
+      grid_free(&f->brk_grid);
+      HIPCK(grid_build(f->brk_p, f->n_brk, brk_need, &f->brk_grid, c->stream), "breakline grid");
+      f->brk_reach = brk_need;
     }
     FragDesc &d = fd[i];
     std::memset(&d, 0, sizeof d);
@@ -209,9 +222,11 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     d.n_brk = f->n_brk;
     d.n_margin = f->n_margin;
     d.margin = view_of(f->margin_grid);
+    d.brk = view_of(f->brk_grid);
     max_pf = std::max(max_pf, f->n_pf);
     max_s = std::max(max_s, f->n_s);
     max_margin = std::max(max_margin, f->n_margin);
+    max_brk = std::max(max_brk, f->n_brk);
   }
   std::vector<PairDesc> pd(n_pairs);
   for (size_t p = 0; p < n_pairs; ++p) {
@@ -233,8 +248,9 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
   const size_t KEY_BUDGET = (size_t)1 << 25;
   const size_t HARD_BUDGET = (size_t)1 << 27;  // closest-approach queue entries (1 GiB)
   // Workgroups per (candidate, side) task of the per-point kernels; a chunk keeps its grid in int.
-  const size_t per_task = (size_t)std::max({max_pf, max_s, max_margin, 1}) / 256 + 1;
-  const size_t MAX_TASK_CAND = (size_t)INT_MAX / (2 * per_task);
+  const size_t per_task = (size_t)std::max({max_pf, max_s, max_margin, max_brk, 1}) / 256 + 1;
+  // Candidates per chunk: a grid's y extent (65535) and x extent in int.
+  const size_t MAX_TASK_CAND = std::min<size_t>(65535, (size_t)INT_MAX / (2 * per_task));
   size_t begin = 0;
   std::vector<uint32_t> task;
   std::vector<CandDesc> cd;
@@ -313,7 +329,7 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     vb.n_cand = (int)nc;
     vb.full = full ? 1 : 0;
     vb.retest = retest ? 1 : 0;
-    HIPCK(launch_verify(vb, max_pf, max_s, max_margin, c->stream, &c->prof), "verify kernels");
+    HIPCK(launch_verify(vb, max_pf, max_s, max_margin, max_brk, c->stream, &c->prof), "verify kernels");
     hs.resize(VS_COUNT * nc);
     ha.resize(nc);
     hacc.resize(nc);

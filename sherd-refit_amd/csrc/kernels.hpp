@@ -98,7 +98,7 @@ hipError_t launch_icp_rung(const GridView &g, const double *tp, const double *tn
 
 // ---- verification (verify.hip) -------------------------------------------------------------------
 // max_pf / max_s / max_margin: the largest per-fragment sample counts in the batch (grid sizes).
-hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_margin, hipStream_t s,
+hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_margin, int max_brk, hipStream_t s,
                          KernelProfiler *prof);
 
 // ---- cone vote (cone.hip) -------------------------------------------------------------------------

@@ -10,7 +10,8 @@
 // Kernels, each over all candidates of the batch:
 //   k_prep     pose inverse per candidate (verify.rs:739-798), counters cleared
 //   k_frac     R §6.1: both sides' fracture samples against the other's fracture scene
-//   k_seam     R §6.2: A's breakline against the moved B breakline, voxel set of the seam points
+//   k_seam     R §6.2: A's breakline against the moved B breakline (through B's breakline grid),
+//              voxel set of the seam points
 //   k_cont     R §6.3: B's margin against A's margin grid, steps and normal agreements
 //   k_pen      R §6.4: surface samples inside the other mesh, deeper-than-pen count, deepest
 //   k_pen_min  R §6.4's fallback when nothing is inside: the closest approach
@@ -110,86 +111,111 @@ __global__ void __launch_bounds__(VT) k_frac(VerifyBatch vb, int per_task) {
   }
 }
 
+// The nearest point of B's breakline moved by T to p (A's frame), with kiddo's f64 distance to
+// T b and ties to the lowest index: exactly what a scan over all of B would pick, provided that
+// point lies within the breakline grid's reach. The grid is over B's own frame and is entered at
+// T^-1 p; a point within the reach of p in A's frame is within the reach (plus rounding, which the
+// reach's margin covers) of T^-1 p in B's, so it is among the 27 cells searched.
+__device__ inline bool seam_nearest(const GridView &g, const double *T, const double *Ti, const double *bp,
+                                    const double *p, uint32_t *out_j, double *out_d2) {
+  if (g.n <= 0) return false;
+  double q[3];
+  apply_f64(Ti, p, q);
+  double fx = floor((q[0] - g.origin[0]) / g.cell);
+  double fy = floor((q[1] - g.origin[1]) / g.cell);
+  double fz = floor((q[2] - g.origin[2]) / g.cell);
+  if (!(fx >= -1.0 && fx <= (double)GRID_MAXC && fy >= -1.0 && fy <= (double)GRID_MAXC && fz >= -1.0 &&
+        fz <= (double)GRID_MAXC))
+    return false;
+  int64_t cx = (int64_t)fx, cy = (int64_t)fy, cz = (int64_t)fz;
+  double best = __builtin_inf();
+  uint32_t bj = 0xFFFFFFFFu;
+  for (int dx = -1; dx <= 1; ++dx) {
+    int64_t x = cx + dx;
+    if (x < 0 || x >= GRID_MAXC) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      int64_t y = cy + dy;
+      if (y < 0 || y >= GRID_MAXC) continue;
+      for (int dz = -1; dz <= 1; ++dz) {
+        int64_t z = cz + dz;
+        if (z < 0 || z >= GRID_MAXC) continue;
+        uint64_t key = grid_pack(x, y, z);
+        uint32_t slot = grid_hash(key, g.mask);
+        for (uint32_t probe = 0; probe <= g.mask; ++probe) {
+          uint64_t k = g.keys[slot];
+          if (k == GRID_EMPTY) break;
+          if (k == key) {
+            uint2 cl = g.cells[slot];
+            for (uint32_t j = cl.x; j < cl.x + cl.y; ++j) {
+              uint32_t oi = g.idx[j];
+              double m[3];
+              apply_f64(T, bp + 3 * (size_t)oi, m);
+              double d2 = d2_f64(p[0], p[1], p[2], m[0], m[1], m[2]);
+              if (d2 < best || (d2 == best && oi < bj)) {
+                best = d2;
+                bj = oi;
+              }
+            }
+            break;
+          }
+          slot = (slot + 1) & g.mask;
+        }
+      }
+    }
+  }
+  if (bj == 0xFFFFFFFFu) return false;
+  *out_j = bj;
+  *out_d2 = best;
+  return true;
+}
+
+// Clears each candidate's voxel set.
+__global__ void __launch_bounds__(VT) k_seam_clear(VerifyBatch vb) {
+  const CandDesc &cd = vb.cand[blockIdx.x];
+  unsigned long long *table = (unsigned long long *)(vb.seam_table + cd.seam_off);
+  for (uint32_t s = threadIdx.x; s <= cd.seam_mask; s += VT) table[s] = ~0ull;
+}
+
 // R §6.2 seam_points / seam_cells (verify.rs:378-442): for each point of A's breakline, the nearest
 // moved point of B's breakline (f64, ties to the lowest index); a seam point when it is strictly
 // inside sc.seam and the two shell normals agree; the score counts distinct t/3 voxels of them.
 __global__ void __launch_bounds__(VT) k_seam(VerifyBatch vb) {
-  int c = blockIdx.x;
+  int c = blockIdx.y;
   const CandDesc &cd = vb.cand[c];
   const PairDesc &pd = vb.pair[cd.pair];
   const FragDesc &A = vb.frag[pd.a];
   const FragDesc &B = vb.frag[pd.b];
-  __shared__ double sp[SEAM_TILE * 3];
-  __shared__ uint32_t s_cells;
+  int i = blockIdx.x * VT + threadIdx.x;
+  double seam = pd.sc[S_SEAM];
+  if (i >= A.n_brk || !(seam > 0.0)) return;
+  const double *p = A.brk_p + 3 * (size_t)i;
+  uint32_t j;
+  double d2;
+  if (!seam_nearest(B.brk, cd.T, cd.Ti, B.brk_p, p, &j, &d2)) return;
+  if (!(sqrt(d2) < seam)) return;
+  double nb[3];
+  rotate_f64(cd.T, B.brk_ns + 3 * (size_t)j, nb);
+  if (!(dot3(A.brk_ns + 3 * (size_t)i, nb) > 0.7)) return;
+  double voxel = pd.sc[S_T] / 3.0;
+  int64_t x = (int64_t)floor(p[0] / voxel), y = (int64_t)floor(p[1] / voxel), z = (int64_t)floor(p[2] / voxel);
+  const int64_t H = 1ll << 20;
+  if (x < -H || x >= H || y < -H || y >= H || z < -H || z >= H) {
+    atomicOr(&vb.acc[c].flags, FLAG_VOXEL_RANGE);
+    return;
+  }
   unsigned long long *table = (unsigned long long *)(vb.seam_table + cd.seam_off);
   uint32_t mask = cd.seam_mask;
-  for (uint32_t s = threadIdx.x; s <= mask; s += VT) table[s] = ~0ull;
-  if (threadIdx.x == 0) s_cells = 0;
-  __syncthreads();
-  double voxel = pd.sc[S_T] / 3.0;
-  double seam = pd.sc[S_SEAM];
-  // Each thread owns A's points t, t + VT, ...; its running best survives across the tiles.
-  constexpr int PER = 16;
-  for (int base = 0; base < A.n_brk; base += VT * PER) {
-    double best[PER];
-    uint32_t bj[PER];
-#pragma unroll
-    for (int r = 0; r < PER; ++r) {
-      best[r] = __builtin_inf();
-      bj[r] = 0xFFFFFFFFu;
+  uint64_t key = (uint64_t)(x + H) | ((uint64_t)(y + H) << 21) | ((uint64_t)(z + H) << 42);
+  uint32_t slot = grid_hash(key, mask);
+  for (uint32_t probe = 0; probe <= mask; ++probe) {
+    unsigned long long prev = atomicCAS(&table[slot], ~0ull, (unsigned long long)key);
+    if (prev == ~0ull) {
+      atomicAdd(&vb.acc[c].seam_cells, 1u);
+      break;
     }
-    for (int tile = 0; tile < B.n_brk; tile += SEAM_TILE) {
-      int m = B.n_brk - tile < SEAM_TILE ? B.n_brk - tile : SEAM_TILE;
-      __syncthreads();
-      for (int j = threadIdx.x; j < m; j += VT) {
-        apply_f64(cd.T, B.brk_p + 3 * (size_t)(tile + j), sp + 3 * j);
-      }
-      __syncthreads();
-#pragma unroll
-      for (int r = 0; r < PER; ++r) {
-        int i = base + r * VT + threadIdx.x;
-        if (i >= A.n_brk) continue;
-        const double *q = A.brk_p + 3 * (size_t)i;
-        double q0 = q[0], q1 = q[1], q2 = q[2];
-        for (int j = 0; j < m; ++j) {
-          double d2 = d2_f64(q0, q1, q2, sp[3 * j], sp[3 * j + 1], sp[3 * j + 2]);
-          if (d2 < best[r]) {  // ascending j: the first minimum is the lowest index
-            best[r] = d2;
-            bj[r] = (uint32_t)(tile + j);
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < PER; ++r) {
-      int i = base + r * VT + threadIdx.x;
-      if (i >= A.n_brk || bj[r] == 0xFFFFFFFFu) continue;
-      if (!(seam > 0.0) || !(sqrt(best[r]) < seam)) continue;
-      double nb[3];
-      rotate_f64(cd.T, B.brk_ns + 3 * (size_t)bj[r], nb);
-      if (!(dot3(A.brk_ns + 3 * (size_t)i, nb) > 0.7)) continue;
-      const double *p = A.brk_p + 3 * (size_t)i;
-      int64_t x = (int64_t)floor(p[0] / voxel), y = (int64_t)floor(p[1] / voxel), z = (int64_t)floor(p[2] / voxel);
-      const int64_t H = 1ll << 20;
-      if (x < -H || x >= H || y < -H || y >= H || z < -H || z >= H) {
-        atomicOr(&vb.acc[c].flags, FLAG_VOXEL_RANGE);
-        continue;
-      }
-      uint64_t key = (uint64_t)(x + H) | ((uint64_t)(y + H) << 21) | ((uint64_t)(z + H) << 42);
-      uint32_t slot = grid_hash(key, mask);
-      for (uint32_t probe = 0; probe <= mask; ++probe) {
-        unsigned long long prev = atomicCAS(&table[slot], ~0ull, (unsigned long long)key);
-        if (prev == ~0ull) {
-          atomicAdd(&s_cells, 1u);
-          break;
-        }
-        if (prev == key) break;
-        slot = (slot + 1) & mask;
-      }
-    }
+    if (prev == key) break;
+    slot = (slot + 1) & mask;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) vb.acc[c].seam_cells = s_cells;
 }
 
 // R §6.3 continuity_scores (verify.rs:450-474).
@@ -517,7 +543,7 @@ inline int blocks_for(long n, int t) { return (int)((n + t - 1) / t); }
 
 }  // namespace
 
-hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_margin, hipStream_t s,
+hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_margin, int max_brk, hipStream_t s,
                          KernelProfiler *prof) {
   if (vb.n_cand <= 0) return hipSuccess;
   int nc = vb.n_cand;
@@ -536,9 +562,14 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
     hipLaunchKernelGGL(k_frac, dim3(2 * nc * per), dim3(VT), 0, s, vb, per);
     E();
   }
-  B("k_seam");
-  hipLaunchKernelGGL(k_seam, dim3(nc), dim3(VT), 0, s, vb);
+  B("k_seam_clear");
+  hipLaunchKernelGGL(k_seam_clear, dim3(nc), dim3(VT), 0, s, vb);
   E();
+  if (max_brk > 0) {
+    B("k_seam");
+    hipLaunchKernelGGL(k_seam, dim3(blocks_for(max_brk, VT), nc), dim3(VT), 0, s, vb);
+    E();
+  }
   if (vb.full) {
     if (max_margin > 0) {
       B("k_cont");

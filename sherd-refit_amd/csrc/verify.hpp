@@ -14,7 +14,6 @@ enum {
   VS_BRK_BEST, VS_COUNT
 };
 
-constexpr int SEAM_TILE = 2048;     // moved B breakline points staged in LDS per pass (48 KiB)
 constexpr int MEDIAN_LDS = 12288;   // a median segment up to this many keys is selected from LDS
 constexpr uint32_t FLAG_VOXEL_RANGE = 1u;  // a seam voxel coordinate beyond +-2^20
 
@@ -33,6 +32,7 @@ struct FragDesc {
   const double *margin_n;
   int n_pf, n_s, n_brk, n_margin;
   GridView margin;  // over margin_p, cell at least the largest `near` of the batch
+  GridView brk;     // over brk_p, cell beyond the largest `seam` of the batch
 };
 
 struct PairDesc {
