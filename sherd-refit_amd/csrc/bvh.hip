@@ -444,7 +444,7 @@ __global__ void k_db_final(float *d, size_t n, float h) {
 
 namespace {
 
-constexpr double PG_CELLS_PER_TRI = 0.5;  // cells per triangle of each parity grid
+constexpr double PG_CELLS_PER_TRI = 2.0;  // cells per triangle of each parity grid
 constexpr int PG_MAX_CELLS = 1 << 26;
 
 __device__ __forceinline__ void pg_range(const Tri &t, int axis, const AxisGrid &g, float pad, int *u0, int *u1,
@@ -476,6 +476,28 @@ __global__ void k_pg_fill(const Tri *tris, int m, AxisGrid g, int axis, float pa
   pg_range(tris[j], axis, g, pad, &u0, &u1, &v0, &v1);
   for (int cu = u0; cu <= u1; ++cu)
     for (int cv = v0; cv <= v1; ++cv) g.item[atomicAdd(&cursor[(size_t)cu * g.nv + cv], 1u)] = (uint32_t)j;
+}
+
+// Orders each cell's triangles by their largest coordinate along the ray axis, largest first, so
+// that a ray can stop at the first triangle wholly behind its origin.
+__global__ void k_pg_sort(const Tri *tris, AxisGrid g, int axis) {
+  size_t cell = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (cell >= (size_t)g.nu * g.nv) return;
+  uint32_t b = g.start[cell], e = g.start[cell + 1];
+  auto key = [&](uint32_t j) {
+    const Tri &t = tris[j];
+    return fmaxf(t.v[axis], fmaxf(t.v[3 + axis], t.v[6 + axis]));
+  };
+  for (uint32_t k = b + 1; k < e; ++k) {
+    uint32_t x = g.item[k];
+    float kx = key(x);
+    uint32_t m = k;
+    while (m > b && key(g.item[m - 1]) < kx) {
+      g.item[m] = g.item[m - 1];
+      --m;
+    }
+    g.item[m] = x;
+  }
 }
 
 }  // namespace
@@ -583,6 +605,7 @@ hipError_t scene_parity_grids(DevScene *sc, hipStream_t s) {
       if ((err = hipMemcpyAsync(cursor, g.start, sizeof(uint32_t) * ncell, hipMemcpyDeviceToDevice, s)) != hipSuccess)
         break;
       hipLaunchKernelGGL(k_pg_fill, dim3(blocks_for(m, BT)), dim3(BT), 0, s, sc->tris, m, g, axis, pad, cursor);
+      hipLaunchKernelGGL(k_pg_sort, dim3(blocks_for((long)ncell, BT)), dim3(BT), 0, s, sc->tris, g, axis);
       if ((err = hipGetLastError()) != hipSuccess) break;
       if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
     } while (0);

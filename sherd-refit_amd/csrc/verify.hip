@@ -324,15 +324,35 @@ __device__ K select_kth(const K *keys, int n, int k, uint32_t *hist, K *s_prefix
       if ((key & pmask) == prefix) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
+      // The digit whose bucket holds the k-th key: lane l owns bins 4l..4l+3; a wave prefix sum
+      // over the lanes' totals finds the lane, then that lane walks its four bins.
+      int lane = threadIdx.x;
       int kk = *s_k;
-      int d = 0;
-      for (; d < 255; ++d) {
-        if (kk < (int)hist[d]) break;
-        kk -= (int)hist[d];
+      uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2], h3 = hist[4 * lane + 3];
+      uint32_t tot = h0 + h1 + h2 + h3, incl = tot;
+      for (int o = 1; o < 64; o <<= 1) {
+        uint32_t v = (uint32_t)__shfl_up((int)incl, o);
+        if (lane >= o) incl += v;
       }
-      *s_k = kk;
-      *s_prefix = prefix | ((K)d << shift);
+      uint32_t excl = incl - tot;
+      if ((uint32_t)kk >= excl && (uint32_t)kk < incl) {
+        int r = kk - (int)excl, d = 4 * lane;
+        if (r >= (int)h0) {
+          r -= (int)h0;
+          ++d;
+          if (r >= (int)h1) {
+            r -= (int)h1;
+            ++d;
+            if (r >= (int)h2) {
+              r -= (int)h2;
+              ++d;
+            }
+          }
+        }
+        *s_k = r;
+        *s_prefix = prefix | ((K)d << shift);
+      }
     }
     __syncthreads();
     prefix = *s_prefix;
@@ -343,8 +363,8 @@ __device__ K select_kth(const K *keys, int n, int k, uint32_t *hist, K *s_prefix
 }
 
 // The closest approach is a minimum over every sample, and a search is cheap once a small
-// window is known. Before the exact pass, every sample walks once from the root to a single leaf,
-// always into the nearer child, and measures that one triangle: a distance the sample really
+// window is known. Before the exact pass, one sample per wave walks once from the root to a single
+// leaf, always into the nearer child, and measures that one triangle: a distance the sample really
 // attains, so the smallest of them is at least the true minimum and makes a window the exact pass
 // can prune against. The minimum that comes out is the same; only the work changes.
 __global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb, int per_task) {
@@ -360,16 +380,20 @@ __global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb, int per_task) {
   uint32_t best = 0xffffffffu;
   const SceneView &sv = other.mesh;
   V3 q = V3{0.0f, 0.0f, 0.0f};
-  bool go = false;
+  uint32_t lb = 0xffffffffu;
+  float window = 0.0f;
   if (i < own.n_s) {
     const double *T = side == 0 ? cd.Ti : cd.T;
     const double *p = own.s + 3 * (size_t)i;
     q = moved_f32(T, p[0], p[1], p[2]);
-    // No need for a bound from a sample that cannot beat the best bound so far.
-    float window =
-        __uint_as_float(__hip_atomic_load(&acc.closest[side], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    go = dist_bound(sv.db, q) <= window;
+    window = __uint_as_float(__hip_atomic_load(&acc.closest[side], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    lb = __float_as_uint(dist_bound(sv.db, q));
   }
+  // One descent per wave, from the sample with the smallest distance bound (the likeliest to be
+  // close), and none when even that one cannot beat the best bound so far.
+  uint32_t wlb = wave_min(lb);
+  uint64_t holders = __ballot(lb == wlb);
+  bool go = lb != 0xffffffffu && lanes_below(holders) == 0 && lb == wlb && __uint_as_float(lb) <= window;
   if (go) {
     int nodes = 2 * sv.n - 1;
     int cur = sv.root;
