@@ -155,10 +155,8 @@ __global__ void k_hierarchy(const uint32_t *code, int m, Node *nodes, int *other
       __hip_atomic_store(&childb[parent], cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     parent_of[cur] = parent;
-    __threadfence();
     int prev = __hip_atomic_exchange(&other[parent], left ? l : r, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (prev < 0) return;
-    __threadfence();
     if (left) r = prev; else l = prev;
     int ca = __hip_atomic_load(&nodes[parent].a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     int cb = __hip_atomic_load(&childb[parent], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -478,28 +476,6 @@ __global__ void k_pg_fill(const Tri *tris, int m, AxisGrid g, int axis, float pa
     for (int cv = v0; cv <= v1; ++cv) g.item[atomicAdd(&cursor[(size_t)cu * g.nv + cv], 1u)] = (uint32_t)j;
 }
 
-// Orders each cell's triangles by their largest coordinate along the ray axis, largest first, so
-// that a ray can stop at the first triangle wholly behind its origin.
-__global__ void k_pg_sort(const Tri *tris, AxisGrid g, int axis) {
-  size_t cell = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (cell >= (size_t)g.nu * g.nv) return;
-  uint32_t b = g.start[cell], e = g.start[cell + 1];
-  auto key = [&](uint32_t j) {
-    const Tri &t = tris[j];
-    return fmaxf(t.v[axis], fmaxf(t.v[3 + axis], t.v[6 + axis]));
-  };
-  for (uint32_t k = b + 1; k < e; ++k) {
-    uint32_t x = g.item[k];
-    float kx = key(x);
-    uint32_t m = k;
-    while (m > b && key(g.item[m - 1]) < kx) {
-      g.item[m] = g.item[m - 1];
-      --m;
-    }
-    g.item[m] = x;
-  }
-}
-
 }  // namespace
 
 hipError_t scene_distance_bounds(DevScene *sc, hipStream_t s) {
@@ -591,8 +567,8 @@ hipError_t scene_parity_grids(DevScene *sc, hipStream_t s) {
       if ((err = dalloc(&g.start, ncell + 1)) != hipSuccess) break;
       if ((err = hipMemsetAsync(g.start, 0, sizeof(uint32_t) * (ncell + 1), s)) != hipSuccess) break;
       hipLaunchKernelGGL(k_pg_count, dim3(blocks_for(m, BT)), dim3(BT), 0, s, sc->tris, m, g, axis, pad, g.start);
-      hipLaunchKernelGGL(rsort::k_scan, dim3(1), dim3(1024), 0, s, g.start, (int)(ncell + 1));
-      if ((err = hipGetLastError()) != hipSuccess) break;
+      if ((err = dalloc(&cursor, std::max(ncell, rsort::scan_parts(ncell + 1)))) != hipSuccess) break;
+      if ((err = rsort::scan_exclusive(g.start, ncell + 1, cursor, s)) != hipSuccess) break;
       uint32_t total = 0;
       if ((err = hipMemcpyAsync(&total, g.start + ncell, sizeof total, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
       if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
@@ -601,11 +577,9 @@ hipError_t scene_parity_grids(DevScene *sc, hipStream_t s) {
         break;
       }
       if ((err = dalloc(&g.item, total)) != hipSuccess) break;
-      if ((err = dalloc(&cursor, ncell)) != hipSuccess) break;
       if ((err = hipMemcpyAsync(cursor, g.start, sizeof(uint32_t) * ncell, hipMemcpyDeviceToDevice, s)) != hipSuccess)
         break;
       hipLaunchKernelGGL(k_pg_fill, dim3(blocks_for(m, BT)), dim3(BT), 0, s, sc->tris, m, g, axis, pad, cursor);
-      hipLaunchKernelGGL(k_pg_sort, dim3(blocks_for((long)ncell, BT)), dim3(BT), 0, s, sc->tris, g, axis);
       if ((err = hipGetLastError()) != hipSuccess) break;
       if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
     } while (0);

@@ -166,9 +166,8 @@ __device__ inline void first_hit(const SceneView &sv, V3 o, V3 d, int retest, fl
 // Crossings of the ray from o along +AXIS (RayScene::crossings, bvh.rs:160-178): every hit, from
 // the triangles of the ray's cell in the scene's parity grid for that axis. The cell lists every
 // triangle whose slack-grown box the ray's line meets, so this counts what an exhaustive scan
-// would. A cell's triangles come largest-extent-along-the-axis first, and the walk stops at the
-// first one wholly behind the origin (by more than the slack): it and all after it cannot be hit.
-// *nearest gets the smallest hit parameter (+inf without a hit).
+// would. Triangles wholly behind the origin (by more than the slack) cannot be hit and are skipped
+// before the intersection test. *nearest gets the smallest hit parameter (+inf without a hit).
 template <int AXIS>
 __device__ inline uint32_t crossings_axis(const SceneView &sv, V3 o, int retest, float *nearest) {
   constexpr int U = AXIS == 0 ? 1 : 0, W = AXIS == 2 ? 1 : 2;
@@ -187,7 +186,7 @@ __device__ inline uint32_t crossings_axis(const SceneView &sv, V3 o, int retest,
     float x0 = AXIS == 0 ? v0.x : (AXIS == 1 ? v0.y : v0.z);
     float x1 = AXIS == 0 ? v1.x : (AXIS == 1 ? v1.y : v1.z);
     float x2 = AXIS == 0 ? v2.x : (AXIS == 1 ? v2.y : v2.z);
-    if (fmaxf(x0, fmaxf(x1, x2)) + pad < oo[AXIS]) break;
+    if (fmaxf(x0, fmaxf(x1, x2)) + pad < oo[AXIS]) continue;
     float t;
     if (mt_hit(o, d, v0, v1, v2, retest, &t)) {
       ++hits;

@@ -55,6 +55,70 @@ static __global__ void __launch_bounds__(1024) k_scan(uint32_t *data, int len) {
   }
 }
 
+// ---- device-wide exclusive scan of u32 counters -------------------------------------------------
+// Chunks of SCAN_CHUNK counters: each chunk's total, an exclusive scan of the totals (one
+// workgroup), then each chunk scanned from its offset.
+constexpr int SCAN_PER = 16;
+constexpr int SCAN_CHUNK = THREADS * SCAN_PER;
+
+static __global__ void __launch_bounds__(THREADS) k_scan_reduce(const uint32_t *data, size_t n, uint32_t *part) {
+  __shared__ uint32_t w[WAVES];
+  size_t base = (size_t)blockIdx.x * SCAN_CHUNK;
+  uint32_t s = 0;
+  for (int r = 0; r < SCAN_PER; ++r) {
+    size_t i = base + (size_t)r * THREADS + threadIdx.x;
+    if (i < n) s += data[i];
+  }
+  for (int o = 32; o > 0; o >>= 1) s += (uint32_t)__shfl_xor((int)s, o);
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int k = 0; k < WAVES; ++k) t += w[k];
+    part[blockIdx.x] = t;
+  }
+}
+
+static __global__ void __launch_bounds__(THREADS) k_scan_apply(uint32_t *data, size_t n, const uint32_t *part) {
+  __shared__ uint32_t sums[THREADS];
+  size_t base = (size_t)blockIdx.x * SCAN_CHUNK + (size_t)threadIdx.x * SCAN_PER;
+  uint32_t v[SCAN_PER];
+  uint32_t s = 0;
+#pragma unroll
+  for (int r = 0; r < SCAN_PER; ++r) {
+    size_t i = base + r;
+    v[r] = i < n ? data[i] : 0u;
+    s += v[r];
+  }
+  sums[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < THREADS; off <<= 1) {
+    uint32_t x = threadIdx.x >= (unsigned)off ? sums[threadIdx.x - off] : 0u;
+    __syncthreads();
+    sums[threadIdx.x] += x;
+    __syncthreads();
+  }
+  uint32_t run = part[blockIdx.x] + sums[threadIdx.x] - s;
+#pragma unroll
+  for (int r = 0; r < SCAN_PER; ++r) {
+    size_t i = base + r;
+    if (i < n) data[i] = run;
+    run += v[r];
+  }
+}
+
+inline size_t scan_parts(size_t n) { return (n + SCAN_CHUNK - 1) / SCAN_CHUNK; }
+
+// Exclusive scan of n counters in place; `part` holds scan_parts(n) counters of scratch.
+inline hipError_t scan_exclusive(uint32_t *data, size_t n, uint32_t *part, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  size_t nb = scan_parts(n);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(THREADS), 0, s, data, n, part);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, part, (int)nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(THREADS), 0, s, data, n, part);
+  return hipGetLastError();
+}
+
 template <class K>
 __global__ void __launch_bounds__(THREADS) k_scatter(const K *keys, const uint32_t *vals, K *okeys, uint32_t *ovals,
                                                      int n, int shift, const uint32_t *offs, int tiles) {
@@ -97,7 +161,7 @@ __global__ void __launch_bounds__(THREADS) k_scatter(const K *keys, const uint32
 }
 
 // Sorts n pairs by bits [begin_bit, end_bit) of the key. `tk`/`tv` are scratch of n entries,
-// `hist` holds 256 * ceil(n / 4096) counters. The sorted pairs end in `keys`/`vals`.
+// `hist` holds hist_words(n) counters. The sorted pairs end in `keys`/`vals`.
 template <class K>
 hipError_t sort_pairs(K *keys, uint32_t *vals, K *tk, uint32_t *tv, uint32_t *hist, int n, int begin_bit,
                       int end_bit, hipStream_t s) {
@@ -108,7 +172,8 @@ hipError_t sort_pairs(K *keys, uint32_t *vals, K *tk, uint32_t *tv, uint32_t *hi
   int passes = 0;
   for (int shift = begin_bit; shift < end_bit; shift += 8, ++passes) {
     hipLaunchKernelGGL(k_hist<K>, dim3(tiles), dim3(THREADS), 0, s, ik, n, shift, hist, tiles);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, hist, tiles * BINS);
+    hipError_t e = scan_exclusive(hist, (size_t)tiles * BINS, hist + (size_t)tiles * BINS, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_scatter<K>, dim3(tiles), dim3(THREADS), 0, s, ik, iv, ok, ov, n, shift, hist, tiles);
     K *xk = ik; ik = ok; ok = xk;
     uint32_t *xv = iv; iv = ov; ov = xv;
@@ -123,7 +188,11 @@ hipError_t sort_pairs(K *keys, uint32_t *vals, K *tk, uint32_t *tv, uint32_t *hi
   return e;
 }
 
-inline size_t hist_words(int n) { return (size_t)BINS * (size_t)((n + TILE - 1) / TILE); }
+// Counters for sort_pairs: the per-tile digit histograms and the scan's chunk totals.
+inline size_t hist_words(int n) {
+  size_t h = (size_t)BINS * (size_t)((n + TILE - 1) / TILE);
+  return h + scan_parts(h);
+}
 
 }  // namespace rsort
 }  // namespace sherd
