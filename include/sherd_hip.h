@@ -146,6 +146,50 @@ int sherd_icp_rung(sherd_ctx *ctx, const double *src, size_t n_src, const double
                    const double *tgt_n, size_t n_tgt, const double *inits, size_t k,
                    int estimation, double distance, int max_iteration, sherd_registration *out);
 
+/* ---- batched ICP: rungs and ladders over many cloud pairs (ladder.rs, icp.rs) ------------------ */
+
+typedef struct sherd_cloud sherd_cloud;
+
+/* IcpTarget::new (crates/sherd-core/src/matching/icp.rs:225-240) and the moving clouds of a rung
+ * (ladder.rs:69-80 `Rung::source`): n x 3 f64 points and, optionally, unit normals (NULL for a
+ * source or a point-to-point target), copied to the device with a point LBVH that stands in for
+ * the KD-tree. At most 262144 points. */
+int sherd_cloud_create(sherd_ctx *ctx, const double *pts, const double *normals, size_t n, sherd_cloud **out);
+void sherd_cloud_free(sherd_cloud *cloud);
+
+/* One cloud pair of a batch and its correspondence radius (`Scales::icp_dist(k)` of the pair,
+ * scales.rs:100, ladder.rs:116). */
+typedef struct {
+  const sherd_cloud *source;
+  const sherd_cloud *target;
+  double distance;
+} sherd_icp_pair;
+
+/* A rung (ladder.rs:69 `Rung`): the update and the iteration cap. */
+typedef struct {
+  int estimation; /* SHERD_POINT_TO_POINT or SHERD_POINT_TO_PLANE */
+  int max_iteration;
+} sherd_icp_rung_spec;
+
+/* ladder::climb_all (crates/sherd-core/src/matching/ladder.rs:105-128) for many pairs at once:
+ * candidate i starts from inits[16 i ..] on pair cand_pair[i] and climbs every rung, each rung from
+ * the previous rung's transform. pairs holds n_rungs x n_pairs entries, rung-major (a rung may use
+ * other clouds, as R §5.6's pc_reg and pc_frac rungs do). out receives n_cand x n_rungs
+ * registrations, candidate-major. Each rung is one persistent launch; the rungs are captured once as
+ * a graph and replayed while the batch's shape is unchanged. Results do not depend on the batch's
+ * composition or the workgroup size: each candidate's rung equals sherd_icp_rung on its own. */
+int sherd_icp_batch_ladder(sherd_ctx *ctx, const sherd_icp_pair *pairs, size_t n_pairs,
+                           const sherd_icp_rung_spec *rungs, size_t n_rungs, const uint32_t *cand_pair,
+                           const double *inits, size_t n_cand, sherd_registration *out);
+/* One rung of the above (Executor::icp_rung over many pairs). */
+int sherd_icp_batch_rung(sherd_ctx *ctx, const sherd_icp_pair *pairs, size_t n_pairs, const uint32_t *cand_pair,
+                         const double *inits, size_t n_cand, int estimation, int max_iteration,
+                         sherd_registration *out);
+/* Fixes the ICP workgroup size (64, 128, 256 or 512 threads) or, with 0, leaves it to the
+ * autotuner, which times each size on the first rung of every source-size bucket and keeps the
+ * fastest. No counterpart in the reference (its wgpu kernels have fixed tiles). */
+int sherd_icp_block(sherd_ctx *ctx, int threads);
+
 /* ---- R §6 verification (crates/sherd-core/src/matching/verify.rs) ------------------------------ */
 
 typedef struct sherd_fragment sherd_fragment;

@@ -161,6 +161,7 @@ void sherd_close(sherd_ctx *ctx) {
   (void)hipStreamSynchronize(c->stream);
   for (auto &b : c->bufs)
     if (b.p) (void)hipFree(b.p);
+  if (c->ladder_exec) (void)hipGraphExecDestroy(c->ladder_exec);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
   (void)hipStreamDestroy(c->stream);
@@ -479,74 +480,6 @@ int sherd_coarse_scores(sherd_ctx *ctx, const double *target_p, const double *ta
   if (counts) HIPCK(hipMemcpyAsync(counts, dcnt, sizeof(uint32_t) * k, hipMemcpyDeviceToHost, c->stream), "download");
   HIPCK(hipMemcpyAsync(scores, dsc, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream), "download");
   HIPCK(hipStreamSynchronize(c->stream), "synchronize");
-  return SHERD_OK;
-}
-
-int sherd_icp_rung(sherd_ctx *ctx, const double *src, size_t n_src, const double *tgt_p, const double *tgt_n,
-                   size_t n_tgt, const double *inits, size_t k, int estimation, double distance, int max_iteration,
-                   sherd_registration *out) {
-  Ctx *c = reinterpret_cast<Ctx *>(ctx);
-  LOCK(c);
-  if (k == 0) return SHERD_OK;
-  if (!inits || !out) return fail(SHERD_E_ARG, "null buffer");
-  if (estimation != SHERD_POINT_TO_POINT && estimation != SHERD_POINT_TO_PLANE)
-    return fail(SHERD_E_ARG, "unknown estimation");
-  if (max_iteration < 0) return fail(SHERD_E_ARG, "negative iteration cap");
-  if (n_src == 0 || n_tgt == 0) {
-    // Registration::unmoved (icp.rs:198-209).
-    for (size_t i = 0; i < k; ++i) {
-      std::memcpy(out[i].transform, inits + 16 * i, sizeof(double) * 16);
-      out[i].fitness = out[i].inlier_rmse = 0.0;
-      out[i].correspondences = 0;
-      out[i].iterations = 0;
-      out[i].converged = 0;
-    }
-    return SHERD_OK;
-  }
-  if (!src || !tgt_p) return fail(SHERD_E_ARG, "null buffer");
-  if (n_src > (size_t)INT_MAX / 4 || n_tgt > (size_t)INT_MAX / 2 || k > (size_t)INT_MAX)
-    return fail(SHERD_E_LIMIT, "batch too large");
-  int est = estimation;
-  if (est == SHERD_POINT_TO_PLANE && !tgt_n) est = 2;  // identity updates (icp.rs:460-462)
-  hipError_t e = hipSuccess;
-  double *dsrc = upload3(c, 0, src, n_src, &e);
-  if (!dsrc) return hip_fail(e, "upload source");
-  double *dtp = upload3(c, 1, tgt_p, n_tgt, &e);
-  if (!dtp) return hip_fail(e, "upload target");
-  double *dtn = nullptr;
-  if (tgt_n) {
-    dtn = upload3(c, 2, tgt_n, n_tgt, &e);
-    if (!dtn) return hip_fail(e, "upload normals");
-  }
-  double *dinit = (double *)c->scratch(3, sizeof(double) * 16 * k, &e);
-  if (!dinit) return hip_fail(e, "scratch");
-  HIPCK(hipMemcpyAsync(dinit, inits, sizeof(double) * 16 * k, hipMemcpyHostToDevice, c->stream), "upload inits");
-  double *dmoved = (double *)c->scratch(4, sizeof(double) * 3 * n_src * k, &e);
-  if (!dmoved) return hip_fail(e, "scratch moved points");
-  uint32_t *dcorr = (uint32_t *)c->scratch(5, sizeof(uint32_t) * n_src * k, &e);
-  if (!dcorr) return hip_fail(e, "scratch correspondences");
-  double *dout = (double *)c->scratch(6, sizeof(double) * 21 * k, &e);
-  if (!dout) return hip_fail(e, "scratch");
-  GridGuard gg;
-  double reach = distance > 0.0 ? distance : 1.0;
-  HIPCK(grid_build(dtp, (int)n_tgt, reach, &gg.g, c->stream), "grid build");
-  Timer tm(c, SHERD_M_ICP, k);
-  HIPCK(launch_icp_rung(view_of(gg.g), dtp, dtn, dsrc, (int)n_src, dinit, (int)k, est, distance, max_iteration,
-                        dmoved, dcorr, dout, c->stream),
-        "icp kernel");
-  tm.stop();
-  std::vector<double> h(21 * k);
-  HIPCK(hipMemcpyAsync(h.data(), dout, sizeof(double) * 21 * k, hipMemcpyDeviceToHost, c->stream), "download");
-  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
-  for (size_t i = 0; i < k; ++i) {
-    const double *o = &h[21 * i];
-    std::memcpy(out[i].transform, o, sizeof(double) * 16);
-    out[i].fitness = o[16];
-    out[i].inlier_rmse = o[17];
-    out[i].correspondences = (uint64_t)o[18];
-    out[i].iterations = (uint32_t)o[19];
-    out[i].converged = (uint32_t)o[20];
-  }
   return SHERD_OK;
 }
 

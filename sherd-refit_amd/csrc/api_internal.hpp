@@ -23,8 +23,14 @@ struct Ctx {
     void *p = nullptr;
     size_t n = 0;
   };
-  Buf bufs[24];
+  Buf bufs[40];
   KernelProfiler prof;
+  // Batched ICP: the workgroup size (0 = autotuned), the autotuner's choices by source-size bucket
+  // and estimation, and the captured ladder graph with the launch signature it was captured for.
+  int icp_block = 0;
+  int icp_tuned[2][40] = {};
+  hipGraphExec_t ladder_exec = nullptr;
+  std::string ladder_sig;
   // A grow-only device buffer per slot, reused across calls (the context is serialised).
   void *scratch(int slot, size_t bytes, hipError_t *err);
 };

@@ -1,4 +1,4 @@
-// One rung of R §7's ICP for a batch of candidates: CpuExecutor::icp_rung
+// R §7's ICP for many candidates over many cloud pairs at once: CpuExecutor::icp_rung
 // (crates/sherd-core/src/executor/cpu.rs:70-82) -> icp::register (crates/sherd-core/src/matching/
 // icp.rs:308-356), Open3D's loop:
 //
@@ -6,219 +6,379 @@
 //   repeat up to max_iteration: U = update(C); T = U.T; moved = U.moved; (fit', rmse', C) = corr(moved)
 //                               stop when |fit' - fit| < 1e-6 and |rmse' - rmse| < 1e-6
 //
-// One 256-thread workgroup per candidate keeps the whole rung on the device. Each correspondence
-// pass applies the last update to the stored moved points, finds the nearest target strictly inside
-// the radius (f64, the grid standing in for the KD-tree), and accumulates what the next update
-// needs in the same pass: the 21 + 6 normal-equation sums for point-to-plane (icp.rs:599-640), the
-// two point sums for point-to-point, whose covariance (icp.rs:539-592) takes a second pass once the
-// means are known. Sums run per thread in a fixed stride and reduce in a fixed tree, so a rung is
-// deterministic run to run; it differs from the CPU's sequential source-order sums in rounding only.
+// A rung is one persistent launch: workgroups take candidates from a queue, longest source cloud
+// first, and keep a candidate's whole rung on the device. Each correspondence pass applies the last
+// update to the stored moved points, finds the nearest target point strictly inside the radius
+// (f64, through the target's point LBVH: kiddo's nearest_within_squared, kdtree.rs:202-218, ties to
+// the lowest index), and accumulates what the next update needs in the same pass: the 21 + 6
+// normal-equation sums for point-to-plane (icp.rs:599-640), the two point sums for point-to-point,
+// whose covariance (icp.rs:539-592) takes a second pass once the means are known.
+//
+// The sums have one fixed order that depends only on the cloud size: the source is cut into spans
+// of SPAN points, lane l of the wave that owns a span adds its points l, l + 64, ... in turn, the
+// 64 lanes meet in a fixed shuffle tree, and the span totals are added by a fixed tree in span
+// order. So a result is the same for any workgroup size and any assignment of candidates to
+// workgroups; it differs from the CPU's sequential source-order sums in rounding only.
 #include <hip/hip_runtime.h>
 
-#include "grid.hpp"
+#include "bvh_query.hpp"
 #include "kernels.hpp"
 #include "linalg.hpp"
 
 namespace sherd {
 namespace {
 
-constexpr int IT = 256;
-constexpr int NV = 29;  // count, err^2, then 27 update sums
+constexpr int NV = ICP_NV;
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 
-// Sums NV values over the workgroup in a fixed order: wave trees, then the four waves in order.
-__device__ __forceinline__ void block_sum(double *v, double *lds, double *out) {
-  int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double x = v[k];
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-    if (lane == 0) lds[wave * NV + k] = x;
+// The nearest point of a cloud to q with d^2 < r2, ties to the lowest index. The cloud's tree is
+// an LBVH over its points (each leaf a point, `face` its index); boxes are f32 and grown by the
+// slack, which exceeds the rounding of the f32 boxes and of q, so no box hides a point nearer than
+// the best so far. Distances are kiddo's f64 ((dx^2 + dy^2) + dz^2) on the f64 points.
+__device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, double q0, double q1, double q2,
+                                     double r2, uint32_t *out_j, double *out_d2) {
+  int nodes = 2 * sv.n - 1;
+  int cur = sv.root;
+  if (cur < 0 || cur >= nodes || !(r2 > 0.0)) return false;
+  V3 qf = V3{(float)q0, (float)q1, (float)q2};
+  float slack = slack_of(sv.absmax, qf);
+  double best = r2;
+  uint32_t bj = NONE;
+  float lim = (float)sqrt(best) + slack;
+  float lim2 = lim * lim;
+  if (box_d2(load_node(sv.nodes, cur), qf, slack) > lim2) return false;
+  int st_n[DSTACK];
+  float st_d[DSTACK];
+  int sp = 0;
+  bool overflow = false;
+  for (int iter = 0; iter <= nodes; ++iter) {
+    Node nd = load_node(sv.nodes, cur);
+    int next = -1;
+    if (nd.a < 0) {
+      V3 a, b, c;
+      uint32_t j;
+      load_tri(sv.tris, ~nd.a, &a, &b, &c, &j);
+      double d2 = d2_f64(q0, q1, q2, pts[3 * (size_t)j], pts[3 * (size_t)j + 1], pts[3 * (size_t)j + 2]);
+      if (d2 < best || (d2 == best && j < bj)) {
+        best = d2;
+        bj = j;
+        lim = (float)sqrt(best) + slack;
+        lim2 = lim * lim;
+      }
+    } else {
+      int l = nd.a;
+      Node ln = load_node(sv.nodes, l);
+      int r = ln.esc;
+      float dl = box_d2(ln, qf, slack);
+      float dr = __builtin_inff();
+      if (r >= 0 && r < nodes) dr = box_d2(load_node(sv.nodes, r), qf, slack);
+      bool hl = dl <= lim2, hr = dr <= lim2;
+      if (hl && hr) {
+        int near = dl <= dr ? l : r, far = dl <= dr ? r : l;
+        float dfar = dl <= dr ? dr : dl;
+        if (sp < DSTACK) {
+          st_n[sp] = far;
+          st_d[sp] = dfar;
+          ++sp;
+        } else {
+          overflow = true;
+        }
+        next = near;
+      } else if (hl) {
+        next = l;
+      } else if (hr) {
+        next = r;
+      }
+    }
+    while (next < 0 && sp > 0) {
+      --sp;
+      if (st_d[sp] <= lim2) next = st_n[sp];
+    }
+    if (next < 0) break;
+    cur = next;
   }
-  __syncthreads();
-  if (t < NV) out[t] = ((lds[t] + lds[NV + t]) + lds[2 * NV + t]) + lds[3 * NV + t];
-  __syncthreads();
+  if (overflow) {
+    // The deferred nodes that did not fit: an exhaustive pre-order walk against the same bound.
+    int steps = 0;
+    for (int i = sv.root; i >= 0 && i < nodes && steps++ < nodes;) {
+      Node nd = load_node(sv.nodes, i);
+      if (nd.a >= 0) {
+        i = box_d2(nd, qf, slack) <= lim2 ? nd.a : nd.esc;
+      } else {
+        V3 a, b, c;
+        uint32_t j;
+        load_tri(sv.tris, ~nd.a, &a, &b, &c, &j);
+        double d2 = d2_f64(q0, q1, q2, pts[3 * (size_t)j], pts[3 * (size_t)j + 1], pts[3 * (size_t)j + 2]);
+        if (d2 < best || (d2 == best && j < bj)) {
+          best = d2;
+          bj = j;
+          lim = (float)sqrt(best) + slack;
+          lim2 = lim * lim;
+        }
+        i = nd.esc;
+      }
+    }
+  }
+  if (bj == NONE || !(best < r2)) return false;
+  *out_j = bj;
+  *out_d2 = best;
+  return true;
 }
 
-__global__ void __launch_bounds__(IT) k_icp(GridView g, const double *tp, const double *tn, const double *src,
-                                            int ns, const double *inits, int est, double distance, int max_iter,
-                                            double *moved_all, uint32_t *corr_all, double *out) {
-  const int c = blockIdx.x, t = threadIdx.x;
-  double *moved = moved_all + (size_t)c * ns * 3;
-  uint32_t *corr = corr_all + (size_t)c * ns;
-  __shared__ double sT[16], sU[16];
-  __shared__ double lds[4 * NV];
-  __shared__ double red[NV];
-  __shared__ double s_fit, s_rmse;
-  __shared__ int s_stop, s_iters, s_conv;
-  const bool radius_ok = distance > 0.0;
-  const double r2 = distance * distance;
-  const bool plane = est == 1 && tn != nullptr;
-  const bool p2p = est == 0;  // otherwise every update is the identity (no normals)
-  if (t < 16) {
-    sT[t] = inits[16 * (size_t)c + t];
-    sU[t] = sT[t];
-  }
-  if (t == 0) {
-    s_fit = 0.0;
-    s_rmse = 0.0;
-    s_stop = 0;
-    s_iters = 0;
-    s_conv = 0;
-  }
-  __syncthreads();
-  for (int it = 0;; ++it) {
-    double acc[NV];
+// Sums v[0..n) over the wave's lanes in a fixed shuffle tree; lane 0 holds the totals.
+template <int N>
+__device__ __forceinline__ void wave_sum(double *v) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    double M[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) M[k] = sU[k];
-    for (int i = t; i < ns; i += IT) {
-      const double *s = it == 0 ? src + 3 * (size_t)i : moved + 3 * (size_t)i;
-      double s0 = s[0], s1 = s[1], s2 = s[2];
-      double p0 = M[0] * s0 + M[1] * s1 + M[2] * s2 + M[3];
-      double p1 = M[4] * s0 + M[5] * s1 + M[6] * s2 + M[7];
-      double p2 = M[8] * s0 + M[9] * s1 + M[10] * s2 + M[11];
-      moved[3 * (size_t)i] = p0;
-      moved[3 * (size_t)i + 1] = p1;
-      moved[3 * (size_t)i + 2] = p2;
-      uint32_t j = NONE;
-      double d2;
-      if (radius_ok && grid_nearest(g, p0, p1, p2, &j, &d2) && d2 < r2) {
-        acc[0] += 1.0;
-        acc[1] += d2;
-        const double *q = tp + 3 * (size_t)j;
-        if (plane) {
-          const double *n = tn + 3 * (size_t)j;
-          double pc0 = p0 - 0.0, pc1 = p1 - 0.0, pc2 = p2 - 0.0;
-          double row[6] = {pc1 * n[2] - pc2 * n[1], pc2 * n[0] - pc0 * n[2], pc0 * n[1] - pc1 * n[0],
-                           n[0], n[1], n[2]};
-          double r = (p0 - q[0]) * n[0] + (p1 - q[1]) * n[1] + (p2 - q[2]) * n[2];
-          int w = 2;
-#pragma unroll
-          for (int k = 0; k < 6; ++k) {
-#pragma unroll
-            for (int l = 0; l <= k; ++l) acc[w++] += row[k] * row[l];
-          }
-#pragma unroll
-          for (int k = 0; k < 6; ++k) acc[23 + k] += row[k] * r;
-        } else if (p2p) {
-          acc[2] += p0;
-          acc[3] += p1;
-          acc[4] += p2;
-          acc[5] += q[0];
-          acc[6] += q[1];
-          acc[7] += q[2];
-        }
-      } else {
-        j = NONE;
-      }
-      corr[i] = j;
-    }
-    block_sum(acc, lds, red);
-    if (t == 0) {
-      double cnt = red[0];
-      double fit = cnt > 0.0 ? cnt / (double)ns : 0.0;
-      double rmse = cnt > 0.0 ? sqrt(red[1] / cnt) : 0.0;
-      int conv = 0;
-      if (it > 0) conv = (fabs(fit - s_fit) < 1e-6 && fabs(rmse - s_rmse) < 1e-6) ? 1 : 0;
-      s_fit = fit;
-      s_rmse = rmse;
-      s_conv = conv;
-      s_iters = it;
-      s_stop = (conv || it >= max_iter) ? 1 : 0;
-      if (!s_stop && plane) {
-        // icp.rs:454-483: solve JTJ x = -JTr, Euler update, identity on an unsolvable system.
-        double U[16];
-        eye4(U);
-        if (cnt > 0.0) {
-          double a[36], rhs[6], x[6];
-          int w = 2;
-          for (int k = 0; k < 6; ++k)
-            for (int l = 0; l <= k; ++l) {
-              a[6 * k + l] = red[w];
-              a[6 * l + k] = red[w];
-              ++w;
-            }
-          for (int k = 0; k < 6; ++k) rhs[k] = -red[23 + k];
-          if (solve_ldlt(a, rhs, x)) {
-            double rot[9];
-            euler_zyx(x[0], x[1], x[2], rot);
-            for (int r = 0; r < 3; ++r) {
-              for (int q = 0; q < 3; ++q) U[4 * r + q] = rot[3 * r + q];
-              U[4 * r + 3] = x[3 + r];
-            }
-          }
-        }
-        for (int k = 0; k < 16; ++k) sU[k] = U[k];
-      } else if (!s_stop && !p2p) {
-        eye4(sU);
-      }
-    }
-    __syncthreads();
-    if (s_stop) break;
-    if (p2p) {
-      // Point-to-point: the covariance of the pairs about their means (icp.rs:557-575).
-      double cnt = red[0];
-      double one_over_n = cnt > 0.0 ? 1.0 / cnt : 0.0;
-      double mp0 = red[2] * one_over_n, mp1 = red[3] * one_over_n, mp2 = red[4] * one_over_n;
-      double mq0 = red[5] * one_over_n, mq1 = red[6] * one_over_n, mq2 = red[7] * one_over_n;
-      double sg[NV];
-#pragma unroll
-      for (int k = 0; k < NV; ++k) sg[k] = 0.0;
-      if (cnt > 0.0) {
-        for (int i = t; i < ns; i += IT) {
-          uint32_t j = corr[i];
-          if (j == NONE) continue;
-          const double *p = moved + 3 * (size_t)i;
-          const double *q = tp + 3 * (size_t)j;
-          double a0 = p[0] - mp0, a1 = p[1] - mp1, a2 = p[2] - mp2;
-          double b0 = (q[0] - mq0) * one_over_n, b1 = (q[1] - mq1) * one_over_n, b2 = (q[2] - mq2) * one_over_n;
-          sg[0] += b0 * a0; sg[1] += b0 * a1; sg[2] += b0 * a2;
-          sg[3] += b1 * a0; sg[4] += b1 * a1; sg[5] += b1 * a2;
-          sg[6] += b2 * a0; sg[7] += b2 * a1; sg[8] += b2 * a2;
-        }
-      }
-      __syncthreads();
-      block_sum(sg, lds, red);
-      if (t == 0) {
-        double U[16];
-        eye4(U);
-        if (cnt > 0.0) {
-          double mp[3] = {mp0, mp1, mp2}, mq[3] = {mq0, mq1, mq2};
-          umeyama_pose(red, mp, mq, U);
-        }
-        for (int k = 0; k < 16; ++k) sU[k] = U[k];
-      }
-      __syncthreads();
-    }
-    if (t == 0) {
-      double nt[16];
-      mul_nn(sU, sT, nt, 4);
-      for (int k = 0; k < 16; ++k) sT[k] = nt[k];
-    }
-    __syncthreads();
-  }
-  if (t == 0) {
-    double *o = out + 21 * (size_t)c;
-    for (int k = 0; k < 16; ++k) o[k] = sT[k];
-    o[16] = s_fit;
-    o[17] = s_rmse;
-    o[18] = red[0];
-    o[19] = (double)s_iters;
-    o[20] = (double)s_conv;
+  for (int k = 0; k < N; ++k) {
+    double x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    v[k] = x;
   }
 }
+
+// Adds the first nspan span partials (N values each, stride NV) in the fixed order: lane l adds
+// spans l, l + 64, ... in turn, then the lanes' tree. Called by the workgroup's first wave; lane 0
+// gets the totals in out.
+template <int N>
+__device__ __forceinline__ void spans_total(const double *part, int nspan, double *out) {
+  int lane = threadIdx.x & 63;
+  double v[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = 0.0;
+  for (int sp = lane; sp < nspan; sp += 64)
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] += part[(size_t)sp * NV + k];
+  wave_sum<N>(v);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) out[k] = v[k];
+}
+
+template <int B>
+__global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
+  constexpr int W = B / 64;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  double *part = v.part + (size_t)blockIdx.x * v.part_spans * NV;
+  __shared__ double sT[16], sU[16];
+  __shared__ double red[NV];
+  __shared__ double s_fit, s_rmse;
+  __shared__ int s_stop, s_iters, s_conv, s_c;
+  for (;;) {
+    if (t == 0) {
+      uint32_t k = atomicAdd(v.next, 1u);
+      s_c = k < (uint32_t)v.n_cand ? (int)v.order[k] : -1;
+    }
+    __syncthreads();
+    const int c = s_c;
+    if (c < 0) break;
+    const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
+    const int ns = pd.ns;
+    const int nspan = (ns + ICP_SPAN - 1) / ICP_SPAN;
+    double *moved = v.moved + 3 * v.off[c];
+    uint32_t *corr = v.corr + v.off[c];
+    const bool radius_ok = pd.r > 0.0;
+    const double r2 = pd.r * pd.r;
+    const bool plane = v.est == 1 && pd.tn != nullptr;
+    const bool p2p = v.est == 0;  // otherwise every update is the identity (no normals)
+    if (t < 16) {
+      sT[t] = v.T_in[16 * (size_t)c + t];
+      sU[t] = sT[t];
+    }
+    if (t == 0) {
+      s_fit = 0.0;
+      s_rmse = 0.0;
+      s_stop = 0;
+      s_iters = 0;
+      s_conv = 0;
+    }
+    __syncthreads();
+    for (int it = 0;; ++it) {
+      double M[12];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) M[k] = sU[k];
+      for (int sp = wave; sp < nspan; sp += W) {
+        double acc[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+        for (int i = sp * ICP_SPAN + lane; i < ns && i < (sp + 1) * ICP_SPAN; i += 64) {
+          const double *s = it == 0 ? pd.src + 3 * (size_t)i : moved + 3 * (size_t)i;
+          double s0 = s[0], s1 = s[1], s2 = s[2];
+          double p0 = M[0] * s0 + M[1] * s1 + M[2] * s2 + M[3];
+          double p1 = M[4] * s0 + M[5] * s1 + M[6] * s2 + M[7];
+          double p2 = M[8] * s0 + M[9] * s1 + M[10] * s2 + M[11];
+          moved[3 * (size_t)i] = p0;
+          moved[3 * (size_t)i + 1] = p1;
+          moved[3 * (size_t)i + 2] = p2;
+          uint32_t j = NONE;
+          double d2;
+          if (radius_ok && cloud_nearest(pd.tree, pd.tp, p0, p1, p2, r2, &j, &d2)) {
+            acc[0] += 1.0;
+            acc[1] += d2;
+            const double *q = pd.tp + 3 * (size_t)j;
+            if (plane) {
+              const double *n = pd.tn + 3 * (size_t)j;
+              double row[6] = {p1 * n[2] - p2 * n[1], p2 * n[0] - p0 * n[2], p0 * n[1] - p1 * n[0], n[0], n[1], n[2]};
+              double r = (p0 - q[0]) * n[0] + (p1 - q[1]) * n[1] + (p2 - q[2]) * n[2];
+              int w = 2;
+#pragma unroll
+              for (int k = 0; k < 6; ++k) {
+#pragma unroll
+                for (int l = 0; l <= k; ++l) acc[w++] += row[k] * row[l];
+              }
+#pragma unroll
+              for (int k = 0; k < 6; ++k) acc[23 + k] += row[k] * r;
+            } else if (p2p) {
+              acc[2] += p0;
+              acc[3] += p1;
+              acc[4] += p2;
+              acc[5] += q[0];
+              acc[6] += q[1];
+              acc[7] += q[2];
+            }
+          } else {
+            j = NONE;
+          }
+          corr[i] = j;
+        }
+        wave_sum<NV>(acc);
+        if (lane == 0)
+#pragma unroll
+          for (int k = 0; k < NV; ++k) part[(size_t)sp * NV + k] = acc[k];
+      }
+      __syncthreads();
+      if (wave == 0) spans_total<NV>(part, nspan, red);
+      __syncthreads();
+      if (t == 0) {
+        double cnt = red[0];
+        double fit = cnt > 0.0 ? cnt / (double)ns : 0.0;
+        double rmse = cnt > 0.0 ? sqrt(red[1] / cnt) : 0.0;
+        int conv = 0;
+        if (it > 0) conv = (fabs(fit - s_fit) < 1e-6 && fabs(rmse - s_rmse) < 1e-6) ? 1 : 0;
+        s_fit = fit;
+        s_rmse = rmse;
+        s_conv = conv;
+        s_iters = it;
+        s_stop = (conv || it >= v.max_iter) ? 1 : 0;
+        if (!s_stop && plane) {
+          // icp.rs:454-483: solve JTJ x = -JTr, Euler update, identity on an unsolvable system.
+          double U[16];
+          eye4(U);
+          if (cnt > 0.0) {
+            double a[36], rhs[6], x[6];
+            int w = 2;
+            for (int k = 0; k < 6; ++k)
+              for (int l = 0; l <= k; ++l) {
+                a[6 * k + l] = red[w];
+                a[6 * l + k] = red[w];
+                ++w;
+              }
+            for (int k = 0; k < 6; ++k) rhs[k] = -red[23 + k];
+            if (solve_ldlt(a, rhs, x)) {
+              double rot[9];
+              euler_zyx(x[0], x[1], x[2], rot);
+              for (int r = 0; r < 3; ++r) {
+                for (int q = 0; q < 3; ++q) U[4 * r + q] = rot[3 * r + q];
+                U[4 * r + 3] = x[3 + r];
+              }
+            }
+          }
+          for (int k = 0; k < 16; ++k) sU[k] = U[k];
+        } else if (!s_stop && !p2p) {
+          eye4(sU);
+        }
+      }
+      __syncthreads();
+      if (s_stop) break;
+      if (p2p) {
+        // Point-to-point: the covariance of the pairs about their means (icp.rs:557-575).
+        double cnt = red[0];
+        double one_over_n = cnt > 0.0 ? 1.0 / cnt : 0.0;
+        double mp0 = red[2] * one_over_n, mp1 = red[3] * one_over_n, mp2 = red[4] * one_over_n;
+        double mq0 = red[5] * one_over_n, mq1 = red[6] * one_over_n, mq2 = red[7] * one_over_n;
+        if (cnt > 0.0) {
+          for (int sp = wave; sp < nspan; sp += W) {
+            double sg[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) sg[k] = 0.0;
+            for (int i = sp * ICP_SPAN + lane; i < ns && i < (sp + 1) * ICP_SPAN; i += 64) {
+              uint32_t j = corr[i];
+              if (j == NONE) continue;
+              const double *p = moved + 3 * (size_t)i;
+              const double *q = pd.tp + 3 * (size_t)j;
+              double a0 = p[0] - mp0, a1 = p[1] - mp1, a2 = p[2] - mp2;
+              double b0 = (q[0] - mq0) * one_over_n, b1 = (q[1] - mq1) * one_over_n, b2 = (q[2] - mq2) * one_over_n;
+              sg[0] += b0 * a0; sg[1] += b0 * a1; sg[2] += b0 * a2;
+              sg[3] += b1 * a0; sg[4] += b1 * a1; sg[5] += b1 * a2;
+              sg[6] += b2 * a0; sg[7] += b2 * a1; sg[8] += b2 * a2;
+            }
+            wave_sum<9>(sg);
+            if (lane == 0)
+#pragma unroll
+              for (int k = 0; k < 9; ++k) part[(size_t)sp * NV + k] = sg[k];
+          }
+          __syncthreads();
+          if (wave == 0) spans_total<9>(part, nspan, red + 2);
+          __syncthreads();
+        }
+        if (t == 0) {
+          double U[16];
+          eye4(U);
+          if (cnt > 0.0) {
+            double mp[3] = {mp0, mp1, mp2}, mq[3] = {mq0, mq1, mq2};
+            umeyama_pose(red + 2, mp, mq, U);
+          }
+          for (int k = 0; k < 16; ++k) sU[k] = U[k];
+        }
+        __syncthreads();
+      }
+      if (t == 0) {
+        double nt[16];
+        mul_nn(sU, sT, nt, 4);
+        for (int k = 0; k < 16; ++k) sT[k] = nt[k];
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+      double *o = v.out + 21 * (size_t)c;
+      for (int k = 0; k < 16; ++k) o[k] = sT[k];
+      o[16] = s_fit;
+      o[17] = s_rmse;
+      o[18] = red[0];
+      o[19] = (double)s_iters;
+      o[20] = (double)s_conv;
+      if (v.T_out)
+        for (int k = 0; k < 16; ++k) v.T_out[16 * (size_t)c + k] = sT[k];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_icp_reset(uint32_t *next) { *next = 0; }
 
 }  // namespace
 
-hipError_t launch_icp_rung(const GridView &g, const double *tp, const double *tn, const double *src, int ns,
-                           const double *inits, int k, int estimation, double distance, int max_iter,
-                           double *moved, uint32_t *corr, double *out, hipStream_t s) {
-  if (k <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_icp, dim3(k), dim3(IT), 0, s, g, tp, tn, src, ns, inits, estimation, distance, max_iter,
-                     moved, corr, out);
+int icp_block_sizes(int *out) {
+  static const int sizes[] = {64, 128, 256, 512};
+  for (int k = 0; k < 4; ++k) out[k] = sizes[k];
+  return 4;
+}
+
+hipError_t launch_icp_batch(const IcpBatchView &v, int block, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_icp_reset, dim3(1), dim3(1), 0, s, v.next);
+  switch (block) {
+    case 64:
+      hipLaunchKernelGGL(k_icp_batch<64>, dim3(grid), dim3(64), 0, s, v);
+      break;
+    case 128:
+      hipLaunchKernelGGL(k_icp_batch<128>, dim3(grid), dim3(128), 0, s, v);
+      break;
+    case 512:
+      hipLaunchKernelGGL(k_icp_batch<512>, dim3(grid), dim3(512), 0, s, v);
+      break;
+    default:
+      hipLaunchKernelGGL(k_icp_batch<256>, dim3(grid), dim3(256), 0, s, v);
+      break;
+  }
   return hipGetLastError();
 }
 

@@ -92,9 +92,44 @@ hipError_t launch_coarse(const GridView &g, const double *tn, const double *pp, 
 // ---- ICP rung (icp.hip) --------------------------------------------------------------------------
 // One workgroup per candidate runs every iteration of the rung (icp::register, icp.rs:308-356).
 // moved: k * ns * 3 f64 scratch; corr: k * ns u32 scratch; out: k * 21 doubles.
-hipError_t launch_icp_rung(const GridView &g, const double *tp, const double *tn, const double *src, int ns,
-                           const double *inits, int k, int estimation, double distance, int max_iter,
-                           double *moved, uint32_t *corr, double *out, hipStream_t s);
+// ---- batched ICP (icp.hip) ----------------------------------------------------------------------
+constexpr int ICP_NV = 29;        // per-pass sums: count, err^2, then 27 update sums
+constexpr int ICP_SPAN = 512;     // source points per span of the fixed summation order
+constexpr int ICP_MAX_SPANS = 512;  // so at most 262144 source points per cloud
+
+// One cloud pair of a batch: the moving source (points only) and the target with its point LBVH
+// and (for point-to-plane) normals, and the pair's correspondence radius.
+struct IcpPairDesc {
+  const double *src;
+  int ns;
+  const double *tp;
+  const double *tn;
+  SceneView tree;
+  double r;
+};
+
+struct IcpBatchView {
+  const IcpPairDesc *pairs;
+  const uint32_t *cand_pair;
+  const uint32_t *order;  // candidates, largest source first
+  const size_t *off;      // each candidate's first point in moved / corr
+  const double *T_in;     // 16 per candidate
+  double *T_out;          // 16 per candidate (may be null)
+  double *out;            // 21 per candidate: transform, fitness, rmse, correspondences, iterations, converged
+  double *moved;
+  uint32_t *corr;
+  double *part;           // per workgroup: part_spans x ICP_NV span sums
+  int part_spans;
+  uint32_t *next;         // queue position
+  int n_cand;
+  int est;                // 0 point-to-point, 1 point-to-plane, 2 identity updates
+  int max_iter;
+};
+
+// Workgroup sizes the batched kernel is built for (the autotuner's choices); returns their count.
+int icp_block_sizes(int *out);
+// Resets the queue and launches one persistent rung over the batch.
+hipError_t launch_icp_batch(const IcpBatchView &v, int block, int grid, hipStream_t s);
 
 // ---- verification (verify.hip) -------------------------------------------------------------------
 // max_pf / max_s / max_margin: the largest per-fragment sample counts in the batch (grid sizes).

@@ -44,6 +44,14 @@ class Registration(C.Structure):
                 ("correspondences", C.c_uint64), ("iterations", C.c_uint32), ("converged", C.c_uint32)]
 
 
+class IcpPair(C.Structure):
+    _fields_ = [("source", C.c_void_p), ("target", C.c_void_p), ("distance", C.c_double)]
+
+
+class IcpRungSpec(C.Structure):
+    _fields_ = [("estimation", C.c_int), ("max_iteration", C.c_int)]
+
+
 class VerifyPair(C.Structure):
     _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("scales", C.c_double * 12)]
 
@@ -95,6 +103,12 @@ def _bind(L):
         "sherd_fragment_create": (ci, [vp, vp, vp, ci, d, P(d), sz, P(d), sz, P(d), P(d), sz, P(d), P(d), sz,
                                        P(vp)]),
         "sherd_fragment_free": (None, [vp]),
+        "sherd_cloud_create": (ci, [vp, P(d), P(d), sz, P(vp)]),
+        "sherd_cloud_free": (None, [vp]),
+        "sherd_icp_block": (ci, [vp, ci]),
+        "sherd_icp_batch_rung": (ci, [vp, P(IcpPair), sz, P(u32), P(d), sz, ci, ci, P(Registration)]),
+        "sherd_icp_batch_ladder": (ci, [vp, P(IcpPair), sz, P(IcpRungSpec), sz, P(u32), P(d), sz,
+                                        P(Registration)]),
         "sherd_profile": (ci, [vp, ci]),
         "sherd_profile_read": (ci, [vp, C.c_char_p, sz]),
         "sherd_copy_probe": (ci, [vp, sz]),
@@ -162,6 +176,42 @@ class Scene:
         hi = np.zeros(3, np.float32)
         _check(lib().sherd_scene_info(self.ptr, C.byref(n), _p(lo, C.c_float), _p(hi, C.c_float)))
         return n.value, lo, hi
+
+
+class Cloud:
+    """A point set for ICP on the device (IcpTarget, crates/sherd-core/src/matching/icp.rs:218-300)."""
+
+    def __init__(self, ex, pts, normals=None):
+        self.ex = ex
+        p = f64(pts)
+        n = None if normals is None else f64(normals)
+        out = C.c_void_p()
+        _check(lib().sherd_cloud_create(ex.ptr, _p(p, C.c_double), _p(n, C.c_double), len(p), C.byref(out)))
+        self.ptr = out
+        self.n = len(p)
+
+    def close(self):
+        if self.ptr:
+            lib().sherd_cloud_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _registrations(out, k):
+    res = np.empty((k, 21))
+    for i in range(k):
+        r = out[i]
+        res[i, :16] = np.ctypeslib.as_array(r.transform)
+        res[i, 16:] = [r.fitness, r.inlier_rmse, r.correspondences, r.iterations, r.converged]
+    return res
+
+
+_EST = {"point_to_point": 0, "point_to_plane": 1}
 
 
 class Fragment:
@@ -308,12 +358,46 @@ class HipExecutor:
         _check(lib().sherd_icp_rung(self.ptr, _p(src, C.c_double), len(src), _p(tp, C.c_double),
                                     _p(tn, C.c_double), len(tp), _p(inits, C.c_double), k, int(est), float(distance),
                                     int(max_iteration), out))
-        res = np.empty((k, 21))
-        for i in range(k):
-            r = out[i]
-            res[i, :16] = np.ctypeslib.as_array(r.transform)
-            res[i, 16:] = [r.fitness, r.inlier_rmse, r.correspondences, r.iterations, r.converged]
-        return res
+        return _registrations(out, k)
+
+    def cloud(self, pts, normals=None):
+        return Cloud(self, pts, normals)
+
+    def icp_block(self, threads):
+        """Fix the ICP workgroup size (64/128/256/512) or leave it to the autotuner (0)."""
+        _check(lib().sherd_icp_block(self.ptr, int(threads)))
+
+    def icp_batch_ladder(self, pairs, rungs, cand_pair, inits):
+        """ladder::climb_all over many pairs. pairs: per rung a list of (source, target, distance)
+        (all rungs the same length); rungs: [(estimation, max_iteration)]. Returns an
+        (n_cand, n_rungs, 21) array in icp_rung's layout."""
+        nr, npairs = len(rungs), len(pairs[0])
+        arr = (IcpPair * (nr * npairs))()
+        for r in range(nr):
+            assert len(pairs[r]) == npairs
+            for p, (src, tgt, dist) in enumerate(pairs[r]):
+                arr[r * npairs + p] = IcpPair(src.ptr, tgt.ptr, float(dist))
+        specs = (IcpRungSpec * nr)(*[IcpRungSpec(int(_EST.get(e, e)), int(it)) for e, it in rungs])
+        cp = np.ascontiguousarray(cand_pair, dtype=np.uint32)
+        inits = np.ascontiguousarray(inits, dtype=np.float64).reshape(-1, 16)
+        k = len(inits)
+        out = (Registration * max(k * nr, 1))()
+        _check(lib().sherd_icp_batch_ladder(self.ptr, arr, npairs, specs, nr, _p(cp, C.c_uint32),
+                                            _p(inits, C.c_double), k, out))
+        return _registrations(out, k * nr).reshape(k, nr, 21)
+
+    def icp_batch_rung(self, pairs, cand_pair, inits, estimation, max_iteration):
+        """One rung over many pairs: pairs [(source, target, distance)] -> (n_cand, 21)."""
+        arr = (IcpPair * max(len(pairs), 1))()
+        for p, (src, tgt, dist) in enumerate(pairs):
+            arr[p] = IcpPair(src.ptr, tgt.ptr, float(dist))
+        cp = np.ascontiguousarray(cand_pair, dtype=np.uint32)
+        inits = np.ascontiguousarray(inits, dtype=np.float64).reshape(-1, 16)
+        k = len(inits)
+        out = (Registration * max(k, 1))()
+        _check(lib().sherd_icp_batch_rung(self.ptr, arr, len(pairs), _p(cp, C.c_uint32), _p(inits, C.c_double), k,
+                                          int(_EST.get(estimation, estimation)), int(max_iteration), out))
+        return _registrations(out, k)
 
     def fragment(self, fracture, mesh, watertight, frac_area, pf, s, brk_p, brk_ns, margin_p, margin_n):
         return Fragment(self, fracture, mesh, watertight, frac_area, pf, s, brk_p, brk_ns, margin_p, margin_n)
