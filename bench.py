@@ -105,6 +105,73 @@ def cpu_baseline(col, pairs, scales, cand_pair, cand_T, limit_s=20.0):
                        f"of the same workload, {cores} OpenMP threads, scene builds excluded"), out
 
 
+def icp_ladder(ex, synth, args):
+    """The batched ICP ladder: pairs x poses candidates climb four point-to-point rungs (1.0, 0.3,
+    0.1, 0.03 t, 20 iterations each) in one call, against the plain schedule (one per-call
+    icp_rung per pair per rung, as ladder::climb_all issues them) and the oracle on a sample."""
+    w = synth.icp_batch_workload(n_pairs=args.icp_pairs, poses=args.icp_poses)
+    P = w["pairs"]
+    clouds = [(ex.cloud(p["source"]), ex.cloud(p["target"], p["target_n"])) for p in P]
+    rungs = [("point_to_point", it) for _, it in w["rungs"]]
+    per_rung = [[(s, t, r) for s, t in clouds] for r, _ in w["rungs"]]
+    cp = np.repeat(np.arange(len(P), dtype=np.uint32), args.icp_poses)
+    inits = np.concatenate([p["inits"] for p in P])
+    ns = np.array([len(p["source"]) for p in P])[cp]
+    t0 = time.perf_counter()
+    ex.icp_batch_ladder(per_rung, rungs, cp, inits)  # tunes the workgroup size per size bucket
+    t_tune = time.perf_counter() - t0
+    ex.icp_batch_ladder(per_rung, rungs, cp, inits)  # captures the graph
+    t0 = time.perf_counter()
+    out = ex.icp_batch_ladder(per_rung, rungs, cp, inits)
+    el = time.perf_counter() - t0
+    iters = out[:, :, 19]
+    cand_iter = float(iters.sum())
+    queries = float((ns[:, None] * (iters + 1)).sum())
+    log(f"icp ladder: {len(cp)} candidates x {len(rungs)} rungs in {el:.3f}s ({cand_iter:.0f} cand-iter, "
+        f"{queries / 1e9:.2f} G point-queries); first call with tuning {t_tune:.2f}s")
+    # Plain: the reference's schedule, one Executor::icp_rung per pair and rung.
+    t0 = time.perf_counter()
+    same = True
+    for p, pr in enumerate(P):
+        T = pr["inits"]
+        for r, (dist, it) in enumerate(w["rungs"]):
+            o = ex.icp_rung(pr["source"], pr["target"], pr["target_n"], T, 0, dist, it)
+            same &= bool(np.array_equal(o, out[cp == p][:, r]))
+            T = o[:, :16].reshape(-1, 4, 4)
+    el_plain = time.perf_counter() - t0
+    log(f"icp plain: {el_plain:.3f}s; identical to batched: {same}")
+    res = dict(candidates=int(len(cp)), rungs=len(rungs), cand_iter=int(cand_iter),
+               point_queries=int(queries), seconds=round(el, 4),
+               cand_iter_per_s=round(cand_iter / el, 1), point_queries_per_s=round(queries / el, 1),
+               plain_seconds=round(el_plain, 4), batched_vs_plain=round(el_plain / el, 3),
+               identical_plain_batched=same,
+               converged_last_rung=int(out[:, -1, 20].sum()),
+               workload=f"{len(P)} breakline pairs x {args.icp_poses} poses, {int(ns.min())}-{int(ns.max())} "
+                        f"source points (log-uniform), rungs 1.0/0.3/0.1/0.03 t x 20 iterations")
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(HERE, "tests"))
+        import oracle_lib as O
+
+        O.set_threads(min(os.cpu_count() or 1, 16))
+        pr = P[0]
+        k = 8
+        t0 = time.perf_counter()
+        T = pr["inits"][:k]
+        c_iters = 0.0
+        for r, (dist, it) in enumerate(w["rungs"]):
+            o = O.icp_rung(pr["source"], pr["target"], pr["target_n"], T, 0, dist, it)
+            c_iters += o[:, 19].sum()
+            T = o[:, :16].reshape(-1, 4, 4)
+        el_cpu = time.perf_counter() - t0
+        res["cpu_cand_iter_per_s"] = round(c_iters / el_cpu, 1)
+        res["cpu_sample"] = (f"oracle icp_rung ladder of pair 0's first {k} candidates ({len(pr['source'])} source "
+                             f"points), {min(os.cpu_count() or 1, 16)} OpenMP threads")
+    for s_, t_ in clouds:
+        s_.close()
+        t_.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +180,8 @@ def main():
     ap.add_argument("--edges-per-t", type=float, default=5.5)
     ap.add_argument("--plain-candidates", type=int, default=280)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--icp-pairs", type=int, default=28, help="pairs of the batched ICP ladder workload (0: skip)")
+    ap.add_argument("--icp-poses", type=int, default=64)
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched steps (for counter collection); prints no bench line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -220,6 +289,8 @@ def main():
                 frac=round(achieved / HBM_PEAK_GBS, 6), traffic=traffic, kernel=dom,
                 algorithmic_bytes_per_launch=nbytes, avg_launch_ms=round(avg_ms, 4))
 
+    icp = icp_ladder(ex, synth, args) if args.icp_pairs > 0 else None
+
     cpu = None
     if not args.no_cpu:
         cpu, oracle_scores = cpu_baseline(col, pairs, scales, cand_pair, cand_T)
@@ -238,7 +309,7 @@ def main():
         plain_pair_verifications_per_s=round(plain, 3), batched_vs_plain=round(value / plain, 3),
         accepted=int(accepted.sum()), accepted_identical_plain_batched=same,
         rays_per_s=round(rays_per_s, 1), rays=int(rays), knn_queries_per_s=round(knn, 1),
-        lbvh_mtri_per_s=round(lbvh_mtri, 2), kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
+        lbvh_mtri_per_s=round(lbvh_mtri, 2), icp=icp, kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
     )
     print(json.dumps(line), flush=True)
     ex.close()

@@ -284,3 +284,72 @@ def candidates(collection, per_pair=10, seed=20):
                 cp.append(len(pairs) - 1)
                 cT.append(T)
     return pairs, np.array(cp, np.uint32), np.array(cT)
+
+
+def _uniform_rotation_within(rng, max_angle):
+    """A rotation uniform on SO(3) restricted to angles <= max_angle (Haar density of the angle is
+    (1 - cos a) / pi; sampled by rejection)."""
+    axis = rng.normal(size=3)
+    axis /= np.linalg.norm(axis)
+    while True:
+        ang = rng.uniform(0.0, max_angle)
+        if rng.uniform() * (1.0 - np.cos(max_angle)) <= 1.0 - np.cos(ang):
+            break
+    k = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + np.sin(ang) * k + (1 - np.cos(ang)) * (k @ k)
+
+
+def _breakline(rng, n, length, t):
+    """n points along a smooth space curve of the given length (a fracture rim), with the shell
+    normal at each: a sum of low-frequency sines, sampled uniformly in arc parameter, with
+    0.02 t of jitter across the curve."""
+    freq = rng.uniform(0.5, 2.0, size=(3, 3))
+    amp = rng.uniform(0.05, 0.15, size=(3, 3)) * length
+    phase = rng.uniform(0.0, 2 * np.pi, size=(3, 3))
+    s = np.sort(rng.uniform(0.0, 1.0, size=n))
+    u = 2 * np.pi * s[:, None]
+    p = np.stack([(amp[k] * np.sin(freq[k] * u + phase[k])).sum(1) for k in range(3)], axis=1)
+    p[:, 0] += length * s
+    dp = np.stack([(amp[k] * freq[k] * np.cos(freq[k] * u + phase[k])).sum(1) for k in range(3)], axis=1)
+    dp[:, 0] += length / (2 * np.pi)
+    tan = dp / np.linalg.norm(dp, axis=1, keepdims=True)
+    ref = np.array([0.0, 0.0, 1.0])
+    nrm = np.cross(tan, ref)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    p += rng.normal(scale=0.02 * t, size=p.shape)
+    return p, nrm
+
+
+def icp_batch_workload(t=1.0, n_pairs=28, poses=64, nmin=2000, nmax=60000, length_t=60.0, seed=20, size_seed=21):
+    """The batched ICP ladder's workload: n_pairs breakline pairs whose point counts per side are
+    log-uniform in [nmin, nmax] (size_seed), each the same curve sampled twice with B's copy under a
+    rigid pose; and `poses` initial poses per pair, the truth composed with a rotation uniform on
+    SO(3) within 45 degrees and a translation in a 2 t ball (seed). Rungs: 1.0, 0.3, 0.1, 0.03 t,
+    20 iterations each, point-to-point (the breakline estimator, ladder.rs:131-140)."""
+    srng = np.random.default_rng(size_seed)
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(nmin), np.log(nmax)
+    pairs = []
+    for p in range(n_pairs):
+        ns, nt = (int(round(np.exp(srng.uniform(lo, hi)))) for _ in range(2))
+        crng = np.random.default_rng(1000 + p)
+        tgt, tn = _breakline(crng, nt, length_t * t, t)
+        # The source samples the same curve: re-draw parameters from the same curve seed.
+        src_curve, _ = _breakline(np.random.default_rng(1000 + p), ns, length_t * t, t)
+        R = _random_rotation(crng)
+        tau = crng.normal(scale=5.0 * t, size=3)
+        truth = np.eye(4)
+        truth[:3, :3] = R
+        truth[:3, 3] = tau
+        inv = np.linalg.inv(truth)
+        src = src_curve @ inv[:3, :3].T + inv[:3, 3]
+        inits = np.empty((poses, 4, 4))
+        for k in range(poses):
+            d = np.eye(4)
+            d[:3, :3] = _uniform_rotation_within(rng, np.radians(45.0))
+            v = rng.normal(size=3)
+            d[:3, 3] = v / np.linalg.norm(v) * 2.0 * t * rng.uniform() ** (1.0 / 3.0)
+            inits[k] = d @ truth
+        pairs.append(dict(source=src, target=tgt, target_n=tn, truth=truth, inits=inits))
+    rungs = [(1.0 * t, 20), (0.3 * t, 20), (0.1 * t, 20), (0.03 * t, 20)]
+    return dict(pairs=pairs, rungs=rungs, t=t)
