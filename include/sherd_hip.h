@@ -58,7 +58,7 @@ typedef struct {
 
 enum {
   SHERD_M_FIRST_HIT, SHERD_M_BOUNDED_DISTANCE, SHERD_M_INSIDE, SHERD_M_COARSE, SHERD_M_ICP,
-  SHERD_M_NEAREST, SHERD_M_VERIFY, SHERD_M_SCENE_BUILD, SHERD_M_CONE, SHERD_M_COUNT
+  SHERD_M_NEAREST, SHERD_M_VERIFY, SHERD_M_SCENE_BUILD, SHERD_M_CONE, SHERD_M_HUBER, SHERD_M_COUNT
 };
 
 /* ---- the executor ------------------------------------------------------------------------- */
@@ -189,6 +189,33 @@ int sherd_icp_batch_rung(sherd_ctx *ctx, const sherd_icp_pair *pairs, size_t n_p
  * autotuner, which times each size on the first rung of every source-size bucket and keeps the
  * fastest. No counterpart in the reference (its wgpu kernels have fixed tiles). */
 int sherd_icp_block(sherd_ctx *ctx, int threads);
+
+/* ---- the robust seam energy and its se(3) gradient (new) --------------------------------------- */
+
+/* New: no reference function computes this energy; its nearest relative is ICP's point-to-plane
+ * normal equations (crates/sherd-core/src/matching/icp.rs:599-640). For pose i (base transform T0,
+ * twist xi = (omega, v), rotation first as icp.rs:432-433 orders x) and frozen correspondences
+ * s_j -> (q, n):  p_j = exp(omega^) T0 s_j + v,  r_j = (p_j - q) . n,  E = sum rho_delta(r_j)
+ * (Huber), dE/dv = sum psi(r_j) n, dE/domega = J_l(omega)^T sum psi(r_j) (exp(omega^) T0 s_j) x n.
+ * At xi = 0 and delta -> infinity the gradient is normal_equations' JTr (icp.rs:629). */
+typedef struct sherd_huber sherd_huber;
+
+/* A batch of k poses: pose i on pair cand_pair[i] (the pair's target must carry normals; its
+ * distance is the correspondence radius) from T0[16 i ..]. The correspondences are taken at T0
+ * (icp.rs:361-392) and frozen. The clouds must outlive the handle. */
+int sherd_huber_create(sherd_ctx *ctx, const sherd_icp_pair *pairs, size_t n_pairs, const uint32_t *cand_pair,
+                       const double *T0, size_t k, sherd_huber **out);
+void sherd_huber_free(sherd_huber *h);
+/* The frozen correspondences, target index or -1, poses in order (n = the sum of their source sizes). */
+int sherd_huber_correspondences(sherd_huber *h, int32_t *corr, size_t n);
+/* E (k) and, if grad is not NULL, dE/dxi (k x 6) at the twists xi (k x 6). precision 32 computes the
+ * per-point terms and sums in f32 about the clouds' centroids; 64 in f64. */
+int sherd_huber_eval(sherd_huber *h, const double *xi, double delta, int precision, double *energy, double *grad);
+/* Batched Levenberg-Marquardt on E from T0: each iteration re-takes the correspondences at the
+ * current pose, solves the reweighted 6x6 system in f64 and keeps the step if E drops. T_out k x 16,
+ * energy k (at the final pose, over its last correspondences), steps k (accepted iterations). */
+int sherd_huber_lm(sherd_huber *h, double delta, int iterations, int precision, double *T_out, double *energy,
+                   uint32_t *steps);
 
 /* ---- R §6 verification (crates/sherd-core/src/matching/verify.rs) ------------------------------ */
 

@@ -503,6 +503,37 @@ void oracle_icp_rung(const double *src, size_t n_src, const double *tgt_p, const
                  out + 21 * c);
 }
 
+void oracle_icp_correspondences(const double *moved, size_t n, const double *tgt_p, size_t n_tgt, double distance,
+                                 int64_t *corr) {
+  oracle::Cloud tree(tgt_p, n_tgt, distance > 0 ? distance : 1.0);
+  std::vector<double> m(moved, moved + 3 * n);
+  Corr c;
+  correspondences(m, tree, distance, c);
+  for (size_t i = 0; i < n; ++i) corr[i] = -1;
+  for (auto &ij : c.pairs) corr[ij.first] = (int64_t)ij.second;
+}
+
+void oracle_normal_equations(const double *moved, size_t n, const double *tgt_p, const double *tgt_n,
+                             const int64_t *corr, double jtj[36], double jtr[6]) {
+  // icp.rs:599-640 over the pairs in source order, rows [p x n, n], residual (p - q) . n.
+  double a[6][6] = {{0}}, b[6] = {0};
+  for (size_t i = 0; i < n; ++i) {
+    if (corr[i] < 0) continue;
+    const double *p = moved + 3 * i, *q = tgt_p + 3 * corr[i], *nn = tgt_n + 3 * corr[i];
+    double row[6] = {p[1] * nn[2] - p[2] * nn[1], p[2] * nn[0] - p[0] * nn[2], p[0] * nn[1] - p[1] * nn[0],
+                     nn[0], nn[1], nn[2]};
+    double r = (p[0] - q[0]) * nn[0] + (p[1] - q[1]) * nn[1] + (p[2] - q[2]) * nn[2];
+    for (int k = 0; k < 6; ++k) {
+      for (int l = 0; l <= k; ++l) a[k][l] = a[k][l] + row[k] * row[l];
+      b[k] = b[k] + row[k] * r;
+    }
+  }
+  for (int k = 0; k < 6; ++k) {
+    for (int l = 0; l <= k; ++l) jtj[6 * k + l] = jtj[6 * l + k] = a[k][l];
+    jtr[k] = b[k];
+  }
+}
+
 void oracle_scales_for_pair(double t, double res, double out[12]) {
   auto f = [&](double k, double m) { return rmax(k * t, m * res); };
   out[S_T] = t;

@@ -87,6 +87,15 @@ void oracle_icp_rung(const double *src, size_t n_src, const double *tgt_p, const
                      size_t n_tgt, const double *inits, size_t k, int estimation, double distance,
                      int max_iteration, double *out);
 
+/* icp.rs:361-392 correspondences of already-moved source points: corr[i] = the nearest target
+ * strictly inside `distance`, or -1. */
+void oracle_icp_correspondences(const double *moved, size_t n, const double *tgt_p, size_t n_tgt, double distance,
+                                int64_t *corr);
+/* icp.rs:599-640 normal_equations (world assembly) over the pairs of `corr`: JTJ (6x6, row-major)
+ * and JTr for rows [p x n, n] and residuals (p - q) . n, summed in source order. */
+void oracle_normal_equations(const double *moved, size_t n, const double *tgt_p, const double *tgt_n,
+                             const int64_t *corr, double jtj[36], double jtr[6]);
+
 /* One fragment as R §6 measures it (crates/sherd-core/src/matching/verify.rs:223-302). */
 typedef struct {
   const oracle_scene *fracture;

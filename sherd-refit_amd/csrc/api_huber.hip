@@ -1,0 +1,200 @@
+// C ABI for the robust seam energy: a batch of poses with frozen correspondences, its energy and
+// se(3) gradient, and the batched LM refinement (huber.hip). Host code only.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstring>
+#include <vector>
+
+#include "../../include/sherd_hip.h"
+#include "api_internal.hpp"
+#include "kernels.hpp"
+
+using namespace sherd;
+
+#define HIPCK(call, what)                            \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+struct sherd_huber {
+  Ctx *ctx = nullptr;
+  int k = 0;
+  size_t points = 0;
+  std::vector<size_t> off;
+  IcpPairDesc *pairs = nullptr;
+  uint32_t *cand_pair = nullptr;
+  size_t *doff = nullptr;
+  double *T0 = nullptr;
+  int32_t *corr = nullptr;
+  int32_t *lm_corr = nullptr;
+  double *xi = nullptr;
+  double *E = nullptr;
+  double *G = nullptr;
+  double *T_out = nullptr;
+  uint32_t *steps = nullptr;
+  HuberView view() const {
+    HuberView v;
+    v.pairs = pairs;
+    v.cand_pair = cand_pair;
+    v.off = doff;
+    v.T0 = T0;
+    v.corr = corr;
+    v.lm_corr = lm_corr;
+    v.k = k;
+    return v;
+  }
+};
+
+namespace {
+
+void free_huber(sherd_huber *h) {
+  void *ptrs[] = {h->pairs, h->cand_pair, h->doff, h->T0, h->corr, h->lm_corr, h->xi, h->E, h->G, h->T_out, h->steps};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  delete h;
+}
+
+template <class X>
+hipError_t dnew(X **p, size_t n) {
+  return hipMalloc((void **)p, sizeof(X) * (n ? n : 1));
+}
+
+}  // namespace
+
+extern "C" {
+
+int sherd_huber_create(sherd_ctx *ctx, const sherd_icp_pair *pairs, size_t n_pairs, const uint32_t *cand_pair,
+                       const double *T0, size_t k, sherd_huber **out) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || !out) return fail(SHERD_E_ARG, "null argument");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lock(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
+  if (k == 0 || !pairs || !cand_pair || !T0) return fail(SHERD_E_ARG, "empty batch or null buffer");
+  if (k > (size_t)INT_MAX || n_pairs > (size_t)INT_MAX) return fail(SHERD_E_LIMIT, "batch too large");
+  std::vector<IcpPairDesc> pd(n_pairs);
+  for (size_t p = 0; p < n_pairs; ++p) {
+    const sherd_cloud *s = pairs[p].source, *t = pairs[p].target;
+    if (!s || !t || s->ctx != c || t->ctx != c) return fail(SHERD_E_ARG, "pair without a cloud of this context");
+    if (t->n > 0 && !t->normals) return fail(SHERD_E_ARG, "the point-to-plane energy needs target normals");
+    IcpPairDesc &d = pd[p];
+    std::memset(&d, 0, sizeof d);
+    d.src = s->pts;
+    d.ns = t->n > 0 ? s->n : 0;
+    d.tp = t->pts;
+    d.tn = t->normals;
+    if (t->n > 0) d.tree = view_of(t->tree);
+    d.r = pairs[p].distance;
+    for (int a = 0; a < 3; ++a) {
+      d.cs[a] = s->centroid[a];
+      d.ct[a] = t->centroid[a];
+    }
+  }
+  sherd_huber *h = new sherd_huber();
+  h->ctx = c;
+  h->k = (int)k;
+  h->off.resize(k);
+  for (size_t i = 0; i < k; ++i) {
+    if (cand_pair[i] >= n_pairs) {
+      delete h;
+      return fail(SHERD_E_ARG, "pose names a pair out of range");
+    }
+    h->off[i] = h->points;
+    h->points += (size_t)pd[cand_pair[i]].ns;
+  }
+  hipError_t e = hipSuccess;
+  do {
+    if ((e = dnew(&h->pairs, n_pairs)) != hipSuccess) break;
+    if ((e = dnew(&h->cand_pair, k)) != hipSuccess) break;
+    if ((e = dnew(&h->doff, k)) != hipSuccess) break;
+    if ((e = dnew(&h->T0, 16 * k)) != hipSuccess) break;
+    if ((e = dnew(&h->corr, h->points)) != hipSuccess) break;
+    if ((e = dnew(&h->lm_corr, h->points)) != hipSuccess) break;
+    if ((e = dnew(&h->xi, 6 * k)) != hipSuccess) break;
+    if ((e = dnew(&h->E, k)) != hipSuccess) break;
+    if ((e = dnew(&h->G, 6 * k)) != hipSuccess) break;
+    if ((e = dnew(&h->T_out, 16 * k)) != hipSuccess) break;
+    if ((e = dnew(&h->steps, k)) != hipSuccess) break;
+    if ((e = hipMemcpyAsync(h->pairs, pd.data(), sizeof(IcpPairDesc) * n_pairs, hipMemcpyHostToDevice, c->stream)))
+      break;
+    if ((e = hipMemcpyAsync(h->cand_pair, cand_pair, sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream))) break;
+    if ((e = hipMemcpyAsync(h->doff, h->off.data(), sizeof(size_t) * k, hipMemcpyHostToDevice, c->stream))) break;
+    if ((e = hipMemcpyAsync(h->T0, T0, sizeof(double) * 16 * k, hipMemcpyHostToDevice, c->stream))) break;
+    if ((e = launch_correspond(h->view(), h->T0, h->corr, c->stream)) != hipSuccess) break;
+    e = hipStreamSynchronize(c->stream);
+  } while (0);
+  if (e != hipSuccess) {
+    free_huber(h);
+    return hip_fail(e, "huber batch");
+  }
+  *out = h;
+  return SHERD_OK;
+}
+
+void sherd_huber_free(sherd_huber *h) {
+  if (!h) return;
+  std::lock_guard<std::mutex> lock(h->ctx->mu);
+  (void)hipSetDevice(h->ctx->device);
+  (void)hipStreamSynchronize(h->ctx->stream);
+  free_huber(h);
+}
+
+int sherd_huber_correspondences(sherd_huber *h, int32_t *corr, size_t n) {
+  if (!h || (!corr && n)) return fail(SHERD_E_ARG, "null argument");
+  Ctx *c = h->ctx;
+  std::lock_guard<std::mutex> lock(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
+  if (n != h->points) return fail(SHERD_E_ARG, "buffer size differs from the batch's source points");
+  HIPCK(hipMemcpyAsync(corr, h->corr, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  return SHERD_OK;
+}
+
+int sherd_huber_eval(sherd_huber *h, const double *xi, double delta, int precision, double *energy, double *grad) {
+  if (!h || !xi || !energy) return fail(SHERD_E_ARG, "null argument");
+  if (precision != 32 && precision != 64) return fail(SHERD_E_ARG, "precision must be 32 or 64");
+  if (!(delta > 0.0)) return fail(SHERD_E_ARG, "delta must be positive");
+  Ctx *c = h->ctx;
+  std::lock_guard<std::mutex> lock(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
+  size_t k = (size_t)h->k;
+  HIPCK(hipMemcpyAsync(h->xi, xi, sizeof(double) * 6 * k, hipMemcpyHostToDevice, c->stream), "upload");
+  Timer tm(c, SHERD_M_HUBER, k);
+  if (c->prof.on) c->prof.begin(grad ? "k_huber_grad" : "k_huber_energy", c->stream);
+  HIPCK(launch_huber_eval(h->view(), h->xi, delta, precision == 64, h->E, grad ? h->G : nullptr, c->stream),
+        "huber kernel");
+  if (c->prof.on) c->prof.end(c->stream);
+  tm.stop();
+  HIPCK(hipMemcpyAsync(energy, h->E, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream), "download");
+  if (grad) HIPCK(hipMemcpyAsync(grad, h->G, sizeof(double) * 6 * k, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  c->prof.collect();
+  return SHERD_OK;
+}
+
+int sherd_huber_lm(sherd_huber *h, double delta, int iterations, int precision, double *T_out, double *energy,
+                   uint32_t *steps) {
+  if (!h || !T_out || !energy) return fail(SHERD_E_ARG, "null argument");
+  if (precision != 32 && precision != 64) return fail(SHERD_E_ARG, "precision must be 32 or 64");
+  if (!(delta > 0.0) || iterations < 0) return fail(SHERD_E_ARG, "bad delta or iteration count");
+  Ctx *c = h->ctx;
+  std::lock_guard<std::mutex> lock(c->mu);
+  if (hipSetDevice(c->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
+  size_t k = (size_t)h->k;
+  Timer tm(c, SHERD_M_HUBER, k);
+  if (c->prof.on) c->prof.begin("k_huber_lm", c->stream);
+  HIPCK(launch_huber_lm(h->view(), delta, iterations, precision == 64, h->T_out, h->E, h->steps, c->stream),
+        "lm kernel");
+  if (c->prof.on) c->prof.end(c->stream);
+  tm.stop();
+  HIPCK(hipMemcpyAsync(T_out, h->T_out, sizeof(double) * 16 * k, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipMemcpyAsync(energy, h->E, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream), "download");
+  if (steps) HIPCK(hipMemcpyAsync(steps, h->steps, sizeof(uint32_t) * k, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  c->prof.collect();
+  return SHERD_OK;
+}
+
+}  // extern "C"

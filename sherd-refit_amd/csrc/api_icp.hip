@@ -27,13 +27,7 @@ using namespace sherd;
     if (e_ != hipSuccess) return hip_fail(e_, what); \
   } while (0)
 
-struct sherd_cloud {
-  Ctx *ctx = nullptr;
-  double *pts = nullptr;
-  double *normals = nullptr;
-  int n = 0;
-  DevScene tree;  // point LBVH (each leaf one point)
-};
+
 
 namespace {
 
@@ -67,6 +61,11 @@ int make_cloud(Ctx *c, const double *pts, const double *normals, size_t n, sherd
   sherd_cloud *cl = new sherd_cloud();
   cl->ctx = c;
   cl->n = (int)n;
+  for (int k = 0; k < 3; ++k) {
+    double m = 0.0;
+    for (size_t i = 0; i < n; ++i) m += pts[3 * i + k];
+    cl->centroid[k] = n ? m / (double)n : 0.0;
+  }
   hipError_t e = hipSuccess;
   float *V = nullptr;
   uint32_t *F = nullptr;
@@ -146,6 +145,10 @@ int plan_rung(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const uint32_
     d.tn = estimation == SHERD_POINT_TO_PLANE ? t->normals : nullptr;
     if (t->n > 0) d.tree = view_of(t->tree);
     d.r = pairs[p].distance;
+    for (int k = 0; k < 3; ++k) {
+      d.cs[k] = s->centroid[k];
+      d.ct[k] = t->centroid[k];
+    }
   }
   // Point-to-plane without normals makes identity updates (icp.rs:460-462): the kernel's est 1
   // with a null normal pointer does that per pair.

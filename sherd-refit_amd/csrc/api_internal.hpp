@@ -58,6 +58,15 @@ struct sherd_scene {
   sherd::DevScene d;
 };
 
+struct sherd_cloud {
+  sherd::Ctx *ctx = nullptr;
+  double *pts = nullptr;
+  double *normals = nullptr;
+  int n = 0;
+  double centroid[3] = {0, 0, 0};
+  sherd::DevScene tree;  // point LBVH (each leaf one point)
+};
+
 struct sherd_fragment {
   sherd::Ctx *ctx = nullptr;
   const sherd_scene *frac = nullptr;

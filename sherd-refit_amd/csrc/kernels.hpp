@@ -106,6 +106,7 @@ struct IcpPairDesc {
   const double *tn;
   SceneView tree;
   double r;
+  double cs[3], ct[3];  // the source's and the target's centroids
 };
 
 struct IcpBatchView {
@@ -125,6 +126,23 @@ struct IcpBatchView {
   int est;                // 0 point-to-point, 1 point-to-plane, 2 identity updates
   int max_iter;
 };
+
+// ---- Huber energy and LM (huber.hip) ----------------------------------------------------------------
+struct HuberView {
+  const IcpPairDesc *pairs;
+  const uint32_t *cand_pair;
+  const size_t *off;     // each pose's first source point in corr / lm_corr
+  const double *T0;      // 16 per pose
+  const int32_t *corr;   // frozen correspondences (target index or -1)
+  int32_t *lm_corr;      // the LM's own correspondences
+  int k;
+};
+
+hipError_t launch_correspond(const HuberView &v, const double *Tc, int32_t *corr, hipStream_t s);
+hipError_t launch_huber_eval(const HuberView &v, const double *xi, double delta, int f64, double *E, double *G,
+                             hipStream_t s);
+hipError_t launch_huber_lm(const HuberView &v, double delta, int iters, int f64, double *T_out, double *E_out,
+                           uint32_t *steps, hipStream_t s);
 
 // Workgroup sizes the batched kernel is built for (the autotuner's choices); returns their count.
 int icp_block_sizes(int *out);

@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("SHERD_HIP_LIB") or os.path.join(HERE, "lib", "libsherd_hip.so")
 
 SHERD_M = dict(first_hit=0, bounded_distance=1, inside=2, coarse=3, icp=4, nearest=5, verify=6,
-               scene_build=7, cone=8)
+               scene_build=7, cone=8, huber=9)
 
 _lib = None
 
@@ -106,6 +106,11 @@ def _bind(L):
         "sherd_cloud_create": (ci, [vp, P(d), P(d), sz, P(vp)]),
         "sherd_cloud_free": (None, [vp]),
         "sherd_icp_block": (ci, [vp, ci]),
+        "sherd_huber_create": (ci, [vp, P(IcpPair), sz, P(u32), P(d), sz, P(vp)]),
+        "sherd_huber_free": (None, [vp]),
+        "sherd_huber_correspondences": (ci, [vp, P(C.c_int32), sz]),
+        "sherd_huber_eval": (ci, [vp, P(d), d, ci, P(d), P(d)]),
+        "sherd_huber_lm": (ci, [vp, d, ci, ci, P(d), P(d), P(u32)]),
         "sherd_icp_batch_rung": (ci, [vp, P(IcpPair), sz, P(u32), P(d), sz, ci, ci, P(Registration)]),
         "sherd_icp_batch_ladder": (ci, [vp, P(IcpPair), sz, P(IcpRungSpec), sz, P(u32), P(d), sz,
                                         P(Registration)]),
@@ -212,6 +217,61 @@ def _registrations(out, k):
 
 
 _EST = {"point_to_point": 0, "point_to_plane": 1}
+
+
+class HuberBatch:
+    """k poses of the robust point-to-plane seam energy with correspondences frozen at their base
+    transforms (the C ABI's sherd_huber_*). pairs: [(source Cloud, target Cloud with normals,
+    correspondence radius)]; cand_pair: the pair of each pose; T0: (k, 4, 4)."""
+
+    def __init__(self, ex, pairs, cand_pair, T0):
+        self.ex = ex
+        self.clouds = [(s_, t_) for s_, t_, _ in pairs]
+        arr = (IcpPair * max(len(pairs), 1))()
+        for p, (src, tgt, dist) in enumerate(pairs):
+            arr[p] = IcpPair(src.ptr, tgt.ptr, float(dist))
+        self.cand_pair = np.ascontiguousarray(cand_pair, dtype=np.uint32)
+        T0 = np.ascontiguousarray(T0, dtype=np.float64).reshape(-1, 16)
+        self.k = len(T0)
+        self.points = int(sum(pairs[int(p)][0].n if pairs[int(p)][1].n else 0 for p in self.cand_pair))
+        out = C.c_void_p()
+        _check(lib().sherd_huber_create(ex.ptr, arr, len(pairs), _p(self.cand_pair, C.c_uint32), _p(T0, C.c_double),
+                                        self.k, C.byref(out)))
+        self.ptr = out
+
+    def correspondences(self):
+        out = np.empty(self.points, np.int32)
+        _check(lib().sherd_huber_correspondences(self.ptr, _p(out, C.c_int32), self.points))
+        return out
+
+    def energy(self, xi, delta, precision=64, grad=False):
+        """E (k,) and, with grad, dE/dxi (k, 6) at the twists xi (k, 6)."""
+        xi = np.ascontiguousarray(xi, dtype=np.float64).reshape(self.k, 6)
+        E = np.empty(self.k)
+        G = np.empty((self.k, 6)) if grad else None
+        _check(lib().sherd_huber_eval(self.ptr, _p(xi, C.c_double), float(delta), int(precision), _p(E, C.c_double),
+                                      _p(G, C.c_double)))
+        return (E, G) if grad else E
+
+    def lm(self, delta, iterations, precision=64):
+        """Batched LM from the base transforms: (T (k, 4, 4), E (k,), accepted steps (k,))."""
+        T = np.empty((self.k, 16))
+        E = np.empty(self.k)
+        st = np.empty(self.k, np.uint32)
+        _check(lib().sherd_huber_lm(self.ptr, float(delta), int(iterations), int(precision), _p(T, C.c_double),
+                                    _p(E, C.c_double), _p(st, C.c_uint32)))
+        return T.reshape(-1, 4, 4), E, st
+
+    def close(self):
+        if self.ptr:
+            lib().sherd_huber_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Fragment:
@@ -362,6 +422,9 @@ class HipExecutor:
 
     def cloud(self, pts, normals=None):
         return Cloud(self, pts, normals)
+
+    def huber(self, pairs, cand_pair, T0):
+        return HuberBatch(self, pairs, cand_pair, T0)
 
     def icp_block(self, threads):
         """Fix the ICP workgroup size (64/128/256/512) or leave it to the autotuner (0)."""

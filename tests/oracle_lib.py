@@ -72,6 +72,8 @@ def _bind(L):
     L.oracle_nearest_f32.argtypes = [P(f), sz, P(f), sz, f, C.c_int, P(i64), P(f)]
     L.oracle_coarse_scores.argtypes = [P(d), P(d), sz, P(d), P(d), sz, P(d), sz, d, d, P(u32), P(d)]
     L.oracle_icp_rung.argtypes = [P(d), sz, P(d), P(d), sz, P(d), sz, C.c_int, d, C.c_int, P(d)]
+    L.oracle_icp_correspondences.argtypes = [P(d), sz, P(d), sz, d, P(i64)]
+    L.oracle_normal_equations.argtypes = [P(d), sz, P(d), P(d), P(i64), P(d), P(d)]
     L.oracle_scales_for_pair.argtypes = [d, d, P(d)]
     L.oracle_verify.argtypes = [P(Side), P(Side), P(d), P(d), C.c_int, C.c_int, P(d)]
     L.oracle_accept.restype = C.c_int
@@ -219,6 +221,25 @@ def icp_rung(src, tgt_p, tgt_n, inits, estimation, distance, max_iteration):
                           _p(inits, C.c_double), len(inits), int(est), float(distance), int(max_iteration),
                           _p(out, C.c_double))
     return out
+
+
+def icp_correspondences(moved, tgt_p, distance):
+    """icp.rs:361-392: per moved source point the nearest target strictly inside distance, or -1."""
+    m, tp = f64(moved).reshape(-1, 3), f64(tgt_p).reshape(-1, 3)
+    out = np.empty(len(m), np.int64)
+    lib().oracle_icp_correspondences(_p(m, C.c_double), len(m), _p(tp, C.c_double), len(tp), float(distance),
+                                     _p(out, C.c_int64))
+    return out
+
+
+def normal_equations(moved, tgt_p, tgt_n, corr):
+    """icp.rs:599-640: (JTJ 6x6, JTr 6) over the pairs of corr, source order."""
+    m, tp, tn = f64(moved).reshape(-1, 3), f64(tgt_p).reshape(-1, 3), f64(tgt_n).reshape(-1, 3)
+    c = np.ascontiguousarray(corr, dtype=np.int64)
+    jtj, jtr = np.empty(36), np.empty(6)
+    lib().oracle_normal_equations(_p(m, C.c_double), len(m), _p(tp, C.c_double), _p(tn, C.c_double),
+                                  _p(c, C.c_int64), _p(jtj, C.c_double), _p(jtr, C.c_double))
+    return jtj.reshape(6, 6), jtr
 
 
 def scales_for_pair(t, res):

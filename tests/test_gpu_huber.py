@@ -1,0 +1,176 @@
+"""The robust seam energy on the device: correspondences, energy, se(3) gradient (the hand-written
+backward), the torch.autograd face and the batched LM, against the FP64 oracle (oracle/huber.py)."""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import huber as H  # noqa: E402
+import oracle_lib as O  # noqa: E402
+from test_gpu_match import pose_gap  # noqa: E402
+from test_oracle_known import patch, rigid  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ex():
+    import sherd_hip
+
+    e = sherd_hip.HipExecutor(0)
+    yield e
+    e.close()
+
+
+def make(ex, sizes=(30, 60), poses=4, seed=0, offset=(40.0, -25.0, 10.0)):
+    """Pairs of wavy patches far from the origin (so that the shift matters), several base poses each."""
+    rng = np.random.default_rng(seed)
+    raw, clouds = [], []
+    for n in sizes:
+        pts, nrm = patch(n, 9.0 / n, -4.5)
+        pts = pts + offset
+        truth = rigid(rng.normal(size=3), 0.04, rng.normal(scale=0.05, size=3))
+        inv = np.linalg.inv(truth)
+        keep = rng.random(len(pts)) < 0.8
+        src = (inv[:3, :3] @ pts[keep].T).T + inv[:3, 3] + rng.normal(scale=0.01, size=(keep.sum(), 3))
+        raw.append((src, pts, nrm, truth))
+        clouds.append((ex.cloud(src), ex.cloud(pts, nrm), 0.6))
+    cp = np.repeat(np.arange(len(sizes), dtype=np.uint32), poses)
+    T0 = np.stack([rigid(rng.normal(size=3), 0.01 * rng.random(), rng.normal(scale=0.02, size=3)) @ raw[p][3]
+                   for p in cp])
+    return raw, clouds, cp, T0
+
+
+def oracle_terms(raw, cp, T0, corr):
+    """Per pose: (p0, q, n) of its frozen pairs, from the device's correspondence list."""
+    out, at = [], 0
+    for i, p in enumerate(cp):
+        src, pts, nrm, _ = raw[p]
+        c = corr[at:at + len(src)]
+        at += len(src)
+        m = c >= 0
+        out.append((H.base_points(T0[i], src)[m], pts[c[m]], nrm[c[m]]))
+    return out
+
+
+def test_correspondences_match_the_oracle(ex):
+    say("case: frozen correspondences == oracle icp correspondences")
+    raw, clouds, cp, T0 = make(ex)
+    b = ex.huber(clouds, cp, T0)
+    corr = b.correspondences()
+    at = 0
+    for i, p in enumerate(cp):
+        src, pts, _, _ = raw[p]
+        want = O.icp_correspondences(H.base_points(T0[i], src), pts, 0.6)
+        assert np.array_equal(corr[at:at + len(src)], want), i
+        at += len(src)
+
+
+@pytest.mark.parametrize("delta", [0.02, 0.1, 1e30])
+def test_energy_and_gradient_f64_against_the_oracle(ex, delta):
+    say(f"case: Huber f64 energy/gradient vs oracle, delta {delta}")
+    raw, clouds, cp, T0 = make(ex, seed=1)
+    b = ex.huber(clouds, cp, T0)
+    terms = oracle_terms(raw, cp, T0, b.correspondences())
+    rng = np.random.default_rng(2)
+    xi = rng.normal(scale=0.01, size=(len(cp), 6))
+    xi[0] = 0.0
+    E, G = b.energy(xi, delta, 64, grad=True)
+    for i, (p0, q, n) in enumerate(terms):
+        e = H.energy(p0, q, n, xi[i], delta)
+        g = H.gradient(p0, q, n, xi[i], delta)
+        assert abs(E[i] - e) <= 1e-11 * abs(e), i
+        assert np.abs(G[i] - g).max() <= 1e-10 * np.abs(g).max(), i
+
+
+def test_energy_and_gradient_f32_error_against_the_oracle(ex):
+    say("case: Huber f32 (centroid-shifted) energy/gradient error vs the FP64 oracle")
+    raw, clouds, cp, T0 = make(ex, seed=3)
+    b = ex.huber(clouds, cp, T0)
+    terms = oracle_terms(raw, cp, T0, b.correspondences())
+    xi = np.random.default_rng(4).normal(scale=0.01, size=(len(cp), 6))
+    E, G = b.energy(xi, 0.05, 32, grad=True)
+    worst_e = worst_g = 0.0
+    for i, (p0, q, n) in enumerate(terms):
+        e = H.energy(p0, q, n, xi[i], 0.05)
+        g = H.gradient(p0, q, n, xi[i], 0.05)
+        worst_e = max(worst_e, abs(E[i] - e) / abs(e))
+        worst_g = max(worst_g, np.abs(G[i] - g).max() / np.abs(g).max())
+    say(f"  f32 relative error: energy {worst_e:.2e}, gradient {worst_g:.2e}")
+    assert worst_e < 1e-4 and worst_g < 1e-3
+
+
+@pytest.mark.parametrize("delta", [0.05, 1e30])
+def test_gradient_against_fp64_central_differences(ex, delta):
+    say(f"case: device gradient vs FP64 central differences (h = 1e-4), delta {delta}")
+    raw, clouds, cp, T0 = make(ex, seed=5)
+    b = ex.huber(clouds, cp, T0)
+    terms = oracle_terms(raw, cp, T0, b.correspondences())
+    xi = np.random.default_rng(6).normal(scale=0.01, size=(len(cp), 6))
+    _, G = b.energy(xi, delta, 64, grad=True)
+    tol = 1e-6 if delta > 1.0 else 5e-4
+    for i, (p0, q, n) in enumerate(terms):
+        fd = H.finite_difference(p0, q, n, xi[i], delta, h=1e-4)
+        rel = np.abs(G[i] - fd).max() / np.abs(fd).max()
+        assert rel <= tol, (i, rel)
+
+
+def test_infinite_delta_gradient_is_the_icp_jtr(ex):
+    say("case: delta -> infinity, xi = 0: gradient == normal_equations JTr")
+    raw, clouds, cp, T0 = make(ex, seed=7)
+    b = ex.huber(clouds, cp, T0)
+    corr = b.correspondences()
+    _, G = b.energy(np.zeros((len(cp), 6)), 1e30, 64, grad=True)
+    at = 0
+    for i, p in enumerate(cp):
+        src, pts, nrm, _ = raw[p]
+        c = corr[at:at + len(src)].astype(np.int64)
+        at += len(src)
+        _, jtr = O.normal_equations(H.base_points(T0[i], src), pts, nrm, c)
+        assert np.abs(G[i] - jtr).max() <= 1e-9 * np.abs(jtr).max(), i
+
+
+def test_autograd_function_backward_is_the_gradient_kernel(ex):
+    say("case: torch.autograd.Function over the device energy")
+    import torch
+
+    from sherd_hip.autograd import huber_energy
+
+    raw, clouds, cp, T0 = make(ex, seed=8)
+    b = ex.huber(clouds, cp, T0)
+    xi = torch.tensor(np.random.default_rng(9).normal(scale=0.01, size=(len(cp), 6)), dtype=torch.float64,
+                      requires_grad=True)
+    w = torch.linspace(0.5, 2.0, len(cp), dtype=torch.float64)
+    loss = (huber_energy(xi, b, 0.05) * w).sum()
+    (g,) = torch.autograd.grad(loss, xi)
+    _, G = b.energy(xi.detach().numpy(), 0.05, 64, grad=True)
+    assert np.allclose(g.numpy(), G * w.numpy()[:, None], rtol=0, atol=0)
+    terms = oracle_terms(raw, cp, T0, b.correspondences())
+    for i, (p0, q, n) in enumerate(terms):
+        fd = H.finite_difference(p0, q, n, xi.detach().numpy()[i], 0.05)
+        assert np.abs(g.numpy()[i] / w.numpy()[i] - fd).max() <= 5e-4 * np.abs(fd).max()
+
+
+def test_batched_lm_reduces_the_energy_and_agrees_across_precisions(ex):
+    say("case: batched LM (FP64 6x6 solve), f64 and f32 sums")
+    raw, clouds, cp, T0 = make(ex, sizes=(30, 50, 70), poses=6, seed=10)
+    b = ex.huber(clouds, cp, T0)
+    E0 = b.energy(np.zeros((len(cp), 6)), 0.05, 64)
+    T64, E64, st64 = b.lm(0.05, 30, 64)
+    T32, E32, st32 = b.lm(0.05, 30, 32)
+    assert np.all(E64 <= E0 * (1 + 1e-12)) and np.all(st64 > 0)
+    # Near the truth: the poses recover the synthetic displacement to the noise level.
+    truths = np.stack([raw[p][3] for p in cp])
+    ang, tr = pose_gap(T64, truths, 1.0)
+    say(f"  LM vs truth: worst {ang.max():.2e} deg {tr.max():.2e}")
+    assert ang.max() < 0.2 and tr.max() < 0.02
+    ang, tr = pose_gap(T32, T64, 1.0)
+    say(f"  LM f32 vs f64: worst {np.radians(ang).max():.2e} rad {tr.max():.2e} t")
+    assert np.radians(ang).max() < 1e-5 and tr.max() < 1e-4
