@@ -172,6 +172,91 @@ def icp_ladder(ex, synth, args):
     return res
 
 
+def huber_section(ex, synth, args):
+    """The robust seam energy over 512 batched poses (16 breakline pairs x 32 poses, 10k-40k points
+    per side): energy+gradient evaluations/s in f32 and f64, the f32 error against the FP64 oracle,
+    the device gradient against FP64 central differences, and the batched LM."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import huber as H
+
+    w = synth.huber_workload()
+    P = w["pairs"]
+    clouds = [(ex.cloud(p["source"]), ex.cloud(p["target"], p["target_n"]), w["radius"]) for p in P]
+    poses = len(P[0]["T0"])
+    cp = np.repeat(np.arange(len(P), dtype=np.uint32), poses)
+    T0 = np.concatenate([p["T0"] for p in P])
+    b = ex.huber(clouds, cp, T0)
+    k = len(cp)
+    xi = np.random.default_rng(31).normal(scale=1e-3, size=(k, 6))
+    corr = b.correspondences()
+    pairs_used = int((corr >= 0).sum())
+    res = dict(poses=k, corresponding_points=pairs_used, delta=w["delta"])
+    for prec in (32, 64):
+        b.energy(xi, w["delta"], prec, grad=True)
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            E, G = b.energy(xi, w["delta"], prec, grad=True)
+        el = (time.perf_counter() - t0) / reps
+        res[f"f{prec}_eval_ms"] = round(1e3 * el, 3)
+        res[f"f{prec}_point_terms_per_s"] = round(pairs_used / el, 1)
+        if prec == 32:
+            E32, G32 = E, G
+        else:
+            E64, G64 = E, G
+    # Error against the FP64 oracle on a sample of poses (one per pair), and FD of the oracle.
+    at, errs_e, errs_g, errs_fd = 0, [], [], []
+    offs = np.concatenate([[0], np.cumsum([len(P[p]["source"]) for p in cp])])
+    for p in range(len(P)):
+        i = p * poses
+        c = corr[offs[i]:offs[i + 1]]
+        m = c >= 0
+        p0 = H.base_points(T0[i], P[p]["source"])[m]
+        q, n = P[p]["target"][c[m]], P[p]["target_n"][c[m]]
+        e = H.energy(p0, q, n, xi[i], w["delta"])
+        g = H.gradient(p0, q, n, xi[i], w["delta"])
+        errs_e.append(abs(E32[i] - e) / abs(e))
+        errs_g.append(np.abs(G32[i] - g).max() / np.abs(g).max())
+        if p < 4:
+            fd = H.finite_difference(p0, q, n, xi[i], w["delta"], h=1e-4)
+            errs_fd.append(np.abs(G64[i] - fd).max() / np.abs(fd).max())
+    res["f32_energy_rel_err_max"] = float(max(errs_e))
+    res["f32_grad_rel_err_max"] = float(max(errs_g))
+    res["f64_grad_vs_fp64_fd_rel_max"] = float(max(errs_fd))
+    E0 = b.energy(np.zeros((k, 6)), w["delta"], 64)
+    for prec in (64, 32):
+        b.lm(w["delta"], 20, prec)
+        t0 = time.perf_counter()
+        T, E, st = b.lm(w["delta"], 20, prec)
+        res[f"lm_f{prec}_ms"] = round(1e3 * (time.perf_counter() - t0), 3)
+        if prec == 64:
+            T64 = T
+            truths = np.stack([P[p]["truth"] for p in cp])
+            dr = T[:, :3, :3] @ np.transpose(truths[:, :3, :3], (0, 2, 1))
+            rot = np.linalg.norm(dr - np.eye(3), axis=(1, 2)) / np.sqrt(2)
+            res["lm_f64_mean_steps"] = round(float(st.mean()), 2)
+            # The energy is over each pose's own final correspondences, so it is reported, not compared.
+            res["lm_f64_energy_final_over_initial_median"] = float(np.median(E / np.maximum(E0, 1e-300)))
+            trn = np.linalg.norm(T[:, :3, 3] - truths[:, :3, 3], axis=1)
+            # Quantiles (median, 90th, max): point-to-plane on a curve leaves the tangent free, so a
+            # few poses slide along it.
+            res["lm_f64_rot_to_truth_rad_q50_q90_max"] = [float(x) for x in np.quantile(rot, [0.5, 0.9, 1.0])]
+            res["lm_f64_trans_to_truth_t_q50_q90_max"] = [float(x) for x in np.quantile(trn, [0.5, 0.9, 1.0])]
+        else:
+            dr = T[:, :3, :3] @ np.transpose(T64[:, :3, :3], (0, 2, 1))
+            rot = np.linalg.norm(dr - np.eye(3), axis=(1, 2)) / np.sqrt(2)
+            trn = np.linalg.norm(T[:, :3, 3] - T64[:, :3, 3], axis=1)
+            res["lm_f32_vs_f64_rot_rad_q50_q90_max"] = [float(x) for x in np.quantile(rot, [0.5, 0.9, 1.0])]
+            res["lm_f32_vs_f64_trans_t_q50_q90_max"] = [float(x) for x in np.quantile(trn, [0.5, 0.9, 1.0])]
+    log(f"huber: {k} poses, {pairs_used} corresponding points; f32 eval {res['f32_eval_ms']} ms, f64 eval "
+        f"{res['f64_eval_ms']} ms; LM f64 {res['lm_f64_ms']} ms")
+    b.close()
+    for s_, t_, _ in clouds:
+        s_.close()
+        t_.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +267,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--icp-pairs", type=int, default=28, help="pairs of the batched ICP ladder workload (0: skip)")
     ap.add_argument("--icp-poses", type=int, default=64)
+    ap.add_argument("--no-huber", action="store_true")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched steps (for counter collection); prints no bench line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -290,6 +376,7 @@ def main():
                 algorithmic_bytes_per_launch=nbytes, avg_launch_ms=round(avg_ms, 4))
 
     icp = icp_ladder(ex, synth, args) if args.icp_pairs > 0 else None
+    huber = huber_section(ex, synth, args) if not args.no_huber else None
 
     cpu = None
     if not args.no_cpu:
@@ -309,7 +396,7 @@ def main():
         plain_pair_verifications_per_s=round(plain, 3), batched_vs_plain=round(value / plain, 3),
         accepted=int(accepted.sum()), accepted_identical_plain_batched=same,
         rays_per_s=round(rays_per_s, 1), rays=int(rays), knn_queries_per_s=round(knn, 1),
-        lbvh_mtri_per_s=round(lbvh_mtri, 2), icp=icp, kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
+        lbvh_mtri_per_s=round(lbvh_mtri, 2), icp=icp, huber=huber, kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
     )
     print(json.dumps(line), flush=True)
     ex.close()

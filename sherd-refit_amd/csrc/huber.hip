@@ -274,8 +274,19 @@ __global__ void __launch_bounds__(HT) k_huber_lm(HuberView v, double delta, int 
           ++q;
         }
       for (int k = 0; k < 6; ++k) g[k] = -(double)red[22 + k];
-      for (int k = 0; k < 6; ++k) H[7 * k] += s_lambda * (H[7 * k] > 0.0 ? H[7 * k] : 1.0);
+      // Marquardt's damping on the diagonal, with a floor of 1e-9 of the mean diagonal so that
+      // directions the seam does not constrain (a breakline's tangent) stay damped.
+      double tr = 0.0;
+      for (int k = 0; k < 6; ++k) tr += H[7 * k];
+      for (int k = 0; k < 6; ++k) H[7 * k] += s_lambda * (fmax(H[7 * k], 0.0) + 1e-9 * tr / 6.0 + 1e-300);
       bool ok = s_E > 0.0 && solve_ldlt(H, g, x);
+      if (ok) {
+        // A trust region: at most 0.05 rad of rotation and the correspondence radius of
+        // translation per step.
+        double wn = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]), vn = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+        double f = fmin(1.0, fmin(0.05 / fmax(wn, 1e-300), pd.r / fmax(vn, 1e-300)));
+        for (int k = 0; k < 6; ++k) x[k] *= f;
+      }
       if (!ok) {
         s_done = 1;
       } else {

@@ -353,3 +353,32 @@ def icp_batch_workload(t=1.0, n_pairs=28, poses=64, nmin=2000, nmax=60000, lengt
         pairs.append(dict(source=src, target=tgt, target_n=tn, truth=truth, inits=inits))
     rungs = [(1.0 * t, 20), (0.3 * t, 20), (0.1 * t, 20), (0.03 * t, 20)]
     return dict(pairs=pairs, rungs=rungs, t=t)
+
+
+def huber_workload(t=1.0, n_pairs=16, poses=32, nmin=10000, nmax=40000, length_t=60.0, seed=30):
+    """The robust seam energy's workload: n_pairs breakline pairs with 10k-40k points per side
+    (log-uniform), each the same curve sampled twice with B's copy under a rigid pose, and `poses`
+    base poses per pair near the truth (rotations within 2 degrees, translations within 0.1 t), so
+    that correspondences at 0.3 t are meaningful. Huber delta 0.05 t."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(nmin), np.log(nmax)
+    pairs = []
+    for p in range(n_pairs):
+        ns, nt = (int(round(np.exp(rng.uniform(lo, hi)))) for _ in range(2))
+        tgt, tn = _breakline(np.random.default_rng(5000 + p), nt, length_t * t, t)
+        src_curve, _ = _breakline(np.random.default_rng(5000 + p), ns, length_t * t, t)
+        crng = np.random.default_rng(7000 + p)
+        truth = np.eye(4)
+        truth[:3, :3] = _random_rotation(crng)
+        truth[:3, 3] = crng.normal(scale=5.0 * t, size=3)
+        inv = np.linalg.inv(truth)
+        src = src_curve @ inv[:3, :3].T + inv[:3, 3]
+        T0 = np.empty((poses, 4, 4))
+        for k in range(poses):
+            d = np.eye(4)
+            d[:3, :3] = _uniform_rotation_within(rng, np.radians(2.0))
+            v = rng.normal(size=3)
+            d[:3, 3] = v / np.linalg.norm(v) * 0.1 * t * rng.uniform() ** (1.0 / 3.0)
+            T0[k] = d @ truth
+        pairs.append(dict(source=src, target=tgt, target_n=tn, truth=truth, T0=T0))
+    return dict(pairs=pairs, delta=0.05 * t, radius=0.3 * t, t=t)
