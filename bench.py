@@ -257,6 +257,68 @@ def huber_section(ex, synth, args):
     return res
 
 
+def collection_section(ex, args):
+    """The 40-fragment, three-vessel collection end to end (ICP ladder, Huber LM, verification with
+    the FP64 retest over all 780 pairs), batched against the plain schedule, and the oracle's
+    verification of a sample of the same final poses."""
+    from sherd_hip import collection as CL
+
+    t0 = time.perf_counter()
+    coll = CL.build()
+    pairs, cp, cT = CL.candidates(coll)
+    scales = CL.pair_scales(coll, pairs)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dev = CL.Device(ex, coll)
+    prep_s = time.perf_counter() - t0
+    n_tri = sum(len(f["F"]) for f in coll["fragments"])
+    log(f"collection: {len(coll['fragments'])} fragments, {n_tri} triangles, {len(pairs)} pairs, {len(cT)} "
+        f"candidates; generated in {gen_s:.1f}s, on the device in {prep_s:.1f}s")
+    CL.run_batched(dev, coll, pairs, cp, cT, scales)  # workgroup tuning and graph capture
+    b = CL.run_batched(dev, coll, pairs, cp, cT, scales)
+    p = CL.run_plain(dev, coll, pairs, cp, cT, scales)
+    same = dict(icp=bool(np.array_equal(b["reg"], p["reg"])), lm=bool(np.array_equal(b["T_lm"], p["T_lm"])),
+                scores=bool(np.array_equal(b["scores"].view(np.uint64), p["scores"].view(np.uint64))),
+                accepted=bool(np.array_equal(b["accepted"], p["accepted"])))
+    log(f"collection batched {b['seconds']:.3f}s (icp {b['icp_s']:.3f}, lm {b['lm_s']:.3f}, verify {b['verify_s']:.3f}); "
+        f"plain {p['seconds']:.3f}s; identical {same}")
+    adj = np.array([tuple(pr) in coll["adjacent"] for pr in pairs])[cp]
+    res = dict(fragments=len(coll["fragments"]), triangles=int(n_tri), pairs=len(pairs), candidates=int(len(cT)),
+               non_watertight=list(CL.OPEN_FRAGMENTS), batched_seconds=round(b["seconds"], 4),
+               batched_icp_lm_verify_s=[round(b["icp_s"], 4), round(b["lm_s"], 4), round(b["verify_s"], 4)],
+               plain_seconds=round(p["seconds"], 4),
+               plain_icp_lm_verify_s=[round(p["icp_s"], 4), round(p["lm_s"], 4), round(p["verify_s"], 4)],
+               batched_vs_plain=round(p["seconds"] / b["seconds"], 3), identical_plain_batched=same,
+               accepted=int(b["accepted"].sum()), accepted_on_adjacent_pairs=int((b["accepted"] & adj).sum()),
+               pen_unavailable=int(b["scores"][:, 16].sum()))
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(HERE, "tests"))
+        import oracle_lib as O
+
+        O.set_threads(min(os.cpu_count() or 1, 16))
+        sample = np.nonzero(adj)[0][:24]
+        need = sorted(set(i for c in sample for i in pairs[cp[c]]))
+        keep, side = [], {}
+        for i in need:
+            fr = coll["fragments"][i]
+            fs, ms = O.Scene(fr["V"], fr["F"], fr["frac"]), O.Scene(fr["V"], fr["F"])
+            keep += [fs, ms]
+            side[i] = O.SideData(fs, ms, fr["watertight"], fr["frac_area"], fr["Pf"], fr["S"], fr["brk_P"],
+                                 fr["brk_ns"], fr["margin_p"], fr["margin_n"])
+        t0 = time.perf_counter()
+        ok = True
+        for c in sample:
+            a_, b_ = pairs[cp[c]]
+            want = O.verify(side[a_], side[b_], b["T_lm"][c], scales[cp[c]], retest=True)
+            ok &= bool(np.array_equal(want.view(np.uint64), b["scores"][c].view(np.uint64)))
+        el = time.perf_counter() - t0
+        res["cpu_verify_per_s"] = round(len(sample) / el, 2)
+        res["cpu_verify_identical_on_sample"] = ok
+        res["cpu_sample"] = f"oracle verify (FP64 retest) of {len(sample)} adjacent-pair candidates at the batched final poses"
+    dev.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,6 +330,8 @@ def main():
     ap.add_argument("--icp-pairs", type=int, default=28, help="pairs of the batched ICP ladder workload (0: skip)")
     ap.add_argument("--icp-poses", type=int, default=64)
     ap.add_argument("--no-huber", action="store_true")
+    ap.add_argument("--no-collection", action="store_true",
+                    help="skip the 40-fragment collection end to end (about 20 s of generation)")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched steps (for counter collection); prints no bench line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -377,6 +441,7 @@ def main():
 
     icp = icp_ladder(ex, synth, args) if args.icp_pairs > 0 else None
     huber = huber_section(ex, synth, args) if not args.no_huber else None
+    coll = collection_section(ex, args) if not args.no_collection else None
 
     cpu = None
     if not args.no_cpu:
@@ -396,7 +461,7 @@ def main():
         plain_pair_verifications_per_s=round(plain, 3), batched_vs_plain=round(value / plain, 3),
         accepted=int(accepted.sum()), accepted_identical_plain_batched=same,
         rays_per_s=round(rays_per_s, 1), rays=int(rays), knn_queries_per_s=round(knn, 1),
-        lbvh_mtri_per_s=round(lbvh_mtri, 2), icp=icp, huber=huber, kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
+        lbvh_mtri_per_s=round(lbvh_mtri, 2), icp=icp, huber=huber, collection=coll, kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
     )
     print(json.dumps(line), flush=True)
     ex.close()
