@@ -394,7 +394,7 @@ def main():
         return
 
     # ---- plain: one candidate per call ------------------------------------------------------------
-    n_plain = min(args.plain_candidates, n_cand)
+    n_plain = max(1, min(args.plain_candidates, n_cand))
     t0 = time.perf_counter()
     plain_acc = np.empty(n_plain, bool)
     plain_scores = np.empty((n_plain, 20))
