@@ -153,3 +153,35 @@ def test_known_answers_through_the_device(ex):
     got, _ = ex.verify_batch([(a, holed, sc1)], [0], np.eye(4)[None])
     assert (got[0][k("pen")], got[0][k("pen_unavailable")]) == (0.0, 1.0)
     assert (got[0][k("cont")], got[0][k("cont_n")]) == (1.0, -1.0)
+
+
+def test_small_vessel_all_pairs_bit_exact(ex):
+    """Six fragments of a small vessel, all fifteen pairs, the true relative poses and perturbations
+    up to 90 degrees and 10 t: adjacent pairs touch, others stand apart, so the closest approach runs
+    through every path (the per-lane search, the whole-wave queue for long queries)."""
+    from sherd_hip import synth
+
+    say("case: small vessel, 6 fragments, 15 pairs x 10 candidates, bit-exact vs oracle")
+    col = synth.vessel(t=1.0, edges_per_t=1.5, n_frag=6, seed=3, height_t=60.0, rmin_t=14.0, rmax_t=18.0)
+    frags = col["fragments"]
+    scales = synth.scales_for_pair(col["t"], col["res"])
+    pairs, cp, cT = synth.candidates(col, per_pair=10, seed=5)
+    dev, ora, keep = [], [], []
+    for fr in frags:
+        fs, ms = ex.scene(fr["V"], fr["F"], fr["frac"]), ex.scene(fr["V"], fr["F"])
+        ofs, oms = O.Scene(fr["V"], fr["F"], fr["frac"]), O.Scene(fr["V"], fr["F"])
+        keep += [fs, ms, ofs, oms]
+        args = (fr["watertight"], fr["frac_area"], fr["Pf"], fr["S"], fr["brk_P"], fr["brk_ns"], fr["margin_p"],
+                fr["margin_n"])
+        dev.append(ex.fragment(fs, ms, *args))
+        ora.append(O.SideData(ofs, oms, *args))
+    for retest in (False, True):
+        scores, accepted = ex.verify_batch([(dev[a], dev[b], scales) for a, b in pairs], cp, cT, retest=retest)
+        for i in range(len(cT)):
+            a, b = pairs[cp[i]]
+            want = O.verify(ora[a], ora[b], cT[i], scales, retest=retest)
+            assert np.array_equal(scores[i].view(np.uint64), want.view(np.uint64)), (i, retest)
+            assert bool(accepted[i]) == O.accept(want, scales)
+    far = scores[:, 15] < -5.0
+    say(f"  closest approach beyond 5 t on {int(far.sum())} of {len(cT)} candidates; accepted {int(accepted.sum())}")
+    assert far.any() and accepted.any()
