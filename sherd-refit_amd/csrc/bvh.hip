@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "bvh_query.hpp"
@@ -81,14 +82,19 @@ __global__ void k_bounds_partial(const Tri *tris, const float *cent, int m, floa
   if (t < 13) part[blockIdx.x * 13 + t] = sh[t][0];
 }
 
+// One wave per bound (13 workgroups of 64): lanes reduce the block partials in stride, then a
+// shuffle tree. min and max are order-free, so the result is exact.
 __global__ void k_bounds_final(const float *part, int blocks, float *out) {
-  int k = threadIdx.x;
-  if (k >= 13) return;
+  int k = blockIdx.x, lane = threadIdx.x;
   bool mx = (k >= 3 && k < 6) || (k >= 9);
   float v = mx ? -__builtin_inff() : __builtin_inff();
   if (k == 12) v = 0.0f;
-  for (int b = 0; b < blocks; ++b) v = mx ? fmaxf(v, part[b * 13 + k]) : fminf(v, part[b * 13 + k]);
-  out[k] = v;
+  for (int b = lane; b < blocks; b += 64) v = mx ? fmaxf(v, part[b * 13 + k]) : fminf(v, part[b * 13 + k]);
+  for (int o = 32; o > 0; o >>= 1) {
+    float w = __shfl_xor(v, o);
+    v = mx ? fmaxf(v, w) : fminf(v, w);
+  }
+  if (lane == 0) out[k] = v;
 }
 
 __device__ __forceinline__ uint32_t spread10(uint32_t x) {
@@ -116,9 +122,10 @@ __global__ void k_morton(const float *cent, int m, const float *bounds, uint32_t
   val[i] = (uint32_t)i;
 }
 
-__global__ void k_leaves(const Tri *in, const uint32_t *order, int m, Tri *out, Node *nodes, int *other) {
+__global__ void k_leaves(const Tri *in, const uint32_t *order, int m, Tri *out, Node *nodes, int *other, int *root) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
+  if (j == 0) *root = m == 1 ? 0 : -1;  // a single leaf is the root; otherwise the climb finds it
   Tri t = in[order[j]];
   out[j] = t;
   Node nd;
@@ -194,9 +201,11 @@ __global__ void k_ropes(int m, Node *nodes, const int *childb, const int *range_
 // ancestors and by every node of the subtrees wholly left of it: L leaves and L minus (the number
 // of ancestors it lies right of) internal nodes. That totals 2L + (the number of ancestors whose
 // left subtree holds it), counted here by walking up the parent links.
-__global__ void k_preorder(int m, const Node *nodes, const int *parent_of, const int *range_l, int root, int *pre) {
+__global__ void k_preorder(int m, const Node *nodes, const int *parent_of, const int *range_l, const int *root_at,
+                           int *pre) {
   int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= 2 * m - 1) return;
+  int root = *root_at;
   int L = x >= m - 1 ? x - (m - 1) : range_l[x];
   int lt = 0, c = x;
   for (int steps = 0; c != root && c >= 0 && steps < 2 * m; ++steps) {
@@ -238,64 +247,63 @@ hipError_t scene_build(const float *dV, const uint32_t *dF, int m, DevScene *out
   out->tris = nullptr;
   out->n = m;
   if (m <= 0) return hipErrorInvalidValue;
-  float *cent = nullptr, *part = nullptr, *bounds = nullptr;
-  Tri *raw = nullptr;
-  uint32_t *code = nullptr, *val = nullptr, *tk = nullptr, *tv = nullptr, *hist = nullptr;
-  int *other = nullptr, *childb = nullptr, *range_l = nullptr, *range_r = nullptr, *root = nullptr;
-  int *parent_of = nullptr, *pre = nullptr;
-  Node *tree = nullptr;  // nodes in build order, before the pre-order relayout
-  hipError_t err = hipSuccess;
   int pblocks = blocks_for(m, BT);
   if (pblocks > 1024) pblocks = 1024;
+  const size_t nn = 2 * (size_t)m - 1, mm = (size_t)m;
+  // One workspace for every temporary, carved in 256-byte aligned pieces; the bounds and the root
+  // sit together so that one copy brings both back.
+  size_t at = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = at;
+    at = (at + bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  size_t o_tree = take(sizeof(Node) * nn), o_parent = take(sizeof(int) * nn), o_pre = take(sizeof(int) * nn);
+  size_t o_rl = take(sizeof(int) * mm), o_rr = take(sizeof(int) * mm), o_other = take(sizeof(int) * mm);
+  size_t o_childb = take(sizeof(int) * mm), o_raw = take(sizeof(Tri) * mm), o_cent = take(sizeof(float) * 3 * mm);
+  size_t o_part = take(sizeof(float) * 13 * pblocks), o_code = take(sizeof(uint32_t) * mm);
+  size_t o_val = take(sizeof(uint32_t) * mm), o_tk = take(sizeof(uint32_t) * mm), o_tv = take(sizeof(uint32_t) * mm);
+  size_t o_hist = take(sizeof(uint32_t) * rsort::hist_words(m)), o_res = take(sizeof(float) * 16);
+  char *ws = nullptr;
+  hipError_t err = hipSuccess;
   do {
-    if ((err = dalloc(&out->nodes, 2 * (size_t)m - 1)) != hipSuccess) break;
-    if ((err = dalloc(&tree, 2 * (size_t)m - 1)) != hipSuccess) break;
-    if ((err = dalloc(&parent_of, 2 * (size_t)m - 1)) != hipSuccess) break;
-    if ((err = dalloc(&pre, 2 * (size_t)m - 1)) != hipSuccess) break;
-    if ((err = dalloc(&range_l, m)) != hipSuccess) break;
-    if ((err = dalloc(&out->tris, m)) != hipSuccess) break;
-    if ((err = dalloc(&raw, m)) != hipSuccess) break;
-    if ((err = dalloc(&cent, 3 * (size_t)m)) != hipSuccess) break;
-    if ((err = dalloc(&part, 13 * (size_t)pblocks)) != hipSuccess) break;
-    if ((err = dalloc(&bounds, 13)) != hipSuccess) break;
-    if ((err = dalloc(&code, m)) != hipSuccess) break;
-    if ((err = dalloc(&val, m)) != hipSuccess) break;
-    if ((err = dalloc(&tk, m)) != hipSuccess) break;
-    if ((err = dalloc(&tv, m)) != hipSuccess) break;
-    if ((err = dalloc(&hist, rsort::hist_words(m))) != hipSuccess) break;
-    if ((err = dalloc(&other, m)) != hipSuccess) break;
-    if ((err = dalloc(&childb, m)) != hipSuccess) break;
-    if ((err = dalloc(&range_r, m)) != hipSuccess) break;
-    if ((err = dalloc(&root, 1)) != hipSuccess) break;
+    if ((err = dalloc(&out->nodes, nn)) != hipSuccess) break;
+    if ((err = dalloc(&out->tris, mm)) != hipSuccess) break;
+    if ((err = hipMalloc((void **)&ws, at)) != hipSuccess) break;
+    Node *tree = (Node *)(ws + o_tree);
+    int *parent_of = (int *)(ws + o_parent), *pre = (int *)(ws + o_pre), *range_l = (int *)(ws + o_rl);
+    int *range_r = (int *)(ws + o_rr), *other = (int *)(ws + o_other), *childb = (int *)(ws + o_childb);
+    Tri *raw = (Tri *)(ws + o_raw);
+    float *cent = (float *)(ws + o_cent), *part = (float *)(ws + o_part);
+    uint32_t *code = (uint32_t *)(ws + o_code), *val = (uint32_t *)(ws + o_val), *tk = (uint32_t *)(ws + o_tk);
+    uint32_t *tv = (uint32_t *)(ws + o_tv), *hist = (uint32_t *)(ws + o_hist);
+    float *bounds = (float *)(ws + o_res);  // 13 bounds, then the root at [13]
+    int *root = (int *)(bounds + 13);
     hipLaunchKernelGGL(k_tri_setup, dim3(blocks_for(m, BT)), dim3(BT), 0, s, dV, dF, m, raw, cent);
     hipLaunchKernelGGL(k_bounds_partial, dim3(pblocks), dim3(BT), 0, s, raw, cent, m, part);
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, s, part, pblocks, bounds);
+    hipLaunchKernelGGL(k_bounds_final, dim3(13), dim3(64), 0, s, part, pblocks, bounds);
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(m, BT)), dim3(BT), 0, s, cent, m, bounds, code, val);
     if ((err = hipGetLastError()) != hipSuccess) break;
     if ((err = rsort::sort_pairs<uint32_t>(code, val, tk, tv, hist, m, 0, 30, s)) != hipSuccess) break;
-    hipLaunchKernelGGL(k_leaves, dim3(blocks_for(m, BT)), dim3(BT), 0, s, raw, val, m, out->tris, tree, other);
-    int zero = m == 1 ? 0 : -1;
-    if ((err = hipMemcpyAsync(root, &zero, sizeof(int), hipMemcpyHostToDevice, s)) != hipSuccess) break;
+    hipLaunchKernelGGL(k_leaves, dim3(blocks_for(m, BT)), dim3(BT), 0, s, raw, val, m, out->tris, tree, other, root);
     if (m > 1) {
       hipLaunchKernelGGL(k_hierarchy, dim3(blocks_for(m, BT)), dim3(BT), 0, s, code, m, tree, other, childb,
                          range_l, range_r, parent_of, root);
     }
     hipLaunchKernelGGL(k_ropes, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, tree, childb, range_r);
+    hipLaunchKernelGGL(k_preorder, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, tree, parent_of, range_l,
+                       root, pre);
+    hipLaunchKernelGGL(k_relayout, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, tree, pre, out->nodes);
     if ((err = hipGetLastError()) != hipSuccess) break;
-    float hb[13];
+    float hb[14];
     if ((err = hipMemcpyAsync(hb, bounds, sizeof hb, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
-    int r = -1;
-    if ((err = hipMemcpyAsync(&r, root, sizeof r, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
     if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
+    int r;
+    std::memcpy(&r, &hb[13], sizeof r);
     if (r < 0 || r >= 2 * m - 1) {
       err = hipErrorUnknown;
       break;
     }
-    hipLaunchKernelGGL(k_preorder, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, tree, parent_of, range_l, r,
-                       pre);
-    hipLaunchKernelGGL(k_relayout, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, tree, pre, out->nodes);
-    if ((err = hipGetLastError()) != hipSuccess) break;
-    if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
     out->root = 0;
     for (int k = 0; k < 3; ++k) {
       out->lo[k] = hb[6 + k];
@@ -303,23 +311,7 @@ hipError_t scene_build(const float *dV, const uint32_t *dF, int m, DevScene *out
     }
     out->absmax = hb[12];
   } while (0);
-  (void)hipFree(tree);
-  (void)hipFree(parent_of);
-  (void)hipFree(pre);
-  (void)hipFree(range_l);
-  (void)hipFree(raw);
-  (void)hipFree(cent);
-  (void)hipFree(part);
-  (void)hipFree(bounds);
-  (void)hipFree(code);
-  (void)hipFree(val);
-  (void)hipFree(tk);
-  (void)hipFree(tv);
-  (void)hipFree(hist);
-  (void)hipFree(other);
-  (void)hipFree(childb);
-  (void)hipFree(range_r);
-  (void)hipFree(root);
+  if (ws) (void)hipFree(ws);
   if (err != hipSuccess) scene_free(out);
   return err;
 }
