@@ -5,9 +5,10 @@
 //   1. one thread per face: the triangle record, its centroid; two-level bound reductions;
 //   2. 30-bit Morton codes of the centroids in the centroid box;
 //   3. a stable LSD radix sort (sort.hpp), so equal codes keep face order;
-//   4. leaves in sorted order, then the hierarchy bottom-up in one pass (agglomerative LBVH: each
+//   4. leaves in sorted order, then the topology bottom-up in one pass (agglomerative LBVH: each
 //      node joins the neighbour with the longer common key prefix, the second child to arrive
-//      unions the boxes and climbs), min/max unions being order-free, so the tree is deterministic;
+//      climbs on), then every internal node's box as the union of its leaf range from a doubling
+//      table; min/max unions being order-free, the tree is deterministic;
 //   5. escape links: the node after subtree [l, r] in pre-order is the right child of split r;
 //   6. the nodes moved to pre-order (a node's left child is the next node).
 // Parity grids (scene_parity_grids), built on first use by an inside query: per ray axis, a uniform
@@ -147,6 +148,10 @@ __device__ __forceinline__ uint64_t delta(const uint32_t *code, int i) {
   return a ^ b;
 }
 
+// The agglomerative climb builds the topology only: each leaf's thread climbs while it is the
+// second child to reach a node. The atomic exchange on `other` hands the first child's range end
+// to the second, which is all the second needs, so no other memory is shared during the climb
+// (links and ranges are read by later kernels). The boxes come afterwards from range unions.
 __global__ void k_hierarchy(const uint32_t *code, int m, Node *nodes, int *other, int *childb, int *range_l,
                             int *range_r, int *parent_of, int *root) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -156,30 +161,14 @@ __global__ void k_hierarchy(const uint32_t *code, int m, Node *nodes, int *other
   while (true) {
     bool left = (l == 0) || (r != m - 1 && delta(code, r) < delta(code, l - 1));
     int parent = left ? r : l - 1;
-    if (left) {
-      __hip_atomic_store(&nodes[parent].a, cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      __hip_atomic_store(&childb[parent], cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (left)
+      nodes[parent].a = cur;
+    else
+      childb[parent] = cur;
     parent_of[cur] = parent;
-    int prev = __hip_atomic_exchange(&other[parent], left ? l : r, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    int prev = __hip_atomic_exchange(&other[parent], left ? l : r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (prev < 0) return;
     if (left) r = prev; else l = prev;
-    int ca = __hip_atomic_load(&nodes[parent].a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    int cb = __hip_atomic_load(&childb[parent], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    float lo[3], hi[3];
-    for (int k = 0; k < 3; ++k) {
-      float la = __hip_atomic_load(&nodes[ca].lo[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      float lb = __hip_atomic_load(&nodes[cb].lo[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      float ha = __hip_atomic_load(&nodes[ca].hi[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      float hb = __hip_atomic_load(&nodes[cb].hi[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      lo[k] = fminf(la, lb);
-      hi[k] = fmaxf(ha, hb);
-    }
-    for (int k = 0; k < 3; ++k) {
-      __hip_atomic_store(&nodes[parent].lo[k], lo[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&nodes[parent].hi[k], hi[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
     range_l[parent] = l;
     range_r[parent] = r;
     cur = parent;
@@ -187,6 +176,46 @@ __global__ void k_hierarchy(const uint32_t *code, int m, Node *nodes, int *other
       *root = parent;
       return;
     }
+  }
+}
+
+// Boxes of leaf ranges by doubling (a sparse table): level k holds the union of leaves
+// [i, i + 2^k). Level 0 is the leaves' own boxes.
+__global__ void k_st_leaves(const Node *nodes, int m, float *st0) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const Node &nd = nodes[m - 1 + i];
+  for (int k = 0; k < 3; ++k) {
+    st0[6 * (size_t)i + k] = nd.lo[k];
+    st0[6 * (size_t)i + 3 + k] = nd.hi[k];
+  }
+}
+
+__global__ void k_st_level(const float *prev, int m, int half, float *next) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 2 * half > m) return;
+  const float *a = prev + 6 * (size_t)i, *b = prev + 6 * (size_t)(i + half);
+  float *o = next + 6 * (size_t)i;
+  for (int k = 0; k < 3; ++k) {
+    o[k] = fminf(a[k], b[k]);
+    o[3 + k] = fmaxf(a[3 + k], b[3 + k]);
+  }
+}
+
+// An internal node's box: the union of its leaves [L, R], two overlapping table entries (min and
+// max are idempotent, so the overlap changes nothing).
+__global__ void k_node_boxes(int m, const int *range_l, const int *range_r, const float *st, Node *nodes) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= m - 1) return;
+  int L = range_l[x], R = range_r[x];
+  int len = R - L + 1;
+  int k = 31 - __clz(len);
+  const float *lv = st + 6 * (size_t)m * k;
+  const float *a = lv + 6 * (size_t)L, *b = lv + 6 * (size_t)(R - (1 << k) + 1);
+  Node &nd = nodes[x];
+  for (int q = 0; q < 3; ++q) {
+    nd.lo[q] = fminf(a[q], b[q]);
+    nd.hi[q] = fmaxf(a[3 + q], b[3 + q]);
   }
 }
 
@@ -264,6 +293,9 @@ hipError_t scene_build(const float *dV, const uint32_t *dF, int m, DevScene *out
   size_t o_part = take(sizeof(float) * 13 * pblocks), o_code = take(sizeof(uint32_t) * mm);
   size_t o_val = take(sizeof(uint32_t) * mm), o_tk = take(sizeof(uint32_t) * mm), o_tv = take(sizeof(uint32_t) * mm);
   size_t o_hist = take(sizeof(uint32_t) * rsort::hist_words(m)), o_res = take(sizeof(float) * 16);
+  int levels = 1;
+  while ((1 << levels) <= m) ++levels;  // table levels 0 .. levels - 1
+  size_t o_st = take(sizeof(float) * 6 * mm * (size_t)levels);
   char *ws = nullptr;
   hipError_t err = hipSuccess;
   do {
@@ -290,6 +322,13 @@ hipError_t scene_build(const float *dV, const uint32_t *dF, int m, DevScene *out
       hipLaunchKernelGGL(k_hierarchy, dim3(blocks_for(m, BT)), dim3(BT), 0, s, code, m, tree, other, childb,
                          range_l, range_r, parent_of, root);
     }
+    float *st = (float *)(ws + o_st);
+    hipLaunchKernelGGL(k_st_leaves, dim3(blocks_for(m, BT)), dim3(BT), 0, s, tree, m, st);
+    for (int k = 1; k < levels; ++k)
+      hipLaunchKernelGGL(k_st_level, dim3(blocks_for(m, BT)), dim3(BT), 0, s, st + 6 * mm * (size_t)(k - 1), m,
+                         1 << (k - 1), st + 6 * mm * (size_t)k);
+    if (m > 1)
+      hipLaunchKernelGGL(k_node_boxes, dim3(blocks_for(m - 1, BT)), dim3(BT), 0, s, m, range_l, range_r, st, tree);
     hipLaunchKernelGGL(k_ropes, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, tree, childb, range_r);
     hipLaunchKernelGGL(k_preorder, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, tree, parent_of, range_l,
                        root, pre);
