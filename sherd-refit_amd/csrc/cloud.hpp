@@ -11,8 +11,12 @@ constexpr uint32_t NONE = 0xFFFFFFFFu;
 // an LBVH over its points (each leaf a point, `face` its index); boxes are f32 and grown by the
 // slack, which exceeds the rounding of the f32 boxes and of q, so no box hides a point nearer than
 // the best so far. Distances are kiddo's f64 ((dx^2 + dy^2) + dz^2) on the f64 points.
+//
+// `seed` (a point index, or NONE) starts the search from that point's distance when it is inside
+// the radius: the answer is the same (every point at most as far is still examined, ties
+// included), the walk shorter when the seed is close, as last iteration's partner usually is.
 __device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, double q0, double q1, double q2,
-                                     double r2, uint32_t *out_j, double *out_d2) {
+                                     double r2, uint32_t *out_j, double *out_d2, uint32_t seed = NONE) {
   int nodes = 2 * sv.n - 1;
   int cur = sv.root;
   if (cur < 0 || cur >= nodes || !(r2 > 0.0)) return false;
@@ -20,14 +24,24 @@ __device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, dou
   float slack = slack_of(sv.absmax, qf);
   double best = r2;
   uint32_t bj = NONE;
+  if (seed != NONE && seed < (uint32_t)sv.n) {
+    double d2 = d2_f64(q0, q1, q2, pts[3 * (size_t)seed], pts[3 * (size_t)seed + 1], pts[3 * (size_t)seed + 2]);
+    if (d2 < best) {
+      best = d2;
+      bj = seed;
+    }
+  }
   float lim = (float)sqrt(best) + slack;
   float lim2 = lim * lim;
-  if (box_d2(load_node(sv.nodes, cur), qf, slack) > lim2) return false;
+  if (box_d2(load_node(sv.nodes, cur), qf, slack) > lim2) {
+    if (bj == NONE) return false;
+    cur = -1;  // (cannot happen with a seed: the root's box holds it) keep the seed
+  }
   int st_n[DSTACK];
   float st_d[DSTACK];
   int sp = 0;
   bool overflow = false;
-  for (int iter = 0; iter <= nodes; ++iter) {
+  for (int iter = 0; cur >= 0 && iter <= nodes; ++iter) {
     Node nd = load_node(sv.nodes, cur);
     int next = -1;
     if (nd.a < 0) {

@@ -117,7 +117,8 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
           moved[3 * (size_t)i + 2] = p2;
           uint32_t j = NONE;
           double d2;
-          if (radius_ok && cloud_nearest(pd.tree, pd.tp, p0, p1, p2, r2, &j, &d2)) {
+          uint32_t seed = it > 0 ? corr[i] : NONE;  // last iteration's partner (this candidate's own)
+          if (radius_ok && cloud_nearest(pd.tree, pd.tp, p0, p1, p2, r2, &j, &d2, seed)) {
             acc[0] += 1.0;
             acc[1] += d2;
             const double *q = pd.tp + 3 * (size_t)j;
