@@ -187,8 +187,66 @@ int sherd_icp_batch_rung(sherd_ctx *ctx, const sherd_icp_pair *pairs, size_t n_p
                          sherd_registration *out);
 /* Fixes the ICP workgroup size (64, 128, 256 or 512 threads) or, with 0, leaves it to the
  * autotuner, which times each size on the first rung of every source-size bucket and keeps the
- * fastest. No counterpart in the reference (its wgpu kernels have fixed tiles). */
+ * fastest. 1 selects the wide form for every rung: each pass a grid over all spans of all
+ * candidates (one thread per source point), which 0 also picks by itself for a rung of fewer
+ * candidates than half the compute units on clouds of 2048 points or more. Every choice gives the
+ * same registrations to the bit. No counterpart in the reference (its wgpu kernels have fixed
+ * tiles). */
 int sherd_icp_block(sherd_ctx *ctx, int threads);
+
+/* ---- R §9 full-resolution refinement (crates/sherd-core/src/refine.rs) ------------------------- */
+
+/* refine.rs:216-227 fracture_cloud: the cloud of an original mesh (V f64 nv x 3, F u32 nf x 3) whose
+ * vertices have a fracture face of the working mesh as nearest face centroid (centroids f64
+ * n_working_faces x 3, `face_geometry`'s (a + b + c) / 3, geometry.rs:88-93; fracture: one byte per
+ * working face) strictly closer than max(0.15 thick, 1.5 res) (select_candidates, refine.rs:163-182),
+ * ascending, then capped at max_points by a seeded draw without replacement (cap_selection,
+ * refine.rs:186-192; rng.rs:127-156); normals are Open3D's area-weighted vertex normals
+ * (vertex_normals, refine.rs:118-150). The cloud has no point tree (place it with one to use it as
+ * an ICP target). */
+int sherd_fracture_cloud(sherd_ctx *ctx, const double *V, size_t nv, const uint32_t *F, size_t nf,
+                         const double *centroids, const uint8_t *fracture, size_t n_working_faces, double thick,
+                         double res, size_t max_points, uint64_t seed, sherd_cloud **out);
+/* refine.rs:207-214 cloud_from_indices with the mesh's vertex normals: the vertices idx, in order. */
+int sherd_cloud_from_indices(sherd_ctx *ctx, const double *V, size_t nv, const uint32_t *F, size_t nf,
+                             const uint32_t *idx, size_t n, sherd_cloud **out);
+/* The number of points of a cloud, and a copy of its vertex indices (fracture clouds only; NULL to
+ * skip), points and normals (NULL to skip). */
+size_t sherd_cloud_size(const sherd_cloud *cloud);
+int sherd_cloud_read(const sherd_cloud *cloud, uint32_t *idx, double *points, double *normals);
+/* FractureCloud::placed (refine.rs:99-104): a new cloud, points through the 4x4 T (row-major) and
+ * normals through its rotation, each row the fused chain of types.rs:202-214; with_tree builds the
+ * point LBVH an ICP target needs. */
+int sherd_cloud_placed(const sherd_cloud *cloud, const double T[16], int with_tree, sherd_cloud **out);
+
+/* RefinePiece (refine.rs:237-245): the fragment's wall and resolution, and its cloud (NULL: none). */
+typedef struct {
+  double thick;
+  double res;
+  const sherd_cloud *cloud;
+} sherd_refine_piece;
+
+/* RefinedJoin (refine.rs:248-262). */
+typedef struct {
+  uint32_t fixed;
+  uint32_t moving;
+  double dist[2];
+  double rungs[2][16];
+  double fitness;
+  double rmse_t;
+} sherd_refined_join;
+
+/* refine.rs:270-303 refine_joins: per group (members flattened, sizes per group) of two or more,
+ * the walk over `used` (n_used (a, b) pairs in assembly order) from the group's first fragment;
+ * each join's moving cloud placed at its pose and the fixed cloud at its final pose, registered
+ * by two point-to-plane rungs at Scales::icp_dist(0.05) and (0.02), 40 iterations each, from the
+ * identity (refine_one, refine.rs:306-350); poses_out[moving] = rung 2 . poses[moving]. The joins
+ * of one depth of the walk run as one device ladder. joins (capacity: one per used join) receives
+ * the joins taken in walk order, *n_joins their count; a join whose either side has no cloud is
+ * skipped and leaves its pose alone. */
+int sherd_refine_joins(sherd_ctx *ctx, const sherd_refine_piece *pieces, size_t n_pieces, const double *poses,
+                       const uint32_t *members, const size_t *sizes, size_t n_groups, const uint32_t *used,
+                       size_t n_used, double *poses_out, sherd_refined_join *joins, size_t *n_joins);
 
 /* ---- the robust seam energy and its se(3) gradient (new) --------------------------------------- */
 

@@ -129,6 +129,37 @@ void oracle_nearest_below(const double *cloud, size_t n, const double *q, size_t
   }
 }
 
+void oracle_nearest_unbounded(const double *cloud, size_t n, const double *q, size_t m, int64_t *idx,
+                              double *dist) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long k = 0; k < (long)m; ++k) {
+    const double *qq = q + 3 * k;
+    double best = std::numeric_limits<double>::infinity();
+    int64_t bi = -1;
+    for (size_t i = 0; i < n; ++i) {
+      const double *p = cloud + 3 * i;
+      double ex = qq[0] - p[0], ey = qq[1] - p[1], ez = qq[2] - p[2];
+      double d = (ex * ex + ey * ey) + ez * ez;
+      if (d < best) {
+        best = d;
+        bi = (int64_t)i;
+      }
+    }
+    idx[k] = bi;
+    dist[k] = bi < 0 ? std::numeric_limits<double>::infinity() : std::sqrt(best);
+  }
+}
+
+void oracle_place(const double *p, size_t n, const double T[16], int direction, double *out) {
+  for (size_t k = 0; k < n; ++k) {
+    const double *x = p + 3 * k;
+    for (int r = 0; r < 3; ++r) {
+      double v = std::fma(T[4 * r + 2], x[2], std::fma(T[4 * r + 1], x[1], T[4 * r] * x[0]));
+      out[3 * k + r] = direction ? v : v + T[4 * r + 3];
+    }
+  }
+}
+
 // grid.rs's own brute force (spatial/grid.rs:299-315), which its HashGrid is tested against.
 void oracle_nearest_f32(const float *cloud, size_t n, const float *q, size_t m, float radius,
                         int strict, int64_t *idx, float *dist) {

@@ -67,6 +67,14 @@ void oracle_exec_inside(const oracle_scene *s, const double *pts, size_t n, cons
  * idx = -1 on a miss. */
 void oracle_nearest_below(const double *cloud, size_t n, const double *q, size_t m, double bound,
                           int64_t *idx, double *dist);
+/* PointTree::nearest_distance (crates/sherd-core/src/spatial/kdtree.rs:120-123, kiddo's
+ * nearest_one): the unbounded nearest of each query by brute force, squared distance
+ * ((dx*dx + dy*dy) + dz*dz), ties to the lowest index; dist = sqrt of it. n must be > 0. */
+void oracle_nearest_unbounded(const double *cloud, size_t n, const double *q, size_t m, int64_t *idx,
+                              double *dist);
+/* apply_transform_fused / rotate_fused (crates/sherd-core/src/types.rs:202-214): each row
+ * fma(t2, p2, fma(t1, p1, t0 * p0)) (+ t3 for points, not for directions). */
+void oracle_place(const double *p, size_t n, const double T[16], int direction, double *out);
 /* The f32 mirror of HashGrid::nearest_below / nearest_within (spatial/grid.rs:600-650),
  * brute force: strict or inclusive d^2 against r*r in f32, ties to the lowest index. */
 void oracle_nearest_f32(const float *cloud, size_t n, const float *q, size_t m, float radius,

@@ -32,17 +32,12 @@ using namespace sherd;
 namespace {
 
 // Scratch slots of this file (the context's other entry points use slots below 21).
-enum { SL_BLOB = 21, SL_CAND = 22, SL_T = 23, SL_OUT = 24, SL_MOVED = 25, SL_CORR = 26, SL_PART = 27 };
+enum { SL_BLOB = 21, SL_CAND = 22, SL_T = 23, SL_OUT = 24, SL_MOVED = 25, SL_CORR = 26, SL_PART = 27, SL_SKIP = 28,
+       SL_WIDE = 29 };
 
 constexpr size_t MAX_POINTS = (size_t)ICP_SPAN * ICP_MAX_SPANS;
 constexpr size_t MAX_RUNGS = 8;
 
-void free_cloud(sherd_cloud *cl) {
-  if (cl->pts) (void)hipFree(cl->pts);
-  if (cl->normals) (void)hipFree(cl->normals);
-  scene_free(&cl->tree);
-  delete cl;
-}
 
 __global__ void k_point_tris(const double *p, int n, float *V, uint32_t *F) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -52,6 +47,36 @@ __global__ void k_point_tris(const double *p, int n, float *V, uint32_t *F) {
     F[3 * (size_t)i + k] = (uint32_t)i;
   }
 }
+
+}  // namespace
+
+void sherd::free_cloud(sherd_cloud *cl) {
+  if (cl->pts) (void)hipFree(cl->pts);
+  if (cl->normals) (void)hipFree(cl->normals);
+  scene_free(&cl->tree);
+  delete cl;
+}
+
+hipError_t sherd::point_tree(Ctx *c, sherd_cloud *cl) {
+  if (cl->n <= 0) return hipSuccess;
+  size_t n = (size_t)cl->n;
+  float *V = nullptr;
+  uint32_t *F = nullptr;
+  hipError_t e;
+  do {
+    if ((e = hipMalloc((void **)&V, sizeof(float) * 3 * n)) != hipSuccess) break;
+    if ((e = hipMalloc((void **)&F, sizeof(uint32_t) * 3 * n)) != hipSuccess) break;
+    hipLaunchKernelGGL(k_point_tris, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, cl->pts, (int)n, V,
+                       F);
+    if ((e = hipGetLastError()) != hipSuccess) break;
+    e = scene_build(V, F, (int)n, &cl->tree, c->stream);
+  } while (0);
+  if (V) (void)hipFree(V);
+  if (F) (void)hipFree(F);
+  return e;
+}
+
+namespace {
 
 // Builds a cloud on the device: the f64 points (and normals), and an LBVH whose leaves are the
 // points (degenerate triangles whose face index is the point's).
@@ -67,8 +92,6 @@ int make_cloud(Ctx *c, const double *pts, const double *normals, size_t n, sherd
     cl->centroid[k] = n ? m / (double)n : 0.0;
   }
   hipError_t e = hipSuccess;
-  float *V = nullptr;
-  uint32_t *F = nullptr;
   if (n > 0) {
     do {
       if ((e = hipMalloc((void **)&cl->pts, sizeof(double) * 3 * n)) != hipSuccess) break;
@@ -81,16 +104,9 @@ int make_cloud(Ctx *c, const double *pts, const double *normals, size_t n, sherd
             hipSuccess)
           break;
       }
-      if ((e = hipMalloc((void **)&V, sizeof(float) * 3 * n)) != hipSuccess) break;
-      if ((e = hipMalloc((void **)&F, sizeof(uint32_t) * 3 * n)) != hipSuccess) break;
-      hipLaunchKernelGGL(k_point_tris, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, cl->pts, (int)n, V,
-                         F);
-      if ((e = hipGetLastError()) != hipSuccess) break;
-      e = scene_build(V, F, (int)n, &cl->tree, c->stream);
+      e = point_tree(c, cl);
     } while (0);
   }
-  if (V) (void)hipFree(V);
-  if (F) (void)hipFree(F);
   if (e != hipSuccess) {
     free_cloud(cl);
     return hip_fail(e, "cloud build");
@@ -98,6 +114,43 @@ int make_cloud(Ctx *c, const double *pts, const double *normals, size_t n, sherd
   *out = cl;
   return SHERD_OK;
 }
+
+}  // namespace
+
+int sherd::cloud_adopt(Ctx *c, double *dpts, double *dnormals, size_t n, bool tree, sherd_cloud **out) {
+  *out = nullptr;
+  sherd_cloud *cl = new sherd_cloud();
+  cl->ctx = c;
+  cl->n = (int)n;
+  cl->pts = dpts;
+  cl->normals = dnormals;
+  if (n > MAX_POINTS) {
+    free_cloud(cl);
+    return fail(SHERD_E_LIMIT, "cloud beyond 262144 points");
+  }
+  std::vector<double> h(3 * n);
+  hipError_t e = hipSuccess;
+  if (n > 0) {
+    e = hipMemcpyAsync(h.data(), dpts, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  if (e == hipSuccess) {
+    for (int k = 0; k < 3; ++k) {
+      double m = 0.0;
+      for (size_t i = 0; i < n; ++i) m += h[3 * i + k];
+      cl->centroid[k] = n ? m / (double)n : 0.0;
+    }
+    if (tree) e = point_tree(c, cl);
+  }
+  if (e != hipSuccess) {
+    free_cloud(cl);
+    return hip_fail(e, "cloud build");
+  }
+  *out = cl;
+  return SHERD_OK;
+}
+
+namespace {
 
 void unmoved(const double *init, sherd_registration *r) {
   // Registration::unmoved (icp.rs:198-209).
@@ -128,7 +181,18 @@ struct RungPlan {
   int est = 0;
   int max_iter = 0;
   double mean_ns = 0.0;
+  // The wide form: per span its candidate, per candidate its first span.
+  bool wide = false;
+  std::vector<uint32_t> span_cand, cand_span;
 };
+
+// The form of a rung: fixed by sherd_icp_block (1 = wide, 64..512 = batched), else wide when the
+// candidates are too few to give every compute unit a workgroup and their clouds are large.
+bool wide_form(const Ctx *c, const RungPlan &pl) {
+  if (c->icp_block == 1) return true;
+  if (c->icp_block > 1) return false;
+  return pl.order.size() * 2 <= (size_t)std::max(1, c->cus) && pl.mean_ns >= 2048.0;
+}
 
 int plan_rung(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const uint32_t *cand_pair, size_t n_cand,
               int estimation, int max_iteration, RungPlan *pl) {
@@ -137,6 +201,7 @@ int plan_rung(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const uint32_
     const sherd_cloud *s = pairs[p].source, *t = pairs[p].target;
     if (!s || !t) return fail(SHERD_E_ARG, "pair without a cloud");
     if (s->ctx != c || t->ctx != c) return fail(SHERD_E_ARG, "cloud of another context");
+    if (t->n > 0 && !t->tree.nodes) return fail(SHERD_E_ARG, "target cloud without a point tree");
     IcpPairDesc &d = pl->pd[p];
     std::memset(&d, 0, sizeof d);
     d.src = s->pts;
@@ -171,6 +236,15 @@ int plan_rung(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const uint32_
   // Longest source first, so the persistent workgroups finish together.
   std::stable_sort(pl->order.begin(), pl->order.end(),
                    [&](uint32_t a, uint32_t b) { return pl->pd[cand_pair[a]].ns > pl->pd[cand_pair[b]].ns; });
+  pl->wide = wide_form(c, *pl);
+  pl->span_cand.clear();
+  pl->cand_span.assign(n_cand, 0u);
+  if (pl->wide)
+    for (uint32_t i : pl->order) {
+      pl->cand_span[i] = (uint32_t)pl->span_cand.size();
+      int nspan = (pl->pd[cand_pair[i]].ns + ICP_SPAN - 1) / ICP_SPAN;
+      for (int k = 0; k < nspan; ++k) pl->span_cand.push_back(i);
+    }
   return SHERD_OK;
 }
 
@@ -196,6 +270,7 @@ int grid_for(Ctx *c, int block, size_t n, int max_spans) {
 // correspondences and span partials.
 struct BatchDev {
   std::vector<IcpBatchView> v;
+  std::vector<IcpWideView> w;  // the wide view of each rung (used by the wide ones)
 };
 
 int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand_pair, size_t n_cand, double *dT,
@@ -210,25 +285,41 @@ int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand
     // The smallest workgroup the autotuner may try has the largest grid.
     max_grid = std::max(max_grid, grid_for(c, 64, pl.order.size(), pl.max_spans));
   }
+  size_t wide_spans = 0;
+  bool any_wide = false;
+  for (const RungPlan &pl : plans) {
+    wide_spans = std::max(wide_spans, pl.span_cand.size());
+    any_wide = any_wide || pl.wide;
+  }
   auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
-  std::vector<size_t> at(nr * 4);
+  constexpr int F = 6;  // blob fields per rung
+  std::vector<size_t> at(nr * F);
   size_t bytes = 0;
   for (size_t r = 0; r < nr; ++r) {
-    at[4 * r] = bytes;
+    at[F * r] = bytes;
     bytes = al(bytes + sizeof(IcpPairDesc) * std::max<size_t>(plans[r].pd.size(), 1));
-    at[4 * r + 1] = bytes;
+    at[F * r + 1] = bytes;
     bytes = al(bytes + sizeof(uint32_t) * std::max<size_t>(plans[r].order.size(), 1));
-    at[4 * r + 2] = bytes;
+    at[F * r + 2] = bytes;
     bytes = al(bytes + sizeof(size_t) * n_cand);
-    at[4 * r + 3] = bytes;
+    at[F * r + 3] = bytes;
     bytes = al(bytes + sizeof(uint32_t));
+    at[F * r + 4] = bytes;
+    bytes = al(bytes + sizeof(uint32_t) * std::max<size_t>(plans[r].span_cand.size(), 1));
+    at[F * r + 5] = bytes;
+    bytes = al(bytes + sizeof(uint32_t) * (plans[r].wide ? n_cand : 1));
   }
   std::vector<char> blob(bytes, 0);
   for (size_t r = 0; r < nr; ++r) {
     const RungPlan &pl = plans[r];
-    std::memcpy(&blob[at[4 * r]], pl.pd.data(), sizeof(IcpPairDesc) * pl.pd.size());
-    if (!pl.order.empty()) std::memcpy(&blob[at[4 * r + 1]], pl.order.data(), sizeof(uint32_t) * pl.order.size());
-    std::memcpy(&blob[at[4 * r + 2]], pl.off.data(), sizeof(size_t) * n_cand);
+    std::memcpy(&blob[at[F * r]], pl.pd.data(), sizeof(IcpPairDesc) * pl.pd.size());
+    if (!pl.order.empty()) std::memcpy(&blob[at[F * r + 1]], pl.order.data(), sizeof(uint32_t) * pl.order.size());
+    std::memcpy(&blob[at[F * r + 2]], pl.off.data(), sizeof(size_t) * n_cand);
+    if (pl.wide) {
+      if (!pl.span_cand.empty())
+        std::memcpy(&blob[at[F * r + 4]], pl.span_cand.data(), sizeof(uint32_t) * pl.span_cand.size());
+      std::memcpy(&blob[at[F * r + 5]], pl.cand_span.data(), sizeof(uint32_t) * n_cand);
+    }
   }
   char *dblob = (char *)c->scratch(SL_BLOB, bytes, &e);
   if (!dblob) return hip_fail(e, "scratch");
@@ -238,18 +329,25 @@ int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand
   if (!dmoved) return hip_fail(e, "scratch moved points");
   uint32_t *dcorr = (uint32_t *)c->scratch(SL_CORR, sizeof(uint32_t) * max_points, &e);
   if (!dcorr) return hip_fail(e, "scratch correspondences");
-  double *dpart = (double *)c->scratch(SL_PART, sizeof(double) * ICP_NV * (size_t)max_spans * max_grid, &e);
+  size_t part_rows = std::max((size_t)max_spans * max_grid, wide_spans);
+  double *dpart = (double *)c->scratch(SL_PART, sizeof(double) * ICP_NV * part_rows, &e);
   if (!dpart) return hip_fail(e, "scratch");
+  IcpWideState *dst = nullptr;
+  if (any_wide) {
+    dst = (IcpWideState *)c->scratch(SL_WIDE, sizeof(IcpWideState) * n_cand, &e);
+    if (!dst) return hip_fail(e, "scratch");
+  }
   HIPCK(hipMemcpyAsync(dblob, blob.data(), bytes, hipMemcpyHostToDevice, c->stream), "upload");
   HIPCK(hipMemcpyAsync(dcp, cand_pair, sizeof(uint32_t) * n_cand, hipMemcpyHostToDevice, c->stream), "upload");
   bd->v.assign(nr, IcpBatchView{});
+  bd->w.assign(nr, IcpWideView{});
   for (size_t r = 0; r < nr; ++r) {
     IcpBatchView &v = bd->v[r];
-    v.pairs = (const IcpPairDesc *)(dblob + at[4 * r]);
+    v.pairs = (const IcpPairDesc *)(dblob + at[F * r]);
     v.cand_pair = dcp;
-    v.order = (const uint32_t *)(dblob + at[4 * r + 1]);
-    v.off = (const size_t *)(dblob + at[4 * r + 2]);
-    v.next = (uint32_t *)(dblob + at[4 * r + 3]);
+    v.order = (const uint32_t *)(dblob + at[F * r + 1]);
+    v.off = (const size_t *)(dblob + at[F * r + 2]);
+    v.next = (uint32_t *)(dblob + at[F * r + 3]);
     v.T_in = dT + 16 * n_cand * r;
     v.T_out = dT + 16 * n_cand * (r + 1);
     v.out = dout + 21 * n_cand * r;
@@ -260,8 +358,20 @@ int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand
     v.n_cand = (int)plans[r].order.size();
     v.est = plans[r].est;
     v.max_iter = plans[r].max_iter;
+    IcpWideView &w = bd->w[r];
+    w.b = v;
+    w.span_cand = (const uint32_t *)(dblob + at[F * r + 4]);
+    w.cand_span = (const uint32_t *)(dblob + at[F * r + 5]);
+    w.n_spans = (int)plans[r].span_cand.size();
+    w.st = dst;
+    w.part = dpart;
   }
   return SHERD_OK;
+}
+
+hipError_t launch_rung(Ctx *c, const BatchDev &bd, size_t r, const RungPlan &pl, int block, int grid) {
+  if (pl.wide) return launch_icp_wide(bd.w[r], c->stream, c->prof.on ? &c->prof : nullptr);
+  return launch_icp_batch(bd.v[r], block, grid, c->stream);
 }
 
 // Candidates the kernel skips (an empty cloud) carry their pose into the next rung.
@@ -277,6 +387,10 @@ __global__ void k_carry(const double *T_in, double *T_out, const uint8_t *skip, 
 int run_rung(Ctx *c, const BatchDev &bd, size_t r, const RungPlan &pl) {
   const IcpBatchView &v = bd.v[r];
   if (v.n_cand == 0) return SHERD_OK;
+  if (pl.wide) {
+    HIPCK(launch_rung(c, bd, r, pl, 0, 0), "icp kernels");
+    return SHERD_OK;
+  }
   int &tuned = tuned_block(c, pl);
   int block = c->icp_block ? c->icp_block : tuned;
   if (block) {
@@ -332,7 +446,7 @@ int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_
   double *dout = (double *)c->scratch(SL_OUT, sizeof(double) * 21 * n_cand * n_rungs, &e);
   if (!dout) return hip_fail(e, "scratch");
   // Per rung, which candidates the kernel skips (one byte each).
-  uint8_t *dskip = (uint8_t *)c->scratch(28, n_cand * n_rungs, &e);
+  uint8_t *dskip = (uint8_t *)c->scratch(SL_SKIP, n_cand * n_rungs, &e);
   if (!dskip) return hip_fail(e, "scratch");
   std::vector<uint8_t> skip(n_cand * n_rungs, 0);
   bool any_skip = false;
@@ -349,6 +463,7 @@ int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_
   bool tuned = true;
   std::vector<int> blocks(n_rungs, 0), grids(n_rungs, 0);
   for (size_t r = 0; r < n_rungs; ++r) {
+    if (plans[r].wide) continue;
     blocks[r] = c->icp_block ? c->icp_block : tuned_block(c, plans[r]);
     if (!blocks[r] && bd.v[r].n_cand > 0) tuned = false;
     if (blocks[r]) grids[r] = grid_for(c, blocks[r], plans[r].order.size(), plans[r].max_spans);
@@ -367,6 +482,8 @@ int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_
     std::string sig;
     for (size_t r = 0; r < n_rungs; ++r) {
       sig.append((const char *)&bd.v[r], sizeof(IcpBatchView));
+      sig.append((const char *)&bd.w[r], sizeof(IcpWideView));
+      sig.append((const char *)&plans[r].wide, sizeof(bool));
       sig.append((const char *)&blocks[r], sizeof(int));
       sig.append((const char *)&grids[r], sizeof(int));
     }
@@ -381,7 +498,7 @@ int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_
       HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "graph capture");
       hipError_t le = hipSuccess;
       for (size_t r = 0; r < n_rungs && le == hipSuccess; ++r) {
-        if (bd.v[r].n_cand > 0) le = launch_icp_batch(bd.v[r], blocks[r], grids[r], c->stream);
+        if (bd.v[r].n_cand > 0) le = launch_rung(c, bd, r, plans[r], blocks[r], grids[r]);
         if (le == hipSuccess && any_skip) {
           hipLaunchKernelGGL(k_carry, dim3(cb), dim3(256), 0, c->stream, bd.v[r].T_in, bd.v[r].T_out,
                              dskip + r * n_cand, (int)n_cand);
@@ -445,8 +562,8 @@ void sherd_cloud_free(sherd_cloud *cloud) {
 int sherd_icp_block(sherd_ctx *ctx, int threads) {
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   LOCK(c);
-  if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512)
-    return fail(SHERD_E_ARG, "workgroup size must be 0, 64, 128, 256 or 512");
+  if (threads != 0 && threads != 1 && threads != 64 && threads != 128 && threads != 256 && threads != 512)
+    return fail(SHERD_E_ARG, "workgroup size must be 0, 1 (wide), 64, 128, 256 or 512");
   c->icp_block = threads;
   return SHERD_OK;
 }

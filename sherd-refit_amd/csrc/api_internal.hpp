@@ -5,6 +5,7 @@
 
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/sherd_hip.h"
 #include "kernels.hpp"
@@ -64,8 +65,19 @@ struct sherd_cloud {
   double *normals = nullptr;
   int n = 0;
   double centroid[3] = {0, 0, 0};
-  sherd::DevScene tree;  // point LBVH (each leaf one point)
+  sherd::DevScene tree;         // point LBVH (each leaf one point); absent on a source-only cloud
+  std::vector<uint32_t> idx;    // a fracture cloud's vertex indices, in summation order
 };
+
+namespace sherd {
+// A cloud over device arrays the cloud takes over (freed with it): the centroid from a copy of the
+// points (summed in index order, as sherd_cloud_create does), and the point LBVH when `tree`.
+// Synchronises the context's stream. On failure the arrays are freed too.
+int cloud_adopt(Ctx *c, double *dpts, double *dnormals, size_t n, bool tree, sherd_cloud **out);
+// The LBVH over a cloud's device points (each leaf a point); synchronises the stream.
+hipError_t point_tree(Ctx *c, sherd_cloud *cl);
+void free_cloud(sherd_cloud *cl);
+}  // namespace sherd
 
 struct sherd_fragment {
   sherd::Ctx *ctx = nullptr;

@@ -60,6 +60,129 @@ __device__ __forceinline__ void spans_total(const double *part, int nspan, doubl
     for (int k = 0; k < N; ++k) out[k] = v[k];
 }
 
+// What the next update needs from one corresponding pair (moved source point p, target q with
+// normal n): the count and squared distance, then the point-to-plane row products (icp.rs:599-640)
+// or the two point sums of point-to-point (icp.rs:557-575).
+__device__ __forceinline__ void icp_accumulate(double *acc, double p0, double p1, double p2, double d2,
+                                               const double *q, const double *n, bool plane, bool p2p) {
+  acc[0] += 1.0;
+  acc[1] += d2;
+  if (plane) {
+    double row[6] = {p1 * n[2] - p2 * n[1], p2 * n[0] - p0 * n[2], p0 * n[1] - p1 * n[0], n[0], n[1], n[2]};
+    double r = (p0 - q[0]) * n[0] + (p1 - q[1]) * n[1] + (p2 - q[2]) * n[2];
+    int w = 2;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+#pragma unroll
+      for (int l = 0; l <= k; ++l) acc[w++] += row[k] * row[l];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc[23 + k] += row[k] * r;
+  } else if (p2p) {
+    acc[2] += p0;
+    acc[3] += p1;
+    acc[4] += p2;
+    acc[5] += q[0];
+    acc[6] += q[1];
+    acc[7] += q[2];
+  }
+}
+
+// One iteration's bookkeeping from a correspondence pass's totals `red` (icp.rs:329-352): fitness
+// and rmse, the convergence test against the previous pass, the stop decision and, for
+// point-to-plane, the update (icp.rs:454-483: JTJ x = -JTr, Euler ZYX, identity on an unsolvable
+// system). Point-to-point leaves U to the covariance pass; an update without normals is the
+// identity.
+struct IcpIter {
+  double fit, rmse;
+  int stop, iters, conv;
+};
+
+__device__ __forceinline__ void icp_iteration(const double *red, int ns, int it, int max_iter, bool plane, bool p2p,
+                                              IcpIter *s, double *U) {
+  double cnt = red[0];
+  double fit = cnt > 0.0 ? cnt / (double)ns : 0.0;
+  double rmse = cnt > 0.0 ? sqrt(red[1] / cnt) : 0.0;
+  int conv = 0;
+  if (it > 0) conv = (fabs(fit - s->fit) < 1e-6 && fabs(rmse - s->rmse) < 1e-6) ? 1 : 0;
+  s->fit = fit;
+  s->rmse = rmse;
+  s->conv = conv;
+  s->iters = it;
+  s->stop = (conv || it >= max_iter) ? 1 : 0;
+  if (!s->stop && plane) {
+    double W[16];
+    eye4(W);
+    if (cnt > 0.0) {
+      double a[36], rhs[6], x[6];
+      int w = 2;
+      for (int k = 0; k < 6; ++k)
+        for (int l = 0; l <= k; ++l) {
+          a[6 * k + l] = red[w];
+          a[6 * l + k] = red[w];
+          ++w;
+        }
+      for (int k = 0; k < 6; ++k) rhs[k] = -red[23 + k];
+      if (solve_ldlt(a, rhs, x)) {
+        double rot[9];
+        euler_zyx(x[0], x[1], x[2], rot);
+        for (int r = 0; r < 3; ++r) {
+          for (int q = 0; q < 3; ++q) W[4 * r + q] = rot[3 * r + q];
+          W[4 * r + 3] = x[3 + r];
+        }
+      }
+    }
+    for (int k = 0; k < 16; ++k) U[k] = W[k];
+  } else if (!s->stop && !p2p) {
+    eye4(U);
+  }
+}
+
+// Point-to-point's means of the pairs (icp.rs:557-560), from a pass's totals.
+struct P2pMeans {
+  double inv, mp0, mp1, mp2, mq0, mq1, mq2;
+};
+__device__ __forceinline__ P2pMeans p2p_means(const double *red) {
+  double cnt = red[0];
+  double one_over_n = cnt > 0.0 ? 1.0 / cnt : 0.0;
+  return P2pMeans{one_over_n,        red[2] * one_over_n, red[3] * one_over_n, red[4] * one_over_n,
+                  red[5] * one_over_n, red[6] * one_over_n, red[7] * one_over_n};
+}
+
+// One span's covariance terms about the means (icp.rs:561-575), lane l adding points l, l + 64, ...
+// in turn; after the wave's tree lane 0 holds the span's nine sums.
+__device__ __forceinline__ void p2p_cov_span(const double *moved, const uint32_t *corr, const double *tp, int ns,
+                                             int sp, const P2pMeans &m, double *sg) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) sg[k] = 0.0;
+  for (int i = sp * ICP_SPAN + lane; i < ns && i < (sp + 1) * ICP_SPAN; i += 64) {
+    uint32_t j = corr[i];
+    if (j == NONE) continue;
+    const double *p = moved + 3 * (size_t)i;
+    const double *q = tp + 3 * (size_t)j;
+    double a0 = p[0] - m.mp0, a1 = p[1] - m.mp1, a2 = p[2] - m.mp2;
+    double b0 = (q[0] - m.mq0) * m.inv, b1 = (q[1] - m.mq1) * m.inv, b2 = (q[2] - m.mq2) * m.inv;
+    sg[0] += b0 * a0; sg[1] += b0 * a1; sg[2] += b0 * a2;
+    sg[3] += b1 * a0; sg[4] += b1 * a1; sg[5] += b1 * a2;
+    sg[6] += b2 * a0; sg[7] += b2 * a1; sg[8] += b2 * a2;
+  }
+  wave_sum<9>(sg);
+}
+
+__device__ __forceinline__ void write_result(const IcpBatchView &v, int c, const double *T, const IcpIter &s,
+                                             double count) {
+  double *o = v.out + 21 * (size_t)c;
+  for (int k = 0; k < 16; ++k) o[k] = T[k];
+  o[16] = s.fit;
+  o[17] = s.rmse;
+  o[18] = count;
+  o[19] = (double)s.iters;
+  o[20] = (double)s.conv;
+  if (v.T_out)
+    for (int k = 0; k < 16; ++k) v.T_out[16 * (size_t)c + k] = T[k];
+}
+
 template <int B>
 __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
   constexpr int W = B / 64;
@@ -67,8 +190,8 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
   double *part = v.part + (size_t)blockIdx.x * v.part_spans * NV;
   __shared__ double sT[16], sU[16];
   __shared__ double red[NV];
-  __shared__ double s_fit, s_rmse;
-  __shared__ int s_stop, s_iters, s_conv, s_c;
+  __shared__ IcpIter s_it;
+  __shared__ int s_c;
   for (;;) {
     if (t == 0) {
       uint32_t k = atomicAdd(v.next, 1u);
@@ -90,13 +213,7 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
       sT[t] = v.T_in[16 * (size_t)c + t];
       sU[t] = sT[t];
     }
-    if (t == 0) {
-      s_fit = 0.0;
-      s_rmse = 0.0;
-      s_stop = 0;
-      s_iters = 0;
-      s_conv = 0;
-    }
+    if (t == 0) s_it = IcpIter{0.0, 0.0, 0, 0, 0};
     __syncthreads();
     for (int it = 0;; ++it) {
       double M[12];
@@ -119,29 +236,8 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
           double d2;
           uint32_t seed = it > 0 ? corr[i] : NONE;  // last iteration's partner (this candidate's own)
           if (radius_ok && cloud_nearest(pd.tree, pd.tp, p0, p1, p2, r2, &j, &d2, seed)) {
-            acc[0] += 1.0;
-            acc[1] += d2;
-            const double *q = pd.tp + 3 * (size_t)j;
-            if (plane) {
-              const double *n = pd.tn + 3 * (size_t)j;
-              double row[6] = {p1 * n[2] - p2 * n[1], p2 * n[0] - p0 * n[2], p0 * n[1] - p1 * n[0], n[0], n[1], n[2]};
-              double r = (p0 - q[0]) * n[0] + (p1 - q[1]) * n[1] + (p2 - q[2]) * n[2];
-              int w = 2;
-#pragma unroll
-              for (int k = 0; k < 6; ++k) {
-#pragma unroll
-                for (int l = 0; l <= k; ++l) acc[w++] += row[k] * row[l];
-              }
-#pragma unroll
-              for (int k = 0; k < 6; ++k) acc[23 + k] += row[k] * r;
-            } else if (p2p) {
-              acc[2] += p0;
-              acc[3] += p1;
-              acc[4] += p2;
-              acc[5] += q[0];
-              acc[6] += q[1];
-              acc[7] += q[2];
-            }
+            icp_accumulate(acc, p0, p1, p2, d2, pd.tp + 3 * (size_t)j, plane ? pd.tn + 3 * (size_t)j : nullptr,
+                           plane, p2p);
           } else {
             j = NONE;
           }
@@ -155,70 +251,17 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
       __syncthreads();
       if (wave == 0) spans_total<NV>(part, nspan, red);
       __syncthreads();
-      if (t == 0) {
-        double cnt = red[0];
-        double fit = cnt > 0.0 ? cnt / (double)ns : 0.0;
-        double rmse = cnt > 0.0 ? sqrt(red[1] / cnt) : 0.0;
-        int conv = 0;
-        if (it > 0) conv = (fabs(fit - s_fit) < 1e-6 && fabs(rmse - s_rmse) < 1e-6) ? 1 : 0;
-        s_fit = fit;
-        s_rmse = rmse;
-        s_conv = conv;
-        s_iters = it;
-        s_stop = (conv || it >= v.max_iter) ? 1 : 0;
-        if (!s_stop && plane) {
-          // icp.rs:454-483: solve JTJ x = -JTr, Euler update, identity on an unsolvable system.
-          double U[16];
-          eye4(U);
-          if (cnt > 0.0) {
-            double a[36], rhs[6], x[6];
-            int w = 2;
-            for (int k = 0; k < 6; ++k)
-              for (int l = 0; l <= k; ++l) {
-                a[6 * k + l] = red[w];
-                a[6 * l + k] = red[w];
-                ++w;
-              }
-            for (int k = 0; k < 6; ++k) rhs[k] = -red[23 + k];
-            if (solve_ldlt(a, rhs, x)) {
-              double rot[9];
-              euler_zyx(x[0], x[1], x[2], rot);
-              for (int r = 0; r < 3; ++r) {
-                for (int q = 0; q < 3; ++q) U[4 * r + q] = rot[3 * r + q];
-                U[4 * r + 3] = x[3 + r];
-              }
-            }
-          }
-          for (int k = 0; k < 16; ++k) sU[k] = U[k];
-        } else if (!s_stop && !p2p) {
-          eye4(sU);
-        }
-      }
+      if (t == 0) icp_iteration(red, ns, it, v.max_iter, plane, p2p, &s_it, sU);
       __syncthreads();
-      if (s_stop) break;
+      if (s_it.stop) break;
       if (p2p) {
         // Point-to-point: the covariance of the pairs about their means (icp.rs:557-575).
-        double cnt = red[0];
-        double one_over_n = cnt > 0.0 ? 1.0 / cnt : 0.0;
-        double mp0 = red[2] * one_over_n, mp1 = red[3] * one_over_n, mp2 = red[4] * one_over_n;
-        double mq0 = red[5] * one_over_n, mq1 = red[6] * one_over_n, mq2 = red[7] * one_over_n;
+        const P2pMeans m = p2p_means(red);
+        const double cnt = red[0];
         if (cnt > 0.0) {
           for (int sp = wave; sp < nspan; sp += W) {
             double sg[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) sg[k] = 0.0;
-            for (int i = sp * ICP_SPAN + lane; i < ns && i < (sp + 1) * ICP_SPAN; i += 64) {
-              uint32_t j = corr[i];
-              if (j == NONE) continue;
-              const double *p = moved + 3 * (size_t)i;
-              const double *q = pd.tp + 3 * (size_t)j;
-              double a0 = p[0] - mp0, a1 = p[1] - mp1, a2 = p[2] - mp2;
-              double b0 = (q[0] - mq0) * one_over_n, b1 = (q[1] - mq1) * one_over_n, b2 = (q[2] - mq2) * one_over_n;
-              sg[0] += b0 * a0; sg[1] += b0 * a1; sg[2] += b0 * a2;
-              sg[3] += b1 * a0; sg[4] += b1 * a1; sg[5] += b1 * a2;
-              sg[6] += b2 * a0; sg[7] += b2 * a1; sg[8] += b2 * a2;
-            }
-            wave_sum<9>(sg);
+            p2p_cov_span(moved, corr, pd.tp, ns, sp, m, sg);
             if (lane == 0)
 #pragma unroll
               for (int k = 0; k < 9; ++k) part[(size_t)sp * NV + k] = sg[k];
@@ -231,7 +274,7 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
           double U[16];
           eye4(U);
           if (cnt > 0.0) {
-            double mp[3] = {mp0, mp1, mp2}, mq[3] = {mq0, mq1, mq2};
+            double mp[3] = {m.mp0, m.mp1, m.mp2}, mq[3] = {m.mq0, m.mq1, m.mq2};
             umeyama_pose(red + 2, mp, mq, U);
           }
           for (int k = 0; k < 16; ++k) sU[k] = U[k];
@@ -245,22 +288,173 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
       }
       __syncthreads();
     }
-    if (t == 0) {
-      double *o = v.out + 21 * (size_t)c;
-      for (int k = 0; k < 16; ++k) o[k] = sT[k];
-      o[16] = s_fit;
-      o[17] = s_rmse;
-      o[18] = red[0];
-      o[19] = (double)s_iters;
-      o[20] = (double)s_conv;
-      if (v.T_out)
-        for (int k = 0; k < 16; ++k) v.T_out[16 * (size_t)c + k] = sT[k];
-    }
+    if (t == 0) write_result(v, c, sT, s_it, red[0]);
     __syncthreads();
   }
 }
 
 __global__ void k_icp_reset(uint32_t *next) { *next = 0; }
+
+// ---- the wide form: one candidate over many workgroups ----------------------------------------
+// For a few candidates on large clouds (R §9's refinement registers one join at a time, up to
+// 150 000 points) one workgroup per candidate leaves the device idle. The wide form runs each pass
+// as a grid over every span of every candidate, one thread per source point, and the iteration's
+// bookkeeping as a second launch; each launch ends at once for a candidate that has stopped, so the
+// host queues max_iteration + 1 passes and the device takes only the ones it needs.
+//
+// The sums are the batched form's to the bit: the points of a span are found in parallel, then the
+// span's first wave adds them lane by lane in the batched order (l, l + 64, ...), and the spans
+// meet in the same tree, so a candidate's registration does not depend on which form ran it.
+
+__global__ void k_icp_wide_init(IcpWideView w) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= w.b.n_cand) return;
+  IcpWideState &st = w.st[w.b.order[k]];
+  const double *T = w.b.T_in + 16 * (size_t)w.b.order[k];
+  for (int q = 0; q < 16; ++q) {
+    st.T[q] = T[q];
+    st.U[q] = T[q];
+  }
+  st.fit = st.rmse = 0.0;
+  st.stop = st.iters = st.conv = 0;
+}
+
+__global__ void __launch_bounds__(ICP_SPAN) k_icp_wide_corr(IcpWideView w, int it) {
+  __shared__ double s_p[3 * ICP_SPAN];
+  __shared__ double s_d2[ICP_SPAN];
+  __shared__ uint32_t s_j[ICP_SPAN];
+  const int s = blockIdx.x;
+  const uint32_t c = w.span_cand[s];
+  const IcpWideState &st = w.st[c];
+  if (st.stop) return;
+  const IcpBatchView &v = w.b;
+  const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
+  const int ns = pd.ns;
+  const int sp = s - (int)w.cand_span[c];
+  double *moved = v.moved + 3 * v.off[c];
+  uint32_t *corr = v.corr + v.off[c];
+  const bool plane = v.est == 1 && pd.tn != nullptr;
+  const bool p2p = v.est == 0;
+  const int t = threadIdx.x;
+  const int i = sp * ICP_SPAN + t;
+  if (i < ns) {
+    double M[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) M[k] = st.U[k];
+    const double *src = it == 0 ? pd.src + 3 * (size_t)i : moved + 3 * (size_t)i;
+    double s0 = src[0], s1 = src[1], s2 = src[2];
+    double p0 = M[0] * s0 + M[1] * s1 + M[2] * s2 + M[3];
+    double p1 = M[4] * s0 + M[5] * s1 + M[6] * s2 + M[7];
+    double p2 = M[8] * s0 + M[9] * s1 + M[10] * s2 + M[11];
+    moved[3 * (size_t)i] = p0;
+    moved[3 * (size_t)i + 1] = p1;
+    moved[3 * (size_t)i + 2] = p2;
+    uint32_t j = NONE;
+    double d2 = 0.0;
+    uint32_t seed = it > 0 ? corr[i] : NONE;
+    if (!(pd.r > 0.0) || !cloud_nearest(pd.tree, pd.tp, p0, p1, p2, pd.r * pd.r, &j, &d2, seed)) j = NONE;
+    corr[i] = j;
+    s_p[3 * t] = p0;
+    s_p[3 * t + 1] = p1;
+    s_p[3 * t + 2] = p2;
+    s_d2[t] = d2;
+    s_j[t] = j;
+  }
+  __syncthreads();
+  if (t >= 64) return;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (int u = t; u < ICP_SPAN && sp * ICP_SPAN + u < ns; u += 64) {
+    uint32_t j = s_j[u];
+    if (j == NONE) continue;
+    icp_accumulate(acc, s_p[3 * u], s_p[3 * u + 1], s_p[3 * u + 2], s_d2[u], pd.tp + 3 * (size_t)j,
+                   plane ? pd.tn + 3 * (size_t)j : nullptr, plane, p2p);
+  }
+  wave_sum<NV>(acc);
+  if (t == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) w.part[(size_t)s * NV + k] = acc[k];
+}
+
+// One wave per candidate: the pass's totals, the iteration's bookkeeping, and for point-to-plane
+// (or no normals) the new pose.
+__global__ void __launch_bounds__(64) k_icp_wide_step(IcpWideView w, int it) {
+  __shared__ double red[NV];
+  const uint32_t c = w.b.order[blockIdx.x];
+  IcpWideState &st = w.st[c];
+  if (st.stop) return;
+  const IcpBatchView &v = w.b;
+  const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
+  const int nspan = (pd.ns + ICP_SPAN - 1) / ICP_SPAN;
+  spans_total<NV>(w.part + (size_t)w.cand_span[c] * NV, nspan, red);
+  if (threadIdx.x != 0) return;
+  const bool plane = v.est == 1 && pd.tn != nullptr;
+  const bool p2p = v.est == 0;
+  IcpIter s{st.fit, st.rmse, st.stop, st.iters, st.conv};
+  double U[16];
+  for (int k = 0; k < 16; ++k) U[k] = st.U[k];
+  icp_iteration(red, pd.ns, it, v.max_iter, plane, p2p, &s, U);
+  st.fit = s.fit;
+  st.rmse = s.rmse;
+  st.iters = s.iters;
+  st.conv = s.conv;
+  for (int k = 0; k < NV; ++k) st.red[k] = red[k];
+  if (s.stop) {
+    write_result(v, (int)c, st.T, s, red[0]);
+    st.stop = 1;
+    return;
+  }
+  if (!p2p) {
+    double nt[16];
+    mul_nn(U, st.T, nt, 4);
+    for (int k = 0; k < 16; ++k) {
+      st.U[k] = U[k];
+      st.T[k] = nt[k];
+    }
+  }
+}
+
+// Point-to-point's covariance pass: one wave per span, the batched form's loop.
+__global__ void __launch_bounds__(64) k_icp_wide_cov(IcpWideView w) {
+  const int s = blockIdx.x;
+  const uint32_t c = w.span_cand[s];
+  const IcpWideState &st = w.st[c];
+  if (st.stop || !(st.red[0] > 0.0)) return;
+  const IcpBatchView &v = w.b;
+  const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
+  double sg[9];
+  p2p_cov_span(v.moved + 3 * v.off[c], v.corr + v.off[c], pd.tp, pd.ns, s - (int)w.cand_span[c], p2p_means(st.red),
+               sg);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) w.part[(size_t)s * NV + k] = sg[k];
+}
+
+// Point-to-point's update (Umeyama over the covariance) and the new pose, one wave per candidate.
+__global__ void __launch_bounds__(64) k_icp_wide_umeyama(IcpWideView w) {
+  __shared__ double sig[9];
+  const uint32_t c = w.b.order[blockIdx.x];
+  IcpWideState &st = w.st[c];
+  if (st.stop) return;
+  const IcpPairDesc pd = w.b.pairs[w.b.cand_pair[c]];
+  const bool any = st.red[0] > 0.0;
+  if (any) spans_total<9>(w.part + (size_t)w.cand_span[c] * NV, (pd.ns + ICP_SPAN - 1) / ICP_SPAN, sig);
+  if (threadIdx.x != 0) return;
+  double U[16];
+  eye4(U);
+  if (any) {
+    const P2pMeans m = p2p_means(st.red);
+    double mp[3] = {m.mp0, m.mp1, m.mp2}, mq[3] = {m.mq0, m.mq1, m.mq2};
+    umeyama_pose(sig, mp, mq, U);
+  }
+  double nt[16];
+  mul_nn(U, st.T, nt, 4);
+  for (int k = 0; k < 16; ++k) {
+    st.U[k] = U[k];
+    st.T[k] = nt[k];
+  }
+}
 
 }  // namespace
 
@@ -285,6 +479,23 @@ hipError_t launch_icp_batch(const IcpBatchView &v, int block, int grid, hipStrea
     default:
       hipLaunchKernelGGL(k_icp_batch<256>, dim3(grid), dim3(256), 0, s, v);
       break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_icp_wide(const IcpWideView &w, hipStream_t s, KernelProfiler *prof) {
+  const int n = w.b.n_cand;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_icp_wide_init, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, w);
+  for (int it = 0; it <= w.b.max_iter; ++it) {
+    if (prof) prof->begin("k_icp_wide_corr", s);
+    hipLaunchKernelGGL(k_icp_wide_corr, dim3((unsigned)w.n_spans), dim3(ICP_SPAN), 0, s, w, it);
+    if (prof) prof->end(s);
+    hipLaunchKernelGGL(k_icp_wide_step, dim3((unsigned)n), dim3(64), 0, s, w, it);
+    if (w.b.est == 0 && it < w.b.max_iter) {
+      hipLaunchKernelGGL(k_icp_wide_cov, dim3((unsigned)w.n_spans), dim3(64), 0, s, w);
+      hipLaunchKernelGGL(k_icp_wide_umeyama, dim3((unsigned)n), dim3(64), 0, s, w);
+    }
   }
   return hipGetLastError();
 }

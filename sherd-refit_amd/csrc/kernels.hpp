@@ -149,6 +149,41 @@ int icp_block_sizes(int *out);
 // Resets the queue and launches one persistent rung over the batch.
 hipError_t launch_icp_batch(const IcpBatchView &v, int block, int grid, hipStream_t s);
 
+// The wide form of a rung (a few candidates, large clouds): every span of every candidate is one
+// workgroup of a pass. Per candidate its registration state between the launches of a rung.
+struct IcpWideState {
+  double T[16], U[16];
+  double red[ICP_NV];
+  double fit, rmse;
+  int stop, iters, conv, pad;
+};
+struct IcpWideView {
+  IcpBatchView b;               // b.order lists the candidates that run, b.n_cand of them
+  const uint32_t *span_cand;    // per span of the rung: its candidate
+  const uint32_t *cand_span;    // per candidate: its first span
+  int n_spans;
+  IcpWideState *st;             // per candidate
+  double *part;                 // per span: ICP_NV sums
+};
+// Queues the rung's max_iteration + 1 passes; results land in b.out / b.T_out as for the batched form.
+hipError_t launch_icp_wide(const IcpWideView &w, hipStream_t s, KernelProfiler *prof);
+
+// ---- R §9 refinement (refine.hip) ---------------------------------------------------------------
+// Vertex selection of an original mesh (nv f64 vertices) against the working mesh's face centroids
+// C (point LBVH `tree`) and fracture labels: sel/pos per vertex, pos scanned exclusive.
+hipError_t launch_select(const double *V, int nv, const SceneView &tree, const double *C, const uint8_t *fracture,
+                         double radius, uint8_t *sel, uint32_t *pos, uint32_t *scan_part, hipStream_t s);
+size_t select_scan_words(int nv);
+hipError_t launch_compact(const uint8_t *sel, const uint32_t *pos, int nv, uint32_t *idx, hipStream_t s);
+// Points and vertex normals of the n vertices idx of a mesh. fn: 3 nf f64 scratch; scratch:
+// normals_scratch_words(nv, nf) u32.
+size_t normals_scratch_words(int nv, int nf);
+hipError_t launch_vertex_cloud(const double *V, int nv, const uint32_t *F, int nf, const uint32_t *idx, int n,
+                               double *fn, uint32_t *scratch, double *pts, double *nrm, hipStream_t s);
+// A cloud through a pose: points by the 4x4, normals (when both pointers are set) by its rotation.
+hipError_t launch_place(const double *p, const double *n, int cnt, const Pose &T, double *po, double *no,
+                        hipStream_t s);
+
 // ---- verification (verify.hip) -------------------------------------------------------------------
 // max_pf / max_s / max_margin: the largest per-fragment sample counts in the batch (grid sizes).
 hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_margin, int max_brk, hipStream_t s,

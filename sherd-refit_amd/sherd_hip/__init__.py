@@ -12,6 +12,7 @@ executor raises.
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -50,6 +51,19 @@ class IcpPair(C.Structure):
 
 class IcpRungSpec(C.Structure):
     _fields_ = [("estimation", C.c_int), ("max_iteration", C.c_int)]
+
+
+class RefinePiece(C.Structure):
+    """`RefinePiece` (crates/sherd-core/src/refine.rs:237-245)."""
+
+    _fields_ = [("thick", C.c_double), ("res", C.c_double), ("cloud", C.c_void_p)]
+
+
+class RefinedJoin(C.Structure):
+    """`RefinedJoin` (crates/sherd-core/src/refine.rs:248-262)."""
+
+    _fields_ = [("fixed", C.c_uint32), ("moving", C.c_uint32), ("dist", C.c_double * 2),
+                ("rungs", C.c_double * 32), ("fitness", C.c_double), ("rmse_t", C.c_double)]
 
 
 class VerifyPair(C.Structure):
@@ -106,6 +120,13 @@ def _bind(L):
         "sherd_cloud_create": (ci, [vp, P(d), P(d), sz, P(vp)]),
         "sherd_cloud_free": (None, [vp]),
         "sherd_icp_block": (ci, [vp, ci]),
+        "sherd_fracture_cloud": (ci, [vp, P(d), sz, P(u32), sz, P(d), P(u8), sz, d, d, sz, C.c_uint64, P(vp)]),
+        "sherd_cloud_from_indices": (ci, [vp, P(d), sz, P(u32), sz, P(u32), sz, P(vp)]),
+        "sherd_cloud_size": (sz, [vp]),
+        "sherd_cloud_read": (ci, [vp, P(u32), P(d), P(d)]),
+        "sherd_cloud_placed": (ci, [vp, P(d), ci, P(vp)]),
+        "sherd_refine_joins": (ci, [vp, P(RefinePiece), sz, P(d), P(u32), P(sz), sz, P(u32), sz, P(d),
+                                    P(RefinedJoin), P(sz)]),
         "sherd_huber_create": (ci, [vp, P(IcpPair), sz, P(u32), P(d), sz, P(vp)]),
         "sherd_huber_free": (None, [vp]),
         "sherd_huber_correspondences": (ci, [vp, P(C.c_int32), sz]),
@@ -163,6 +184,7 @@ class Scene:
         _check(lib().sherd_scene_build(ex.ptr, _p(V, C.c_float), len(V), _p(F, C.c_uint32), len(F),
                                        _p(m, C.c_uint8), C.byref(out)))
         self.ptr = out
+        ex._own(self)
 
     def close(self):
         if self.ptr:
@@ -186,14 +208,39 @@ class Scene:
 class Cloud:
     """A point set for ICP on the device (IcpTarget, crates/sherd-core/src/matching/icp.rs:218-300)."""
 
-    def __init__(self, ex, pts, normals=None):
+    def __init__(self, ex, pts, normals=None, _ptr=None):
         self.ex = ex
+        if _ptr is not None:
+            self.ptr = _ptr
+            self.n = int(lib().sherd_cloud_size(_ptr))
+            ex._own(self)
+            return
         p = f64(pts)
         n = None if normals is None else f64(normals)
         out = C.c_void_p()
         _check(lib().sherd_cloud_create(ex.ptr, _p(p, C.c_double), _p(n, C.c_double), len(p), C.byref(out)))
         self.ptr = out
+        ex._own(self)
         self.n = len(p)
+
+    def read(self, idx=False, normals=True):
+        """(points, normals or None, vertex indices or None) copied from the device."""
+        pts = np.empty((self.n, 3))
+        nrm = np.empty((self.n, 3)) if normals else None
+        ix = np.empty(self.n, np.uint32) if idx else None
+        _check(lib().sherd_cloud_read(self.ptr, _p(ix, C.c_uint32), _p(pts, C.c_double), _p(nrm, C.c_double)))
+        return pts, nrm, ix
+
+    @property
+    def idx(self):
+        return self.read(idx=True, normals=False)[2]
+
+    def placed(self, T, tree=True):
+        """FractureCloud::placed (refine.rs:99-104) as a new device cloud, with its point tree."""
+        T = np.ascontiguousarray(T, dtype=np.float64).reshape(16)
+        out = C.c_void_p()
+        _check(lib().sherd_cloud_placed(self.ptr, _p(T, C.c_double), int(bool(tree)), C.byref(out)))
+        return Cloud(self.ex, None, _ptr=out)
 
     def close(self):
         if self.ptr:
@@ -205,6 +252,14 @@ class Cloud:
             self.close()
         except Exception:
             pass
+
+
+def face_centroids(V, F):
+    """face_geometry's centroids (crates/sherd-core/src/mesh/geometry.rs:88-93): (a + b + c) / 3,
+    left to right, per face."""
+    V = f64(V).reshape(-1, 3)
+    F = np.asarray(F).reshape(-1, 3)
+    return (V[F[:, 0]] + V[F[:, 1]] + V[F[:, 2]]) / 3.0
 
 
 def _registrations(out, k):
@@ -238,6 +293,7 @@ class HuberBatch:
         _check(lib().sherd_huber_create(ex.ptr, arr, len(pairs), _p(self.cand_pair, C.c_uint32), _p(T0, C.c_double),
                                         self.k, C.byref(out)))
         self.ptr = out
+        ex._own(self)
 
     def correspondences(self):
         out = np.empty(self.points, np.int32)
@@ -287,6 +343,7 @@ class Fragment:
             _p(pf, C.c_double), len(pf), _p(s, C.c_double), len(s), _p(bp, C.c_double), _p(bn, C.c_double), len(bp),
             _p(mp, C.c_double), _p(mn, C.c_double), len(mp), C.byref(out)))
         self.ptr = out
+        ex._own(self)
 
     def close(self):
         if self.ptr:
@@ -307,9 +364,16 @@ class HipExecutor:
         out = C.c_void_p()
         _check(lib().sherd_open(int(device), C.byref(out)))
         self.ptr = out
+        self._handles = weakref.WeakSet()
+
+    def _own(self, handle):
+        """Handles (scenes, clouds, fragments, Huber batches) are freed before the context is."""
+        self._handles.add(handle)
 
     def close(self):
         if self.ptr:
+            for h in list(self._handles):
+                h.close()
             lib().sherd_close(self.ptr)
             self.ptr = None
 
@@ -426,8 +490,58 @@ class HipExecutor:
     def huber(self, pairs, cand_pair, T0):
         return HuberBatch(self, pairs, cand_pair, T0)
 
+    def fracture_cloud(self, V, F, centroids, fracture, thick, res, max_points=150000, seed=0):
+        """refine.rs:216-227 fracture_cloud of an original mesh against the working mesh's face
+        centroids (face_centroids) and fracture labels."""
+        V, F = f64(V).reshape(-1, 3), np.ascontiguousarray(F, dtype=np.uint32).reshape(-1, 3)
+        Cc = f64(centroids).reshape(-1, 3)
+        fr = np.ascontiguousarray(fracture, dtype=np.uint8).reshape(-1)
+        assert len(fr) == len(Cc)
+        out = C.c_void_p()
+        _check(lib().sherd_fracture_cloud(self.ptr, _p(V, C.c_double), len(V), _p(F, C.c_uint32), len(F),
+                                          _p(Cc, C.c_double), _p(fr, C.c_uint8), len(Cc), float(thick), float(res),
+                                          int(max_points), int(seed), C.byref(out)))
+        return Cloud(self, None, _ptr=out)
+
+    def cloud_from_indices(self, V, F, idx):
+        """refine.rs:207-214: the vertices idx of a mesh, with its vertex normals."""
+        V, F = f64(V).reshape(-1, 3), np.ascontiguousarray(F, dtype=np.uint32).reshape(-1, 3)
+        ix = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+        out = C.c_void_p()
+        _check(lib().sherd_cloud_from_indices(self.ptr, _p(V, C.c_double), len(V), _p(F, C.c_uint32), len(F),
+                                              _p(ix, C.c_uint32), len(ix), C.byref(out)))
+        return Cloud(self, None, _ptr=out)
+
+    def refine_joins(self, pieces, poses, groups, used):
+        """refine.rs:270-303 refine_joins. pieces: [(thick, res, Cloud or None)] per fragment;
+        poses: (n, 4, 4); groups: lists of fragment ids; used: [(a, b)] in assembly order.
+        Returns (poses (n, 4, 4), joins: [dict(fixed, moving, dist, rungs (2, 4, 4), fitness,
+        rmse_t)] in walk order)."""
+        n = len(pieces)
+        pc = (RefinePiece * max(n, 1))()
+        for i, (t, r, cl) in enumerate(pieces):
+            pc[i] = RefinePiece(float(t), float(r), None if cl is None else cl.ptr)
+        P = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 16)
+        assert len(P) == n
+        members = np.ascontiguousarray([m for g in groups for m in g], dtype=np.uint32)
+        sizes = np.ascontiguousarray([len(g) for g in groups], dtype=np.uintp)
+        us = np.ascontiguousarray(used, dtype=np.uint32).reshape(-1, 2)
+        out = np.empty((n, 16))
+        joins = (RefinedJoin * max(len(us), 1))()
+        nj = C.c_size_t(0)
+        _check(lib().sherd_refine_joins(self.ptr, pc, n, _p(P, C.c_double), _p(members, C.c_uint32),
+                                        _p(sizes, C.c_size_t), len(sizes), _p(us, C.c_uint32), len(us),
+                                        _p(out, C.c_double), joins, C.byref(nj)))
+        res = []
+        for j in joins[:nj.value]:
+            res.append(dict(fixed=int(j.fixed), moving=int(j.moving), dist=np.array(j.dist[:]),
+                            rungs=np.array(j.rungs[:]).reshape(2, 4, 4), fitness=float(j.fitness),
+                            rmse_t=float(j.rmse_t)))
+        return out.reshape(n, 4, 4), res
+
     def icp_block(self, threads):
-        """Fix the ICP workgroup size (64/128/256/512) or leave it to the autotuner (0)."""
+        """Fix the ICP workgroup size (64/128/256/512), the wide form (1), or leave the choice to
+        the autotuner (0)."""
         _check(lib().sherd_icp_block(self.ptr, int(threads)))
 
     def icp_batch_ladder(self, pairs, rungs, cand_pair, inits):

@@ -15,6 +15,11 @@ own arithmetic, spelled out below with the file:line each follows:
   (crates/sherd-parity/src/stages/pairs.rs:414-416);
 * the fracture-sample normals `FN[fp]` (crates/sherd-parity/src/stages/pairs.rs:286).
 
+R §9's row (refine.npz) is the dump's own: refine/<piece>.idx, refine/joins.json and
+refine/poses_final.json, the assembly's poses.json, groups.json and used.json it starts from, and
+each fragment's mesh.stats.json thickness and resolution (crates/sherd-parity/src/stages/refine.rs:
+139-205 reads the same files).
+
 Only numpy is used, and only `numpy.load` with its default `allow_pickle=False`.
 """
 
@@ -149,6 +154,37 @@ def main(dump):
         gt = json.load(fh)
     with open(os.path.join(OUT, "ground_truth.json"), "w") as fh:
         json.dump(gt, fh, indent=1)
+    write_refine(dump)
+
+
+def write_refine(dump):
+    """refine.npz: R §9's inputs and outputs for the slab's one join."""
+    rd, ad = os.path.join(dump, "refine"), os.path.join(dump, "assembly")
+    jl = lambda path: json.load(open(path))
+    index = {p: i for i, p in enumerate(PIECES)}
+    poses = lambda path: np.array([jl(path)[p] for p in PIECES], dtype=np.float64)
+    stats = [jl(os.path.join(dump, "fragments", p, "mesh.stats.json")) for p in PIECES]
+    joins = jl(os.path.join(rd, "joins.json"))
+    groups = jl(os.path.join(ad, "groups.json"))
+    used = jl(os.path.join(ad, "used.json"))
+    out = dict(
+        poses_before=poses(os.path.join(ad, "poses.json")),
+        poses_final=poses(os.path.join(rd, "poses_final.json")),
+        thick=np.array([s["thick"] for s in stats], dtype=np.float64),
+        res=np.array([s["res"] for s in stats], dtype=np.float64),
+        group_members=np.array([index[n] for g in groups for n in g], dtype=np.int64),
+        group_sizes=np.array([len(g) for g in groups], dtype=np.int64),
+        used=np.array([[index[u["a"]], index[u["b"]]] for u in used], dtype=np.int64).reshape(-1, 2),
+        join_fixed=np.array([index[j["fixed"]] for j in joins], dtype=np.int64),
+        join_moving=np.array([index[j["moving"]] for j in joins], dtype=np.int64),
+        join_dist=np.array([j["dist"] for j in joins], dtype=np.float64),
+        join_T=np.array([j["T_rung"] for j in joins], dtype=np.float64),
+        join_fitness=np.array([j["fitness"] for j in joins], dtype=np.float64),
+        join_rmse_t=np.array([j["rmse_t"] for j in joins], dtype=np.float64),
+    )
+    for p in PIECES:
+        out["idx_" + p] = np.load(os.path.join(rd, p + ".idx.npy")).astype(np.int64)
+    np.savez_compressed(os.path.join(OUT, "refine.npz"), **out)
 
 
 if __name__ == "__main__":

@@ -55,6 +55,8 @@ class Side(C.Structure):
 def _bind(L):
     d, f, u32, i64, u8, sz = C.c_double, C.c_float, C.c_uint32, C.c_int64, C.c_uint8, C.c_size_t
     P = C.POINTER
+    L.oracle_nearest_unbounded.argtypes = [P(d), sz, P(d), sz, P(i64), P(d)]
+    L.oracle_place.argtypes = [P(d), sz, P(d), C.c_int, P(d)]
     L.oracle_scene_new.restype = C.c_void_p
     L.oracle_scene_new.argtypes = [P(f), sz, P(u32), sz, P(u8)]
     L.oracle_scene_free.argtypes = [C.c_void_p]
@@ -182,6 +184,26 @@ def nearest_below(cloud, q, bound):
     lib().oracle_nearest_below(_p(cloud, C.c_double), len(cloud), _p(q, C.c_double), len(q), float(bound),
                                _p(idx, C.c_int64), _p(dist, C.c_double))
     return idx, dist
+
+
+def nearest_unbounded(cloud, q):
+    """kdtree.rs:120-123 nearest_distance: (index, distance) of each query's nearest point."""
+    cloud = f64(cloud).reshape(-1, 3)
+    q = f64(q).reshape(-1, 3)
+    idx = np.empty(len(q), np.int64)
+    dist = np.empty(len(q), np.float64)
+    lib().oracle_nearest_unbounded(_p(cloud, C.c_double), len(cloud), _p(q, C.c_double), len(q),
+                                   _p(idx, C.c_int64), _p(dist, C.c_double))
+    return idx, dist
+
+
+def place(pts, T, direction=False):
+    """types.rs:202-214: points (or directions) through a 4x4 with fused rows."""
+    pts = f64(pts).reshape(-1, 3)
+    T = f64(T).reshape(16)
+    out = np.empty_like(pts)
+    lib().oracle_place(_p(pts, C.c_double), len(pts), _p(T, C.c_double), int(bool(direction)), _p(out, C.c_double))
+    return out
 
 
 def nearest_f32(cloud, q, radius, strict=True):
