@@ -257,6 +257,133 @@ def huber_section(ex, synth, args):
     return res
 
 
+def _surface_join(n, seed=51):
+    """One R §9-shaped join of n points a side: a smooth height field sampled every 0.05 (wall 1),
+    the moving copy displaced by 0.05 degrees and 0.01."""
+    rng = np.random.default_rng(seed)
+    side = int(np.ceil(np.sqrt(n)))
+    h = 0.05
+    x, y = np.meshgrid(np.arange(side) * h, np.arange(side) * h, indexing="ij")
+    x, y = x.ravel()[:n], y.ravel()[:n]
+    z = 0.4 * np.sin(x / 1.7) * np.cos(y / 2.3) + 0.1 * np.sin(x * y / 9.0)
+    gx = 0.4 / 1.7 * np.cos(x / 1.7) * np.cos(y / 2.3) + 0.1 * y / 9.0 * np.cos(x * y / 9.0)
+    gy = -0.4 / 2.3 * np.sin(x / 1.7) * np.sin(y / 2.3) + 0.1 * x / 9.0 * np.cos(x * y / 9.0)
+    pts = np.stack([x, y, z], axis=1)
+    nrm = np.stack([-gx, -gy, np.ones_like(x)], axis=1)
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    a = rng.normal(size=3)
+    a /= np.linalg.norm(a)
+    ang = np.radians(0.05)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    D = np.eye(4)
+    D[:3, :3] = np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+    D[:3, 3] = rng.normal(size=3) * 0.01 / np.sqrt(3)
+    return pts, nrm, D, h
+
+
+def refine_section(ex, col, args):
+    """R §9 (refine.rs:270-350) on the vessel: each fragment's fracture cloud (its mesh serving as
+    its own original), assembly poses 0.3 degrees / 0.05 t off the truth, the walk over the
+    adjacent joins; the device as it runs by itself (the wide ICP form for these batches of one),
+    the one-workgroup-per-candidate form, and the oracle on the host. Then one join of 150 000
+    points a side, the size R §9 caps a cloud at."""
+    from sherd_hip import face_centroids
+
+    frags = col["fragments"]
+    t, res = col["t"], col["res"]
+    t0 = time.perf_counter()
+    clouds = [ex.fracture_cloud(f["V"], f["F"], face_centroids(f["V"], f["F"]), f["frac"], t, res) for f in frags]
+    el_clouds = time.perf_counter() - t0
+    rng = np.random.default_rng(50)
+    poses = []
+    for P in col["poses"]:
+        a = rng.normal(size=3)
+        a /= np.linalg.norm(a)
+        K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+        ang = np.radians(0.3)
+        D = np.eye(4)
+        D[:3, :3] = np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
+        v = rng.normal(size=3)
+        D[:3, 3] = 0.05 * t * v / np.linalg.norm(v)
+        poses.append(D @ np.linalg.inv(P))
+    poses = np.array(poses)
+    pieces = [(t, res, c) for c in clouds]
+    groups = [list(range(len(frags)))]
+    used = sorted(tuple(sorted(p)) for p in col["adjacent"])
+
+    def gap(W):
+        # Relative pose error of every adjacent pair against the truth, degrees and t.
+        worst = [0.0, 0.0]
+        for a, b in used:
+            E = (W[a] @ col["poses"][a]) @ np.linalg.inv(W[b] @ col["poses"][b])
+            c = np.clip((np.trace(E[:3, :3]) - 1.0) / 2.0, -1.0, 1.0)
+            worst[0] = max(worst[0], float(np.degrees(np.arccos(c))))
+            worst[1] = max(worst[1], float(np.abs(E[:3, 3]).max() / t))
+        return worst
+
+    ex.refine_joins(pieces, poses, groups, used)
+    t0 = time.perf_counter()
+    out, joins = ex.refine_joins(pieces, poses, groups, used)
+    el = time.perf_counter() - t0
+    ex.icp_block(512)
+    ex.refine_joins(pieces, poses, groups, used)
+    t0 = time.perf_counter()
+    out1, joins1 = ex.refine_joins(pieces, poses, groups, used)
+    el1 = time.perf_counter() - t0
+    ex.icp_block(0)
+    same = bool(np.array_equal(out, out1))
+    sizes = [c.n for c in clouds]
+    log(f"refine: {len(joins)} joins, fracture clouds {min(sizes)}-{max(sizes)} points (built in {el_clouds:.3f}s); "
+        f"device {el * 1e3:.1f} ms, one workgroup per join {el1 * 1e3:.1f} ms, identical {same}")
+    res_ = dict(joins=len(joins), cloud_points=[int(min(sizes)), int(max(sizes))],
+                fracture_clouds_seconds=round(el_clouds, 4), seconds=round(el, 4),
+                joins_per_s=round(len(joins) / el, 2), one_workgroup_seconds=round(el1, 4),
+                wide_vs_one_workgroup=round(el1 / el, 2), identical_wide_one_workgroup=same,
+                adjacent_gap_before_deg_t=gap(poses), adjacent_gap_after_deg_t=gap(out),
+                workload=f"vessel, {len(frags)} fragments, walk over {len(used)} adjacent joins, "
+                         "2 point-to-plane rungs x 40 iterations")
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import refine as R
+
+        host = [(t, res, c.read()[:2]) for c in clouds]
+        t0 = time.perf_counter()
+        oout, ojoins = R.refine_joins(host, poses, groups, used)
+        el_cpu = time.perf_counter() - t0
+        dev = [np.degrees(np.arccos(np.clip((np.trace(out[i][:3, :3] @ oout[i][:3, :3].T) - 1) / 2, -1, 1)))
+               for i in range(len(out))]
+        res_["cpu_seconds"] = round(el_cpu, 3)
+        res_["cpu_joins_per_s"] = round(len(ojoins) / el_cpu, 3)
+        res_["cpu_vs_device_pose_deg_max"] = float(max(dev))
+        res_["cpu_sample"] = "the oracle's refine_joins (icp_rung, one thread per candidate) over the same walk"
+        log(f"refine oracle: {el_cpu:.2f}s; worst pose difference to the device {max(dev):.2e} deg")
+    for c in clouds:
+        c.close()
+    # One join at R §9's cap: 150 000 points a side, a batch of one.
+    pts, nrm, D, h = _surface_join(150_000)
+    src = ex.cloud(pts, nrm)
+    big = [(1.0, h, src), (1.0, h, src)]
+    bp = np.stack([np.eye(4), D])
+    ex.refine_joins(big, bp, [[0, 1]], [(0, 1)])
+    t0 = time.perf_counter()
+    bout, bj = ex.refine_joins(big, bp, [[0, 1]], [(0, 1)])
+    el_big = time.perf_counter() - t0
+    ex.icp_block(512)
+    t0 = time.perf_counter()
+    bout1, _ = ex.refine_joins(big, bp, [[0, 1]], [(0, 1)])
+    el_big1 = time.perf_counter() - t0
+    ex.icp_block(0)
+    err = np.abs(bout[1] - np.eye(4)).max()
+    log(f"refine 150k join: device {el_big * 1e3:.1f} ms, one workgroup {el_big1 * 1e3:.1f} ms, identical "
+        f"{bool(np.array_equal(bout, bout1))}; pose back to identity within {err:.1e}")
+    res_["join_150k"] = dict(seconds=round(el_big, 4), one_workgroup_seconds=round(el_big1, 4),
+                             wide_vs_one_workgroup=round(el_big1 / el_big, 2),
+                             identical=bool(np.array_equal(bout, bout1)), fitness=bj[0]["fitness"],
+                             pose_error_max=float(err))
+    src.close()
+    return res_
+
+
 def collection_section(ex, args):
     """The 40-fragment, three-vessel collection end to end (ICP ladder, Huber LM, verification with
     the FP64 retest over all 780 pairs), batched against the plain schedule, and the oracle's
@@ -330,6 +457,7 @@ def main():
     ap.add_argument("--icp-pairs", type=int, default=28, help="pairs of the batched ICP ladder workload (0: skip)")
     ap.add_argument("--icp-poses", type=int, default=64)
     ap.add_argument("--no-huber", action="store_true")
+    ap.add_argument("--no-refine", action="store_true", help="skip R §9's full-resolution refinement")
     ap.add_argument("--no-collection", action="store_true",
                     help="skip the 40-fragment collection end to end (about 20 s of generation)")
     ap.add_argument("--batched-only", action="store_true",
@@ -442,6 +570,7 @@ def main():
     icp = icp_ladder(ex, synth, args) if args.icp_pairs > 0 else None
     huber = huber_section(ex, synth, args) if not args.no_huber else None
     coll = collection_section(ex, args) if not args.no_collection else None
+    refine = refine_section(ex, col, args) if not args.no_refine else None
 
     cpu = None
     if not args.no_cpu:
@@ -455,13 +584,13 @@ def main():
         metric=METRIC, value=round(value, 3), unit="pair-verifications/s", n_gpus=1, steps=args.steps,
         warmup=args.warmup, ms_per_step=round(1e3 * el / args.steps, 3), higher_is_better=True, scaling="weak",
         vs_baseline=None, dtype="f32+f64", data="synthetic",
-        config=dict(workload=f"vessel (configs[1]): 8 fragments, {args.edges_per_t} edges/t, "
+        config=dict(workload=f"vessel: 8 fragments, {args.edges_per_t} edges/t, "
                              f"{sum(len(f['F']) for f in frags)} triangles, 28 pairs x 10 candidates",
                     candidates_per_step=n_cand, parallelism="single GPU"),
         plain_pair_verifications_per_s=round(plain, 3), batched_vs_plain=round(value / plain, 3),
         accepted=int(accepted.sum()), accepted_identical_plain_batched=same,
         rays_per_s=round(rays_per_s, 1), rays=int(rays), knn_queries_per_s=round(knn, 1),
-        lbvh_mtri_per_s=round(lbvh_mtri, 2), icp=icp, huber=huber, collection=coll, kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
+        lbvh_mtri_per_s=round(lbvh_mtri, 2), icp=icp, huber=huber, collection=coll, refine=refine, kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
     )
     print(json.dumps(line), flush=True)
     ex.close()

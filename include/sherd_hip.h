@@ -185,13 +185,12 @@ int sherd_icp_batch_ladder(sherd_ctx *ctx, const sherd_icp_pair *pairs, size_t n
 int sherd_icp_batch_rung(sherd_ctx *ctx, const sherd_icp_pair *pairs, size_t n_pairs, const uint32_t *cand_pair,
                          const double *inits, size_t n_cand, int estimation, int max_iteration,
                          sherd_registration *out);
-/* Fixes the ICP workgroup size (64, 128, 256 or 512 threads) or, with 0, leaves it to the
- * autotuner, which times each size on the first rung of every source-size bucket and keeps the
- * fastest. 1 selects the wide form for every rung: each pass a grid over all spans of all
- * candidates (one thread per source point), which 0 also picks by itself for a rung of fewer
- * candidates than half the compute units on clouds of 2048 points or more. Every choice gives the
- * same registrations to the bit. No counterpart in the reference (its wgpu kernels have fixed
- * tiles). */
+/* Fixes the form of every ICP rung: the persistent batched kernel at 64, 128, 256 or 512 threads
+ * per workgroup (one candidate per workgroup at a time), or 1, the wide form (each pass a grid over
+ * all spans of all candidates, one thread per source point: the form for a few candidates on large
+ * clouds). With 0 the autotuner times every form on the first rung of each (source-size,
+ * candidate-count) bucket and keeps the fastest. Every form gives the same registrations to the
+ * bit. No counterpart in the reference (its wgpu kernels have fixed tiles). */
 int sherd_icp_block(sherd_ctx *ctx, int threads);
 
 /* ---- R §9 full-resolution refinement (crates/sherd-core/src/refine.rs) ------------------------- */

@@ -181,18 +181,13 @@ struct RungPlan {
   int est = 0;
   int max_iter = 0;
   double mean_ns = 0.0;
-  // The wide form: per span its candidate, per candidate its first span.
-  bool wide = false;
-  std::vector<uint32_t> span_cand, cand_span;
+  // The wide form's span table: per queue position its first span, then the total.
+  std::vector<uint32_t> span_first;
 };
 
-// The form of a rung: fixed by sherd_icp_block (1 = wide, 64..512 = batched), else wide when the
-// candidates are too few to give every compute unit a workgroup and their clouds are large.
-bool wide_form(const Ctx *c, const RungPlan &pl) {
-  if (c->icp_block == 1) return true;
-  if (c->icp_block > 1) return false;
-  return pl.order.size() * 2 <= (size_t)std::max(1, c->cus) && pl.mean_ns >= 2048.0;
-}
+// The forms a rung can run in: the persistent batched kernel at a workgroup size, or (1) the wide
+// form. All give the same registrations to the bit; the autotuner picks by speed.
+constexpr int WIDE = 1;
 
 int plan_rung(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const uint32_t *cand_pair, size_t n_cand,
               int estimation, int max_iteration, RungPlan *pl) {
@@ -236,24 +231,21 @@ int plan_rung(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const uint32_
   // Longest source first, so the persistent workgroups finish together.
   std::stable_sort(pl->order.begin(), pl->order.end(),
                    [&](uint32_t a, uint32_t b) { return pl->pd[cand_pair[a]].ns > pl->pd[cand_pair[b]].ns; });
-  pl->wide = wide_form(c, *pl);
-  pl->span_cand.clear();
-  pl->cand_span.assign(n_cand, 0u);
-  if (pl->wide)
-    for (uint32_t i : pl->order) {
-      pl->cand_span[i] = (uint32_t)pl->span_cand.size();
-      int nspan = (pl->pd[cand_pair[i]].ns + ICP_SPAN - 1) / ICP_SPAN;
-      for (int k = 0; k < nspan; ++k) pl->span_cand.push_back(i);
-    }
+  pl->span_first.assign(1, 0u);
+  for (uint32_t i : pl->order)
+    pl->span_first.push_back(pl->span_first.back() + (uint32_t)((pl->pd[cand_pair[i]].ns + ICP_SPAN - 1) / ICP_SPAN));
   return SHERD_OK;
 }
 
-int bucket_of(double mean_ns) {
-  int b = mean_ns > 1.0 ? (int)std::floor(std::log2(mean_ns)) : 0;
-  return std::min(39, std::max(0, b));
+int log2_bucket(double x, int top) {
+  int b = x > 1.0 ? (int)std::floor(std::log2(x)) : 0;
+  return std::min(top, std::max(0, b));
 }
 
-int &tuned_block(Ctx *c, const RungPlan &pl) { return c->icp_tuned[pl.est][bucket_of(pl.mean_ns)]; }
+// The autotuner's choice for a rung, by estimation, source size and candidate count.
+int &tuned_block(Ctx *c, const RungPlan &pl) {
+  return c->icp_tuned[pl.est][log2_bucket(pl.mean_ns, 39)][log2_bucket((double)pl.order.size(), 23)];
+}
 
 // Persistent workgroups for a rung: enough to fill the device, no more than candidates, and the
 // span partials within 512 MiB.
@@ -286,13 +278,9 @@ int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand
     max_grid = std::max(max_grid, grid_for(c, 64, pl.order.size(), pl.max_spans));
   }
   size_t wide_spans = 0;
-  bool any_wide = false;
-  for (const RungPlan &pl : plans) {
-    wide_spans = std::max(wide_spans, pl.span_cand.size());
-    any_wide = any_wide || pl.wide;
-  }
+  for (const RungPlan &pl : plans) wide_spans = std::max<size_t>(wide_spans, pl.span_first.back());
   auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
-  constexpr int F = 6;  // blob fields per rung
+  constexpr int F = 5;  // blob fields per rung
   std::vector<size_t> at(nr * F);
   size_t bytes = 0;
   for (size_t r = 0; r < nr; ++r) {
@@ -305,9 +293,7 @@ int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand
     at[F * r + 3] = bytes;
     bytes = al(bytes + sizeof(uint32_t));
     at[F * r + 4] = bytes;
-    bytes = al(bytes + sizeof(uint32_t) * std::max<size_t>(plans[r].span_cand.size(), 1));
-    at[F * r + 5] = bytes;
-    bytes = al(bytes + sizeof(uint32_t) * (plans[r].wide ? n_cand : 1));
+    bytes = al(bytes + sizeof(uint32_t) * plans[r].span_first.size());
   }
   std::vector<char> blob(bytes, 0);
   for (size_t r = 0; r < nr; ++r) {
@@ -315,11 +301,7 @@ int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand
     std::memcpy(&blob[at[F * r]], pl.pd.data(), sizeof(IcpPairDesc) * pl.pd.size());
     if (!pl.order.empty()) std::memcpy(&blob[at[F * r + 1]], pl.order.data(), sizeof(uint32_t) * pl.order.size());
     std::memcpy(&blob[at[F * r + 2]], pl.off.data(), sizeof(size_t) * n_cand);
-    if (pl.wide) {
-      if (!pl.span_cand.empty())
-        std::memcpy(&blob[at[F * r + 4]], pl.span_cand.data(), sizeof(uint32_t) * pl.span_cand.size());
-      std::memcpy(&blob[at[F * r + 5]], pl.cand_span.data(), sizeof(uint32_t) * n_cand);
-    }
+    std::memcpy(&blob[at[F * r + 4]], pl.span_first.data(), sizeof(uint32_t) * pl.span_first.size());
   }
   char *dblob = (char *)c->scratch(SL_BLOB, bytes, &e);
   if (!dblob) return hip_fail(e, "scratch");
@@ -332,11 +314,8 @@ int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand
   size_t part_rows = std::max((size_t)max_spans * max_grid, wide_spans);
   double *dpart = (double *)c->scratch(SL_PART, sizeof(double) * ICP_NV * part_rows, &e);
   if (!dpart) return hip_fail(e, "scratch");
-  IcpWideState *dst = nullptr;
-  if (any_wide) {
-    dst = (IcpWideState *)c->scratch(SL_WIDE, sizeof(IcpWideState) * n_cand, &e);
-    if (!dst) return hip_fail(e, "scratch");
-  }
+  IcpWideState *dst = (IcpWideState *)c->scratch(SL_WIDE, sizeof(IcpWideState) * n_cand, &e);
+  if (!dst) return hip_fail(e, "scratch");
   HIPCK(hipMemcpyAsync(dblob, blob.data(), bytes, hipMemcpyHostToDevice, c->stream), "upload");
   HIPCK(hipMemcpyAsync(dcp, cand_pair, sizeof(uint32_t) * n_cand, hipMemcpyHostToDevice, c->stream), "upload");
   bd->v.assign(nr, IcpBatchView{});
@@ -360,17 +339,16 @@ int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand
     v.max_iter = plans[r].max_iter;
     IcpWideView &w = bd->w[r];
     w.b = v;
-    w.span_cand = (const uint32_t *)(dblob + at[F * r + 4]);
-    w.cand_span = (const uint32_t *)(dblob + at[F * r + 5]);
-    w.n_spans = (int)plans[r].span_cand.size();
+    w.span_first = (const uint32_t *)(dblob + at[F * r + 4]);
+    w.n_spans = (int)plans[r].span_first.back();
     w.st = dst;
     w.part = dpart;
   }
   return SHERD_OK;
 }
 
-hipError_t launch_rung(Ctx *c, const BatchDev &bd, size_t r, const RungPlan &pl, int block, int grid) {
-  if (pl.wide) return launch_icp_wide(bd.w[r], c->stream, c->prof.on ? &c->prof : nullptr);
+hipError_t launch_rung(Ctx *c, const BatchDev &bd, size_t r, int block, int grid) {
+  if (block == WIDE) return launch_icp_wide(bd.w[r], c->stream, c->prof.on ? &c->prof : nullptr);
   return launch_icp_batch(bd.v[r], block, grid, c->stream);
 }
 
@@ -387,26 +365,23 @@ __global__ void k_carry(const double *T_in, double *T_out, const uint8_t *skip, 
 int run_rung(Ctx *c, const BatchDev &bd, size_t r, const RungPlan &pl) {
   const IcpBatchView &v = bd.v[r];
   if (v.n_cand == 0) return SHERD_OK;
-  if (pl.wide) {
-    HIPCK(launch_rung(c, bd, r, pl, 0, 0), "icp kernels");
-    return SHERD_OK;
-  }
   int &tuned = tuned_block(c, pl);
   int block = c->icp_block ? c->icp_block : tuned;
   if (block) {
-    if (c->prof.on) c->prof.begin("k_icp_batch", c->stream);
-    HIPCK(launch_icp_batch(v, block, grid_for(c, block, pl.order.size(), pl.max_spans), c->stream), "icp kernel");
-    if (c->prof.on) c->prof.end(c->stream);
+    if (c->prof.on && block != WIDE) c->prof.begin("k_icp_batch", c->stream);
+    HIPCK(launch_rung(c, bd, r, block, grid_for(c, block, pl.order.size(), pl.max_spans)), "icp kernel");
+    if (c->prof.on && block != WIDE) c->prof.end(c->stream);
     return SHERD_OK;
   }
   int sizes[8];
   int n = icp_block_sizes(sizes);
+  sizes[n++] = WIDE;
   float best = 0.0f;
   int pick = 0;
   for (int k = 0; k < n; ++k) {
     int b = sizes[k];
     HIPCK(hipEventRecord(c->ev0, c->stream), "event");
-    HIPCK(launch_icp_batch(v, b, grid_for(c, b, pl.order.size(), pl.max_spans), c->stream), "icp kernel");
+    HIPCK(launch_rung(c, bd, r, b, b == WIDE ? 0 : grid_for(c, b, pl.order.size(), pl.max_spans)), "icp kernel");
     HIPCK(hipEventRecord(c->ev1, c->stream), "event");
     HIPCK(hipEventSynchronize(c->ev1), "event");
     float ms = 0.0f;
@@ -428,8 +403,9 @@ int check_rung(const sherd_icp_rung_spec &r) {
 }
 
 // The ladder: rungs in order on the device, each from the previous rung's poses. When every rung
-// has a workgroup size, the launches are captured once as a graph and replayed while the batch's
-// launch signature (every kernel argument) is unchanged; otherwise the rungs run eagerly (and tune).
+// has a workgroup size and the batch repeats the previous call's launch signature (every kernel
+// argument), the launches are captured once as a graph and replayed while the signature holds;
+// otherwise the rungs run eagerly (and tune).
 int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_icp_rung_spec *rungs, size_t n_rungs,
                const uint32_t *cand_pair, const double *inits, size_t n_cand, sherd_registration *out) {
   int rc;
@@ -463,14 +439,29 @@ int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_
   bool tuned = true;
   std::vector<int> blocks(n_rungs, 0), grids(n_rungs, 0);
   for (size_t r = 0; r < n_rungs; ++r) {
-    if (plans[r].wide) continue;
     blocks[r] = c->icp_block ? c->icp_block : tuned_block(c, plans[r]);
     if (!blocks[r] && bd.v[r].n_cand > 0) tuned = false;
-    if (blocks[r]) grids[r] = grid_for(c, blocks[r], plans[r].order.size(), plans[r].max_spans);
+    if (blocks[r] > WIDE) grids[r] = grid_for(c, blocks[r], plans[r].order.size(), plans[r].max_spans);
   }
   int cb = (int)((16 * n_cand + 255) / 256);
   Timer tm(c, SHERD_M_ICP, n_cand * n_rungs);
-  if (!tuned || c->prof.on) {
+  // The launch signature (every kernel argument of every rung): a batch whose signature the last
+  // call also had is captured as a graph and replayed while it repeats; a one-off batch (the
+  // refinement's walk issues one per depth) runs its launches directly.
+  std::string sig;
+  if (tuned && !c->prof.on) {
+    for (size_t r = 0; r < n_rungs; ++r) {
+      sig.append((const char *)&bd.v[r], sizeof(IcpBatchView));
+      sig.append((const char *)&bd.w[r], sizeof(IcpWideView));
+      sig.append((const char *)&blocks[r], sizeof(int));
+      sig.append((const char *)&grids[r], sizeof(int));
+    }
+    sig.append((const char *)&dskip, sizeof dskip);
+    sig.append((const char *)&any_skip, sizeof any_skip);
+  }
+  bool replay = tuned && !c->prof.on && ((c->ladder_exec && sig == c->ladder_sig) || sig == c->ladder_seen);
+  c->ladder_seen = sig;
+  if (!replay) {
     for (size_t r = 0; r < n_rungs; ++r) {
       if ((rc = run_rung(c, bd, r, plans[r]))) return rc;
       if (any_skip)
@@ -479,16 +470,6 @@ int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_
     }
     HIPCK(hipGetLastError(), "carry");
   } else {
-    std::string sig;
-    for (size_t r = 0; r < n_rungs; ++r) {
-      sig.append((const char *)&bd.v[r], sizeof(IcpBatchView));
-      sig.append((const char *)&bd.w[r], sizeof(IcpWideView));
-      sig.append((const char *)&plans[r].wide, sizeof(bool));
-      sig.append((const char *)&blocks[r], sizeof(int));
-      sig.append((const char *)&grids[r], sizeof(int));
-    }
-    sig.append((const char *)&dskip, sizeof dskip);
-    sig.append((const char *)&any_skip, sizeof any_skip);
     if (!c->ladder_exec || sig != c->ladder_sig) {
       if (c->ladder_exec) {
         (void)hipGraphExecDestroy(c->ladder_exec);
@@ -498,7 +479,7 @@ int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_
       HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "graph capture");
       hipError_t le = hipSuccess;
       for (size_t r = 0; r < n_rungs && le == hipSuccess; ++r) {
-        if (bd.v[r].n_cand > 0) le = launch_rung(c, bd, r, plans[r], blocks[r], grids[r]);
+        if (bd.v[r].n_cand > 0) le = launch_rung(c, bd, r, blocks[r], grids[r]);
         if (le == hipSuccess && any_skip) {
           hipLaunchKernelGGL(k_carry, dim3(cb), dim3(256), 0, c->stream, bd.v[r].T_in, bd.v[r].T_out,
                              dskip + r * n_cand, (int)n_cand);

@@ -26,12 +26,15 @@ struct Ctx {
   };
   Buf bufs[40];
   KernelProfiler prof;
-  // Batched ICP: the workgroup size (0 = autotuned), the autotuner's choices by source-size bucket
-  // and estimation, and the captured ladder graph with the launch signature it was captured for.
+  // Batched ICP: the workgroup size (0 = autotuned, 1 = the wide form), the autotuner's choices
+  // by estimation, source-size bucket and candidate-count bucket, and the captured ladder graph
+  // with the launch signature it was captured for.
   int icp_block = 0;
-  int icp_tuned[2][40] = {};
+  int icp_tuned[2][40][24] = {};
   hipGraphExec_t ladder_exec = nullptr;
-  std::string ladder_sig;
+  std::string ladder_sig;   // the captured graph's signature
+  std::string ladder_seen;  // the last call's signature
+
   // A grow-only device buffer per slot, reused across calls (the context is serialised).
   void *scratch(int slot, size_t bytes, hipError_t *err);
 };

@@ -306,6 +306,19 @@ __global__ void k_icp_reset(uint32_t *next) { *next = 0; }
 // span's first wave adds them lane by lane in the batched order (l, l + 64, ...), and the spans
 // meet in the same tree, so a candidate's registration does not depend on which form ran it.
 
+// The queue position whose spans hold span s (span_first is ascending).
+__device__ __forceinline__ int span_owner(const IcpWideView &w, int s) {
+  int lo = 0, hi = w.b.n_cand - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if ((int)w.span_first[mid] <= s)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 __global__ void k_icp_wide_init(IcpWideView w) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= w.b.n_cand) return;
@@ -324,13 +337,14 @@ __global__ void __launch_bounds__(ICP_SPAN) k_icp_wide_corr(IcpWideView w, int i
   __shared__ double s_d2[ICP_SPAN];
   __shared__ uint32_t s_j[ICP_SPAN];
   const int s = blockIdx.x;
-  const uint32_t c = w.span_cand[s];
+  const int k = span_owner(w, s);
+  const uint32_t c = w.b.order[k];
   const IcpWideState &st = w.st[c];
   if (st.stop) return;
   const IcpBatchView &v = w.b;
   const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
   const int ns = pd.ns;
-  const int sp = s - (int)w.cand_span[c];
+  const int sp = s - (int)w.span_first[k];
   double *moved = v.moved + 3 * v.off[c];
   uint32_t *corr = v.corr + v.off[c];
   const bool plane = v.est == 1 && pd.tn != nullptr;
@@ -387,7 +401,7 @@ __global__ void __launch_bounds__(64) k_icp_wide_step(IcpWideView w, int it) {
   const IcpBatchView &v = w.b;
   const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
   const int nspan = (pd.ns + ICP_SPAN - 1) / ICP_SPAN;
-  spans_total<NV>(w.part + (size_t)w.cand_span[c] * NV, nspan, red);
+  spans_total<NV>(w.part + (size_t)w.span_first[blockIdx.x] * NV, nspan, red);
   if (threadIdx.x != 0) return;
   const bool plane = v.est == 1 && pd.tn != nullptr;
   const bool p2p = v.est == 0;
@@ -418,13 +432,14 @@ __global__ void __launch_bounds__(64) k_icp_wide_step(IcpWideView w, int it) {
 // Point-to-point's covariance pass: one wave per span, the batched form's loop.
 __global__ void __launch_bounds__(64) k_icp_wide_cov(IcpWideView w) {
   const int s = blockIdx.x;
-  const uint32_t c = w.span_cand[s];
+  const int k = span_owner(w, s);
+  const uint32_t c = w.b.order[k];
   const IcpWideState &st = w.st[c];
   if (st.stop || !(st.red[0] > 0.0)) return;
   const IcpBatchView &v = w.b;
   const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
   double sg[9];
-  p2p_cov_span(v.moved + 3 * v.off[c], v.corr + v.off[c], pd.tp, pd.ns, s - (int)w.cand_span[c], p2p_means(st.red),
+  p2p_cov_span(v.moved + 3 * v.off[c], v.corr + v.off[c], pd.tp, pd.ns, s - (int)w.span_first[k], p2p_means(st.red),
                sg);
   if (threadIdx.x == 0)
 #pragma unroll
@@ -439,7 +454,7 @@ __global__ void __launch_bounds__(64) k_icp_wide_umeyama(IcpWideView w) {
   if (st.stop) return;
   const IcpPairDesc pd = w.b.pairs[w.b.cand_pair[c]];
   const bool any = st.red[0] > 0.0;
-  if (any) spans_total<9>(w.part + (size_t)w.cand_span[c] * NV, (pd.ns + ICP_SPAN - 1) / ICP_SPAN, sig);
+  if (any) spans_total<9>(w.part + (size_t)w.span_first[blockIdx.x] * NV, (pd.ns + ICP_SPAN - 1) / ICP_SPAN, sig);
   if (threadIdx.x != 0) return;
   double U[16];
   eye4(U);

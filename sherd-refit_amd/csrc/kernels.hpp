@@ -159,8 +159,7 @@ struct IcpWideState {
 };
 struct IcpWideView {
   IcpBatchView b;               // b.order lists the candidates that run, b.n_cand of them
-  const uint32_t *span_cand;    // per span of the rung: its candidate
-  const uint32_t *cand_span;    // per candidate: its first span
+  const uint32_t *span_first;   // per position in b.order its first span, then the total (n_cand + 1)
   int n_spans;
   IcpWideState *st;             // per candidate
   double *part;                 // per span: ICP_NV sums

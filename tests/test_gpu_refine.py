@@ -63,7 +63,8 @@ def test_wide_form_equals_the_batched_form(ex, estimation):
     for mode in (256, 1, 64):
         ex.icp_block(mode)
         out[mode] = ex.icp_batch_ladder(pairs, specs_r, cand_pair, inits)
-        out[mode, "again"] = ex.icp_batch_ladder(pairs, specs_r, cand_pair, inits)  # the captured graph
+        out[mode, "captured"] = ex.icp_batch_ladder(pairs, specs_r, cand_pair, inits)  # captured on the repeat
+        out[mode, "replayed"] = ex.icp_batch_ladder(pairs, specs_r, cand_pair, inits)
     ex.icp_block(0)
     for key, v in out.items():
         assert np.array_equal(v, out[256]), key
