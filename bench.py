@@ -283,7 +283,8 @@ def _surface_join(n, seed=51):
 
 def refine_section(ex, col, args):
     """R §9 (refine.rs:270-350) on the vessel: each fragment's fracture cloud (its mesh serving as
-    its own original), assembly poses 0.3 degrees / 0.05 t off the truth, the walk over the
+    its own original), assembly poses off the truth by 0.02 degrees about the fragment's centroid
+    and 0.02 t (a pose right to a fraction of a wall, as R §8 leaves it), the walk over the
     adjacent joins; the device as it runs by itself (the wide ICP form for these batches of one),
     the one-workgroup-per-candidate form, and the oracle on the host. Then one join of 150 000
     points a side, the size R §9 caps a cloud at."""
@@ -296,16 +297,18 @@ def refine_section(ex, col, args):
     el_clouds = time.perf_counter() - t0
     rng = np.random.default_rng(50)
     poses = []
-    for P in col["poses"]:
+    for f, P in zip(frags, col["poses"]):
+        W = np.linalg.inv(P)
+        c = W[:3, :3] @ f["V"].mean(axis=0) + W[:3, 3]
         a = rng.normal(size=3)
         a /= np.linalg.norm(a)
         K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
-        ang = np.radians(0.3)
+        ang = np.radians(0.02)
         D = np.eye(4)
         D[:3, :3] = np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K
         v = rng.normal(size=3)
-        D[:3, 3] = 0.05 * t * v / np.linalg.norm(v)
-        poses.append(D @ np.linalg.inv(P))
+        D[:3, 3] = c - D[:3, :3] @ c + 0.02 * t * v / np.linalg.norm(v)
+        poses.append(D @ W)
     poses = np.array(poses)
     pieces = [(t, res, c) for c in clouds]
     groups = [list(range(len(frags)))]
