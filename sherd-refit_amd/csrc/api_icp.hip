@@ -373,8 +373,14 @@ int run_rung(Ctx *c, const BatchDev &bd, size_t r, const RungPlan &pl) {
     if (c->prof.on && block != WIDE) c->prof.end(c->stream);
     return SHERD_OK;
   }
+  // Fewer candidates than a quarter of the compute units leave the batched kernel at most one
+  // workgroup per candidate on a nearly idle device: only its largest size is worth timing there.
   int sizes[8];
   int n = icp_block_sizes(sizes);
+  if (pl.order.size() * 4 < (size_t)std::max(1, c->cus)) {
+    sizes[0] = sizes[n - 1];
+    n = 1;
+  }
   sizes[n++] = WIDE;
   float best = 0.0f;
   int pick = 0;
