@@ -193,6 +193,72 @@ int sherd_icp_batch_rung(sherd_ctx *ctx, const sherd_icp_pair *pairs, size_t n_p
  * bit. No counterpart in the reference (its wgpu kernels have fixed tiles). */
 int sherd_icp_block(sherd_ctx *ctx, int threads);
 
+/* ---- R §5.1-5.5 over many pairs: hypotheses, suppression, the stage-1 hand-off ----------------- */
+
+typedef struct sherd_frames sherd_frames;
+
+/* hypotheses::Frames (crates/sherd-core/src/matching/hypotheses.rs:47-68): one fragment's breakline
+ * frames (n points: brk_P, shell normals ns, in-plane axes f, tangents t, all f64 n x 3; dihedral
+ * angles dih in degrees) and its hypothesis subset brk_sub (n_sub indices, in the order the caller
+ * means; PMC-4), on the device with a neighbour grid over the points for the coarse scores. */
+int sherd_frames_create(sherd_ctx *ctx, const double *p, const double *ns, const double *f, const double *t,
+                        const double *dih, size_t n, const uint32_t *sub, size_t n_sub, sherd_frames **out);
+void sherd_frames_free(sherd_frames *frames);
+
+/* hypotheses::build_with (hypotheses.rs:149-165) and poses (:172-212): the frame pairs of the
+ * subsets ia (of A) and ib (of B) whose dihedral angles satisfy |dih_A + dih_B - 180| < tol,
+ * row-major over (ia, ib); pa / pb are positions into ia / ib; poses get R (row-major) then tau,
+ * 12 f64 each, in the reference's summation order. *n gets the count; fails if it exceeds
+ * capacity. */
+int sherd_hypotheses(sherd_ctx *ctx, const sherd_frames *a, const sherd_frames *b, const uint32_t *ia, size_t nia,
+                     const uint32_t *ib, size_t nib, double dihedral_tol, size_t capacity, uint32_t *pa, uint32_t *pb,
+                     double *poses, size_t *n);
+
+/* nms::nms (crates/sherd-core/src/matching/nms.rs:62-84): the greedy walk over `order` (item
+ * indices) of items with poses (R row-major then tau, 12 f64) and scores: stop at the first score
+ * under floor; drop an item nearer than trans_tol to a kept one and turned by trace(R^T R') > 2.9;
+ * stop once topk are kept (tested after each item, so topk = 0 keeps one). kept receives the kept
+ * items in the order kept (room for min(n_order, topk + 1)). */
+int sherd_nms(sherd_ctx *ctx, const uint32_t *order, size_t n_order, const double *poses, const double *score,
+              size_t n_items, double trans_tol, size_t topk, double floor, uint32_t *kept, size_t *n_kept);
+
+/* One pair of a matching batch: A fixed, B moving (Pair, crates/sherd-core/src/matching/pair.rs). */
+typedef struct {
+  const sherd_frames *a;
+  const sherd_frames *b;
+  const uint32_t *ia;        /* the subsets the hypotheses pair, or NULL for the frames' own */
+  size_t nia;
+  const uint32_t *ib;
+  size_t nib;
+  const double *probe_p;     /* R §5.2's probe of B (Probe::draw, pair.rs:175-177): n_probe x 3 */
+  const double *probe_n;
+  size_t n_probe;
+  const sherd_cloud *s1_source; /* R §5.4's ladder: B's pc_brk, A's pc_brk_full (with a point tree) */
+  const sherd_cloud *s1_target;
+  double scales[12];         /* Scales in scales.json order */
+} sherd_match_pair;
+
+typedef struct {
+  uint32_t n_hyp;            /* hypotheses of the pair */
+  uint32_t n_kept1;          /* poses R §5.3 kept (stage 1) */
+  uint32_t n_kept2;          /* stage-1 poses R §5.5 kept (stage 2) */
+  uint32_t pad;
+  double best1;              /* the best stage-1 score (0 without a stage-1 pose) */
+} sherd_match_result;
+
+/* Pair::suppress, stage1 and suppress_stage1 (pair.rs:188-264; hypotheses.rs:139-141;
+ * coarse::scores; ladder.rs:131-139, 177-210) for n_pairs pairs in the same launches: hypotheses,
+ * coarse scores at Scales::coarse, the walk (descending score, ties by index; PMC-6), suppression
+ * (5000 walked, stage1 kept, floor 0.1), the two point-to-point rungs (0.2, 0.08; 20 iterations)
+ * from every kept hypothesis, the re-score of B's subset at Scales::stage1, and the second
+ * suppression (stage2 kept, floor 0.05). Per pair p: res[p]; kept1[p*stage1 + k] the kept
+ * hypotheses (indices into the pair's hypothesis list); s1_T[16 (p*stage1 + k)] and
+ * s1_score[p*stage1 + k] their stage-1 transforms and scores; kept2[p*stage2 + k] the stage-2
+ * poses as positions into the pair's kept1 list. dihedral_tol: Params::dihedral_tol (25). */
+int sherd_match_stage1(sherd_ctx *ctx, const sherd_match_pair *pairs, size_t n_pairs, size_t stage1, size_t stage2,
+                       double dihedral_tol, sherd_match_result *res, uint32_t *kept1, double *s1_T, double *s1_score,
+                       uint32_t *kept2);
+
 /* ---- R §9 full-resolution refinement (crates/sherd-core/src/refine.rs) ------------------------- */
 
 /* refine.rs:216-227 fracture_cloud: the cloud of an original mesh (V f64 nv x 3, F u32 nf x 3) whose

@@ -72,6 +72,16 @@ struct sherd_cloud {
   std::vector<uint32_t> idx;    // a fracture cloud's vertex indices, in summation order
 };
 
+struct sherd_frames {
+  sherd::Ctx *ctx = nullptr;
+  double *p = nullptr, *ns = nullptr, *f = nullptr, *t = nullptr, *dih = nullptr;
+  double *sub_p = nullptr, *sub_n = nullptr;  // the subset's points and shell normals
+  uint32_t *sub = nullptr;
+  int n = 0, n_sub = 0;
+  sherd::DevGrid grid;  // over p, rebuilt when a batch needs a larger reach
+  double reach = 0.0;
+};
+
 namespace sherd {
 // A cloud over device arrays the cloud takes over (freed with it): the centroid from a copy of the
 // points (summed in index order, as sherd_cloud_create does), and the point LBVH when `tree`.

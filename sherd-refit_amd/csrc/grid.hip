@@ -114,40 +114,14 @@ __global__ void __launch_bounds__(256) k_nearest(GridView g, const double *q, in
   }
 }
 
-// One wave per pose, lanes striding the probe (cpu.rs:136-169 `agreeing`): move the point and its
-// normal by the pose in f64, the nearest target strictly inside the radius, the normal test.
+// One wave per pose (coarse_agree_wave).
 __global__ void __launch_bounds__(256) k_coarse(GridView g, const double *tn, const double *pp, const double *pn,
                                                 int np, const double *poses, int k, double radius, double agree,
                                                 uint32_t *counts, double *scores) {
   int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  int lane = threadIdx.x & 63;
   if (wave >= k) return;
-  const double *r = poses + 12 * (size_t)wave;
-  const double *tau = r + 9;
-  uint32_t cnt = 0;
-  for (int base = 0; base < np; base += 64) {
-    int i = base + lane;
-    bool ok = false;
-    if (i < np) {
-      double px = pp[3 * i], py = pp[3 * i + 1], pz = pp[3 * i + 2];
-      double mx = r[0] * px + r[1] * py + r[2] * pz + tau[0];
-      double my = r[3] * px + r[4] * py + r[5] * pz + tau[1];
-      double mz = r[6] * px + r[7] * py + r[8] * pz + tau[2];
-      uint32_t near;
-      double dist;
-      if (grid_nearest_below(g, mx, my, mz, radius, &near, &dist)) {
-        double nx = pn[3 * i], ny = pn[3 * i + 1], nz = pn[3 * i + 2];
-        double ux = r[0] * nx + r[1] * ny + r[2] * nz;
-        double uy = r[3] * nx + r[4] * ny + r[5] * nz;
-        double uz = r[6] * nx + r[7] * ny + r[8] * nz;
-        const double *th = tn + 3 * (size_t)near;
-        double dd = th[0] * ux + th[1] * uy + th[2] * uz;
-        ok = dd > agree;
-      }
-    }
-    cnt += (uint32_t)__popcll(__ballot(ok));
-  }
-  if (lane == 0) {
+  uint32_t cnt = coarse_agree_wave(g, tn, pp, pn, np, poses + 12 * (size_t)wave, radius, agree);
+  if ((threadIdx.x & 63) == 0) {
     counts[wave] = cnt;
     scores[wave] = (double)cnt / (double)np;
   }

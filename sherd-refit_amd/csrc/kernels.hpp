@@ -167,6 +167,50 @@ struct IcpWideView {
 // Queues the rung's max_iteration + 1 passes; results land in b.out / b.T_out as for the batched form.
 hipError_t launch_icp_wide(const IcpWideView &w, hipStream_t s, KernelProfiler *prof);
 
+// ---- R §5.1-5.5 over many pairs (stage.hip) -----------------------------------------------------
+// One fragment's breakline frames on the device (hypotheses.rs:47-68 `Frames`).
+struct FramesView {
+  const double *p, *ns, *f, *t, *dih;
+  const double *sub_p, *sub_n;  // the subset's points and shell normals (R §5.4's re-score probe)
+  int n, n_sub;
+  GridView grid;                // over p, cell at least every radius it is queried at
+};
+// One pair of a matching batch: A is the fixed fragment (its breakline the target), B the moving.
+struct StagePair {
+  FramesView a, b;
+  const uint32_t *ia, *ib;      // the subsets the hypotheses pair (their positions are pa / pb)
+  int nia, nib;
+  const double *probe_p, *probe_n;  // R §5.2's probe of B
+  int n_probe;
+  double dihedral_tol, coarse_r, stage1_r;
+};
+// One greedy suppression (nms.rs:62-84) over items of a global pose / score table.
+struct NmsProblem {
+  const uint32_t *order;  // the walk, item indices
+  int n;
+  const double *pose;     // 12 per item: R row-major, then tau
+  const double *score;
+  double tol;
+  int topk;
+  double floor;
+  uint32_t *kept;         // item indices, in the order kept
+  uint32_t *n_kept;
+};
+hipError_t launch_hyp_count(const StagePair *sp, const uint32_t *row_first, int n_pairs, uint32_t rows, uint32_t *cnt,
+                            hipStream_t s);
+hipError_t launch_hyp_build(const StagePair *sp, const uint32_t *row_first, int n_pairs, uint32_t rows,
+                            const uint32_t *off, uint32_t n_hyp, uint32_t *pa, uint32_t *pb, uint32_t *hpair,
+                            double *pose, hipStream_t s);
+hipError_t launch_stage_coarse(const StagePair *sp, const uint32_t *ipair, const double *pose, uint32_t n, int which,
+                               uint32_t *counts, double *scores, hipStream_t s);
+// Sorts items by (pair, count descending, index within the pair ascending); vals gets item indices.
+hipError_t launch_order(const uint32_t *ipair, const uint32_t *item_first, const uint32_t *counts, uint32_t n,
+                        uint64_t *keys, uint32_t *vals, uint64_t *tk, uint32_t *tv, uint32_t *hist, hipStream_t s);
+size_t order_hist_words(uint32_t n);
+hipError_t launch_nms(const NmsProblem *probs, int n, hipStream_t s);
+hipError_t launch_pose12(const double *T, uint32_t n, double *pose, hipStream_t s);
+hipError_t launch_gather12(const double *pose, const uint32_t *idx, uint32_t n, double *out, hipStream_t s);
+
 // ---- R §9 refinement (refine.hip) ---------------------------------------------------------------
 // Vertex selection of an original mesh (nv f64 vertices) against the working mesh's face centroids
 // C (point LBVH `tree`) and fracture labels: sel/pos per vertex, pos scanned exclusive.

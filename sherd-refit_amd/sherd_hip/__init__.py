@@ -66,6 +66,18 @@ class RefinedJoin(C.Structure):
                 ("rungs", C.c_double * 32), ("fitness", C.c_double), ("rmse_t", C.c_double)]
 
 
+class MatchPair(C.Structure):
+    _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("ia", C.POINTER(C.c_uint32)), ("nia", C.c_size_t),
+                ("ib", C.POINTER(C.c_uint32)), ("nib", C.c_size_t), ("probe_p", C.POINTER(C.c_double)),
+                ("probe_n", C.POINTER(C.c_double)), ("n_probe", C.c_size_t), ("s1_source", C.c_void_p),
+                ("s1_target", C.c_void_p), ("scales", C.c_double * 12)]
+
+
+class MatchResult(C.Structure):
+    _fields_ = [("n_hyp", C.c_uint32), ("n_kept1", C.c_uint32), ("n_kept2", C.c_uint32), ("pad", C.c_uint32),
+                ("best1", C.c_double)]
+
+
 class VerifyPair(C.Structure):
     _fields_ = [("a", C.c_void_p), ("b", C.c_void_p), ("scales", C.c_double * 12)]
 
@@ -120,6 +132,11 @@ def _bind(L):
         "sherd_cloud_create": (ci, [vp, P(d), P(d), sz, P(vp)]),
         "sherd_cloud_free": (None, [vp]),
         "sherd_icp_block": (ci, [vp, ci]),
+        "sherd_frames_create": (ci, [vp, P(d), P(d), P(d), P(d), P(d), sz, P(u32), sz, P(vp)]),
+        "sherd_frames_free": (None, [vp]),
+        "sherd_hypotheses": (ci, [vp, vp, vp, P(u32), sz, P(u32), sz, d, sz, P(u32), P(u32), P(d), P(sz)]),
+        "sherd_nms": (ci, [vp, P(u32), sz, P(d), P(d), sz, d, sz, d, P(u32), P(sz)]),
+        "sherd_match_stage1": (ci, [vp, P(MatchPair), sz, sz, sz, d, P(MatchResult), P(u32), P(d), P(d), P(u32)]),
         "sherd_fracture_cloud": (ci, [vp, P(d), sz, P(u32), sz, P(d), P(u8), sz, d, d, sz, C.c_uint64, P(vp)]),
         "sherd_cloud_from_indices": (ci, [vp, P(d), sz, P(u32), sz, P(u32), sz, P(vp)]),
         "sherd_cloud_size": (sz, [vp]),
@@ -245,6 +262,33 @@ class Cloud:
     def close(self):
         if self.ptr:
             lib().sherd_cloud_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Frames:
+    """A fragment's breakline frames on the device (hypotheses.rs:47-68): points, shell normals,
+    in-plane axes, tangents, dihedral angles and the hypothesis subset."""
+
+    def __init__(self, ex, p, ns, f, t, dih, sub):
+        self.ex = ex
+        self._keep = [f64(p), f64(ns), f64(f), f64(t), np.ascontiguousarray(dih, dtype=np.float64).reshape(-1)]
+        sub = np.ascontiguousarray(sub, dtype=np.uint32).reshape(-1)
+        out = C.c_void_p()
+        _check(lib().sherd_frames_create(ex.ptr, *[_p(a, C.c_double) for a in self._keep], len(self._keep[0]),
+                                         _p(sub, C.c_uint32), len(sub), C.byref(out)))
+        self.ptr = out
+        self.n = len(self._keep[0])
+        ex._own(self)
+
+    def close(self):
+        if self.ptr:
+            lib().sherd_frames_free(self.ptr)
             self.ptr = None
 
     def __del__(self):
@@ -489,6 +533,72 @@ class HipExecutor:
 
     def huber(self, pairs, cand_pair, T0):
         return HuberBatch(self, pairs, cand_pair, T0)
+
+    def frames(self, p, ns, f, t, dih, sub):
+        return Frames(self, p, ns, f, t, dih, sub)
+
+    def hypotheses(self, a, b, ia, ib, tol=25.0):
+        """hypotheses::build_with + poses on the device: (pa, pb, poses (k, 12))."""
+        ia = np.ascontiguousarray(ia, dtype=np.uint32).reshape(-1)
+        ib = np.ascontiguousarray(ib, dtype=np.uint32).reshape(-1)
+        cap = max(len(ia) * len(ib), 1)
+        pa, pb = np.empty(cap, np.uint32), np.empty(cap, np.uint32)
+        poses = np.empty((cap, 12))
+        n = C.c_size_t(0)
+        _check(lib().sherd_hypotheses(self.ptr, a.ptr, b.ptr, _p(ia, C.c_uint32), len(ia), _p(ib, C.c_uint32), len(ib),
+                                      float(tol), cap, _p(pa, C.c_uint32), _p(pb, C.c_uint32), _p(poses, C.c_double),
+                                      C.byref(n)))
+        k = n.value
+        return pa[:k].copy(), pb[:k].copy(), poses[:k].copy()
+
+    def nms(self, order, poses, score, tol, topk, floor):
+        """nms::nms over a given walk: the kept items in the order kept."""
+        order = np.ascontiguousarray(order, dtype=np.uint32).reshape(-1)
+        poses = f64(poses, 12)
+        score = np.ascontiguousarray(score, dtype=np.float64).reshape(-1)
+        kept = np.empty(max(min(len(order), int(topk) + 1), 1), np.uint32)
+        n = C.c_size_t(0)
+        _check(lib().sherd_nms(self.ptr, _p(order, C.c_uint32), len(order), _p(poses, C.c_double),
+                               _p(score, C.c_double), len(score), float(tol), int(topk), float(floor),
+                               _p(kept, C.c_uint32), C.byref(n)))
+        return kept[:n.value].copy()
+
+    def match_stage1(self, pairs, stage1=250, stage2=10, tol=25.0):
+        """R §5.1-5.5 over many pairs. pairs: dicts with a, b (Frames), optional ia / ib, probe_p,
+        probe_n, s1_source, s1_target (Clouds) and scales (12). Returns per pair a dict: n_hyp,
+        kept1, s1_T (k, 4, 4), s1_score, kept2, best1."""
+        n = len(pairs)
+        arr = (MatchPair * max(n, 1))()
+        keep = []
+        for i, m in enumerate(pairs):
+            x = arr[i]
+            x.a, x.b = m["a"].ptr, m["b"].ptr
+            for side in ("ia", "ib"):
+                if m.get(side) is not None:
+                    v = np.ascontiguousarray(m[side], dtype=np.uint32).reshape(-1)
+                    keep.append(v)
+                    setattr(x, side, _p(v, C.c_uint32))
+                    setattr(x, "n" + side, len(v))
+            pp, pn = f64(m["probe_p"]), f64(m["probe_n"])
+            keep += [pp, pn]
+            x.probe_p, x.probe_n, x.n_probe = _p(pp, C.c_double), _p(pn, C.c_double), len(pp)
+            x.s1_source, x.s1_target = m["s1_source"].ptr, m["s1_target"].ptr
+            x.scales = (C.c_double * 12)(*[float(v) for v in m["scales"]])
+        res = (MatchResult * max(n, 1))()
+        k1 = np.zeros((n, stage1), np.uint32)
+        T = np.zeros((n, stage1, 16))
+        s1 = np.zeros((n, stage1))
+        k2 = np.zeros((n, stage2), np.uint32)
+        _check(lib().sherd_match_stage1(self.ptr, arr, n, int(stage1), int(stage2), float(tol), res,
+                                        _p(k1, C.c_uint32), _p(T, C.c_double), _p(s1, C.c_double),
+                                        _p(k2, C.c_uint32)))
+        out = []
+        for i in range(n):
+            r = res[i]
+            out.append(dict(n_hyp=int(r.n_hyp), kept1=k1[i, :r.n_kept1].copy(),
+                            s1_T=T[i, :r.n_kept1].reshape(-1, 4, 4).copy(), s1_score=s1[i, :r.n_kept1].copy(),
+                            kept2=k2[i, :r.n_kept2].copy(), best1=float(r.best1)))
+        return out
 
     def fracture_cloud(self, V, F, centroids, fracture, thick, res, max_points=150000, seed=0):
         """refine.rs:216-227 fracture_cloud of an original mesh against the working mesh's face

@@ -155,6 +155,15 @@ def main(dump):
     with open(os.path.join(OUT, "ground_truth.json"), "w") as fh:
         json.dump(gt, fh, indent=1)
     write_refine(dump)
+    write_stage(dump)
+
+
+def write_stage(dump):
+    """stage.npz: R §5.3 and R §5.5's own walk orders (PMC-6: numpy's unstable argsort), which the
+    parity of the greedy suppression is measured on."""
+    pair_dir = os.path.join(dump, "pairs", PAIR)
+    ld = lambda name: np.load(os.path.join(pair_dir, name + ".npy")).astype(np.int64)
+    np.savez_compressed(os.path.join(OUT, "stage.npz"), nms1_order=ld("nms1.order"), nms2_order=ld("nms2.order"))
 
 
 def write_refine(dump):
