@@ -387,6 +387,84 @@ def refine_section(ex, col, args):
     return res_
 
 
+def match_section(ex, col, args):
+    """R §5.1-5.5 (pair.rs:188-264) over the vessel's 28 pairs: hypotheses from the breakline frames,
+    coarse scores on a 60-point probe, the walk and suppression (250 kept), the breakline ladder
+    from every kept hypothesis, its re-score and the second suppression (10 kept): one batched call
+    against one call per pair (the reference's schedule), and the oracle on one pair."""
+    frags = col["fragments"]
+    scales = synth_scales(col)
+    frames = [ex.frames(f["brk_P"], f["brk_ns"], f["brk_f"], f["brk_t"], f["brk_dih"], f["brk_sub"]) for f in frags]
+    srcs = [ex.cloud(f["brk_P"][f["brk_sub"]]) for f in frags]
+    tgts = [ex.cloud(f["brk_P"], f["brk_ns"]) for f in frags]
+    rng = np.random.default_rng(60)
+    probes = [np.sort(rng.choice(f["brk_sub"], size=min(60, len(f["brk_sub"])), replace=False)) for f in frags]
+    pairs = [(a, b) for a in range(len(frags)) for b in range(a + 1, len(frags))]
+    mp = [dict(a=frames[a], b=frames[b], probe_p=frags[b]["brk_P"][probes[b]], probe_n=frags[b]["brk_ns"][probes[b]],
+               s1_source=srcs[b], s1_target=tgts[a], scales=scales) for a, b in pairs]
+    ex.match_stage1(mp)
+    t0 = time.perf_counter()
+    out = ex.match_stage1(mp)
+    el = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plain = [ex.match_stage1([m])[0] for m in mp]
+    el_plain = time.perf_counter() - t0
+    same = all(np.array_equal(x["kept1"], y["kept1"]) and np.array_equal(x["kept2"], y["kept2"]) and
+               np.array_equal(x["s1_T"], y["s1_T"]) for x, y in zip(out, plain))
+    n_hyp = sum(o["n_hyp"] for o in out)
+    # How close the best of the ten stage-2 inits comes to the true relative pose, on adjacent pairs.
+    gaps = []
+    for (a, b), o in zip(pairs, out):
+        if (a, b) not in col["adjacent"] or len(o["kept2"]) == 0:
+            continue
+        truth = col["poses"][a] @ np.linalg.inv(col["poses"][b])
+        best = np.inf
+        for k in o["kept2"]:
+            T = o["s1_T"][k]
+            E = T @ np.linalg.inv(truth)
+            ang = np.degrees(np.arccos(np.clip((np.trace(E[:3, :3]) - 1) / 2, -1, 1)))
+            best = min(best, max(ang / 5.0, np.abs(E[:3, 3]).max() / col["t"]))
+        gaps.append(best)
+    log(f"match stage 1: {len(pairs)} pairs, {n_hyp} hypotheses in {el * 1e3:.1f} ms batched, "
+        f"{el_plain * 1e3:.1f} ms one pair per call; identical {same}")
+    res = dict(pairs=len(pairs), hypotheses=int(n_hyp), seconds=round(el, 4),
+               hypotheses_per_s=round(n_hyp / el, 1), plain_seconds=round(el_plain, 4),
+               batched_vs_plain=round(el_plain / el, 3), identical_plain_batched=bool(same),
+               kept1=int(sum(len(o["kept1"]) for o in out)), kept2=int(sum(len(o["kept2"]) for o in out)),
+               adjacent_pairs_with_an_init_within_5deg_1t=int(sum(g <= 1.0 for g in gaps)),
+               adjacent_pairs=len(gaps),
+               workload="vessel breaklines (frames from the meshes, 2 t subset), 60-point probes, "
+                        "250 / 10 kept, rungs 0.2 / 0.08 t x 20")
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import stage as G
+
+        a, b = pairs[0]
+        fa, fb = frags[a], frags[b]
+        A = dict(p=fa["brk_P"], ns=fa["brk_ns"], f=fa["brk_f"], t=fa["brk_t"], dih=fa["brk_dih"], sub=fa["brk_sub"])
+        B = dict(p=fb["brk_P"], ns=fb["brk_ns"], f=fb["brk_f"], t=fb["brk_t"], dih=fb["brk_dih"], sub=fb["brk_sub"])
+        t0 = time.perf_counter()
+        o = G.stage1(A, B, A["sub"], B["sub"], probes[b], fb["brk_P"][fb["brk_sub"]], fa["brk_P"], scales)
+        el_cpu = time.perf_counter() - t0
+        agree = bool(list(o["kept1"]) == list(out[0]["kept1"]) and list(o["kept2"]) == list(out[0]["kept2"]))
+        res["cpu_seconds_one_pair"] = round(el_cpu, 3)
+        res["cpu_hypotheses_per_s"] = round(len(o["pa"]) / el_cpu, 1)
+        res["cpu_kept_lists_identical"] = agree
+        res["cpu_sample"] = "oracle stage1 (numpy + the C oracle's coarse and ICP) on the first pair"
+        log(f"match oracle: {el_cpu:.2f}s for one pair; kept lists identical {agree}")
+    for c in srcs + tgts:
+        c.close()
+    for f in frames:
+        f.close()
+    return res
+
+
+def synth_scales(col):
+    from sherd_hip import synth
+
+    return synth.scales_for_pair(col["t"], col["res"])
+
+
 def collection_section(ex, args):
     """The 40-fragment, three-vessel collection end to end (ICP ladder, Huber LM, verification with
     the FP64 retest over all 780 pairs), batched against the plain schedule, and the oracle's
@@ -461,6 +539,7 @@ def main():
     ap.add_argument("--icp-poses", type=int, default=64)
     ap.add_argument("--no-huber", action="store_true")
     ap.add_argument("--no-refine", action="store_true", help="skip R §9's full-resolution refinement")
+    ap.add_argument("--no-match", action="store_true", help="skip R §5.1-5.5 over the vessel's pairs")
     ap.add_argument("--no-collection", action="store_true",
                     help="skip the 40-fragment collection end to end (about 20 s of generation)")
     ap.add_argument("--batched-only", action="store_true",
@@ -574,6 +653,7 @@ def main():
     huber = huber_section(ex, synth, args) if not args.no_huber else None
     coll = collection_section(ex, args) if not args.no_collection else None
     refine = refine_section(ex, col, args) if not args.no_refine else None
+    match = match_section(ex, col, args) if not args.no_match else None
 
     cpu = None
     if not args.no_cpu:
@@ -593,7 +673,7 @@ def main():
         plain_pair_verifications_per_s=round(plain, 3), batched_vs_plain=round(value / plain, 3),
         accepted=int(accepted.sum()), accepted_identical_plain_batched=same,
         rays_per_s=round(rays_per_s, 1), rays=int(rays), knn_queries_per_s=round(knn, 1),
-        lbvh_mtri_per_s=round(lbvh_mtri, 2), icp=icp, huber=huber, collection=coll, refine=refine, kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
+        lbvh_mtri_per_s=round(lbvh_mtri, 2), icp=icp, huber=huber, collection=coll, refine=refine, match=match, kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
     )
     print(json.dumps(line), flush=True)
     ex.close()

@@ -24,6 +24,7 @@ from scipy.spatial import cKDTree
 FRAC_PER_T2, MIN_FRAC, MAX_FRAC = 150.0, 5000, 12000
 SURFACE_POINTS, MARGIN_POINTS = 20000, 6000
 BRK_VOXEL, MACRO_INNER, MACRO_OUTER = 0.5, 0.15, 0.60
+SUB_VOXEL = 2.0  # the hypothesis subset: breakline points thinned on a 2 t voxel grid
 
 
 def scales_for_pair(t, res):
@@ -220,6 +221,20 @@ def _fragment_data(v, f, frac, t, rng):
     brk = brk[np.sort(first)]
     brk_p = v[brk]
     brk_ns = vn[brk] / np.linalg.norm(vn[brk], axis=1, keepdims=True)
+    # Frames (R §3.5.4 / R §3.6): the fracture macro-normal around the vertex, the in-plane axis
+    # pointing away from the fracture (minus the fracture normal's part across the shell normal),
+    # the tangent ns x f, and the dihedral angle between the two normals. Across a seam the shell
+    # normals agree and the fracture normals oppose, so the dihedral angles add to 180 degrees.
+    vf = np.zeros_like(v)
+    np.add.at(vf, f[frac].ravel(), np.repeat(fn[frac], 3, axis=0))
+    nf = vf[brk] / np.maximum(np.linalg.norm(vf[brk], axis=1, keepdims=True), 1e-300)
+    inplane = nf - (nf * brk_ns).sum(axis=1, keepdims=True) * brk_ns
+    brk_f = -inplane / np.maximum(np.linalg.norm(inplane, axis=1, keepdims=True), 1e-300)
+    brk_t = np.cross(brk_ns, brk_f)
+    brk_dih = np.degrees(np.arccos(np.clip((brk_ns * nf).sum(axis=1), -1.0, 1.0)))
+    skey = np.floor(brk_p / (SUB_VOXEL * t)).astype(np.int64)
+    _, sfirst = np.unique(skey, axis=0, return_index=True)
+    brk_sub = np.sort(sfirst).astype(np.int64)
     # Shell margin: shell faces whose centroid lies 0.15 t .. 0.6 t from the breakline.
     cen = (a + b + c) / 3.0
     shell_faces = fi[shell]
@@ -227,8 +242,8 @@ def _fragment_data(v, f, frac, t, rng):
     band = shell_faces[(dist >= MACRO_INNER * t) & (dist <= MACRO_OUTER * t)]
     margin_p, margin_face, _ = _sample_faces(v, f, band, MARGIN_POINTS, rng)
     return dict(V=v.astype(np.float32), F=f, frac=frac, frac_area=float(frac_area), watertight=True,
-                Pf=pf, S=s, brk_P=brk_p, brk_ns=brk_ns, margin_p=margin_p, margin_n=fn[margin_face],
-                face_normals=fn)
+                Pf=pf, S=s, brk_P=brk_p, brk_ns=brk_ns, brk_f=brk_f, brk_t=brk_t, brk_dih=brk_dih, brk_sub=brk_sub,
+                margin_p=margin_p, margin_n=fn[margin_face], face_normals=fn)
 
 
 def _posed(frag, P):
@@ -237,7 +252,7 @@ def _posed(frag, P):
     out["V"] = (frag["V"].astype(np.float64) @ R.T + tau).astype(np.float32)
     for key in ("Pf", "S", "brk_P", "margin_p"):
         out[key] = frag[key] @ R.T + tau
-    for key in ("brk_ns", "margin_n", "face_normals"):
+    for key in ("brk_ns", "brk_f", "brk_t", "margin_n", "face_normals"):
         out[key] = frag[key] @ R.T
     return out
 
