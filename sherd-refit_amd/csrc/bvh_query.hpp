@@ -227,6 +227,22 @@ __device__ inline uint32_t crossings_axis(const SceneView &sv, V3 o, int retest,
   return hits;
 }
 
+// How many triangles the parity grid lists in the cell of the ray from o along +AXIS.
+template <int AXIS>
+__device__ __forceinline__ uint32_t cell_items(const SceneView &sv, V3 o) {
+  constexpr int U = AXIS == 0 ? 1 : 0, W = AXIS == 2 ? 1 : 2;
+  const AxisGrid &g = sv.pg[AXIS];
+  float oo[3] = {o.x, o.y, o.z};
+  int cell = grid_cell(oo[U], g.org_u, g.inv, g.nu) * g.nv + grid_cell(oo[W], g.org_v, g.inv, g.nv);
+  return g.start[cell + 1] - g.start[cell];
+}
+
+__device__ __forceinline__ uint32_t crossings_on(int axis, const SceneView &sv, V3 o, int retest, float *nearest) {
+  if (axis == 0) return crossings_axis<0>(sv, o, retest, nearest);
+  if (axis == 1) return crossings_axis<1>(sv, o, retest, nearest);
+  return crossings_axis<2>(sv, o, retest, nearest);
+}
+
 // Smallest squared distance to the scene among triangles within `window` (all when window is
 // +inf); +inf when none is. Pruned against min(best, window) grown by the slack.
 __device__ inline float min_d2_stackless(const SceneView &sv, V3 p, float window) {
@@ -425,13 +441,17 @@ __device__ __forceinline__ bool inside_depth(const SceneView &sv, V3 p, int rete
   if (p.x < sv.lo[0] || p.x > sv.hi[0] || p.y < sv.lo[1] || p.y > sv.hi[1] || p.z < sv.lo[2] ||
       p.z > sv.hi[2])
     return false;
-  // Majority of three odd counts: two agreeing rays decide it, and the third is cast only when
-  // the first two disagree.
+  // Majority of three odd counts: any two agreeing rays decide it, and the third is cast only when
+  // the first two disagree. The two rays whose cells list the fewest triangles go first (a wall
+  // seen edge-on fills one axis's cell with hundreds); the vote does not depend on the order.
+  uint32_t n[3] = {cell_items<0>(sv, p), cell_items<1>(sv, p), cell_items<2>(sv, p)};
+  int hi = n[2] >= n[1] ? (n[2] >= n[0] ? 2 : 0) : (n[1] >= n[0] ? 1 : 0);
+  int a0 = hi == 0 ? 1 : 0, a1 = hi == 2 ? 1 : 2;
   float t0, t1, t2 = __builtin_inff();
-  int v0 = (int)(crossings_axis<0>(sv, p, retest, &t0) & 1u);
-  int v1 = (int)(crossings_axis<1>(sv, p, retest, &t1) & 1u);
+  int v0 = (int)(crossings_on(a0, sv, p, retest, &t0) & 1u);
+  int v1 = (int)(crossings_on(a1, sv, p, retest, &t1) & 1u);
   int votes = v0 + v1;
-  if (v0 != v1) votes += (int)(crossings_axis<2>(sv, p, retest, &t2) & 1u);
+  if (v0 != v1) votes += (int)(crossings_on(hi, sv, p, retest, &t2) & 1u);
   if (votes < 2) return false;
   // The depth is the distance to the nearest triangle. A ray's nearest crossing is a point of the
   // surface t away, so a search bounded a little beyond the smallest t finds the same minimum
