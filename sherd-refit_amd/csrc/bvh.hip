@@ -246,12 +246,13 @@ __global__ void k_preorder(int m, const Node *nodes, const int *parent_of, const
 }
 
 // Moves every node to its pre-order position: an internal node's left child is then the next
-// node, so a descent reads consecutive memory. Leaves get a = ~(triangle slot).
-__global__ void k_relayout(int m, const Node *in, const int *pre, Node *out) {
+// node, and its `a` becomes the right child's position, so a descent has both children's
+// addresses at hand and loads them together. Leaves get a = ~(triangle slot).
+__global__ void k_relayout(int m, const Node *in, const int *childb, const int *pre, Node *out) {
   int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= 2 * m - 1) return;
   Node nd = in[x];
-  nd.a = x < m - 1 ? pre[nd.a] : ~nd.a;
+  nd.a = x < m - 1 ? pre[childb[x]] : ~nd.a;
   nd.esc = nd.esc < 0 ? -1 : pre[nd.esc];
   out[pre[x]] = nd;
 }
@@ -332,7 +333,8 @@ hipError_t scene_build(const float *dV, const uint32_t *dF, int m, DevScene *out
     hipLaunchKernelGGL(k_ropes, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, tree, childb, range_r);
     hipLaunchKernelGGL(k_preorder, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, tree, parent_of, range_l,
                        root, pre);
-    hipLaunchKernelGGL(k_relayout, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, tree, pre, out->nodes);
+    hipLaunchKernelGGL(k_relayout, dim3(blocks_for(2L * m - 1, BT)), dim3(BT), 0, s, m, tree, childb, pre,
+                       out->nodes);
     if ((err = hipGetLastError()) != hipSuccess) break;
     float hb[14];
     if ((err = hipMemcpyAsync(hb, bounds, sizeof hb, hipMemcpyDeviceToHost, s)) != hipSuccess) break;

@@ -14,7 +14,7 @@ constexpr int DSTACK = 24;  // deferred nodes of an ordered traversal
 
 __device__ __forceinline__ Node load_node(const Node *nodes, int i) {
   const float4 *p = reinterpret_cast<const float4 *>(nodes + i);
-  float4 a = p[0], b = p[1];
+  float4 a = gld4(p), b = gld4(p + 1);
   Node n;
   n.lo[0] = a.x; n.lo[1] = a.y; n.lo[2] = a.z; n.a = __float_as_int(a.w);
   n.hi[0] = b.x; n.hi[1] = b.y; n.hi[2] = b.z; n.esc = __float_as_int(b.w);
@@ -23,7 +23,7 @@ __device__ __forceinline__ Node load_node(const Node *nodes, int i) {
 
 __device__ __forceinline__ void load_tri(const Tri *tris, int j, V3 *a, V3 *b, V3 *c, uint32_t *face) {
   const float4 *p = reinterpret_cast<const float4 *>(tris + j);
-  float4 x = p[0], y = p[1], z = p[2];
+  float4 x = gld4(p), y = gld4(p + 1), z = gld4(p + 2);
   *a = V3{x.x, x.y, x.z};
   *b = V3{x.w, y.x, y.y};
   *c = V3{y.z, y.w, z.x};
@@ -68,7 +68,7 @@ __device__ inline void first_hit_stackless(const SceneView &sv, V3 o, V3 d, int 
     Node nd = load_node(sv.nodes, i);
     if (nd.a >= 0) {
       float te = ray_enter(nd, o, d, inv, pad, best + tpad);
-      i = te >= 0.0f ? nd.a : nd.esc;
+      i = te >= 0.0f ? i + 1 : nd.esc;
     } else {
       V3 a, b, c;
       uint32_t face;
@@ -104,13 +104,13 @@ __device__ inline void first_hit(const SceneView &sv, V3 o, V3 d, int retest, fl
   *t_out = best;
   *face_out = bface;
   if (cur < 0 || cur >= nodes) return;
-  if (ray_enter(load_node(sv.nodes, cur), o, d, inv, pad, best) < 0.0f) return;
+  Node nd = load_node(sv.nodes, cur);
+  if (ray_enter(nd, o, d, inv, pad, best) < 0.0f) return;
   int st_n[DSTACK];
   float st_t[DSTACK];
   int sp = 0;
   bool overflow = false;
   for (int iter = 0; iter <= nodes; ++iter) {
-    Node nd = load_node(sv.nodes, cur);
     int next = -1;
     if (nd.a < 0) {
       V3 a, b, c;
@@ -123,37 +123,40 @@ __device__ inline void first_hit(const SceneView &sv, V3 o, V3 d, int retest, fl
           bface = face;
         }
       }
-    } else {
-      int l = nd.a;
-      Node ln = load_node(sv.nodes, l);
-      int r = ln.esc;
+    } else if (nd.a > cur && nd.a < nodes) {
+      int l = cur + 1, r = nd.a;
+      Node ln = load_node(sv.nodes, l), rn = load_node(sv.nodes, r);
       float tmax = best + tpad;
       float tl = ray_enter(ln, o, d, inv, pad, tmax);
-      float tr = -1.0f;
-      if (r >= 0 && r < nodes) tr = ray_enter(load_node(sv.nodes, r), o, d, inv, pad, tmax);
+      float tr = ray_enter(rn, o, d, inv, pad, tmax);
       bool hl = tl >= 0.0f, hr = tr >= 0.0f;
       if (hl && hr) {
-        int near = tl <= tr ? l : r, far = tl <= tr ? r : l;
-        float tfar = tl <= tr ? tr : tl;
+        bool lf = tl <= tr;
         if (sp < DSTACK) {
-          st_n[sp] = far;
-          st_t[sp] = tfar;
+          st_n[sp] = lf ? r : l;
+          st_t[sp] = lf ? tr : tl;
           ++sp;
         } else {
           overflow = true;
         }
-        next = near;
+        next = lf ? l : r;
+        nd = lf ? ln : rn;
       } else if (hl) {
         next = l;
+        nd = ln;
       } else if (hr) {
         next = r;
+        nd = rn;
       }
     }
-    while (next < 0 && sp > 0) {
-      --sp;
-      if (st_t[sp] <= best + tpad) next = st_n[sp];
+    if (next < 0) {
+      while (next < 0 && sp > 0) {
+        --sp;
+        if (st_t[sp] <= best + tpad) next = st_n[sp];
+      }
+      if (next < 0) break;
+      nd = load_node(sv.nodes, next);
     }
-    if (next < 0) break;
     cur = next;
   }
   if (overflow) {
@@ -199,29 +202,30 @@ __device__ inline uint32_t crossings_axis(const SceneView &sv, V3 o, int retest,
   float oo[3] = {o.x, o.y, o.z};
   float pad = slack_of(sv.absmax, o);
   int cell = grid_cell(oo[U], g.org_u, g.inv, g.nu) * g.nv + grid_cell(oo[W], g.org_v, g.inv, g.nv);
-  uint32_t b = g.start[cell], e = g.start[cell + 1];
+  uint32_t b = gldu(g.start + cell), e = gldu(g.start + cell + 1);
   uint32_t hits = 0;
   float tmin = __builtin_inff();
   uint32_t k = b;
   for (; k + XCH <= e; k += XCH) {
     uint32_t idx[XCH];
 #pragma unroll
-    for (int u = 0; u < XCH; ++u) idx[u] = g.item[k + u];
+    for (int u = 0; u < XCH; ++u) idx[u] = gldu(g.item + k + u);
     float4 x[XCH], y[XCH];
     float zx[XCH];
 #pragma unroll
     for (int u = 0; u < XCH; ++u) {
       const float4 *p = reinterpret_cast<const float4 *>(sv.tris + idx[u]);
-      x[u] = p[0];
-      y[u] = p[1];
-      zx[u] = reinterpret_cast<const float *>(p)[8];
+      x[u] = gld4(p);
+      y[u] = gld4(p + 1);
+      zx[u] = gldf(reinterpret_cast<const float *>(p) + 8);
     }
 #pragma unroll
     for (int u = 0; u < XCH; ++u) crossing_test<AXIS>(o, retest, pad, x[u], y[u], zx[u], &hits, &tmin);
   }
   for (; k < e; ++k) {
-    const float4 *p = reinterpret_cast<const float4 *>(sv.tris + g.item[k]);
-    crossing_test<AXIS>(o, retest, pad, p[0], p[1], reinterpret_cast<const float *>(p)[8], &hits, &tmin);
+    const float4 *p = reinterpret_cast<const float4 *>(sv.tris + gldu(g.item + k));
+    crossing_test<AXIS>(o, retest, pad, gld4(p), gld4(p + 1), gldf(reinterpret_cast<const float *>(p) + 8), &hits,
+                        &tmin);
   }
   *nearest = tmin;
   return hits;
@@ -234,7 +238,7 @@ __device__ __forceinline__ uint32_t cell_items(const SceneView &sv, V3 o) {
   const AxisGrid &g = sv.pg[AXIS];
   float oo[3] = {o.x, o.y, o.z};
   int cell = grid_cell(oo[U], g.org_u, g.inv, g.nu) * g.nv + grid_cell(oo[W], g.org_v, g.inv, g.nv);
-  return g.start[cell + 1] - g.start[cell];
+  return gldu(g.start + cell + 1) - gldu(g.start + cell);
 }
 
 __device__ __forceinline__ uint32_t crossings_on(int axis, const SceneView &sv, V3 o, int retest, float *nearest) {
@@ -256,7 +260,7 @@ __device__ inline float min_d2_stackless(const SceneView &sv, V3 p, float window
   while (i >= 0 && i < nodes && steps++ < nodes) {
     Node nd = load_node(sv.nodes, i);
     if (nd.a >= 0) {
-      i = box_d2(nd, p, slack) <= lim2 ? nd.a : nd.esc;
+      i = box_d2(nd, p, slack) <= lim2 ? i + 1 : nd.esc;
     } else {
       V3 a, b, c;
       uint32_t face;
@@ -275,9 +279,10 @@ __device__ inline float min_d2_stackless(const SceneView &sv, V3 p, float window
 
 // The same minimum, nearest child first: both children's boxes are measured, the nearer is
 // entered and the farther deferred on a short stack, and a deferred node is dropped when the
-// minimum found since has moved past it. In this layout the right child of an internal node is the
-// escape link of its left child, so no child pointer beyond the tree's own is needed. Should the
-// stack ever fill, the exhaustive walk above finishes the query, so the answer is the same.
+// minimum found since has moved past it. Both children are loaded together (the left is the next
+// node, the right is the parent's `a`) and the entered child's node is carried into the next step,
+// so a level of the descent costs one memory round trip. Should the stack ever fill, the
+// exhaustive walk above finishes the query, so the answer is the same.
 //
 // With a step budget (steps > 0) the walk gives up after that many nodes: *complete is then false
 // and the result is only the smallest distance met so far, one the point really attains.
@@ -290,7 +295,8 @@ __device__ inline float min_d2(const SceneView &sv, V3 p, float window, int step
   int nodes = 2 * sv.n - 1;
   int cur = sv.root;
   if (cur < 0 || cur >= nodes) return best;
-  if (box_d2(load_node(sv.nodes, cur), p, slack) > lim2) return best;
+  Node nd = load_node(sv.nodes, cur);
+  if (box_d2(nd, p, slack) > lim2) return best;
   int st_n[DSTACK];
   float st_d[DSTACK];
   int sp = 0;
@@ -300,7 +306,6 @@ __device__ inline float min_d2(const SceneView &sv, V3 p, float window, int step
       *complete = false;
       return best;
     }
-    Node nd = load_node(sv.nodes, cur);
     int next = -1;
     if (nd.a < 0) {
       V3 a, b, c;
@@ -312,36 +317,38 @@ __device__ inline float min_d2(const SceneView &sv, V3 p, float window, int step
         float l = mnf(sqrtf(best), window) + slack;
         lim2 = l * l;
       }
-    } else {
-      int l = nd.a;
-      Node ln = load_node(sv.nodes, l);
-      int r = ln.esc;
-      float dl = box_d2(ln, p, slack);
-      float dr = __builtin_inff();
-      if (r >= 0 && r < nodes) dr = box_d2(load_node(sv.nodes, r), p, slack);
+    } else if (nd.a > cur && nd.a < nodes) {
+      int l = cur + 1, r = nd.a;
+      Node ln = load_node(sv.nodes, l), rn = load_node(sv.nodes, r);
+      float dl = box_d2(ln, p, slack), dr = box_d2(rn, p, slack);
       bool hl = dl <= lim2, hr = dr <= lim2;
       if (hl && hr) {
-        int near = dl <= dr ? l : r, far = dl <= dr ? r : l;
-        float dfar = dl <= dr ? dr : dl;
+        bool lf = dl <= dr;
         if (sp < DSTACK) {
-          st_n[sp] = far;
-          st_d[sp] = dfar;
+          st_n[sp] = lf ? r : l;
+          st_d[sp] = lf ? dr : dl;
           ++sp;
         } else {
           overflow = true;
         }
-        next = near;
+        next = lf ? l : r;
+        nd = lf ? ln : rn;
       } else if (hl) {
         next = l;
+        nd = ln;
       } else if (hr) {
         next = r;
+        nd = rn;
       }
     }
-    while (next < 0 && sp > 0) {
-      --sp;
-      if (st_d[sp] <= lim2) next = st_n[sp];
+    if (next < 0) {
+      while (next < 0 && sp > 0) {
+        --sp;
+        if (st_d[sp] <= lim2) next = st_n[sp];
+      }
+      if (next < 0) break;
+      nd = load_node(sv.nodes, next);
     }
-    if (next < 0) break;
     cur = next;
   }
   if (overflow) {
@@ -385,12 +392,11 @@ __device__ inline float min_d2_wave(const SceneView &sv, V3 p, float window, int
         uint32_t face;
         load_tri(sv.tris, ~nd.a, &a, &b, &c, &face);
         mine = tri_d2(p, a, b, c);
-      } else if (nd.a < nodes) {
-        int l = nd.a;
-        Node ln = load_node(sv.nodes, l);
-        int r = ln.esc;
+      } else if (nd.a > node && nd.a < nodes) {
+        int l = node + 1, r = nd.a;
+        Node ln = load_node(sv.nodes, l), rn = load_node(sv.nodes, r);
         if (box_d2(ln, p, slack) <= lim2) c0 = l;
-        if (r >= 0 && r < nodes && box_d2(load_node(sv.nodes, r), p, slack) <= lim2) c1 = r;
+        if (box_d2(rn, p, slack) <= lim2) c1 = r;
       }
     }
     uint32_t wb = wave_min(__float_as_uint(mine));

@@ -7,6 +7,12 @@ namespace sherd {
 
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 
+// kiddo's squared distance from q to point j of the cloud.
+__device__ __forceinline__ double point_d2(const double *pts, uint32_t j, double q0, double q1, double q2) {
+  const double *p = pts + 3 * (size_t)j;
+  return d2_f64(q0, q1, q2, gldd(p), gldd(p + 1), gldd(p + 2));
+}
+
 // The nearest point of a cloud to q with d^2 < r2, ties to the lowest index. The cloud's tree is
 // an LBVH over its points (each leaf a point, `face` its index); boxes are f32 and grown by the
 // slack, which exceeds the rounding of the f32 boxes and of q, so no box hides a point nearer than
@@ -25,7 +31,7 @@ __device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, dou
   double best = r2;
   uint32_t bj = NONE;
   if (seed != NONE && seed < (uint32_t)sv.n) {
-    double d2 = d2_f64(q0, q1, q2, pts[3 * (size_t)seed], pts[3 * (size_t)seed + 1], pts[3 * (size_t)seed + 2]);
+    double d2 = point_d2(pts, seed, q0, q1, q2);
     if (d2 < best) {
       best = d2;
       bj = seed;
@@ -33,7 +39,8 @@ __device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, dou
   }
   float lim = (float)sqrt(best) + slack;
   float lim2 = lim * lim;
-  if (box_d2(load_node(sv.nodes, cur), qf, slack) > lim2) {
+  Node nd = load_node(sv.nodes, cur);
+  if (box_d2(nd, qf, slack) > lim2) {
     if (bj == NONE) return false;
     cur = -1;  // (cannot happen with a seed: the root's box holds it) keep the seed
   }
@@ -42,49 +49,51 @@ __device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, dou
   int sp = 0;
   bool overflow = false;
   for (int iter = 0; cur >= 0 && iter <= nodes; ++iter) {
-    Node nd = load_node(sv.nodes, cur);
     int next = -1;
     if (nd.a < 0) {
       V3 a, b, c;
       uint32_t j;
       load_tri(sv.tris, ~nd.a, &a, &b, &c, &j);
-      double d2 = d2_f64(q0, q1, q2, pts[3 * (size_t)j], pts[3 * (size_t)j + 1], pts[3 * (size_t)j + 2]);
+      double d2 = point_d2(pts, j, q0, q1, q2);
       if (d2 < best || (d2 == best && j < bj)) {
         best = d2;
         bj = j;
         lim = (float)sqrt(best) + slack;
         lim2 = lim * lim;
       }
-    } else {
-      int l = nd.a;
-      Node ln = load_node(sv.nodes, l);
-      int r = ln.esc;
-      float dl = box_d2(ln, qf, slack);
-      float dr = __builtin_inff();
-      if (r >= 0 && r < nodes) dr = box_d2(load_node(sv.nodes, r), qf, slack);
+    } else if (nd.a > cur && nd.a < nodes) {
+      // Both children at once: the left is the next node, the right the parent's `a`.
+      int l = cur + 1, r = nd.a;
+      Node ln = load_node(sv.nodes, l), rn = load_node(sv.nodes, r);
+      float dl = box_d2(ln, qf, slack), dr = box_d2(rn, qf, slack);
       bool hl = dl <= lim2, hr = dr <= lim2;
       if (hl && hr) {
-        int near = dl <= dr ? l : r, far = dl <= dr ? r : l;
-        float dfar = dl <= dr ? dr : dl;
+        bool lf = dl <= dr;
         if (sp < DSTACK) {
-          st_n[sp] = far;
-          st_d[sp] = dfar;
+          st_n[sp] = lf ? r : l;
+          st_d[sp] = lf ? dr : dl;
           ++sp;
         } else {
           overflow = true;
         }
-        next = near;
+        next = lf ? l : r;
+        nd = lf ? ln : rn;
       } else if (hl) {
         next = l;
+        nd = ln;
       } else if (hr) {
         next = r;
+        nd = rn;
       }
     }
-    while (next < 0 && sp > 0) {
-      --sp;
-      if (st_d[sp] <= lim2) next = st_n[sp];
+    if (next < 0) {
+      while (next < 0 && sp > 0) {
+        --sp;
+        if (st_d[sp] <= lim2) next = st_n[sp];
+      }
+      if (next < 0) break;
+      nd = load_node(sv.nodes, next);
     }
-    if (next < 0) break;
     cur = next;
   }
   if (overflow) {
@@ -93,12 +102,12 @@ __device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, dou
     for (int i = sv.root; i >= 0 && i < nodes && steps++ < nodes;) {
       Node nd = load_node(sv.nodes, i);
       if (nd.a >= 0) {
-        i = box_d2(nd, qf, slack) <= lim2 ? nd.a : nd.esc;
+        i = box_d2(nd, qf, slack) <= lim2 ? i + 1 : nd.esc;
       } else {
         V3 a, b, c;
         uint32_t j;
         load_tri(sv.tris, ~nd.a, &a, &b, &c, &j);
-        double d2 = d2_f64(q0, q1, q2, pts[3 * (size_t)j], pts[3 * (size_t)j + 1], pts[3 * (size_t)j + 2]);
+        double d2 = point_d2(pts, j, q0, q1, q2);
         if (d2 < best || (d2 == best && j < bj)) {
           best = d2;
           bj = j;

@@ -153,10 +153,10 @@ SH_HD V3 moved_f32(const double *T, double px, double py, double pz) {
 }
 
 // ---- BVH layout --------------------------------------------------------------------------------
-// 2n-1 nodes in pre-order, the root at 0. `a` >= 0 marks an internal node and is its left child
-// (always the next node); a leaf holds a = ~(its triangle slot). `esc` is the next node in
-// pre-order after this node's subtree, or -1, so the right child of an internal node is the escape
-// of its left child. 32 bytes, one dwordx4 pair per node.
+// 2n-1 nodes in pre-order, the root at 0. `a` >= 0 marks an internal node and is its right child
+// (the left child is always the next node); a leaf holds a = ~(its triangle slot). `esc` is the
+// next node in pre-order after this node's subtree, or -1. A descent thus knows both children's
+// addresses from the parent and loads them together. 32 bytes, one dwordx4 pair per node.
 struct Node {
   float lo[3];
   int a;
@@ -245,14 +245,14 @@ SH_HD float slack_of(float absmax, V3 p) {
   return 1e-5f * m + 1e-30f;
 }
 
-// Squared distance from p to the node's box grown by pad (conservative).
+// Squared distance from p to the node's box grown by pad (conservative). An axis's gap is the
+// larger of lo - x and x - hi, or 0 inside the slab: the values of the case split (lo - x when x is
+// below, x - hi when above), taken without branches.
+SH_HD float box_gap(float lo, float hi, float x) { return fmaxf(fmaxf(lo - x, x - hi), 0.0f); }
 SH_HD float box_d2(const Node &nd, V3 p, float pad) {
-  float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
-  float l0 = nd.lo[0] - pad, l1 = nd.lo[1] - pad, l2 = nd.lo[2] - pad;
-  float h0 = nd.hi[0] + pad, h1 = nd.hi[1] + pad, h2 = nd.hi[2] + pad;
-  if (p.x < l0) g0 = l0 - p.x; else if (p.x > h0) g0 = p.x - h0;
-  if (p.y < l1) g1 = l1 - p.y; else if (p.y > h1) g1 = p.y - h1;
-  if (p.z < l2) g2 = l2 - p.z; else if (p.z > h2) g2 = p.z - h2;
+  float g0 = box_gap(nd.lo[0] - pad, nd.hi[0] + pad, p.x);
+  float g1 = box_gap(nd.lo[1] - pad, nd.hi[1] + pad, p.y);
+  float g2 = box_gap(nd.lo[2] - pad, nd.hi[2] + pad, p.z);
   return (g0 * g0 + g1 * g1) + g2 * g2;
 }
 
@@ -271,6 +271,24 @@ SH_HD double from_total_key(uint64_t k) {
   uint64_t u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
   return __builtin_bit_cast(double, u);
 }
+
+// ---- loads from device memory ------------------------------------------------------------------
+// Scene, tree and cloud pointers reach the kernels inside structures, so the compiler cannot tell
+// they address global memory and emits flat loads, which also wait on the LDS counter. These read
+// through the global address space instead (every such buffer is device or pinned memory).
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const f32x4_t g_f32x4;
+typedef __attribute__((address_space(1))) const float g_f32;
+typedef __attribute__((address_space(1))) const uint32_t g_u32;
+typedef __attribute__((address_space(1))) const double g_f64;
+
+__device__ __forceinline__ float4 gld4(const void *p) {
+  f32x4_t v = *(g_f32x4 *)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float gldf(const float *p) { return *(g_f32 *)p; }
+__device__ __forceinline__ uint32_t gldu(const uint32_t *p) { return *(g_u32 *)p; }
+__device__ __forceinline__ double gldd(const double *p) { return *(g_f64 *)p; }
 
 // ---- wave64 helpers ----------------------------------------------------------------------------
 // How many lanes below this one are set in m.

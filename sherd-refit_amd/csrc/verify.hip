@@ -398,16 +398,9 @@ __global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb, int per_task) {
     int nodes = 2 * sv.n - 1;
     int cur = sv.root;
     Node nd = load_node(sv.nodes, cur);
-    for (int depth = 0; depth < nodes && nd.a >= 0 && nd.a < nodes; ++depth) {
-      int l = nd.a;
-      Node ln = load_node(sv.nodes, l);
-      int r = ln.esc;
-      if (r < 0 || r >= nodes) {
-        cur = l;
-        nd = ln;
-        continue;
-      }
-      Node rn = load_node(sv.nodes, r);
+    for (int depth = 0; depth < nodes && nd.a > cur && nd.a < nodes; ++depth) {
+      int l = cur + 1, r = nd.a;
+      Node ln = load_node(sv.nodes, l), rn = load_node(sv.nodes, r);
       bool go_left = box_d2(ln, q, 0.0f) <= box_d2(rn, q, 0.0f);
       cur = go_left ? l : r;
       nd = go_left ? ln : rn;
