@@ -60,16 +60,37 @@ __device__ __forceinline__ void spans_total(const double *part, int nspan, doubl
     for (int k = 0; k < N; ++k) out[k] = v[k];
 }
 
-// What the next update needs from one corresponding pair (moved source point p, target q with
-// normal n): the count and squared distance, then the point-to-plane row products (icp.rs:599-640)
-// or the two point sums of point-to-point (icp.rs:557-575).
+// The kernels' sums by estimation method: EST_P2P for point-to-point, EST_PLANE for point-to-plane
+// and for identity updates (est 2, or point-to-plane without target normals), which read only the
+// count and err^2.
+constexpr int EST_P2P = 0, EST_PLANE = 1;
+
+// How many per-pass sums a method keeps: count, err^2, then 27 normal-equation sums for
+// point-to-plane or the two point sums of point-to-point.
+template <int EST>
+constexpr int icp_sums() {
+  return EST == EST_PLANE ? NV : 8;
+}
+
+// What the next update needs from one corresponding pair (moved source point p, target point j):
+// the count and squared distance, then the point-to-plane row products (icp.rs:599-640) or the two
+// point sums of point-to-point (icp.rs:557-575). Without normals (tn null) the plane products are
+// of a zero normal; such an update is the identity and never reads them.
+template <int EST>
 __device__ __forceinline__ void icp_accumulate(double *acc, double p0, double p1, double p2, double d2,
-                                               const double *q, const double *n, bool plane, bool p2p) {
+                                               const double *tp, const double *tn, uint32_t j) {
   acc[0] += 1.0;
   acc[1] += d2;
-  if (plane) {
-    double row[6] = {p1 * n[2] - p2 * n[1], p2 * n[0] - p0 * n[2], p0 * n[1] - p1 * n[0], n[0], n[1], n[2]};
-    double r = (p0 - q[0]) * n[0] + (p1 - q[1]) * n[1] + (p2 - q[2]) * n[2];
+  const double *q = tp + 3 * (size_t)j;
+  if (EST == EST_PLANE) {
+    double n0 = 0.0, n1 = 0.0, n2 = 0.0;
+    if (tn) {
+      n0 = gldd(tn + 3 * (size_t)j);
+      n1 = gldd(tn + 3 * (size_t)j + 1);
+      n2 = gldd(tn + 3 * (size_t)j + 2);
+    }
+    double row[6] = {p1 * n2 - p2 * n1, p2 * n0 - p0 * n2, p0 * n1 - p1 * n0, n0, n1, n2};
+    double r = (p0 - gldd(q)) * n0 + (p1 - gldd(q + 1)) * n1 + (p2 - gldd(q + 2)) * n2;
     int w = 2;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
@@ -78,13 +99,13 @@ __device__ __forceinline__ void icp_accumulate(double *acc, double p0, double p1
     }
 #pragma unroll
     for (int k = 0; k < 6; ++k) acc[23 + k] += row[k] * r;
-  } else if (p2p) {
+  } else {
     acc[2] += p0;
     acc[3] += p1;
     acc[4] += p2;
-    acc[5] += q[0];
-    acc[6] += q[1];
-    acc[7] += q[2];
+    acc[5] += gldd(q);
+    acc[6] += gldd(q + 1);
+    acc[7] += gldd(q + 2);
   }
 }
 
@@ -183,9 +204,11 @@ __device__ __forceinline__ void write_result(const IcpBatchView &v, int c, const
     for (int k = 0; k < 16; ++k) v.T_out[16 * (size_t)c + k] = T[k];
 }
 
-template <int B>
+template <int B, int EST>
 __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
   constexpr int W = B / 64;
+  constexpr int NS = icp_sums<EST>();
+  constexpr bool p2p = EST == EST_P2P;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   double *part = v.part + (size_t)blockIdx.x * v.part_spans * NV;
   __shared__ double sT[16], sU[16];
@@ -198,7 +221,7 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
       s_c = k < (uint32_t)v.n_cand ? (int)v.order[k] : -1;
     }
     __syncthreads();
-    const int c = s_c;
+    const int c = __builtin_amdgcn_readfirstlane(s_c);
     if (c < 0) break;
     const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
     const int ns = pd.ns;
@@ -207,8 +230,7 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
     uint32_t *corr = v.corr + v.off[c];
     const bool radius_ok = pd.r > 0.0;
     const double r2 = pd.r * pd.r;
-    const bool plane = v.est == 1 && pd.tn != nullptr;
-    const bool p2p = v.est == 0;  // otherwise every update is the identity (no normals)
+    const bool plane = v.est == 1 && pd.tn != nullptr;  // otherwise every update is the identity
     if (t < 16) {
       sT[t] = v.T_in[16 * (size_t)c + t];
       sU[t] = sT[t];
@@ -220,10 +242,10 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
 #pragma unroll
       for (int k = 0; k < 12; ++k) M[k] = sU[k];
       for (int sp = wave; sp < nspan; sp += W) {
-        double acc[NV];
-#pragma unroll
-        for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-        for (int i = sp * ICP_SPAN + lane; i < ns && i < (sp + 1) * ICP_SPAN; i += 64) {
+        const int end = ns < (sp + 1) * ICP_SPAN ? ns : (sp + 1) * ICP_SPAN;
+        // The span's correspondences first, each lane its points l, l + 64, ... Nothing of the sums
+        // is live during the searches, which keeps the wave's registers to the tree walk's.
+        for (int i = sp * ICP_SPAN + lane; i < end; i += 64) {
           const double *s = it == 0 ? pd.src + 3 * (size_t)i : moved + 3 * (size_t)i;
           double s0 = s[0], s1 = s[1], s2 = s[2];
           double p0 = M[0] * s0 + M[1] * s1 + M[2] * s2 + M[3];
@@ -235,21 +257,27 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
           uint32_t j = NONE;
           double d2;
           uint32_t seed = it > 0 ? corr[i] : NONE;  // last iteration's partner (this candidate's own)
-          if (radius_ok && cloud_nearest(pd.tree, pd.tp, p0, p1, p2, r2, &j, &d2, seed)) {
-            icp_accumulate(acc, p0, p1, p2, d2, pd.tp + 3 * (size_t)j, plane ? pd.tn + 3 * (size_t)j : nullptr,
-                           plane, p2p);
-          } else {
-            j = NONE;
-          }
+          if (!radius_ok || !cloud_nearest(pd.tree, pd.tp, p0, p1, p2, r2, &j, &d2, seed)) j = NONE;
           corr[i] = j;
         }
-        wave_sum<NV>(acc);
+        // Then the sums over the same points in the same order, from what the lane just stored (the
+        // distance is the search's own arithmetic on the same values).
+        double acc[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+        for (int i = sp * ICP_SPAN + lane; i < end; i += 64) {
+          uint32_t j = corr[i];
+          if (j == NONE) continue;
+          double p0 = moved[3 * (size_t)i], p1 = moved[3 * (size_t)i + 1], p2 = moved[3 * (size_t)i + 2];
+          icp_accumulate<EST>(acc, p0, p1, p2, point_d2(pd.tp, j, p0, p1, p2), pd.tp, pd.tn, j);
+        }
+        wave_sum<NS>(acc);
         if (lane == 0)
 #pragma unroll
-          for (int k = 0; k < NV; ++k) part[(size_t)sp * NV + k] = acc[k];
+          for (int k = 0; k < NS; ++k) part[(size_t)sp * NV + k] = acc[k];
       }
       __syncthreads();
-      if (wave == 0) spans_total<NV>(part, nspan, red);
+      if (wave == 0) spans_total<NS>(part, nspan, red);
       __syncthreads();
       if (t == 0) icp_iteration(red, ns, it, v.max_iter, plane, p2p, &s_it, sU);
       __syncthreads();
@@ -332,7 +360,9 @@ __global__ void k_icp_wide_init(IcpWideView w) {
   st.stop = st.iters = st.conv = 0;
 }
 
+template <int EST>
 __global__ void __launch_bounds__(ICP_SPAN) k_icp_wide_corr(IcpWideView w, int it) {
+  constexpr int NS = icp_sums<EST>();
   __shared__ double s_p[3 * ICP_SPAN];
   __shared__ double s_d2[ICP_SPAN];
   __shared__ uint32_t s_j[ICP_SPAN];
@@ -347,8 +377,6 @@ __global__ void __launch_bounds__(ICP_SPAN) k_icp_wide_corr(IcpWideView w, int i
   const int sp = s - (int)w.span_first[k];
   double *moved = v.moved + 3 * v.off[c];
   uint32_t *corr = v.corr + v.off[c];
-  const bool plane = v.est == 1 && pd.tn != nullptr;
-  const bool p2p = v.est == 0;
   const int t = threadIdx.x;
   const int i = sp * ICP_SPAN + t;
   if (i < ns) {
@@ -376,19 +404,18 @@ __global__ void __launch_bounds__(ICP_SPAN) k_icp_wide_corr(IcpWideView w, int i
   }
   __syncthreads();
   if (t >= 64) return;
-  double acc[NV];
+  double acc[NS];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (int k = 0; k < NS; ++k) acc[k] = 0.0;
   for (int u = t; u < ICP_SPAN && sp * ICP_SPAN + u < ns; u += 64) {
     uint32_t j = s_j[u];
     if (j == NONE) continue;
-    icp_accumulate(acc, s_p[3 * u], s_p[3 * u + 1], s_p[3 * u + 2], s_d2[u], pd.tp + 3 * (size_t)j,
-                   plane ? pd.tn + 3 * (size_t)j : nullptr, plane, p2p);
+    icp_accumulate<EST>(acc, s_p[3 * u], s_p[3 * u + 1], s_p[3 * u + 2], s_d2[u], pd.tp, pd.tn, j);
   }
-  wave_sum<NV>(acc);
+  wave_sum<NS>(acc);
   if (t == 0)
 #pragma unroll
-    for (int k = 0; k < NV; ++k) w.part[(size_t)s * NV + k] = acc[k];
+    for (int k = 0; k < NS; ++k) w.part[(size_t)s * NV + k] = acc[k];
 }
 
 // One wave per candidate: the pass's totals, the iteration's bookkeeping, and for point-to-plane
@@ -473,6 +500,24 @@ __global__ void __launch_bounds__(64) k_icp_wide_umeyama(IcpWideView w) {
 
 }  // namespace
 
+template <int EST>
+void launch_batch_sized(const IcpBatchView &v, int block, int grid, hipStream_t s) {
+  switch (block) {
+    case 64:
+      hipLaunchKernelGGL((k_icp_batch<64, EST>), dim3(grid), dim3(64), 0, s, v);
+      break;
+    case 128:
+      hipLaunchKernelGGL((k_icp_batch<128, EST>), dim3(grid), dim3(128), 0, s, v);
+      break;
+    case 512:
+      hipLaunchKernelGGL((k_icp_batch<512, EST>), dim3(grid), dim3(512), 0, s, v);
+      break;
+    default:
+      hipLaunchKernelGGL((k_icp_batch<256, EST>), dim3(grid), dim3(256), 0, s, v);
+      break;
+  }
+}
+
 int icp_block_sizes(int *out) {
   static const int sizes[] = {64, 128, 256, 512};
   for (int k = 0; k < 4; ++k) out[k] = sizes[k];
@@ -481,20 +526,10 @@ int icp_block_sizes(int *out) {
 
 hipError_t launch_icp_batch(const IcpBatchView &v, int block, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_icp_reset, dim3(1), dim3(1), 0, s, v.next);
-  switch (block) {
-    case 64:
-      hipLaunchKernelGGL(k_icp_batch<64>, dim3(grid), dim3(64), 0, s, v);
-      break;
-    case 128:
-      hipLaunchKernelGGL(k_icp_batch<128>, dim3(grid), dim3(128), 0, s, v);
-      break;
-    case 512:
-      hipLaunchKernelGGL(k_icp_batch<512>, dim3(grid), dim3(512), 0, s, v);
-      break;
-    default:
-      hipLaunchKernelGGL(k_icp_batch<256>, dim3(grid), dim3(256), 0, s, v);
-      break;
-  }
+  if (v.est == EST_P2P)
+    launch_batch_sized<EST_P2P>(v, block, grid, s);
+  else
+    launch_batch_sized<EST_PLANE>(v, block, grid, s);
   return hipGetLastError();
 }
 
@@ -504,7 +539,10 @@ hipError_t launch_icp_wide(const IcpWideView &w, hipStream_t s, KernelProfiler *
   hipLaunchKernelGGL(k_icp_wide_init, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, w);
   for (int it = 0; it <= w.b.max_iter; ++it) {
     if (prof) prof->begin("k_icp_wide_corr", s);
-    hipLaunchKernelGGL(k_icp_wide_corr, dim3((unsigned)w.n_spans), dim3(ICP_SPAN), 0, s, w, it);
+    if (w.b.est == EST_P2P)
+      hipLaunchKernelGGL(k_icp_wide_corr<EST_P2P>, dim3((unsigned)w.n_spans), dim3(ICP_SPAN), 0, s, w, it);
+    else
+      hipLaunchKernelGGL(k_icp_wide_corr<EST_PLANE>, dim3((unsigned)w.n_spans), dim3(ICP_SPAN), 0, s, w, it);
     if (prof) prof->end(s);
     hipLaunchKernelGGL(k_icp_wide_step, dim3((unsigned)n), dim3(64), 0, s, w, it);
     if (w.b.est == 0 && it < w.b.max_iter) {
