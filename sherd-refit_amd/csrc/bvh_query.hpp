@@ -358,6 +358,37 @@ __device__ inline float min_d2(const SceneView &sv, V3 p, float window, int step
   return best;
 }
 
+// The minimum over the triangles of node c's subtree within window (+inf when none is): the
+// stackless pre-order walk from c to the end of its subtree, which is where c's escape link points
+// (or the end of the tree): every escape taken inside the subtree lands at most there.
+__device__ inline float min_d2_subtree(const SceneView &sv, int c, V3 p, float window) {
+  float best = __builtin_inff();
+  float slack = slack_of(sv.absmax, p);
+  float lim = window + slack;
+  float lim2 = lim * lim;
+  int nodes = 2 * sv.n - 1, steps = 0;
+  if (c < 0 || c >= nodes) return best;
+  const int end = load_node(sv.nodes, c).esc;
+  for (int i = c; i >= 0 && i < nodes && i != end && steps++ < nodes;) {
+    Node nd = load_node(sv.nodes, i);
+    if (nd.a >= 0) {
+      i = box_d2(nd, p, slack) <= lim2 ? i + 1 : nd.esc;
+    } else {
+      V3 a, b, cc;
+      uint32_t face;
+      load_tri(sv.tris, ~nd.a, &a, &b, &cc, &face);
+      float d2 = tri_d2(p, a, b, cc);
+      if (d2 < best) {
+        best = d2;
+        float l = mnf(sqrtf(best), window) + slack;
+        lim2 = l * l;
+      }
+      i = nd.esc;
+    }
+  }
+  return best;
+}
+
 // min_d2 for one point by a whole wave: p and window are wave-uniform, and the wave keeps its
 // pending nodes in a stack in LDS (stk, cap entries, this wave's own), expanding up to 64 of them
 // per step, one per lane. Children are pushed when their box is within the current bound, leaves
@@ -409,18 +440,12 @@ __device__ inline float min_d2_wave(const SceneView &sv, V3 p, float window, int
       if (c1 >= 0) stk[sp + n0 + (int)lanes_below(m1)] = c1;
       sp += n0 + n1;
     } else {
-      // No room: each lane searches its own children's subtrees (the same bound).
+      // No room: each lane walks its own children's subtrees (the same bound).
       float w = mnf(sqrtf(best), window);
       float own = __builtin_inff();
-      if (c0 >= 0) {
-        SceneView sub = sv;
-        sub.root = c0;
-        own = min_d2(sub, p, w);
-      }
+      if (c0 >= 0) own = min_d2_subtree(sv, c0, p, w);
       if (c1 >= 0) {
-        SceneView sub = sv;
-        sub.root = c1;
-        float e = min_d2(sub, p, w);
+        float e = min_d2_subtree(sv, c1, p, w);
         own = e < own ? e : own;
       }
       uint32_t ob = wave_min(__float_as_uint(own));

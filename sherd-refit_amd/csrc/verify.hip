@@ -32,7 +32,7 @@ namespace {
 enum { S_T, S_RES, S_COARSE, S_STAGE1, S_TIGHT, S_FACING, S_GAP, S_SEAM, S_NEAR, S_PEN, S_NMS, S_ICP };
 
 constexpr int VT = 256;
-constexpr int PEN_STEPS = 192;     // nodes one lane walks for a closest-approach query before a wave takes over
+constexpr int PEN_STEPS = 64;      // nodes one lane walks for a closest-approach query before a wave takes over
 constexpr int HARD_STACK = 2048;   // pending nodes of a wave's cooperative query (LDS, 8 KiB per wave)
 constexpr int HARD_BLOCKS = 1024;  // workgroups of the cooperative pass
 
