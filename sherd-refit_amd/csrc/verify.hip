@@ -363,10 +363,11 @@ __device__ K select_kth(const K *keys, int n, int k, uint32_t *hist, K *s_prefix
 }
 
 // The closest approach is a minimum over every sample, and a search is cheap once a small
-// window is known. Before the exact pass, one sample per wave walks once from the root to a single
-// leaf, always into the nearer child, and measures that one triangle: a distance the sample really
-// attains, so the smallest of them is at least the true minimum and makes a window the exact pass
-// can prune against. The minimum that comes out is the same; only the work changes.
+// window is known. Before the exact pass, one sample per workgroup walks once from the root to a
+// single leaf, always into the nearer child, and measures that one triangle: a distance the sample
+// really attains, so the smallest of them is at least the true minimum and makes a window the
+// exact pass can prune against. The minimum that comes out is the same; only the work changes.
+// (Every workgroup belongs to one task, so its early returns are uniform.)
 __global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb, int per_task) {
   TaskPos tp = task_pos(vb, per_task);
   int c = tp.c, side = tp.side, i = tp.i;
@@ -389,11 +390,24 @@ __global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb, int per_task) {
     window = __uint_as_float(__hip_atomic_load(&acc.closest[side], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     lb = __float_as_uint(dist_bound(sv.db, q));
   }
-  // One descent per wave, from the sample with the smallest distance bound (the likeliest to be
-  // close), and none when even that one cannot beat the best bound so far.
+  // One descent per workgroup, from the sample with the smallest distance bound (the likeliest to
+  // be close), and none when even that one cannot beat the best bound so far.
+  __shared__ uint32_t s_lb[VT / 64];
   uint32_t wlb = wave_min(lb);
-  uint64_t holders = __ballot(lb == wlb);
-  bool go = lb != 0xffffffffu && lanes_below(holders) == 0 && lb == wlb && __uint_as_float(lb) <= window;
+  if ((threadIdx.x & 63) == 0) s_lb[threadIdx.x >> 6] = wlb;
+  __syncthreads();
+  uint32_t blb = s_lb[0];
+  int wsel = 0;
+#pragma unroll
+  for (int w = 1; w < VT / 64; ++w) {
+    if (s_lb[w] < blb) {
+      blb = s_lb[w];
+      wsel = w;
+    }
+  }
+  uint64_t holders = __ballot(lb == blb);
+  bool go = (int)(threadIdx.x >> 6) == wsel && lb != 0xffffffffu && lb == blb && lanes_below(holders) == 0 &&
+            __uint_as_float(lb) <= window;
   if (go) {
     int nodes = 2 * sv.n - 1;
     int cur = sv.root;
