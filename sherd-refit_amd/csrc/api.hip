@@ -37,6 +37,25 @@ int hip_fail(hipError_t e, const char *what) {
   return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? SHERD_E_NOMEM : SHERD_E_HIP;
 }
 
+void *Ctx::pinned(size_t bytes, hipError_t *err) {
+  *err = hipSuccess;
+  if (host.n >= bytes && host.p) return host.p;
+  if (host.p) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipHostFree(host.p);
+    host.p = nullptr;
+    host.n = 0;
+  }
+  size_t want = bytes < 4096 ? 4096 : bytes + bytes / 4;
+  *err = hipHostMalloc(&host.p, want, hipHostMallocDefault);
+  if (*err != hipSuccess) {
+    host.p = nullptr;
+    return nullptr;
+  }
+  host.n = want;
+  return host.p;
+}
+
 void *Ctx::scratch(int slot, size_t bytes, hipError_t *err) {
   *err = hipSuccess;
   Buf &b = bufs[slot];
@@ -161,6 +180,7 @@ void sherd_close(sherd_ctx *ctx) {
   (void)hipStreamSynchronize(c->stream);
   for (auto &b : c->bufs)
     if (b.p) (void)hipFree(b.p);
+  if (c->host.p) (void)hipHostFree(c->host.p);
   if (c->ladder_exec) (void)hipGraphExecDestroy(c->ladder_exec);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);

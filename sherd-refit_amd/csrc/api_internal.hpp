@@ -37,6 +37,10 @@ struct Ctx {
 
   // A grow-only device buffer per slot, reused across calls (the context is serialised).
   void *scratch(int slot, size_t bytes, hipError_t *err);
+  // A grow-only pinned host buffer for uploads and downloads that must not go through pageable
+  // staging; its previous contents are not kept when it grows.
+  Buf host;
+  void *pinned(size_t bytes, hipError_t *err);
 };
 
 // Accumulates one method's call count, item count and device time (HIP events on the stream).

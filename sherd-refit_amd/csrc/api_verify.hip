@@ -237,12 +237,19 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     for (int k = 0; k < 12; ++k) pd[p].sc[k] = pairs[p].scales[k];
   }
   hipError_t e;
-  FragDesc *dfd = (FragDesc *)c->scratch(10, sizeof(FragDesc) * fd.size(), &e);
-  if (!dfd) return hip_fail(e, "scratch");
-  PairDesc *dpd = (PairDesc *)c->scratch(11, sizeof(PairDesc) * pd.size(), &e);
-  if (!dpd) return hip_fail(e, "scratch");
-  HIPCK(hipMemcpyAsync(dfd, fd.data(), sizeof(FragDesc) * fd.size(), hipMemcpyHostToDevice, c->stream), "upload");
-  HIPCK(hipMemcpyAsync(dpd, pd.data(), sizeof(PairDesc) * pd.size(), hipMemcpyHostToDevice, c->stream), "upload");
+  // Fragment and pair descriptors go up together from the front of the pinned staging buffer; the
+  // chunks stage after them (stage0), so a chunk never writes what a copy may still be reading.
+  auto up256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t fd_bytes = up256(sizeof(FragDesc) * fd.size()), stage0 = fd_bytes + up256(sizeof(PairDesc) * pd.size());
+  char *dfp = (char *)c->scratch(10, stage0, &e);
+  if (!dfp) return hip_fail(e, "scratch");
+  char *h0 = (char *)c->pinned(stage0, &e);
+  if (!h0) return hip_fail(e, "pinned staging");
+  std::memcpy(h0, fd.data(), sizeof(FragDesc) * fd.size());
+  std::memcpy(h0 + fd_bytes, pd.data(), sizeof(PairDesc) * pd.size());
+  HIPCK(hipMemcpyAsync(dfp, h0, stage0, hipMemcpyHostToDevice, c->stream), "upload");
+  FragDesc *dfd = (FragDesc *)dfp;
+  PairDesc *dpd = (PairDesc *)(dfp + fd_bytes);
   Timer tm(c, SHERD_M_VERIFY, n_cand);
   // Chunks bounded by the key and voxel-table scratch they need (256 MiB of keys per chunk).
   const size_t KEY_BUDGET = (size_t)1 << 25;
@@ -254,9 +261,6 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
   size_t begin = 0;
   std::vector<uint32_t> task;
   std::vector<CandDesc> cd;
-  std::vector<double> hs;
-  std::vector<uint8_t> ha;
-  std::vector<CandAcc> hacc;
   while (begin < n_cand) {
     cd.clear();
     size_t keys = 0, table = 0;
@@ -285,18 +289,6 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
       ++end;
     }
     size_t nc = cd.size();
-    CandDesc *dcd = (CandDesc *)c->scratch(12, sizeof(CandDesc) * nc, &e);
-    if (!dcd) return hip_fail(e, "scratch");
-    CandAcc *dacc = (CandAcc *)c->scratch(13, sizeof(CandAcc) * nc, &e);
-    if (!dacc) return hip_fail(e, "scratch");
-    uint64_t *dkeys = (uint64_t *)c->scratch(14, sizeof(uint64_t) * (keys ? keys : 1), &e);
-    if (!dkeys) return hip_fail(e, "scratch keys");
-    uint64_t *dtab = (uint64_t *)c->scratch(15, sizeof(uint64_t) * table, &e);
-    if (!dtab) return hip_fail(e, "scratch seam tables");
-    double *dsc = (double *)c->scratch(16, sizeof(double) * VS_COUNT * nc, &e);
-    if (!dsc) return hip_fail(e, "scratch");
-    uint8_t *dac = (uint8_t *)c->scratch(17, nc, &e);
-    if (!dac) return hip_fail(e, "scratch");
     // (candidate, side) tasks grouped by the fragment they query: side 0 queries B, side 1 A.
     task.resize(2 * nc);
     for (size_t i = 0; i < 2 * nc; ++i) task[i] = (uint32_t)i;
@@ -305,15 +297,36 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
       return (t & 1u) ? p.a : p.b;
     };
     std::stable_sort(task.begin(), task.end(), [&](uint32_t x, uint32_t y) { return queried(x) < queried(y); });
-    uint32_t *dtask = (uint32_t *)c->scratch(18, sizeof(uint32_t) * 2 * nc, &e);
-    if (!dtask) return hip_fail(e, "scratch");
+    // What goes up (descriptors, then tasks) and what comes back (accumulators, scores, verdicts)
+    // each sit in one device piece, so one pinned copy moves each way.
+    const size_t up_cd = 0, up_task = up256(sizeof(CandDesc) * nc), up_bytes = up_task + sizeof(uint32_t) * 2 * nc;
+    const size_t dn_acc = 0, dn_sc = up256(sizeof(CandAcc) * nc), dn_ac = dn_sc + up256(sizeof(double) * VS_COUNT * nc),
+                 dn_bytes = dn_ac + nc;
+    char *dup = (char *)c->scratch(12, up_bytes, &e);
+    if (!dup) return hip_fail(e, "scratch");
+    char *ddn = (char *)c->scratch(13, dn_bytes, &e);
+    if (!ddn) return hip_fail(e, "scratch");
+    // (Growing the buffer waits for the stream first, so the descriptors' copy has finished.)
+    char *hpin = (char *)c->pinned(stage0 + std::max(up_bytes, dn_bytes), &e);
+    if (!hpin) return hip_fail(e, "pinned staging");
+    char *hbuf = hpin + stage0;
+    CandDesc *dcd = (CandDesc *)(dup + up_cd);
+    uint32_t *dtask = (uint32_t *)(dup + up_task);
+    CandAcc *dacc = (CandAcc *)(ddn + dn_acc);
+    double *dsc = (double *)(ddn + dn_sc);
+    uint8_t *dac = (uint8_t *)(ddn + dn_ac);
+    uint64_t *dkeys = (uint64_t *)c->scratch(14, sizeof(uint64_t) * (keys ? keys : 1), &e);
+    if (!dkeys) return hip_fail(e, "scratch keys");
+    uint64_t *dtab = (uint64_t *)c->scratch(15, sizeof(uint64_t) * table, &e);
+    if (!dtab) return hip_fail(e, "scratch seam tables");
     uint64_t *dhard = (uint64_t *)c->scratch(19, sizeof(uint64_t) * 2 * nc * (size_t)std::max(max_s, 1), &e);
     if (!dhard) return hip_fail(e, "scratch");
     uint32_t *dhard_n = (uint32_t *)c->scratch(20, sizeof(uint32_t) * 2, &e);
     if (!dhard_n) return hip_fail(e, "scratch");
-    HIPCK(hipMemcpyAsync(dcd, cd.data(), sizeof(CandDesc) * nc, hipMemcpyHostToDevice, c->stream), "upload");
-    HIPCK(hipMemcpyAsync(dtask, task.data(), sizeof(uint32_t) * 2 * nc, hipMemcpyHostToDevice, c->stream),
-          "upload");
+    // The staging buffer is free: the previous chunk's download was waited for.
+    std::memcpy(hbuf + up_cd, cd.data(), sizeof(CandDesc) * nc);
+    std::memcpy(hbuf + up_task, task.data(), sizeof(uint32_t) * 2 * nc);
+    HIPCK(hipMemcpyAsync(dup, hbuf, up_bytes, hipMemcpyHostToDevice, c->stream), "upload");
     VerifyBatch vb;
     vb.frag = dfd;
     vb.pair = dpd;
@@ -330,18 +343,15 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     vb.full = full ? 1 : 0;
     vb.retest = retest ? 1 : 0;
     HIPCK(launch_verify(vb, max_pf, max_s, max_margin, max_brk, c->stream, &c->prof), "verify kernels");
-    hs.resize(VS_COUNT * nc);
-    ha.resize(nc);
-    hacc.resize(nc);
-    HIPCK(hipMemcpyAsync(hs.data(), dsc, sizeof(double) * VS_COUNT * nc, hipMemcpyDeviceToHost, c->stream), "download");
-    HIPCK(hipMemcpyAsync(ha.data(), dac, nc, hipMemcpyDeviceToHost, c->stream), "download");
-    HIPCK(hipMemcpyAsync(hacc.data(), dacc, sizeof(CandAcc) * nc, hipMemcpyDeviceToHost, c->stream), "download");
+    // Stream order puts the download after the upload has read the staging buffer.
+    HIPCK(hipMemcpyAsync(hbuf, ddn, dn_bytes, hipMemcpyDeviceToHost, c->stream), "download");
     HIPCK(hipStreamSynchronize(c->stream), "synchronize");
     c->prof.collect();
+    const CandAcc *hacc = (const CandAcc *)(hbuf + dn_acc);
     for (size_t i = 0; i < nc; ++i)
       if (hacc[i].flags & FLAG_VOXEL_RANGE) return fail(SHERD_E_LIMIT, "a seam voxel coordinate exceeds 2^20");
-    std::memcpy(scores + VS_COUNT * begin, hs.data(), sizeof(double) * VS_COUNT * nc);
-    std::memcpy(accepted + begin, ha.data(), nc);
+    std::memcpy(scores + VS_COUNT * begin, hbuf + dn_sc, sizeof(double) * VS_COUNT * nc);
+    std::memcpy(accepted + begin, hbuf + dn_ac, nc);
     begin = end;
   }
   tm.stop();
