@@ -48,6 +48,20 @@ __global__ void k_point_tris(const double *p, int n, float *V, uint32_t *F) {
   }
 }
 
+// After the build a point tree's leaf records need only their point: each record's first 24 bytes
+// take the point's f64 coordinates (the f32 copies there served the build and nothing reads them
+// again) and `face` keeps its index, so a leaf costs one fetch (cloud_nearest).
+__global__ void k_point_records(Tri *t, int n, const double *p) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  uint32_t j = t[s].face;
+  double x = p[3 * (size_t)j], y = p[3 * (size_t)j + 1], z = p[3 * (size_t)j + 2];
+  double *d = reinterpret_cast<double *>(t + s);
+  d[0] = x;
+  d[1] = y;
+  d[2] = z;
+}
+
 }  // namespace
 
 void sherd::free_cloud(sherd_cloud *cl) {
@@ -69,7 +83,10 @@ hipError_t sherd::point_tree(Ctx *c, sherd_cloud *cl) {
     hipLaunchKernelGGL(k_point_tris, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, cl->pts, (int)n, V,
                        F);
     if ((e = hipGetLastError()) != hipSuccess) break;
-    e = scene_build(V, F, (int)n, &cl->tree, c->stream);
+    if ((e = scene_build(V, F, (int)n, &cl->tree, c->stream)) != hipSuccess) break;
+    hipLaunchKernelGGL(k_point_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, cl->tree.tris,
+                       (int)n, cl->pts);
+    e = hipGetLastError();
   } while (0);
   if (V) (void)hipFree(V);
   if (F) (void)hipFree(F);

@@ -13,6 +13,16 @@ __device__ __forceinline__ double point_d2(const double *pts, uint32_t j, double
   return d2_f64(q0, q1, q2, gldd(p), gldd(p + 1), gldd(p + 2));
 }
 
+// A point tree's leaf record at `slot`: after the build its first 24 bytes hold the point's f64
+// coordinates and `face` its index (k_point_records), all fetched at once.
+__device__ __forceinline__ double leaf_d2(const Tri *tris, int slot, double q0, double q1, double q2, uint32_t *j) {
+  const Tri *r = tris + slot;
+  const double *d = reinterpret_cast<const double *>(r);
+  double x = gldd(d), y = gldd(d + 1), z = gldd(d + 2);
+  *j = gldu(&r->face);
+  return d2_f64(q0, q1, q2, x, y, z);
+}
+
 // The nearest point of a cloud to q with d^2 < r2, ties to the lowest index. The cloud's tree is
 // an LBVH over its points (each leaf a point, `face` its index); boxes are f32 and grown by the
 // slack, which exceeds the rounding of the f32 boxes and of q, so no box hides a point nearer than
@@ -51,10 +61,8 @@ __device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, dou
   for (int iter = 0; cur >= 0 && iter <= nodes; ++iter) {
     int next = -1;
     if (nd.a < 0) {
-      V3 a, b, c;
       uint32_t j;
-      load_tri(sv.tris, ~nd.a, &a, &b, &c, &j);
-      double d2 = point_d2(pts, j, q0, q1, q2);
+      double d2 = leaf_d2(sv.tris, ~nd.a, q0, q1, q2, &j);
       if (d2 < best || (d2 == best && j < bj)) {
         best = d2;
         bj = j;
@@ -104,10 +112,8 @@ __device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, dou
       if (nd.a >= 0) {
         i = box_d2(nd, qf, slack) <= lim2 ? i + 1 : nd.esc;
       } else {
-        V3 a, b, c;
         uint32_t j;
-        load_tri(sv.tris, ~nd.a, &a, &b, &c, &j);
-        double d2 = point_d2(pts, j, q0, q1, q2);
+        double d2 = leaf_d2(sv.tris, ~nd.a, q0, q1, q2, &j);
         if (d2 < best || (d2 == best && j < bj)) {
           best = d2;
           bj = j;
