@@ -67,32 +67,32 @@ __global__ void k_gather(const double *p, const uint32_t *order, int n, double *
   sidx[j] = i;
 }
 
-__global__ void k_insert(const uint64_t *key, int n, uint64_t *tkeys, uint2 *cells, uint32_t mask) {
+__global__ void k_insert(const uint64_t *key, int n, GridSlot *table, uint32_t mask) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   uint64_t k = key[j];
   if (j > 0 && key[j - 1] == k) return;
   uint32_t slot = grid_hash(k, mask);
   for (uint32_t probe = 0; probe <= mask; ++probe) {
-    unsigned long long prev = atomicCAS((unsigned long long *)&tkeys[slot], (unsigned long long)GRID_EMPTY,
+    unsigned long long prev = atomicCAS((unsigned long long *)&table[slot].key, (unsigned long long)GRID_EMPTY,
                                         (unsigned long long)k);
     if (prev == GRID_EMPTY) {
-      cells[slot].x = (uint32_t)j;
+      table[slot].start = (uint32_t)j;
       return;
     }
     slot = (slot + 1) & mask;
   }
 }
 
-__global__ void k_counts(const uint64_t *key, int n, const uint64_t *tkeys, uint2 *cells, uint32_t mask) {
+__global__ void k_counts(const uint64_t *key, int n, GridSlot *table, uint32_t mask) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   uint64_t k = key[j];
   if (j + 1 < n && key[j + 1] == k) return;
   uint32_t slot = grid_hash(k, mask);
   for (uint32_t probe = 0; probe <= mask; ++probe) {
-    if (tkeys[slot] == k) {
-      cells[slot].y = (uint32_t)(j + 1) - cells[slot].x;
+    if (table[slot].key == k) {
+      table[slot].count = (uint32_t)(j + 1) - table[slot].start;
       return;
     }
     slot = (slot + 1) & mask;
@@ -166,17 +166,16 @@ hipError_t grid_build(const double *dp, int n, double reach, DevGrid *g, hipStre
     if ((e = hipMalloc(&hist, sizeof(uint32_t) * rsort::hist_words(n))) != hipSuccess) break;
     if ((e = hipMalloc(&g->pts, sizeof(double) * 3 * n)) != hipSuccess) break;
     if ((e = hipMalloc(&g->idx, sizeof(uint32_t) * n)) != hipSuccess) break;
-    if ((e = hipMalloc(&g->keys, sizeof(uint64_t) * cap)) != hipSuccess) break;
-    if ((e = hipMalloc(&g->cells, sizeof(uint2) * cap)) != hipSuccess) break;
+    if ((e = hipMalloc(&g->table, sizeof(GridSlot) * cap)) != hipSuccess) break;
     hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, BT)), dim3(BT), 0, s, dp, n, g->origin[0], g->origin[1],
                        g->origin[2], cell, key, val);
     if ((e = hipGetLastError()) != hipSuccess) break;
     if ((e = rsort::sort_pairs<uint64_t>(key, val, tkey, tval, hist, n, 0, 3 * GRID_BITS, s)) != hipSuccess) break;
     hipLaunchKernelGGL(k_gather, dim3(blocks_for(n, BT)), dim3(BT), 0, s, dp, val, n, g->pts, g->idx);
-    if ((e = hipMemsetAsync(g->keys, 0xFF, sizeof(uint64_t) * cap, s)) != hipSuccess) break;
-    if ((e = hipMemsetAsync(g->cells, 0, sizeof(uint2) * cap, s)) != hipSuccess) break;
-    hipLaunchKernelGGL(k_insert, dim3(blocks_for(n, BT)), dim3(BT), 0, s, key, n, g->keys, g->cells, g->mask);
-    hipLaunchKernelGGL(k_counts, dim3(blocks_for(n, BT)), dim3(BT), 0, s, key, n, g->keys, g->cells, g->mask);
+    // Every byte 0xFF: free keys (and runs that no probe reads).
+    if ((e = hipMemsetAsync(g->table, 0xFF, sizeof(GridSlot) * cap, s)) != hipSuccess) break;
+    hipLaunchKernelGGL(k_insert, dim3(blocks_for(n, BT)), dim3(BT), 0, s, key, n, g->table, g->mask);
+    hipLaunchKernelGGL(k_counts, dim3(blocks_for(n, BT)), dim3(BT), 0, s, key, n, g->table, g->mask);
     e = hipGetLastError();
   } while (0);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -194,20 +193,17 @@ hipError_t grid_build(const double *dp, int n, double reach, DevGrid *g, hipStre
 void grid_free(DevGrid *g) {
   if (g->pts) (void)hipFree(g->pts);
   if (g->idx) (void)hipFree(g->idx);
-  if (g->keys) (void)hipFree(g->keys);
-  if (g->cells) (void)hipFree(g->cells);
+  if (g->table) (void)hipFree(g->table);
   g->pts = nullptr;
   g->idx = nullptr;
-  g->keys = nullptr;
-  g->cells = nullptr;
+  g->table = nullptr;
 }
 
 GridView view_of(const DevGrid &d) {
   GridView v;
   v.pts = d.pts;
   v.idx = d.idx;
-  v.keys = d.keys;
-  v.cells = d.cells;
+  v.table = d.table;
   v.mask = d.mask;
   v.n = d.n;
   for (int k = 0; k < 3; ++k) v.origin[k] = d.origin[k];

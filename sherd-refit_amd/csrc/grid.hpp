@@ -16,12 +16,18 @@ constexpr uint64_t GRID_EMPTY = ~0ull;
 constexpr int GRID_BITS = 21;  // per axis in the packed key
 constexpr int64_t GRID_MAXC = (1ll << 20);  // cell coordinates stay below this
 
+// One slot of the open-addressed cell table: the packed cell key (GRID_EMPTY when free) and the
+// cell's run of sorted points, read together in one 16-byte load.
+struct alignas(16) GridSlot {
+  uint64_t key;
+  uint32_t start, count;
+};
+
 struct GridView {
-  const double *pts;    // sorted by cell, 3 doubles each
-  const uint32_t *idx;  // original index of each sorted point
-  const uint64_t *keys; // hash table keys, GRID_EMPTY when free
-  const uint2 *cells;   // (start, count) per table slot
-  uint32_t mask;        // table capacity - 1
+  const double *pts;       // sorted by cell, 3 doubles each
+  const uint32_t *idx;     // original index of each sorted point
+  const GridSlot *table;   // mask + 1 slots
+  uint32_t mask;           // table capacity - 1
   int n;
   double origin[3];
   double cell;
@@ -38,6 +44,53 @@ SH_HD uint32_t grid_hash(uint64_t k, uint32_t mask) {
   return (uint32_t)k & mask;
 }
 
+// A table slot in one load: (key, start, count).
+__device__ __forceinline__ GridSlot load_slot(const GridSlot *t, uint32_t i) {
+  typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+  u32x4_t v = *(__attribute__((address_space(1))) const u32x4_t *)(t + i);
+  GridSlot s;
+  s.key = (uint64_t)v.x | ((uint64_t)v.y << 32);
+  s.start = v.z;
+  s.count = v.w;
+  return s;
+}
+
+// The points of the 27 cells around cell (cx, cy, cz), a slab of nine cells at a time: the nine
+// table probes are issued together, each bringing its key and its run of points at once, so a
+// slab costs one memory round trip rather than eighteen (a probe that lands on another key walks
+// on alone, exactly as the table was filled). visit(j) is called for every sorted point j of every occupied cell; the
+// callers keep an order-independent minimum, so the order of the calls does not matter.
+template <class Visit>
+__device__ __forceinline__ void grid_scan27(const GridView &g, int64_t cx, int64_t cy, int64_t cz, Visit &&visit) {
+  for (int dx = -1; dx <= 1; ++dx) {
+    const int64_t x = cx + dx;
+    if (x < 0 || x >= GRID_MAXC) continue;
+    uint64_t want[9];
+    GridSlot got[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int64_t y = cy + (t / 3 - 1), z = cz + (t % 3 - 1);
+      const bool ok = y >= 0 && y < GRID_MAXC && z >= 0 && z < GRID_MAXC;
+      want[t] = ok ? grid_pack(x, y, z) : GRID_EMPTY;  // a packed cell is never GRID_EMPTY
+      got[t] = GridSlot{GRID_EMPTY, 0u, 0u};
+      if (ok) got[t] = load_slot(g.table, grid_hash(want[t], g.mask));
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      uint32_t slot = grid_hash(want[t], g.mask);
+      for (uint32_t probe = 0;
+           want[t] != GRID_EMPTY && got[t].key != want[t] && got[t].key != GRID_EMPTY && probe < g.mask; ++probe) {
+        slot = (slot + 1) & g.mask;
+        got[t] = load_slot(g.table, slot);
+      }
+      if (got[t].key != want[t]) got[t].count = 0;
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+      for (uint32_t k = got[t].start; k < got[t].start + got[t].count; ++k) visit(k);
+  }
+}
+
 // The global nearest point to q among the 27 cells around it, ties to the lowest original index.
 // Returns false when those cells hold no point.
 __device__ __forceinline__ bool grid_nearest(const GridView &g, double qx, double qy, double qz, uint32_t *out_idx,
@@ -50,40 +103,17 @@ __device__ __forceinline__ bool grid_nearest(const GridView &g, double qx, doubl
   if (!(fx >= -1.0 && fx <= (double)GRID_MAXC && fy >= -1.0 && fy <= (double)GRID_MAXC && fz >= -1.0 &&
         fz <= (double)GRID_MAXC))
     return false;
-  int64_t cx = (int64_t)fx, cy = (int64_t)fy, cz = (int64_t)fz;
   double best = __builtin_inf();
   uint32_t bi = 0xFFFFFFFFu;
-  for (int dx = -1; dx <= 1; ++dx) {
-    int64_t x = cx + dx;
-    if (x < 0 || x >= GRID_MAXC) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      int64_t y = cy + dy;
-      if (y < 0 || y >= GRID_MAXC) continue;
-      for (int dz = -1; dz <= 1; ++dz) {
-        int64_t z = cz + dz;
-        if (z < 0 || z >= GRID_MAXC) continue;
-        uint64_t key = grid_pack(x, y, z);
-        uint32_t slot = grid_hash(key, g.mask);
-        for (uint32_t probe = 0; probe <= g.mask; ++probe) {
-          uint64_t k = g.keys[slot];
-          if (k == GRID_EMPTY) break;
-          if (k == key) {
-            uint2 c = g.cells[slot];
-            for (uint32_t j = c.x; j < c.x + c.y; ++j) {
-              double d2 = d2_f64(qx, qy, qz, g.pts[3 * j], g.pts[3 * j + 1], g.pts[3 * j + 2]);
-              uint32_t oi = g.idx[j];
-              if (d2 < best || (d2 == best && oi < bi)) {
-                best = d2;
-                bi = oi;
-              }
-            }
-            break;
-          }
-          slot = (slot + 1) & g.mask;
-        }
-      }
+  grid_scan27(g, (int64_t)fx, (int64_t)fy, (int64_t)fz, [&](uint32_t j) {
+    const double *p = g.pts + 3 * (size_t)j;
+    double d2 = d2_f64(qx, qy, qz, gldd(p), gldd(p + 1), gldd(p + 2));
+    uint32_t oi = gldu(g.idx + j);
+    if (d2 < best || (d2 == best && oi < bi)) {
+      best = d2;
+      bi = oi;
     }
-  }
+  });
   if (bi == 0xFFFFFFFFu) return false;
   *out_idx = bi;
   *out_d2 = best;

@@ -69,8 +69,7 @@ hipError_t launch_bounded_min(const SceneView &sv, const double *pts, int n, con
 struct DevGrid {
   double *pts = nullptr;
   uint32_t *idx = nullptr;
-  uint64_t *keys = nullptr;
-  uint2 *cells = nullptr;
+  GridSlot *table = nullptr;
   uint32_t mask = 0;
   int n = 0;
   double origin[3] = {0, 0, 0};

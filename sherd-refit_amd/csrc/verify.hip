@@ -127,42 +127,18 @@ __device__ inline bool seam_nearest(const GridView &g, const double *T, const do
   if (!(fx >= -1.0 && fx <= (double)GRID_MAXC && fy >= -1.0 && fy <= (double)GRID_MAXC && fz >= -1.0 &&
         fz <= (double)GRID_MAXC))
     return false;
-  int64_t cx = (int64_t)fx, cy = (int64_t)fy, cz = (int64_t)fz;
   double best = __builtin_inf();
   uint32_t bj = 0xFFFFFFFFu;
-  for (int dx = -1; dx <= 1; ++dx) {
-    int64_t x = cx + dx;
-    if (x < 0 || x >= GRID_MAXC) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      int64_t y = cy + dy;
-      if (y < 0 || y >= GRID_MAXC) continue;
-      for (int dz = -1; dz <= 1; ++dz) {
-        int64_t z = cz + dz;
-        if (z < 0 || z >= GRID_MAXC) continue;
-        uint64_t key = grid_pack(x, y, z);
-        uint32_t slot = grid_hash(key, g.mask);
-        for (uint32_t probe = 0; probe <= g.mask; ++probe) {
-          uint64_t k = g.keys[slot];
-          if (k == GRID_EMPTY) break;
-          if (k == key) {
-            uint2 cl = g.cells[slot];
-            for (uint32_t j = cl.x; j < cl.x + cl.y; ++j) {
-              uint32_t oi = g.idx[j];
-              double m[3];
-              apply_f64(T, bp + 3 * (size_t)oi, m);
-              double d2 = d2_f64(p[0], p[1], p[2], m[0], m[1], m[2]);
-              if (d2 < best || (d2 == best && oi < bj)) {
-                best = d2;
-                bj = oi;
-              }
-            }
-            break;
-          }
-          slot = (slot + 1) & g.mask;
-        }
-      }
+  grid_scan27(g, (int64_t)fx, (int64_t)fy, (int64_t)fz, [&](uint32_t k) {
+    uint32_t oi = gldu(g.idx + k);
+    double m[3];
+    apply_f64(T, bp + 3 * (size_t)oi, m);
+    double d2 = d2_f64(p[0], p[1], p[2], m[0], m[1], m[2]);
+    if (d2 < best || (d2 == best && oi < bj)) {
+      best = d2;
+      bj = oi;
     }
-  }
+  });
   if (bj == 0xFFFFFFFFu) return false;
   *out_j = bj;
   *out_d2 = best;
