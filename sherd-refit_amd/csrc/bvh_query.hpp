@@ -11,6 +11,7 @@
 namespace sherd {
 
 constexpr int DSTACK = 24;  // deferred nodes of an ordered traversal
+constexpr int MSTACK = 16;  // the same for min_d2, whose overflow walk starts from the best found
 
 __device__ __forceinline__ Node load_node(const Node *nodes, int i) {
   const float4 *p = reinterpret_cast<const float4 *>(nodes + i);
@@ -297,8 +298,8 @@ __device__ inline float min_d2(const SceneView &sv, V3 p, float window, int step
   if (cur < 0 || cur >= nodes) return best;
   Node nd = load_node(sv.nodes, cur);
   if (box_d2(nd, p, slack) > lim2) return best;
-  int st_n[DSTACK];
-  float st_d[DSTACK];
+  int st_n[MSTACK];
+  float st_d[MSTACK];
   int sp = 0;
   bool overflow = false;
   for (int iter = 0; iter <= nodes; ++iter) {
@@ -324,7 +325,7 @@ __device__ inline float min_d2(const SceneView &sv, V3 p, float window, int step
       bool hl = dl <= lim2, hr = dr <= lim2;
       if (hl && hr) {
         bool lf = dl <= dr;
-        if (sp < DSTACK) {
+        if (sp < MSTACK) {
           st_n[sp] = lf ? r : l;
           st_d[sp] = lf ? dr : dl;
           ++sp;
@@ -352,7 +353,9 @@ __device__ inline float min_d2(const SceneView &sv, V3 p, float window, int step
     cur = next;
   }
   if (overflow) {
-    float e = min_d2_stackless(sv, p, window);
+    // The nearest triangle is no farther than the best met so far, so the exhaustive walk needs
+    // only that bound.
+    float e = min_d2_stackless(sv, p, mnf(sqrtf(best), window));
     best = e < best ? e : best;
   }
   return best;
