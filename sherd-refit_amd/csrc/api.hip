@@ -165,6 +165,9 @@ int sherd_open(int device, sherd_ctx **out) {
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipStreamCreateWithFlags(&c->side.side[k], hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.fork, hipEventDisableTiming);
+  for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&c->side.join[k], hipEventDisableTiming);
   if (e != hipSuccess) {
     delete c;
     return hip_fail(e, "stream/event creation");
@@ -184,6 +187,14 @@ void sherd_close(sherd_ctx *ctx) {
   if (c->ladder_exec) (void)hipGraphExecDestroy(c->ladder_exec);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
+  for (int k = 0; k < 3; ++k) {
+    if (c->side.side[k]) {
+      (void)hipStreamSynchronize(c->side.side[k]);
+      (void)hipStreamDestroy(c->side.side[k]);
+    }
+    if (c->side.join[k]) (void)hipEventDestroy(c->side.join[k]);
+  }
+  if (c->side.fork) (void)hipEventDestroy(c->side.fork);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }

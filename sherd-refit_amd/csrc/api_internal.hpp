@@ -17,6 +17,7 @@ struct Ctx {
   int cus = 0;
   std::string name;
   hipStream_t stream = nullptr;
+  SideStreams side;  // verification's independent sections
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::mutex mu;
   sherd_method_stats stats[SHERD_M_COUNT] = {};

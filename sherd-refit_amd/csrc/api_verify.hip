@@ -342,7 +342,7 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     vb.n_cand = (int)nc;
     vb.full = full ? 1 : 0;
     vb.retest = retest ? 1 : 0;
-    HIPCK(launch_verify(vb, max_pf, max_s, max_margin, max_brk, c->stream, &c->prof), "verify kernels");
+    HIPCK(launch_verify(vb, max_pf, max_s, max_margin, max_brk, c->stream, &c->side, &c->prof), "verify kernels");
     // Stream order puts the download after the upload has read the staging buffer.
     HIPCK(hipMemcpyAsync(hbuf, ddn, dn_bytes, hipMemcpyDeviceToHost, c->stream), "download");
     HIPCK(hipStreamSynchronize(c->stream), "synchronize");

@@ -227,9 +227,19 @@ hipError_t launch_place(const double *p, const double *n, int cnt, const Pose &T
                         hipStream_t s);
 
 // ---- verification (verify.hip) -------------------------------------------------------------------
+// Three more streams (default priority) and the events that fork them from the main stream and join
+// them back: the independent sections of a verification run side by side.
+struct SideStreams {
+  hipStream_t side[3] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[3] = {};
+};
+
 // max_pf / max_s / max_margin: the largest per-fragment sample counts in the batch (grid sizes).
+// With `ss`, R §6.1 (k_frac), R §6.2 (k_seam) and R §6.3 (k_cont) run on its side streams while
+// R §6.4 runs on s; without, everything runs on s in turn.
 hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_margin, int max_brk, hipStream_t s,
-                         KernelProfiler *prof);
+                         const SideStreams *ss, KernelProfiler *prof);
 
 // ---- cone vote (cone.hip) -------------------------------------------------------------------------
 struct ConeConst {

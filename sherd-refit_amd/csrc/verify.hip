@@ -551,60 +551,80 @@ inline int blocks_for(long n, int t) { return (int)((n + t - 1) / t); }
 }  // namespace
 
 hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_margin, int max_brk, hipStream_t s,
-                         KernelProfiler *prof) {
+                         const SideStreams *ss, KernelProfiler *prof) {
   if (vb.n_cand <= 0) return hipSuccess;
   int nc = vb.n_cand;
-  auto B = [&](const char *k) {
-    if (prof && prof->on) prof->begin(k, s);
+  auto B = [&](const char *k, hipStream_t st) {
+    if (prof && prof->on) prof->begin(k, st);
   };
-  auto E = [&]() {
-    if (prof && prof->on) prof->end(s);
+  auto E = [&](hipStream_t st) {
+    if (prof && prof->on) prof->end(st);
   };
-  B("k_prep");
+  B("k_prep", s);
   hipLaunchKernelGGL(k_prep, dim3(blocks_for(nc, 64)), dim3(64), 0, s, vb);
-  E();
+  E(s);
+  const bool pen = vb.full && max_s > 0;
+  const int per_s = blocks_for(max_s, VT);
+  if (pen) {
+    // R §6.4's inside pass first and alone: it is the longest kernel and fills the device.
+    B("k_pen", s);
+    hipLaunchKernelGGL(k_pen, dim3(2 * nc * per_s), dim3(VT), 0, s, vb, per_s);
+    E(s);
+  }
+  // The sections below read the poses k_prep completed and write disjoint accumulators and key
+  // segments, so they may run side by side with the rest of R §6.4, whose closest-approach
+  // passes leave most of the device idle; k_median waits for all of them.
+  hipStream_t s1 = s, s2 = s, s3 = s;
+  if (ss) {
+    hipError_t e = hipEventRecord(ss->fork, s);
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipStreamWaitEvent(ss->side[k], ss->fork, 0);
+    if (e != hipSuccess) return e;
+    s1 = ss->side[0];
+    s2 = ss->side[1];
+    s3 = ss->side[2];
+  }
   if (max_pf > 0) {
     int per = blocks_for(max_pf, VT);
-    B("k_frac");
-    hipLaunchKernelGGL(k_frac, dim3(2 * nc * per), dim3(VT), 0, s, vb, per);
-    E();
+    B("k_frac", s1);
+    hipLaunchKernelGGL(k_frac, dim3(2 * nc * per), dim3(VT), 0, s1, vb, per);
+    E(s1);
   }
-  B("k_seam_clear");
-  hipLaunchKernelGGL(k_seam_clear, dim3(nc), dim3(VT), 0, s, vb);
-  E();
+  B("k_seam_clear", s2);
+  hipLaunchKernelGGL(k_seam_clear, dim3(nc), dim3(VT), 0, s2, vb);
+  E(s2);
   if (max_brk > 0) {
-    B("k_seam");
-    hipLaunchKernelGGL(k_seam, dim3(blocks_for(max_brk, VT), nc), dim3(VT), 0, s, vb);
-    E();
+    B("k_seam", s2);
+    hipLaunchKernelGGL(k_seam, dim3(blocks_for(max_brk, VT), nc), dim3(VT), 0, s2, vb);
+    E(s2);
   }
-  if (vb.full) {
-    if (max_margin > 0) {
-      B("k_cont");
-      hipLaunchKernelGGL(k_cont, dim3(blocks_for(max_margin, VT), nc), dim3(VT), 0, s, vb);
-      E();
-    }
-    if (max_s > 0) {
-      int per = blocks_for(max_s, VT);
-      B("k_pen");
-      hipLaunchKernelGGL(k_pen, dim3(2 * nc * per), dim3(VT), 0, s, vb, per);
-      E();
-      B("k_pen_seed");
-      hipLaunchKernelGGL(k_pen_seed, dim3(2 * nc * per), dim3(VT), 0, s, vb, per);
-      E();
-      B("k_pen_min");
-      hipLaunchKernelGGL(k_pen_min, dim3(2 * nc * per), dim3(VT), 0, s, vb, per);
-      E();
-      B("k_pen_hard");
-      hipLaunchKernelGGL(k_pen_hard, dim3(HARD_BLOCKS), dim3(VT), 0, s, vb);
-      E();
-    }
+  if (vb.full && max_margin > 0) {
+    B("k_cont", s3);
+    hipLaunchKernelGGL(k_cont, dim3(blocks_for(max_margin, VT), nc), dim3(VT), 0, s3, vb);
+    E(s3);
   }
-  B("k_median");
+  if (pen) {
+    B("k_pen_seed", s);
+    hipLaunchKernelGGL(k_pen_seed, dim3(2 * nc * per_s), dim3(VT), 0, s, vb, per_s);
+    E(s);
+    B("k_pen_min", s);
+    hipLaunchKernelGGL(k_pen_min, dim3(2 * nc * per_s), dim3(VT), 0, s, vb, per_s);
+    E(s);
+    B("k_pen_hard", s);
+    hipLaunchKernelGGL(k_pen_hard, dim3(HARD_BLOCKS), dim3(VT), 0, s, vb);
+    E(s);
+  }
+  if (ss) {
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventRecord(ss->join[k], ss->side[k]);
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipStreamWaitEvent(s, ss->join[k], 0);
+    if (e != hipSuccess) return e;
+  }
+  B("k_median", s);
   hipLaunchKernelGGL(k_median, dim3(4 * nc), dim3(VT), 0, s, vb);
-  E();
-  B("k_final");
+  E(s);
+  B("k_final", s);
   hipLaunchKernelGGL(k_final, dim3(blocks_for(nc, 64)), dim3(64), 0, s, vb);
-  E();
+  E(s);
   return hipGetLastError();
 }
 
