@@ -228,14 +228,35 @@ __global__ void __launch_bounds__(VT) k_pen(VerifyBatch vb, int per_task) {
   const PairDesc &pd = vb.pair[cd.pair];
   const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
   const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
-  if (!pd.pen_ok) return;
-  bool in = false, deeper = false;
-  float depth = 0.0f;
+  if (!pd.pen_ok) return;  // (uniform: a workgroup belongs to one task)
+  // Most samples fall outside the other mesh's box (RayScene::inside's reject). The workgroup
+  // gathers the ones inside it in LDS and its first lanes take them, one each, so the rays and
+  // depth searches run in as few waves as there are such samples; waves left without one finish.
+  __shared__ float s_q[3][VT];
+  __shared__ uint32_t s_n;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  const SceneView &sv = other.mesh;
   if (i < own.n_s) {
     const double *T = side == 0 ? cd.Ti : cd.T;
     const double *p = own.s + 3 * (size_t)i;
     V3 q = moved_f32(T, p[0], p[1], p[2]);
-    in = inside_depth(other.mesh, q, vb.retest, &depth);
+    if (!(q.x < sv.lo[0] || q.x > sv.hi[0] || q.y < sv.lo[1] || q.y > sv.hi[1] || q.z < sv.lo[2] ||
+          q.z > sv.hi[2])) {
+      uint32_t k = atomicAdd(&s_n, 1u);
+      s_q[0][k] = q.x;
+      s_q[1][k] = q.y;
+      s_q[2][k] = q.z;
+    }
+  }
+  __syncthreads();
+  const uint32_t n = s_n;
+  if (threadIdx.x >= ((n + 63u) & ~63u)) return;  // whole waves only
+  bool in = false, deeper = false;
+  float depth = 0.0f;
+  if (threadIdx.x < n) {
+    V3 q = V3{s_q[0][threadIdx.x], s_q[1][threadIdx.x], s_q[2][threadIdx.x]};
+    in = inside_depth(sv, q, vb.retest, &depth);
     deeper = in && (double)depth > pd.sc[S_PEN];
   }
   CandAcc &acc = vb.acc[c];
