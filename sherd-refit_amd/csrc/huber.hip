@@ -205,6 +205,7 @@ __global__ void __launch_bounds__(HT) k_huber_lm(HuberView v, double delta, int 
   int c = blockIdx.x;
   const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
   int32_t *corr = v.lm_corr + v.off[c];
+  const int32_t *base = v.corr + v.off[c];
   __shared__ double sT[16], sTn[16];
   __shared__ T sA[9], sb[3];
   __shared__ T lds[4 * 28];
@@ -223,7 +224,9 @@ __global__ void __launch_bounds__(HT) k_huber_lm(HuberView v, double delta, int 
   T dl = (T)delta;
   double r2 = pd.r * pd.r;
   for (int it = 0; it < iters && !s_done; ++it) {
-    // Correspondences and the shifted map at the current pose.
+    // Correspondences and the shifted map at the current pose. Each search starts from the point's
+    // previous partner (at the first iteration, its partner at the base pose): the same answer,
+    // found sooner.
     for (int i = threadIdx.x; i < pd.ns; i += HT) {
       const double *s = pd.src + 3 * (size_t)i;
       double p0 = sT[0] * s[0] + sT[1] * s[1] + sT[2] * s[2] + sT[3];
@@ -231,7 +234,9 @@ __global__ void __launch_bounds__(HT) k_huber_lm(HuberView v, double delta, int 
       double p2 = sT[8] * s[0] + sT[9] * s[1] + sT[10] * s[2] + sT[11];
       uint32_t j;
       double d2;
-      corr[i] = (pd.r > 0.0 && cloud_nearest(pd.tree, pd.tp, p0, p1, p2, r2, &j, &d2)) ? (int32_t)j : -1;
+      const int32_t prev = it > 0 ? corr[i] : base[i];
+      const uint32_t seed = prev >= 0 ? (uint32_t)prev : NONE;
+      corr[i] = (pd.r > 0.0 && cloud_nearest(pd.tree, pd.tp, p0, p1, p2, r2, &j, &d2, seed)) ? (int32_t)j : -1;
     }
     if (threadIdx.x == 0) {
       double R[9] = {sT[0], sT[1], sT[2], sT[4], sT[5], sT[6], sT[8], sT[9], sT[10]};
