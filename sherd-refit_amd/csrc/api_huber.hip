@@ -2,6 +2,7 @@
 // se(3) gradient, and the batched LM refinement (huber.hip). Host code only.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cstring>
 #include <vector>
@@ -34,6 +35,8 @@ struct sherd_huber {
   double *G = nullptr;
   double *T_out = nullptr;
   uint32_t *steps = nullptr;
+  LmState *lm = nullptr;
+  int max_ns = 0;
   HuberView view() const {
     HuberView v;
     v.pairs = pairs;
@@ -42,7 +45,9 @@ struct sherd_huber {
     v.T0 = T0;
     v.corr = corr;
     v.lm_corr = lm_corr;
+    v.lm = lm;
     v.k = k;
+    v.max_ns = max_ns;
     return v;
   }
 };
@@ -50,7 +55,8 @@ struct sherd_huber {
 namespace {
 
 void free_huber(sherd_huber *h) {
-  void *ptrs[] = {h->pairs, h->cand_pair, h->doff, h->T0, h->corr, h->lm_corr, h->xi, h->E, h->G, h->T_out, h->steps};
+  void *ptrs[] = {h->pairs, h->cand_pair, h->doff, h->T0, h->corr, h->lm_corr, h->xi, h->E, h->G, h->T_out, h->steps,
+                  h->lm};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete h;
@@ -103,6 +109,7 @@ int sherd_huber_create(sherd_ctx *ctx, const sherd_icp_pair *pairs, size_t n_pai
     }
     h->off[i] = h->points;
     h->points += (size_t)pd[cand_pair[i]].ns;
+    h->max_ns = std::max(h->max_ns, pd[cand_pair[i]].ns);
   }
   hipError_t e = hipSuccess;
   do {
@@ -117,6 +124,7 @@ int sherd_huber_create(sherd_ctx *ctx, const sherd_icp_pair *pairs, size_t n_pai
     if ((e = dnew(&h->G, 6 * k)) != hipSuccess) break;
     if ((e = dnew(&h->T_out, 16 * k)) != hipSuccess) break;
     if ((e = dnew(&h->steps, k)) != hipSuccess) break;
+    if ((e = dnew(&h->lm, k)) != hipSuccess) break;
     if ((e = hipMemcpyAsync(h->pairs, pd.data(), sizeof(IcpPairDesc) * n_pairs, hipMemcpyHostToDevice, c->stream)))
       break;
     if ((e = hipMemcpyAsync(h->cand_pair, cand_pair, sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream))) break;

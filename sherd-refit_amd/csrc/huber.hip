@@ -17,6 +17,8 @@
 // are deterministic run to run.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "cloud.hpp"
 #include "kernels.hpp"
 #include "linalg.hpp"
@@ -194,131 +196,149 @@ __global__ void __launch_bounds__(HT) k_correspond(HuberView v, const double *Tc
   }
 }
 
-// Batched Levenberg-Marquardt on the Huber energy, one workgroup per pose. Each iteration takes
-// the correspondences at the current pose, forms the iteratively reweighted normal equations
-// (weights psi(r)/r) with rows [p' x n, n] about the target centroid, solves (H + lambda diag H) dx
-// = -g in f64 (icp.rs's Eigen-pivoted LDLT), and keeps the step when the energy over the same
-// correspondences drops (lambda / 10), else raises lambda x 10 and stays.
-// (At least four waves per SIMD: the allocator then keeps the search loop in registers and
-// spills only the one-thread solve, 92 -> 82 ms for 512 poses.)
-template <class T>
-__global__ void __launch_bounds__(HT, 4) k_huber_lm(HuberView v, double delta, int iters, double *T_out, double *E_out,
-                                                 uint32_t *steps_out) {
-  int c = blockIdx.x;
+// Batched Levenberg-Marquardt on the Huber energy. Each iteration takes the correspondences at the
+// current pose, forms the iteratively reweighted normal equations (weights psi(r)/r) with rows
+// [p' x n, n] about the target centroid, solves (H + lambda diag H) dx = -g in f64 (icp.rs's
+// Eigen-pivoted LDLT), and keeps the step when the energy over the same correspondences drops
+// (lambda / 10), else raises lambda x 10 and stays.
+//
+// An iteration is two launches: k_lm_search finds every point's correspondence for every pose
+// still running, one thread per point (each search starts from the point's previous partner, at
+// the first iteration its partner at the base pose: the same answer, found sooner), and
+// k_lm_step, one workgroup per pose, does the rest. A pose's state (LmState) waits in global
+// memory between them; a pose that has stopped returns at once from both.
+
+__global__ void k_lm_init(HuberView v) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= v.k) return;
+  LmState &st = v.lm[c];
+  for (int q = 0; q < 16; ++q) st.T[q] = v.T0[16 * (size_t)c + q];
+  st.lambda = 1e-3;
+  st.E = st.Ef = 0.0;
+  st.done = 0;
+  st.steps = 0;
+}
+
+__global__ void __launch_bounds__(HT) k_lm_search(HuberView v, int it, int c0) {
+  const int c = c0 + (int)blockIdx.y;
+  const LmState &st = v.lm[c];
+  if (st.done) return;
   const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
+  const int i = blockIdx.x * HT + threadIdx.x;
+  if (i >= pd.ns) return;
   int32_t *corr = v.lm_corr + v.off[c];
   const int32_t *base = v.corr + v.off[c];
+  const double *sT = st.T;
+  const double *s = pd.src + 3 * (size_t)i;
+  double p0 = sT[0] * s[0] + sT[1] * s[1] + sT[2] * s[2] + sT[3];
+  double p1 = sT[4] * s[0] + sT[5] * s[1] + sT[6] * s[2] + sT[7];
+  double p2 = sT[8] * s[0] + sT[9] * s[1] + sT[10] * s[2] + sT[11];
+  uint32_t j;
+  double d2;
+  const int32_t prev = it > 0 ? corr[i] : base[i];
+  const uint32_t seed = prev >= 0 ? (uint32_t)prev : NONE;
+  corr[i] = (pd.r > 0.0 && cloud_nearest(pd.tree, pd.tp, p0, p1, p2, pd.r * pd.r, &j, &d2, seed)) ? (int32_t)j : -1;
+}
+
+template <class T>
+__global__ void __launch_bounds__(HT) k_lm_step(HuberView v, double delta, int it) {
+  const int c = blockIdx.x;
+  LmState &st = v.lm[c];
+  if (st.done) return;  // (uniform: one pose per workgroup)
+  const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
+  const int32_t *corr = v.lm_corr + v.off[c];
   __shared__ double sT[16], sTn[16];
   __shared__ T sA[9], sb[3];
   __shared__ T lds[4 * 28];
   __shared__ T red[28];
   __shared__ double s_lambda, s_E, s_Ef;
   __shared__ int s_done, s_steps;
-  if (threadIdx.x < 16) sT[threadIdx.x] = v.T0[16 * (size_t)c + threadIdx.x];
+  if (threadIdx.x < 16) sT[threadIdx.x] = st.T[threadIdx.x];
   if (threadIdx.x == 0) {
-    s_lambda = 1e-3;
+    s_lambda = st.lambda;
     s_done = 0;
-    s_steps = 0;
-    s_E = 0.0;
-    s_Ef = 0.0;
+    s_steps = st.steps;
+    s_E = st.E;
+    s_Ef = st.Ef;
+    double R[9] = {sT[0], sT[1], sT[2], sT[4], sT[5], sT[6], sT[8], sT[9], sT[10]};
+    double t[3] = {sT[3], sT[7], sT[11]}, A[9], b[3];
+    shifted_map(R, t, pd.cs, pd.ct, A, b);
+    for (int k = 0; k < 9; ++k) sA[k] = (T)A[k];
+    for (int k = 0; k < 3; ++k) sb[k] = (T)b[k];
   }
   __syncthreads();
   T dl = (T)delta;
-  double r2 = pd.r * pd.r;
-  for (int it = 0; it < iters && !s_done; ++it) {
-    // Correspondences and the shifted map at the current pose. Each search starts from the point's
-    // previous partner (at the first iteration, its partner at the base pose): the same answer,
-    // found sooner.
-    for (int i = threadIdx.x; i < pd.ns; i += HT) {
-      const double *s = pd.src + 3 * (size_t)i;
-      double p0 = sT[0] * s[0] + sT[1] * s[1] + sT[2] * s[2] + sT[3];
-      double p1 = sT[4] * s[0] + sT[5] * s[1] + sT[6] * s[2] + sT[7];
-      double p2 = sT[8] * s[0] + sT[9] * s[1] + sT[10] * s[2] + sT[11];
-      uint32_t j;
-      double d2;
-      const int32_t prev = it > 0 ? corr[i] : base[i];
-      const uint32_t seed = prev >= 0 ? (uint32_t)prev : NONE;
-      corr[i] = (pd.r > 0.0 && cloud_nearest(pd.tree, pd.tp, p0, p1, p2, r2, &j, &d2, seed)) ? (int32_t)j : -1;
+  T A[9], b[3];
+  for (int k = 0; k < 9; ++k) A[k] = sA[k];
+  for (int k = 0; k < 3; ++k) b[k] = sb[k];
+  T acc[28];
+#pragma unroll
+  for (int k = 0; k < 28; ++k) acc[k] = T(0);
+  for_pairs<T>(pd, corr, A, b, [&](T r, const T *n, const T *p) {
+    T rho, psi;
+    huber(r, dl, &rho, &psi);
+    T w = r == T(0) ? T(1) : psi / r;
+    T row[6] = {p[1] * n[2] - p[2] * n[1], p[2] * n[0] - p[0] * n[2], p[0] * n[1] - p[1] * n[0], n[0], n[1], n[2]};
+    acc[0] += rho;
+    int q = 1;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+#pragma unroll
+      for (int l = 0; l <= k; ++l) acc[q++] += w * row[k] * row[l];
     }
-    if (threadIdx.x == 0) {
-      double R[9] = {sT[0], sT[1], sT[2], sT[4], sT[5], sT[6], sT[8], sT[9], sT[10]};
-      double t[3] = {sT[3], sT[7], sT[11]}, A[9], b[3];
-      shifted_map(R, t, pd.cs, pd.ct, A, b);
-      for (int k = 0; k < 9; ++k) sA[k] = (T)A[k];
-      for (int k = 0; k < 3; ++k) sb[k] = (T)b[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc[22 + k] += w * r * row[k];
+  });
+  block_sum<T, 28>(acc, lds, red);
+  if (threadIdx.x == 0) {
+    s_E = (double)red[0];
+    s_Ef = s_E;
+    double H[36], g[6], x[6];
+    int q = 1;
+    for (int k = 0; k < 6; ++k)
+      for (int l = 0; l <= k; ++l) {
+        H[6 * k + l] = H[6 * l + k] = (double)red[q];
+        ++q;
+      }
+    for (int k = 0; k < 6; ++k) g[k] = -(double)red[22 + k];
+    // Marquardt's damping on the diagonal, with a floor of 1e-9 of the mean diagonal so that
+    // directions the seam does not constrain (a breakline's tangent) stay damped.
+    double tr = 0.0;
+    for (int k = 0; k < 6; ++k) tr += H[7 * k];
+    for (int k = 0; k < 6; ++k) H[7 * k] += s_lambda * (fmax(H[7 * k], 0.0) + 1e-9 * tr / 6.0 + 1e-300);
+    bool ok = s_E > 0.0 && solve_ldlt(H, g, x);
+    if (ok) {
+      // A trust region: at most 0.05 rad of rotation and the correspondence radius of
+      // translation per step.
+      double wn = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]), vn = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+      double f = fmin(1.0, fmin(0.05 / fmax(wn, 1e-300), pd.r / fmax(vn, 1e-300)));
+      for (int k = 0; k < 6; ++k) x[k] *= f;
     }
-    __syncthreads();
-    T A[9], b[3];
-    for (int k = 0; k < 9; ++k) A[k] = sA[k];
-    for (int k = 0; k < 3; ++k) b[k] = sb[k];
-    T acc[28];
-#pragma unroll
-    for (int k = 0; k < 28; ++k) acc[k] = T(0);
-    for_pairs<T>(pd, corr, A, b, [&](T r, const T *n, const T *p) {
-      T rho, psi;
-      huber(r, dl, &rho, &psi);
-      T w = r == T(0) ? T(1) : psi / r;
-      T row[6] = {p[1] * n[2] - p[2] * n[1], p[2] * n[0] - p[0] * n[2], p[0] * n[1] - p[1] * n[0], n[0], n[1], n[2]};
-      acc[0] += rho;
-      int q = 1;
-#pragma unroll
-      for (int k = 0; k < 6; ++k) {
-#pragma unroll
-        for (int l = 0; l <= k; ++l) acc[q++] += w * row[k] * row[l];
+    if (!ok) {
+      s_done = 1;
+    } else {
+      // The step as a rotation about the target centroid and a translation.
+      double Rw[9];
+      rodrigues(x, Rw);
+      double tw[3];
+      for (int i = 0; i < 3; ++i)
+        tw[i] = pd.ct[i] - ((Rw[3 * i] * pd.ct[0] + Rw[3 * i + 1] * pd.ct[1]) + Rw[3 * i + 2] * pd.ct[2]) + x[3 + i];
+      for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 4; ++j)
+          sTn[4 * i + j] = (Rw[3 * i] * sT[j] + Rw[3 * i + 1] * sT[4 + j]) + Rw[3 * i + 2] * sT[8 + j];
+        sTn[4 * i + 3] += tw[i];
       }
-#pragma unroll
-      for (int k = 0; k < 6; ++k) acc[22 + k] += w * r * row[k];
-    });
-    block_sum<T, 28>(acc, lds, red);
-    if (threadIdx.x == 0) {
-      s_E = (double)red[0];
-      s_Ef = s_E;
-      double H[36], g[6], x[6];
-      int q = 1;
-      for (int k = 0; k < 6; ++k)
-        for (int l = 0; l <= k; ++l) {
-          H[6 * k + l] = H[6 * l + k] = (double)red[q];
-          ++q;
-        }
-      for (int k = 0; k < 6; ++k) g[k] = -(double)red[22 + k];
-      // Marquardt's damping on the diagonal, with a floor of 1e-9 of the mean diagonal so that
-      // directions the seam does not constrain (a breakline's tangent) stay damped.
-      double tr = 0.0;
-      for (int k = 0; k < 6; ++k) tr += H[7 * k];
-      for (int k = 0; k < 6; ++k) H[7 * k] += s_lambda * (fmax(H[7 * k], 0.0) + 1e-9 * tr / 6.0 + 1e-300);
-      bool ok = s_E > 0.0 && solve_ldlt(H, g, x);
-      if (ok) {
-        // A trust region: at most 0.05 rad of rotation and the correspondence radius of
-        // translation per step.
-        double wn = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]), vn = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
-        double f = fmin(1.0, fmin(0.05 / fmax(wn, 1e-300), pd.r / fmax(vn, 1e-300)));
-        for (int k = 0; k < 6; ++k) x[k] *= f;
-      }
-      if (!ok) {
-        s_done = 1;
-      } else {
-        // The step as a rotation about the target centroid and a translation.
-        double Rw[9];
-        rodrigues(x, Rw);
-        double tw[3];
-        for (int i = 0; i < 3; ++i)
-          tw[i] = pd.ct[i] - ((Rw[3 * i] * pd.ct[0] + Rw[3 * i + 1] * pd.ct[1]) + Rw[3 * i + 2] * pd.ct[2]) + x[3 + i];
-        for (int i = 0; i < 3; ++i) {
-          for (int j = 0; j < 4; ++j)
-            sTn[4 * i + j] = (Rw[3 * i] * sT[j] + Rw[3 * i + 1] * sT[4 + j]) + Rw[3 * i + 2] * sT[8 + j];
-          sTn[4 * i + 3] += tw[i];
-        }
-        sTn[12] = sTn[13] = sTn[14] = 0.0;
-        sTn[15] = 1.0;
-        double R[9] = {sTn[0], sTn[1], sTn[2], sTn[4], sTn[5], sTn[6], sTn[8], sTn[9], sTn[10]};
-        double t[3] = {sTn[3], sTn[7], sTn[11]}, A2[9], b2[3];
-        shifted_map(R, t, pd.cs, pd.ct, A2, b2);
-        for (int k = 0; k < 9; ++k) sA[k] = (T)A2[k];
-        for (int k = 0; k < 3; ++k) sb[k] = (T)b2[k];
-      }
+      sTn[12] = sTn[13] = sTn[14] = 0.0;
+      sTn[15] = 1.0;
+      double R[9] = {sTn[0], sTn[1], sTn[2], sTn[4], sTn[5], sTn[6], sTn[8], sTn[9], sTn[10]};
+      double t[3] = {sTn[3], sTn[7], sTn[11]}, A2[9], b2[3];
+      shifted_map(R, t, pd.cs, pd.ct, A2, b2);
+      for (int k = 0; k < 9; ++k) sA[k] = (T)A2[k];
+      for (int k = 0; k < 3; ++k) sb[k] = (T)b2[k];
     }
-    __syncthreads();
-    if (s_done) break;
+  }
+  __syncthreads();
+  if (!s_done) {
     // The trial pose's energy over the same correspondences.
     for (int k = 0; k < 9; ++k) A[k] = sA[k];
     for (int k = 0; k < 3; ++k) b[k] = sb[k];
@@ -342,13 +362,24 @@ __global__ void __launch_bounds__(HT, 4) k_huber_lm(HuberView v, double delta, i
         if (s_lambda > 1e12) s_done = 1;
       }
     }
-    __syncthreads();
   }
-  if (threadIdx.x < 16) T_out[16 * (size_t)c + threadIdx.x] = sT[threadIdx.x];
   if (threadIdx.x == 0) {
-    E_out[c] = s_Ef;
-    steps_out[c] = (uint32_t)s_steps;
+    for (int k = 0; k < 16; ++k) st.T[k] = sT[k];
+    st.lambda = s_lambda;
+    st.E = s_E;
+    st.Ef = s_Ef;
+    st.steps = s_steps;
+    st.done = s_done;
   }
+}
+
+__global__ void k_lm_final(HuberView v, double *T_out, double *E_out, uint32_t *steps_out) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= v.k) return;
+  const LmState &st = v.lm[c];
+  for (int q = 0; q < 16; ++q) T_out[16 * (size_t)c + q] = st.T[q];
+  E_out[c] = st.Ef;
+  steps_out[c] = (uint32_t)st.steps;
 }
 
 }  // namespace
@@ -372,10 +403,19 @@ hipError_t launch_huber_eval(const HuberView &v, const double *xi, double delta,
 hipError_t launch_huber_lm(const HuberView &v, double delta, int iters, int f64, double *T_out, double *E_out,
                            uint32_t *steps, hipStream_t s) {
   if (v.k <= 0) return hipSuccess;
-  if (f64)
-    hipLaunchKernelGGL(k_huber_lm<double>, dim3(v.k), dim3(HT), 0, s, v, delta, iters, T_out, E_out, steps);
-  else
-    hipLaunchKernelGGL(k_huber_lm<float>, dim3(v.k), dim3(HT), 0, s, v, delta, iters, T_out, E_out, steps);
+  const unsigned kb = (unsigned)((v.k + 63) / 64);
+  hipLaunchKernelGGL(k_lm_init, dim3(kb), dim3(64), 0, s, v);
+  const unsigned spans = (unsigned)((v.max_ns + HT - 1) / HT);
+  for (int it = 0; it < iters; ++it) {
+    // (poses in groups of at most 65535, the grid's y extent)
+    for (int c0 = 0; spans > 0 && c0 < v.k; c0 += 65535)
+      hipLaunchKernelGGL(k_lm_search, dim3(spans, (unsigned)std::min(65535, v.k - c0)), dim3(HT), 0, s, v, it, c0);
+    if (f64)
+      hipLaunchKernelGGL(k_lm_step<double>, dim3(v.k), dim3(HT), 0, s, v, delta, it);
+    else
+      hipLaunchKernelGGL(k_lm_step<float>, dim3(v.k), dim3(HT), 0, s, v, delta, it);
+  }
+  hipLaunchKernelGGL(k_lm_final, dim3(kb), dim3(64), 0, s, v, T_out, E_out, steps);
   return hipGetLastError();
 }
 

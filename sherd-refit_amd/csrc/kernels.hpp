@@ -127,6 +127,13 @@ struct IcpBatchView {
 };
 
 // ---- Huber energy and LM (huber.hip) ----------------------------------------------------------------
+// One pose's LM state between the launches of an iteration.
+struct LmState {
+  double T[16];
+  double lambda, E, Ef;
+  int done, steps;
+};
+
 struct HuberView {
   const IcpPairDesc *pairs;
   const uint32_t *cand_pair;
@@ -134,7 +141,9 @@ struct HuberView {
   const double *T0;      // 16 per pose
   const int32_t *corr;   // frozen correspondences (target index or -1)
   int32_t *lm_corr;      // the LM's own correspondences
+  LmState *lm;           // per pose
   int k;
+  int max_ns;            // the largest source cloud of the batch
 };
 
 hipError_t launch_correspond(const HuberView &v, const double *Tc, int32_t *corr, hipStream_t s);
