@@ -281,6 +281,60 @@ def _surface_join(n, seed=51):
     return pts, nrm, D, h
 
 
+def thickness_section(ex, col, args):
+    """R §3.2 (thickness.rs:100-161) on each of the vessel's fragments: one inward ray from every
+    face of the stride (every ceil(F / 300 000)-th face), the look-back filter and the two
+    histogram modes, one call per fragment (the call uploads the f64 mesh); and, as the CPU
+    baseline of this row, the oracle's filter and modes over the same rays on the host."""
+    frags = col["fragments"]
+    scenes = [ex.scene(f["V"].astype(np.float32), f["F"]) for f in frags]
+    for sc, f in zip(scenes, frags):
+        ex.thickness(sc, f["V"], f["F"])
+    ex.profile(True)
+    t0 = time.perf_counter()
+    out = [ex.thickness(sc, f["V"], f["F"]) for sc, f in zip(scenes, frags)]
+    el = time.perf_counter() - t0
+    prof = ex.profile_read()
+    ex.profile(False)
+    strides = [-(-len(f["F"]) // 300_000) for f in frags]
+    rays = sum(-(-len(f["F"]) // s_) for f, s_ in zip(frags, strides))
+    ray_ms = prof.get("k_thick_rays", {}).get("ms", float("nan"))
+    res = dict(fragments=len(frags), rays=int(rays), seconds=round(el, 4),
+               fragments_per_s=round(len(frags) / el, 2), ray_kernel_ms=round(ray_ms, 3),
+               rays_per_s=round(rays / (ray_ms * 1e-3), 1),
+               t_over_wall=[round(float(o[0]) / col["t"], 4) if o else None for o in out])
+    if not args.no_cpu:
+        # The host's share (filter and modes) over fragment 0's rays, cast again through the
+        # device's first hit so that both sides see the same hits.
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import thickness as TH
+
+        f = frags[0]
+        idx = TH.face_indices(len(f["F"]))
+        V, F = np.asarray(f["V"], dtype=np.float64), np.asarray(f["F"]).astype(np.int64)  # (the device's f64 too)
+        a, b, c = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+        e1, e2 = b - a, c - a
+        n = np.stack([e1[:, 1] * e2[:, 2] - e1[:, 2] * e2[:, 1], e1[:, 2] * e2[:, 0] - e1[:, 0] * e2[:, 2],
+                      e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]], axis=1)
+        ln = np.sqrt((n[:, 0] * n[:, 0] + n[:, 1] * n[:, 1]) + n[:, 2] * n[:, 2])
+        n = n / np.maximum(ln, 1e-12)[:, None]
+        cen = ((a + b) + c) / 3.0
+        d = -n[idx]
+        t_hit, prim = ex.first_hit(scenes[0], (cen[idx] + d * 1e-3).astype(np.float32), d.astype(np.float32))
+        t0 = time.perf_counter()
+        want = TH.thickness_from_hits(n, idx, t_hit, prim)
+        res["cpu_modes_seconds"] = round(time.perf_counter() - t0, 4)
+        res["identical_to_oracle_on_fragment_0"] = bool(want is not None and out[0] is not None and
+                                                        want[0].tobytes() == out[0][0].tobytes() and
+                                                        want[1].tobytes() == out[0][1].tobytes())
+    res["workload"] = "vessel, 8 fragments: one inward ray per face of the 300 000-ray stride, two f32 histogram modes"
+    log(f"thickness: {len(frags)} fragments, {rays} rays in {el * 1e3:.1f} ms ({ray_ms:.2f} ms of ray kernel); "
+        f"t/wall {res['t_over_wall'][0]}; identical to the oracle {res.get('identical_to_oracle_on_fragment_0')}")
+    for sc in scenes:
+        sc.close()
+    return res
+
+
 def refine_section(ex, col, args):
     """R §9 (refine.rs:270-350) on the vessel: each fragment's fracture cloud (its mesh serving as
     its own original), assembly poses off the truth by 0.02 degrees about the fragment's centroid
@@ -540,6 +594,7 @@ def main():
     ap.add_argument("--no-huber", action="store_true")
     ap.add_argument("--no-refine", action="store_true", help="skip R §9's full-resolution refinement")
     ap.add_argument("--no-match", action="store_true", help="skip R §5.1-5.5 over the vessel's pairs")
+    ap.add_argument("--no-thickness", action="store_true", help="skip R §3.2 over the vessel's fragments")
     ap.add_argument("--no-collection", action="store_true",
                     help="skip the 40-fragment collection end to end (about 20 s of generation)")
     ap.add_argument("--batched-only", action="store_true",
@@ -654,6 +709,7 @@ def main():
     coll = collection_section(ex, args) if not args.no_collection else None
     refine = refine_section(ex, col, args) if not args.no_refine else None
     match = match_section(ex, col, args) if not args.no_match else None
+    thick = thickness_section(ex, col, args) if not args.no_thickness else None
 
     cpu = None
     if not args.no_cpu:
@@ -673,7 +729,7 @@ def main():
         plain_pair_verifications_per_s=round(plain, 3), batched_vs_plain=round(value / plain, 3),
         accepted=int(accepted.sum()), accepted_identical_plain_batched=same,
         rays_per_s=round(rays_per_s, 1), rays=int(rays), knn_queries_per_s=round(knn, 1),
-        lbvh_mtri_per_s=round(lbvh_mtri, 2), icp=icp, huber=huber, collection=coll, refine=refine, match=match, kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
+        lbvh_mtri_per_s=round(lbvh_mtri, 2), icp=icp, huber=huber, collection=coll, refine=refine, match=match, thickness=thick, kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
     )
     print(json.dumps(line), flush=True)
     ex.close()

@@ -392,6 +392,24 @@ int sherd_cone_votes(sherd_ctx *ctx, const sherd_scene *scene, const double *V, 
                      const uint32_t *F, size_t nf, const int64_t *faces, const double *ns,
                      size_t n_faces, double thick, int retest, uint8_t *good);
 
+/* ---- R §3.2, the wall thickness ---------------------------------------------------------------- */
+
+/* thickness::estimate_thickness (crates/sherd-core/src/fragment/thickness.rs:100-110, with
+ * cast_inward_rays :115-132 and thickness_from_hits :139-161): one inward ray from the centroid of
+ * every face of R §3.2's stride (all of them up to 300 000, else every ceil(nf / 300000)-th), the
+ * f64 face normals of V (nv x 3) and F (nf x 3), and the histogram modes of the hits. scene is the
+ * mesh's own LBVH (V narrowed to f32, no mask). On return *estimated is 1 with *t (the filtered
+ * mode) and *thick_mode (the mode over every valid hit), or 0 when fewer than 100 rays hit (the
+ * reference's None, whose caller falls back to the oriented box). */
+int sherd_thickness(sherd_ctx *ctx, const sherd_scene *scene, const double *V, size_t nv, const uint32_t *F,
+                    size_t nf, float *t, float *thick_mode, int *estimated);
+/* thickness::thickness_from_hits (thickness.rs:139-161) on given rays: n_rays hits (t_hit f32,
+ * +inf on a miss; prim, 0xFFFFFFFF on a miss) of the rays cast from faces idx, over a mesh of nf
+ * faces with f64 unit normals (nf x 3). Outputs as sherd_thickness. */
+int sherd_thickness_from_hits(sherd_ctx *ctx, const double *normals, size_t nf, const uint32_t *idx,
+                              const float *t_hit, const uint32_t *prim, size_t n_rays, float *t,
+                              float *thick_mode, int *estimated);
+
 /* ---- per-kernel timing ------------------------------------------------------------------------ */
 
 /* Turns per-kernel HIP-event timing on the context's stream on (1) or off (0), clearing the

@@ -576,6 +576,109 @@ int sherd_cone_votes(sherd_ctx *ctx, const sherd_scene *scene, const double *V, 
 
 namespace {
 
+constexpr size_t THICK_RAYS = 300000;  // thickness.rs RAYS
+constexpr uint32_t THICK_MIN_HITS = 100;
+
+// thickness_from_hits's decisions over the device's two lists (thickness.rs:150-160): too few
+// valid hits is None; the filtered mode when it has enough hits, else the plain one.
+int thick_modes(Ctx *c, const double *dN, int nf, const uint32_t *didx, int stride, const float *dt,
+                const uint32_t *dp, int m, float *t, float *thick_mode, int *estimated) {
+  hipError_t e;
+  float *ok_d = (float *)c->scratch(6, sizeof(float) * (size_t)(m > 0 ? m : 1), &e);
+  if (!ok_d) return hip_fail(e, "scratch");
+  float *far_d = (float *)c->scratch(7, sizeof(float) * (size_t)(m > 0 ? m : 1), &e);
+  if (!far_d) return hip_fail(e, "scratch");
+  uint32_t *dcnt = (uint32_t *)c->scratch(8, 16, &e);
+  if (!dcnt) return hip_fail(e, "scratch");
+  float *dmodes = (float *)(dcnt + 2);
+  HIPCK(hipMemsetAsync(dcnt, 0, 16, c->stream), "clear");
+  HIPCK(launch_thick_modes(dN, nf, didx, stride, dt, dp, m, ok_d, far_d, dcnt, dmodes, c->stream), "modes");
+  uint32_t h[4];
+  HIPCK(hipMemcpyAsync(h, dcnt, 16, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  float modes[2];
+  std::memcpy(modes, h + 2, sizeof modes);
+  *estimated = h[0] >= THICK_MIN_HITS ? 1 : 0;
+  if (*estimated) {
+    *thick_mode = modes[0];
+    *t = h[1] >= THICK_MIN_HITS ? modes[1] : modes[0];
+  }
+  return SHERD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sherd_thickness(sherd_ctx *ctx, const sherd_scene *scene, const double *V, size_t nv, const uint32_t *F,
+                    size_t nf, float *t, float *thick_mode, int *estimated) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  LOCK(c);
+  if (!scene || scene->ctx != c) return fail(SHERD_E_ARG, "scene of another context");
+  if (!V || !F || !t || !thick_mode || !estimated) return fail(SHERD_E_ARG, "null buffer");
+  *estimated = 0;
+  if (nf == 0) return SHERD_OK;
+  if (nf > (size_t)INT_MAX / 3 || nv > (size_t)INT_MAX / 3) return fail(SHERD_E_LIMIT, "mesh too large");
+  if ((size_t)scene->d.n != nf) return fail(SHERD_E_ARG, "the scene must be built over every face of F");
+  for (size_t i = 0; i < 3 * nf; ++i)
+    if (F[i] >= nv) return fail(SHERD_E_ARG, "face index out of range");
+  hipError_t e = hipSuccess;
+  double *dV = upload3(c, 0, V, nv, &e);
+  if (!dV) return hip_fail(e, "upload vertices");
+  uint32_t *dF = (uint32_t *)c->scratch(1, sizeof(uint32_t) * 3 * nf, &e);
+  if (!dF) return hip_fail(e, "scratch");
+  HIPCK(hipMemcpyAsync(dF, F, sizeof(uint32_t) * 3 * nf, hipMemcpyHostToDevice, c->stream), "upload faces");
+  double *dN = (double *)c->scratch(2, sizeof(double) * 3 * nf, &e);
+  if (!dN) return hip_fail(e, "scratch");
+  // R §3.2's stride: every face up to THICK_RAYS, else every ceil(nf / THICK_RAYS)-th.
+  const int stride = (int)((nf + THICK_RAYS - 1) / THICK_RAYS);
+  const int m = (int)((nf + (size_t)stride - 1) / (size_t)stride);
+  float *dt = (float *)c->scratch(3, sizeof(float) * (size_t)m, &e);
+  if (!dt) return hip_fail(e, "scratch");
+  uint32_t *dp = (uint32_t *)c->scratch(4, sizeof(uint32_t) * (size_t)m, &e);
+  if (!dp) return hip_fail(e, "scratch");
+  Timer tm(c, SHERD_M_FIRST_HIT, (uint64_t)m);
+  if (c->prof.on) c->prof.begin("k_thick_rays", c->stream);
+  HIPCK(launch_thick_normals(dV, dF, (int)nf, dN, c->stream), "normals");
+  HIPCK(launch_thick_rays(view_of(scene->d), dV, dF, dN, stride, m, dt, dp, c->stream), "rays");
+  if (c->prof.on) c->prof.end(c->stream);
+  tm.stop();
+  int rc = thick_modes(c, dN, (int)nf, nullptr, stride, dt, dp, m, t, thick_mode, estimated);
+  c->prof.collect();
+  return rc;
+}
+
+int sherd_thickness_from_hits(sherd_ctx *ctx, const double *normals, size_t nf, const uint32_t *idx,
+                              const float *t_hit, const uint32_t *prim, size_t n_rays, float *t,
+                              float *thick_mode, int *estimated) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  LOCK(c);
+  if (!t || !thick_mode || !estimated) return fail(SHERD_E_ARG, "null buffer");
+  *estimated = 0;
+  if (n_rays == 0) return SHERD_OK;
+  if (!normals || !idx || !t_hit || !prim) return fail(SHERD_E_ARG, "null buffer");
+  if (nf > (size_t)INT_MAX / 3 || n_rays > (size_t)INT_MAX) return fail(SHERD_E_LIMIT, "too many faces or rays");
+  for (size_t r = 0; r < n_rays; ++r)
+    if (idx[r] >= nf) return fail(SHERD_E_ARG, "a ray's face is out of range");
+  hipError_t e = hipSuccess;
+  double *dN = upload3(c, 0, normals, nf, &e);
+  if (!dN) return hip_fail(e, "upload normals");
+  uint32_t *didx = (uint32_t *)c->scratch(1, sizeof(uint32_t) * n_rays, &e);
+  if (!didx) return hip_fail(e, "scratch");
+  float *dt = (float *)c->scratch(3, sizeof(float) * n_rays, &e);
+  if (!dt) return hip_fail(e, "scratch");
+  uint32_t *dp = (uint32_t *)c->scratch(4, sizeof(uint32_t) * n_rays, &e);
+  if (!dp) return hip_fail(e, "scratch");
+  HIPCK(hipMemcpyAsync(didx, idx, sizeof(uint32_t) * n_rays, hipMemcpyHostToDevice, c->stream), "upload");
+  HIPCK(hipMemcpyAsync(dt, t_hit, sizeof(float) * n_rays, hipMemcpyHostToDevice, c->stream), "upload");
+  HIPCK(hipMemcpyAsync(dp, prim, sizeof(uint32_t) * n_rays, hipMemcpyHostToDevice, c->stream), "upload");
+  return thick_modes(c, dN, (int)nf, didx, 1, dt, dp, (int)n_rays, t, thick_mode, estimated);
+}
+
+}  // extern "C"
+
+namespace {
+
 __global__ void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     dst[i] = src[i];

@@ -144,6 +144,26 @@ SH_HD float tri_d2(V3 p, V3 a, V3 b, V3 c) {
   return (s * s) / nn;
 }
 
+// mesh/geometry.rs:70-96: a face's unit normal in f64, the cross product of its two edges from
+// the first vertex over max(|n|, 1e-12).
+SH_HD void face_normal(const double *V, const uint32_t *F, uint32_t f, double *n) {
+  const double *a = V + 3 * (size_t)F[3 * f], *b = V + 3 * (size_t)F[3 * f + 1], *c = V + 3 * (size_t)F[3 * f + 2];
+  double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  double x = e1[1] * e2[2] - e1[2] * e2[1], y = e1[2] * e2[0] - e1[0] * e2[2], z = e1[0] * e2[1] - e1[1] * e2[0];
+  double len = sqrt((x * x + y * y) + z * z);
+  double den = len < 1e-12 ? 1e-12 : len;  // f64::max(len, 1e-12)
+  n[0] = x / den;
+  n[1] = y / den;
+  n[2] = z / den;
+}
+
+// The same face's centroid, ((a + b) + c) / 3 per coordinate.
+SH_HD void face_centroid(const double *V, const uint32_t *F, uint32_t f, double *c) {
+  const double *a = V + 3 * (size_t)F[3 * f], *b = V + 3 * (size_t)F[3 * f + 1], *d = V + 3 * (size_t)F[3 * f + 2];
+  for (int k = 0; k < 3; ++k) c[k] = ((a[k] + b[k]) + d[k]) / 3.0;
+}
+
 // executor/cpu.rs:192-200: T.p in f64, left to right, then narrowed once to f32.
 SH_HD V3 moved_f32(const double *T, double px, double py, double pz) {
   float x = (float)(T[0] * px + T[1] * py + T[2] * pz + T[3]);

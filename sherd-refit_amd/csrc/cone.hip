@@ -14,18 +14,6 @@
 namespace sherd {
 namespace {
 
-__device__ __forceinline__ void face_normal(const double *V, const uint32_t *F, uint32_t f, double *n) {
-  const double *a = V + 3 * (size_t)F[3 * f], *b = V + 3 * (size_t)F[3 * f + 1], *c = V + 3 * (size_t)F[3 * f + 2];
-  double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-  double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
-  double x = e1[1] * e2[2] - e1[2] * e2[1], y = e1[2] * e2[0] - e1[0] * e2[2], z = e1[0] * e2[1] - e1[1] * e2[0];
-  double len = sqrt((x * x + y * y) + z * z);
-  double den = len < 1e-12 ? 1e-12 : len;  // f64::max(len, 1e-12)
-  n[0] = x / den;
-  n[1] = y / den;
-  n[2] = z / den;
-}
-
 __global__ void __launch_bounds__(256) k_cone(SceneView sv, const double *V, const uint32_t *F, int nf,
                                               const int64_t *faces, const double *ns, int n, ConeConst cc,
                                               int retest, uint8_t *good) {

@@ -261,4 +261,14 @@ struct ConeConst {
 hipError_t launch_cone(const SceneView &sv, const double *V, const uint32_t *F, int nf, const int64_t *faces,
                        const double *ns, int n, const ConeConst &cc, int retest, uint8_t *good, hipStream_t s);
 
+// ---- R §3.2 (thick.hip) -----------------------------------------------------------------------------
+hipError_t launch_thick_normals(const double *V, const uint32_t *F, int nf, double *N, hipStream_t s);
+hipError_t launch_thick_rays(const SceneView &sv, const double *V, const uint32_t *F, const double *N, int stride, int m,
+                             float *t_hit, uint32_t *prim, hipStream_t s);
+// The valid and the looking-back distances into ok_d / far_d (m floats each), their counts into
+// counts[0..1] (zeroed by the caller), and their histogram modes into modes[0..1] (NaN when empty).
+hipError_t launch_thick_modes(const double *N, int nf, const uint32_t *idx, int stride, const float *t_hit,
+                              const uint32_t *prim, int m, float *ok_d, float *far_d, uint32_t *counts,
+                              float *modes, hipStream_t s);
+
 }  // namespace sherd

@@ -156,6 +156,9 @@ def _bind(L):
         "sherd_profile_read": (ci, [vp, C.c_char_p, sz]),
         "sherd_copy_probe": (ci, [vp, sz]),
         "sherd_cone_votes": (ci, [vp, vp, P(d), sz, P(u32), sz, P(i64), P(d), sz, d, ci, P(u8)]),
+        "sherd_thickness": (ci, [vp, vp, P(d), sz, P(u32), sz, P(C.c_float), P(C.c_float), P(ci)]),
+        "sherd_thickness_from_hits": (ci, [vp, P(d), sz, P(u32), P(C.c_float), P(u32), sz, P(C.c_float),
+                                           P(C.c_float), P(ci)]),
         "sherd_verify_batch": (ci, [vp, P(VerifyPair), sz, P(u32), P(d), sz, ci, ci, P(d), P(u8)]),
     }
     for name, (res, args) in sig.items():
@@ -706,6 +709,28 @@ class HipExecutor:
         _check(lib().sherd_verify_batch(self.ptr, pv, len(pairs), _p(cp, C.c_uint32), _p(T, C.c_double), n,
                                         int(full), int(retest), _p(scores, C.c_double), _p(acc, C.c_uint8)))
         return scores, acc.astype(bool)
+
+    def thickness(self, scene, V64, F):
+        """thickness::estimate_thickness (thickness.rs:100-110): (t, thick_mode) as float32, or None
+        when fewer than 100 inward rays hit. scene is the mesh's own (V narrowed to f32)."""
+        V64 = f64(V64)
+        F = np.ascontiguousarray(F, dtype=np.uint32).reshape(-1, 3)
+        t, m, ok = C.c_float(), C.c_float(), C.c_int()
+        _check(lib().sherd_thickness(self.ptr, scene.ptr, _p(V64, C.c_double), len(V64), _p(F, C.c_uint32), len(F),
+                                     C.byref(t), C.byref(m), C.byref(ok)))
+        return (np.float32(t.value), np.float32(m.value)) if ok.value else None
+
+    def thickness_from_hits(self, normals, idx, t_hit, prim):
+        """thickness::thickness_from_hits (thickness.rs:139-161) on given rays."""
+        normals = f64(normals)
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        t_hit = np.ascontiguousarray(t_hit, dtype=np.float32)
+        prim = np.ascontiguousarray(prim, dtype=np.uint32)
+        t, m, ok = C.c_float(), C.c_float(), C.c_int()
+        _check(lib().sherd_thickness_from_hits(self.ptr, _p(normals, C.c_double), len(normals), _p(idx, C.c_uint32),
+                                               _p(t_hit, C.c_float), _p(prim, C.c_uint32), len(t_hit), C.byref(t),
+                                               C.byref(m), C.byref(ok)))
+        return (np.float32(t.value), np.float32(m.value)) if ok.value else None
 
     def cone_votes(self, scene, V64, F, faces, ns, thick, retest=False):
         """segment::classify_faces' seven-ray vote (segment.rs:318-381) for the listed faces."""
