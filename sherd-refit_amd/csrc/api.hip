@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -581,27 +582,38 @@ constexpr uint32_t THICK_MIN_HITS = 100;
 
 // thickness_from_hits's decisions over the device's two lists (thickness.rs:150-160): too few
 // valid hits is None; the filtered mode when it has enough hits, else the plain one.
-int thick_modes(Ctx *c, const double *dN, int nf, const uint32_t *didx, int stride, const float *dt,
+// The estimate's device state (counts, modes, the face-index flag, the selection's counts), zeroed
+// on the context's stream.
+ThickState *thick_state(Ctx *c, hipError_t *e) {
+  ThickState *st = (ThickState *)c->scratch(8, sizeof(ThickState), e);
+  if (!st) return nullptr;
+  if ((*e = hipMemsetAsync(st, 0, sizeof(ThickState), c->stream)) != hipSuccess) return nullptr;
+  return st;
+}
+
+int thick_modes(Ctx *c, ThickState *st, const double *dN, int nf, const uint32_t *didx, int stride, const float *dt,
                 const uint32_t *dp, int m, float *t, float *thick_mode, int *estimated) {
   hipError_t e;
   float *ok_d = (float *)c->scratch(6, sizeof(float) * (size_t)(m > 0 ? m : 1), &e);
   if (!ok_d) return hip_fail(e, "scratch");
   float *far_d = (float *)c->scratch(7, sizeof(float) * (size_t)(m > 0 ? m : 1), &e);
   if (!far_d) return hip_fail(e, "scratch");
-  uint32_t *dcnt = (uint32_t *)c->scratch(8, 16, &e);
-  if (!dcnt) return hip_fail(e, "scratch");
-  float *dmodes = (float *)(dcnt + 2);
-  HIPCK(hipMemsetAsync(dcnt, 0, 16, c->stream), "clear");
-  HIPCK(launch_thick_modes(dN, nf, didx, stride, dt, dp, m, ok_d, far_d, dcnt, dmodes, c->stream), "modes");
-  uint32_t h[4];
-  HIPCK(hipMemcpyAsync(h, dcnt, 16, hipMemcpyDeviceToHost, c->stream), "download");
+  if (c->prof.on) c->prof.begin("k_thick_modes", c->stream);
+  HIPCK(launch_thick_modes(dN, nf, didx, stride, dt, dp, m, ok_d, far_d, st, c->stream), "modes");
+  if (c->prof.on) c->prof.end(c->stream);
+  struct {
+    uint32_t counts[2];
+    float modes[2];
+    uint32_t bad_face;
+  } h;
+  static_assert(sizeof h <= offsetof(ThickState, sel), "the state's head");
+  HIPCK(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, c->stream), "download");
   HIPCK(hipStreamSynchronize(c->stream), "synchronize");
-  float modes[2];
-  std::memcpy(modes, h + 2, sizeof modes);
-  *estimated = h[0] >= THICK_MIN_HITS ? 1 : 0;
+  if (h.bad_face) return fail(SHERD_E_ARG, "face index out of range");
+  *estimated = h.counts[0] >= THICK_MIN_HITS ? 1 : 0;
   if (*estimated) {
-    *thick_mode = modes[0];
-    *t = h[1] >= THICK_MIN_HITS ? modes[1] : modes[0];
+    *thick_mode = h.modes[0];
+    *t = h.counts[1] >= THICK_MIN_HITS ? h.modes[1] : h.modes[0];
   }
   return SHERD_OK;
 }
@@ -620,9 +632,11 @@ int sherd_thickness(sherd_ctx *ctx, const sherd_scene *scene, const double *V, s
   if (nf == 0) return SHERD_OK;
   if (nf > (size_t)INT_MAX / 3 || nv > (size_t)INT_MAX / 3) return fail(SHERD_E_LIMIT, "mesh too large");
   if ((size_t)scene->d.n != nf) return fail(SHERD_E_ARG, "the scene must be built over every face of F");
-  for (size_t i = 0; i < 3 * nf; ++i)
-    if (F[i] >= nv) return fail(SHERD_E_ARG, "face index out of range");
   hipError_t e = hipSuccess;
+  // Face indices are checked on the device (k_thick_normals raises the state's flag; the call then
+  // fails as an out-of-range index would on the host).
+  ThickState *st = thick_state(c, &e);
+  if (!st) return hip_fail(e, "scratch");
   double *dV = upload3(c, 0, V, nv, &e);
   if (!dV) return hip_fail(e, "upload vertices");
   uint32_t *dF = (uint32_t *)c->scratch(1, sizeof(uint32_t) * 3 * nf, &e);
@@ -639,11 +653,11 @@ int sherd_thickness(sherd_ctx *ctx, const sherd_scene *scene, const double *V, s
   if (!dp) return hip_fail(e, "scratch");
   Timer tm(c, SHERD_M_FIRST_HIT, (uint64_t)m);
   if (c->prof.on) c->prof.begin("k_thick_rays", c->stream);
-  HIPCK(launch_thick_normals(dV, dF, (int)nf, dN, c->stream), "normals");
-  HIPCK(launch_thick_rays(view_of(scene->d), dV, dF, dN, stride, m, dt, dp, c->stream), "rays");
+  HIPCK(launch_thick_normals(dV, (int)nv, dF, (int)nf, dN, &st->bad_face, c->stream), "normals");
+  HIPCK(launch_thick_rays(view_of(scene->d), dV, (int)nv, dF, dN, stride, m, dt, dp, c->stream), "rays");
   if (c->prof.on) c->prof.end(c->stream);
   tm.stop();
-  int rc = thick_modes(c, dN, (int)nf, nullptr, stride, dt, dp, m, t, thick_mode, estimated);
+  int rc = thick_modes(c, st, dN, (int)nf, nullptr, stride, dt, dp, m, t, thick_mode, estimated);
   c->prof.collect();
   return rc;
 }
@@ -672,7 +686,9 @@ int sherd_thickness_from_hits(sherd_ctx *ctx, const double *normals, size_t nf, 
   HIPCK(hipMemcpyAsync(didx, idx, sizeof(uint32_t) * n_rays, hipMemcpyHostToDevice, c->stream), "upload");
   HIPCK(hipMemcpyAsync(dt, t_hit, sizeof(float) * n_rays, hipMemcpyHostToDevice, c->stream), "upload");
   HIPCK(hipMemcpyAsync(dp, prim, sizeof(uint32_t) * n_rays, hipMemcpyHostToDevice, c->stream), "upload");
-  return thick_modes(c, dN, (int)nf, didx, 1, dt, dp, (int)n_rays, t, thick_mode, estimated);
+  ThickState *st = thick_state(c, &e);
+  if (!st) return hip_fail(e, "scratch");
+  return thick_modes(c, st, dN, (int)nf, didx, 1, dt, dp, (int)n_rays, t, thick_mode, estimated);
 }
 
 }  // extern "C"

@@ -262,13 +262,30 @@ hipError_t launch_cone(const SceneView &sv, const double *V, const uint32_t *F, 
                        const double *ns, int n, const ConeConst &cc, int retest, uint8_t *good, hipStream_t s);
 
 // ---- R §3.2 (thick.hip) -----------------------------------------------------------------------------
-hipError_t launch_thick_normals(const double *V, const uint32_t *F, int nf, double *N, hipStream_t s);
-hipError_t launch_thick_rays(const SceneView &sv, const double *V, const uint32_t *F, const double *N, int stride, int m,
-                             float *t_hit, uint32_t *prim, hipStream_t s);
-// The valid and the looking-back distances into ok_d / far_d (m floats each), their counts into
-// counts[0..1] (zeroed by the caller), and their histogram modes into modes[0..1] (NaN when empty).
+// R §3.2's device state, zeroed before each estimate: the two list counts, the two modes, a flag
+// raised by a face index outside V, and per list the selection's and the histogram's counts.
+constexpr int THICK_BINS = 60;
+struct ThickSel {
+  uint32_t hist[4][256];   // per digit pass, the keys with the selected prefix by their next digit
+  uint32_t gt_inv;         // ~(smallest key above the lower order statistic), 0 if none
+  uint32_t bins[THICK_BINS];
+  float first, last;       // the histogram's range
+  uint32_t pad;
+};
+struct ThickState {
+  uint32_t counts[2];  // valid hits, looking-back hits
+  float modes[2];      // their histogram modes (NaN when empty)
+  uint32_t bad_face;
+  uint32_t pad[3];
+  ThickSel sel[2];
+};
+hipError_t launch_thick_normals(const double *V, int nv, const uint32_t *F, int nf, double *N, uint32_t *bad,
+                                hipStream_t s);
+hipError_t launch_thick_rays(const SceneView &sv, const double *V, int nv, const uint32_t *F, const double *N,
+                             int stride, int m, float *t_hit, uint32_t *prim, hipStream_t s);
+// The valid and the looking-back distances into ok_d / far_d (m floats each), their counts and
+// histogram modes into st (zeroed by the caller).
 hipError_t launch_thick_modes(const double *N, int nf, const uint32_t *idx, int stride, const float *t_hit,
-                              const uint32_t *prim, int m, float *ok_d, float *far_d, uint32_t *counts,
-                              float *modes, hipStream_t s);
+                              const uint32_t *prim, int m, float *ok_d, float *far_d, ThickState *st, hipStream_t s);
 
 }  // namespace sherd

@@ -97,3 +97,20 @@ def test_too_few_hits(ex):
     scene = ex.scene(V.astype(np.float32), F)
     assert ex.thickness(scene, V, F) is None
     scene.close()
+
+
+def test_a_face_index_outside_the_mesh_is_an_error(ex):
+    say("case: thickness with a face index outside V (checked on the device)")
+    import sherd_hip
+
+    pc = S.piece("pieceA")
+    scene = ex.scene(pc["V0"].astype(np.float32), pc["F"])
+    good = ex.thickness(scene, pc["V0"], pc["F"])
+    F = pc["F"].copy()
+    F[len(F) // 2, 1] = len(pc["V0"])
+    with pytest.raises(sherd_hip.HipError, match="out of range"):
+        ex.thickness(scene, pc["V0"], F)
+    # The context stays usable and the next estimate starts from a clean state.
+    again = ex.thickness(scene, pc["V0"], pc["F"])
+    assert again[0].tobytes() == good[0].tobytes() and again[1].tobytes() == good[1].tobytes()
+    scene.close()
