@@ -58,7 +58,7 @@ typedef struct {
 
 enum {
   SHERD_M_FIRST_HIT, SHERD_M_BOUNDED_DISTANCE, SHERD_M_INSIDE, SHERD_M_COARSE, SHERD_M_ICP,
-  SHERD_M_NEAREST, SHERD_M_VERIFY, SHERD_M_SCENE_BUILD, SHERD_M_CONE, SHERD_M_HUBER, SHERD_M_COUNT
+  SHERD_M_NEAREST, SHERD_M_VERIFY, SHERD_M_SCENE_BUILD, SHERD_M_CONE, SHERD_M_HUBER, SHERD_M_SEGMENT, SHERD_M_COUNT
 };
 
 /* ---- the executor ------------------------------------------------------------------------- */
@@ -409,6 +409,36 @@ int sherd_thickness(sherd_ctx *ctx, const sherd_scene *scene, const double *V, s
 int sherd_thickness_from_hits(sherd_ctx *ctx, const double *normals, size_t nf, const uint32_t *idx,
                               const float *t_hit, const uint32_t *prim, size_t n_rays, float *t,
                               float *thick_mode, int *estimated);
+
+/* ---- R §3.4.4-3.4.8, the clean-up of the cone vote's mask -------------------------------------- */
+
+/* Optional outputs of sherd_segment_cleanup (each pointer may be null): the mask after the
+ * majority filter and after the two island passes (nf bytes each), the fixed shell reference
+ * normal and whether its ball held a shell face (nf x 3 f64, nf bytes), and the number of growth
+ * passes that found candidates. The fields of segment.rs:146-171 SegTrace this step produces. */
+typedef struct {
+  uint8_t *frac_majority;
+  uint8_t *frac_islands;
+  double *reference;
+  uint8_t *has_ref;
+  int passes;
+} sherd_seg_trace;
+
+/* segment.rs:544-635, steps 3.4.4 to 3.4.8 of segment::run after the cone vote, on the working
+ * mesh V (nv x 3 f64), F (nf x 3) with wall t = thick: the majority filter (majority_filter,
+ * segment.rs:403-428; balls of t/4 around the voxel representatives rep[n_rep], a face taking its
+ * nearest representative near[f]), fracture islands below 0.5 t^2 and shell islands below 2 t^2
+ * (mesh/components.rs:170-205 over mesh/adjacency.rs:138-158's pairs), the shell's reference
+ * normal over balls of t/2 (ball_normals, segment.rs:267-305), the boundary growth within
+ * boundary_angle degrees (refine_boundary, segment.rs:438-480; the default is 25), and the last
+ * fracture island pass. frac_raw is the vote's mask (1 = fracture); labels receives the final mask.
+ * rep and near are the voxel grid of coarse_grid (segment.rs:189-205). A component's area is the
+ * exact sum of its faces' areas in units of 2^(E-64), 2^(E-1) <= t^2 < 2^E; the reference sums in
+ * face order, which decides differently only for a component within its rounding of the bound.
+ * A non-positive or non-finite thick labels every face shell, as segment.rs:520-529 does. */
+int sherd_segment_cleanup(sherd_ctx *ctx, const double *V, size_t nv, const uint32_t *F, size_t nf, double thick,
+                          const uint8_t *frac_raw, const uint32_t *rep, size_t n_rep, const uint32_t *near,
+                          double boundary_angle, uint8_t *labels, sherd_seg_trace *trace);
 
 /* ---- per-kernel timing ------------------------------------------------------------------------ */
 

@@ -261,6 +261,30 @@ struct ConeConst {
 hipError_t launch_cone(const SceneView &sv, const double *V, const uint32_t *F, int nf, const int64_t *faces,
                        const double *ns, int n, const ConeConst &cc, int retest, uint8_t *good, hipStream_t s);
 
+// ---- R §3.4.4-3.4.8 (segment.hip) -----------------------------------------------------------------
+constexpr int SEG_BALL_CAP = 1024;  // faces within one representative's ball
+hipError_t launch_seg_geom(const double *V, int nv, const uint32_t *F, int nf, double *N, double *A, double *C,
+                           uint32_t *bad, hipStream_t s);
+hipError_t launch_seg_edge_keys(const uint32_t *F, int nf, uint32_t nv, uint64_t *key, uint32_t *pos, hipStream_t s);
+hipError_t launch_seg_pairs(const uint64_t *key, const uint32_t *pos, int m, int nf, uint32_t *fa, uint32_t *fb,
+                            uint32_t *count, hipStream_t s);
+// err bits: 1 a representative outside the faces, 2 a ball over SEG_BALL_CAP faces, 4 a face's
+// representative outside the list.
+hipError_t launch_seg_majority(const GridView &g, const double *C, const double *A, const uint8_t *frac_raw, int nf,
+                               const uint32_t *rep, int n_rep, const uint32_t *near, double r, uint8_t *maj,
+                               uint8_t *frac, uint32_t *err, hipStream_t s);
+hipError_t launch_seg_reference(const GridView &g, const double *C, const double *A, const double *N,
+                                const uint8_t *frac, int nf, const uint32_t *rep, int n_rep, const uint32_t *near,
+                                double r, double cos_grow, double *ref_g, uint8_t *has_g, double *ref, uint8_t *has,
+                                uint8_t *elig, uint32_t *err, hipStream_t s);
+hipError_t launch_cc_init(int n, uint32_t *parent, hipStream_t s);
+hipError_t launch_cc_round(const uint32_t *fa, const uint32_t *fb, int np, const uint8_t *mask, uint8_t target, int n,
+                           uint32_t *parent, uint32_t *changed, hipStream_t s);
+hipError_t launch_cc_drop(int n, uint8_t *mask, uint8_t target, const uint32_t *parent, const double *A, int sh,
+                          unsigned long long *acc, uint64_t bound_hi, uint64_t bound_lo, hipStream_t s);
+hipError_t launch_grow_pass(const uint32_t *fa, const uint32_t *fb, int np, int n, uint8_t *frac, uint8_t *cand,
+                            const uint8_t *elig, uint32_t *flags, hipStream_t s);
+
 // ---- R §3.2 (thick.hip) -----------------------------------------------------------------------------
 // R §3.2's device state, zeroed before each estimate: the two list counts, the two modes, a flag
 // raised by a face index outside V, and per list the selection's and the histogram's counts.

@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("SHERD_HIP_LIB") or os.path.join(HERE, "lib", "libsherd_hip.so")
 
 SHERD_M = dict(first_hit=0, bounded_distance=1, inside=2, coarse=3, icp=4, nearest=5, verify=6,
-               scene_build=7, cone=8, huber=9)
+               scene_build=7, cone=8, huber=9, segment=10)
 
 _lib = None
 
@@ -76,6 +76,11 @@ class MatchPair(C.Structure):
 class MatchResult(C.Structure):
     _fields_ = [("n_hyp", C.c_uint32), ("n_kept1", C.c_uint32), ("n_kept2", C.c_uint32), ("pad", C.c_uint32),
                 ("best1", C.c_double)]
+
+
+class SegTrace(C.Structure):
+    _fields_ = [("frac_majority", C.POINTER(C.c_uint8)), ("frac_islands", C.POINTER(C.c_uint8)),
+                ("reference", C.POINTER(C.c_double)), ("has_ref", C.POINTER(C.c_uint8)), ("passes", C.c_int)]
 
 
 class VerifyPair(C.Structure):
@@ -160,6 +165,8 @@ def _bind(L):
         "sherd_thickness_from_hits": (ci, [vp, P(d), sz, P(u32), P(C.c_float), P(u32), sz, P(C.c_float),
                                            P(C.c_float), P(ci)]),
         "sherd_verify_batch": (ci, [vp, P(VerifyPair), sz, P(u32), P(d), sz, ci, ci, P(d), P(u8)]),
+        "sherd_segment_cleanup": (ci, [vp, P(d), sz, P(u32), sz, d, P(u8), P(u32), sz, P(u32), d, P(u8),
+                                       P(SegTrace)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name, None)
@@ -731,6 +738,35 @@ class HipExecutor:
                                                _p(t_hit, C.c_float), _p(prim, C.c_uint32), len(t_hit), C.byref(t),
                                                C.byref(m), C.byref(ok)))
         return (np.float32(t.value), np.float32(m.value)) if ok.value else None
+
+    def segment_cleanup(self, V64, F, thick, frac_raw, rep, near, boundary_angle=25.0, trace=False):
+        """segment::run's steps 3.4.4-3.4.8 (segment.rs:544-635) after the cone vote: the final
+        fracture mask (bool per face); with trace, also a dict of the intermediate masks, the shell
+        reference normals, has_ref and the number of growth passes."""
+        V64 = f64(V64)
+        F = np.ascontiguousarray(F, dtype=np.uint32).reshape(-1, 3)
+        raw = np.ascontiguousarray(frac_raw, dtype=np.uint8)
+        rep = np.ascontiguousarray(rep, dtype=np.uint32)
+        near = np.ascontiguousarray(near, dtype=np.uint32)
+        n = len(F)
+        if len(raw) != n or len(near) != n:
+            raise HipError("frac_raw and near need one entry per face")
+        labels = np.empty(n, np.uint8)
+        tr = None
+        if trace:
+            bufs = dict(majority=np.empty(n, np.uint8), islands=np.empty(n, np.uint8),
+                        reference=np.empty((n, 3)), has_ref=np.empty(n, np.uint8))
+            tr = SegTrace(_p(bufs["majority"], C.c_uint8), _p(bufs["islands"], C.c_uint8),
+                          _p(bufs["reference"], C.c_double), _p(bufs["has_ref"], C.c_uint8), 0)
+        _check(lib().sherd_segment_cleanup(self.ptr, _p(V64, C.c_double), len(V64), _p(F, C.c_uint32), n, float(thick),
+                                           _p(raw, C.c_uint8), _p(rep, C.c_uint32), len(rep), _p(near, C.c_uint32),
+                                           float(boundary_angle), _p(labels, C.c_uint8),
+                                           C.byref(tr) if tr is not None else None))
+        if not trace:
+            return labels.astype(bool)
+        out = dict(majority=bufs["majority"].astype(bool), islands=bufs["islands"].astype(bool),
+                   reference=bufs["reference"], has_ref=bufs["has_ref"].astype(bool), passes=tr.passes)
+        return labels.astype(bool), out
 
     def cone_votes(self, scene, V64, F, faces, ns, thick, retest=False):
         """segment::classify_faces' seven-ray vote (segment.rs:318-381) for the listed faces."""

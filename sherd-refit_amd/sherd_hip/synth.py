@@ -397,3 +397,20 @@ def huber_workload(t=1.0, n_pairs=16, poses=32, nmin=10000, nmax=40000, length_t
             T0[k] = d @ truth
         pairs.append(dict(source=src, target=tgt, target_n=tn, truth=truth, T0=T0))
     return dict(pairs=pairs, delta=0.05 * t, radius=0.3 * t, t=t)
+
+
+def coarse_grid(centroids, spacing):
+    """The voxel grid R §3.4 cleans its vote over (crates/sherd-core/src/fragment/segment.rs:
+    189-260): the lowest face index of every occupied voxel of side `spacing` (voxel
+    floor((c - (min - 1)) / spacing)), ascending, and each face's nearest representative (a
+    KD-tree; ties may resolve otherwise than the reference's). Input for the segmentation clean-up
+    on synthetic meshes."""
+    from scipy.spatial import cKDTree
+
+    c = np.asarray(centroids, dtype=np.float64)
+    lo = c.min(axis=0) - 1.0
+    vox = np.floor((c - lo) / spacing).astype(np.int64)
+    _, first = np.unique(vox, axis=0, return_index=True)
+    rep = np.sort(first)
+    _, near = cKDTree(c[rep]).query(c)
+    return rep.astype(np.uint32), near.astype(np.uint32)

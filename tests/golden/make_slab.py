@@ -15,6 +15,9 @@ own arithmetic, spelled out below with the file:line each follows:
   (crates/sherd-parity/src/stages/pairs.rs:414-416);
 * the fracture-sample normals `FN[fp]` (crates/sherd-parity/src/stages/pairs.rs:286).
 
+R §3.4.4-3.4.8's row (seg.npz) is the dump's own seg.rep, seg.near, seg.frac_raw, seg.frac_majority,
+seg.frac_islands, seg.frac_final, seg.ref and seg.has_ref of each piece.
+
 R §9's row (refine.npz) is the dump's own: refine/<piece>.idx, refine/joins.json and
 refine/poses_final.json, the assembly's poses.json, groups.json and used.json it starts from, and
 each fragment's mesh.stats.json thickness and resolution (crates/sherd-parity/src/stages/refine.rs:
@@ -194,6 +197,20 @@ def write_refine(dump):
     for p in PIECES:
         out["idx_" + p] = np.load(os.path.join(rd, p + ".idx.npy")).astype(np.int64)
     np.savez_compressed(os.path.join(OUT, "refine.npz"), **out)
+
+    # R §3.4.4-3.4.8's inputs and intermediate masks for the segmentation post-processing row: the
+    # voxel representatives and each face's nearest one, the cone vote's mask, and what the majority
+    # filter, the island passes and the boundary growth left (crates/sherd-parity/src/stages/
+    # segmentation.rs compares the same files).
+    seg = {}
+    for p in PIECES:
+        d = os.path.join(dump, "fragments", p)
+        for k in ("rep", "near", "frac_raw", "frac_majority", "frac_islands", "frac_final", "ref", "has_ref"):
+            a = np.load(os.path.join(d, "seg." + k + ".npy"))
+            if a.dtype == np.int64:
+                a = a.astype(np.uint32)
+            seg[p + "_" + k] = a
+    np.savez_compressed(os.path.join(OUT, "seg.npz"), **seg)
 
 
 if __name__ == "__main__":
