@@ -292,6 +292,70 @@ def _surface_join(n, seed=51):
     return pts, nrm, D, h
 
 
+def segment_section(ex, col, args):
+    """R §3.4.4-3.4.8 (segment.rs:544-635) on each of the vessel's fragments: the cone vote's mask
+    (good < 5 of seven rays, the face normal standing in for the smoothed one) cleaned on the device
+    -- majority filter, islands, reference normals, growth, islands -- one call per fragment (the call
+    uploads the f64 mesh, the vote and the voxel grid); and, as the CPU baseline of this row, the
+    oracle and the device on a bounded sample (a 20 t high vessel fragment, same resolution)."""
+    from sherd_hip import synth
+
+    frags, t = col["fragments"], col["t"]
+    inputs = []
+    for fr in frags:
+        V = fr["V"].astype(np.float64)
+        F = fr["F"]
+        cen = ((V[F[:, 0]] + V[F[:, 1]]) + V[F[:, 2]]) / 3.0
+        rep, near = synth.coarse_grid(cen, t / 8.0)
+        sc = ex.scene(fr["V"].astype(np.float32), F)
+        good = ex.cone_votes(sc, V, F, None, fr["face_normals"], t)
+        sc.close()
+        inputs.append((V, F, good < 5, rep, near))
+    V, F, raw, rep, near = inputs[0]
+    ex.segment_cleanup(V, F, t, raw, rep, near)
+    ex.profile(True)
+    t0 = time.perf_counter()
+    out = [ex.segment_cleanup(V, F, t, raw, rep, near) for (V, F, raw, rep, near) in inputs]
+    el = time.perf_counter() - t0
+    prof = ex.profile_read()
+    ex.profile(False)
+    faces = sum(len(i[1]) for i in inputs)
+    res = dict(fragments=len(frags), faces=int(faces), representatives=int(sum(len(i[3]) for i in inputs)),
+               seconds=round(el, 4), faces_per_s=round(faces / el, 1),
+               phase_ms={k: round(v["ms"], 3) for k, v in prof.items() if k.startswith("k_seg") and v["ms"] > 0},
+               fracture_fraction_vote_cleaned=[[round(float(np.mean(i[2])), 4), round(float(np.mean(o)), 4)]
+                                               for i, o in zip(inputs, out)],
+               agreement_with_truth=[round(float(np.mean(o == fr["frac"])), 4) for o, fr in zip(out, frags)])
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import segment as SG
+
+        small = synth.vessel(t=t, edges_per_t=args.edges_per_t, n_frag=4, seed=5, height_t=20.0, rmin_t=10.0,
+                             rmax_t=16.0)["fragments"][0]
+        V = small["V"].astype(np.float64)
+        F = small["F"]
+        _, _, cen = SG.face_geometry(V, F)
+        rep, near = synth.coarse_grid(cen, t / 8.0)
+        raw = small["frac"].copy()
+        flip = np.random.default_rng(7).random(len(raw)) < 0.2
+        raw[flip] = ~raw[flip]
+        ex.segment_cleanup(V, F, t, raw, rep, near)
+        t0 = time.perf_counter()
+        got = ex.segment_cleanup(V, F, t, raw, rep, near)
+        dev = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        want = SG.cleanup(V, F, t, raw, rep, near)["final"]
+        cpu_s = time.perf_counter() - t0
+        res.update(cpu_sample=f"oracle cleanup (numpy + Python loops, 1 thread) of a {len(F)}-face vessel fragment "
+                              f"(20 t high, 20 % of the vote flipped)",
+                   cpu_seconds=round(cpu_s, 3), device_seconds_on_sample=round(dev, 4),
+                   identical_to_oracle_on_sample=bool(np.array_equal(got, want)))
+    log(f"segment: {len(frags)} fragments, {faces} faces in {1e3 * el:.1f} ms"
+        + (f"; sample {res['device_seconds_on_sample'] * 1e3:.1f} ms vs oracle {res['cpu_seconds']:.1f} s, identical "
+           f"{res['identical_to_oracle_on_sample']}" if "cpu_seconds" in res else ""))
+    return res
+
+
 def thickness_section(ex, col, args):
     """R §3.2 (thickness.rs:100-161) on each of the vessel's fragments: one inward ray from every
     face of the stride (every ceil(F / 300 000)-th face), the look-back filter and the two
@@ -610,6 +674,7 @@ def main():
     ap.add_argument("--no-refine", action="store_true", help="skip R §9's full-resolution refinement")
     ap.add_argument("--no-match", action="store_true", help="skip R §5.1-5.5 over the vessel's pairs")
     ap.add_argument("--no-thickness", action="store_true", help="skip R §3.2 over the vessel's fragments")
+    ap.add_argument("--no-segment", action="store_true", help="skip R §3.4.4-3.4.8 over the vessel's fragments")
     ap.add_argument("--no-collection", action="store_true",
                     help="skip the 40-fragment collection end to end (about 20 s of generation)")
     ap.add_argument("--batched-only", action="store_true",
@@ -735,6 +800,7 @@ def main():
     refine = refine_section(ex, col, args) if not args.no_refine else None
     match = match_section(ex, col, args) if not args.no_match else None
     thick = thickness_section(ex, col, args) if not args.no_thickness else None
+    segm = segment_section(ex, col, args) if not args.no_segment else None
 
     cpu = None
     if not args.no_cpu:
@@ -754,7 +820,7 @@ def main():
         plain_pair_verifications_per_s=round(plain, 3), batched_vs_plain=round(value / plain, 3),
         accepted=int(accepted.sum()), accepted_identical_plain_batched=same,
         rays_per_s=round(rays_per_s, 1), rays=int(rays), knn_queries_per_s=round(knn, 1),
-        lbvh_mtri_per_s=round(lbvh_mtri, 2), icp=icp, huber=huber, collection=coll, refine=refine, match=match, thickness=thick, kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
+        lbvh_mtri_per_s=round(lbvh_mtri, 2), icp=icp, huber=huber, collection=coll, refine=refine, match=match, thickness=thick, segment=segm, kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
     )
     print(json.dumps(line), flush=True)
     ex.close()

@@ -97,6 +97,17 @@ __device__ __forceinline__ GridSlot find_cell(const GridView &g, int64_t x, int6
   return s.key == want ? s : none;
 }
 
+// Lane k's value, read by every lane (k uniform across the wave).
+__device__ __forceinline__ double lane_read(double x, uint32_t k) {
+  const uint64_t b = (uint64_t)__double_as_longlong(x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, (int)k);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)k);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ uint32_t lane_read_u32(uint32_t x, uint32_t k) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)k);
+}
+
 // One wave per representative q (face rep[q]); lanes 0..26 take one cell each of the 27 around it.
 template <bool SHELL>
 __global__ void __launch_bounds__(64 * SEG_WAVES)
@@ -149,30 +160,45 @@ __global__ void __launch_bounds__(64 * SEG_WAVES)
     }
   }
   __syncthreads();
-  if (!live || over || lane != 0) return;
+  if (!live || over) return;
+  // The sums in ascending index order, 64 faces at a time: every lane fetches one face's terms
+  // (one memory round trip for 64 faces), then the whole wave adds them in order from lane reads,
+  // so each lane ends with the same sums the reference's sequential loop makes.
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, sw = 0.0;
+  for (uint32_t b = 0; b < n; b += 64) {
+    const uint32_t i = b + lane;
+    double a = 0.0, p0 = 0.0, p1 = 0.0, p2 = 0.0;
+    uint32_t wet = 0;
+    if (i < n) {
+      const uint32_t j = sorted[wv][i];
+      a = A[j];
+      if (SHELL) {
+        p0 = N[3 * (size_t)j] * a;
+        p1 = N[3 * (size_t)j + 1] * a;
+        p2 = N[3 * (size_t)j + 2] * a;
+      } else {
+        wet = frac[j] ? 1u : 0u;
+      }
+    }
+    const uint32_t cnt = n - b < 64u ? n - b : 64u;
+    for (uint32_t k = 0; k < cnt; ++k) {
+      const double ak = lane_read(a, k);
+      sw += ak;
+      if (SHELL) {
+        s0 += lane_read(p0, k);
+        s1 += lane_read(p1, k);
+        s2 += lane_read(p2, k);
+      } else if (lane_read_u32(wet, k)) {
+        s0 += ak;
+      }
+    }
+  }
+  if (lane != 0) return;
   if (!SHELL) {
-    double wet = 0.0, total = 0.0;
-    for (uint32_t i = 0; i < n; ++i) {
-      const uint32_t j = sorted[wv][i];
-      const double a = A[j];
-      total += a;
-      if (frac[j]) wet += a;
-    }
-    maj[q] = wet > 0.5 * total ? 1 : 0;
+    maj[q] = s0 > 0.5 * sw ? 1 : 0;  // wet > 0.5 total
   } else {
-    double m0 = 0.0, m1 = 0.0, m2 = 0.0, w = 0.0;
-    for (uint32_t i = 0; i < n; ++i) {
-      const uint32_t j = sorted[wv][i];
-      const double a = A[j];
-      m0 += N[3 * (size_t)j] * a;
-      m1 += N[3 * (size_t)j + 1] * a;
-      m2 += N[3 * (size_t)j + 2] * a;
-      w += a;
-    }
-    const double den = fmax(w, 1e-12);
-    m0 /= den;
-    m1 /= den;
-    m2 /= den;
+    const double den = fmax(sw, 1e-12);
+    const double m0 = s0 / den, m1 = s1 / den, m2 = s2 / den;
     const double norm = fmax(sqrt((m0 * m0 + m1 * m1) + m2 * m2), 1e-12);
     ref[3 * (size_t)q] = m0 / norm;
     ref[3 * (size_t)q + 1] = m1 / norm;
