@@ -108,7 +108,7 @@ __device__ __forceinline__ uint32_t lane_read_u32(uint32_t x, uint32_t k) {
   return (uint32_t)__builtin_amdgcn_readlane((int)x, (int)k);
 }
 
-// One wave per representative q (face rep[q]); lanes 0..26 take one cell each of the 27 around it.
+// One wave per representative q (face rep[q]).
 template <bool SHELL>
 __global__ void __launch_bounds__(64 * SEG_WAVES)
     k_seg_ball(GridView g, const double *C, const double *A, const double *N, const uint8_t *frac, int nf,
@@ -116,6 +116,7 @@ __global__ void __launch_bounds__(64 * SEG_WAVES)
   __shared__ uint32_t list[SEG_WAVES][SEG_BALL_CAP];
   __shared__ uint32_t sorted[SEG_WAVES][SEG_BALL_CAP];
   __shared__ uint32_t cnt[SEG_WAVES];
+  __shared__ uint32_t runs[SEG_WAVES][2][32];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int q = blockIdx.x * SEG_WAVES + wv;
   uint32_t f = 0;
@@ -127,23 +128,40 @@ __global__ void __launch_bounds__(64 * SEG_WAVES)
       live = false;
     }
   }
+  // Lanes 0..26 find the 27 cells' runs of points; then the runs, laid end to end, are tested 64
+  // points at a time across the whole wave.
   if (lane == 0) cnt[wv] = 0;
-  __syncthreads();
+  if (lane < 32) runs[wv][0][lane] = runs[wv][1][lane] = 0;
+  double qx = 0.0, qy = 0.0, qz = 0.0;
+  if (live) {
+    qx = C[3 * (size_t)f];
+    qy = C[3 * (size_t)f + 1];
+    qz = C[3 * (size_t)f + 2];
+  }
   if (live && lane < 27) {
-    const double qx = C[3 * (size_t)f], qy = C[3 * (size_t)f + 1], qz = C[3 * (size_t)f + 2];
-    const double r2 = r * r;
     const int64_t cx = (int64_t)floor((qx - g.origin[0]) / g.cell) + (lane / 9 - 1);
     const int64_t cy = (int64_t)floor((qy - g.origin[1]) / g.cell) + ((lane / 3) % 3 - 1);
     const int64_t cz = (int64_t)floor((qz - g.origin[2]) / g.cell) + (lane % 3 - 1);
     const GridSlot s = find_cell(g, cx, cy, cz);
-    for (uint32_t k = s.start; k < s.start + s.count; ++k) {
-      const double *p = g.pts + 3 * (size_t)k;
+    runs[wv][0][lane] = s.count;
+    runs[wv][1][lane] = s.start;
+  }
+  __syncthreads();
+  if (live) {
+    const double r2 = r * r;
+    uint32_t total = 0;
+    for (int k = 0; k < 27; ++k) total += runs[wv][0][k];
+    uint32_t k = 0, before = 0;  // the run holding point idx and the points ahead of it
+    for (uint32_t idx = lane; idx < total; idx += 64) {
+      while (idx >= before + runs[wv][0][k]) before += runs[wv][0][k++];
+      const uint32_t at = runs[wv][1][k] + (idx - before);
+      const double *p = g.pts + 3 * (size_t)at;
       const double d2 = d2_f64(qx, qy, qz, gldd(p), gldd(p + 1), gldd(p + 2));
       if (!(d2 <= r2)) continue;
-      const uint32_t j = gldu(g.idx + k);
+      const uint32_t j = gldu(g.idx + at);
       if (SHELL && frac[j]) continue;
-      const uint32_t at = atomicAdd(&cnt[wv], 1u);
-      if (at < SEG_BALL_CAP) list[wv][at] = j;
+      const uint32_t slot = atomicAdd(&cnt[wv], 1u);
+      if (slot < SEG_BALL_CAP) list[wv][slot] = j;
     }
   }
   __syncthreads();
