@@ -440,6 +440,14 @@ int sherd_segment_cleanup(sherd_ctx *ctx, const double *V, size_t nv, const uint
                           const uint8_t *frac_raw, const uint32_t *rep, size_t n_rep, const uint32_t *near,
                           double boundary_angle, uint8_t *labels, sherd_seg_trace *trace);
 
+/* R §3.4.2's smoothed normals (segment.rs:540-546 with ball_normals, :267-305): per voxel
+ * representative rep[q] the area-weighted mean normal of every face whose centroid lies within
+ * radius of its centroid (kiddo's squared distance, the ball summed in ascending face order),
+ * normalised with the reference's two 1e-12 floors; face f receives its representative near[f]'s
+ * (ns, nf x 3 f64). The reference's radius is max(t/3, smooth_res res). */
+int sherd_smoothed_normals(sherd_ctx *ctx, const double *V, size_t nv, const uint32_t *F, size_t nf,
+                           const uint32_t *rep, size_t n_rep, const uint32_t *near, double radius, double *ns);
+
 /* ---- per-kernel timing ------------------------------------------------------------------------ */
 
 /* Turns per-kernel HIP-event timing on the context's stream on (1) or off (0), clearing the

@@ -231,4 +231,60 @@ int sherd_segment_cleanup(sherd_ctx *ctx, const double *V, size_t nv, const uint
   return SHERD_OK;
 }
 
+int sherd_smoothed_normals(sherd_ctx *ctx, const double *V, size_t nv, const uint32_t *F, size_t nf,
+                           const uint32_t *rep, size_t n_rep, const uint32_t *near, double radius, double *ns) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  LOCK(c);
+  if (nf == 0) return SHERD_OK;
+  if (!V || !F || !rep || !near || !ns) return fail(SHERD_E_ARG, "null buffer");
+  if (nf > (size_t)INT_MAX / 3 || nv > (size_t)UINT32_MAX || n_rep > (size_t)INT_MAX)
+    return fail(SHERD_E_LIMIT, "mesh too large");
+  if (n_rep == 0) return fail(SHERD_E_ARG, "no voxel representatives");
+  if (!std::isfinite(radius) || radius < 0.0) return fail(SHERD_E_ARG, "radius must be finite and non-negative");
+  const int n = (int)nf;
+  hipStream_t s = c->stream;
+  Arena ar;
+  const size_t oV = ar.take<double>(3 * nv), oF = ar.take<uint32_t>(3 * nf), oN = ar.take<double>(3 * nf),
+               oA = ar.take<double>(nf), oC = ar.take<double>(3 * nf), oRep = ar.take<uint32_t>(n_rep),
+               oNear = ar.take<uint32_t>(nf), oRefG = ar.take<double>(3 * n_rep), oHasG = ar.take<uint8_t>(n_rep),
+               oNs = ar.take<double>(3 * nf), oFlags = ar.take<uint32_t>(8);
+  hipError_t e;
+  char *base = (char *)c->scratch(SL_SEG, ar.used, &e);
+  if (!base) return hip_fail(e, "scratch");
+  double *dV = (double *)(base + oV), *dN = (double *)(base + oN), *dA = (double *)(base + oA),
+         *dC = (double *)(base + oC), *dRefG = (double *)(base + oRefG), *dNs = (double *)(base + oNs);
+  uint32_t *dF = (uint32_t *)(base + oF), *dRep = (uint32_t *)(base + oRep), *dNear = (uint32_t *)(base + oNear),
+           *dFlags = (uint32_t *)(base + oFlags);
+  uint8_t *dHasG = (uint8_t *)(base + oHasG);
+  HIPCK(hipMemsetAsync(dFlags, 0, 8 * sizeof(uint32_t), s), "clear");
+  HIPCK(hipMemcpyAsync(dV, V, sizeof(double) * 3 * nv, hipMemcpyHostToDevice, s), "upload");
+  HIPCK(hipMemcpyAsync(dF, F, sizeof(uint32_t) * 3 * nf, hipMemcpyHostToDevice, s), "upload");
+  HIPCK(hipMemcpyAsync(dRep, rep, sizeof(uint32_t) * n_rep, hipMemcpyHostToDevice, s), "upload");
+  HIPCK(hipMemcpyAsync(dNear, near, sizeof(uint32_t) * nf, hipMemcpyHostToDevice, s), "upload");
+  Timer tm(c, SHERD_M_SEGMENT, nf);
+  HIPCK(launch_seg_geom(dV, (int)nv, dF, n, dN, dA, dC, dFlags, s), "face geometry");
+  uint32_t bad = 0;
+  HIPCK(hipMemcpyAsync(&bad, dFlags, sizeof bad, hipMemcpyDeviceToHost, s), "download");
+  HIPCK(hipStreamSynchronize(s), "synchronize");
+  if (bad & 1u) return fail(SHERD_E_ARG, "face index out of range");
+  DevGrid grid;
+  HIPCK(grid_build(dC, n, radius > 0.0 ? radius : 1.0, &grid, s), "centroid grid");
+  struct GridGuard {
+    DevGrid *g;
+    ~GridGuard() { grid_free(g); }
+  } guard{&grid};
+  HIPCK(launch_seg_smoothed(view_of(grid), dC, dA, dN, n, dRep, (int)n_rep, dNear, radius, dRefG, dHasG, dNs, dFlags,
+                            s),
+        "smoothed normals");
+  HIPCK(hipMemcpyAsync(ns, dNs, sizeof(double) * 3 * nf, hipMemcpyDeviceToHost, s), "download");
+  uint32_t err = 0;
+  HIPCK(hipMemcpyAsync(&err, dFlags, sizeof err, hipMemcpyDeviceToHost, s), "download");
+  HIPCK(hipStreamSynchronize(s), "synchronize");
+  tm.stop();
+  if (err & 1u) return fail(SHERD_E_ARG, "a representative outside the faces");
+  if (err & 2u) return fail(SHERD_E_LIMIT, "more faces within a representative's ball than the kernel holds");
+  if (err & 4u) return fail(SHERD_E_ARG, "a face's representative outside the list");
+  return SHERD_OK;
+}
+
 }  // extern "C"

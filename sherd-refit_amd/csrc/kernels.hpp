@@ -277,6 +277,10 @@ hipError_t launch_seg_reference(const GridView &g, const double *C, const double
                                 const uint8_t *frac, int nf, const uint32_t *rep, int n_rep, const uint32_t *near,
                                 double r, double cos_grow, double *ref_g, uint8_t *has_g, double *ref, uint8_t *has,
                                 uint8_t *elig, uint32_t *err, hipStream_t s);
+// R §3.4.2: the area-weighted normal of every face within r of each representative, per face.
+hipError_t launch_seg_smoothed(const GridView &g, const double *C, const double *A, const double *N, int nf,
+                               const uint32_t *rep, int n_rep, const uint32_t *near, double r, double *ref_g,
+                               uint8_t *has_g, double *ns, uint32_t *err, hipStream_t s);
 hipError_t launch_cc_init(int n, uint32_t *parent, hipStream_t s);
 hipError_t launch_cc_round(const uint32_t *fa, const uint32_t *fb, int np, const uint8_t *mask, uint8_t target, int n,
                            uint32_t *parent, uint32_t *changed, hipStream_t s);

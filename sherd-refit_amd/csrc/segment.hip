@@ -10,7 +10,8 @@
 //                     neighbour grid (kiddo's (dx^2 + dy^2) + dz^2 <= r^2), put in ascending index
 //                     order by rank, then summed in that order as PointTree::within + the
 //                     reference's loop do: the majority's area sums (S = false), or the shell's
-//                     area-weighted normal (S = true, shell faces only)
+//                     area-weighted normal (S = true, shell faces only; every face when no mask
+//                     is given: R §3.4.2's smoothed normals)
 //   k_cc_*            components of mask == target: hooking of roots over the pairs and pointer
 //                     jumping until nothing changes; each component's exact area (fixed point,
 //                     four 32-bit digits summed in 64-bit counters, a wave of one component adding
@@ -159,7 +160,7 @@ __global__ void __launch_bounds__(64 * SEG_WAVES)
       const double d2 = d2_f64(qx, qy, qz, gldd(p), gldd(p + 1), gldd(p + 2));
       if (!(d2 <= r2)) continue;
       const uint32_t j = gldu(g.idx + at);
-      if (SHELL && frac[j]) continue;
+      if (SHELL && frac && frac[j]) continue;
       const uint32_t slot = atomicAdd(&cnt[wv], 1u);
       if (slot < SEG_BALL_CAP) list[wv][slot] = j;
     }
@@ -254,6 +255,21 @@ __global__ void k_seg_take_reference(const uint32_t *near, int nf, int n_rep, co
   has[f] = has_g[r];
   const double *n = N + 3 * (size_t)f;
   elig[f] = (has_g[r] && (n[0] * r0 + n[1] * r1) + n[2] * r2 > cos_grow) ? 1 : 0;
+}
+
+// A face's smoothed normal is its representative's ball normal (segment.rs:545-546).
+__global__ void k_seg_take_normal(const uint32_t *near, int nf, int n_rep, const double *ref_g, double *ns,
+                                  uint32_t *err) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const uint32_t r = near[f];
+  if (r >= (uint32_t)n_rep) {
+    atomicOr(err, 4u);
+    return;
+  }
+  ns[3 * (size_t)f] = ref_g[3 * (size_t)r];
+  ns[3 * (size_t)f + 1] = ref_g[3 * (size_t)r + 1];
+  ns[3 * (size_t)f + 2] = ref_g[3 * (size_t)r + 2];
 }
 
 // ---- components -------------------------------------------------------------------------------
@@ -423,6 +439,17 @@ hipError_t launch_seg_reference(const GridView &g, const double *C, const double
                        0, s, g, C, A, N, frac, nf, rep, n_rep, r, (uint8_t *)nullptr, ref_g, has_g, err);
   hipLaunchKernelGGL(k_seg_take_reference, blocks(nf), dim3(256), 0, s, near, nf, n_rep, ref_g, has_g, N, cos_grow, ref,
                      has, elig);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_smoothed(const GridView &g, const double *C, const double *A, const double *N, int nf,
+                               const uint32_t *rep, int n_rep, const uint32_t *near, double r, double *ref_g,
+                               uint8_t *has_g, double *ns, uint32_t *err, hipStream_t s) {
+  if (n_rep > 0)
+    hipLaunchKernelGGL(k_seg_ball<true>, dim3((unsigned)((n_rep + SEG_WAVES - 1) / SEG_WAVES)), dim3(64 * SEG_WAVES),
+                       0, s, g, C, A, N, (const uint8_t *)nullptr, nf, rep, n_rep, r, (uint8_t *)nullptr, ref_g, has_g,
+                       err);
+  hipLaunchKernelGGL(k_seg_take_normal, blocks(nf), dim3(256), 0, s, near, nf, n_rep, ref_g, ns, err);
   return hipGetLastError();
 }
 

@@ -165,6 +165,7 @@ def _bind(L):
         "sherd_thickness_from_hits": (ci, [vp, P(d), sz, P(u32), P(C.c_float), P(u32), sz, P(C.c_float),
                                            P(C.c_float), P(ci)]),
         "sherd_verify_batch": (ci, [vp, P(VerifyPair), sz, P(u32), P(d), sz, ci, ci, P(d), P(u8)]),
+        "sherd_smoothed_normals": (ci, [vp, P(d), sz, P(u32), sz, P(u32), sz, P(u32), d, P(d)]),
         "sherd_segment_cleanup": (ci, [vp, P(d), sz, P(u32), sz, d, P(u8), P(u32), sz, P(u32), d, P(u8),
                                        P(SegTrace)]),
     }
@@ -738,6 +739,21 @@ class HipExecutor:
                                                _p(t_hit, C.c_float), _p(prim, C.c_uint32), len(t_hit), C.byref(t),
                                                C.byref(m), C.byref(ok)))
         return (np.float32(t.value), np.float32(m.value)) if ok.value else None
+
+    def smoothed_normals(self, V64, F, rep, near, radius):
+        """R §3.4.2's smoothed normal per face (segment.rs:540-546): the ball normal of its voxel
+        representative."""
+        V64 = f64(V64)
+        F = np.ascontiguousarray(F, dtype=np.uint32).reshape(-1, 3)
+        rep = np.ascontiguousarray(rep, dtype=np.uint32)
+        near = np.ascontiguousarray(near, dtype=np.uint32)
+        if len(near) != len(F):
+            raise HipError("near needs one entry per face")
+        ns = np.empty((len(F), 3))
+        _check(lib().sherd_smoothed_normals(self.ptr, _p(V64, C.c_double), len(V64), _p(F, C.c_uint32), len(F),
+                                            _p(rep, C.c_uint32), len(rep), _p(near, C.c_uint32), float(radius),
+                                            _p(ns, C.c_double)))
+        return ns
 
     def segment_cleanup(self, V64, F, thick, frac_raw, rep, near, boundary_angle=25.0, trace=False):
         """segment::run's steps 3.4.4-3.4.8 (segment.rs:544-635) after the cone vote: the final

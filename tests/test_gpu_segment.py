@@ -105,3 +105,20 @@ def test_errors_and_no_wall(ex, seg):
     # The context stays usable.
     got = ex.segment_cleanup(V, F, float(g["thick_t"]), raw, rep, seg["pieceB_near"])
     assert np.array_equal(got, seg["pieceB_frac_final"])
+
+
+@pytest.mark.parametrize("name", ["pieceA", "pieceB"])
+def test_smoothed_normals(ex, seg, name):
+    say(f"case: R 3.4.2 smoothed normals, {name}")
+    g = np.load(os.path.join(GOLD, name + ".npz"))
+    V, F, t = g["V"], g["F"], float(g["thick_t"])
+    rep, near = seg[name + "_rep"], seg[name + "_near"]
+    got = ex.smoothed_normals(V, F, rep, near, t / 3.0)
+    normals, areas, cen = SG.face_geometry(V, F)
+    ref_g, _ = SG.ball_normals(cen, normals, areas, np.ones(len(F), bool), cen[rep.astype(np.int64)], t / 3.0)
+    want = ref_g[near.astype(np.int64)]
+    assert got.tobytes() == want.tobytes()
+    # The dump's own NS (every fourth face), whose Python balls sum in another order.
+    faces = g["seg_faces"]
+    cosang = np.sum(got[faces] * g["seg_NS"], axis=1)
+    assert cosang.min() > np.cos(np.radians(1e-5))
