@@ -18,7 +18,11 @@ the device. `value` is pair-verifications per second over the timed steps. Besid
     measured with HIP events on the library's own stream around every launch (the sections that
     run concurrently on side streams are listed beside it);
   * the CPU baseline: the oracle (the C++ restatement of the CPU executor) on a bounded sample of
-    the same candidates, on this host's cores.
+    the same candidates, on this host's cores;
+  * the rows next to the path, each with its own oracle sample: the batched ICP ladder (`icp`),
+    the robust seam energy and its LM (`huber`), the 40-fragment collection (`collection`), R §9's
+    refinement (`refine`), R §5.1-5.5 over the vessel's pairs (`match`), R §3.2's wall thickness
+    (`thickness`) and R §3.4.4-3.4.8's segmentation clean-up (`segment`) over its fragments.
 """
 
 import argparse
