@@ -132,17 +132,6 @@ void KernelProfiler::collect() {
 
 }  // namespace sherd
 
-#define LOCK(ctx)                                                    \
-  if (!(ctx)) return fail(SHERD_E_ARG, "null context");             \
-  std::lock_guard<std::mutex> lock_((ctx)->mu);                      \
-  if (hipSetDevice((ctx)->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice")
-
-#define HIPCK(call, what)                                            \
-  do {                                                               \
-    hipError_t e_ = (call);                                          \
-    if (e_ != hipSuccess) return hip_fail(e_, what);                 \
-  } while (0)
-
 extern "C" {
 
 const char *sherd_name(void) { return "hip"; }

@@ -13,12 +13,6 @@
 
 using namespace sherd;
 
-#define HIPCK(call, what)                            \
-  do {                                               \
-    hipError_t e_ = (call);                          \
-    if (e_ != hipSuccess) return hip_fail(e_, what); \
-  } while (0)
-
 struct sherd_huber {
   Ctx *ctx = nullptr;
   int k = 0;
@@ -76,8 +70,7 @@ int sherd_huber_create(sherd_ctx *ctx, const sherd_icp_pair *pairs, size_t n_pai
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   if (!c || !out) return fail(SHERD_E_ARG, "null argument");
   *out = nullptr;
-  std::lock_guard<std::mutex> lock(c->mu);
-  if (hipSetDevice(c->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
+  LOCK(c);
   if (k == 0 || !pairs || !cand_pair || !T0) return fail(SHERD_E_ARG, "empty batch or null buffer");
   if (k > (size_t)INT_MAX || n_pairs > (size_t)INT_MAX) return fail(SHERD_E_LIMIT, "batch too large");
   std::vector<IcpPairDesc> pd(n_pairs);
@@ -152,8 +145,7 @@ void sherd_huber_free(sherd_huber *h) {
 int sherd_huber_correspondences(sherd_huber *h, int32_t *corr, size_t n) {
   if (!h || (!corr && n)) return fail(SHERD_E_ARG, "null argument");
   Ctx *c = h->ctx;
-  std::lock_guard<std::mutex> lock(c->mu);
-  if (hipSetDevice(c->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
+  LOCK(c);
   if (n != h->points) return fail(SHERD_E_ARG, "buffer size differs from the batch's source points");
   HIPCK(hipMemcpyAsync(corr, h->corr, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream), "download");
   HIPCK(hipStreamSynchronize(c->stream), "synchronize");
@@ -165,8 +157,7 @@ int sherd_huber_eval(sherd_huber *h, const double *xi, double delta, int precisi
   if (precision != 32 && precision != 64) return fail(SHERD_E_ARG, "precision must be 32 or 64");
   if (!(delta > 0.0)) return fail(SHERD_E_ARG, "delta must be positive");
   Ctx *c = h->ctx;
-  std::lock_guard<std::mutex> lock(c->mu);
-  if (hipSetDevice(c->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
+  LOCK(c);
   size_t k = (size_t)h->k;
   HIPCK(hipMemcpyAsync(h->xi, xi, sizeof(double) * 6 * k, hipMemcpyHostToDevice, c->stream), "upload");
   Timer tm(c, SHERD_M_HUBER, k);
@@ -188,8 +179,7 @@ int sherd_huber_lm(sherd_huber *h, double delta, int iterations, int precision, 
   if (precision != 32 && precision != 64) return fail(SHERD_E_ARG, "precision must be 32 or 64");
   if (!(delta > 0.0) || iterations < 0) return fail(SHERD_E_ARG, "bad delta or iteration count");
   Ctx *c = h->ctx;
-  std::lock_guard<std::mutex> lock(c->mu);
-  if (hipSetDevice(c->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
+  LOCK(c);
   size_t k = (size_t)h->k;
   Timer tm(c, SHERD_M_HUBER, k);
   if (c->prof.on) c->prof.begin("k_huber_lm", c->stream);

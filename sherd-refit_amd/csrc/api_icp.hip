@@ -16,17 +16,6 @@
 
 using namespace sherd;
 
-#define LOCK(ctx)                                                    \
-  if (!(ctx)) return fail(SHERD_E_ARG, "null context");             \
-  std::lock_guard<std::mutex> lock_((ctx)->mu);                      \
-  if (hipSetDevice((ctx)->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice")
-
-#define HIPCK(call, what)                            \
-  do {                                               \
-    hipError_t e_ = (call);                          \
-    if (e_ != hipSuccess) return hip_fail(e_, what); \
-  } while (0)
-
 
 
 namespace {
@@ -280,6 +269,7 @@ int grid_for(Ctx *c, int block, size_t n, int max_spans) {
 struct BatchDev {
   std::vector<IcpBatchView> v;
   std::vector<IcpWideView> w;  // the wide view of each rung (used by the wide ones)
+  std::vector<char> host;      // the upload's host copy, alive until the stream has drained
 };
 
 int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand_pair, size_t n_cand, double *dT,
@@ -312,7 +302,8 @@ int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand
     at[F * r + 4] = bytes;
     bytes = al(bytes + sizeof(uint32_t) * plans[r].span_first.size());
   }
-  std::vector<char> blob(bytes, 0);
+  std::vector<char> &blob = bd->host;
+  blob.assign(bytes, 0);
   for (size_t r = 0; r < nr; ++r) {
     const RungPlan &pl = plans[r];
     std::memcpy(&blob[at[F * r]], pl.pd.data(), sizeof(IcpPairDesc) * pl.pd.size());
@@ -456,6 +447,9 @@ int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_
         any_skip = true;
       }
   BatchDev bd;
+  // Every return below leaves with the stream drained, so the copies queued from `skip` and
+  // `bd.host` have finished before those buffers go.
+  StreamFence fence(c->stream);
   if ((rc = stage_batch(c, plans, cand_pair, n_cand, dT, dout, &bd))) return rc;
   HIPCK(hipMemcpyAsync(dT, inits, sizeof(double) * 16 * n_cand, hipMemcpyHostToDevice, c->stream), "upload");
   HIPCK(hipMemcpyAsync(dskip, skip.data(), skip.size(), hipMemcpyHostToDevice, c->stream), "upload");

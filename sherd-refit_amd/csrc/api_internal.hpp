@@ -60,7 +60,33 @@ class Timer {
 int fail(int code, const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
 
+// Synchronises a stream when it goes out of scope. Declared after the host buffers that queued
+// copies read from or write to, it keeps them alive until the stream has drained, on every return
+// path (an early error return included).
+struct StreamFence {
+  hipStream_t s;
+  explicit StreamFence(hipStream_t stream) : s(stream) {}
+  StreamFence(const StreamFence &) = delete;
+  StreamFence &operator=(const StreamFence &) = delete;
+  ~StreamFence() { (void)hipStreamSynchronize(s); }
+};
+
 }  // namespace sherd
+
+// An entry point's prologue: the context's lock, its device, and a fence that drains the context's
+// stream before the lock is released, so no copy the call queued outlives the call (the caller's
+// buffers and the pinned staging buffer are free to change once it returns).
+#define LOCK(ctx)                                                                \
+  if (!(ctx)) return ::sherd::fail(SHERD_E_ARG, "null context");                 \
+  std::lock_guard<std::mutex> lock_((ctx)->mu);                                  \
+  ::sherd::StreamFence fence_((ctx)->stream);                                    \
+  if (hipSetDevice((ctx)->device) != hipSuccess) return ::sherd::hip_fail(hipGetLastError(), "hipSetDevice")
+
+#define HIPCK(call, what)                                    \
+  do {                                                       \
+    hipError_t e_ = (call);                                  \
+    if (e_ != hipSuccess) return ::sherd::hip_fail(e_, what); \
+  } while (0)
 
 struct sherd_scene {
   sherd::Ctx *ctx = nullptr;

@@ -20,17 +20,6 @@ using namespace sherd;
 
 namespace {
 
-#define LOCK(ctx)                                                    \
-  if (!(ctx)) return fail(SHERD_E_ARG, "null context");             \
-  std::lock_guard<std::mutex> lock_((ctx)->mu);                      \
-  if (hipSetDevice((ctx)->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice")
-
-#define HIPCK(call, what)                            \
-  do {                                               \
-    hipError_t e_ = (call);                          \
-    if (e_ != hipSuccess) return hip_fail(e_, what); \
-  } while (0)
-
 // pair.rs:50-52 and ladder.rs:51-61.
 constexpr uint32_t COARSE_ORDER_LIMIT = 5000;
 constexpr double COARSE_FLOOR = 0.1;

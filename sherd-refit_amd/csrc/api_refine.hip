@@ -216,11 +216,6 @@ int select_vertices(Ctx *c, const DevMesh &m, size_t nv, const double *centroids
   return SHERD_OK;
 }
 
-#define LOCK(ctx)                                                    \
-  if (!(ctx)) return fail(SHERD_E_ARG, "null context");             \
-  std::lock_guard<std::mutex> lock_((ctx)->mu);                      \
-  if (hipSetDevice((ctx)->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice")
-
 // One join of the walk: its endpoints and its depth below the group's first fragment.
 struct Step {
   uint32_t fixed, moving;

@@ -27,17 +27,6 @@ constexpr double MIN_FRACTURE_T2 = 0.5;   // segment.rs:71
 constexpr double MIN_SHELL_T2 = 2.0;      // segment.rs:73
 constexpr int MAX_GROWTH_PASSES = 60;     // segment.rs:75
 
-#define LOCK(ctx)                                                    \
-  if (!(ctx)) return fail(SHERD_E_ARG, "null context");             \
-  std::lock_guard<std::mutex> lock_((ctx)->mu);                      \
-  if (hipSetDevice((ctx)->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice")
-
-#define HIPCK(call, what)                            \
-  do {                                               \
-    hipError_t e_ = (call);                          \
-    if (e_ != hipSuccess) return hip_fail(e_, what); \
-  } while (0)
-
 // Bump allocation of one call's buffers inside one grow-only scratch slot.
 struct Arena {
   size_t used = 0;

@@ -18,12 +18,6 @@
 
 using namespace sherd;
 
-#define HIPCK(call, what)                            \
-  do {                                               \
-    hipError_t e_ = (call);                          \
-    if (e_ != hipSuccess) return hip_fail(e_, what); \
-  } while (0)
-
 namespace {
 
 // Copies n x 3 f64 to a fresh device allocation (nullptr for n == 0).
@@ -102,8 +96,7 @@ int sherd_fragment_create(sherd_ctx *ctx, const sherd_scene *fracture, const she
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   if (!c || !out) return fail(SHERD_E_ARG, "null argument");
   *out = nullptr;
-  std::lock_guard<std::mutex> lock(c->mu);
-  if (hipSetDevice(c->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
+  LOCK(c);
   if (!fracture || fracture->ctx != c || (mesh && mesh->ctx != c)) return fail(SHERD_E_ARG, "scene of another context");
   if ((n_pf && !pf) || (n_s && !s) || (n_brk && (!brk_p || !brk_ns)) || (n_margin && (!margin_p || !margin_n)))
     return fail(SHERD_E_ARG, "null sample array");
@@ -157,8 +150,7 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
                        uint8_t *accepted) {
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   if (!c) return fail(SHERD_E_ARG, "null context");
-  std::lock_guard<std::mutex> lock(c->mu);
-  if (hipSetDevice(c->device) != hipSuccess) return hip_fail(hipGetLastError(), "hipSetDevice");
+  LOCK(c);
   if (n_cand == 0) return SHERD_OK;
   if (!pairs || !cand_pair || !cand_T || !scores || !accepted) return fail(SHERD_E_ARG, "null buffer");
   if (n_pairs > (size_t)INT_MAX || n_cand > (size_t)INT_MAX / 4) return fail(SHERD_E_LIMIT, "batch too large");
