@@ -99,6 +99,16 @@ int sherd_first_hit(sherd_ctx *ctx, const sherd_scene *scene, const float *orig,
 int sherd_bounded_distance(sherd_ctx *ctx, const sherd_scene *scene, const double *pts, size_t n,
                            const double T[16], double max_dist, int reduce, double *out);
 
+/* No counterpart in the reference. sherd_bounded_distance without any allocation, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same kernels and results, with every buffer beyond the caller's host
+ * arrays and the scene in a fixed 9 MiB device array per device (taken at sherd_open). At most
+ * 262144 points per call (SHERD_E_LIMIT above). Calls on one device share that array: calls that
+ * overlap on one device (two contexts on the same GPU, from two threads) race; calls through one
+ * context are serialised like every other entry point. */
+int sherd_bounded_distance_noalloc(sherd_ctx *ctx, const sherd_scene *scene, const double *pts, size_t n,
+                                   const double T[16], double max_dist, int reduce, double *out);
+
 /* Executor::inside (executor/mod.rs:124; cpu.rs:115-128, 182-188; InsideBatch at batch.rs:316):
  * the AABB reject, three axis rays with a majority of odd crossing counts, the depth of an inside
  * point. retest = 0 is the reference's arithmetic; 1 adds the FP64 retest of degenerate hits. */

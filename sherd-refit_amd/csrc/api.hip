@@ -132,6 +132,40 @@ void KernelProfiler::collect() {
 
 }  // namespace sherd
 
+namespace {
+
+// ---- the no-allocation form of bounded_distance -------------------------------------------------
+//
+// `scratch` is the only device memory sherd_bounded_distance_noalloc uses beyond the scene: the
+// moved-from points (n x 3 f64), the per-point f32 distances, their f64 widening, and the running
+// minimum's bits. Its size is fixed here; each device that opens a context holds one copy until the
+// process ends, and every call on that device shares it.
+constexpr size_t NOALLOC_MAX_POINTS = (size_t)1 << 18;
+constexpr size_t NA_PTS = 0;                                                // f64 x 3n
+constexpr size_t NA_DIST = NA_PTS + sizeof(double) * 3 * NOALLOC_MAX_POINTS;  // f32 x n
+constexpr size_t NA_WIDE = NA_DIST + sizeof(float) * NOALLOC_MAX_POINTS;      // f64 x n
+constexpr size_t NA_BEST = NA_WIDE + sizeof(double) * NOALLOC_MAX_POINTS;     // u32 (own 256 B)
+constexpr size_t NA_BYTES = NA_BEST + 256;
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// The device address of this device's `scratch` (call with the device current).
+hipError_t noalloc_address(void **ptr) {
+  *ptr = nullptr;
+  return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+}
+
+__global__ void k_set_u32(unsigned int *p, unsigned int v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+__global__ void k_widen(const float *__restrict__ in, double *__restrict__ out, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (double)in[i];
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *sherd_name(void) { return "hip"; }
@@ -161,6 +195,11 @@ int sherd_open(int device, sherd_ctx **out) {
   if (e != hipSuccess) {
     delete c;
     return hip_fail(e, "stream/event creation");
+  }
+  // The no-allocation entry point's array: each device holds its own copy at its own address.
+  if ((e = noalloc_address(&c->noalloc)) != hipSuccess) {
+    sherd_close(reinterpret_cast<sherd_ctx *>(c));
+    return hip_fail(e, "hipGetSymbolAddress(scratch)");
   }
   *out = reinterpret_cast<sherd_ctx *>(c);
   return SHERD_OK;
@@ -357,6 +396,55 @@ int sherd_bounded_distance(sherd_ctx *ctx, const sherd_scene *scene, const doubl
   }
   HIPCK(hipMemcpyAsync(d_b, &init, sizeof init, hipMemcpyHostToDevice, c->stream), "upload");
   Timer tm(c, SHERD_M_BOUNDED_DISTANCE, n);
+  HIPCK(launch_bounded_min(view_of(scene->d), d_p, (int)n, P, d_b, c->stream), "bounded min kernel");
+  tm.stop();
+  unsigned int bits = 0;
+  HIPCK(hipMemcpyAsync(&bits, d_b, sizeof bits, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  out[0] = (double)__builtin_bit_cast(float, bits);
+  return SHERD_OK;
+}
+
+int sherd_bounded_distance_noalloc(sherd_ctx *ctx, const sherd_scene *scene, const double *pts, size_t n,
+                                   const double T[16], double max_dist, int reduce, double *out) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  LOCK(c);
+  Pose P;
+  if (!scene || scene->ctx != c) return fail(SHERD_E_ARG, "scene of another context");
+  if (!pose_ok(T, &P) || !out || (n && !pts)) return fail(SHERD_E_ARG, "null buffer");
+  if (reduce != 0 && reduce != 1) return fail(SHERD_E_ARG, "reduce must be 0 (All) or 1 (Min)");
+  if (n > NOALLOC_MAX_POINTS)
+    return fail(SHERD_E_LIMIT, "sherd_bounded_distance_noalloc takes at most 262144 points");
+  if (!c->noalloc) return fail(SHERD_E_HIP, "no scratch address for this device");
+  float window = (float)max_dist;
+  if (n == 0) {
+    if (reduce == 1) out[0] = (double)window;
+    return SHERD_OK;
+  }
+  if (reduce == 1 && !(window > 0.0f)) {  // bvh.rs:124: a negative or NaN window answers itself
+    out[0] = (double)window;
+    return SHERD_OK;
+  }
+  char *base = (char *)c->noalloc;
+  double *d_p = (double *)(base + NA_PTS);
+  HIPCK(hipMemcpyAsync(d_p, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream), "upload");
+  Timer tm(c, SHERD_M_BOUNDED_DISTANCE, n);
+  if (reduce == 0) {
+    float *d_o = (float *)(base + NA_DIST);
+    double *d_w = (double *)(base + NA_WIDE);
+    HIPCK(launch_bounded(view_of(scene->d), d_p, (int)n, P, window, d_o, c->stream), "bounded kernel");
+    hipLaunchKernelGGL(k_widen, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, d_o, d_w, (int)n);
+    HIPCK(hipGetLastError(), "widen kernel");
+    tm.stop();
+    HIPCK(hipMemcpyAsync(out, d_w, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream), "download");
+    HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+    return SHERD_OK;
+  }
+  // The running minimum starts at the window's bits (set by a kernel, not a memset, so the call
+  // could sit in a captured graph), every call afresh: the array keeps the last call's value.
+  unsigned int *d_b = (unsigned int *)(base + NA_BEST);
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c->stream, d_b, __builtin_bit_cast(unsigned int, window));
+  HIPCK(hipGetLastError(), "window kernel");
   HIPCK(launch_bounded_min(view_of(scene->d), d_p, (int)n, P, d_b, c->stream), "bounded min kernel");
   tm.stop();
   unsigned int bits = 0;

@@ -42,6 +42,9 @@ struct Ctx {
   // staging; its previous contents are not kept when it grows.
   Buf host;
   void *pinned(size_t bytes, hipError_t *err);
+  // This device's copy of the file-scope `scratch` array of the no-allocation entry point (its
+  // address from hipGetSymbolAddress, taken at open with the device current).
+  void *noalloc = nullptr;
 };
 
 // Accumulates one method's call count, item count and device time (HIP events on the stream).

@@ -127,6 +127,7 @@ def _bind(L):
         "sherd_scene_info": (ci, [vp, P(sz), P(f), P(f)]),
         "sherd_first_hit": (ci, [vp, vp, P(f), P(f), sz, ci, P(f), P(u32)]),
         "sherd_bounded_distance": (ci, [vp, vp, P(d), sz, P(d), d, ci, P(d)]),
+        "sherd_bounded_distance_noalloc": (ci, [vp, vp, P(d), sz, P(d), d, ci, P(d)]),
         "sherd_inside": (ci, [vp, vp, P(d), sz, P(d), ci, P(InsideOutcome)]),
         "sherd_nearest_below": (ci, [vp, P(d), sz, P(d), sz, d, P(i64), P(d)]),
         "sherd_coarse_scores": (ci, [vp, P(d), P(d), sz, P(d), P(d), sz, P(d), sz, d, d, P(u32), P(d)]),
@@ -489,6 +490,16 @@ class HipExecutor:
         out = np.empty(1 if rd else len(pts), np.float64)
         _check(lib().sherd_bounded_distance(self.ptr, scene.ptr, _p(pts, C.c_double), len(pts), _p(T, C.c_double),
                                             float(max_dist), rd, _p(out, C.c_double)))
+        return out
+
+    def bounded_distance_noalloc(self, scene, points, T, max_dist, reduce="all"):
+        """bounded_distance through the entry point that allocates nothing (at most 262144 points)."""
+        pts = f64(points)
+        T = pose16(T)
+        rd = {"all": 0, "min": 1}[reduce]
+        out = np.empty(1 if rd else len(pts), np.float64)
+        _check(lib().sherd_bounded_distance_noalloc(self.ptr, scene.ptr, _p(pts, C.c_double), len(pts),
+                                                    _p(T, C.c_double), float(max_dist), rd, _p(out, C.c_double)))
         return out
 
     def inside(self, scene, points, T, retest=False):
