@@ -345,7 +345,8 @@ int sherd_huber_correspondences(sherd_huber *h, int32_t *corr, size_t n);
  * per-point terms and sums in f32 about the clouds' centroids; 64 in f64. */
 int sherd_huber_eval(sherd_huber *h, const double *xi, double delta, int precision, double *energy, double *grad);
 /* Batched Levenberg-Marquardt on E from T0: each iteration re-takes the correspondences at the
- * current pose, solves the reweighted 6x6 system in f64 and keeps the step if E drops. T_out k x 16,
+ * current pose, solves the reweighted 6x6 system in f64 and keeps the step if E drops (the drop
+ * summed point by point, sum rho(r') - rho(r), which f32 still resolves near the minimum). T_out k x 16,
  * energy k (at the final pose, over its last correspondences), steps k (accepted iterations). */
 int sherd_huber_lm(sherd_huber *h, double delta, int iterations, int precision, double *T_out, double *energy,
                    uint32_t *steps);

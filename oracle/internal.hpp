@@ -42,6 +42,11 @@ class Cloud {
   size_t n_;
 };
 
+// icp.rs:661-674 eigen_pivots and icp.rs:692-753 solve_ldlt: Eigen's pivoted LDLT of a symmetric
+// 6 x 6 with the pseudo-inverse of D; false on a non-finite solution.
+void eigen_pivots(const double a[36], int perm[6]);
+bool solve_ldlt(const double a[36], const double b[6], double x[6]);
+
 // kdtree.rs:182-192 nearest_below over a Cloud whose cell is at least `bound`.
 bool nearest_below(const Cloud &c, const double q[3], double bound, int64_t *idx, double *dist);
 

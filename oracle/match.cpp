@@ -230,6 +230,10 @@ void umeyama(const std::vector<double> &moved, const double *tp, const Corr &c, 
   homogeneous(rot, t, out);
 }
 
+}  // namespace
+
+namespace oracle {
+
 // icp.rs:661-674 eigen_pivots.
 void eigen_pivots(const double a[36], int perm[6]) {
   for (int i = 0; i < 6; ++i) perm[i] = i;
@@ -286,6 +290,13 @@ bool solve_ldlt(const double a[36], const double b[6], double x[6]) {
     if (!std::isfinite(x[k])) return false;
   return true;
 }
+
+}  // namespace oracle
+
+namespace {
+
+using oracle::eigen_pivots;
+using oracle::solve_ldlt;
 
 // icp.rs:770-778 euler_zyx: Rz(gamma) Ry(beta) Rx(alpha).
 void euler_zyx(double al, double be, double ga, double r[9]) {

@@ -149,6 +149,13 @@ void oracle_cone_votes(const oracle_scene *s, const double *V, size_t nv, const 
                        size_t nf, const int64_t *faces, const double *ns, size_t n_faces,
                        double thick, int retest, uint8_t *good);
 
+/* The batched Huber LM (oracle/huber_lm.cpp) for k poses of one pair: source n_src x 3, target
+ * n_tgt x 3 with unit normals, radius, base poses T0 (k x 16). T_out (k x 16), E_out, steps (last
+ * accepted iteration), status (0 cap, 1 converged, 2 damped out, 3 no step). threads: 0 = all. */
+void oracle_huber_lm(const double *src, size_t n_src, const double *tp, const double *tn, size_t n_tgt,
+                     double radius, const double *T0, size_t k, double delta, int iterations, int threads,
+                     double *T_out, double *E_out, uint32_t *steps, int32_t *status);
+
 /* Threads the parallel loops use (rayon's axes); 0 = all. Returns the count in force. */
 int oracle_set_threads(int n);
 

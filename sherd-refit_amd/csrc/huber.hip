@@ -118,6 +118,50 @@ __device__ __forceinline__ void for_pairs(const IcpPairDesc &pd, const int32_t *
   }
 }
 
+// The residual of source point i against target j at the shifted map (A, b), as for_pairs forms it.
+template <class T>
+__device__ __forceinline__ T pair_residual(const IcpPairDesc &pd, int i, int32_t j, const T *A, const T *b) {
+  const double *s = pd.src + 3 * (size_t)i;
+  const double *q = pd.tp + 3 * (size_t)j;
+  const double *n = pd.tn + 3 * (size_t)j;
+  T s0 = (T)(s[0] - pd.cs[0]), s1 = (T)(s[1] - pd.cs[1]), s2 = (T)(s[2] - pd.cs[2]);
+  T p[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) p[k] = ((A[3 * k] * s0 + A[3 * k + 1] * s1) + A[3 * k + 2] * s2) + b[k];
+  T qq[3] = {(T)(q[0] - pd.ct[0]), (T)(q[1] - pd.ct[1]), (T)(q[2] - pd.ct[2])};
+  T nn[3] = {(T)n[0], (T)n[1], (T)n[2]};
+  return ((p[0] - qq[0]) * nn[0] + (p[1] - qq[1]) * nn[1]) + (p[2] - qq[2]) * nn[2];
+}
+
+// The change of that residual under a change (dA, db) of the map: (dA (s - c_s) + db) . n.
+template <class T>
+__device__ __forceinline__ T residual_change(const IcpPairDesc &pd, int i, int32_t j, const T *dA, const T *db) {
+  const double *s = pd.src + 3 * (size_t)i;
+  const double *n = pd.tn + 3 * (size_t)j;
+  T s0 = (T)(s[0] - pd.cs[0]), s1 = (T)(s[1] - pd.cs[1]), s2 = (T)(s[2] - pd.cs[2]);
+  T dp[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) dp[k] = ((dA[3 * k] * s0 + dA[3 * k + 1] * s1) + dA[3 * k + 2] * s2) + db[k];
+  return (dp[0] * (T)n[0] + dp[1] * (T)n[1]) + dp[2] * (T)n[2];
+}
+
+// rho(r + dr) and its change from rho(r): within one branch of the Huber function the change is
+// formed from dr itself (dr (r + dr / 2) inside, +-delta dr outside), so it carries no cancellation.
+template <class T>
+__device__ __forceinline__ void huber_change(T r, T dr, T delta, T *rho1, T *d) {
+  T r1 = r + dr, psi, rho0;
+  huber(r1, delta, rho1, &psi);
+  T a0 = r < T(0) ? -r : r, a1 = r1 < T(0) ? -r1 : r1;
+  if (a0 <= delta && a1 <= delta) {
+    *d = dr * (r + T(0.5) * dr);
+  } else if (a0 > delta && a1 > delta && ((r < T(0)) == (r1 < T(0)))) {
+    *d = (r < T(0) ? -delta : delta) * dr;
+  } else {
+    huber(r, delta, &rho0, &psi);
+    *d = *rho1 - rho0;
+  }
+}
+
 // Energy (and gradient) of every pose at its twist. One workgroup per pose.
 template <class T>
 __global__ void __launch_bounds__(HT) k_huber_eval(HuberView v, const double *xi, double delta, double *E, double *G) {
@@ -199,8 +243,8 @@ __global__ void __launch_bounds__(HT) k_correspond(HuberView v, const double *Tc
 // Batched Levenberg-Marquardt on the Huber energy. Each iteration takes the correspondences at the
 // current pose, forms the iteratively reweighted normal equations (weights psi(r)/r) with rows
 // [p' x n, n] about the target centroid, solves (H + lambda diag H) dx = -g in f64 (icp.rs's
-// Eigen-pivoted LDLT), and keeps the step when the energy over the same correspondences drops
-// (lambda / 10), else raises lambda x 10 and stays.
+// Eigen-pivoted LDLT), and keeps the step when the energy over the same correspondences drops,
+// the drop summed point by point (lambda / 10), else raises lambda x 10 and stays.
 //
 // An iteration is two launches: k_lm_search finds every point's correspondence for every pose
 // still running, one thread per point (each search starts from the point's previous partner, at
@@ -330,33 +374,46 @@ __global__ void __launch_bounds__(HT) k_lm_step(HuberView v, double delta, int i
       }
       sTn[12] = sTn[13] = sTn[14] = 0.0;
       sTn[15] = 1.0;
+      // The step as a change of the shifted map, (A2 - A, b2 - b), taken in f64 and rounded once,
+      // so each residual's change is computed from the small difference itself.
+      double R0[9] = {sT[0], sT[1], sT[2], sT[4], sT[5], sT[6], sT[8], sT[9], sT[10]};
+      double t0[3] = {sT[3], sT[7], sT[11]}, A0[9], b0[3];
+      shifted_map(R0, t0, pd.cs, pd.ct, A0, b0);
       double R[9] = {sTn[0], sTn[1], sTn[2], sTn[4], sTn[5], sTn[6], sTn[8], sTn[9], sTn[10]};
       double t[3] = {sTn[3], sTn[7], sTn[11]}, A2[9], b2[3];
       shifted_map(R, t, pd.cs, pd.ct, A2, b2);
-      for (int k = 0; k < 9; ++k) sA[k] = (T)A2[k];
-      for (int k = 0; k < 3; ++k) sb[k] = (T)b2[k];
+      for (int k = 0; k < 9; ++k) sA[k] = (T)(A2[k] - A0[k]);
+      for (int k = 0; k < 3; ++k) sb[k] = (T)(b2[k] - b0[k]);
     }
   }
   __syncthreads();
   if (!s_done) {
-    // The trial pose's energy over the same correspondences.
-    for (int k = 0; k < 9; ++k) A[k] = sA[k];
-    for (int k = 0; k < 3; ++k) b[k] = sb[k];
-    T e[1] = {T(0)};
-    for_pairs<T>(pd, corr, A, b, [&](T r, const T *, const T *) {
-      T rho, psi;
-      huber(r, dl, &rho, &psi);
-      e[0] += rho;
-    });
-    block_sum<T, 1>(e, lds, red);
+    // The trial pose's energy over the same correspondences, and its change summed point by point
+    // (sum of rho(r') - rho(r), not the difference of two sums: the change stays resolved in f32
+    // long after the two totals agree to their last bit). r' = r + dr, with dr = (dA s' + db) . n
+    // from the map's change.
+    T dA[9], db[3];
+    for (int k = 0; k < 9; ++k) dA[k] = sA[k];
+    for (int k = 0; k < 3; ++k) db[k] = sb[k];
+    T e[2] = {T(0), T(0)};
+    for (int i = threadIdx.x; i < pd.ns; i += HT) {
+      int32_t j = corr[i];
+      if (j < 0) continue;
+      T r0 = pair_residual<T>(pd, i, j, A, b), dr = residual_change<T>(pd, i, j, dA, db);
+      T rho1, d;
+      huber_change(r0, dr, dl, &rho1, &d);
+      e[0] += rho1;
+      e[1] += d;
+    }
+    block_sum<T, 2>(e, lds, red);
     if (threadIdx.x == 0) {
-      double en = (double)red[0];
-      if (en < s_E) {
+      double en = (double)red[0], de = (double)red[1];
+      if (de < 0.0) {
         for (int k = 0; k < 16; ++k) sT[k] = sTn[k];
         s_Ef = en;
         s_lambda = fmax(s_lambda * 0.1, 1e-12);
         s_steps = it + 1;
-        if (s_E - en <= 1e-12 * s_E) s_done = 1;
+        if (-de <= 1e-12 * s_E) s_done = 1;
       } else {
         s_lambda *= 10.0;
         if (s_lambda > 1e12) s_done = 1;

@@ -119,12 +119,14 @@ def vessel(t=1.0, edges_per_t=5.5, n_frag=8, seed=10, height_t=200.0, rmin_t=40.
     Z = np.broadcast_to(zs[None, :, None], X.shape)
     verts = np.stack([X, Y, Z], axis=-1).reshape(-1, 3).astype(np.float32).astype(np.float64)
 
-    tris, owner, is_frac = [], [], []
+    tris, owner, is_frac, partner = [], [], [], []
 
-    def add(a, b, c, lab, frac):
+    def add(a, b, c, lab, frac, other=-1):
         tris.append(np.stack([a, b, c], axis=-1))
         owner.append(np.broadcast_to(lab, np.shape(a)).ravel() if np.ndim(lab) else np.full(np.size(a), lab))
         is_frac.append(np.full(np.size(a), frac, dtype=bool))
+        # The fragment on the other side of a fracture face (-1 on the shell).
+        partner.append(np.broadcast_to(other, np.shape(a)).ravel() if np.ndim(other) else np.full(np.size(a), other))
 
     I, J = ci, cj
     lab = label
@@ -141,19 +143,19 @@ def vessel(t=1.0, edges_per_t=5.5, n_frag=8, seed=10, height_t=200.0, rmin_t=40.
         m = nb != label
         ii, jj = I[m], J[m]
         p0, p1, p2, p3 = node(ii + 1, jj, l), node(ii + 1, jj + 1, l), node(ii + 1, jj + 1, l + 1), node(ii + 1, jj, l + 1)
-        add(p0, p1, p2, label[m], True)
-        add(p0, p2, p3, label[m], True)
-        add(p0, p2, p1, nb[m], True)
-        add(p0, p3, p2, nb[m], True)
+        add(p0, p1, p2, label[m], True, nb[m])
+        add(p0, p2, p3, label[m], True, nb[m])
+        add(p0, p2, p1, nb[m], True, label[m])
+        add(p0, p3, p2, nb[m], True, label[m])
         # Walls at z_{j+1} between cell (i, j) and (i, j+1).
         up = label[:, 1:]
         m = up != label[:, :-1]
         ii, jj = I[:, :-1][m], J[:, :-1][m]
         p0, p1, p2, p3 = node(ii, jj + 1, l), node(ii, jj + 1, l + 1), node(ii + 1, jj + 1, l + 1), node(ii + 1, jj + 1, l)
-        add(p0, p1, p2, label[:, :-1][m], True)
-        add(p0, p2, p3, label[:, :-1][m], True)
-        add(p0, p2, p1, up[m], True)
-        add(p0, p3, p2, up[m], True)
+        add(p0, p1, p2, label[:, :-1][m], True, up[m])
+        add(p0, p2, p3, label[:, :-1][m], True, up[m])
+        add(p0, p2, p1, up[m], True, label[:, :-1][m])
+        add(p0, p3, p2, up[m], True, label[:, :-1][m])
         # End caps (shell): bottom faces -z, top faces +z.
         ib = np.arange(nth)
         p0, p1, p2, p3 = node(ib, 0, l), node(ib, 0, l + 1), node(ib + 1, 0, l + 1), node(ib + 1, 0, l)
@@ -166,6 +168,7 @@ def vessel(t=1.0, edges_per_t=5.5, n_frag=8, seed=10, height_t=200.0, rmin_t=40.
     tris = np.concatenate([x.reshape(-1, 3) for x in tris])
     owner = np.concatenate(owner)
     is_frac = np.concatenate(is_frac)
+    partner = np.concatenate(partner).astype(np.int32)
 
     prng = np.random.default_rng(seed + 3 if pose_seed is None else pose_seed)
     frags, poses, adjacent = [], [], set()
@@ -191,6 +194,7 @@ def vessel(t=1.0, edges_per_t=5.5, n_frag=8, seed=10, height_t=200.0, rmin_t=40.
         frag = _fragment_data(v, f, frac, t, np.random.default_rng(seed + 100 + k))
         frag = _posed(frag, P)
         frag["name"] = f"frag{k}"
+        frag["partner"] = partner[sel]
         frags.append(frag)
         poses.append(P)
     return dict(fragments=frags, poses=poses, adjacent=adjacent, t=t, res=res)
@@ -268,6 +272,8 @@ def remove_patch(frag, fraction, rng):
     out = dict(frag)
     out["F"] = f[keep]
     out["frac"] = frag["frac"][keep]
+    if "partner" in frag:
+        out["partner"] = frag["partner"][keep]
     out["watertight"] = False
     return out
 
@@ -397,6 +403,65 @@ def huber_workload(t=1.0, n_pairs=16, poses=32, nmin=10000, nmax=40000, length_t
             T0[k] = d @ truth
         pairs.append(dict(source=src, target=tgt, target_n=tn, truth=truth, T0=T0))
     return dict(pairs=pairs, delta=0.05 * t, radius=0.3 * t, t=t)
+
+
+def _seam_side(frag, other, n, t, band_t, frac_share, rng):
+    """n points on `frag`'s side of its seam with fragment `other`, each with its face's normal: a
+    frac_share of them on the fracture faces the two share, the rest on the shell faces whose centroid
+    lies within band_t of the seam's breakline (the shell/fracture junction of those faces)."""
+    V = frag["V"].astype(np.float64)
+    F = frag["F"]
+    fn = frag["face_normals"]
+    faces = np.nonzero(frag["frac"] & (frag["partner"] == other))[0]
+    if len(faces) == 0:
+        return None
+    n_f = int(round(frac_share * n))
+    pf, pff, _ = _sample_faces(V, F, faces, n_f, rng)
+    on_shell = np.zeros(len(V), bool)
+    on_shell[F[~frag["frac"]].ravel()] = True
+    brk = np.unique(F[faces].ravel())
+    brk = brk[on_shell[brk]]
+    shell = np.nonzero(~frag["frac"])[0]
+    cen = V[F[shell]].mean(axis=1)
+    d, _ = cKDTree(V[brk]).query(cen, distance_upper_bound=1.01 * band_t * t, workers=-1)
+    band = shell[d <= band_t * t]
+    ps, psf, _ = _sample_faces(V, F, band, n - n_f, rng)
+    return np.concatenate([pf, ps]), fn[np.concatenate([pff, psf])]
+
+
+def seam_workload(col, poses=512, nmin=10000, nmax=40000, seed=30, band_t=0.6, frac_share=0.75, max_deg=1.0,
+                  max_shift_t=0.1):
+    """The robust seam energy's workload on a vessel collection's seams: for every adjacent pair
+    (a, b), a cloud on each side of their seam (log-uniform in [nmin, nmax] points per side, a
+    frac_share of them on the shared fracture faces and the rest on the shell band within band_t of
+    the breakline, each point with its face's normal: fracture normals across the wall in two
+    directions, shell normals radial, so together they span 3-D), B's cloud as the source and A's as
+    the target. `poses` base poses spread over the pairs in turn: the true relative pose composed
+    with a rotation uniform within max_deg about the target seam's centroid and a translation in a
+    ball of max_shift_t. Huber delta 0.05 t, correspondence radius 0.3 t."""
+    t = col["t"]
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(nmin), np.log(nmax)
+    frags, P = col["fragments"], col["poses"]
+    pairs = []
+    for a, b in sorted(col["adjacent"]):
+        nt, ns = (int(round(np.exp(rng.uniform(lo, hi)))) for _ in range(2))
+        tgt = _seam_side(frags[a], b, nt, t, band_t, frac_share, rng)
+        src = _seam_side(frags[b], a, ns, t, band_t, frac_share, rng)
+        if tgt is None or src is None:
+            continue
+        pairs.append(dict(a=a, b=b, source=src[0], target=tgt[0], target_n=tgt[1], truth=P[a] @ np.linalg.inv(P[b])))
+    cp = np.sort(np.arange(poses) % len(pairs)).astype(np.uint32)
+    T0 = np.empty((poses, 4, 4))
+    for i, p in enumerate(cp):
+        c = pairs[p]["target"].mean(axis=0)
+        R = _uniform_rotation_within(rng, np.radians(max_deg))
+        v = rng.normal(size=3)
+        d = np.eye(4)
+        d[:3, :3] = R
+        d[:3, 3] = c - R @ c + v / np.linalg.norm(v) * max_shift_t * t * rng.uniform() ** (1.0 / 3.0)
+        T0[i] = d @ pairs[p]["truth"]
+    return dict(pairs=pairs, cand_pair=cp, T0=T0, delta=0.05 * t, radius=0.3 * t, t=t)
 
 
 def coarse_grid(centroids, spacing):

@@ -84,6 +84,8 @@ def _bind(L):
     L.oracle_median.restype = d
     L.oracle_median.argtypes = [P(d), sz]
     L.oracle_cone_votes.argtypes = [C.c_void_p, P(d), sz, P(u32), sz, P(i64), P(d), sz, d, C.c_int, P(u8)]
+    L.oracle_huber_lm.argtypes = [P(d), sz, P(d), P(d), sz, d, P(d), sz, d, C.c_int, C.c_int, P(d), P(d),
+                                  P(u32), P(C.c_int32)]
     L.oracle_set_threads.restype = C.c_int
     L.oracle_set_threads.argtypes = [C.c_int]
     return L
@@ -262,6 +264,25 @@ def normal_equations(moved, tgt_p, tgt_n, corr):
     lib().oracle_normal_equations(_p(m, C.c_double), len(m), _p(tp, C.c_double), _p(tn, C.c_double),
                                   _p(c, C.c_int64), _p(jtj, C.c_double), _p(jtr, C.c_double))
     return jtj.reshape(6, 6), jtr
+
+
+LM_RAN_OUT, LM_CONVERGED, LM_DAMPED_OUT, LM_NO_STEP = 0, 1, 2, 3
+
+
+def huber_lm(src, tgt_p, tgt_n, radius, T0, delta, iterations, threads=0):
+    """The batched Huber LM's FP64 restatement (oracle/huber_lm.cpp) for poses T0 (k x 4 x 4) of one
+    pair: (T k x 4 x 4, energy k, steps k, status k)."""
+    s, tp, tn = (f64(x).reshape(-1, 3) for x in (src, tgt_p, tgt_n))
+    T0 = f64(T0).reshape(-1, 16)
+    k = len(T0)
+    T = np.empty((k, 16), np.float64)
+    E = np.empty(k, np.float64)
+    st = np.empty(k, np.uint32)
+    status = np.empty(k, np.int32)
+    lib().oracle_huber_lm(_p(s, C.c_double), len(s), _p(tp, C.c_double), _p(tn, C.c_double), len(tp), float(radius),
+                          _p(T0, C.c_double), k, float(delta), int(iterations), int(threads), _p(T, C.c_double),
+                          _p(E, C.c_double), _p(st, C.c_uint32), _p(status, C.c_int32))
+    return T.reshape(k, 4, 4), E, st, status
 
 
 def scales_for_pair(t, res):

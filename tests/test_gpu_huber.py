@@ -174,3 +174,47 @@ def test_batched_lm_reduces_the_energy_and_agrees_across_precisions(ex):
     ang, tr = pose_gap(T32, T64, 1.0)
     say(f"  LM f32 vs f64: worst {np.radians(ang).max():.2e} rad {tr.max():.2e} t")
     assert np.radians(ang).max() < 1e-5 and tr.max() < 1e-4
+
+
+def _gap(T, U):
+    dr = T[..., :3, :3] @ np.swapaxes(U[..., :3, :3], -1, -2)
+    ang = 2.0 * np.arcsin(np.minimum(np.linalg.norm(dr - np.eye(3), axis=(-2, -1)) / (2.0 * np.sqrt(2.0)), 1.0))
+    return ang, np.linalg.norm(T[..., :3, 3] - U[..., :3, 3], axis=-1)
+
+
+def test_batched_lm_against_the_fp64_oracle_lm_on_vessel_seams(ex):
+    """A sample of the seam workload's shape: 4 vessel seams (10k-40k points per side, normals that
+    span 3-D) x 8 base poses, 20 iterations. Device f64 LM and device f32 LM against the oracle's LM
+    (oracle/huber_lm.cpp): every pose the oracle sees converge within 1e-6 rad / 1e-5 t, in f64 and
+    in the f32 form; the share that did not converge is printed."""
+    from sherd_hip import synth
+
+    col = synth.vessel(t=1.0, edges_per_t=2.0, n_frag=4, seed=11, height_t=40.0, rmin_t=15.0, rmax_t=20.0)
+    w = synth.seam_workload(col, poses=48, seed=30)
+    keep = w["cand_pair"] < 4
+    cp, T0 = w["cand_pair"][keep], w["T0"][keep]
+    pairs = w["pairs"][:4]
+    say(f"case: LM vs oracle LM, {len(cp)} poses on seams of {[(len(p['source']), len(p['target'])) for p in pairs]}")
+    clouds = [(ex.cloud(p["source"]), ex.cloud(p["target"], p["target_n"]), w["radius"]) for p in pairs]
+    b = ex.huber(clouds, cp, T0)
+    T64, E64, st64 = b.lm(w["delta"], 20, 64)
+    T32, E32, st32 = b.lm(w["delta"], 20, 32)
+    To = np.empty_like(T64)
+    status = np.empty(len(cp), np.int32)
+    for pi, p in enumerate(pairs):
+        m = cp == pi
+        To[m], _, _, status[m] = O.huber_lm(p["source"], p["target"], p["target_n"], w["radius"], T0[m], w["delta"], 20)
+    conv = (status == O.LM_CONVERGED) | (status == O.LM_DAMPED_OUT)
+    a64, t64 = _gap(T64, To)
+    a32, t32 = _gap(T32, To)
+    say(f"  converged (oracle): {conv.sum()} of {len(cp)}; f64 worst {a64[conv].max():.2e} rad {t64[conv].max():.2e} t; "
+        f"f32 worst {a32[conv].max():.2e} rad {t32[conv].max():.2e} t; all poses f64 worst {a64.max():.2e} rad "
+        f"{t64.max():.2e} t")
+    assert conv.mean() >= 0.75
+    assert a64[conv].max() < 1e-6 and t64[conv].max() < 1e-5
+    assert a32[conv].max() < 1e-6 and t32[conv].max() < 1e-5
+    assert np.all(st64 > 0)
+    b.close()
+    for s_, t_, _ in clouds:
+        s_.close()
+        t_.close()

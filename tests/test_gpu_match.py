@@ -11,6 +11,7 @@ import sys
 import numpy as np
 import pytest
 
+import icp_chaos
 import oracle_lib as O
 import slab_data as S
 
@@ -104,18 +105,22 @@ def test_stage1_ladder_against_the_oracle_and_the_golden(ex):
     inits[:, :3, :3] = poses[p["nms1_kept"], :9].reshape(-1, 3, 3)
     inits[:, :3, 3] = poses[p["nms1_kept"], 9:]
     src = b["brk_P"][b["brk_sub"]]
-    Tg = To = inits
+    Tg = inits
     for k in (0.2, 0.08):
         d = k * sc[0] * sc[11]
-        g = ex.icp_rung(src, a["brk_P"], a["brk_ns"], Tg, 0, d, 20)
-        o = O.icp_rung(src, a["brk_P"], a["brk_ns"], To, 0, d, 20)
-        Tg, To = g[:, :16].reshape(-1, 4, 4), o[:, :16].reshape(-1, 4, 4)
+        Tg = ex.icp_rung(src, a["brk_P"], a["brk_ns"], Tg, 0, d, 20)[:, :16].reshape(-1, 4, 4)
+    # The north star's transform bar, 1e-6 rad and 1e-5 t, on every candidate whose ladder is a
+    # function of its input (the one-ULP probe of crosscheck.rs:173-215); the others are counted.
+    rungs = [(src, a["brk_P"], a["brk_ns"], 0, k * sc[0] * sc[11], 20) for k in (0.2, 0.08)]
+    det, To = icp_chaos.determined(rungs, inits)
     ang, tr = pose_gap(Tg, To, sc[0])
-    # The north star's transform bar: 1e-6 rad and 1e-5 t.
-    det = (np.radians(ang) < 1e-6) & (tr < 1e-5)
-    say(f"  stage 1 vs oracle: {det.sum()}/{len(det)} within 1e-6 rad / 1e-5 t; worst {ang.max():.2e} deg {tr.max():.2e} t")
-    assert det.mean() > 0.95
+    say(f"  stage 1 vs oracle: {(~det).sum()} of {len(det)} chaotic (excused); worst over determined "
+        f"{ang[det].max():.2e} deg {tr[det].max():.2e} t, over all {ang.max():.2e} deg {tr.max():.2e} t")
+    assert (~det).sum() <= len(det) // 20
+    assert np.radians(ang[det]).max() < 1e-6 and tr[det].max() < 1e-5
     ang2, tr2 = pose_gap(Tg, p["s1_T"], sc[0])
+    # Against the reference's own dump (Open3D's sums): its tolerance, 0.05 degrees / 0.01 t
+    # (sherd-parity/src/stages/stage1.rs:55-63).
     assert np.mean((ang2 < 0.05) & (tr2 < 0.01)) > 0.95
 
 
@@ -125,17 +130,22 @@ def test_stage2_ladder_against_the_golden(ex):
 
     p = S.pair()
     sc = p["scales"]
-    T = p["s1_T"][p["nms2_kept"]]
-    To = T
+    T0 = T = p["s1_T"][p["nms2_kept"]]
+    rungs = []
     for name, src, tp, tn, k in stage2_rungs():
         d = k * sc[0] * sc[11]
         T = ex.icp_rung(src, tp, tn, T, 1, d, 30)[:, :16].reshape(-1, 4, 4)
-        To = O.icp_rung(src, tp, tn, To, 1, d, 30)[:, :16].reshape(-1, 4, 4)
+        # Against the reference's dump: its tolerance (sherd-parity/src/stages/stage2.rs:42-48).
         ang, tr = pose_gap(T, p[name], sc[0])
         assert ang.max() < 0.05 and tr.max() < 0.01, name
+        # Against the oracle: the transform bar on every determined candidate after this rung.
+        rungs.append((src, tp, tn, 1, d, 30))
+        det, To = icp_chaos.determined(rungs, T0)
         ang, tr = pose_gap(T, To, sc[0])
-        say(f"  {name} vs oracle: worst {ang.max():.2e} deg {tr.max():.2e} t")
-        assert np.median(tr) < 1e-5
+        say(f"  {name} vs oracle: {(~det).sum()} of {len(det)} chaotic (excused); worst over determined "
+            f"{ang[det].max():.2e} deg {tr[det].max():.2e} t")
+        assert (~det).sum() <= 1
+        assert np.radians(ang[det]).max() < 1e-6 and tr[det].max() < 1e-5, name
 
 
 def test_icp_known_answers(ex):
