@@ -187,25 +187,29 @@ def icp_ladder(ex, synth, args):
     return res
 
 
-def huber_section(ex, synth, args):
-    """The robust seam energy over 512 batched poses (16 breakline pairs x 32 poses, 10k-40k points
-    per side): energy+gradient evaluations/s in f32 and f64, the f32 error against the FP64 oracle,
-    the device gradient against FP64 central differences, and the batched LM."""
+def huber_section(ex, synth, col, args):
+    """the robust seam energy over 512 batched poses on the vessel's seams
+    (configuration 2's generator; per adjacent pair a cloud on each side of the seam, 10k-40k points
+    per side, fracture and shell normals that span 3-D; base poses within 1 degree and 0.1 t of the
+    truth): energy+gradient evaluations/s in f32 and f64, the f32 error against the FP64 oracle, the
+    device gradient against FP64 central differences (h = 1e-4), and the batched LM in f64 and f32
+    with its transform error against the FP64 oracle's LM on a bounded sample of the poses."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
+    sys.path.insert(0, os.path.join(HERE, "tests"))
     import huber as H
+    import oracle_lib as O
 
-    w = synth.huber_workload()
-    P = w["pairs"]
+    w = synth.seam_workload(col, poses=512)
+    P, cp, T0 = w["pairs"], w["cand_pair"], w["T0"]
     clouds = [(ex.cloud(p["source"]), ex.cloud(p["target"], p["target_n"]), w["radius"]) for p in P]
-    poses = len(P[0]["T0"])
-    cp = np.repeat(np.arange(len(P), dtype=np.uint32), poses)
-    T0 = np.concatenate([p["T0"] for p in P])
     b = ex.huber(clouds, cp, T0)
     k = len(cp)
     xi = np.random.default_rng(31).normal(scale=1e-3, size=(k, 6))
     corr = b.correspondences()
     pairs_used = int((corr >= 0).sum())
-    res = dict(poses=k, corresponding_points=pairs_used, delta=w["delta"])
+    sizes = [len(p["source"]) for p in P] + [len(p["target"]) for p in P]
+    res = dict(poses=k, seams=len(P), points_per_side=[int(min(sizes)), int(max(sizes))],
+               corresponding_points=pairs_used, delta=w["delta"], radius=w["radius"])
     for prec in (32, 64):
         b.energy(xi, w["delta"], prec, grad=True)
         reps = 5
@@ -214,16 +218,17 @@ def huber_section(ex, synth, args):
             E, G = b.energy(xi, w["delta"], prec, grad=True)
         el = (time.perf_counter() - t0) / reps
         res[f"f{prec}_eval_ms"] = round(1e3 * el, 3)
+        res[f"f{prec}_evals_per_s"] = round(k / el, 1)
         res[f"f{prec}_point_terms_per_s"] = round(pairs_used / el, 1)
         if prec == 32:
             E32, G32 = E, G
         else:
             E64, G64 = E, G
-    # Error against the FP64 oracle on a sample of poses (one per pair), and FD of the oracle.
-    at, errs_e, errs_g, errs_fd = 0, [], [], []
+    # Error against the FP64 oracle on one pose per seam, and the oracle's central differences.
     offs = np.concatenate([[0], np.cumsum([len(P[p]["source"]) for p in cp])])
+    errs_e, errs_g, errs_fd = [], [], []
     for p in range(len(P)):
-        i = p * poses
+        i = int(np.nonzero(cp == p)[0][0])
         c = corr[offs[i]:offs[i + 1]]
         m = c >= 0
         p0 = H.base_points(T0[i], P[p]["source"])[m]
@@ -238,37 +243,55 @@ def huber_section(ex, synth, args):
     res["f32_energy_rel_err_max"] = float(max(errs_e))
     res["f32_grad_rel_err_max"] = float(max(errs_g))
     res["f64_grad_vs_fp64_fd_rel_max"] = float(max(errs_fd))
-    E0 = b.energy(np.zeros((k, 6)), w["delta"], 64)
+    iters = 20
+    out = {}
     for prec in (64, 32):
-        b.lm(w["delta"], 20, prec)
+        b.lm(w["delta"], iters, prec)
         t0 = time.perf_counter()
-        T, E, st = b.lm(w["delta"], 20, prec)
+        out[prec] = b.lm(w["delta"], iters, prec)
         res[f"lm_f{prec}_ms"] = round(1e3 * (time.perf_counter() - t0), 3)
-        if prec == 64:
-            T64 = T
-            truths = np.stack([P[p]["truth"] for p in cp])
-            dr = T[:, :3, :3] @ np.transpose(truths[:, :3, :3], (0, 2, 1))
-            rot = np.linalg.norm(dr - np.eye(3), axis=(1, 2)) / np.sqrt(2)
-            res["lm_f64_mean_steps"] = round(float(st.mean()), 2)
-            # The energy is over each pose's own final correspondences, so it is reported, not compared.
-            res["lm_f64_energy_final_over_initial_median"] = float(np.median(E / np.maximum(E0, 1e-300)))
-            trn = np.linalg.norm(T[:, :3, 3] - truths[:, :3, 3], axis=1)
-            # Quantiles (median, 90th, max): point-to-plane on a curve leaves the tangent free, so a
-            # few poses slide along it.
-            res["lm_f64_rot_to_truth_rad_q50_q90_max"] = [float(x) for x in np.quantile(rot, [0.5, 0.9, 1.0])]
-            res["lm_f64_trans_to_truth_t_q50_q90_max"] = [float(x) for x in np.quantile(trn, [0.5, 0.9, 1.0])]
-        else:
-            dr = T[:, :3, :3] @ np.transpose(T64[:, :3, :3], (0, 2, 1))
-            rot = np.linalg.norm(dr - np.eye(3), axis=(1, 2)) / np.sqrt(2)
-            trn = np.linalg.norm(T[:, :3, 3] - T64[:, :3, 3], axis=1)
-            res["lm_f32_vs_f64_rot_rad_q50_q90_max"] = [float(x) for x in np.quantile(rot, [0.5, 0.9, 1.0])]
-            res["lm_f32_vs_f64_trans_t_q50_q90_max"] = [float(x) for x in np.quantile(trn, [0.5, 0.9, 1.0])]
-    log(f"huber: {k} poses, {pairs_used} corresponding points; f32 eval {res['f32_eval_ms']} ms, f64 eval "
-        f"{res['f64_eval_ms']} ms; LM f64 {res['lm_f64_ms']} ms")
+    res["lm_iterations"] = iters
+    res["lm_f64_mean_accepted_steps"] = round(float(out[64][2].mean()), 2)
+    truths = np.stack([P[p]["truth"] for p in cp])
+    ang, tr = pose_gap_deg_t(out[64][0], truths)
+    res["lm_f64_to_truth_deg_t_q50_q90_max"] = [[float(x) for x in np.quantile(ang, [0.5, 0.9, 1.0])],
+                                                [float(x) for x in np.quantile(tr, [0.5, 0.9, 1.0])]]
+    ang0, tr0 = pose_gap_deg_t(T0, truths)
+    res["base_to_truth_deg_t_max"] = [float(ang0.max()), float(tr0.max())]
+    if not args.no_cpu:
+        # The FP64 oracle's LM on the first 4 poses of every seam (all host threads over poses).
+        sel, To, status = [], [], []
+        t0 = time.perf_counter()
+        for p in range(len(P)):
+            ii = np.nonzero(cp == p)[0][:4]
+            T, _, _, st = O.huber_lm(P[p]["source"], P[p]["target"], P[p]["target_n"], w["radius"], T0[ii],
+                                     w["delta"], iters)
+            sel += list(ii)
+            To.append(T)
+            status.append(st)
+        el_cpu = time.perf_counter() - t0
+        To, status, sel = np.concatenate(To), np.concatenate(status), np.array(sel)
+        conv = (status == O.LM_CONVERGED) | (status == O.LM_DAMPED_OUT)
+        for prec in (64, 32):
+            a, d = pose_gap_deg_t(out[prec][0][sel], To)
+            a = np.radians(a)
+            res[f"lm_f{prec}_vs_oracle_rad_t_max_converged"] = [float(a[conv].max()), float(d[conv].max())]
+            res[f"lm_f{prec}_vs_oracle_rad_t_max_all"] = [float(a.max()), float(d.max())]
+        res["oracle_lm_poses"] = int(len(sel))
+        res["oracle_lm_not_converged_share"] = round(float(1.0 - conv.mean()), 4)
+        res["oracle_lm_poses_per_s"] = round(len(sel) / el_cpu, 2)
+        res["oracle_lm_sample"] = (f"oracle/huber_lm.cpp (FP64) on the first 4 base poses of each of the {len(P)} "
+                                   f"seams, {iters} iterations, OpenMP over poses")
+    log(f"huber: {k} poses on {len(P)} seams, {pairs_used} corresponding points; f32 eval {res['f32_eval_ms']} ms, "
+        f"f64 eval {res['f64_eval_ms']} ms; LM f64 {res['lm_f64_ms']} ms, f32 {res['lm_f32_ms']} ms"
+        + (f"; f32 LM vs oracle {res['lm_f32_vs_oracle_rad_t_max_converged']}" if not args.no_cpu else ""))
     b.close()
     for s_, t_, _ in clouds:
         s_.close()
         t_.close()
+    res["workload"] = ("vessel (configuration 2) seams: per adjacent pair, fracture-face and shell-band samples "
+                       "on each side, 10k-40k points per side log-uniform, 512 base poses within 1 deg / 0.1 t, "
+                       "delta 0.05 t, radius 0.3 t")
     return res
 
 
@@ -600,10 +623,96 @@ def synth_scales(col):
     return synth.scales_for_pair(col["t"], col["res"])
 
 
-def collection_section(ex, args):
-    """The 40-fragment, three-vessel collection end to end (ICP ladder, Huber LM, verification with
-    the FP64 retest over all 780 pairs), batched against the plain schedule, and the oracle's
-    verification of a sample of the same final poses."""
+def pose_gap_deg_t(a, b):
+    """Rotation gap in degrees (from |Ra Rb^T - I|_F, exact to rounding at small angles) and
+    translation gap between two stacks of 4 x 4 poses."""
+    dr = a[:, :3, :3] @ np.transpose(b[:, :3, :3], (0, 2, 1)) - np.eye(3)
+    ang = 2.0 * np.arcsin(np.clip(np.linalg.norm(dr, axis=(1, 2)) / (2.0 * np.sqrt(2.0)), 0, 1))
+    return np.degrees(ang), np.linalg.norm(a[:, :3, 3] - b[:, :3, 3], axis=1)
+
+
+def cloud_tree_bytes(n):
+    """A point cloud's resident search structure: its LBVH nodes and one 48-byte leaf record per point."""
+    return 32 * (2 * n - 1) + 48 * n
+
+
+def icp_launch_bytes(srcs, tgts, cp, iters):
+    """Compulsory bytes of one batched ICP rung (point-to-point): per candidate-iteration and source
+    point, the point (24 B) and its matched target point (24 B), over the iterations plus the final
+    correspondence pass; every target cloud's tree once."""
+    per = sum(48 * len(srcs[int(p)]) * (int(k) + 1) for p, k in zip(cp, iters))
+    return per + sum(cloud_tree_bytes(len(tgts[int(p)])) for p in sorted(set(int(x) for x in cp)))
+
+
+def host_info():
+    """This host's CPU model (lscpu's "Model name") and core count (nproc)."""
+    import subprocess
+
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:  # noqa: BLE001 (lscpu missing: the model stays unknown)
+        pass
+    return model, os.cpu_count() or 1
+
+
+def collection_oracle(coll, pairs, cp, cT, scales, sample, threads, repeats=5):
+    """The oracle's end-to-end pipeline per candidate (the breakline ladder, the FP64 LM, verify with
+    the FP64 retest), threaded over candidates as Pair::stage2_batch's rayon loop is
+    (crates/sherd-core/src/matching/pair.rs:350-359): `threads` host threads, each running whole
+    candidates with the oracle's own OpenMP at one thread. Returns (candidates/s median of `repeats`,
+    the per-run rates, the last run's (T_icp, T_lm, scores) per sampled candidate)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import oracle_lib as O
+    from sherd_hip import collection as CL
+
+    frags = coll["fragments"]
+    need = sorted(set(i for c in sample for i in pairs[cp[c]]))
+    keep, side = [], {}
+    for i in need:
+        fr = frags[i]
+        fs, ms = O.Scene(fr["V"], fr["F"], fr["frac"]), O.Scene(fr["V"], fr["F"])
+        keep += [fs, ms]
+        side[i] = O.SideData(fs, ms, fr["watertight"], fr["frac_area"], fr["Pf"], fr["S"], fr["brk_P"],
+                              fr["brk_ns"], fr["margin_p"], fr["margin_n"])
+
+    def one(c):
+        a, b = pairs[cp[c]]
+        sc = scales[cp[c]]
+        fa, fb = frags[a], frags[b]
+        T = cT[c][None]
+        for k in CL.RUNGS:
+            T = O.icp_rung(fb["brk_P"], fa["brk_P"], fa["brk_ns"], T, 0, k * sc[0] * sc[11], CL.ITERATIONS)[:, :16]
+            T = T.reshape(-1, 4, 4)
+        T1 = T[0]
+        T2 = O.huber_lm(fb["brk_P"], fa["brk_P"], fa["brk_ns"], CL.RUNGS[-1] * sc[0] * sc[11], T, CL.LM_DELTA,
+                        CL.LM_ITERATIONS, threads=1)[0][0]
+        return T1, T2, O.verify(side[a], side[b], T2, sc, retest=True)
+
+    rates, out = [], None
+    with ThreadPoolExecutor(max_workers=threads, initializer=O.set_threads, initargs=(1,)) as pool:
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            out = list(pool.map(one, sample))
+            rates.append(len(sample) / (time.perf_counter() - t0))
+    return float(np.median(rates)), rates, out
+
+
+def collection_headline(ex, args):
+    """BASELINE configuration 5 end to end, the headline: the 40-fragment, three-vessel collection
+    (walls t = 0.8 / 1.0 / 1.3 at 4-6 edges per t, two fragments non-watertight), all 780 pairs x 4
+    candidate poses. One step = the batched pipeline over every candidate: the breakline ICP
+    ladder (one launch per rung, graph-replayed), the Huber LM (f32 sums, FP64 6x6 solve), and R §6's
+    verification with the FP64 retest (sherd_hip/collection.py). The plain HIP schedule (one call per
+    pair and rung, per pair, per candidate) is timed over its own steps on the same candidates, with
+    every result checked identical. Setup (scenes, parity grids, distance bounds, clouds) is timed
+    apart and stated."""
     from sherd_hip import collection as CL
 
     t0 = time.perf_counter()
@@ -611,107 +720,113 @@ def collection_section(ex, args):
     pairs, cp, cT = CL.candidates(coll)
     scales = CL.pair_scales(coll, pairs)
     gen_s = time.perf_counter() - t0
+    n_tri = int(sum(len(f["F"]) for f in coll["fragments"]))
     t0 = time.perf_counter()
     dev = CL.Device(ex, coll)
-    prep_s = time.perf_counter() - t0
-    n_tri = sum(len(f["F"]) for f in coll["fragments"])
+    setup_s = time.perf_counter() - t0
     log(f"collection: {len(coll['fragments'])} fragments, {n_tri} triangles, {len(pairs)} pairs, {len(cT)} "
-        f"candidates; generated in {gen_s:.1f}s, on the device in {prep_s:.1f}s")
-    CL.run_batched(dev, coll, pairs, cp, cT, scales)  # workgroup tuning and graph capture
-    b = CL.run_batched(dev, coll, pairs, cp, cT, scales)
-    p = CL.run_plain(dev, coll, pairs, cp, cT, scales)
+        f"candidates; generated in {gen_s:.1f}s, setup on the device {setup_s:.2f}s")
+    for _ in range(max(args.warmup, 2)):  # the first pass tunes the ICP workgroups, the second captures
+        CL.run_batched(dev, coll, pairs, cp, cT, scales)
+    ex.profile(True)
+    t0 = time.perf_counter()
+    parts = np.zeros(3)
+    for _ in range(args.steps):
+        b = CL.run_batched(dev, coll, pairs, cp, cT, scales)
+        parts += [b["icp_s"], b["lm_s"], b["verify_s"]]
+    el = time.perf_counter() - t0
+    prof = ex.profile_read()
+    ex.profile(False)
+    n_cand = len(cT)
+    value = n_cand * args.steps / el
+    log(f"collection batched: {args.steps} steps in {el:.3f}s ({1e3 * el / args.steps:.2f} ms/step; icp "
+        f"{1e3 * parts[0] / args.steps:.2f}, lm {1e3 * parts[1] / args.steps:.2f}, verify "
+        f"{1e3 * parts[2] / args.steps:.2f} ms)")
+    # The plain schedule over its own timed steps, after one untimed pass.
+    CL.run_plain(dev, coll, pairs, cp, cT, scales)
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        p = CL.run_plain(dev, coll, pairs, cp, cT, scales)
+    el_plain = time.perf_counter() - t0
+    plain = n_cand * args.plain_steps / el_plain
     same = dict(icp=bool(np.array_equal(b["reg"], p["reg"])), lm=bool(np.array_equal(b["T_lm"], p["T_lm"])),
                 scores=bool(np.array_equal(b["scores"].view(np.uint64), p["scores"].view(np.uint64))),
                 accepted=bool(np.array_equal(b["accepted"], p["accepted"])))
-    log(f"collection batched {b['seconds']:.3f}s (icp {b['icp_s']:.3f}, lm {b['lm_s']:.3f}, verify {b['verify_s']:.3f}); "
-        f"plain {p['seconds']:.3f}s; identical {same}")
+    log(f"collection plain: {args.plain_steps} steps in {el_plain:.2f}s; batched/plain {value / plain:.1f}x; "
+        f"identical {same}")
     adj = np.array([tuple(pr) in coll["adjacent"] for pr in pairs])[cp]
-    res = dict(fragments=len(coll["fragments"]), triangles=int(n_tri), pairs=len(pairs), candidates=int(len(cT)),
-               non_watertight=list(CL.OPEN_FRAGMENTS), batched_seconds=round(b["seconds"], 4),
-               batched_icp_lm_verify_s=[round(b["icp_s"], 4), round(b["lm_s"], 4), round(b["verify_s"], 4)],
-               plain_seconds=round(p["seconds"], 4),
-               plain_icp_lm_verify_s=[round(p["icp_s"], 4), round(p["lm_s"], 4), round(p["verify_s"], 4)],
-               batched_vs_plain=round(p["seconds"] / b["seconds"], 3), identical_plain_batched=same,
-               accepted=int(b["accepted"].sum()), accepted_on_adjacent_pairs=int((b["accepted"] & adj).sum()),
-               pen_unavailable=int(b["scores"][:, 16].sum()))
+    res = dict(
+        value=value, ms_per_step=1e3 * el / args.steps, candidates=n_cand, prof=prof, coll=coll, pairs=pairs, cp=cp,
+        reg=b["reg"],
+        section=dict(
+            fragments=len(coll["fragments"]), triangles=n_tri, pairs=len(pairs), candidates=int(n_cand),
+            non_watertight=list(CL.OPEN_FRAGMENTS), generation_seconds=round(gen_s, 2),
+            setup_seconds=round(setup_s, 4),
+            setup="scenes (LBVH) of every fragment's mesh and fracture subset, parity grids, distance bounds, "
+                  "sample sets, breakline clouds; once per collection, outside the steps",
+            step_ms=round(1e3 * el / args.steps, 4),
+            step_icp_lm_verify_ms=[round(1e3 * x / args.steps, 4) for x in parts],
+            end_to_end_with_setup_seconds=round(setup_s + el / args.steps, 4),
+            plain_steps=args.plain_steps, plain_step_seconds=round(el_plain / args.plain_steps, 4),
+            plain_icp_lm_verify_s=[round(p["icp_s"], 4), round(p["lm_s"], 4), round(p["verify_s"], 4)],
+            plain_pair_verifications_per_s=round(plain, 3), batched_vs_plain=round(value / plain, 3),
+            batched_vs_plain_with_setup=round((el_plain / args.plain_steps + setup_s) / (el / args.steps + setup_s), 3),
+            target_3x_met=bool(value / plain >= 3.0), identical_plain_batched=same,
+            accepted=int(b["accepted"].sum()), accepted_on_adjacent_pairs=int((b["accepted"] & adj).sum()),
+            adjacent_candidates=int(adj.sum()), pen_unavailable=int(b["scores"][:, 16].sum())))
+    cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(HERE, "tests"))
+        # A bounded sample: every candidate of the pairs among six fragments (the first four of the
+        # first vessel, the first two of the second), so adjacent, same-vessel and cross-vessel pairs
+        # all appear; all host cores threaded over candidates, and one thread, median of five each.
+        S = {0, 1, 2, 3, 13, 14}
+        sample = [c for c in range(n_cand) if set(pairs[cp[c]]) <= S]
+        model, nproc = host_info()
+        cores = min(nproc, 16)
+        v_all, runs_all, out = collection_oracle(coll, pairs, cp, cT, scales, sample, cores)
+        v_one, runs_one, _ = collection_oracle(coll, pairs, cp, cT, scales, sample[:8], 1)
+        # Parity on the sample: the oracle's ladder against the device's (the transform bar), and
+        # verify at the device's final poses bit for bit (checked by the tests; here the oracle's own
+        # pipeline's accept verdicts against the device's).
         import oracle_lib as O
 
-        O.set_threads(min(os.cpu_count() or 1, 16))
-        sample = np.nonzero(adj)[0][:24]
-        need = sorted(set(i for c in sample for i in pairs[cp[c]]))
-        keep, side = [], {}
-        for i in need:
-            fr = coll["fragments"][i]
-            fs, ms = O.Scene(fr["V"], fr["F"], fr["frac"]), O.Scene(fr["V"], fr["F"])
-            keep += [fs, ms]
-            side[i] = O.SideData(fs, ms, fr["watertight"], fr["frac_area"], fr["Pf"], fr["S"], fr["brk_P"],
-                                 fr["brk_ns"], fr["margin_p"], fr["margin_n"])
-        t0 = time.perf_counter()
-        ok = True
-        for c in sample:
-            a_, b_ = pairs[cp[c]]
-            want = O.verify(side[a_], side[b_], b["T_lm"][c], scales[cp[c]], retest=True)
-            ok &= bool(np.array_equal(want.view(np.uint64), b["scores"][c].view(np.uint64)))
-        el = time.perf_counter() - t0
-        res["cpu_verify_per_s"] = round(len(sample) / el, 2)
-        res["cpu_verify_identical_on_sample"] = ok
-        res["cpu_sample"] = f"oracle verify (FP64 retest) of {len(sample)} adjacent-pair candidates at the batched final poses"
-    dev.close()
+        T1o = np.stack([o[0] for o in out])
+        T1d = b["reg"][sample, -1, :16].reshape(-1, 4, 4)
+        ang, tr = pose_gap_deg_t(T1o, T1d)
+        acc_o = np.array([bool(O.accept(o[2], scales[cp[c]])) for o, c in zip(out, sample)])
+        cpu = dict(value=round(v_all, 3), unit="pair-verifications/s", cores=cores, kind="port",
+                   sample=f"the oracle's end-to-end pipeline (breakline ladder, FP64 LM, verify with the FP64 retest) "
+                          f"on all {len(sample)} candidates of the 15 pairs among fragments {sorted(S)} of the same "
+                          f"collection, {cores} host threads over candidates (pair.rs:350-359), median of 5; "
+                          f"scene builds excluded",
+                   one_thread_value=round(v_one, 3), one_thread_sample=f"the first 8 of those candidates, 1 thread, "
+                                                                         f"median of 5",
+                   runs=[round(x, 3) for x in runs_all], one_thread_runs=[round(x, 3) for x in runs_one],
+                   cpu_model=model, nproc=nproc,
+                   icp_vs_device_worst_deg_t=[float(ang.max()), float(tr.max())],
+                   accept_identical_to_device=bool(np.array_equal(acc_o, b["accepted"][sample])))
+        log(f"collection oracle: {v_all:.2f} candidates/s on {cores} threads, {v_one:.2f} on 1 ({model}); ladder "
+            f"vs device worst {ang.max():.1e} deg {tr.max():.1e} t; accept identical {cpu['accept_identical_to_device']}")
+    res["cpu"] = cpu
+    res["dev"] = dev
     return res
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--edges-per-t", type=float, default=5.5)
-    ap.add_argument("--plain-candidates", type=int, default=280)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--icp-pairs", type=int, default=28, help="pairs of the batched ICP ladder workload (0: skip)")
-    ap.add_argument("--icp-poses", type=int, default=64)
-    ap.add_argument("--icp-plain-pairs", type=int, default=-1,
-                    help="pairs run in the plain per-call ICP schedule (-1: all, 0: none)")
-    ap.add_argument("--no-huber", action="store_true")
-    ap.add_argument("--no-refine", action="store_true", help="skip R §9's full-resolution refinement")
-    ap.add_argument("--no-match", action="store_true", help="skip R §5.1-5.5 over the vessel's pairs")
-    ap.add_argument("--no-thickness", action="store_true", help="skip R §3.2 over the vessel's fragments")
-    ap.add_argument("--no-segment", action="store_true", help="skip R §3.4.4-3.4.8 over the vessel's fragments")
-    ap.add_argument("--no-collection", action="store_true",
-                    help="skip the 40-fragment collection end to end (about 20 s of generation)")
-    ap.add_argument("--batched-only", action="store_true",
-                    help="run only the batched steps (for counter collection); prints no bench line")
-    ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="launch one device-to-device copy of this many MiB and exit: the known-size "
-                         "kernel that scales FETCH_SIZE / WRITE_SIZE readings to bytes")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the pair list would shard across GPUs, see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-
-    import sherd_hip
-    from sherd_hip import synth
-
-    if args.copy_probe_mib:
-        ex = sherd_hip.HipExecutor(0)
-        nbytes = ex.copy_probe(args.copy_probe_mib << 20)
-        log(f"copy probe: {nbytes} bytes read and {nbytes} bytes written by one launch")
-        ex.close()
-        return
-
+def vessel_section(ex, synth, args):
+    """BASELINE configuration 2 (plain method on GPU vs CPU), the vessel: 8 fragments at
+    --edges-per-t edges per t (5.5: 0.9-1.6 M triangles each, the per-fragment size the configuration
+    states; its 17 edges per t would be about 87 M triangles in all), all 28 pairs x 10 candidate
+    poses through R §6 in one batch per step, timed over the same steps; beside it the plain
+    schedule, the cone-vote rays, kNN queries, the LBVH build rate and the roofline of the inside
+    pass (k_pen) with the side-stream sections listed as overlapped."""
     t0 = time.perf_counter()
     col = synth.vessel(edges_per_t=args.edges_per_t)
     frags = col["fragments"]
-    log(f"generated {len(frags)} fragments, {sum(len(f['F']) for f in frags)} triangles in "
-        f"{time.perf_counter() - t0:.1f}s")
+    n_tri = int(sum(len(f["F"]) for f in frags))
+    log(f"vessel: {len(frags)} fragments, {n_tri} triangles generated in {time.perf_counter() - t0:.1f}s")
     scales = synth.scales_for_pair(col["t"], col["res"])
     pairs, cand_pair, cand_T = synth.candidates(col)
-
-    ex = sherd_hip.HipExecutor(0)
-    log("device:", ex.device_name())
+    b_sb = ex.stats("scene_build")
     scenes, handles = [], []
     for fr in frags:
         fs = ex.scene(fr["V"], fr["F"], fr["frac"])
@@ -720,10 +835,8 @@ def main():
         handles.append(ex.fragment(fs, ms, fr["watertight"], fr["frac_area"], fr["Pf"], fr["S"], fr["brk_P"],
                                    fr["brk_ns"], fr["margin_p"], fr["margin_n"]))
     st = ex.stats("scene_build")
-    lbvh_mtri = st["items"] / (st["device_ns"] * 1e-9) / 1e6
+    lbvh_mtri = (st["items"] - b_sb["items"]) / ((st["device_ns"] - b_sb["device_ns"]) * 1e-9) / 1e6
     vpairs = [(handles[a], handles[b], scales) for (a, b) in pairs]
-
-    # ---- batched verification (the timed steps) -----------------------------------------------
     for _ in range(args.warmup):
         ex.verify_batch(vpairs, cand_pair, cand_T)
     ex.profile(True)
@@ -735,14 +848,6 @@ def main():
     ex.profile(False)
     n_cand = len(cand_T)
     value = n_cand * args.steps / el
-    log(f"batched: {n_cand} candidates x {args.steps} steps in {el:.3f}s; accepted {int(accepted.sum())}")
-    if args.batched_only:
-        for k in sorted(prof, key=lambda k: -prof[k]["ms"]):
-            log(f"  {k}: {prof[k]['ms'] / prof[k]['launches']:.4f} ms/launch over {prof[k]['launches']} launches")
-        ex.close()
-        return
-
-    # ---- plain: one candidate per call ------------------------------------------------------------
     n_plain = max(1, min(args.plain_candidates, n_cand))
     t0 = time.perf_counter()
     plain_acc = np.empty(n_plain, bool)
@@ -754,77 +859,178 @@ def main():
     plain = n_plain / el_plain
     same = bool(np.array_equal(plain_acc, accepted[:n_plain]) and
                 np.array_equal(plain_scores.view(np.uint64), scores[:n_plain].view(np.uint64)))
-    log(f"plain: {n_plain} candidates in {el_plain:.3f}s; identical to batched: {same}")
-
-    # ---- rays: the seven-ray cone vote over every face ----------------------------------------
+    log(f"vessel verify: {value:.0f} pair-verifications/s batched, {plain:.0f} plain; identical {same}")
+    # Rays: the seven-ray cone vote over every face.
     before = ex.stats("cone")
     for fr, (fs, ms) in zip(frags, scenes):
-        ns = fr["face_normals"]
-        ex.cone_votes(ms, fr["V"].astype(np.float64), fr["F"], None, ns, col["t"])
+        ex.cone_votes(ms, fr["V"].astype(np.float64), fr["F"], None, fr["face_normals"], col["t"])
     after = ex.stats("cone")
     rays = after["items"] - before["items"]
     rays_per_s = rays / ((after["device_ns"] - before["device_ns"]) * 1e-9)
-
-    # ---- kNN queries/s: R §6.3's margin query shape, all margins against one -----------------------
+    # kNN queries/s: R §6.3's margin query shape, all margins against one.
     q = np.concatenate([fr["margin_p"] for fr in frags])
-    cloud = frags[0]["margin_p"]
     b0 = ex.stats("nearest")
     for _ in range(3):
-        ex.nearest_below(cloud, q, scales[8])
+        ex.nearest_below(frags[0]["margin_p"], q, scales[8])
     b1 = ex.stats("nearest")
     knn = (b1["items"] - b0["items"]) / ((b1["device_ns"] - b0["device_ns"]) * 1e-9)
-
-    # ---- roofline of the dominant verification kernel -------------------------------------------
-    # k_frac, k_seam and k_cont run on side streams, concurrently with each other and with the
-    # closest-approach passes: their launch durations are stretched by the sharing and bound no
-    # single kernel. The roofline is the inside pass's; the overlapped ones are reported beside it.
-    dom = ROOFLINE_KERNEL
-    avg_ms = prof[dom]["ms"] / prof[dom]["launches"]
-    nbytes = algorithmic_bytes(dom, frags, pairs, cand_pair)
-    achieved = nbytes / (avg_ms * 1e-3) / 1e9
+    avg_ms = prof[ROOFLINE_KERNEL]["ms"] / prof[ROOFLINE_KERNEL]["launches"]
+    nbytes = algorithmic_bytes(ROOFLINE_KERNEL, frags, pairs, cand_pair)
     overlapped = {}
     for k in OVERLAPPED_KERNELS:
         if k in prof:
             ms_k = prof[k]["ms"] / prof[k]["launches"]
             b_k = algorithmic_bytes(k, frags, pairs, cand_pair)
             overlapped[k] = dict(avg_launch_ms=round(ms_k, 4), algorithmic_bytes_per_launch=b_k,
-                                 achieved=round(b_k / (ms_k * 1e-3) / 1e9, 3))
-    traffic = None
-    pmc = os.path.join(HERE, "profiles", "verify_vessel_pmc.json")
-    if os.path.exists(pmc):
-        with open(pmc) as fh:
-            traffic = json.load(fh).get("kernels", {}).get(dom, {}).get("bytes_per_launch")
-    roof = dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 6), traffic=traffic, kernel=dom,
-                algorithmic_bytes_per_launch=nbytes, avg_launch_ms=round(avg_ms, 4), overlapped=overlapped)
-
-    icp = icp_ladder(ex, synth, args) if args.icp_pairs > 0 else None
-    huber = huber_section(ex, synth, args) if not args.no_huber else None
-    coll = collection_section(ex, args) if not args.no_collection else None
-    refine = refine_section(ex, col, args) if not args.no_refine else None
-    match = match_section(ex, col, args) if not args.no_match else None
-    thick = thickness_section(ex, col, args) if not args.no_thickness else None
-    segm = segment_section(ex, col, args) if not args.no_segment else None
-
-    cpu = None
+                                 achieved_gbs=round(b_k / (ms_k * 1e-3) / 1e9, 3))
+    roof = kernel_roofline(ROOFLINE_KERNEL, nbytes, avg_ms, "verify_vessel_pmc.json")
+    roof["overlapped"] = overlapped
+    res = dict(workload=f"vessel: 8 fragments, {args.edges_per_t} edges/t, {n_tri} triangles, 28 pairs x 10 "
+                        f"candidates", pair_verifications_per_s=round(value, 3), steps=args.steps,
+               ms_per_step=round(1e3 * el / args.steps, 4), plain_pair_verifications_per_s=round(plain, 3),
+               plain_candidates=n_plain, batched_vs_plain=round(value / plain, 3), accepted=int(accepted.sum()),
+               identical_plain_batched=same, rays_per_s=round(rays_per_s, 1), rays=int(rays),
+               knn_queries_per_s=round(knn, 1), lbvh_mtri_per_s=round(lbvh_mtri, 2), kernel_ms=prof, roofline=roof)
     if not args.no_cpu:
         cpu, oracle_scores = cpu_baseline(col, pairs, scales, cand_pair, cand_T)
         k = len(oracle_scores)
-        parity = bool(np.array_equal(np.array(oracle_scores).view(np.uint64), scores[:k].view(np.uint64)))
-        cpu["scores_identical_to_hip"] = parity
-        log(f"cpu baseline: {cpu['value']:.2f} pair-verifications/s; oracle == HIP on the sample: {parity}")
+        cpu["scores_identical_to_hip"] = bool(np.array_equal(np.array(oracle_scores).view(np.uint64),
+                                                             scores[:k].view(np.uint64)))
+        res["cpu"] = cpu
+    for h in handles:
+        h.close()
+    for fs, ms in scenes:
+        fs.close()
+        ms.close()
+    return res, col
 
+
+def kernel_roofline(kernel, nbytes, avg_ms, pmc_file):
+    """The roofline object of one kernel: algorithmic bytes per launch over its average launch time
+    (HIP events on the library's stream over the timed steps) against the HBM peak; beside it the
+    HBM bytes per launch read from a committed rocprofv3 counter summary (FETCH_SIZE, WRITE_SIZE,
+    scaled by the copy-kernel calibration) when one exists for this kernel, and the bandwidth those
+    measured bytes give."""
+    achieved = nbytes / (avg_ms * 1e-3) / 1e9
+    traffic, src = None, None
+    path = os.path.join(HERE, "profiles", pmc_file)
+    if os.path.exists(path):
+        with open(path) as fh:
+            traffic = json.load(fh).get("kernels", {}).get(kernel, {}).get("bytes_per_launch")
+        src = f"profiles/{pmc_file} (rocprofv3 --pmc of the same workload, committed)" if traffic else None
+    out = dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
+               frac=round(achieved / HBM_PEAK_GBS, 6), traffic=traffic, kernel=kernel,
+               algorithmic_bytes_per_launch=int(nbytes), avg_launch_ms=round(avg_ms, 4))
+    if traffic:
+        out["traffic_source"] = src
+        out["measured_gbs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 3)
+        out["measured_frac"] = round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 6)
+    return out
+
+
+# Kernel records that time a sequence of launches, not one kernel (no roofline of their own).
+SEQUENCES = ("k_huber_lm", "k_huber_grad", "k_huber_energy")
+
+
+def collection_roofline(head):
+    """The roofline of the collection step's dominant kernel: the one with the most device time
+    among those that run alone on the library's stream (the verification sections on side streams
+    and the records of launch sequences are left out)."""
+    prof, coll, pairs, cp, reg = head["prof"], head["coll"], head["pairs"], head["cp"], head["reg"]
+    frags = coll["fragments"]
+    known = {"k_icp_batch", ROOFLINE_KERNEL}
+    cands = {k: v for k, v in prof.items() if k not in OVERLAPPED_KERNELS and k not in SEQUENCES and v["launches"]}
+    dom = max(cands, key=lambda k: cands[k]["ms"])
+    if dom not in known:
+        dom = max((k for k in cands if k in known), key=lambda k: cands[k]["ms"])
+    avg_ms = prof[dom]["ms"] / prof[dom]["launches"]
+    if dom == "k_icp_batch":
+        from sherd_hip import collection as CL
+
+        srcs = [frags[b]["brk_P"] for a, b in pairs]
+        tgt_of = [a for a, b in pairs]
+        rung_bytes = []
+        for r in range(len(CL.RUNGS)):
+            iters = reg[:, r, 19]
+            per = int(sum(48 * len(srcs[int(p)]) * (int(k) + 1) for p, k in zip(cp, iters)))
+            used = sorted(set(tgt_of[int(p)] for p in cp))
+            rung_bytes.append(per + sum(cloud_tree_bytes(len(frags[a]["brk_P"])) for a in used))
+        nbytes = int(np.mean(rung_bytes))
+    else:
+        nbytes = algorithmic_bytes(dom, frags, pairs, cp)
+    roof = kernel_roofline(dom, nbytes, avg_ms, "collection_pmc.json")
+    roof["share_of_step_device_ms"] = round(prof[dom]["ms"] / sum(v["ms"] for v in cands.values()), 4)
+    return roof
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=2,
+                    help="timed steps of the plain schedule on the collection (after one untimed)")
+    ap.add_argument("--edges-per-t", type=float, default=5.5)
+    ap.add_argument("--plain-candidates", type=int, default=280)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-vessel", action="store_true", help="skip configuration 2 (and the sections on its vessel)")
+    ap.add_argument("--icp-pairs", type=int, default=28, help="pairs of the batched ICP ladder workload (0: skip)")
+    ap.add_argument("--icp-poses", type=int, default=64)
+    ap.add_argument("--icp-plain-pairs", type=int, default=-1,
+                    help="pairs run in the plain per-call ICP schedule (-1: all, 0: none)")
+    ap.add_argument("--no-huber", action="store_true")
+    ap.add_argument("--no-refine", action="store_true", help="skip R §9's full-resolution refinement")
+    ap.add_argument("--no-match", action="store_true", help="skip R §5.1-5.5 over the vessel's pairs")
+    ap.add_argument("--no-thickness", action="store_true", help="skip R §3.2 over the vessel's fragments")
+    ap.add_argument("--no-segment", action="store_true", help="skip R §3.4.4-3.4.8 over the vessel's fragments")
+    ap.add_argument("--collection-only", action="store_true",
+                    help="run only the collection's steps (for counter collection); prints no bench line")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the pair list would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import sherd_hip
+    from sherd_hip import synth
+
+    ex = sherd_hip.HipExecutor(0)
+    log("device:", ex.device_name())
+    if args.collection_only:
+        args.no_cpu = True
+    head = collection_headline(ex, args)
+    roof = collection_roofline(head)
+    log(f"collection roofline: {roof['kernel']} {roof['avg_launch_ms']} ms/launch, {roof['achieved']} GB/s "
+        f"({roof['frac']:.3f} of peak)")
+    head["dev"].close()
+    if args.collection_only:
+        for k in sorted(head["prof"], key=lambda k: -head["prof"][k]["ms"]):
+            v = head["prof"][k]
+            log(f"  {k}: {v['ms'] / v['launches']:.4f} ms/launch over {v['launches']} launches")
+        ex.close()
+        return
+    vessel, col = (None, None) if args.no_vessel else vessel_section(ex, synth, args)
+    icp = icp_ladder(ex, synth, args) if args.icp_pairs > 0 else None
+    huber = huber_section(ex, synth, col, args) if not args.no_huber and col is not None else None
+    refine = refine_section(ex, col, args) if not args.no_refine and col is not None else None
+    match = match_section(ex, col, args) if not args.no_match and col is not None else None
+    thick = thickness_section(ex, col, args) if not args.no_thickness and col is not None else None
+    segm = segment_section(ex, col, args) if not args.no_segment and col is not None else None
+
+    sec = head["section"]
     line = dict(
-        metric=METRIC, value=round(value, 3), unit="pair-verifications/s", n_gpus=1, steps=args.steps,
-        warmup=args.warmup, ms_per_step=round(1e3 * el / args.steps, 3), higher_is_better=True, scaling="weak",
+        metric=METRIC, value=round(head["value"], 3), unit="pair-verifications/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=round(head["ms_per_step"], 3), higher_is_better=True, scaling="weak",
         vs_baseline=None, dtype="f32+f64", data="synthetic",
-        config=dict(workload=f"vessel: 8 fragments, {args.edges_per_t} edges/t, "
-                             f"{sum(len(f['F']) for f in frags)} triangles, 28 pairs x 10 candidates",
-                    candidates_per_step=n_cand, parallelism="single GPU"),
-        plain_pair_verifications_per_s=round(plain, 3), batched_vs_plain=round(value / plain, 3),
-        accepted=int(accepted.sum()), accepted_identical_plain_batched=same,
-        rays_per_s=round(rays_per_s, 1), rays=int(rays), knn_queries_per_s=round(knn, 1),
-        lbvh_mtri_per_s=round(lbvh_mtri, 2), icp=icp, huber=huber, collection=coll, refine=refine, match=match, thickness=thick, segment=segm, kernel_ms=prof, roofline=roof, cpu_baseline=cpu,
+        config=dict(workload=f"collection (configuration 5): 40 fragments of 3 vessels (t 0.8/1.0/1.3 at 4/5/6 "
+                             f"edges per t), {sec['triangles']} triangles, fragments 7 and 23 non-watertight; all "
+                             f"780 pairs x 4 candidates per step through the batched breakline ICP ladder, the "
+                             f"Huber LM and verification with the FP64 retest",
+                    candidates_per_step=sec["candidates"], pairs=sec["pairs"], parallelism="single GPU"),
+        plain_pair_verifications_per_s=sec["plain_pair_verifications_per_s"], batched_vs_plain=sec["batched_vs_plain"],
+        accepted=sec["accepted"], accepted_identical_plain_batched=sec["identical_plain_batched"]["accepted"],
+        collection=sec, vessel=vessel, icp=icp, huber=huber, refine=refine, match=match, thickness=thick,
+        segment=segm, kernel_ms=head["prof"], roofline=roof, cpu_baseline=head["cpu"],
     )
     print(json.dumps(line), flush=True)
     ex.close()

@@ -467,10 +467,6 @@ int sherd_profile(sherd_ctx *ctx, int on);
 /* The totals as a JSON object {"kernel": {"ms": total, "launches": n}, ...}. */
 int sherd_profile_read(sherd_ctx *ctx, char *buf, size_t len);
 
-/* Diagnostic with no counterpart in the reference: one launch of a plain 16-byte-vector copy kernel
- * (k_copy_probe) reading and writing `bytes` bytes (rounded down to a multiple of 16) of the
- * context's scratch memory; the known-size kernel against which hardware byte counters are scaled. */
-int sherd_copy_probe(sherd_ctx *ctx, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -2,7 +2,8 @@
 
     python sherd-refit_amd/build.py
 
-The library lands in sherd-refit_amd/sherd_hip/lib/ (in-tree, git-ignored).
+The library lands in sherd-refit_amd/sherd_hip/lib/ (in-tree, git-ignored); the counter-calibration
+program in sherd-refit_amd/calib/copy_probe (git-ignored).
 """
 
 import concurrent.futures as cf
@@ -57,6 +58,12 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    # The counter-calibration program (not part of the library): a known-size copy kernel.
+    probe = os.path.join(HERE, "calib", "copy_probe")
+    r = subprocess.run([HIPCC] + FLAGS + [os.path.join(HERE, "calib", "copy_probe.hip"), "-o", probe],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed on calib/copy_probe.hip:\n" + r.stdout + r.stderr)
     return LIB
 
 

@@ -770,36 +770,7 @@ int sherd_thickness_from_hits(sherd_ctx *ctx, const double *normals, size_t nf, 
 
 }  // extern "C"
 
-namespace {
-
-__global__ void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
-}
-
-}  // namespace
-
 extern "C" {
-
-int sherd_copy_probe(sherd_ctx *ctx, size_t bytes) {
-  Ctx *c = reinterpret_cast<Ctx *>(ctx);
-  LOCK(c);
-  size_t n = bytes / sizeof(uint4);
-  if (n == 0) return fail(SHERD_E_ARG, "copy probe needs at least 16 bytes");
-  hipError_t e;
-  const uint4 *src = (const uint4 *)c->scratch(0, n * sizeof(uint4), &e);
-  if (!src) return hip_fail(e, "scratch");
-  uint4 *dst = (uint4 *)c->scratch(1, n * sizeof(uint4), &e);
-  if (!dst) return hip_fail(e, "scratch");
-  int blocks = (int)std::min<size_t>((n + 255) / 256, 256 * 64);
-  if (c->prof.on) c->prof.begin("k_copy_probe", c->stream);
-  k_copy_probe<<<blocks, 256, 0, c->stream>>>(src, dst, n);
-  HIPCK(hipGetLastError(), "copy probe launch");
-  if (c->prof.on) c->prof.end(c->stream);
-  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
-  if (c->prof.on) c->prof.collect();
-  return SHERD_OK;
-}
 
 int sherd_profile(sherd_ctx *ctx, int on) {
   Ctx *c = reinterpret_cast<Ctx *>(ctx);

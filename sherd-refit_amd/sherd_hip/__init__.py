@@ -160,7 +160,6 @@ def _bind(L):
                                         P(Registration)]),
         "sherd_profile": (ci, [vp, ci]),
         "sherd_profile_read": (ci, [vp, C.c_char_p, sz]),
-        "sherd_copy_probe": (ci, [vp, sz]),
         "sherd_cone_votes": (ci, [vp, vp, P(d), sz, P(u32), sz, P(i64), P(d), sz, d, ci, P(u8)]),
         "sherd_thickness": (ci, [vp, vp, P(d), sz, P(u32), sz, P(C.c_float), P(C.c_float), P(ci)]),
         "sherd_thickness_from_hits": (ci, [vp, P(d), sz, P(u32), P(C.c_float), P(u32), sz, P(C.c_float),
@@ -462,12 +461,6 @@ class HipExecutor:
         buf = C.create_string_buffer(8192)
         _check(lib().sherd_profile_read(self.ptr, buf, 8192))
         return json.loads(buf.value.decode())
-
-    def copy_probe(self, nbytes):
-        """One launch of the plain copy kernel k_copy_probe over `nbytes` (rounded down to 16 B);
-        returns the bytes it read (and wrote)."""
-        _check(lib().sherd_copy_probe(self.ptr, int(nbytes)))
-        return int(nbytes) // 16 * 16
 
     def scene(self, V, F, mask=None):
         return Scene(self, V, F, mask)
