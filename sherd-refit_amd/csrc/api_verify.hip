@@ -243,9 +243,11 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
   FragDesc *dfd = (FragDesc *)dfp;
   PairDesc *dpd = (PairDesc *)(dfp + fd_bytes);
   Timer tm(c, SHERD_M_VERIFY, n_cand);
-  // Chunks bounded by the key and voxel-table scratch they need (256 MiB of keys per chunk).
-  const size_t KEY_BUDGET = (size_t)1 << 25;
-  const size_t HARD_BUDGET = (size_t)1 << 27;  // closest-approach queue entries (1 GiB)
+  // Chunks bounded by the key and voxel-table scratch they need: up to 2 GiB of keys and 2 GiB of
+  // closest-approach queue per chunk, so a collection of thousands of candidates runs as one chunk
+  // (one pass of every kernel) on a 288 GB device.
+  const size_t KEY_BUDGET = (size_t)1 << 28;
+  const size_t HARD_BUDGET = (size_t)1 << 28;  // closest-approach queue entries
   // Workgroups per (candidate, side) task of the per-point kernels; a chunk keeps its grid in int.
   const size_t per_task = (size_t)std::max({max_pf, max_s, max_margin, max_brk, 1}) / 256 + 1;
   // Candidates per chunk: a grid's y extent (65535) and x extent in int.
@@ -340,8 +342,10 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     HIPCK(hipStreamSynchronize(c->stream), "synchronize");
     c->prof.collect();
     const CandAcc *hacc = (const CandAcc *)(hbuf + dn_acc);
-    for (size_t i = 0; i < nc; ++i)
+    for (size_t i = 0; i < nc; ++i) {
       if (hacc[i].flags & FLAG_VOXEL_RANGE) return fail(SHERD_E_LIMIT, "a seam voxel coordinate exceeds 2^20");
+      if (hacc[i].flags & FLAG_GAVE_UP) return fail(SHERD_E_LIMIT, "a closest-approach query exceeded its step budget");
+    }
     std::memcpy(scores + VS_COUNT * begin, hbuf + dn_sc, sizeof(double) * VS_COUNT * nc);
     std::memcpy(accepted + begin, hbuf + dn_ac, nc);
     begin = end;

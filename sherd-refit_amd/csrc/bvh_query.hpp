@@ -398,7 +398,12 @@ __device__ inline float min_d2_subtree(const SceneView &sv, int c, V3 p, float w
 // are measured, and the wave's minimum tightens the bound. The result is the same minimum; a query
 // whose frontier would overflow the stack has the overflowing lanes finish their subtrees alone.
 // Every lane of the wave calls it.
-__device__ inline float min_d2_wave(const SceneView &sv, V3 p, float window, int *stk, int cap) {
+//
+// max_steps bounds the steps (each up to 64 nodes); a query that reaches it stops with *complete
+// false and the smallest distance met so far (one the point attains).
+__device__ inline float min_d2_wave(const SceneView &sv, V3 p, float window, int *stk, int cap, int max_steps,
+                                    bool *complete) {
+  *complete = true;
   int lane = threadIdx.x & 63;
   float slack = slack_of(sv.absmax, p);
   float best = __builtin_inff();
@@ -411,6 +416,10 @@ __device__ inline float min_d2_wave(const SceneView &sv, V3 p, float window, int
   if (lane == 0) stk[0] = sv.root;
   int sp = 1;
   for (int guard = 0; sp > 0 && guard <= nodes; ++guard) {
+    if (guard >= max_steps) {
+      *complete = false;
+      break;
+    }
     float lim = mnf(sqrtf(best), window) + slack;
     float lim2 = lim * lim;
     int take = sp < 64 ? sp : 64;

@@ -14,7 +14,9 @@
 //              voxel set of the seam points
 //   k_cont     R §6.3: B's margin against A's margin grid, steps and normal agreements
 //   k_pen      R §6.4: surface samples inside the other mesh, deeper-than-pen count, deepest
-//   k_pen_min  R §6.4's fallback when nothing is inside: the closest approach
+//   k_pen_lb   R §6.4's fallback when nothing is inside: each side's sample of smallest bound
+//   k_pen_seed the exact distance of that sample, one wave each: a first attained window
+//   k_pen_min  the closest approach
 //   k_pen_hard the closest-approach queries too long for one lane, one wave each
 //   k_median   the four medians (numpy semantics, verify.rs:693-700)
 //   k_final    Scores and the accept verdict
@@ -35,6 +37,11 @@ constexpr int VT = 256;
 constexpr int PEN_STEPS = 64;      // nodes one lane walks for a closest-approach query before a wave takes over
 constexpr int HARD_STACK = 2048;   // pending nodes of a wave's cooperative query (LDS, 8 KiB per wave)
 constexpr int HARD_BLOCKS = 1024;  // workgroups of the cooperative pass
+// Steps (up to 64 nodes each) one wave may spend on one closest-approach query: sixteen times the
+// 2^21 nodes an exhaustive walk of a 2^20-triangle mesh visits. A query that needs more sets
+// FLAG_GAVE_UP and the batch fails with a status (the caller answers it on the CPU), so the kernel
+// always ends in bounded time.
+constexpr int WAVE_STEPS = 1 << 19;
 
 __device__ __forceinline__ void apply_f64(const double *T, const double *p, double *o) {
   o[0] = T[0] * p[0] + T[1] * p[1] + T[2] * p[2] + T[3];
@@ -76,6 +83,7 @@ __global__ void k_prep(VerifyBatch vb) {
   if (c == 0) vb.hard_n[0] = vb.hard_n[1] = 0;
   a.seam_cells = 0;
   a.flags = 0;
+  a.seed[0] = a.seed[1] = ~0ull;
 }
 
 // R §6.1 one_side inputs (verify.rs:318-349): side 0 is A's samples moved by T^-1 against B's
@@ -269,8 +277,62 @@ __global__ void __launch_bounds__(VT) k_pen(VerifyBatch vb, int per_task) {
   }
 }
 
-// verify.rs:567-578: nothing inside, so min(sd) is the closest approach, a running minimum whose
-// window shrinks to the best so far (DistReduce::Min, cpu.rs:100-111), from f32::MAX.
+// Whether side `side`'s closest approach can change the scores. pen_depth is
+// max(-min_a, -min_b) / t (verify.rs:494), min being -deepest on a side with a sample inside and the
+// closest approach (at least 0) on a side without: when the other side has a sample inside at a
+// positive depth, this side's term is at most 0 < that depth and is never read (at depth 0 exactly
+// it still is, for the sign of a zero).
+__device__ __forceinline__ bool closest_needed(const CandAcc &acc, int side) {
+  return acc.inside[side] == 0 && !(acc.inside[1 - side] != 0 && acc.deepest[1 - side] != 0u);
+}
+
+// The window a side's closest approach may prune with: the best distance attained so far on this
+// side, and on the other side too when that side has nothing inside. Only min(closest_a, closest_b)
+// is read then, so a side whose own minimum is above the other's may stop at any attained value
+// above it, and a side whose minimum is below is found exactly, every window being at least it.
+__device__ __forceinline__ float closest_window(CandAcc &acc, int side) {
+  uint32_t w = __hip_atomic_load(&acc.closest[side], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (acc.inside[1 - side] == 0) {
+    uint32_t o = __hip_atomic_load(&acc.closest[1 - side], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    w = o < w ? o : w;
+  }
+  return __uint_as_float(w);
+}
+
+// ---- R §6.4's closest approach when nothing is inside (verify.rs:567-578) --------------------------
+//
+// min(sd) is then the closest approach, a running minimum over the samples whose window shrinks to
+// the best attained so far (DistReduce::Min, cpu.rs:100-111), from f32::MAX.
+
+// The seed's first half: each side's sample of smallest bound (the distance-bound grid's value at the
+// moved sample), (bound bits << 32) | sample, kept by an atomic minimum.
+__global__ void __launch_bounds__(VT) k_pen_lb(VerifyBatch vb, int per_task) {
+  TaskPos tp = task_pos(vb, per_task);
+  const int c = tp.c, side = tp.side, i = tp.i;
+  const CandDesc &cd = vb.cand[c];
+  const PairDesc &pd = vb.pair[cd.pair];
+  if (!pd.pen_ok) return;
+  CandAcc &acc = vb.acc[c];
+  if (!closest_needed(acc, side)) return;
+  const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
+  const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
+  uint64_t key = ~0ull;
+  if (i < own.n_s) {
+    const double *T = side == 0 ? cd.Ti : cd.T;
+    const double *p = own.s + 3 * (size_t)i;
+    const V3 q = moved_f32(T, p[0], p[1], p[2]);
+    key = ((uint64_t)__float_as_uint(dist_bound(other.mesh.db, q)) << 32) | (uint32_t)i;
+  }
+  const uint32_t hi = wave_min((uint32_t)(key >> 32));
+  const uint32_t lo = wave_min((uint32_t)(key >> 32) == hi ? (uint32_t)key : 0xffffffffu);
+  if ((threadIdx.x & 63) == 0 && hi != 0xffffffffu)
+    atomicMin((unsigned long long *)&acc.seed[side], ((unsigned long long)hi << 32) | lo);
+}
+
+// The exact pass, one lane per sample: a sample whose distance bound already exceeds the window (by
+// more than the slack, which covers the rounding of the moved point) cannot lower the minimum; the
+// others search (min_d2). A query that needs more than PEN_STEPS nodes is finished by a whole wave
+// (k_pen_hard); what it met so far is a distance it attains, so it may still lower the window.
 __global__ void __launch_bounds__(VT) k_pen_min(VerifyBatch vb, int per_task) {
   TaskPos tp = task_pos(vb, per_task);
   int c = tp.c, side = tp.side, i = tp.i;
@@ -278,7 +340,7 @@ __global__ void __launch_bounds__(VT) k_pen_min(VerifyBatch vb, int per_task) {
   const PairDesc &pd = vb.pair[cd.pair];
   if (!pd.pen_ok) return;
   CandAcc &acc = vb.acc[c];
-  if (acc.inside[side] != 0) return;
+  if (!closest_needed(acc, side)) return;
   const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
   const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
   uint32_t best = 0xffffffffu;
@@ -287,13 +349,8 @@ __global__ void __launch_bounds__(VT) k_pen_min(VerifyBatch vb, int per_task) {
     const double *T = side == 0 ? cd.Ti : cd.T;
     const double *p = own.s + 3 * (size_t)i;
     V3 q = moved_f32(T, p[0], p[1], p[2]);
-    float window =
-        __uint_as_float(__hip_atomic_load(&acc.closest[side], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    // A sample whose distance bound already exceeds the window (by more than the slack, which
-    // covers the rounding of the moved point) cannot lower the minimum.
+    float window = closest_window(acc, side);
     if (dist_bound(other.mesh.db, q) <= window + slack_of(other.mesh.absmax, q) && window > 0.0f) {
-      // A query that needs more than PEN_STEPS nodes is finished by a whole wave (k_pen_hard);
-      // what it met so far is a distance it attains, so it may still lower the window.
       bool complete;
       float dist = sqrtf(min_d2(other.mesh, q, window, PEN_STEPS, &complete));
       if (dist <= window) best = __float_as_uint(dist);
@@ -360,72 +417,35 @@ __device__ K select_kth(const K *keys, int n, int k, uint32_t *hist, K *s_prefix
 }
 
 // The closest approach is a minimum over every sample, and a search is cheap once a small
-// window is known. Before the exact pass, one sample per workgroup walks once from the root to a
-// single leaf, always into the nearer child, and measures that one triangle: a distance the sample
-// really attains, so the smallest of them is at least the true minimum and makes a window the
-// exact pass can prune against. The minimum that comes out is the same; only the work changes.
-// (Every workgroup belongs to one task, so its early returns are uniform.)
-__global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb, int per_task) {
-  TaskPos tp = task_pos(vb, per_task);
-  int c = tp.c, side = tp.side, i = tp.i;
+// window is known. Before the exact pass, each task's sample with the smallest distance bound is
+// found (k_pen_lb), and a whole wave measures that sample's exact distance (k_pen_seed,
+// min_d2_wave): a distance the samples really attain, so at least the true minimum, and, the bound
+// being within a couple of grid cells of the distance, close to it. The minimum that comes out is
+// the same; only the work changes.
+__global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb) {
+  __shared__ int stk[VT / 64][HARD_STACK];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int t = (int)blockIdx.x * (VT / 64) + w;
+  if (t >= 2 * vb.n_cand) return;  // (uniform across the wave; the kernel has no block barrier)
+  const uint32_t cs = vb.task[t];
+  const int c = (int)(cs >> 1), side = (int)(cs & 1u);
   const CandDesc &cd = vb.cand[c];
   const PairDesc &pd = vb.pair[cd.pair];
   if (!pd.pen_ok) return;
   CandAcc &acc = vb.acc[c];
-  if (acc.inside[side] != 0) return;
+  if (!closest_needed(acc, side)) return;
+  const uint64_t key = acc.seed[side];
+  if (key == ~0ull) return;
   const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
   const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
-  uint32_t best = 0xffffffffu;
-  const SceneView &sv = other.mesh;
-  V3 q = V3{0.0f, 0.0f, 0.0f};
-  uint32_t lb = 0xffffffffu;
-  float window = 0.0f;
-  if (i < own.n_s) {
-    const double *T = side == 0 ? cd.Ti : cd.T;
-    const double *p = own.s + 3 * (size_t)i;
-    q = moved_f32(T, p[0], p[1], p[2]);
-    window = __uint_as_float(__hip_atomic_load(&acc.closest[side], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    lb = __float_as_uint(dist_bound(sv.db, q));
-  }
-  // One descent per workgroup, from the sample with the smallest distance bound (the likeliest to
-  // be close), and none when even that one cannot beat the best bound so far.
-  __shared__ uint32_t s_lb[VT / 64];
-  uint32_t wlb = wave_min(lb);
-  if ((threadIdx.x & 63) == 0) s_lb[threadIdx.x >> 6] = wlb;
-  __syncthreads();
-  uint32_t blb = s_lb[0];
-  int wsel = 0;
-#pragma unroll
-  for (int w = 1; w < VT / 64; ++w) {
-    if (s_lb[w] < blb) {
-      blb = s_lb[w];
-      wsel = w;
-    }
-  }
-  uint64_t holders = __ballot(lb == blb);
-  bool go = (int)(threadIdx.x >> 6) == wsel && lb != 0xffffffffu && lb == blb && lanes_below(holders) == 0 &&
-            __uint_as_float(lb) <= window;
-  if (go) {
-    int nodes = 2 * sv.n - 1;
-    int cur = sv.root;
-    Node nd = load_node(sv.nodes, cur);
-    for (int depth = 0; depth < nodes && nd.a > cur && nd.a < nodes; ++depth) {
-      int l = cur + 1, r = nd.a;
-      Node ln = load_node(sv.nodes, l), rn = load_node(sv.nodes, r);
-      bool go_left = box_d2(ln, q, 0.0f) <= box_d2(rn, q, 0.0f);
-      cur = go_left ? l : r;
-      nd = go_left ? ln : rn;
-    }
-    if (nd.a < 0) {
-      V3 a, b, cc;
-      uint32_t face;
-      load_tri(sv.tris, ~nd.a, &a, &b, &cc, &face);
-      float d = sqrtf(tri_d2(q, a, b, cc));
-      if (d - d == 0.0f) best = __float_as_uint(d);
-    }
-  }
-  best = wave_min(best);
-  if ((threadIdx.x & 63) == 0 && best != 0xffffffffu) atomicMin(&acc.closest[side], best);
+  const double *T = side == 0 ? cd.Ti : cd.T;
+  const double *p = own.s + 3 * (size_t)(uint32_t)key;
+  V3 q = moved_f32(T, p[0], p[1], p[2]);
+  float window = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(closest_window(acc, side))));
+  bool complete;
+  float dist = sqrtf(min_d2_wave(other.mesh, q, window, stk[w], HARD_STACK, WAVE_STEPS, &complete));
+  if (lane == 0 && dist <= window) atomicMin(&acc.closest[side], __float_as_uint(dist));
+  if (lane == 0 && !complete) atomicOr(&acc.flags, FLAG_GAVE_UP);
 }
 
 // The closest-approach queries k_pen_min left unfinished, one per wave (min_d2_wave), taken from
@@ -449,10 +469,13 @@ __global__ void __launch_bounds__(VT) k_pen_hard(VerifyBatch vb) {
     const double *T = side == 0 ? cd.Ti : cd.T;
     const double *p = own.s + 3 * (size_t)i;
     V3 q = moved_f32(T, p[0], p[1], p[2]);
-    float window = __uint_as_float(__builtin_amdgcn_readfirstlane(
-        __hip_atomic_load(&acc.closest[side], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-    float dist = sqrtf(min_d2_wave(other.mesh, q, window, stk[w], HARD_STACK));
+    float window = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(closest_window(acc, side))));
+    // The window may have shrunk since the query was queued: one beyond it is dropped unwalked.
+    if (!(dist_bound(other.mesh.db, q) <= window + slack_of(other.mesh.absmax, q))) continue;
+    bool complete;
+    float dist = sqrtf(min_d2_wave(other.mesh, q, window, stk[w], HARD_STACK, WAVE_STEPS, &complete));
     if (lane == 0 && dist <= window) atomicMin(&acc.closest[side], __float_as_uint(dist));
+    if (lane == 0 && !complete) atomicOr(&acc.flags, FLAG_GAVE_UP);
   }
 }
 
@@ -624,8 +647,11 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
     E(s3);
   }
   if (pen) {
+    B("k_pen_lb", s);
+    hipLaunchKernelGGL(k_pen_lb, dim3(2 * nc * per_s), dim3(VT), 0, s, vb, per_s);
+    E(s);
     B("k_pen_seed", s);
-    hipLaunchKernelGGL(k_pen_seed, dim3(2 * nc * per_s), dim3(VT), 0, s, vb, per_s);
+    hipLaunchKernelGGL(k_pen_seed, dim3(blocks_for(2L * nc, VT / 64)), dim3(VT), 0, s, vb);
     E(s);
     B("k_pen_min", s);
     hipLaunchKernelGGL(k_pen_min, dim3(2 * nc * per_s), dim3(VT), 0, s, vb, per_s);

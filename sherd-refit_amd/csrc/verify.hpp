@@ -16,6 +16,7 @@ enum {
 
 constexpr int MEDIAN_LDS = 12288;   // a median segment up to this many keys is selected from LDS
 constexpr uint32_t FLAG_VOXEL_RANGE = 1u;  // a seam voxel coordinate beyond +-2^20
+constexpr uint32_t FLAG_GAVE_UP = 2u;      // a closest-approach query ran out of its step budget
 
 // One fragment as R §6 reads it (verify.rs:223-255 `Surfaces`).
 struct FragDesc {
@@ -60,6 +61,7 @@ struct CandAcc {
   uint32_t flags;
   uint32_t pad;
   double med[4];
+  uint64_t seed[2];  // per side, (distance bound bits << 32) | sample of the smallest bound: the seed
 };
 
 struct VerifyBatch {

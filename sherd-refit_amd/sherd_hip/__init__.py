@@ -317,12 +317,42 @@ def face_centroids(V, F):
     return (V[F[:, 0]] + V[F[:, 1]] + V[F[:, 2]]) / 3.0
 
 
+# numpy views of the ABI's arrays of structures (the same layouts as the ctypes structures), so
+# that batches of hundreds of pairs and thousands of registrations cross without Python loops.
+_REG_DT = np.dtype([("transform", np.float64, 16), ("fitness", np.float64), ("inlier_rmse", np.float64),
+                    ("correspondences", np.uint64), ("iterations", np.uint32), ("converged", np.uint32)])
+_ICP_PAIR_DT = np.dtype([("source", np.uintp), ("target", np.uintp), ("distance", np.float64)])
+_VERIFY_PAIR_DT = np.dtype([("a", np.uintp), ("b", np.uintp), ("scales", np.float64, 12)])
+assert _REG_DT.itemsize == C.sizeof(Registration) and _ICP_PAIR_DT.itemsize == C.sizeof(IcpPair)
+assert _VERIFY_PAIR_DT.itemsize == C.sizeof(VerifyPair)
+
+
+def _addr(h):
+    """The address behind a handle object (its c_void_p), 0 for None."""
+    return 0 if h is None else (h.ptr.value or 0)
+
+
+def _icp_pairs(pairs):
+    """[(source, target, distance)] -> an IcpPair array (numpy, ABI layout)."""
+    arr = np.zeros(max(len(pairs), 1), _ICP_PAIR_DT)
+    if pairs:
+        arr["source"] = [_addr(s) for s, _, _ in pairs]
+        arr["target"] = [_addr(t) for _, t, _ in pairs]
+        arr["distance"] = [float(d) for _, _, d in pairs]
+    return arr
+
+
 def _registrations(out, k):
+    """A Registration array (ctypes) -> (k, 21): the transform, fitness, rmse, correspondences,
+    iterations, converged."""
+    a = np.frombuffer(out, dtype=_REG_DT, count=k)
     res = np.empty((k, 21))
-    for i in range(k):
-        r = out[i]
-        res[i, :16] = np.ctypeslib.as_array(r.transform)
-        res[i, 16:] = [r.fitness, r.inlier_rmse, r.correspondences, r.iterations, r.converged]
+    res[:, :16] = a["transform"]
+    res[:, 16] = a["fitness"]
+    res[:, 17] = a["inlier_rmse"]
+    res[:, 18] = a["correspondences"]
+    res[:, 19] = a["iterations"]
+    res[:, 20] = a["converged"]
     return res
 
 
@@ -337,15 +367,13 @@ class HuberBatch:
     def __init__(self, ex, pairs, cand_pair, T0):
         self.ex = ex
         self.clouds = [(s_, t_) for s_, t_, _ in pairs]
-        arr = (IcpPair * max(len(pairs), 1))()
-        for p, (src, tgt, dist) in enumerate(pairs):
-            arr[p] = IcpPair(src.ptr, tgt.ptr, float(dist))
+        arr = _icp_pairs(pairs)
         self.cand_pair = np.ascontiguousarray(cand_pair, dtype=np.uint32)
         T0 = np.ascontiguousarray(T0, dtype=np.float64).reshape(-1, 16)
         self.k = len(T0)
         self.points = int(sum(pairs[int(p)][0].n if pairs[int(p)][1].n else 0 for p in self.cand_pair))
         out = C.c_void_p()
-        _check(lib().sherd_huber_create(ex.ptr, arr, len(pairs), _p(self.cand_pair, C.c_uint32), _p(T0, C.c_double),
+        _check(lib().sherd_huber_create(ex.ptr, _p(arr, IcpPair), len(pairs), _p(self.cand_pair, C.c_uint32), _p(T0, C.c_double),
                                         self.k, C.byref(out)))
         self.ptr = out
         ex._own(self)
@@ -674,30 +702,26 @@ class HipExecutor:
         (all rungs the same length); rungs: [(estimation, max_iteration)]. Returns an
         (n_cand, n_rungs, 21) array in icp_rung's layout."""
         nr, npairs = len(rungs), len(pairs[0])
-        arr = (IcpPair * (nr * npairs))()
-        for r in range(nr):
-            assert len(pairs[r]) == npairs
-            for p, (src, tgt, dist) in enumerate(pairs[r]):
-                arr[r * npairs + p] = IcpPair(src.ptr, tgt.ptr, float(dist))
+        if any(len(pr) != npairs for pr in pairs):
+            raise HipError(-1, "every rung needs the same number of pairs")
+        arr = _icp_pairs([x for pr in pairs for x in pr])
         specs = (IcpRungSpec * nr)(*[IcpRungSpec(int(_EST.get(e, e)), int(it)) for e, it in rungs])
         cp = np.ascontiguousarray(cand_pair, dtype=np.uint32)
         inits = np.ascontiguousarray(inits, dtype=np.float64).reshape(-1, 16)
         k = len(inits)
         out = (Registration * max(k * nr, 1))()
-        _check(lib().sherd_icp_batch_ladder(self.ptr, arr, npairs, specs, nr, _p(cp, C.c_uint32),
+        _check(lib().sherd_icp_batch_ladder(self.ptr, _p(arr, IcpPair), npairs, specs, nr, _p(cp, C.c_uint32),
                                             _p(inits, C.c_double), k, out))
         return _registrations(out, k * nr).reshape(k, nr, 21)
 
     def icp_batch_rung(self, pairs, cand_pair, inits, estimation, max_iteration):
         """One rung over many pairs: pairs [(source, target, distance)] -> (n_cand, 21)."""
-        arr = (IcpPair * max(len(pairs), 1))()
-        for p, (src, tgt, dist) in enumerate(pairs):
-            arr[p] = IcpPair(src.ptr, tgt.ptr, float(dist))
+        arr = _icp_pairs(pairs)
         cp = np.ascontiguousarray(cand_pair, dtype=np.uint32)
         inits = np.ascontiguousarray(inits, dtype=np.float64).reshape(-1, 16)
         k = len(inits)
         out = (Registration * max(k, 1))()
-        _check(lib().sherd_icp_batch_rung(self.ptr, arr, len(pairs), _p(cp, C.c_uint32), _p(inits, C.c_double), k,
+        _check(lib().sherd_icp_batch_rung(self.ptr, _p(arr, IcpPair), len(pairs), _p(cp, C.c_uint32), _p(inits, C.c_double), k,
                                           int(_EST.get(estimation, estimation)), int(max_iteration), out))
         return _registrations(out, k)
 
@@ -707,18 +731,17 @@ class HipExecutor:
     def verify_batch(self, pairs, cand_pair, cand_T, full=True, retest=False):
         """verify::verify + accept over candidates of many pairs. pairs: [(frag_a, frag_b, scales12)].
         Returns (scores (n, 20), accepted (n,) bool)."""
-        pv = (VerifyPair * max(len(pairs), 1))()
-        for i, (a, b, sc) in enumerate(pairs):
-            pv[i].a = a.ptr
-            pv[i].b = b.ptr
-            for k in range(12):
-                pv[i].scales[k] = float(sc[k])
+        pv = np.zeros(max(len(pairs), 1), _VERIFY_PAIR_DT)
+        if pairs:
+            pv["a"] = [_addr(a) for a, _, _ in pairs]
+            pv["b"] = [_addr(b) for _, b, _ in pairs]
+            pv["scales"] = np.asarray([sc for _, _, sc in pairs], dtype=np.float64).reshape(len(pairs), 12)
         cp = np.ascontiguousarray(cand_pair, dtype=np.uint32)
         T = np.ascontiguousarray(cand_T, dtype=np.float64).reshape(-1, 16)
         n = len(T)
         scores = np.empty((n, 20), np.float64)
         acc = np.empty(n, np.uint8)
-        _check(lib().sherd_verify_batch(self.ptr, pv, len(pairs), _p(cp, C.c_uint32), _p(T, C.c_double), n,
+        _check(lib().sherd_verify_batch(self.ptr, _p(pv, VerifyPair), len(pairs), _p(cp, C.c_uint32), _p(T, C.c_double), n,
                                         int(full), int(retest), _p(scores, C.c_double), _p(acc, C.c_uint8)))
         return scores, acc.astype(bool)
 

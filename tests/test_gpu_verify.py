@@ -185,3 +185,46 @@ def test_small_vessel_all_pairs_bit_exact(ex):
     far = scores[:, 15] < -5.0
     say(f"  closest approach beyond 5 t on {int(far.sum())} of {len(cT)} candidates; accepted {int(accepted.sum())}")
     assert far.any() and accepted.any()
+
+
+def test_closest_approach_at_every_separation(ex):
+    """Two fragments of a 3 edges/t vessel (tens of thousands of triangles each) set apart at
+    0.05 t to 100 t under random rotations: nothing is inside, so R §6.4 reads the closest approach
+    (verify.rs:567-578), which the pair search over sample boxes and mesh nodes must find bit for bit
+    as the oracle's per-sample minimum does."""
+    from sherd_hip import synth
+
+    say("case: closest approach, 2 fragments, separations 0.05-100 t, bit-exact vs oracle")
+    col = synth.vessel(t=1.0, edges_per_t=3.0, n_frag=2, seed=21, height_t=40.0, rmin_t=15.0, rmax_t=20.0)
+    frags = col["fragments"]
+    scales = synth.scales_for_pair(col["t"], col["res"])
+    rng = np.random.default_rng(8)
+    truth = col["poses"][0] @ np.linalg.inv(col["poses"][1])
+    cand = []
+    for gap in (0.05, 0.3, 1.0, 3.0, 10.0, 30.0, 100.0):
+        for _ in range(3):
+            d = np.eye(4)
+            d[:3, :3] = synth._random_rotation(rng, np.pi)
+            v = rng.normal(size=3)
+            d[:3, 3] = v / np.linalg.norm(v) * (gap + 60.0 * (gap > 1.0))
+            cand.append(d @ truth)
+    cT = np.array(cand)
+    dev, ora, keep = [], [], []
+    for fr in frags:
+        fs, ms = ex.scene(fr["V"], fr["F"], fr["frac"]), ex.scene(fr["V"], fr["F"])
+        ofs, oms = O.Scene(fr["V"], fr["F"], fr["frac"]), O.Scene(fr["V"], fr["F"])
+        keep += [fs, ms, ofs, oms]
+        args = (fr["watertight"], fr["frac_area"], fr["Pf"], fr["S"], fr["brk_P"], fr["brk_ns"], fr["margin_p"],
+                fr["margin_n"])
+        dev.append(ex.fragment(fs, ms, *args))
+        ora.append(O.SideData(ofs, oms, *args))
+    cp = np.zeros(len(cT), np.uint32)
+    for retest in (False, True):
+        scores, _ = ex.verify_batch([(dev[0], dev[1], scales)], cp, cT, retest=retest)
+        for i in range(len(cT)):
+            want = O.verify(ora[0], ora[1], cT[i], scales, retest=retest)
+            assert np.array_equal(scores[i].view(np.uint64), want.view(np.uint64)), (i, retest, scores[i] - want)
+    apart = scores[:, 15] < 0.0
+    say(f"  {int(apart.sum())} of {len(cT)} candidates with nothing inside; closest approach "
+        f"{-scores[apart, 15].min():.3f}-{-scores[apart, 15].max():.3f} t")
+    assert apart.sum() >= len(cT) // 2
