@@ -50,36 +50,26 @@ def log(*a):
 
 
 def algorithmic_bytes(kernel, frags, pairs, cand_pair):
-    """Compulsory bytes of one launch of a verification kernel over the batch: every input element
-    read once, every output written once, every resident structure used counted once."""
-    nodes = lambda f: 32 * (2 * f - 1)
-    used = {}
+    """Bytes of one launch of a verification kernel over the batch in its per-point unit: every
+    point of a (candidate, side) task read once (f64 samples, 24 B) and every per-point output
+    written once. The meshes and fracture scenes the points are tested against are not counted: a
+    launch reads only the parts near its points, and the bytes it does move are the measured
+    traffic reported beside it. A margin cloud an R §6.3 task searches is counted once per launch."""
+    used = set()
     total = 0
     for c in cand_pair:
         a, b = pairs[c]
         fa, fb = frags[a], frags[b]
-        if kernel in ("k_pen", "k_pen_seed", "k_pen_min"):
+        if kernel in ("k_pen", "k_pen_lb", "k_pen_min"):
             total += 24 * (len(fa["S"]) + len(fb["S"]))
-            used[("mesh", a)] = used[("mesh", b)] = 0
         elif kernel == "k_frac":
-            total += 24 * (len(fa["Pf"]) + len(fb["Pf"])) + 8 * (len(fa["Pf"]) + len(fb["Pf"]))
-            used[("frac", a)] = used[("frac", b)] = 0
+            total += (24 + 8) * (len(fa["Pf"]) + len(fb["Pf"]))  # the point, its distance key
         elif kernel == "k_seam":
             total += 48 * (len(fa["brk_P"]) + len(fb["brk_P"]))
         elif kernel == "k_cont":
-            total += 48 * len(fb["margin_p"]) + 16 * len(fb["margin_p"])
-            used[("margin", a)] = 0
-    for (kind, i) in used:
-        fr = frags[i]
-        if kind == "mesh":
-            n = len(fr["F"])
-            total += nodes(n) + 48 * n
-        elif kind == "frac":
-            n = int(fr["frac"].sum())
-            total += nodes(n) + 48 * n
-        else:
-            total += 48 * len(fr["margin_p"])
-    return total
+            total += (48 + 16) * len(fb["margin_p"])  # point and normal, two keys
+            used.add(a)
+    return total + sum(48 * len(frags[i]["margin_p"]) for i in used)
 
 
 def cpu_baseline(col, pairs, scales, cand_pair, cand_T, limit_s=20.0):
@@ -631,19 +621,6 @@ def pose_gap_deg_t(a, b):
     return np.degrees(ang), np.linalg.norm(a[:, :3, 3] - b[:, :3, 3], axis=1)
 
 
-def cloud_tree_bytes(n):
-    """A point cloud's resident search structure: its LBVH nodes and one 48-byte leaf record per point."""
-    return 32 * (2 * n - 1) + 48 * n
-
-
-def icp_launch_bytes(srcs, tgts, cp, iters):
-    """Compulsory bytes of one batched ICP rung (point-to-point): per candidate-iteration and source
-    point, the point (24 B) and its matched target point (24 B), over the iterations plus the final
-    correspondence pass; every target cloud's tree once."""
-    per = sum(48 * len(srcs[int(p)]) * (int(k) + 1) for p, k in zip(cp, iters))
-    return per + sum(cloud_tree_bytes(len(tgts[int(p)])) for p in sorted(set(int(x) for x in cp)))
-
-
 def host_info():
     """This host's CPU model (lscpu's "Model name") and core count (nproc)."""
     import subprocess
@@ -742,13 +719,17 @@ def collection_headline(ex, args):
     log(f"collection batched: {args.steps} steps in {el:.3f}s ({1e3 * el / args.steps:.2f} ms/step; icp "
         f"{1e3 * parts[0] / args.steps:.2f}, lm {1e3 * parts[1] / args.steps:.2f}, verify "
         f"{1e3 * parts[2] / args.steps:.2f} ms)")
-    # The plain schedule over its own timed steps, after one untimed pass.
-    CL.run_plain(dev, coll, pairs, cp, cT, scales)
-    t0 = time.perf_counter()
-    for _ in range(args.plain_steps):
-        p = CL.run_plain(dev, coll, pairs, cp, cT, scales)
-    el_plain = time.perf_counter() - t0
-    plain = n_cand * args.plain_steps / el_plain
+    # The plain schedule over its own timed steps, after one untimed pass (none with --plain-steps 0,
+    # for counter collection).
+    if args.plain_steps <= 0:
+        p, el_plain, plain = b, float("nan"), float("nan")
+    else:
+        CL.run_plain(dev, coll, pairs, cp, cT, scales)
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            p = CL.run_plain(dev, coll, pairs, cp, cT, scales)
+        el_plain = time.perf_counter() - t0
+        plain = n_cand * args.plain_steps / el_plain
     same = dict(icp=bool(np.array_equal(b["reg"], p["reg"])), lm=bool(np.array_equal(b["T_lm"], p["T_lm"])),
                 scores=bool(np.array_equal(b["scores"].view(np.uint64), p["scores"].view(np.uint64))),
                 accepted=bool(np.array_equal(b["accepted"], p["accepted"])))
@@ -757,7 +738,7 @@ def collection_headline(ex, args):
     adj = np.array([tuple(pr) in coll["adjacent"] for pr in pairs])[cp]
     res = dict(
         value=value, ms_per_step=1e3 * el / args.steps, candidates=n_cand, prof=prof, coll=coll, pairs=pairs, cp=cp,
-        reg=b["reg"],
+        reg=b["reg"], scores=b["scores"],
         section=dict(
             fragments=len(coll["fragments"]), triangles=n_tri, pairs=len(pairs), candidates=int(n_cand),
             non_watertight=list(CL.OPEN_FRAGMENTS), generation_seconds=round(gen_s, 2),
@@ -767,11 +748,14 @@ def collection_headline(ex, args):
             step_ms=round(1e3 * el / args.steps, 4),
             step_icp_lm_verify_ms=[round(1e3 * x / args.steps, 4) for x in parts],
             end_to_end_with_setup_seconds=round(setup_s + el / args.steps, 4),
-            plain_steps=args.plain_steps, plain_step_seconds=round(el_plain / args.plain_steps, 4),
+            plain_steps=args.plain_steps,
+            plain_step_seconds=round(el_plain / args.plain_steps, 4) if args.plain_steps > 0 else None,
             plain_icp_lm_verify_s=[round(p["icp_s"], 4), round(p["lm_s"], 4), round(p["verify_s"], 4)],
-            plain_pair_verifications_per_s=round(plain, 3), batched_vs_plain=round(value / plain, 3),
-            batched_vs_plain_with_setup=round((el_plain / args.plain_steps + setup_s) / (el / args.steps + setup_s), 3),
-            target_3x_met=bool(value / plain >= 3.0), identical_plain_batched=same,
+            plain_pair_verifications_per_s=round(plain, 3) if args.plain_steps > 0 else None,
+            batched_vs_plain=round(value / plain, 3) if args.plain_steps > 0 else None,
+            batched_vs_plain_with_setup=(round((el_plain / args.plain_steps + setup_s) / (el / args.steps + setup_s), 3)
+                                         if args.plain_steps > 0 else None),
+            target_3x_met=bool(value / plain >= 3.0) if args.plain_steps > 0 else None, identical_plain_batched=same,
             accepted=int(b["accepted"].sum()), accepted_on_adjacent_pairs=int((b["accepted"] & adj).sum()),
             adjacent_candidates=int(adj.sum()), pen_unavailable=int(b["scores"][:, 16].sum())))
     cpu = None
@@ -928,38 +912,37 @@ def kernel_roofline(kernel, nbytes, avg_ms, pmc_file):
     return out
 
 
-# Kernel records that time a sequence of launches, not one kernel (no roofline of their own).
-SEQUENCES = ("k_huber_lm", "k_huber_grad", "k_huber_energy")
+# Main-stream kernels of the collection step and the §8(d) unit each is counted in: bytes per point
+# processed (the f64 sample read, 24 B; their results are reduced in the kernel), over every
+# (candidate, side) task (k_pen) or over the tasks whose closest approach is needed (both sides
+# without a sample inside: k_pen_lb, k_pen_min). The meshes the points are tested against are not
+# counted: a launch reads only the parts near its samples (92 % of samples fail the box reject), and
+# the bytes it does read are the measured traffic beside it.
+POINT_KERNELS = ("k_pen", "k_pen_lb", "k_pen_min")
 
 
 def collection_roofline(head):
     """The roofline of the collection step's dominant kernel: the one with the most device time
-    among those that run alone on the library's stream (the verification sections on side streams
-    and the records of launch sequences are left out)."""
-    prof, coll, pairs, cp, reg = head["prof"], head["coll"], head["pairs"], head["cp"], head["reg"]
+    among those that run alone on the library's stream and have a per-point unit (the verification
+    sections on side streams, the queue of long queries and the records of launch sequences are
+    listed beside it, not chosen)."""
+    prof, coll, pairs, cp, scores = head["prof"], head["coll"], head["pairs"], head["cp"], head["scores"]
     frags = coll["fragments"]
-    known = {"k_icp_batch", ROOFLINE_KERNEL}
-    cands = {k: v for k, v in prof.items() if k not in OVERLAPPED_KERNELS and k not in SEQUENCES and v["launches"]}
+    n_s = np.array([len(frags[a]["S"]) + len(frags[b]["S"]) for a, b in pairs])[cp]
+    pen_ok = scores[:, 16] == 0.0
+    apart = pen_ok & (scores[:, 15] < 0.0)  # pen_depth < 0: no sample inside on either side
+    points = {"k_pen": int(n_s[pen_ok].sum()), "k_pen_lb": int(n_s[apart].sum()),
+              "k_pen_min": int(n_s[apart].sum())}
+    cands = {k: v for k, v in prof.items() if k in POINT_KERNELS and v["launches"]}
     dom = max(cands, key=lambda k: cands[k]["ms"])
-    if dom not in known:
-        dom = max((k for k in cands if k in known), key=lambda k: cands[k]["ms"])
     avg_ms = prof[dom]["ms"] / prof[dom]["launches"]
-    if dom == "k_icp_batch":
-        from sherd_hip import collection as CL
-
-        srcs = [frags[b]["brk_P"] for a, b in pairs]
-        tgt_of = [a for a, b in pairs]
-        rung_bytes = []
-        for r in range(len(CL.RUNGS)):
-            iters = reg[:, r, 19]
-            per = int(sum(48 * len(srcs[int(p)]) * (int(k) + 1) for p, k in zip(cp, iters)))
-            used = sorted(set(tgt_of[int(p)] for p in cp))
-            rung_bytes.append(per + sum(cloud_tree_bytes(len(frags[a]["brk_P"])) for a in used))
-        nbytes = int(np.mean(rung_bytes))
-    else:
-        nbytes = algorithmic_bytes(dom, frags, pairs, cp)
+    nbytes = 24 * points[dom]
     roof = kernel_roofline(dom, nbytes, avg_ms, "collection_pmc.json")
-    roof["share_of_step_device_ms"] = round(prof[dom]["ms"] / sum(v["ms"] for v in cands.values()), 4)
+    roof["points_per_launch"] = points[dom]
+    roof["unit_bytes"] = "24 B per point processed (the f64 sample), meshes excluded (read only near the samples)"
+    main = [k for k in prof if k not in OVERLAPPED_KERNELS]
+    roof["share_of_main_stream_ms"] = round(prof[dom]["ms"] / sum(prof[k]["ms"] for k in main), 4)
+    roof["others_ms_per_launch"] = {k: round(v["ms"] / v["launches"], 4) for k, v in prof.items() if k != dom}
     return roof
 
 
@@ -985,6 +968,11 @@ def main():
     ap.add_argument("--no-segment", action="store_true", help="skip R §3.4.4-3.4.8 over the vessel's fragments")
     ap.add_argument("--collection-only", action="store_true",
                     help="run only the collection's steps (for counter collection); prints no bench line")
+    ap.add_argument("--no-collection", action="store_true",
+                    help="skip the collection (the headline) and run the sections only (for kernel traces); "
+                         "prints the sections as JSON, not a bench line")
+    ap.add_argument("--vessel-only", action="store_true",
+                    help="run only configuration 2's verification steps (for counter collection); prints no bench line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -996,8 +984,27 @@ def main():
 
     ex = sherd_hip.HipExecutor(0)
     log("device:", ex.device_name())
-    if args.collection_only:
+    if args.collection_only or args.vessel_only:
         args.no_cpu = True
+    if args.vessel_only:
+        args.plain_candidates = 1
+        vessel, _ = vessel_section(ex, synth, args)
+        for k in sorted(vessel["kernel_ms"], key=lambda k: -vessel["kernel_ms"][k]["ms"]):
+            v = vessel["kernel_ms"][k]
+            log(f"  {k}: {v['ms'] / v['launches']:.4f} ms/launch over {v['launches']} launches")
+        ex.close()
+        return
+    if args.no_collection:
+        vessel, col = (None, None) if args.no_vessel else vessel_section(ex, synth, args)
+        out = dict(vessel=vessel, icp=icp_ladder(ex, synth, args) if args.icp_pairs > 0 else None,
+                   huber=huber_section(ex, synth, col, args) if not args.no_huber and col is not None else None,
+                   refine=refine_section(ex, col, args) if not args.no_refine and col is not None else None,
+                   match=match_section(ex, col, args) if not args.no_match and col is not None else None,
+                   thickness=thickness_section(ex, col, args) if not args.no_thickness and col is not None else None,
+                   segment=segment_section(ex, col, args) if not args.no_segment and col is not None else None)
+        print(json.dumps(dict(sections=out)), flush=True)
+        ex.close()
+        return
     head = collection_headline(ex, args)
     roof = collection_roofline(head)
     log(f"collection roofline: {roof['kernel']} {roof['avg_launch_ms']} ms/launch, {roof['achieved']} GB/s "
