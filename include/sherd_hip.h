@@ -403,6 +403,17 @@ int sherd_cone_votes(sherd_ctx *ctx, const sherd_scene *scene, const double *V, 
                      const uint32_t *F, size_t nf, const int64_t *faces, const double *ns,
                      size_t n_faces, double thick, int retest, uint8_t *good);
 
+/* The per-vertex form of the preprocessing rays (the benchmark's slab plate casts one ray per vertex
+ * along the inward normal; the reference's own form is the cone above, segment.rs:318-381, whose
+ * first ray and test this one applies per vertex): vertex v's normal is Open3D's area-weighted
+ * vertex normal of V (nv x 3 f64) and F (refine.rs:118-150), its ray starts 1e-3 inside along it
+ * (narrowed to f32) and goes straight down; good[v] = 1 when the first hit lies in the wall window
+ * (0.1 t < dh, 0.5 < dh / t < 1.8 in f32) on a face that looks back along the ray (h . d > 0.7 in
+ * f64). frac (nf bytes, may be NULL): 1 for a face fewer than two of whose vertices voted. scene is
+ * the mesh's own LBVH (V narrowed to f32, no mask). */
+int sherd_vertex_votes(sherd_ctx *ctx, const sherd_scene *scene, const double *V, size_t nv, const uint32_t *F,
+                       size_t nf, double thick, int retest, uint8_t *good, uint8_t *frac);
+
 /* ---- R §3.2, the wall thickness ---------------------------------------------------------------- */
 
 /* thickness::estimate_thickness (crates/sherd-core/src/fragment/thickness.rs:100-110, with

@@ -17,6 +17,10 @@ it calls, in numpy and Python integers:
                      flipped when its area is below the bound
   grow               segment.rs:438-480: the growth passes, candidates read before any flip
   cleanup            segment.rs:544-635's steps 3.4.4 to 3.4.8 in order
+  vertex_votes       the per-vertex form of R §3.4.3's rays (one ray per vertex along the inward
+                     vertex normal, refine.rs:118-150's normals), with the test of the cone's first
+                     ray (segment.rs:330-372), and a face fracture when fewer than two of its
+                     vertices voted
 
 One deliberate difference, shared with the device: a component's area is the exact sum of its faces'
 areas, each first truncated to a multiple of 2^(E - 64) where 2^E is the least power of two above
@@ -227,3 +231,36 @@ def cleanup(V, F, t, frac_raw, rep, near, angle_deg=BOUNDARY_ANGLE, exact=True):
 def agreement(a, b, areas):
     """Area-weighted fraction of faces whose two labels agree (segment.rs:655)."""
     return float(np.sum(areas[np.asarray(a) == np.asarray(b)]) / np.sum(areas))
+
+
+def vertex_votes(scene, V, F, t, retest=False):
+    """One ray per vertex from 1e-3 inside along its normal (refine.rs:118-150's area-weighted vertex
+    normal), straight down, narrowed to f32 once; good when the first hit (scene: the oracle's
+    scene of V narrowed to f32) lies in the wall window and its face looks back along the ray
+    (segment.rs:357-372: prim valid, dh > f32(0.1 t), 0.5 < dh / f32(t) < 1.8 in f32, h . d > 0.7 in
+    f64). Returns (good per vertex as u8, fracture label per face as u8)."""
+    import refine as R
+
+    V = np.asarray(V, dtype=np.float64)
+    F = np.asarray(F, dtype=np.int64).reshape(-1, 3)
+    n = R.vertex_normals(V, F)
+    ok = np.isfinite(n).all(axis=1) & (n != 0.0).any(axis=1)
+    origin = (V - n * 1e-3).astype(np.float32)
+    d = -n
+    hit_t, prim = scene.first_hit(origin, d.astype(np.float32), retest=retest)
+    hit_t = np.asarray(hit_t, dtype=np.float32)
+    prim = np.asarray(prim, dtype=np.int64)
+    min_hit, th = np.float32(0.1 * t), np.float32(t)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ratio = hit_t / th
+    valid = ok & (prim != 0xFFFFFFFF) & np.isfinite(hit_t) & (prim < len(F)) & (hit_t > min_hit)
+    valid &= (ratio > np.float32(0.5)) & (ratio < np.float32(1.8))
+    good = np.zeros(len(V), np.uint8)
+    idx = np.nonzero(valid)[0]
+    if len(idx):
+        normals, _, _ = face_geometry(V, F)
+        h = normals[prim[idx]]
+        al = (h[:, 0] * d[idx, 0] + h[:, 1] * d[idx, 1]) + h[:, 2] * d[idx, 2]
+        good[idx[al > 0.7]] = 1
+    frac = ((good[F[:, 0]].astype(int) + good[F[:, 1]] + good[F[:, 2]]) < 2).astype(np.uint8)
+    return good, frac

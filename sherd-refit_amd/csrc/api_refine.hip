@@ -20,7 +20,8 @@ using namespace sherd;
 namespace {
 
 // Scratch slots of this file.
-enum { SL_SEL = 30, SL_POS = 31, SL_FN = 32, SL_NSCR = 33, SL_IDX = 34 };
+enum { SL_SEL = 30, SL_POS = 31, SL_FN = 32, SL_NSCR = 33, SL_IDX = 34, SL_VPTS = 36, SL_VNRM = 37, SL_VGOOD = 38,
+       SL_VFRAC = 39 };
 
 constexpr int MAX_RUNG_POINTS = ICP_SPAN * ICP_MAX_SPANS;
 
@@ -309,6 +310,53 @@ int sherd_cloud_from_indices(sherd_ctx *ctx, const double *V, size_t nv, const u
   hipError_t e = m.upload(c, V, nv, F, nf);
   if (e != hipSuccess) return hip_fail(e, "upload");
   return vertex_cloud(c, m, nv, nf, std::vector<uint32_t>(idx, idx + n), out);
+}
+
+int sherd_vertex_votes(sherd_ctx *ctx, const sherd_scene *scene, const double *V, size_t nv, const uint32_t *F,
+                       size_t nf, double thick, int retest, uint8_t *good, uint8_t *frac) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  LOCK(c);
+  if (!scene || scene->ctx != c) return fail(SHERD_E_ARG, "scene of another context");
+  int rc = check_mesh(V, nv, F, nf);
+  if (rc) return rc;
+  if (!good && nv) return fail(SHERD_E_ARG, "null output");
+  if (nv > (size_t)INT_MAX / 3 || nf > (size_t)INT_MAX / 3) return fail(SHERD_E_LIMIT, "mesh too large");
+  if (nv == 0) return SHERD_OK;
+  if (!(thick > 0.0) || !std::isfinite(thick)) return fail(SHERD_E_ARG, "thickness must be positive");
+  DevMesh m;
+  hipError_t e = m.upload(c, V, nv, F, nf);
+  if (e != hipSuccess) return hip_fail(e, "upload");
+  uint32_t *didx = (uint32_t *)c->scratch(SL_IDX, sizeof(uint32_t) * nv, &e);
+  if (!didx) return hip_fail(e, "scratch");
+  double *fn = (double *)c->scratch(SL_FN, sizeof(double) * 3 * std::max<size_t>(nf, 1), &e);
+  if (!fn) return hip_fail(e, "scratch");
+  uint32_t *ns = (uint32_t *)c->scratch(SL_NSCR, sizeof(uint32_t) * normals_scratch_words((int)nv, (int)nf), &e);
+  if (!ns) return hip_fail(e, "scratch");
+  double *pts = (double *)c->scratch(SL_VPTS, sizeof(double) * 3 * nv, &e);
+  if (!pts) return hip_fail(e, "scratch");
+  double *nrm = (double *)c->scratch(SL_VNRM, sizeof(double) * 3 * nv, &e);
+  if (!nrm) return hip_fail(e, "scratch");
+  uint8_t *dgood = (uint8_t *)c->scratch(SL_VGOOD, nv, &e);
+  if (!dgood) return hip_fail(e, "scratch");
+  uint8_t *dfrac = nullptr;
+  if (frac && nf) {
+    dfrac = (uint8_t *)c->scratch(SL_VFRAC, nf, &e);
+    if (!dfrac) return hip_fail(e, "scratch");
+  }
+  Timer tm(c, SHERD_M_SEGMENT, nv);
+  HIPCK(launch_iota(didx, (int)nv, c->stream), "iota");
+  HIPCK(launch_vertex_cloud(m.V, (int)nv, m.F, (int)nf, didx, (int)nv, fn, ns, pts, nrm, c->stream), "vertex normals");
+  if (c->prof.on) c->prof.begin("k_vertex_votes", c->stream);
+  HIPCK(launch_vertex_votes(view_of(scene->d), m.V, (int)nv, m.F, (int)nf, nrm, thick, retest, dgood, dfrac,
+                            c->stream),
+        "vertex rays");
+  if (c->prof.on) c->prof.end(c->stream);
+  tm.stop();
+  HIPCK(hipMemcpyAsync(good, dgood, nv, hipMemcpyDeviceToHost, c->stream), "download");
+  if (dfrac) HIPCK(hipMemcpyAsync(frac, dfrac, nf, hipMemcpyDeviceToHost, c->stream), "download");
+  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  c->prof.collect();
+  return SHERD_OK;
 }
 
 size_t sherd_cloud_size(const sherd_cloud *cloud) { return cloud ? (size_t)cloud->n : 0; }

@@ -6,6 +6,9 @@
 // a 15 degree cone), each a first hit through the LBVH; a hit votes when 0.1 t < dh, 0.5 < dh/t < 1.8
 // in f32, and the hit face looks back along the ray (h . d > 0.7 in f64). The cone's sines and
 // cosines are computed once on the host and passed in, so every lane uses the same doubles.
+//
+// Beside it, the per-vertex form the benchmark's slab plate asks for: one ray per vertex along the
+// inward vertex normal, with the cone's first ray's test, and a face label from its vertices.
 #include <hip/hip_runtime.h>
 
 #include "bvh_query.hpp"
@@ -67,7 +70,68 @@ __global__ void __launch_bounds__(256) k_cone(SceneView sv, const double *V, con
   good[q] = votes;
 }
 
+// The per-vertex form of the same test (one ray per vertex along the inward normal): the vertex
+// normal n (Open3D's area-weighted normal, refine.rs:118-150), the origin pushed 1e-3 along n and
+// narrowed to f32, the ray straight down -n, and the cone's first ray's test on its first hit.
+__global__ void __launch_bounds__(256) k_vertex_votes(SceneView sv, const double *V, int nv, const uint32_t *F,
+                                                      int nf, const double *nrm, float min_hit, float thick,
+                                                      int retest, uint8_t *good) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nv) return;
+  const double *p = V + 3 * (size_t)q, *n = nrm + 3 * (size_t)q;
+  double n0 = n[0], n1 = n[1], n2 = n[2];
+  uint8_t g = 0;
+  if (n0 - n0 == 0.0 && n1 - n1 == 0.0 && n2 - n2 == 0.0 && (n0 != 0.0 || n1 != 0.0 || n2 != 0.0)) {
+    V3 o = V3{(float)(p[0] - n0 * 1e-3), (float)(p[1] - n1 * 1e-3), (float)(p[2] - n2 * 1e-3)};
+    double d0 = -n0, d1 = -n1, d2 = -n2;
+    V3 dir = V3{(float)d0, (float)d1, (float)d2};
+    float t;
+    uint32_t prim;
+    first_hit(sv, o, dir, retest, &t, &prim);
+    if (prim != 0xFFFFFFFFu && t - t == 0.0f && prim < (uint32_t)nf && t > min_hit) {
+      float ratio = t / thick;
+      if (ratio > 0.5f && ratio < 1.8f) {
+        double h[3];
+        face_normal(V, F, prim, h);
+        g = ((h[0] * d0 + h[1] * d1) + h[2] * d2) > 0.7 ? 1 : 0;
+      }
+    }
+  }
+  good[q] = g;
+}
+
+// A face is fracture when fewer than two of its vertices voted shell.
+__global__ void k_vertex_labels(const uint32_t *F, int nf, const uint8_t *good, uint8_t *frac) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  int s = (int)good[F[3 * f]] + (int)good[F[3 * f + 1]] + (int)good[F[3 * f + 2]];
+  frac[f] = s < 2 ? 1 : 0;
+}
+
+__global__ void k_iota(uint32_t *x, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = (uint32_t)i;
+}
+
 }  // namespace
+
+hipError_t launch_vertex_votes(const SceneView &sv, const double *V, int nv, const uint32_t *F, int nf,
+                               const double *nrm, double thick, int retest, uint8_t *good, uint8_t *frac,
+                               hipStream_t s) {
+  if (nv <= 0) return hipSuccess;
+  // segment.rs:337-338: the hit floor and the wall in f32.
+  float min_hit = (float)(0.1 * thick), th = (float)thick;
+  hipLaunchKernelGGL(k_vertex_votes, dim3((nv + 255) / 256), dim3(256), 0, s, sv, V, nv, F, nf, nrm, min_hit, th,
+                     retest, good);
+  if (frac && nf > 0) hipLaunchKernelGGL(k_vertex_labels, dim3((nf + 255) / 256), dim3(256), 0, s, F, nf, good, frac);
+  return hipGetLastError();
+}
+
+hipError_t launch_iota(uint32_t *x, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, x, n);
+  return hipGetLastError();
+}
 
 hipError_t launch_cone(const SceneView &sv, const double *V, const uint32_t *F, int nf, const int64_t *faces,
                        const double *ns, int n, const ConeConst &cc, int retest, uint8_t *good, hipStream_t s) {

@@ -258,6 +258,12 @@ struct ConeConst {
   float thick;          // t as f32
 };
 
+// The per-vertex rays (cone.hip): good[v] = 1 when vertex v's inward ray meets the opposite wall;
+// frac (may be null) per face, 1 when fewer than two of its vertices did.
+hipError_t launch_vertex_votes(const SceneView &sv, const double *V, int nv, const uint32_t *F, int nf,
+                               const double *nrm, double thick, int retest, uint8_t *good, uint8_t *frac,
+                               hipStream_t s);
+hipError_t launch_iota(uint32_t *x, int n, hipStream_t s);
 hipError_t launch_cone(const SceneView &sv, const double *V, const uint32_t *F, int nf, const int64_t *faces,
                        const double *ns, int n, const ConeConst &cc, int retest, uint8_t *good, hipStream_t s);
 

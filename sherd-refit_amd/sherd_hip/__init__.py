@@ -128,6 +128,7 @@ def _bind(L):
         "sherd_first_hit": (ci, [vp, vp, P(f), P(f), sz, ci, P(f), P(u32)]),
         "sherd_bounded_distance": (ci, [vp, vp, P(d), sz, P(d), d, ci, P(d)]),
         "sherd_bounded_distance_noalloc": (ci, [vp, vp, P(d), sz, P(d), d, ci, P(d)]),
+        "sherd_vertex_votes": (ci, [vp, vp, P(d), sz, P(u32), sz, d, ci, P(u8), P(u8)]),
         "sherd_inside": (ci, [vp, vp, P(d), sz, P(d), ci, P(InsideOutcome)]),
         "sherd_nearest_below": (ci, [vp, P(d), sz, P(d), sz, d, P(i64), P(d)]),
         "sherd_coarse_scores": (ci, [vp, P(d), P(d), sz, P(d), P(d), sz, P(d), sz, d, d, P(u32), P(d)]),
@@ -744,6 +745,16 @@ class HipExecutor:
         _check(lib().sherd_verify_batch(self.ptr, _p(pv, VerifyPair), len(pairs), _p(cp, C.c_uint32), _p(T, C.c_double), n,
                                         int(full), int(retest), _p(scores, C.c_double), _p(acc, C.c_uint8)))
         return scores, acc.astype(bool)
+
+    def vertex_votes(self, scene, V64, F, thick, retest=False):
+        """One inward ray per vertex (sherd_vertex_votes): (good per vertex, fracture label per face)."""
+        V64 = f64(V64)
+        F = np.ascontiguousarray(F, dtype=np.uint32).reshape(-1, 3)
+        good = np.empty(len(V64), np.uint8)
+        frac = np.empty(len(F), np.uint8)
+        _check(lib().sherd_vertex_votes(self.ptr, scene.ptr, _p(V64, C.c_double), len(V64), _p(F, C.c_uint32), len(F),
+                                        float(thick), int(retest), _p(good, C.c_uint8), _p(frac, C.c_uint8)))
+        return good, frac
 
     def thickness(self, scene, V64, F):
         """thickness::estimate_thickness (thickness.rs:100-110): (t, thick_mode) as float32, or None

@@ -250,6 +250,86 @@ def _fragment_data(v, f, frac, t, rng):
                 margin_p=margin_p, margin_n=fn[margin_face], face_normals=fn)
 
 
+def _perlin1(x, seed):
+    """Seeded 1-D gradient (Perlin) noise in about [-1, 1]: a random slope at every integer, the
+    two neighbouring ramps blended by the quintic fade."""
+    i0 = np.floor(x).astype(np.int64)
+    f = x - i0
+    lo, hi = int(i0.min()), int(i0.max()) + 1
+    g = np.random.default_rng(seed).uniform(-1.0, 1.0, size=hi - lo + 1)
+    a = g[i0 - lo] * f
+    b = g[i0 + 1 - lo] * (f - 1.0)
+    w = f * f * f * (f * (6.0 * f - 15.0) + 10.0)
+    return 2.0 * (a + w * (b - a))
+
+
+def _quads(G):
+    """Two triangles per cell of an index grid G (m+1, n+1), wound so that the normal is
+    d/d(first index) x d/d(second index)."""
+    a, b, c, d = G[:-1, :-1], G[1:, :-1], G[1:, 1:], G[:-1, 1:]
+    return np.concatenate([np.stack([a, b, c], -1).reshape(-1, 3), np.stack([a, c, d], -1).reshape(-1, 3)])
+
+
+def slab_plate(t=1.0, edge_t=0.06, length_t=60.0, width_t=40.0, amp_t=0.3, wavelength_t=2.0, jitter_t=0.01,
+               seed=0, jitter_seed=1, pose_seed=2, max_deg=30.0, max_shift_t=5.0):
+    """a plate length_t x width_t x t (t thick) split in two across its
+    width by a z-extruded polyline x = length/2 + amp * perlin(y / wavelength) (seed), every vertex
+    jittered by N(0, jitter) (jitter_seed; the shared fracture lattice identically on both sides),
+    meshed closed at edge_t per edge (both pieces on one fracture lattice, so the fracture surfaces
+    are exactly complementary), fragment B moved by a seeded rigid transform (axis uniform on the
+    sphere, angle uniform in (0, max_deg], translation uniform in a ball of max_shift_t; pose_seed).
+    Returns the collection dict of `vessel` (fragments with V, F, frac, samples, frames; poses;
+    adjacent; t; res)."""
+    L, W = length_t * t, width_t * t
+    nu = max(4, int(round(0.5 * L / (edge_t * t))))
+    nv = max(4, int(round(W / (edge_t * t))))
+    nw = max(2, int(round(t / (edge_t * t))))
+    vv = np.linspace(0.0, 1.0, nv + 1)
+    ww = np.linspace(0.0, 1.0, nw + 1)
+    y = -0.5 * W + W * vv
+    z = -0.5 * t + t * ww
+    xc = 0.5 * L + amp_t * t * _perlin1(y / (wavelength_t * t), seed)
+    jrng = np.random.default_rng(jitter_seed)
+    cut_jitter = jrng.normal(scale=jitter_t * t, size=(nv + 1, nw + 1, 3))
+    frags = []
+    for side in (0, 1):
+        uu = np.linspace(0.0, 1.0, nu + 1)
+        U, Vg, Wg = np.meshgrid(uu, np.arange(nv + 1), np.arange(nw + 1), indexing="ij")
+        X = uu[:, None, None] * xc[None, :, None] if side == 0 else xc[None, :, None] + uu[:, None, None] * (L - xc[None, :, None])
+        X = np.broadcast_to(X, U.shape)
+        P = np.stack([X, np.broadcast_to(y[None, :, None], U.shape), np.broadcast_to(z[None, None, :], U.shape)], -1)
+        P = P + jrng.normal(scale=jitter_t * t, size=P.shape)
+        cut = nu if side == 0 else 0
+        P[cut] = np.stack([np.broadcast_to(xc[:, None], (nv + 1, nw + 1)), np.broadcast_to(y[:, None], (nv + 1, nw + 1)),
+                           np.broadcast_to(z[None, :], (nv + 1, nw + 1))], -1) + cut_jitter
+        idx = np.arange(U.size).reshape(U.shape)
+        parts = [_quads(idx[:, :, -1]), _quads(idx[:, :, 0].T), _quads(idx[:, -1, :].T), _quads(idx[:, 0, :]),
+                 _quads(idx[-1]), _quads(idx[0].T)]
+        frac_part = 4 if side == 0 else 5  # the fracture is the u = 1 face of A, the u = 0 face of B
+        F = np.concatenate(parts)
+        is_frac = np.concatenate([np.full(len(q), k == frac_part) for k, q in enumerate(parts)])
+        used, inv = np.unique(F, return_inverse=True)
+        F = inv.reshape(-1, 3).astype(np.uint32)
+        V = P.reshape(-1, 3)[used]
+        del P, U, Vg, Wg, X
+        frag = _fragment_data(V, F, is_frac, t, np.random.default_rng(seed + 100 + side))
+        frag["partner"] = np.where(is_frac, 1 - side, -1).astype(np.int32)
+        frags.append(frag)
+    prng = np.random.default_rng(pose_seed)
+    axis = prng.normal(size=3)
+    axis /= np.linalg.norm(axis)
+    ang = np.radians(max_deg) * (1.0 - prng.uniform())  # in (0, max_deg]
+    K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    PB = np.eye(4)
+    PB[:3, :3] = np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * (K @ K)
+    v = prng.normal(size=3)
+    PB[:3, 3] = v / np.linalg.norm(v) * max_shift_t * t * prng.uniform() ** (1.0 / 3.0)
+    frags[1] = dict(_posed(frags[1], PB), partner=frags[1]["partner"])
+    for k, fr in enumerate(frags):
+        fr["name"] = f"slab{k}"
+    return dict(fragments=frags, poses=[np.eye(4), PB], adjacent={(0, 1)}, t=t, res=edge_t * t)
+
+
 def _posed(frag, P):
     R, tau = P[:3, :3], P[:3, 3]
     out = dict(frag)

@@ -75,3 +75,32 @@ def test_the_slab_dumps_masks_step_by_step(seg, name):
     # The exact island sums and the reference's face-order f64 sums decide alike here.
     out2 = SG.cleanup(V, F, t, seg[name + "_frac_raw"], seg[name + "_rep"], seg[name + "_near"], exact=False)
     assert np.array_equal(out["final"], out2["final"]) and np.array_equal(out["islands"], out2["islands"])
+
+
+def test_vertex_votes_known_answers_on_a_plate():
+    """The per-vertex rays on a flat plate of wall 1: a shell vertex away from the rim looks straight
+    through the wall (hit at 1.0, the far face looking back) and votes; a vertex on the middle of a rim
+    looks along the plate (hit 8 or more away, beyond 1.8 t) and does not; a shell vertex's ray at
+    0.5 t (a plate of wall 0.5 read with t = 1) falls out of the window. Faces take the vertices'
+    majority."""
+    import oracle_lib as O
+    from sherd_hip import synth
+
+    col = synth.slab_plate(length_t=8.0, width_t=6.0, edge_t=0.25, amp_t=0.0, jitter_t=0.0, max_deg=1e-9,
+                           max_shift_t=0.0)
+    fr = col["fragments"][0]
+    V, F = fr["V"].astype(np.float64), fr["F"].astype(np.int64)
+    good, frac = SG.vertex_votes(O.Scene(fr["V"], fr["F"]), V, F, 1.0)
+    top = np.isclose(V[:, 2], 0.5) & (V[:, 0] > 0.6) & (V[:, 0] < 3.4) & (np.abs(V[:, 1]) < 2.4)
+    rim_mid = np.isclose(V[:, 1], 3.0) & np.isclose(V[:, 2], 0.0, atol=0.13) & (V[:, 0] > 1.0) & (V[:, 0] < 3.0)
+    assert top.sum() > 50 and rim_mid.sum() > 3
+    assert good[top].all() and not good[rim_mid].any()
+    cen = V[F].mean(axis=1)
+    inner_top = np.isclose(cen[:, 2], 0.5) & (cen[:, 0] > 0.6) & (cen[:, 0] < 3.4) & (np.abs(cen[:, 1]) < 2.4)
+    # Fracture faces off the shell edges (no vertex on a shell, whose edge normals lean 45 degrees
+    # and see the other shell at sqrt(2) t) are all labelled fracture.
+    off_edge = fr["frac"] & (np.abs(V[F][:, :, 2]).max(axis=1) < 0.49)
+    assert not frac[inner_top].any() and off_edge.sum() > 20 and frac[off_edge].all()
+    # Read as a wall of 2 t: the hit at 1.0 is 0.5 t, on the window's edge (not inside it).
+    good2, _ = SG.vertex_votes(O.Scene(fr["V"], fr["F"]), V, F, 2.0)
+    assert not good2[top].any()
