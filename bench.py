@@ -177,6 +177,60 @@ def icp_ladder(ex, synth, args):
     return res
 
 
+def slab_section(ex, synth, args):
+    """the slab plate (60 t x 40 t x t, Perlin-cut, jittered, edge 0.06 t)
+    through the whole path on the device (sherd_hip/pipeline.py): one inward ray per vertex for the
+    segmentation, R §3.6's sampling of the labelled meshes, R §5.1-5.5's match, a stage-2 ladder and
+    verification; rays/s of the per-vertex rays, and the oracle's rays on fragment A as the CPU
+    baseline of this row (checked bit for bit)."""
+    from sherd_hip import pipeline
+
+    t0 = time.perf_counter()
+    col = synth.slab_plate()
+    gen_s = time.perf_counter() - t0
+    fa, fb = col["fragments"]
+    ex.profile(True)
+    out = pipeline.run_pair(ex, fa, fb, col["t"], col["res"])
+    prof = ex.profile_read()
+    ex.profile(False)
+    truth = col["poses"][0] @ np.linalg.inv(col["poses"][1])
+    ang, tr = pose_gap_deg_t(out["T"], np.broadcast_to(truth, out["T"].shape))
+    acc = out["accepted"]
+    rays = len(fa["V"]) + len(fb["V"])
+    ray_ms = prof.get("k_vertex_votes", {}).get("ms", float("nan"))
+    agree = []
+    for fr, fp, P in zip(col["fragments"], out["frac"], col["poses"]):
+        shell = (~fr["frac"]) & (np.abs(fr["face_normals"] @ P[:3, 2]) > 0.9)
+        agree.append([round(float(np.mean(~fp[shell])), 4), round(float(np.mean(fp[fr["frac"]])), 4)])
+    res = dict(workload=f"slab plate 60 t x 40 t x t, edge 0.06 t, {len(fa['F'])} + {len(fb['F'])} triangles, "
+                        f"Perlin cut (0.3 t at 2 t), jitter 0.01 t, B posed {np.degrees(np.arccos(np.clip((np.trace(truth[:3, :3]) - 1) / 2, -1, 1))):.1f} deg",
+               generation_seconds=round(gen_s, 2), triangles=[int(len(fa["F"])), int(len(fb["F"]))], rays=int(rays),
+               vertex_ray_kernel_ms=round(ray_ms, 3), rays_per_s=round(rays / (ray_ms * 1e-3), 1),
+               labels_agree_shell_fracture=agree, seconds={k: round(v, 4) for k, v in out["seconds"].items()},
+               pipeline_seconds=round(sum(out["seconds"].values()), 4), hypotheses=int(out["match"]["n_hyp"]),
+               stage2_candidates=int(len(out["T"])), accepted=int(acc.sum()),
+               accepted_nearest_to_truth_deg_t=[float(ang[acc].min()), float(tr[acc].min())] if acc.any() else None)
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        sys.path.insert(0, os.path.join(HERE, "tests"))
+        import oracle_lib as O
+        import segment as SG
+
+        cores = min(os.cpu_count() or 1, 16)
+        O.set_threads(cores)
+        osc = O.Scene(fa["V"], fa["F"])
+        t0 = time.perf_counter()
+        og, _ = SG.vertex_votes(osc, fa["V"].astype(np.float64), fa["F"], col["t"])
+        el = time.perf_counter() - t0
+        res.update(cpu_rays_per_s=round(len(fa["V"]) / el, 1), cpu_identical_on_fragment_a=bool(np.array_equal(og, out["good"][0])),
+                   cpu_sample=f"oracle vertex_votes (normals in numpy, first hits by the C++ oracle on {cores} threads) "
+                              f"over fragment A's {len(fa['V'])} vertices")
+    log(f"slab: {rays} vertex rays in {ray_ms:.2f} ms; pipeline {res['pipeline_seconds']:.3f}s; accepted {int(acc.sum())} of "
+        f"{len(acc)}; nearest accepted {res['accepted_nearest_to_truth_deg_t']}"
+        + (f"; oracle {res['cpu_rays_per_s']:.0f} rays/s, identical {res['cpu_identical_on_fragment_a']}" if not args.no_cpu else ""))
+    return res
+
+
 def huber_section(ex, synth, col, args):
     """the robust seam energy over 512 batched poses on the vessel's seams
     (configuration 2's generator; per adjacent pair a cloud on each side of the seam, 10k-40k points
@@ -962,6 +1016,7 @@ def main():
     ap.add_argument("--icp-plain-pairs", type=int, default=-1,
                     help="pairs run in the plain per-call ICP schedule (-1: all, 0: none)")
     ap.add_argument("--no-huber", action="store_true")
+    ap.add_argument("--no-slab", action="store_true", help="skip configuration 1 (the slab plate, whole path)")
     ap.add_argument("--no-refine", action="store_true", help="skip R §9's full-resolution refinement")
     ap.add_argument("--no-match", action="store_true", help="skip R §5.1-5.5 over the vessel's pairs")
     ap.add_argument("--no-thickness", action="store_true", help="skip R §3.2 over the vessel's fragments")
@@ -996,7 +1051,8 @@ def main():
         return
     if args.no_collection:
         vessel, col = (None, None) if args.no_vessel else vessel_section(ex, synth, args)
-        out = dict(vessel=vessel, icp=icp_ladder(ex, synth, args) if args.icp_pairs > 0 else None,
+        out = dict(vessel=vessel, slab=slab_section(ex, synth, args) if not args.no_slab else None,
+                   icp=icp_ladder(ex, synth, args) if args.icp_pairs > 0 else None,
                    huber=huber_section(ex, synth, col, args) if not args.no_huber and col is not None else None,
                    refine=refine_section(ex, col, args) if not args.no_refine and col is not None else None,
                    match=match_section(ex, col, args) if not args.no_match and col is not None else None,
@@ -1017,6 +1073,7 @@ def main():
         ex.close()
         return
     vessel, col = (None, None) if args.no_vessel else vessel_section(ex, synth, args)
+    slab = slab_section(ex, synth, args) if not args.no_slab else None
     icp = icp_ladder(ex, synth, args) if args.icp_pairs > 0 else None
     huber = huber_section(ex, synth, col, args) if not args.no_huber and col is not None else None
     refine = refine_section(ex, col, args) if not args.no_refine and col is not None else None
@@ -1036,7 +1093,7 @@ def main():
                     candidates_per_step=sec["candidates"], pairs=sec["pairs"], parallelism="single GPU"),
         plain_pair_verifications_per_s=sec["plain_pair_verifications_per_s"], batched_vs_plain=sec["batched_vs_plain"],
         accepted=sec["accepted"], accepted_identical_plain_batched=sec["identical_plain_batched"]["accepted"],
-        collection=sec, vessel=vessel, icp=icp, huber=huber, refine=refine, match=match, thickness=thick,
+        collection=sec, vessel=vessel, slab=slab, icp=icp, huber=huber, refine=refine, match=match, thickness=thick,
         segment=segm, kernel_ms=head["prof"], roofline=roof, cpu_baseline=head["cpu"],
     )
     print(json.dumps(line), flush=True)

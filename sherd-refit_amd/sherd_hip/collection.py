@@ -28,11 +28,13 @@ VESSELS = [(0.8, 4.0, 13, 40), (1.0, 5.0, 13, 41), (1.3, 6.0, 14, 42)]  # (t, ed
 OPEN_FRAGMENTS = (7, 23)
 
 
-def build(height_t=200.0):
+def build(height_t=200.0, vessels=VESSELS, open_fragments=OPEN_FRAGMENTS, **vessel_kw):
+    """The collection: `vessels` (t, edges per t, fragments, seed) from the vessel generator (extra
+    keywords go to it), the fragments of `open_fragments` made non-watertight."""
     frags, poses, vessel_of, t_of, res_of = [], [], [], [], []
     adjacent = set()
-    for v, (t, e, n, seed) in enumerate(VESSELS):
-        col = synth.vessel(t=t, edges_per_t=e, n_frag=n, seed=seed, height_t=height_t)
+    for v, (t, e, n, seed) in enumerate(vessels):
+        col = synth.vessel(t=t, edges_per_t=e, n_frag=n, seed=seed, height_t=height_t, **vessel_kw)
         base = len(frags)
         for i, fr in enumerate(col["fragments"]):
             frags.append(fr)
@@ -43,7 +45,7 @@ def build(height_t=200.0):
         for a, b in col["adjacent"]:
             adjacent.add((base + a, base + b))
     rng = np.random.default_rng(43)
-    for k in OPEN_FRAGMENTS:
+    for k in open_fragments:
         frags[k] = synth.remove_patch(frags[k], 0.05, rng)
     return dict(fragments=frags, poses=poses, vessel_of=vessel_of, t_of=t_of, res_of=res_of, adjacent=adjacent)
 
