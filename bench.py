@@ -183,7 +183,7 @@ def slab_section(ex, synth, args):
     segmentation, R §3.6's sampling of the labelled meshes, R §5.1-5.5's match, a stage-2 ladder and
     verification; rays/s of the per-vertex rays, and the oracle's rays on fragment A as the CPU
     baseline of this row (checked bit for bit)."""
-    from sherd_hip import pipeline
+    from sherd_hip import SCORE_KEYS, pipeline
 
     t0 = time.perf_counter()
     col = synth.slab_plate()
@@ -208,8 +208,14 @@ def slab_section(ex, synth, args):
                vertex_ray_kernel_ms=round(ray_ms, 3), rays_per_s=round(rays / (ray_ms * 1e-3), 1),
                labels_agree_shell_fracture=agree, seconds={k: round(v, 4) for k, v in out["seconds"].items()},
                pipeline_seconds=round(sum(out["seconds"].values()), 4), hypotheses=int(out["match"]["n_hyp"]),
-               stage2_candidates=int(len(out["T"])), accepted=int(acc.sum()),
-               accepted_nearest_to_truth_deg_t=[float(ang[acc].min()), float(tr[acc].min())] if acc.any() else None)
+               stage2_candidates=int(len(out["T"])), accepted=int(acc.sum()))
+    best = int(out["order"][0])
+    res.update(best=dict(accepted=bool(acc[best]), gap_to_truth_deg_t=[round(float(ang[best]), 4), round(float(tr[best]), 4)],
+                         seam_x_tight=round(float(out["scores"][best, SCORE_KEYS.index("seam")]
+                                                  * out["scores"][best, SCORE_KEYS.index("tight")]), 3),
+                         runner_up_seam_x_tight=(round(float(out["scores"][out["order"][1], SCORE_KEYS.index("seam")]
+                                                             * out["scores"][out["order"][1], SCORE_KEYS.index("tight")]), 3)
+                                                 if len(out["order"]) > 1 else None)))
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(HERE, "oracle"))
         sys.path.insert(0, os.path.join(HERE, "tests"))
@@ -226,7 +232,7 @@ def slab_section(ex, synth, args):
                    cpu_sample=f"oracle vertex_votes (normals in numpy, first hits by the C++ oracle on {cores} threads) "
                               f"over fragment A's {len(fa['V'])} vertices")
     log(f"slab: {rays} vertex rays in {ray_ms:.2f} ms; pipeline {res['pipeline_seconds']:.3f}s; accepted {int(acc.sum())} of "
-        f"{len(acc)}; nearest accepted {res['accepted_nearest_to_truth_deg_t']}"
+        f"{len(acc)}; best {res['best']}"
         + (f"; oracle {res['cpu_rays_per_s']:.0f} rays/s, identical {res['cpu_identical_on_fragment_a']}" if not args.no_cpu else ""))
     return res
 

@@ -200,7 +200,30 @@ def vessel(t=1.0, edges_per_t=5.5, n_frag=8, seed=10, height_t=200.0, rmin_t=40.
     return dict(fragments=frags, poses=poses, adjacent=adjacent, t=t, res=res)
 
 
-def _fragment_data(v, f, frac, t, rng):
+def _macro_normals(cen, fn, area, mask, pts, r):
+    """Per point, the area-weighted mean normal of the faces of `mask` whose centroid lies within r
+    (normalised; zero where none is)."""
+    from scipy.sparse import csr_matrix
+
+    sel = np.nonzero(mask)[0]
+    out = np.zeros((len(pts), 3))
+    if len(sel) == 0 or len(pts) == 0:
+        return out
+    ball = cKDTree(cen[sel]).query_ball_point(pts, r, workers=-1)
+    rows = np.repeat(np.arange(len(pts)), [len(b) for b in ball])
+    cols = sel[np.concatenate([np.asarray(b, dtype=np.int64) for b in ball])] if len(rows) else np.zeros(0, np.int64)
+    M = csr_matrix((area[cols], (rows, cols)), shape=(len(pts), len(cen)))
+    out = M @ fn
+    ln = np.linalg.norm(out, axis=1, keepdims=True)
+    return np.where(ln > 0, out / np.maximum(ln, 1e-300), 0.0)
+
+
+def _fragment_data(v, f, frac, t, rng, brk_voxel_t=BRK_VOXEL, sub_voxel_t=SUB_VOXEL):
+    """R §3.6's samples of one labelled mesh. The breakline is the shell/fracture junction's
+    vertices thinned on a brk_voxel_t grid with shell ring normals; with brk_voxel_t None it is every
+    junction vertex (the reference takes every junction edge, fragment/breakline.rs:230-232) with
+    macro normals over 0.6 t (breakline.rs's annulus, without its inner radius), and the hypothesis
+    subset on a sub_voxel_t grid."""
     fi = np.arange(len(f))
     a, b, c = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
     n = np.cross(b - a, c - a)
@@ -220,23 +243,32 @@ def _fragment_data(v, f, frac, t, rng):
     vn = np.zeros_like(v)
     np.add.at(vn, f[shell].ravel(), np.repeat(fn[shell], 3, axis=0))
     brk = np.nonzero(on_frac & on_shell & (np.linalg.norm(vn, axis=1) > 0))[0]
-    key = np.floor(v[brk] / (BRK_VOXEL * t)).astype(np.int64)
-    _, first = np.unique(key, axis=0, return_index=True)
-    brk = brk[np.sort(first)]
+    if brk_voxel_t is not None:
+        key = np.floor(v[brk] / (brk_voxel_t * t)).astype(np.int64)
+        _, first = np.unique(key, axis=0, return_index=True)
+        brk = brk[np.sort(first)]
     brk_p = v[brk]
-    brk_ns = vn[brk] / np.linalg.norm(vn[brk], axis=1, keepdims=True)
     # Frames (R §3.5.4 / R §3.6): the fracture macro-normal around the vertex, the in-plane axis
     # pointing away from the fracture (minus the fracture normal's part across the shell normal),
     # the tangent ns x f, and the dihedral angle between the two normals. Across a seam the shell
     # normals agree and the fracture normals oppose, so the dihedral angles add to 180 degrees.
-    vf = np.zeros_like(v)
-    np.add.at(vf, f[frac].ravel(), np.repeat(fn[frac], 3, axis=0))
-    nf = vf[brk] / np.maximum(np.linalg.norm(vf[brk], axis=1, keepdims=True), 1e-300)
+    if brk_voxel_t is not None:
+        brk_ns = vn[brk] / np.linalg.norm(vn[brk], axis=1, keepdims=True)
+        vf = np.zeros_like(v)
+        np.add.at(vf, f[frac].ravel(), np.repeat(fn[frac], 3, axis=0))
+        nf = vf[brk] / np.maximum(np.linalg.norm(vf[brk], axis=1, keepdims=True), 1e-300)
+    else:
+        cen0 = (a + b + c) / 3.0
+        area0 = 0.5 * ln
+        brk_ns = _macro_normals(cen0, fn, area0, shell, brk_p, MACRO_OUTER * t)
+        nf = _macro_normals(cen0, fn, area0, frac, brk_p, MACRO_OUTER * t)
+        ok = (np.linalg.norm(brk_ns, axis=1) > 0.5) & (np.linalg.norm(nf, axis=1) > 0.5)
+        brk, brk_p, brk_ns, nf = brk[ok], brk_p[ok], brk_ns[ok], nf[ok]
     inplane = nf - (nf * brk_ns).sum(axis=1, keepdims=True) * brk_ns
     brk_f = -inplane / np.maximum(np.linalg.norm(inplane, axis=1, keepdims=True), 1e-300)
     brk_t = np.cross(brk_ns, brk_f)
     brk_dih = np.degrees(np.arccos(np.clip((brk_ns * nf).sum(axis=1), -1.0, 1.0)))
-    skey = np.floor(brk_p / (SUB_VOXEL * t)).astype(np.int64)
+    skey = np.floor(brk_p / (sub_voxel_t * t)).astype(np.int64)
     _, sfirst = np.unique(skey, axis=0, return_index=True)
     brk_sub = np.sort(sfirst).astype(np.int64)
     # Shell margin: shell faces whose centroid lies 0.15 t .. 0.6 t from the breakline.
@@ -271,14 +303,17 @@ def _quads(G):
 
 
 def slab_plate(t=1.0, edge_t=0.06, length_t=60.0, width_t=40.0, amp_t=0.3, wavelength_t=2.0, jitter_t=0.01,
-               seed=0, jitter_seed=1, pose_seed=2, max_deg=30.0, max_shift_t=5.0):
+               seed=0, jitter_seed=1, pose_seed=2, max_deg=30.0, max_shift_t=5.0, rounded_rims=True):
     """a plate length_t x width_t x t (t thick) split in two across its
     width by a z-extruded polyline x = length/2 + amp * perlin(y / wavelength) (seed), every vertex
     jittered by N(0, jitter) (jitter_seed; the shared fracture lattice identically on both sides),
     meshed closed at edge_t per edge (both pieces on one fracture lattice, so the fracture surfaces
     are exactly complementary), fragment B moved by a seeded rigid transform (axis uniform on the
     sphere, angle uniform in (0, max_deg], translation uniform in a ball of max_shift_t; pose_seed).
-    Returns the collection dict of `vessel` (fragments with V, F, frac, samples, frames; poses;
+    With rounded_rims the plate's three outer sides of each piece are half-round (every rim vertex
+    pushed outward by sqrt((t/2)^2 - z^2), the cut's end columns too, identically on both pieces):
+    two straight square rims would fit each other face to face as exactly as the cut does, and a
+    half-round rim touches another only along a line. Returns the collection dict of `vessel` (fragments with V, F, frac, samples, frames; poses;
     adjacent; t; res)."""
     L, W = length_t * t, width_t * t
     nu = max(4, int(round(0.5 * L / (edge_t * t))))
@@ -302,6 +337,12 @@ def slab_plate(t=1.0, edge_t=0.06, length_t=60.0, width_t=40.0, amp_t=0.3, wavel
         cut = nu if side == 0 else 0
         P[cut] = np.stack([np.broadcast_to(xc[:, None], (nv + 1, nw + 1)), np.broadcast_to(y[:, None], (nv + 1, nw + 1)),
                            np.broadcast_to(z[None, :], (nv + 1, nw + 1))], -1) + cut_jitter
+        if rounded_rims:
+            bulge = np.sqrt(np.clip(0.25 * t * t - z * z, 0.0, None))
+            outer = 0 if side == 0 else nu
+            P[outer, :, :, 0] += (-1.0 if side == 0 else 1.0) * bulge[None, :]
+            P[:, 0, :, 1] -= bulge[None, :]
+            P[:, -1, :, 1] += bulge[None, :]
         idx = np.arange(U.size).reshape(U.shape)
         parts = [_quads(idx[:, :, -1]), _quads(idx[:, :, 0].T), _quads(idx[:, -1, :].T), _quads(idx[:, 0, :]),
                  _quads(idx[-1]), _quads(idx[0].T)]

@@ -87,7 +87,7 @@ def test_vertex_votes_known_answers_on_a_plate():
     from sherd_hip import synth
 
     col = synth.slab_plate(length_t=8.0, width_t=6.0, edge_t=0.25, amp_t=0.0, jitter_t=0.0, max_deg=1e-9,
-                           max_shift_t=0.0)
+                           max_shift_t=0.0, rounded_rims=False)
     fr = col["fragments"][0]
     V, F = fr["V"].astype(np.float64), fr["F"].astype(np.int64)
     good, frac = SG.vertex_votes(O.Scene(fr["V"], fr["F"]), V, F, 1.0)
