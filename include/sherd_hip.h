@@ -37,7 +37,8 @@ enum {
   SHERD_E_HIP = -2,     /* the HIP runtime reported an error */
   SHERD_E_NOMEM = -3,   /* a device allocation failed */
   SHERD_E_LIMIT = -4,   /* the input exceeds a documented device limit */
-  SHERD_E_NODEV = -5    /* no usable gfx950 device */
+  SHERD_E_NODEV = -5,   /* no usable gfx950 device */
+  SHERD_E_CORRUPT = -6  /* a result's receipt did not come back (the readback is not trusted) */
 };
 
 typedef struct sherd_ctx sherd_ctx;
@@ -74,6 +75,68 @@ const char *sherd_last_error(void);
 int sherd_device_name(sherd_ctx *ctx, char *buf, size_t len);
 /* Counters of one method (SHERD_M_*). */
 int sherd_stats(sherd_ctx *ctx, int method, sherd_method_stats *out);
+
+/* ---- the self-test gate and the failure contract --------------------------------------------- */
+
+/* One row of the self-test (crates/sherd-gpu/src/selftest.rs:124-131 `Check`). */
+typedef struct {
+  char name[16];
+  int passed;
+  char detail[240];
+} sherd_check;
+
+/* What the self-test found (selftest.rs:135-150 `SelfTest`). */
+typedef struct {
+  int passed;                  /* every check held (selftest.rs:153 `passed`) */
+  int n_checks;
+  sherd_check checks[4];       /* limits, reduction, bounded nn, throughput, in that order */
+  uint32_t reduce_device_bits; /* the f32 sum of the reduction check, device and host mirror */
+  uint32_t reduce_host_bits;
+  uint64_t queries;            /* bounded-NN queries the throughput figures are over */
+  double gpu_ms, cpu_ms;       /* that batch: the kernel's host wall time, the host mirror's */
+  double speedup;              /* cpu_ms / gpu_ms, what backend selection compares */
+  int cpu_threads;
+} sherd_selftest_report;
+
+/* SelfTest::run (crates/sherd-gpu/src/selftest.rs:179), the gate of sherd_backend::resolve
+ * (crates/sherd-backend/src/lib.rs:230-247): (1) limits: a gfx950 target, 64-lane waves, 64 KiB of
+ * LDS per workgroup, 1024-lane workgroups; (2) a fixed-order f32 reduction of the 1e7 terms of
+ * selftest.rs:405-418 over 256 workgroups of 256 lanes, then one, bit-identical to the host mirror
+ * of the same order (selftest.rs:423-460; the reference's bits are 0x49a7230c); (3) the bounded
+ * nearest-neighbour kernel of sherd_nearest_below on two 6000-point clouds on a warped unit sheet
+ * under 64 small poses (r = 0.03): no hit/miss disagreement, |delta d| <= 1e-6, and at most 1e-5
+ * of the queries choosing another neighbour, each a tie to 1e-6 (selftest.rs:83-120, 231-270);
+ * (4) throughput: that batch's kernel wall time against the host mirror on every hardware thread
+ * (recorded, never failed). A status other than SHERD_OK means the test could not run. */
+int sherd_selftest(sherd_ctx *ctx, sherd_selftest_report *out);
+
+/* The failure contract of crates/sherd-gpu/src/executor.rs:19-33, 425-487: every device failure is
+ * per call and the shim answers that call from the CPU executor and counts it. These counters are
+ * the library's side of that count (the shim keeps its own `delegated`). */
+typedef struct {
+  uint64_t calls;        /* entry-point calls on the context */
+  uint64_t failed;       /* calls that returned a status other than SHERD_OK */
+  uint64_t arg_errors;   /* SHERD_E_ARG */
+  uint64_t host_errors;  /* SHERD_E_HIP */
+  uint64_t nomem;        /* SHERD_E_NOMEM: a device allocation refused */
+  uint64_t limit;        /* SHERD_E_LIMIT: a size policy or step budget */
+  uint64_t corrupt;      /* SHERD_E_CORRUPT: a receipt that did not come back */
+  uint64_t injected;     /* faults fired by sherd_fault_inject */
+} sherd_health;
+int sherd_health_read(sherd_ctx *ctx, sherd_health *out);
+
+/* Fault injection for testing the shim's contract (the build's answer to executor.rs's host
+ * error, corrupt readback and budget refusal triggers): arms one fault of `kind` to fire at its
+ * `after`-th opportunity from now (0: the next), then disarms. SHERD_FAULT_HOST fires at an entry
+ * point's prologue (the call returns SHERD_E_HIP having done nothing); SHERD_FAULT_NOMEM at a
+ * request for device memory from the context's buffers (SHERD_E_NOMEM on the call's own error
+ * path); SHERD_FAULT_CORRUPT at a receipt check (the receipt kernel writes the stale value and the
+ * call returns SHERD_E_CORRUPT after its kernels ran). SHERD_FAULT_NONE disarms. Receipts are
+ * checked by sherd_first_hit, sherd_bounded_distance, sherd_inside, sherd_nearest_below,
+ * sherd_coarse_scores, the ICP entry points, sherd_verify_batch, sherd_match_stage1 and
+ * sherd_huber_lm. The context keeps working after a fault. */
+enum { SHERD_FAULT_NONE = 0, SHERD_FAULT_HOST = 1, SHERD_FAULT_NOMEM = 2, SHERD_FAULT_CORRUPT = 3 };
+int sherd_fault_inject(sherd_ctx *ctx, int kind, int after);
 
 /* ---- scenes (RayScene, crates/sherd-core/src/spatial/bvh.rs:36-82) -------------------------- */
 

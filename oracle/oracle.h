@@ -132,6 +132,10 @@ enum {
 /* Scales in scales.json order: t, res, coarse, stage1, tight, facing, gap, seam, near, pen, nms,
  * icp (crates/sherd-core/src/matching/scales.rs:29-54). */
 void oracle_scales_for_pair(double t, double res, double out[12]);
+
+/* The self-test's fixed-order f32 reduction (crates/sherd-gpu/src/selftest.rs:405-460): the bits
+ * of the sum of reduce_terms(n) over `groups` workgroups, then one. */
+uint32_t oracle_reduce_bits(size_t n, size_t groups);
 /* verify::verify (verify.rs:590-626). retest: FP64 retest in the parity rays. */
 void oracle_verify(const oracle_side *a, const oracle_side *b, const double T[16],
                    const double scales[12], int full, int retest, double scores[OS_COUNT]);

@@ -23,19 +23,68 @@ using namespace sherd;
 
 namespace {
 thread_local std::string g_error;
+thread_local uint64_t g_fails = 0;  // failures set on this thread, and the last one's status
+thread_local int g_last = SHERD_OK;
+
+__global__ void k_receipt(uint32_t *word, uint32_t value) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *word = value;
 }
+}  // namespace
 
 namespace sherd {
 
 int fail(int code, const std::string &msg) {
   g_error = msg;
+  g_fails += 1;
+  g_last = code;
   return code;
 }
 
 int hip_fail(hipError_t e, const char *what) {
   std::string m = std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
-  g_error = m;
-  return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? SHERD_E_NOMEM : SHERD_E_HIP;
+  return fail((e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? SHERD_E_NOMEM : SHERD_E_HIP, m);
+}
+
+CallScope::CallScope(Ctx *c) : c_(c), n0_(g_fails) { c_->health[Ctx::H_CALLS] += 1; }
+
+CallScope::~CallScope() {
+  if (g_fails == n0_) return;
+  c_->health[Ctx::H_FAILED] += 1;
+  switch (g_last) {
+    case SHERD_E_ARG: c_->health[Ctx::H_ARG] += 1; break;
+    case SHERD_E_NOMEM: c_->health[Ctx::H_NOMEM] += 1; break;
+    case SHERD_E_LIMIT: c_->health[Ctx::H_LIMIT] += 1; break;
+    case SHERD_E_CORRUPT: c_->health[Ctx::H_CORRUPT] += 1; break;
+    default: c_->health[Ctx::H_HOST] += 1; break;
+  }
+}
+
+bool take_fault(Ctx *c, int kind) {
+  if (c->fault_kind != kind) return false;
+  if (c->fault_after > 0) {
+    c->fault_after -= 1;
+    return false;
+  }
+  c->fault_kind = SHERD_FAULT_NONE;
+  c->health[Ctx::H_INJECTED] += 1;
+  return true;
+}
+
+int sync_receipt(Ctx *c) {
+  c->nonce += 1;
+  uint32_t want = c->nonce + 1u;
+  // A fired corrupt fault leaves the word as a stale readback would hold it.
+  uint32_t value = take_fault(c, SHERD_FAULT_CORRUPT) ? c->nonce : want;
+  *c->receipt_host = 0u;
+  hipLaunchKernelGGL(k_receipt, dim3(1), dim3(64), 0, c->stream, c->receipt_dev, value);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(c->receipt_host, c->receipt_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_fail(e, "receipt");
+  if (*c->receipt_host != want)
+    return fail(SHERD_E_CORRUPT, "the receipt came back as " + std::to_string(*c->receipt_host) + ", not " +
+                                     std::to_string(want));
+  return SHERD_OK;
 }
 
 void *Ctx::pinned(size_t bytes, hipError_t *err) {
@@ -59,6 +108,10 @@ void *Ctx::pinned(size_t bytes, hipError_t *err) {
 
 void *Ctx::scratch(int slot, size_t bytes, hipError_t *err) {
   *err = hipSuccess;
+  if (take_fault(this, SHERD_FAULT_NOMEM)) {
+    *err = hipErrorOutOfMemory;
+    return nullptr;
+  }
   Buf &b = bufs[slot];
   if (b.n >= bytes && b.p) return b.p;
   if (b.p) {
@@ -171,6 +224,25 @@ extern "C" {
 const char *sherd_name(void) { return "hip"; }
 const char *sherd_last_error(void) { return g_error.c_str(); }
 
+int sherd_health_read(sherd_ctx *ctx, sherd_health *out) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c || !out) return fail(SHERD_E_ARG, "null argument");
+  static_assert(sizeof(sherd_health) == sizeof(uint64_t) * Ctx::H_COUNT, "sherd_health's fields");
+  uint64_t *o = &out->calls;  // sherd_health is H_COUNT consecutive uint64_t in Ctx's order
+  for (int k = 0; k < Ctx::H_COUNT; ++k) o[k] = c->health[k].load();
+  return SHERD_OK;
+}
+
+int sherd_fault_inject(sherd_ctx *ctx, int kind, int after) {
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!c) return fail(SHERD_E_ARG, "null context");
+  if (kind < SHERD_FAULT_NONE || kind > SHERD_FAULT_CORRUPT || after < 0) return fail(SHERD_E_ARG, "bad fault");
+  std::lock_guard<std::mutex> lock(c->mu);
+  c->fault_kind = kind;
+  c->fault_after = after;
+  return SHERD_OK;
+}
+
 int sherd_open(int device, sherd_ctx **out) {
   if (!out) return fail(SHERD_E_ARG, "null output");
   *out = nullptr;
@@ -201,6 +273,12 @@ int sherd_open(int device, sherd_ctx **out) {
     sherd_close(reinterpret_cast<sherd_ctx *>(c));
     return hip_fail(e, "hipGetSymbolAddress(scratch)");
   }
+  e = hipMalloc(&c->receipt_dev, 256);
+  if (e == hipSuccess) e = hipHostMalloc(&c->receipt_host, 256, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    sherd_close(reinterpret_cast<sherd_ctx *>(c));
+    return hip_fail(e, "receipt buffers");
+  }
   *out = reinterpret_cast<sherd_ctx *>(c);
   return SHERD_OK;
 }
@@ -213,6 +291,8 @@ void sherd_close(sherd_ctx *ctx) {
   for (auto &b : c->bufs)
     if (b.p) (void)hipFree(b.p);
   if (c->host.p) (void)hipHostFree(c->host.p);
+  if (c->receipt_dev) (void)hipFree(c->receipt_dev);
+  if (c->receipt_host) (void)hipHostFree(c->receipt_host);
   if (c->ladder_exec) (void)hipGraphExecDestroy(c->ladder_exec);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
@@ -342,7 +422,7 @@ int sherd_first_hit(sherd_ctx *ctx, const sherd_scene *scene, const float *orig,
   tm.stop();
   HIPCK(hipMemcpyAsync(t, d_t, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream), "download");
   HIPCK(hipMemcpyAsync(prim, d_p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream), "download");
-  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  SYNC_RECEIPT(c);
   return SHERD_OK;
 }
 
@@ -382,7 +462,7 @@ int sherd_bounded_distance(sherd_ctx *ctx, const sherd_scene *scene, const doubl
     tm.stop();
     std::vector<float> h(n);
     HIPCK(hipMemcpyAsync(h.data(), d_o, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream), "download");
-    HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+    SYNC_RECEIPT(c);
     for (size_t i = 0; i < n; ++i) out[i] = (double)h[i];
     return SHERD_OK;
   }
@@ -400,7 +480,7 @@ int sherd_bounded_distance(sherd_ctx *ctx, const sherd_scene *scene, const doubl
   tm.stop();
   unsigned int bits = 0;
   HIPCK(hipMemcpyAsync(&bits, d_b, sizeof bits, hipMemcpyDeviceToHost, c->stream), "download");
-  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  SYNC_RECEIPT(c);
   out[0] = (double)__builtin_bit_cast(float, bits);
   return SHERD_OK;
 }
@@ -480,7 +560,7 @@ int sherd_inside(sherd_ctx *ctx, const sherd_scene *scene, const double *pts, si
   std::vector<uint32_t> hi(n);
   HIPCK(hipMemcpyAsync(hd.data(), d_depth, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream), "download");
   HIPCK(hipMemcpyAsync(hi.data(), d_in, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream), "download");
-  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  SYNC_RECEIPT(c);
   for (size_t i = 0; i < n; ++i) {
     out[i].depth = hd[i];
     out[i].inside = hi[i];
@@ -541,7 +621,7 @@ int sherd_nearest_below(sherd_ctx *ctx, const double *cloud, size_t n, const dou
   tm.stop();
   HIPCK(hipMemcpyAsync(idx, di, sizeof(int64_t) * m, hipMemcpyDeviceToHost, c->stream), "download");
   HIPCK(hipMemcpyAsync(dist, dd, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream), "download");
-  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  SYNC_RECEIPT(c);
   return SHERD_OK;
 }
 
@@ -588,7 +668,7 @@ int sherd_coarse_scores(sherd_ctx *ctx, const double *target_p, const double *ta
   tm.stop();
   if (counts) HIPCK(hipMemcpyAsync(counts, dcnt, sizeof(uint32_t) * k, hipMemcpyDeviceToHost, c->stream), "download");
   HIPCK(hipMemcpyAsync(scores, dsc, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream), "download");
-  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  SYNC_RECEIPT(c);
   return SHERD_OK;
 }
 

@@ -190,7 +190,7 @@ int sherd_huber_lm(sherd_huber *h, double delta, int iterations, int precision, 
   HIPCK(hipMemcpyAsync(T_out, h->T_out, sizeof(double) * 16 * k, hipMemcpyDeviceToHost, c->stream), "download");
   HIPCK(hipMemcpyAsync(energy, h->E, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream), "download");
   if (steps) HIPCK(hipMemcpyAsync(steps, h->steps, sizeof(uint32_t) * k, hipMemcpyDeviceToHost, c->stream), "download");
-  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  SYNC_RECEIPT(c);
   c->prof.collect();
   return SHERD_OK;
 }

@@ -523,7 +523,7 @@ int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_
   HIPCK(hipMemcpyAsync(h.data(), dout, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream), "download");
   std::vector<double> hT(16 * n_cand * (n_rungs + 1));
   HIPCK(hipMemcpyAsync(hT.data(), dT, sizeof(double) * hT.size(), hipMemcpyDeviceToHost, c->stream), "download");
-  HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  SYNC_RECEIPT(c);
   c->prof.collect();
   for (size_t i = 0; i < n_cand; ++i)
     for (size_t r = 0; r < n_rungs; ++r) {

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -45,6 +46,35 @@ struct Ctx {
   // This device's copy of the file-scope `scratch` array of the no-allocation entry point (its
   // address from hipGetSymbolAddress, taken at open with the device current).
   void *noalloc = nullptr;
+  // The failure contract: counters, an armed fault, and the receipt word (device and pinned host).
+  // sherd_health's fields in order (atomic: an entry point counts outside its locked sections).
+  enum { H_CALLS, H_FAILED, H_ARG, H_HOST, H_NOMEM, H_LIMIT, H_CORRUPT, H_INJECTED, H_COUNT };
+  std::atomic<uint64_t> health[H_COUNT] = {};
+  int fault_kind = SHERD_FAULT_NONE;
+  int fault_after = 0;
+  uint32_t nonce = 0;
+  uint32_t *receipt_dev = nullptr;
+  uint32_t *receipt_host = nullptr;
+};
+
+// Whether the armed fault of `kind` fires at this opportunity (counting the armed countdown down).
+bool take_fault(Ctx *c, int kind);
+// Queues the receipt kernel (the context's next nonce + 1, or the stale nonce when a corrupt fault
+// fires) and its readback behind everything on the stream, synchronises, and checks it.
+int sync_receipt(Ctx *c);
+
+// Counts a call and, when it fails, its status (any status a failure helper set on this thread
+// during the call).
+class CallScope {
+ public:
+  explicit CallScope(Ctx *c);
+  ~CallScope();
+  CallScope(const CallScope &) = delete;
+  CallScope &operator=(const CallScope &) = delete;
+
+ private:
+  Ctx *c_;
+  uint64_t n0_;
 };
 
 // Accumulates one method's call count, item count and device time (HIP events on the stream).
@@ -79,11 +109,37 @@ struct StreamFence {
 // An entry point's prologue: the context's lock, its device, and a fence that drains the context's
 // stream before the lock is released, so no copy the call queued outlives the call (the caller's
 // buffers and the pinned staging buffer are free to change once it returns).
-#define LOCK(ctx)                                                                \
-  if (!(ctx)) return ::sherd::fail(SHERD_E_ARG, "null context");                 \
-  std::lock_guard<std::mutex> lock_((ctx)->mu);                                  \
-  ::sherd::StreamFence fence_((ctx)->stream);                                    \
+#define LOCK(ctx)                                                                              \
+  if (!(ctx)) return ::sherd::fail(SHERD_E_ARG, "null context");                               \
+  std::lock_guard<std::mutex> lock_((ctx)->mu);                                                \
+  ::sherd::StreamFence fence_((ctx)->stream);                                                  \
+  ::sherd::CallScope scope_(ctx);                                                              \
+  if (hipSetDevice((ctx)->device) != hipSuccess) return ::sherd::hip_fail(hipGetLastError(), "hipSetDevice"); \
+  if (::sherd::take_fault((ctx), SHERD_FAULT_HOST)) return ::sherd::fail(SHERD_E_HIP, "injected host error")
+
+// For an entry point whose work spans several locked sections (RELOCK): counts the call and offers
+// the host fault once, up front.
+#define CALL_SCOPE(ctx)                                                                        \
+  if (!(ctx)) return ::sherd::fail(SHERD_E_ARG, "null context");                               \
+  ::sherd::CallScope scope_(ctx);                                                              \
+  {                                                                                            \
+    std::lock_guard<std::mutex> l_((ctx)->mu);                                                 \
+    if (::sherd::take_fault((ctx), SHERD_FAULT_HOST)) return ::sherd::fail(SHERD_E_HIP, "injected host error"); \
+  }
+
+// LOCK for a section of an entry point that opened with CALL_SCOPE: the same lock, fence and
+// device, without counting the call again or offering a host fault.
+#define RELOCK(ctx)                                                                            \
+  std::lock_guard<std::mutex> lock_((ctx)->mu);                                                \
+  ::sherd::StreamFence fence_((ctx)->stream);                                                  \
   if (hipSetDevice((ctx)->device) != hipSuccess) return ::sherd::hip_fail(hipGetLastError(), "hipSetDevice")
+
+// An entry point's last synchronisation, with its receipt checked.
+#define SYNC_RECEIPT(ctx)                                     \
+  do {                                                        \
+    int r_ = ::sherd::sync_receipt(ctx);                      \
+    if (r_ != SHERD_OK) return r_;                            \
+  } while (0)
 
 #define HIPCK(call, what)                                    \
   do {                                                       \

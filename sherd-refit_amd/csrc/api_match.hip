@@ -306,7 +306,7 @@ int sherd_match_stage1(sherd_ctx *ctx, const sherd_match_pair *pairs, size_t n_p
                        double dihedral_tol, sherd_match_result *res, uint32_t *kept1, double *s1_T, double *s1_score,
                        uint32_t *kept2) {
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
-  if (!c) return fail(SHERD_E_ARG, "null context");
+  CALL_SCOPE(c);
   if (n_pairs == 0) return SHERD_OK;
   if (!pairs || !res || !kept1 || !s1_T || !s1_score || !kept2) return fail(SHERD_E_ARG, "null buffer");
   if (n_pairs > 65535) return fail(SHERD_E_LIMIT, "more than 65535 pairs in one batch");
@@ -322,7 +322,7 @@ int sherd_match_stage1(sherd_ctx *ctx, const sherd_match_pair *pairs, size_t n_p
   int rc;
   std::vector<double> tol(np);
   {
-    LOCK(c);
+    RELOCK(c);
     // Every target grid at the largest reach any pair asks of it, before any descriptor reads it.
     std::vector<std::pair<sherd_frames *, double>> reach;
     for (size_t p = 0; p < np; ++p) {
@@ -433,7 +433,7 @@ int sherd_match_stage1(sherd_ctx *ctx, const sherd_match_pair *pairs, size_t n_p
     for (size_t i = 0; i < nc; ++i) std::memcpy(&T[16 * i], reg[2 * i + 1].transform, sizeof(double) * 16);
   }
   {
-    LOCK(c);
+    RELOCK(c);
     std::vector<StagePair> sp(np);
     for (size_t p = 0; p < np; ++p) {
       StagePair &s = sp[p];
@@ -459,7 +459,8 @@ int sherd_match_stage1(sherd_ctx *ctx, const sherd_match_pair *pairs, size_t n_p
     if ((rc = order_and_suppress(c, db, dcp, dfirst, first, cnt, pose, score, (uint32_t)nc, tol, UINT32_MAX,
                                  (int)stage2, STAGE1_FLOOR, &k2, &n2)))
       return rc;
-    if (nc) HIPCK(hipMemcpy(s1.data(), score, sizeof(double) * nc, hipMemcpyDeviceToHost), "download");
+    if (nc) HIPCK(hipMemcpyAsync(s1.data(), score, sizeof(double) * nc, hipMemcpyDeviceToHost, c->stream), "download");
+    SYNC_RECEIPT(c);
   }
   for (size_t p = 0; p < np; ++p) {
     res[p].n_kept2 = n2[p];

@@ -339,7 +339,7 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     HIPCK(launch_verify(vb, max_pf, max_s, max_margin, max_brk, c->stream, &c->side, &c->prof), "verify kernels");
     // Stream order puts the download after the upload has read the staging buffer.
     HIPCK(hipMemcpyAsync(hbuf, ddn, dn_bytes, hipMemcpyDeviceToHost, c->stream), "download");
-    HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+    SYNC_RECEIPT(c);
     c->prof.collect();
     const CandAcc *hacc = (const CandAcc *)(hbuf + dn_acc);
     for (size_t i = 0; i < nc; ++i) {

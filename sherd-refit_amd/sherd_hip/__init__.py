@@ -98,6 +98,30 @@ class MethodStats(C.Structure):
     _fields_ = [("calls", C.c_uint64), ("items", C.c_uint64), ("device_ns", C.c_uint64)]
 
 
+class Check(C.Structure):
+    _fields_ = [("name", C.c_char * 16), ("passed", C.c_int), ("detail", C.c_char * 240)]
+
+
+class SelfTestReport(C.Structure):
+    """`sherd_selftest_report` (crates/sherd-gpu/src/selftest.rs:135-150 `SelfTest`)."""
+
+    _fields_ = [("passed", C.c_int), ("n_checks", C.c_int), ("checks", Check * 4),
+                ("reduce_device_bits", C.c_uint32), ("reduce_host_bits", C.c_uint32), ("queries", C.c_uint64),
+                ("gpu_ms", C.c_double), ("cpu_ms", C.c_double), ("speedup", C.c_double), ("cpu_threads", C.c_int)]
+
+
+HEALTH_KEYS = ["calls", "failed", "arg_errors", "host_errors", "nomem", "limit", "corrupt", "injected"]
+
+
+class Health(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in HEALTH_KEYS]
+
+
+# Status codes (include/sherd_hip.h) and the faults sherd_fault_inject arms.
+E_ARG, E_HIP, E_NOMEM, E_LIMIT, E_NODEV, E_CORRUPT = -1, -2, -3, -4, -5, -6
+FAULT = dict(none=0, host=1, nomem=2, corrupt=3)
+
+
 def lib():
     """The loaded library; raises if it was never built."""
     global _lib
@@ -122,6 +146,9 @@ def _bind(L):
         "sherd_last_error": (C.c_char_p, []),
         "sherd_device_name": (ci, [vp, C.c_char_p, sz]),
         "sherd_stats": (ci, [vp, ci, P(MethodStats)]),
+        "sherd_selftest": (ci, [vp, P(SelfTestReport)]),
+        "sherd_health_read": (ci, [vp, P(Health)]),
+        "sherd_fault_inject": (ci, [vp, ci, ci]),
         "sherd_scene_build": (ci, [vp, P(f), sz, P(u32), sz, P(u8), P(vp)]),
         "sherd_scene_free": (None, [vp]),
         "sherd_scene_info": (ci, [vp, P(sz), P(f), P(f)]),
@@ -480,6 +507,27 @@ class HipExecutor:
         s = MethodStats()
         _check(lib().sherd_stats(self.ptr, SHERD_M[method], C.byref(s)))
         return dict(calls=s.calls, items=s.items, device_ns=s.device_ns)
+
+    def selftest(self):
+        """SelfTest::run (selftest.rs:179) through sherd_selftest: dict(passed, checks [(name, passed,
+        detail)], reduce bits device/host, queries, gpu_ms, cpu_ms, speedup, cpu_threads)."""
+        r = SelfTestReport()
+        _check(lib().sherd_selftest(self.ptr, C.byref(r)))
+        return dict(passed=bool(r.passed),
+                    checks=[(r.checks[k].name.decode(), bool(r.checks[k].passed), r.checks[k].detail.decode())
+                            for k in range(r.n_checks)],
+                    reduce_device_bits=r.reduce_device_bits, reduce_host_bits=r.reduce_host_bits, queries=r.queries,
+                    gpu_ms=r.gpu_ms, cpu_ms=r.cpu_ms, speedup=r.speedup, cpu_threads=r.cpu_threads)
+
+    def health(self):
+        """The context's call and failure counters (sherd_health_read)."""
+        h = Health()
+        _check(lib().sherd_health_read(self.ptr, C.byref(h)))
+        return {k: int(getattr(h, k)) for k in HEALTH_KEYS}
+
+    def fault_inject(self, kind, after=0):
+        """Arms one fault ('host', 'nomem', 'corrupt'; 'none' disarms) at its after-th opportunity."""
+        _check(lib().sherd_fault_inject(self.ptr, FAULT[kind], int(after)))
 
     def profile(self, on=True):
         _check(lib().sherd_profile(self.ptr, int(on)))

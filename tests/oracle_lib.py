@@ -77,6 +77,8 @@ def _bind(L):
     L.oracle_icp_correspondences.argtypes = [P(d), sz, P(d), sz, d, P(i64)]
     L.oracle_normal_equations.argtypes = [P(d), sz, P(d), P(d), P(i64), P(d), P(d)]
     L.oracle_scales_for_pair.argtypes = [d, d, P(d)]
+    L.oracle_reduce_bits.argtypes = [C.c_size_t, C.c_size_t]
+    L.oracle_reduce_bits.restype = C.c_uint32
     L.oracle_verify.argtypes = [P(Side), P(Side), P(d), P(d), C.c_int, C.c_int, P(d)]
     L.oracle_accept.restype = C.c_int
     L.oracle_accept.argtypes = [P(d), P(d)]
@@ -283,6 +285,11 @@ def huber_lm(src, tgt_p, tgt_n, radius, T0, delta, iterations, threads=0):
                           _p(T0, C.c_double), k, float(delta), int(iterations), int(threads), _p(T, C.c_double),
                           _p(E, C.c_double), _p(st, C.c_uint32), _p(status, C.c_int32))
     return T.reshape(k, 4, 4), E, st, status
+
+
+def reduce_bits(n, groups):
+    """The self-test reduction's f32 sum bits (selftest.rs:405-460)."""
+    return int(lib().oracle_reduce_bits(int(n), int(groups)))
 
 
 def scales_for_pair(t, res):
