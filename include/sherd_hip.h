@@ -38,7 +38,8 @@ enum {
   SHERD_E_NOMEM = -3,   /* a device allocation failed */
   SHERD_E_LIMIT = -4,   /* the input exceeds a documented device limit */
   SHERD_E_NODEV = -5,   /* no usable gfx950 device */
-  SHERD_E_CORRUPT = -6  /* a result's receipt did not come back (the readback is not trusted) */
+  SHERD_E_CORRUPT = -6, /* a result's receipt did not come back (the readback is not trusted) */
+  SHERD_E_FORMAT = -7   /* a file that is missing, unreadable, or not a cache this build reads */
 };
 
 typedef struct sherd_ctx sherd_ctx;
@@ -137,6 +138,86 @@ int sherd_health_read(sherd_ctx *ctx, sherd_health *out);
  * sherd_huber_lm. The context keeps working after a fault. */
 enum { SHERD_FAULT_NONE = 0, SHERD_FAULT_HOST = 1, SHERD_FAULT_NOMEM = 2, SHERD_FAULT_CORRUPT = 3 };
 int sherd_fault_inject(sherd_ctx *ctx, int kind, int after);
+
+/* ---- the fragment cache and MatchData (host side: what the pair stages read) ------------------ */
+
+typedef struct sherd_cache sherd_cache;
+
+/* The metadata block of a `.sherd` file (crates/sherd-core/src/fragment/cache.rs:104-176
+ * `CacheMeta`) that the pair stages read, and the tensors' sizes. */
+typedef struct {
+  char name[256];
+  uint32_t cache_version, target_faces, face_budget, n_boundary;
+  int watertight;
+  double thick, thick_mode, res, area, area0, frac_area;
+  double brk_t, md_t;  /* brk_params.t and md_params.t: the t the arrays were built at */
+  size_t nv, nf, n_brk, n_sub, n_s, n_pf, n_margin;
+} sherd_cache_info;
+
+/* cache::read / from_bytes (cache.rs:329-438, 384-389): a `.sherd` safetensors file whose
+ * metadata says format "sherd-cache", cache_version 7 and algo_ref "2026-09-06/9d4b9d3"
+ * (crates/sherd-core/src/lib.rs:130, 152), refused with SHERD_E_FORMAT and cache.rs's message
+ * otherwise: a tensor of the wrong dtype or shape, a triangle outside V, labels that do not
+ * describe F, breakline frames that do not describe brk_P, a brk_sub outside it, sp/fp that do not
+ * describe S/Pf, a sample face outside F, a margin_idx outside S. Host memory only. */
+int sherd_cache_read(const char *path, sherd_cache **out);
+int sherd_cache_from_bytes(const uint8_t *bytes, size_t n, sherd_cache **out);
+void sherd_cache_free(sherd_cache *cache);
+int sherd_cache_info_get(const sherd_cache *cache, sherd_cache_info *out);
+
+/* The stored tensors (cache.rs:11-26), copied into caller buffers sized by the info (any pointer
+ * may be NULL): V nv x 3, F nf x 3, labels nf (0 shell, 1 fracture, 2 solid), brk_P / brk_ns /
+ * brk_nf / brk_f n_brk x 3, brk_sub n_sub, brk_valid n_brk, S n_s x 3, sp n_s, Pf n_pf x 3, fp n_pf,
+ * margin_idx n_margin. */
+typedef struct {
+  float *V;
+  uint32_t *F;
+  uint8_t *labels;
+  float *brk_P, *brk_ns, *brk_nf, *brk_f;
+  uint32_t *brk_sub;
+  uint8_t *brk_valid;
+  float *S;
+  uint32_t *sp;
+  float *Pf;
+  uint32_t *fp;
+  uint32_t *margin_idx;
+} sherd_cache_arrays_out;
+int sherd_cache_arrays(const sherd_cache *cache, const sherd_cache_arrays_out *out);
+
+/* Sizes of MatchData::own's arrays. */
+typedef struct {
+  size_t n_reg, n_reg_frac, n_reg_margin; /* pc_reg and its two prefixes */
+  size_t n_pf, n_brk, n_sub, n_margin, n_s, nf;
+  double t, frac_area;
+  int watertight;
+} sherd_match_info;
+
+/* MatchData::own's arrays (each pointer may be NULL), f32 values widened to f64 exactly:
+ * pc_reg (reg_p, reg_n: n_reg x 3), pc_frac (pf, pf_n = FN[fp]: n_pf x 3), the breakline (brk_p,
+ * brk_ns, brk_f: n_brk x 3; brk_t = ns x f in f64, unnarrowed; brk_dih in degrees: n_brk), the
+ * hypothesis subset (brk_sub: n_sub), the margin (margin_p = S[margin_idx], margin_n =
+ * FN[sp[margin_idx]]: n_margin x 3), the surface samples (s, s_n = FN[sp]: n_s x 3; s_frac =
+ * labels[sp] == fracture: n_s), and the face normals FN (nf x 3). */
+typedef struct {
+  double *reg_p, *reg_n;
+  double *pf, *pf_n;
+  double *brk_p, *brk_ns, *brk_f, *brk_t, *brk_dih;
+  uint32_t *brk_sub;
+  double *margin_p, *margin_n;
+  double *s, *s_n;
+  uint8_t *s_frac;
+  double *face_normals;
+} sherd_match_arrays;
+
+/* MatchData::own (crates/sherd-core/src/fragment/samples.rs:499-560) at the fragment's own t: FN
+ * from face_geometry in f64 over the f32 vertices, narrowed (types.rs:100-112, geometry.rs:70-96);
+ * pc_reg the prefixes of registration_split(n_pf, n_margin, reg_points) (samples.rs:585-615;
+ * numpy's half-to-even rounding, at least one fracture point; reg_points 0 keeps all, the
+ * reference's default is 6000). With out NULL only info is filled. SHERD_E_LIMIT when the cached
+ * arrays were built at another t than the fragment's (the reference rebuilds those,
+ * samples.rs:562-575). */
+int sherd_match_data(const sherd_cache *cache, size_t reg_points, sherd_match_info *info,
+                     const sherd_match_arrays *out);
 
 /* ---- scenes (RayScene, crates/sherd-core/src/spatial/bvh.rs:36-82) -------------------------- */
 

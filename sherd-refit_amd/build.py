@@ -29,11 +29,11 @@ FLAGS = [
 
 
 def sources():
-    return sorted(f for f in os.listdir(SRC) if f.endswith(".hip"))
+    return sorted(f for f in os.listdir(SRC) if f.endswith(".hip") or f.endswith(".cpp"))
 
 
 def _compile(name):
-    out = os.path.join(OBJ, name.replace(".hip", ".o"))
+    out = os.path.join(OBJ, os.path.splitext(name)[0] + ".o")
     cmd = [HIPCC] + FLAGS + ["-c", os.path.join(SRC, name), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     noise = "packed-fp32-ops"  # hipcc's host pass does not know the feature; harmless

@@ -110,6 +110,105 @@ class SelfTestReport(C.Structure):
                 ("gpu_ms", C.c_double), ("cpu_ms", C.c_double), ("speedup", C.c_double), ("cpu_threads", C.c_int)]
 
 
+class CacheInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 256), ("cache_version", C.c_uint32), ("target_faces", C.c_uint32),
+                ("face_budget", C.c_uint32), ("n_boundary", C.c_uint32), ("watertight", C.c_int),
+                ("thick", C.c_double), ("thick_mode", C.c_double), ("res", C.c_double), ("area", C.c_double),
+                ("area0", C.c_double), ("frac_area", C.c_double), ("brk_t", C.c_double), ("md_t", C.c_double),
+                ("nv", C.c_size_t), ("nf", C.c_size_t), ("n_brk", C.c_size_t), ("n_sub", C.c_size_t),
+                ("n_s", C.c_size_t), ("n_pf", C.c_size_t), ("n_margin", C.c_size_t)]
+
+
+_CACHE_ARRAYS = [("V", C.c_float, "nv", 3), ("F", C.c_uint32, "nf", 3), ("labels", C.c_uint8, "nf", 0),
+                 ("brk_P", C.c_float, "n_brk", 3), ("brk_ns", C.c_float, "n_brk", 3),
+                 ("brk_nf", C.c_float, "n_brk", 3), ("brk_f", C.c_float, "n_brk", 3),
+                 ("brk_sub", C.c_uint32, "n_sub", 0), ("brk_valid", C.c_uint8, "n_brk", 0),
+                 ("S", C.c_float, "n_s", 3), ("sp", C.c_uint32, "n_s", 0), ("Pf", C.c_float, "n_pf", 3),
+                 ("fp", C.c_uint32, "n_pf", 0), ("margin_idx", C.c_uint32, "n_margin", 0)]
+
+
+class CacheArraysOut(C.Structure):
+    _fields_ = [(n, C.POINTER(ct)) for n, ct, _, _ in _CACHE_ARRAYS]
+
+
+class MatchInfo(C.Structure):
+    _fields_ = [("n_reg", C.c_size_t), ("n_reg_frac", C.c_size_t), ("n_reg_margin", C.c_size_t),
+                ("n_pf", C.c_size_t), ("n_brk", C.c_size_t), ("n_sub", C.c_size_t), ("n_margin", C.c_size_t),
+                ("n_s", C.c_size_t), ("nf", C.c_size_t), ("t", C.c_double), ("frac_area", C.c_double),
+                ("watertight", C.c_int)]
+
+
+_MATCH_ARRAYS = [("reg_p", C.c_double, "n_reg", 3), ("reg_n", C.c_double, "n_reg", 3),
+                 ("pf", C.c_double, "n_pf", 3), ("pf_n", C.c_double, "n_pf", 3),
+                 ("brk_p", C.c_double, "n_brk", 3), ("brk_ns", C.c_double, "n_brk", 3),
+                 ("brk_f", C.c_double, "n_brk", 3), ("brk_t", C.c_double, "n_brk", 3),
+                 ("brk_dih", C.c_double, "n_brk", 0), ("brk_sub", C.c_uint32, "n_sub", 0),
+                 ("margin_p", C.c_double, "n_margin", 3), ("margin_n", C.c_double, "n_margin", 3),
+                 ("s", C.c_double, "n_s", 3), ("s_n", C.c_double, "n_s", 3), ("s_frac", C.c_uint8, "n_s", 0),
+                 ("face_normals", C.c_double, "nf", 3)]
+
+
+class MatchArrays(C.Structure):
+    _fields_ = [(n, C.POINTER(ct)) for n, ct, _, _ in _MATCH_ARRAYS]
+
+
+_NP = {C.c_float: np.float32, C.c_double: np.float64, C.c_uint32: np.uint32, C.c_uint8: np.uint8}
+
+
+def _arrays(spec, sizes, struct):
+    out, ptrs = {}, {}
+    for name, ct, size, cols in spec:
+        n = int(getattr(sizes, size))
+        a = np.zeros((n, cols) if cols else n, _NP[ct])
+        out[name] = a
+        ptrs[name] = _p(a, ct)
+    return out, struct(**ptrs)
+
+
+class FragmentCache:
+    """A `.sherd` fragment cache read by the library (sherd_cache_read; cache.rs:329-438): `info`
+    (the metadata and sizes), `arrays()` (the stored tensors), `match_data(reg_points)`
+    (MatchData::own's arrays, samples.rs:499-560). Host memory; no device context needed."""
+
+    def __init__(self, path=None, data=None):
+        self.ptr = C.c_void_p()
+        if data is not None:
+            buf = np.frombuffer(bytes(data), np.uint8)
+            _check(lib().sherd_cache_from_bytes(_p(buf, C.c_uint8), len(buf), C.byref(self.ptr)))
+        else:
+            _check(lib().sherd_cache_read(os.fsencode(path), C.byref(self.ptr)))
+        i = CacheInfo()
+        _check(lib().sherd_cache_info_get(self.ptr, C.byref(i)))
+        self._info = i
+        self.info = {k: (getattr(i, k).decode() if k == "name" else getattr(i, k)) for k, _ in CacheInfo._fields_}
+
+    def arrays(self):
+        out, st = _arrays(_CACHE_ARRAYS, self._info, CacheArraysOut)
+        _check(lib().sherd_cache_arrays(self.ptr, C.byref(st)))
+        out["brk_valid"] = out["brk_valid"].astype(bool)
+        return out
+
+    def match_data(self, reg_points=6000):
+        mi = MatchInfo()
+        _check(lib().sherd_match_data(self.ptr, int(reg_points), C.byref(mi), None))
+        out, st = _arrays(_MATCH_ARRAYS, mi, MatchArrays)
+        _check(lib().sherd_match_data(self.ptr, int(reg_points), C.byref(mi), C.byref(st)))
+        out["s_frac"] = out["s_frac"].astype(bool)
+        out.update({k: getattr(mi, k) for k, _ in MatchInfo._fields_})
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib().sherd_cache_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 HEALTH_KEYS = ["calls", "failed", "arg_errors", "host_errors", "nomem", "limit", "corrupt", "injected"]
 
 
@@ -118,7 +217,7 @@ class Health(C.Structure):
 
 
 # Status codes (include/sherd_hip.h) and the faults sherd_fault_inject arms.
-E_ARG, E_HIP, E_NOMEM, E_LIMIT, E_NODEV, E_CORRUPT = -1, -2, -3, -4, -5, -6
+E_ARG, E_HIP, E_NOMEM, E_LIMIT, E_NODEV, E_CORRUPT, E_FORMAT = -1, -2, -3, -4, -5, -6, -7
 FAULT = dict(none=0, host=1, nomem=2, corrupt=3)
 
 
@@ -149,6 +248,12 @@ def _bind(L):
         "sherd_selftest": (ci, [vp, P(SelfTestReport)]),
         "sherd_health_read": (ci, [vp, P(Health)]),
         "sherd_fault_inject": (ci, [vp, ci, ci]),
+        "sherd_cache_read": (ci, [C.c_char_p, P(vp)]),
+        "sherd_cache_from_bytes": (ci, [P(u8), sz, P(vp)]),
+        "sherd_cache_free": (None, [vp]),
+        "sherd_cache_info_get": (ci, [vp, P(CacheInfo)]),
+        "sherd_cache_arrays": (ci, [vp, P(CacheArraysOut)]),
+        "sherd_match_data": (ci, [vp, sz, P(MatchInfo), P(MatchArrays)]),
         "sherd_scene_build": (ci, [vp, P(f), sz, P(u32), sz, P(u8), P(vp)]),
         "sherd_scene_free": (None, [vp]),
         "sherd_scene_info": (ci, [vp, P(sz), P(f), P(f)]),
