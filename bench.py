@@ -1045,6 +1045,13 @@ def main():
 
     ex = sherd_hip.HipExecutor(0)
     log("device:", ex.device_name())
+    st = ex.selftest()  # the gate backend selection reads (selftest.rs:179)
+    log(f"selftest: {'passed' if st['passed'] else 'FAILED'}; " + "; ".join(f"{n}: {d}" for n, _, d in st["checks"]))
+    selftest = dict(passed=st["passed"], checks={n: dict(passed=ok, detail=d) for n, ok, d in st["checks"]},
+                    reduce_bits=hex(st["reduce_device_bits"]), nn_gpu_ms=round(st["gpu_ms"], 4),
+                    nn_cpu_ms=round(st["cpu_ms"], 3), speedup=round(st["speedup"], 2), queries=st["queries"])
+    if not st["passed"]:
+        sys.exit("the self-test failed: this device is not a backend")
     if args.collection_only or args.vessel_only:
         args.no_cpu = True
     if args.vessel_only:
@@ -1100,7 +1107,7 @@ def main():
         plain_pair_verifications_per_s=sec["plain_pair_verifications_per_s"], batched_vs_plain=sec["batched_vs_plain"],
         accepted=sec["accepted"], accepted_identical_plain_batched=sec["identical_plain_batched"]["accepted"],
         collection=sec, vessel=vessel, slab=slab, icp=icp, huber=huber, refine=refine, match=match, thickness=thick,
-        segment=segm, kernel_ms=head["prof"], roofline=roof, cpu_baseline=head["cpu"],
+        segment=segm, kernel_ms=head["prof"], roofline=roof, cpu_baseline=head["cpu"], selftest=selftest,
     )
     print(json.dumps(line), flush=True)
     ex.close()

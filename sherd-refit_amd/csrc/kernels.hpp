@@ -14,7 +14,7 @@ namespace sherd {
 // read back once the stream has synchronised.
 struct KernelProfiler {
   bool on = false;
-  static constexpr int MAXK = 16;
+  static constexpr int MAXK = 64;
   const char *name[MAXK] = {};
   double ms[MAXK] = {};
   uint64_t launches[MAXK] = {};

@@ -640,8 +640,8 @@ class HipExecutor:
     def profile_read(self):
         import json
 
-        buf = C.create_string_buffer(8192)
-        _check(lib().sherd_profile_read(self.ptr, buf, 8192))
+        buf = C.create_string_buffer(16384)
+        _check(lib().sherd_profile_read(self.ptr, buf, 16384))
         return json.loads(buf.value.decode())
 
     def scene(self, V, F, mask=None):
