@@ -189,6 +189,9 @@ struct sherd_fragment {
   int watertight = 0;
   double frac_area = 0.0;
   double *pf = nullptr, *s = nullptr, *brk_p = nullptr, *brk_ns = nullptr, *margin_p = nullptr, *margin_n = nullptr;
+  double *pf_blk = nullptr, *s_blk = nullptr;  // bounding spheres of Pf's and S's 64-point blocks
+  double pf_all[4] = {0, 0, 0, -1}, s_all[4] = {0, 0, 0, -1};  // and of each whole set
+  double brk_all[4] = {0, 0, 0, -1}, margin_all[4] = {0, 0, 0, -1};
   int n_pf = 0, n_s = 0, n_brk = 0, n_margin = 0;
   sherd::DevGrid margin_grid;  // rebuilt when a batch needs a larger reach
   double margin_reach = 0.0;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <climits>
 #include <cstring>
 #include <limits>
@@ -30,7 +31,7 @@ hipError_t upload_new(const double *h, size_t n, double **d, hipStream_t s) {
 }
 
 void free_fragment(sherd_fragment *f) {
-  double *ptrs[] = {f->pf, f->s, f->brk_p, f->brk_ns, f->margin_p, f->margin_n};
+  double *ptrs[] = {f->pf, f->s, f->brk_p, f->brk_ns, f->margin_p, f->margin_n, f->pf_blk, f->s_blk};
   for (double *p : ptrs)
     if (p) (void)hipFree(p);
   grid_free(&f->margin_grid);
@@ -77,6 +78,48 @@ std::vector<double> morton_ordered(const double *h, size_t n) {
   std::vector<double> out(3 * n);
   for (size_t j = 0; j < n; ++j) std::memcpy(&out[3 * j], h + 3 * (size_t)order[j].second, 3 * sizeof(double));
   return out;
+}
+
+// The bounding sphere of every block of `per` consecutive points (x, y, z, r): the centre of the
+// block's box and the largest distance from it to a point of the block, stepped up two ulps so the
+// radius is never below the exact one.
+std::vector<double> block_spheres(const std::vector<double> &p, size_t n, size_t per = 64) {
+  size_t nb = (n + per - 1) / per;
+  std::vector<double> out(4 * nb);
+  for (size_t b = 0; b < nb; ++b) {
+    size_t s = per * b, e = std::min(n, s + per);
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::numeric_limits<double>::infinity();
+      hi[k] = -std::numeric_limits<double>::infinity();
+    }
+    for (size_t i = s; i < e; ++i)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], p[3 * i + k]);
+        hi[k] = std::max(hi[k], p[3 * i + k]);
+      }
+    double c[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
+    double r2 = 0.0;
+    for (size_t i = s; i < e; ++i) {
+      double dx = p[3 * i] - c[0], dy = p[3 * i + 1] - c[1], dz = p[3 * i + 2] - c[2];
+      r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
+    }
+    double r = std::sqrt(r2);
+    r = std::nextafter(std::nextafter(r, INFINITY), INFINITY);
+    out[4 * b] = c[0];
+    out[4 * b + 1] = c[1];
+    out[4 * b + 2] = c[2];
+    out[4 * b + 3] = std::isfinite(r) ? r : INFINITY;
+  }
+  return out;
+}
+
+hipError_t upload_doubles(const std::vector<double> &h, double **d, hipStream_t s) {
+  *d = nullptr;
+  if (h.empty()) return hipSuccess;
+  hipError_t e = hipMalloc((void **)d, sizeof(double) * h.size());
+  if (e != hipSuccess) return e;
+  return hipMemcpyAsync(*d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, s);
 }
 
 uint32_t pow2_at_least(uint64_t x) {
@@ -128,6 +171,18 @@ int sherd_fragment_create(sherd_ctx *ctx, const sherd_scene *fracture, const she
   if (e == hipSuccess) e = upload_new(brk_ns, n_brk, &f->brk_ns, c->stream);
   if (e == hipSuccess) e = upload_new(margin_p, n_margin, &f->margin_p, c->stream);
   if (e == hipSuccess) e = upload_new(margin_n, n_margin, &f->margin_n, c->stream);
+  std::vector<double> pf_blk = block_spheres(pf_sorted, n_pf), s_blk = block_spheres(s_sorted, n_s);
+  if (n_pf) std::memcpy(f->pf_all, block_spheres(pf_sorted, n_pf, n_pf).data(), sizeof f->pf_all);
+  if (n_s) std::memcpy(f->s_all, block_spheres(s_sorted, n_s, n_s).data(), sizeof f->s_all);
+  if (n_brk)
+    std::memcpy(f->brk_all, block_spheres(std::vector<double>(brk_p, brk_p + 3 * n_brk), n_brk, n_brk).data(),
+                sizeof f->brk_all);
+  if (n_margin)
+    std::memcpy(f->margin_all,
+                block_spheres(std::vector<double>(margin_p, margin_p + 3 * n_margin), n_margin, n_margin).data(),
+                sizeof f->margin_all);
+  if (e == hipSuccess) e = upload_doubles(pf_blk, &f->pf_blk, c->stream);
+  if (e == hipSuccess) e = upload_doubles(s_blk, &f->s_blk, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) {
     free_fragment(f);
@@ -205,6 +260,12 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     d.frac_area = f->frac_area;
     d.pf = f->pf;
     d.s = f->s;
+    d.pf_blk = f->pf_blk;
+    d.s_blk = f->s_blk;
+    std::memcpy(d.pf_all, f->pf_all, sizeof d.pf_all);
+    std::memcpy(d.s_all, f->s_all, sizeof d.s_all);
+    std::memcpy(d.brk_all, f->brk_all, sizeof d.brk_all);
+    std::memcpy(d.margin_all, f->margin_all, sizeof d.margin_all);
     d.brk_p = f->brk_p;
     d.brk_ns = f->brk_ns;
     d.margin_p = f->margin_p;

@@ -12,6 +12,31 @@ namespace sherd {
 
 constexpr int DSTACK = 24;  // deferred nodes of an ordered traversal
 constexpr int MSTACK = 16;  // the same for min_d2, whose overflow walk starts from the best found
+constexpr int LSTACK = 8;   // min_d2's deferred nodes when they live in LDS (see LdsStack)
+
+// Where min_d2 keeps its deferred nodes. A per-lane array indexed by the depth compiles to
+// private (scratch) memory, whose spills go out to HBM; LdsStack is the lane's column of a
+// workgroup's LDS array [LSTACK][STRIDE] (STRIDE = the workgroup size, lane-minor, so the lanes of
+// a wave touch consecutive words; N entries), and RegStack the per-lane array.
+template <int STRIDE, int N = LSTACK>
+struct LdsStack {
+  static constexpr int CAP = N;
+  uint2 *col;  // &lds[0][threadIdx.x]
+  __device__ __forceinline__ void put(int k, int node, float d) { col[k * STRIDE] = make_uint2((uint32_t)node, __float_as_uint(d)); }
+  __device__ __forceinline__ int node(int k) const { return (int)col[k * STRIDE].x; }
+  __device__ __forceinline__ float dist(int k) const { return __uint_as_float(col[k * STRIDE].y); }
+};
+struct RegStack {
+  static constexpr int CAP = MSTACK;
+  int n[MSTACK];
+  float d[MSTACK];
+  __device__ __forceinline__ void put(int k, int node, float dd) {
+    n[k] = node;
+    d[k] = dd;
+  }
+  __device__ __forceinline__ int node(int k) const { return n[k]; }
+  __device__ __forceinline__ float dist(int k) const { return d[k]; }
+};
 
 __device__ __forceinline__ Node load_node(const Node *nodes, int i) {
   const float4 *p = reinterpret_cast<const float4 *>(nodes + i);
@@ -287,7 +312,8 @@ __device__ inline float min_d2_stackless(const SceneView &sv, V3 p, float window
 //
 // With a step budget (steps > 0) the walk gives up after that many nodes: *complete is then false
 // and the result is only the smallest distance met so far, one the point really attains.
-__device__ inline float min_d2(const SceneView &sv, V3 p, float window, int steps = 0, bool *complete = nullptr) {
+template <class Stack>
+__device__ inline float min_d2_with(Stack &st, const SceneView &sv, V3 p, float window, int steps, bool *complete) {
   if (complete) *complete = true;
   float best = __builtin_inff();
   float slack = slack_of(sv.absmax, p);
@@ -298,8 +324,6 @@ __device__ inline float min_d2(const SceneView &sv, V3 p, float window, int step
   if (cur < 0 || cur >= nodes) return best;
   Node nd = load_node(sv.nodes, cur);
   if (box_d2(nd, p, slack) > lim2) return best;
-  int st_n[MSTACK];
-  float st_d[MSTACK];
   int sp = 0;
   bool overflow = false;
   for (int iter = 0; iter <= nodes; ++iter) {
@@ -325,9 +349,8 @@ __device__ inline float min_d2(const SceneView &sv, V3 p, float window, int step
       bool hl = dl <= lim2, hr = dr <= lim2;
       if (hl && hr) {
         bool lf = dl <= dr;
-        if (sp < MSTACK) {
-          st_n[sp] = lf ? r : l;
-          st_d[sp] = lf ? dr : dl;
+        if (sp < Stack::CAP) {
+          st.put(sp, lf ? r : l, lf ? dr : dl);
           ++sp;
         } else {
           overflow = true;
@@ -345,7 +368,7 @@ __device__ inline float min_d2(const SceneView &sv, V3 p, float window, int step
     if (next < 0) {
       while (next < 0 && sp > 0) {
         --sp;
-        if (st_d[sp] <= lim2) next = st_n[sp];
+        if (st.dist(sp) <= lim2) next = st.node(sp);
       }
       if (next < 0) break;
       nd = load_node(sv.nodes, next);
@@ -359,6 +382,11 @@ __device__ inline float min_d2(const SceneView &sv, V3 p, float window, int step
     best = e < best ? e : best;
   }
   return best;
+}
+
+__device__ inline float min_d2(const SceneView &sv, V3 p, float window, int steps = 0, bool *complete = nullptr) {
+  RegStack st;
+  return min_d2_with(st, sv, p, window, steps, complete);
 }
 
 // The minimum over the triangles of node c's subtree within window (+inf when none is): the
@@ -470,9 +498,10 @@ __device__ inline float min_d2_wave(const SceneView &sv, V3 p, float window, int
 }
 
 // RayScene::bounded_distance (bvh.rs:123-135): the distance when it is at most max_dist.
-__device__ __forceinline__ bool bounded_distance(const SceneView &sv, V3 p, float max_dist, float *d) {
+template <class Stack>
+__device__ __forceinline__ bool bounded_distance_with(Stack &st, const SceneView &sv, V3 p, float max_dist, float *d) {
   if (!(max_dist > 0.0f)) return false;
-  float dist = sqrtf(min_d2(sv, p, max_dist));
+  float dist = sqrtf(min_d2_with(st, sv, p, max_dist, 0, nullptr));
   if (!(dist <= max_dist)) return false;
   *d = dist;
   return true;
@@ -480,7 +509,8 @@ __device__ __forceinline__ bool bounded_distance(const SceneView &sv, V3 p, floa
 
 // CpuExecutor::inside's per-point body (cpu.rs:182-188, bvh.rs:146-158): the AABB reject, three
 // axis rays with a majority of odd counts, and the unbounded distance of an inside point.
-__device__ __forceinline__ bool inside_depth(const SceneView &sv, V3 p, int retest, float *depth) {
+template <class Stack>
+__device__ __forceinline__ bool inside_depth_with(Stack &st, const SceneView &sv, V3 p, int retest, float *depth) {
   if (p.x < sv.lo[0] || p.x > sv.hi[0] || p.y < sv.lo[1] || p.y > sv.hi[1] || p.z < sv.lo[2] ||
       p.z > sv.hi[2])
     return false;
@@ -504,12 +534,21 @@ __device__ __forceinline__ bool inside_depth(const SceneView &sv, V3 p, int rete
   float d2 = __builtin_inff();
   if (w < __builtin_inff()) {
     w = w * 1.0001f + slack_of(sv.absmax, p);
-    d2 = min_d2(sv, p, w);
+    d2 = min_d2_with(st, sv, p, w, 0, nullptr);
     if (!(d2 <= w * w)) d2 = __builtin_inff();
   }
-  if (d2 == __builtin_inff()) d2 = min_d2(sv, p, __builtin_inff());
+  if (d2 == __builtin_inff()) d2 = min_d2_with(st, sv, p, __builtin_inff(), 0, nullptr);
   *depth = sqrtf(d2);
   return true;
+}
+
+__device__ __forceinline__ bool bounded_distance(const SceneView &sv, V3 p, float max_dist, float *d) {
+  RegStack st;
+  return bounded_distance_with(st, sv, p, max_dist, d);
+}
+__device__ __forceinline__ bool inside_depth(const SceneView &sv, V3 p, int retest, float *depth) {
+  RegStack st;
+  return inside_depth_with(st, sv, p, retest, depth);
 }
 
 }  // namespace sherd

@@ -149,6 +149,7 @@ hipError_t grid_build(const double *dp, int n, double reach, DevGrid *g, hipStre
     double ext = 0.0;
     for (int k = 0; k < 3; ++k) {
       g->origin[k] = hb[k];
+      g->hi[k] = hb[3 + k];
       ext = std::fmax(ext, hb[3 + k] - hb[k]);
     }
     // At most 2^19 cells per axis, and a little slack so that a point and a query one reach apart
@@ -206,7 +207,10 @@ GridView view_of(const DevGrid &d) {
   v.table = d.table;
   v.mask = d.mask;
   v.n = d.n;
-  for (int k = 0; k < 3; ++k) v.origin[k] = d.origin[k];
+  for (int k = 0; k < 3; ++k) {
+    v.origin[k] = d.origin[k];
+    v.hi[k] = d.hi[k];
+  }
   v.cell = d.cell;
   return v;
 }

@@ -29,9 +29,23 @@ struct GridView {
   const GridSlot *table;   // mask + 1 slots
   uint32_t mask;           // table capacity - 1
   int n;
-  double origin[3];
+  double origin[3];  // the low corner of the points' box
+  double hi[3];      // its high corner
   double cell;
 };
+
+// Whether every point of the grid is farther than `reach` from (x, y, z): the distance to the
+// points' box exceeds the reach by more than the rounding of any distance computed near it.
+SH_HD bool grid_far(const GridView &g, double x, double y, double z, double reach) {
+  double q[3] = {x, y, z}, s = 0.0, m = 1.0;
+  for (int k = 0; k < 3; ++k) {
+    double gap = fmax(fmax(g.origin[k] - q[k], q[k] - g.hi[k]), 0.0);
+    s += gap * gap;
+    m = fmax(m, fmax(fabs(q[k]), fmax(fabs(g.origin[k]), fabs(g.hi[k]))));
+  }
+  double r = reach * (1.0 + 1e-9) + 1e-9 * m;
+  return s > r * r;
+}
 
 SH_HD uint64_t grid_pack(int64_t x, int64_t y, int64_t z) {
   return (uint64_t)x | ((uint64_t)y << GRID_BITS) | ((uint64_t)z << (2 * GRID_BITS));

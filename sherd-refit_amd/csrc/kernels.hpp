@@ -72,7 +72,8 @@ struct DevGrid {
   GridSlot *table = nullptr;
   uint32_t mask = 0;
   int n = 0;
-  double origin[3] = {0, 0, 0};
+  double origin[3] = {0, 0, 0};  // the points' box: origin is its low corner
+  double hi[3] = {0, 0, 0};
   double cell = 1.0;
 };
 

@@ -42,6 +42,10 @@ constexpr int HARD_BLOCKS = 1024;  // workgroups of the cooperative pass
 // FLAG_GAVE_UP and the batch fails with a status (the caller answers it on the CPU), so the kernel
 // always ends in bounded time.
 constexpr int WAVE_STEPS = 1 << 19;
+// k_pen's depth searches start from an inside point with a window as wide as the nearest ray
+// crossing, so they defer more nodes than the windowed searches: a deeper LDS stack (27 KB of LDS
+// per workgroup, which keeps the five waves per SIMD its registers allow).
+constexpr int PEN_LSTACK = 12;
 
 __device__ __forceinline__ void apply_f64(const double *T, const double *p, double *o) {
   o[0] = T[0] * p[0] + T[1] * p[1] + T[2] * p[2] + T[3];
@@ -58,6 +62,62 @@ __device__ __forceinline__ double dot3(const double *a, const double *b) { retur
 // The (candidate, side) task and the point of this thread in a launch of per_task workgroups per
 // task. The tasks are ordered by the fragment they query (VerifyBatch::task), and xcd_logical keeps
 // a run of them on one XCD, so the workgroups that walk one mesh share that XCD's L2.
+// The bounding sphere of the 64-sample block holding sample i (a wave's lanes are one block),
+// moved by T: its centre in f64 and its radius.
+struct MovedBlock {
+  double c[3];
+  double r;
+};
+__device__ __forceinline__ MovedBlock moved_block(const double *blk, int i, const double *T) {
+  const double *q = blk + 4 * (size_t)(i >> 6);
+  MovedBlock m;
+  apply_f64(T, q, m.c);
+  m.r = q[3];
+  return m;
+}
+__device__ __forceinline__ V3 centre_f32(const MovedBlock &m) {
+  return V3{(float)m.c[0], (float)m.c[1], (float)m.c[2]};
+}
+// How far every point of the block is from the box [lo, hi] at least, less the pruning slack (which
+// covers the f32 rounding of each moved sample and of the distances computed from it): the
+// Euclidean gap from the centre to the box, minus the radius and the slack. Positive: no sample
+// of the block reaches the box, or anything in it, by that much.
+__device__ __forceinline__ double block_box_gap(const MovedBlock &m, const float *lo, const float *hi, float absmax) {
+  double g2 = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    double g = fmax(fmax((double)lo[k] - m.c[k], m.c[k] - (double)hi[k]), 0.0);
+    g2 += g * g;
+  }
+  return sqrt(g2) - m.r - (double)slack_of(absmax, centre_f32(m));
+}
+// A lower bound of every block sample's distance to the mesh (the distance bound at the centre,
+// less the radius and the slack; 0 when that is negative).
+__device__ __forceinline__ float block_bound(const SceneView &sv, const MovedBlock &m) {
+  V3 cf = centre_f32(m);
+  double b = (double)dist_bound(sv.db, cf) - m.r - (double)slack_of(sv.absmax, cf);
+  return b > 0.0 ? (float)b * 0.99999f : 0.0f;
+}
+
+__device__ __forceinline__ MovedBlock moved_sphere(const double *s, const double *T) {
+  MovedBlock m;
+  apply_f64(T, s, m.c);
+  m.r = s[3];
+  return m;
+}
+
+// The per-sample passes give a workgroup CHUNKS consecutive chunks of VT samples of one
+// (candidate, side) task. One: a workgroup walking eight chunks ran the vessel's passes 2.8x
+// slower (fewer waves in flight to hide the dependent loads of the traversals).
+constexpr int CHUNKS = 1;
+struct TaskGroup {
+  int c, side, chunk0;
+};
+__device__ __forceinline__ TaskGroup task_group(const VerifyBatch &vb, int per_task) {
+  uint32_t lb = xcd_logical(blockIdx.x, gridDim.x);
+  uint32_t cs = vb.task[lb / (uint32_t)per_task];
+  return TaskGroup{(int)(cs >> 1), (int)(cs & 1u), (int)(lb % (uint32_t)per_task) * CHUNKS};
+}
+
 struct TaskPos {
   int c, side, i;
 };
@@ -89,33 +149,48 @@ __global__ void k_prep(VerifyBatch vb) {
 // R §6.1 one_side inputs (verify.rs:318-349): side 0 is A's samples moved by T^-1 against B's
 // fracture, side 1 is B's samples moved by T against A's fracture.
 __global__ void __launch_bounds__(VT) k_frac(VerifyBatch vb, int per_task) {
-  TaskPos tp = task_pos(vb, per_task);
-  int c = tp.c, side = tp.side, i = tp.i;
+  __shared__ uint2 s_stk[LSTACK][VT];
+  const TaskGroup g = task_group(vb, per_task);
+  const int c = g.c, side = g.side;
   const CandDesc &cd = vb.cand[c];
   const PairDesc &pd = vb.pair[cd.pair];
   const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
   const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
-  bool valid = i < own.n_pf;
-  bool facing = false, tight = false, touching = false;
-  double d64 = 0.0;
-  if (valid) {
-    const double *T = side == 0 ? cd.Ti : cd.T;
-    const double *p = own.pf + 3 * (size_t)i;
-    V3 q = moved_f32(T, p[0], p[1], p[2]);
-    float d;
-    float window = (float)pd.sc[S_FACING];
-    d64 = bounded_distance(other.frac, q, window, &d) ? (double)d : __builtin_inf();
-    facing = d64 < pd.sc[S_FACING];
-    tight = facing && d64 < pd.sc[S_TIGHT];
-    touching = d64 < 2.0 * pd.sc[S_TIGHT];
-  }
+  const double *T = side == 0 ? cd.Ti : cd.T;
+  const double facing_r = pd.sc[S_FACING];
+  // Samples beyond the facing distance of the fracture scene's box add nothing (no facing, tight
+  // or touching sample): the whole set, then each wave's block, is tested first.
+  if (own.n_pf == 0 ||
+      block_box_gap(moved_sphere(own.pf_all, T), other.frac.lo, other.frac.hi, other.frac.absmax) > facing_r)
+    return;
   CandAcc &acc = vb.acc[c];
-  uint32_t slot = wave_slot(&acc.facing[side], facing);
-  if (facing) vb.keys[cd.seg[side] + slot] = total_key(d64);
-  uint32_t nt = (uint32_t)__popcll(__ballot(tight)), nh = (uint32_t)__popcll(__ballot(touching));
-  if ((threadIdx.x & 63) == 0) {
-    if (nt) atomicAdd(&acc.tight[side], nt);
-    if (nh) atomicAdd(&acc.touching[side], nh);
+  LdsStack<VT> st{&s_stk[0][threadIdx.x]};
+  for (int k = 0; k < CHUNKS; ++k) {
+    const int base = (g.chunk0 + k) * VT;
+    if (base >= own.n_pf) break;  // (uniform)
+    const int i = base + threadIdx.x, i0 = i & ~63;
+    if (i0 >= own.n_pf ||
+        block_box_gap(moved_block(own.pf_blk, i0, T), other.frac.lo, other.frac.hi, other.frac.absmax) > facing_r)
+      continue;  // (uniform across the wave)
+    bool facing = false, tight = false, touching = false;
+    double d64 = 0.0;
+    if (i < own.n_pf) {
+      const double *p = own.pf + 3 * (size_t)i;
+      V3 q = moved_f32(T, p[0], p[1], p[2]);
+      float d;
+      float window = (float)facing_r;
+      d64 = bounded_distance_with(st, other.frac, q, window, &d) ? (double)d : __builtin_inf();
+      facing = d64 < facing_r;
+      tight = facing && d64 < pd.sc[S_TIGHT];
+      touching = d64 < 2.0 * pd.sc[S_TIGHT];
+    }
+    uint32_t slot = wave_slot(&acc.facing[side], facing);
+    if (facing) vb.keys[cd.seg[side] + slot] = total_key(d64);
+    uint32_t nt = (uint32_t)__popcll(__ballot(tight)), nh = (uint32_t)__popcll(__ballot(touching));
+    if ((threadIdx.x & 63) == 0) {
+      if (nt) atomicAdd(&acc.tight[side], nt);
+      if (nh) atomicAdd(&acc.touching[side], nh);
+    }
   }
 }
 
@@ -125,10 +200,12 @@ __global__ void __launch_bounds__(VT) k_frac(VerifyBatch vb, int per_task) {
 // T^-1 p; a point within the reach of p in A's frame is within the reach (plus rounding, which the
 // reach's margin covers) of T^-1 p in B's, so it is among the 27 cells searched.
 __device__ inline bool seam_nearest(const GridView &g, const double *T, const double *Ti, const double *bp,
-                                    const double *p, uint32_t *out_j, double *out_d2) {
+                                    const double *p, double reach, uint32_t *out_j, double *out_d2) {
   if (g.n <= 0) return false;
   double q[3];
   apply_f64(Ti, p, q);
+  // Nothing of B within the reach (the caller keeps only points strictly inside it): no scan.
+  if (grid_far(g, q[0], q[1], q[2], reach)) return false;
   double fx = floor((q[0] - g.origin[0]) / g.cell);
   double fy = floor((q[1] - g.origin[1]) / g.cell);
   double fz = floor((q[2] - g.origin[2]) / g.cell);
@@ -164,116 +241,159 @@ __global__ void __launch_bounds__(VT) k_seam_clear(VerifyBatch vb) {
 // moved point of B's breakline (f64, ties to the lowest index); a seam point when it is strictly
 // inside sc.seam and the two shell normals agree; the score counts distinct t/3 voxels of them.
 __global__ void __launch_bounds__(VT) k_seam(VerifyBatch vb) {
-  int c = blockIdx.y;
+  const int c = blockIdx.y;
   const CandDesc &cd = vb.cand[c];
   const PairDesc &pd = vb.pair[cd.pair];
   const FragDesc &A = vb.frag[pd.a];
   const FragDesc &B = vb.frag[pd.b];
-  int i = blockIdx.x * VT + threadIdx.x;
-  double seam = pd.sc[S_SEAM];
-  if (i >= A.n_brk || !(seam > 0.0)) return;
-  const double *p = A.brk_p + 3 * (size_t)i;
-  uint32_t j;
-  double d2;
-  if (!seam_nearest(B.brk, cd.T, cd.Ti, B.brk_p, p, &j, &d2)) return;
-  if (!(sqrt(d2) < seam)) return;
-  double nb[3];
-  rotate_f64(cd.T, B.brk_ns + 3 * (size_t)j, nb);
-  if (!(dot3(A.brk_ns + 3 * (size_t)i, nb) > 0.7)) return;
-  double voxel = pd.sc[S_T] / 3.0;
-  int64_t x = (int64_t)floor(p[0] / voxel), y = (int64_t)floor(p[1] / voxel), z = (int64_t)floor(p[2] / voxel);
-  const int64_t H = 1ll << 20;
-  if (x < -H || x >= H || y < -H || y >= H || z < -H || z >= H) {
-    atomicOr(&vb.acc[c].flags, FLAG_VOXEL_RANGE);
-    return;
+  const double seam = pd.sc[S_SEAM];
+  if (A.n_brk == 0 || B.brk.n <= 0 || !(seam > 0.0)) return;
+  // A's whole breakline, moved into B's frame, farther than the seam from B's: nothing to find.
+  {
+    const MovedBlock all = moved_sphere(A.brk_all, cd.Ti);
+    if (grid_far(B.brk, all.c[0], all.c[1], all.c[2], seam + all.r)) return;
   }
+  const double voxel = pd.sc[S_T] / 3.0;
+  const int64_t H = 1ll << 20;
   unsigned long long *table = (unsigned long long *)(vb.seam_table + cd.seam_off);
-  uint32_t mask = cd.seam_mask;
-  uint64_t key = (uint64_t)(x + H) | ((uint64_t)(y + H) << 21) | ((uint64_t)(z + H) << 42);
-  uint32_t slot = grid_hash(key, mask);
-  for (uint32_t probe = 0; probe <= mask; ++probe) {
-    unsigned long long prev = atomicCAS(&table[slot], ~0ull, (unsigned long long)key);
-    if (prev == ~0ull) {
-      atomicAdd(&vb.acc[c].seam_cells, 1u);
-      break;
+  const uint32_t mask = cd.seam_mask;
+  for (int k = 0; k < CHUNKS; ++k) {
+    const int base = ((int)blockIdx.x * CHUNKS + k) * VT;
+    if (base >= A.n_brk) break;  // (uniform)
+    const int i = base + threadIdx.x;
+    if (i >= A.n_brk) continue;
+    const double *p = A.brk_p + 3 * (size_t)i;
+    uint32_t j;
+    double d2;
+    if (!seam_nearest(B.brk, cd.T, cd.Ti, B.brk_p, p, seam, &j, &d2)) continue;
+    if (!(sqrt(d2) < seam)) continue;
+    double nb[3];
+    rotate_f64(cd.T, B.brk_ns + 3 * (size_t)j, nb);
+    if (!(dot3(A.brk_ns + 3 * (size_t)i, nb) > 0.7)) continue;
+    int64_t x = (int64_t)floor(p[0] / voxel), y = (int64_t)floor(p[1] / voxel), z = (int64_t)floor(p[2] / voxel);
+    if (x < -H || x >= H || y < -H || y >= H || z < -H || z >= H) {
+      atomicOr(&vb.acc[c].flags, FLAG_VOXEL_RANGE);
+      continue;
     }
-    if (prev == key) break;
-    slot = (slot + 1) & mask;
+    uint64_t key = (uint64_t)(x + H) | ((uint64_t)(y + H) << 21) | ((uint64_t)(z + H) << 42);
+    uint32_t slot = grid_hash(key, mask);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+      unsigned long long prev = atomicCAS(&table[slot], ~0ull, (unsigned long long)key);
+      if (prev == ~0ull) {
+        atomicAdd(&vb.acc[c].seam_cells, 1u);
+        break;
+      }
+      if (prev == key) break;
+      slot = (slot + 1) & mask;
+    }
   }
 }
 
 // R §6.3 continuity_scores (verify.rs:450-474).
 __global__ void __launch_bounds__(VT) k_cont(VerifyBatch vb) {
-  int c = blockIdx.y;
+  const int c = blockIdx.y;
   const CandDesc &cd = vb.cand[c];
   const PairDesc &pd = vb.pair[cd.pair];
   const FragDesc &A = vb.frag[pd.a];
   const FragDesc &B = vb.frag[pd.b];
-  int i = blockIdx.x * VT + threadIdx.x;
-  if (i >= B.n_margin || A.n_margin == 0) return;
-  double moved[3];
-  apply_f64(cd.T, B.margin_p + 3 * (size_t)i, moved);
-  uint32_t j;
-  double dist;
-  if (!grid_nearest_below(A.margin, moved[0], moved[1], moved[2], pd.sc[S_NEAR], &j, &dist)) return;
-  const double *np = A.margin_p + 3 * (size_t)j, *nn = A.margin_n + 3 * (size_t)j;
-  double delta[3] = {moved[0] - np[0], moved[1] - np[1], moved[2] - np[2]};
-  double step = fabs(dot3(delta, nn));
-  double turned[3];
-  rotate_f64(cd.T, B.margin_n + 3 * (size_t)i, turned);
-  double agree = dot3(turned, nn);
-  uint32_t slot = wave_slot(&vb.acc[c].n_cont, true);
-  vb.keys[cd.seg[2] + slot] = total_key(step);
-  vb.keys[cd.seg[3] + slot] = total_key(agree);
+  const double near = pd.sc[S_NEAR];
+  if (B.n_margin == 0 || A.n_margin == 0) return;
+  // B's whole margin, moved into A's frame, farther than `near` from A's: nothing to pair.
+  {
+    const MovedBlock all = moved_sphere(B.margin_all, cd.T);
+    if (grid_far(A.margin, all.c[0], all.c[1], all.c[2], near + all.r)) return;
+  }
+  CandAcc &acc = vb.acc[c];
+  for (int k = 0; k < CHUNKS; ++k) {
+    const int base = ((int)blockIdx.x * CHUNKS + k) * VT;
+    if (base >= B.n_margin) break;  // (uniform)
+    const int i = base + threadIdx.x;
+    bool take = false;
+    double step = 0.0, agree = 0.0;
+    if (i < B.n_margin) {
+      double moved[3];
+      apply_f64(cd.T, B.margin_p + 3 * (size_t)i, moved);
+      uint32_t j;
+      double dist;
+      if (!grid_far(A.margin, moved[0], moved[1], moved[2], near) &&
+          grid_nearest_below(A.margin, moved[0], moved[1], moved[2], near, &j, &dist)) {
+        const double *np = A.margin_p + 3 * (size_t)j, *nn = A.margin_n + 3 * (size_t)j;
+        double delta[3] = {moved[0] - np[0], moved[1] - np[1], moved[2] - np[2]};
+        step = fabs(dot3(delta, nn));
+        double turned[3];
+        rotate_f64(cd.T, B.margin_n + 3 * (size_t)i, turned);
+        agree = dot3(turned, nn);
+        take = true;
+      }
+    }
+    uint32_t slot = wave_slot(&acc.n_cont, take);
+    if (take) {
+      vb.keys[cd.seg[2] + slot] = total_key(step);
+      vb.keys[cd.seg[3] + slot] = total_key(agree);
+    }
+  }
 }
 
 // R §6.4 one_penetration's first pass (verify.rs:541-566): side 0 is A's surface samples moved by
 // T^-1 against B's mesh, side 1 is B's moved by T against A's mesh.
 __global__ void __launch_bounds__(VT) k_pen(VerifyBatch vb, int per_task) {
-  TaskPos tp = task_pos(vb, per_task);
-  int c = tp.c, side = tp.side, i = tp.i;
+  const TaskGroup g = task_group(vb, per_task);
+  const int c = g.c, side = g.side;
   const CandDesc &cd = vb.cand[c];
   const PairDesc &pd = vb.pair[cd.pair];
   const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
   const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
-  if (!pd.pen_ok) return;  // (uniform: a workgroup belongs to one task)
-  // Most samples fall outside the other mesh's box (RayScene::inside's reject). The workgroup
-  // gathers the ones inside it in LDS and its first lanes take them, one each, so the rays and
-  // depth searches run in as few waves as there are such samples; waves left without one finish.
+  if (!pd.pen_ok || own.n_s == 0) return;  // (uniform: a workgroup belongs to one task)
+  const SceneView &sv = other.mesh;
+  const double *T = side == 0 ? cd.Ti : cd.T;
+  // No sample can be inside when the whole set lies outside the mesh's box (by more than the
+  // rounding), nor a block's when its sphere does: those skip the moves.
+  if (block_box_gap(moved_sphere(own.s_all, T), sv.lo, sv.hi, sv.absmax) > 0.0) return;
+  // Most samples fall outside the other mesh's box (RayScene::inside's reject). Per chunk, the
+  // workgroup gathers the ones inside it in LDS and its first lanes take them, one each, so the
+  // rays and depth searches run in as few waves as there are such samples.
   __shared__ float s_q[3][VT];
   __shared__ uint32_t s_n;
-  if (threadIdx.x == 0) s_n = 0;
-  __syncthreads();
-  const SceneView &sv = other.mesh;
-  if (i < own.n_s) {
-    const double *T = side == 0 ? cd.Ti : cd.T;
-    const double *p = own.s + 3 * (size_t)i;
-    V3 q = moved_f32(T, p[0], p[1], p[2]);
-    if (!(q.x < sv.lo[0] || q.x > sv.hi[0] || q.y < sv.lo[1] || q.y > sv.hi[1] || q.z < sv.lo[2] ||
-          q.z > sv.hi[2])) {
-      uint32_t k = atomicAdd(&s_n, 1u);
-      s_q[0][k] = q.x;
-      s_q[1][k] = q.y;
-      s_q[2][k] = q.z;
-    }
-  }
-  __syncthreads();
-  const uint32_t n = s_n;
-  if (threadIdx.x >= ((n + 63u) & ~63u)) return;  // whole waves only
-  bool in = false, deeper = false;
-  float depth = 0.0f;
-  if (threadIdx.x < n) {
-    V3 q = V3{s_q[0][threadIdx.x], s_q[1][threadIdx.x], s_q[2][threadIdx.x]};
-    in = inside_depth(sv, q, vb.retest, &depth);
-    deeper = in && (double)depth > pd.sc[S_PEN];
-  }
+  __shared__ uint2 s_stk[PEN_LSTACK][VT];
   CandAcc &acc = vb.acc[c];
-  uint32_t deepest = wave_max(in ? __float_as_uint(depth) : 0u);
-  uint32_t ni = (uint32_t)__popcll(__ballot(in)), nd = (uint32_t)__popcll(__ballot(deeper));
-  if ((threadIdx.x & 63) == 0) {
-    if (ni) atomicAdd(&acc.inside[side], ni);
-    if (nd) atomicAdd(&acc.deeper[side], nd);
-    if (deepest) atomicMax(&acc.deepest[side], deepest);
+  LdsStack<VT, PEN_LSTACK> st{&s_stk[0][threadIdx.x]};
+  for (int k = 0; k < CHUNKS; ++k) {
+    const int base = (g.chunk0 + k) * VT;
+    if (base >= own.n_s) break;  // (uniform)
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const int i = base + threadIdx.x, i0 = i & ~63;
+    const bool skip = i0 >= own.n_s || block_box_gap(moved_block(own.s_blk, i0, T), sv.lo, sv.hi, sv.absmax) > 0.0;
+    if (i < own.n_s && !skip) {
+      const double *p = own.s + 3 * (size_t)i;
+      V3 q = moved_f32(T, p[0], p[1], p[2]);
+      if (!(q.x < sv.lo[0] || q.x > sv.hi[0] || q.y < sv.lo[1] || q.y > sv.hi[1] || q.z < sv.lo[2] ||
+            q.z > sv.hi[2])) {
+        uint32_t s = atomicAdd(&s_n, 1u);
+        s_q[0][s] = q.x;
+        s_q[1][s] = q.y;
+        s_q[2][s] = q.z;
+      }
+    }
+    __syncthreads();
+    const uint32_t n = s_n;
+    if (threadIdx.x < ((n + 63u) & ~63u)) {  // whole waves only
+      bool in = false, deeper = false;
+      float depth = 0.0f;
+      if (threadIdx.x < n) {
+        V3 q = V3{s_q[0][threadIdx.x], s_q[1][threadIdx.x], s_q[2][threadIdx.x]};
+        in = inside_depth_with(st, sv, q, vb.retest, &depth);
+        deeper = in && (double)depth > pd.sc[S_PEN];
+      }
+      uint32_t deepest = wave_max(in ? __float_as_uint(depth) : 0u);
+      uint32_t ni = (uint32_t)__popcll(__ballot(in)), nd = (uint32_t)__popcll(__ballot(deeper));
+      if ((threadIdx.x & 63) == 0) {
+        if (ni) atomicAdd(&acc.inside[side], ni);
+        if (nd) atomicAdd(&acc.deeper[side], nd);
+        if (deepest) atomicMax(&acc.deepest[side], deepest);
+      }
+    }
+    __syncthreads();  // the gathered samples are read before the next chunk writes them
   }
 }
 
@@ -304,11 +424,12 @@ __device__ __forceinline__ float closest_window(CandAcc &acc, int side) {
 // min(sd) is then the closest approach, a running minimum over the samples whose window shrinks to
 // the best attained so far (DistReduce::Min, cpu.rs:100-111), from f32::MAX.
 
-// The seed's first half: each side's sample of smallest bound (the distance-bound grid's value at the
-// moved sample), (bound bits << 32) | sample, kept by an atomic minimum.
+// The seed's first half: each side's 64-sample block of smallest bound (the distance bound at the
+// moved centre less the radius), (bound bits << 32) | block, kept by an atomic minimum. One lane
+// per block.
 __global__ void __launch_bounds__(VT) k_pen_lb(VerifyBatch vb, int per_task) {
   TaskPos tp = task_pos(vb, per_task);
-  const int c = tp.c, side = tp.side, i = tp.i;
+  const int c = tp.c, side = tp.side, b = tp.i;
   const CandDesc &cd = vb.cand[c];
   const PairDesc &pd = vb.pair[cd.pair];
   if (!pd.pen_ok) return;
@@ -317,11 +438,9 @@ __global__ void __launch_bounds__(VT) k_pen_lb(VerifyBatch vb, int per_task) {
   const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
   const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
   uint64_t key = ~0ull;
-  if (i < own.n_s) {
+  if (64 * (size_t)b < (size_t)own.n_s) {
     const double *T = side == 0 ? cd.Ti : cd.T;
-    const double *p = own.s + 3 * (size_t)i;
-    const V3 q = moved_f32(T, p[0], p[1], p[2]);
-    key = ((uint64_t)__float_as_uint(dist_bound(other.mesh.db, q)) << 32) | (uint32_t)i;
+    key = ((uint64_t)__float_as_uint(block_bound(other.mesh, moved_block(own.s_blk, 64 * b, T))) << 32) | (uint32_t)b;
   }
   const uint32_t hi = wave_min((uint32_t)(key >> 32));
   const uint32_t lo = wave_min((uint32_t)(key >> 32) == hi ? (uint32_t)key : 0xffffffffu);
@@ -334,8 +453,9 @@ __global__ void __launch_bounds__(VT) k_pen_lb(VerifyBatch vb, int per_task) {
 // others search (min_d2). A query that needs more than PEN_STEPS nodes is finished by a whole wave
 // (k_pen_hard); what it met so far is a distance it attains, so it may still lower the window.
 __global__ void __launch_bounds__(VT) k_pen_min(VerifyBatch vb, int per_task) {
-  TaskPos tp = task_pos(vb, per_task);
-  int c = tp.c, side = tp.side, i = tp.i;
+  __shared__ uint2 s_stk[LSTACK][VT];
+  const TaskGroup g = task_group(vb, per_task);
+  const int c = g.c, side = g.side;
   const CandDesc &cd = vb.cand[c];
   const PairDesc &pd = vb.pair[cd.pair];
   if (!pd.pen_ok) return;
@@ -343,24 +463,36 @@ __global__ void __launch_bounds__(VT) k_pen_min(VerifyBatch vb, int per_task) {
   if (!closest_needed(acc, side)) return;
   const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
   const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
-  uint32_t best = 0xffffffffu;
-  bool hard = false;
-  if (i < own.n_s) {
-    const double *T = side == 0 ? cd.Ti : cd.T;
-    const double *p = own.s + 3 * (size_t)i;
-    V3 q = moved_f32(T, p[0], p[1], p[2]);
-    float window = closest_window(acc, side);
-    if (dist_bound(other.mesh.db, q) <= window + slack_of(other.mesh.absmax, q) && window > 0.0f) {
-      bool complete;
-      float dist = sqrtf(min_d2(other.mesh, q, window, PEN_STEPS, &complete));
-      if (dist <= window) best = __float_as_uint(dist);
-      hard = !complete;
+  const double *T = side == 0 ? cd.Ti : cd.T;
+  if (own.n_s == 0) return;
+  LdsStack<VT> st{&s_stk[0][threadIdx.x]};
+  for (int k = 0; k < CHUNKS; ++k) {
+    const int base = (g.chunk0 + k) * VT;
+    if (base >= own.n_s) break;  // (uniform)
+    const int i = base + threadIdx.x, i0 = i & ~63;
+    if (i0 >= own.n_s) break;
+    // A block every sample of which is farther than the window (read once for the wave) cannot
+    // lower the minimum: the wave goes on to its next block. Uniform across the wave.
+    const float w0 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(closest_window(acc, side))));
+    if (block_bound(other.mesh, moved_block(own.s_blk, i0, T)) > w0) continue;
+    uint32_t best = 0xffffffffu;
+    bool hard = false;
+    if (i < own.n_s) {
+      const double *p = own.s + 3 * (size_t)i;
+      V3 q = moved_f32(T, p[0], p[1], p[2]);
+      float window = closest_window(acc, side);
+      if (dist_bound(other.mesh.db, q) <= window + slack_of(other.mesh.absmax, q) && window > 0.0f) {
+        bool complete;
+        float dist = sqrtf(min_d2_with(st, other.mesh, q, window, PEN_STEPS, &complete));
+        if (dist <= window) best = __float_as_uint(dist);
+        hard = !complete;
+      }
     }
+    uint32_t slot = wave_slot(&vb.hard_n[0], hard);
+    if (hard) vb.hard[slot] = ((uint64_t)c << 33) | ((uint64_t)side << 32) | (uint32_t)i;
+    best = wave_min(best);
+    if ((threadIdx.x & 63) == 0 && best != 0xffffffffu) atomicMin(&acc.closest[side], best);
   }
-  uint32_t slot = wave_slot(&vb.hard_n[0], hard);
-  if (hard) vb.hard[slot] = ((uint64_t)c << 33) | ((uint64_t)side << 32) | (uint32_t)i;
-  best = wave_min(best);
-  if ((threadIdx.x & 63) == 0 && best != 0xffffffffu) atomicMin(&acc.closest[side], best);
 }
 
 // The k-th smallest key (0-based) of n by an 8-pass MSD radix select.
@@ -417,11 +549,11 @@ __device__ K select_kth(const K *keys, int n, int k, uint32_t *hist, K *s_prefix
 }
 
 // The closest approach is a minimum over every sample, and a search is cheap once a small
-// window is known. Before the exact pass, each task's sample with the smallest distance bound is
-// found (k_pen_lb), and a whole wave measures that sample's exact distance (k_pen_seed,
-// min_d2_wave): a distance the samples really attain, so at least the true minimum, and, the bound
-// being within a couple of grid cells of the distance, close to it. The minimum that comes out is
-// the same; only the work changes.
+// window is known. Before the exact pass, each task's sample of smallest distance bound is found
+// (k_pen_lb over blocks, then k_pen_seed over the samples of the blocks that can hold it), and the
+// same wave measures that sample's exact distance (min_d2_wave): a distance the samples really attain, so at least the true minimum,
+// and, the bound being within a couple of grid cells of the distance, close to it. The minimum that
+// comes out is the same whichever sample seeds it; only the work changes.
 __global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb) {
   __shared__ int stk[VT / 64][HARD_STACK];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -434,12 +566,51 @@ __global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb) {
   if (!pd.pen_ok) return;
   CandAcc &acc = vb.acc[c];
   if (!closest_needed(acc, side)) return;
-  const uint64_t key = acc.seed[side];
-  if (key == ~0ull) return;
   const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
   const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
   const double *T = side == 0 ? cd.Ti : cd.T;
-  const double *p = own.s + 3 * (size_t)(uint32_t)key;
+  const uint64_t key = acc.seed[side];
+  if (key == ~0ull) return;
+  // The task's sample of smallest distance bound, (bound bits, index) least: first over the block
+  // of smallest block bound, then over every other block whose block bound (less the bound grid's
+  // cell diagonal, by which a sample's bound may fall below its block's) does not exceed the best
+  // so far; the blocks are tested 64 at a time, each qualifying one by the whole wave.
+  const int bstar = (int)(uint32_t)key;
+  const int nb = (own.n_s + 63) >> 6;
+  const float cell = 1.8f * other.mesh.db.h;
+  uint32_t khi, seed;
+  {
+    const int i = 64 * bstar + lane;
+    uint64_t k2 = ~0ull;
+    if (i < own.n_s) {
+      const double *pi = own.s + 3 * (size_t)i;
+      k2 = ((uint64_t)__float_as_uint(dist_bound(other.mesh.db, moved_f32(T, pi[0], pi[1], pi[2]))) << 32) | (uint32_t)i;
+    }
+    khi = wave_min((uint32_t)(k2 >> 32));
+    seed = wave_min((uint32_t)(k2 >> 32) == khi ? (uint32_t)k2 : 0xffffffffu);
+  }
+  if (seed == 0xffffffffu) return;  // (uniform)
+  for (int base = 0; base < nb; base += 64) {
+    const int b = base + lane;
+    const bool take = b < nb && b != bstar &&
+                      block_bound(other.mesh, moved_block(own.s_blk, 64 * b, T)) - cell <= __uint_as_float(khi);
+    uint64_t m = __ballot(take);
+    while (m) {
+      const int bb = base + __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      const int i = 64 * bb + lane;
+      uint64_t k2 = ~0ull;
+      if (i < own.n_s) {
+        const double *pi = own.s + 3 * (size_t)i;
+        k2 = ((uint64_t)__float_as_uint(dist_bound(other.mesh.db, moved_f32(T, pi[0], pi[1], pi[2]))) << 32) | (uint32_t)i;
+      }
+      const uint64_t cur = ((uint64_t)khi << 32) | seed;
+      k2 = k2 < cur ? k2 : cur;
+      khi = wave_min((uint32_t)(k2 >> 32));
+      seed = wave_min((uint32_t)(k2 >> 32) == khi ? (uint32_t)k2 : 0xffffffffu);
+    }
+  }
+  const double *p = own.s + 3 * (size_t)seed;
   V3 q = moved_f32(T, p[0], p[1], p[2]);
   float window = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(closest_window(acc, side))));
   bool complete;
@@ -609,10 +780,11 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
   E(s);
   const bool pen = vb.full && max_s > 0;
   const int per_s = blocks_for(max_s, VT);
+  const int per_sg = blocks_for(per_s, CHUNKS);  // workgroups per task of the per-sample passes
   if (pen) {
     // R §6.4's inside pass first and alone: it is the longest kernel and fills the device.
     B("k_pen", s);
-    hipLaunchKernelGGL(k_pen, dim3(2 * nc * per_s), dim3(VT), 0, s, vb, per_s);
+    hipLaunchKernelGGL(k_pen, dim3(2 * nc * per_sg), dim3(VT), 0, s, vb, per_sg);
     E(s);
   }
   // The sections below read the poses k_prep completed and write disjoint accumulators and key
@@ -628,7 +800,7 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
     s3 = ss->side[2];
   }
   if (max_pf > 0) {
-    int per = blocks_for(max_pf, VT);
+    int per = blocks_for(blocks_for(max_pf, VT), CHUNKS);
     B("k_frac", s1);
     hipLaunchKernelGGL(k_frac, dim3(2 * nc * per), dim3(VT), 0, s1, vb, per);
     E(s1);
@@ -638,23 +810,24 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
   E(s2);
   if (max_brk > 0) {
     B("k_seam", s2);
-    hipLaunchKernelGGL(k_seam, dim3(blocks_for(max_brk, VT), nc), dim3(VT), 0, s2, vb);
+    hipLaunchKernelGGL(k_seam, dim3(blocks_for(blocks_for(max_brk, VT), CHUNKS), nc), dim3(VT), 0, s2, vb);
     E(s2);
   }
   if (vb.full && max_margin > 0) {
     B("k_cont", s3);
-    hipLaunchKernelGGL(k_cont, dim3(blocks_for(max_margin, VT), nc), dim3(VT), 0, s3, vb);
+    hipLaunchKernelGGL(k_cont, dim3(blocks_for(blocks_for(max_margin, VT), CHUNKS), nc), dim3(VT), 0, s3, vb);
     E(s3);
   }
   if (pen) {
     B("k_pen_lb", s);
-    hipLaunchKernelGGL(k_pen_lb, dim3(2 * nc * per_s), dim3(VT), 0, s, vb, per_s);
+    const int per_b = blocks_for(blocks_for(max_s, 64), VT);
+    hipLaunchKernelGGL(k_pen_lb, dim3(2 * nc * per_b), dim3(VT), 0, s, vb, per_b);
     E(s);
     B("k_pen_seed", s);
     hipLaunchKernelGGL(k_pen_seed, dim3(blocks_for(2L * nc, VT / 64)), dim3(VT), 0, s, vb);
     E(s);
     B("k_pen_min", s);
-    hipLaunchKernelGGL(k_pen_min, dim3(2 * nc * per_s), dim3(VT), 0, s, vb, per_s);
+    hipLaunchKernelGGL(k_pen_min, dim3(2 * nc * per_sg), dim3(VT), 0, s, vb, per_sg);
     E(s);
     B("k_pen_hard", s);
     hipLaunchKernelGGL(k_pen_hard, dim3(HARD_BLOCKS), dim3(VT), 0, s, vb);

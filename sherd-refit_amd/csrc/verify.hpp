@@ -31,6 +31,15 @@ struct FragDesc {
   const double *brk_ns;
   const double *margin_p;
   const double *margin_n;
+  // Per 64-point block of pf and of s (Morton order, so a block is a compact patch): the centre of
+  // its box and the largest distance from it (x, y, z, r; f64, r rounded up). A wave's 64 lanes are
+  // one block, so one sphere test can dismiss the whole wave's samples.
+  const double *pf_blk;
+  const double *s_blk;
+  double pf_all[4];  // the bounding spheres of all of pf, s, brk_p and margin_p (x, y, z, r)
+  double s_all[4];
+  double brk_all[4];
+  double margin_all[4];
   int n_pf, n_s, n_brk, n_margin;
   GridView margin;  // over margin_p, cell at least the largest `near` of the batch
   GridView brk;     // over brk_p, cell beyond the largest `seam` of the batch
@@ -61,7 +70,7 @@ struct CandAcc {
   uint32_t flags;
   uint32_t pad;
   double med[4];
-  uint64_t seed[2];  // per side, (distance bound bits << 32) | sample of the smallest bound: the seed
+  uint64_t seed[2];  // per side, (block bound bits << 32) | the 64-sample block of the smallest bound
 };
 
 struct VerifyBatch {
