@@ -389,18 +389,22 @@ __global__ void k_db_fill(float *d, size_t n, float v) {
   if (i < n) d[i] = v;
 }
 
-__global__ void k_db_mark(const Tri *tris, int m, DistBound g, float pad, float *d) {
+__global__ void k_db_mark(const Tri *tris, int m, DistBound g, float pad, float *d, uint32_t *diag) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
   const Tri &t = tris[j];
   int c0[3], c1[3];
   float inv = 1.0f / g.h;
+  float e2 = 0.0f;
   for (int k = 0; k < 3; ++k) {
     float lo = fminf(t.v[k], fminf(t.v[3 + k], t.v[6 + k])) - pad;
     float hi = fmaxf(t.v[k], fmaxf(t.v[3 + k], t.v[6 + k])) + pad;
+    e2 += (hi - lo) * (hi - lo);
     c0[k] = grid_cell(lo, g.org[k], inv, g.n[k]);
     c1[k] = grid_cell(hi, g.org[k], inv, g.n[k]);
   }
+  // The largest padded triangle box diagonal (non-negative f32 bits order as the values).
+  atomicMax(diag, __float_as_uint(sqrtf(e2)));
   for (int z = c0[2]; z <= c1[2]; ++z)
     for (int y = c0[1]; y <= c1[1]; ++y)
       for (int x = c0[0]; x <= c1[0]; ++x) d[((size_t)z * g.n[1] + y) * g.n[0] + x] = 0.0f;
@@ -538,15 +542,19 @@ hipError_t scene_distance_bounds(DevScene *sc, hipStream_t s) {
   int *vwork = nullptr;
   float *fwork = nullptr;
   double *zwork = nullptr;
+  float *diag = nullptr;
   hipError_t err = hipSuccess;
   do {
     if ((err = dalloc(&g.lb, ncell)) != hipSuccess) break;
+    if ((err = dalloc(&diag, 1)) != hipSuccess) break;
+    hipLaunchKernelGGL(k_db_fill, dim3(1), dim3(BT), 0, s, diag, (size_t)1, 0.0f);
     // Row r of a pass uses entries [r n, r n + n) and [r (n + 1), r (n + 1) + n + 1).
     if ((err = dalloc(&vwork, ncell)) != hipSuccess) break;
     if ((err = dalloc(&fwork, ncell)) != hipSuccess) break;
     if ((err = dalloc(&zwork, 2 * ncell + 1)) != hipSuccess) break;
     hipLaunchKernelGGL(k_db_fill, dim3(blocks_for((long)ncell, BT)), dim3(BT), 0, s, g.lb, ncell, DB_FAR);
-    hipLaunchKernelGGL(k_db_mark, dim3(blocks_for(sc->n, BT)), dim3(BT), 0, s, sc->tris, sc->n, g, pad, g.lb);
+    hipLaunchKernelGGL(k_db_mark, dim3(blocks_for(sc->n, BT)), dim3(BT), 0, s, sc->tris, sc->n, g, pad, g.lb,
+                       (uint32_t *)diag);
     for (int axis = 0; axis < 3; ++axis) {
       long nrows = (long)(ncell / (size_t)g.n[axis]);
       hipLaunchKernelGGL(k_db_edt, dim3(blocks_for(nrows, 64)), dim3(64), 0, s, g.lb, g.n[0], g.n[1], g.n[2], axis,
@@ -554,8 +562,12 @@ hipError_t scene_distance_bounds(DevScene *sc, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_db_final, dim3(blocks_for((long)ncell, BT)), dim3(BT), 0, s, g.lb, ncell, g.h);
     if ((err = hipGetLastError()) != hipSuccess) break;
-    err = hipStreamSynchronize(s);
+    float hd = 0.0f;
+    if ((err = hipMemcpyAsync(&hd, diag, sizeof hd, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
+    if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
+    g.ub_add = (float)((2.0 * 1.7320508075688772 * h + (double)hd + 2.0 * (double)pad) * 1.001);
   } while (0);
+  if (diag) (void)hipFree(diag);
   if (vwork) (void)hipFree(vwork);
   if (fwork) (void)hipFree(fwork);
   if (zwork) (void)hipFree(zwork);

@@ -208,6 +208,7 @@ struct DistBound {
   float *lb;  // n[0] * n[1] * n[2] bounds, x fastest
   float org[3], h;
   int n[3];
+  float ub_add;  // what dist_upper adds back: two cell diagonals, the largest triangle box, the pads
 };
 
 struct SceneView {
@@ -251,6 +252,27 @@ SH_HD float dist_bound(const DistBound &g, V3 p) {
   float in = g.lb[((size_t)c[2] * g.n[1] + c[1]) * g.n[0] + c[0]];
   if (out2 == 0.0f) return in;
   return sqrtf(out2 + in * in) * 0.999f;
+}
+
+// An upper bound of the distance from p to the mesh: the stored bound is (e - sqrt 3) h for the
+// distance e h between its cell's centre and the nearest marked cell's, and a marked cell meets the
+// (padded) box of a triangle, so a point of that triangle lies within the cell's half diagonal plus
+// the triangle box's diagonal of its centre. Outside the grid, plus the distance to the grid. +inf
+// without bounds.
+SH_HD float dist_upper(const DistBound &g, V3 p) {
+  if (!g.lb) return __builtin_inff();
+  float q[3] = {p.x, p.y, p.z};
+  float out2 = 0.0f;
+  int c[3];
+  for (int k = 0; k < 3; ++k) {
+    float lo = g.org[k], hi = g.org[k] + g.h * (float)g.n[k];
+    float e = q[k] < lo ? lo - q[k] : (q[k] > hi ? q[k] - hi : 0.0f);
+    out2 += e * e;
+    c[k] = grid_cell(q[k], g.org[k], 1.0f / g.h, g.n[k]);
+  }
+  float in = g.lb[((size_t)c[2] * g.n[1] + c[1]) * g.n[0] + c[0]];
+  float u = in / 0.999f + g.ub_add;
+  return (out2 == 0.0f ? u : sqrtf(out2) + u) * 1.0001f;
 }
 
 // The pruning slack: ~80 ulps of the largest coordinate in play, more than the f32 rounding of

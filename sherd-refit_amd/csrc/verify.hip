@@ -483,7 +483,8 @@ __global__ void __launch_bounds__(VT) k_pen_min(VerifyBatch vb, int per_task) {
       float window = closest_window(acc, side);
       if (dist_bound(other.mesh.db, q) <= window + slack_of(other.mesh.absmax, q) && window > 0.0f) {
         bool complete;
-        float dist = sqrtf(min_d2_with(st, other.mesh, q, window, PEN_STEPS, &complete));
+        float dist = sqrtf(min_d2_with(st, other.mesh, q, fminf(window, dist_upper(other.mesh.db, q)), PEN_STEPS,
+                                       &complete));
         if (dist <= window) best = __float_as_uint(dist);
         hard = !complete;
       }
@@ -613,8 +614,10 @@ __global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb) {
   const double *p = own.s + 3 * (size_t)seed;
   V3 q = moved_f32(T, p[0], p[1], p[2]);
   float window = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(closest_window(acc, side))));
+  // Searched no farther than the seed's upper bound: the exact distance is within it.
+  const float radius = fminf(window, dist_upper(other.mesh.db, q));
   bool complete;
-  float dist = sqrtf(min_d2_wave(other.mesh, q, window, stk[w], HARD_STACK, WAVE_STEPS, &complete));
+  float dist = sqrtf(min_d2_wave(other.mesh, q, radius, stk[w], HARD_STACK, WAVE_STEPS, &complete));
   if (lane == 0 && dist <= window) atomicMin(&acc.closest[side], __float_as_uint(dist));
   if (lane == 0 && !complete) atomicOr(&acc.flags, FLAG_GAVE_UP);
 }
@@ -643,8 +646,11 @@ __global__ void __launch_bounds__(VT) k_pen_hard(VerifyBatch vb) {
     float window = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(closest_window(acc, side))));
     // The window may have shrunk since the query was queued: one beyond it is dropped unwalked.
     if (!(dist_bound(other.mesh.db, q) <= window + slack_of(other.mesh.absmax, q))) continue;
+    // The search needs to reach no farther than the point's own upper bound (its distance is
+    // within it), which prunes from the first node on.
+    const float radius = fminf(window, dist_upper(other.mesh.db, q));
     bool complete;
-    float dist = sqrtf(min_d2_wave(other.mesh, q, window, stk[w], HARD_STACK, WAVE_STEPS, &complete));
+    float dist = sqrtf(min_d2_wave(other.mesh, q, radius, stk[w], HARD_STACK, WAVE_STEPS, &complete));
     if (lane == 0 && dist <= window) atomicMin(&acc.closest[side], __float_as_uint(dist));
     if (lane == 0 && !complete) atomicOr(&acc.flags, FLAG_GAVE_UP);
   }
