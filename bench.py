@@ -918,6 +918,7 @@ def vessel_section(ex, synth, args):
         ex.nearest_below(frags[0]["margin_p"], q, scales[8])
     b1 = ex.stats("nearest")
     knn = (b1["items"] - b0["items"]) / ((b1["device_ns"] - b0["device_ns"]) * 1e-9)
+    log(f"vessel rays: {rays} cone rays at {rays_per_s / 1e6:.1f} M/s; kNN {knn / 1e6:.1f} M queries/s")
     avg_ms = prof[ROOFLINE_KERNEL]["ms"] / prof[ROOFLINE_KERNEL]["launches"]
     nbytes = algorithmic_bytes(ROOFLINE_KERNEL, frags, pairs, cand_pair)
     overlapped = {}

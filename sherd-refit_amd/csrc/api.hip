@@ -721,12 +721,17 @@ int sherd_cone_votes(sherd_ctx *ctx, const sherd_scene *scene, const double *V, 
   }
   cc.min_hit = (float)(0.1 * thick);
   cc.thick = (float)thick;
+  double *dfn = (double *)c->scratch(5, sizeof(double) * 3 * nf, &e);
+  if (!dfn) return hip_fail(e, "scratch");
   Timer tm(c, SHERD_M_CONE, 7 * (uint64_t)n_faces);
-  HIPCK(launch_cone(view_of(scene->d), dV, dF, (int)nf, dfaces, dns, (int)n_faces, cc, retest, dg, c->stream),
+  if (c->prof.on) c->prof.begin("k_cone", c->stream);
+  HIPCK(launch_cone(view_of(scene->d), dV, dF, (int)nf, dfaces, dns, (int)n_faces, cc, retest, dg, dfn, c->stream),
         "cone kernel");
+  if (c->prof.on) c->prof.end(c->stream);
   tm.stop();
   HIPCK(hipMemcpyAsync(good, dg, n_faces, hipMemcpyDeviceToHost, c->stream), "download");
   HIPCK(hipStreamSynchronize(c->stream), "synchronize");
+  c->prof.collect();
   return SHERD_OK;
 }
 

@@ -116,7 +116,9 @@ __device__ inline void first_hit_stackless(const SceneView &sv, V3 o, V3 d, int 
 // The same first hit, front to back: both children are entered by their entry parameter, the
 // nearer first, the farther deferred and dropped once a hit closer than its entry is known. A full
 // short stack hands the ray to the exhaustive walk above, so the answer is the same.
-__device__ inline void first_hit(const SceneView &sv, V3 o, V3 d, int retest, float *t_out, uint32_t *face_out) {
+template <class Stack>
+__device__ inline void first_hit_with(Stack &st, const SceneView &sv, V3 o, V3 d, int retest, float *t_out,
+                                      uint32_t *face_out) {
   float best = __builtin_inff();
   uint32_t bface = 0xFFFFFFFFu;
   float pad = slack_of(sv.absmax, o);
@@ -132,8 +134,6 @@ __device__ inline void first_hit(const SceneView &sv, V3 o, V3 d, int retest, fl
   if (cur < 0 || cur >= nodes) return;
   Node nd = load_node(sv.nodes, cur);
   if (ray_enter(nd, o, d, inv, pad, best) < 0.0f) return;
-  int st_n[DSTACK];
-  float st_t[DSTACK];
   int sp = 0;
   bool overflow = false;
   for (int iter = 0; iter <= nodes; ++iter) {
@@ -158,9 +158,8 @@ __device__ inline void first_hit(const SceneView &sv, V3 o, V3 d, int retest, fl
       bool hl = tl >= 0.0f, hr = tr >= 0.0f;
       if (hl && hr) {
         bool lf = tl <= tr;
-        if (sp < DSTACK) {
-          st_n[sp] = lf ? r : l;
-          st_t[sp] = lf ? tr : tl;
+        if (sp < Stack::CAP) {
+          st.put(sp, lf ? r : l, lf ? tr : tl);
           ++sp;
         } else {
           overflow = true;
@@ -178,7 +177,7 @@ __device__ inline void first_hit(const SceneView &sv, V3 o, V3 d, int retest, fl
     if (next < 0) {
       while (next < 0 && sp > 0) {
         --sp;
-        if (st_t[sp] <= best + tpad) next = st_n[sp];
+        if (st.dist(sp) <= best + tpad) next = st.node(sp);
       }
       if (next < 0) break;
       nd = load_node(sv.nodes, next);
@@ -190,6 +189,23 @@ __device__ inline void first_hit(const SceneView &sv, V3 o, V3 d, int retest, fl
   }
   *t_out = best;
   *face_out = bface;
+}
+
+// A first hit's deferred nodes in a per-lane array (DSTACK entries).
+struct RayRegStack {
+  static constexpr int CAP = DSTACK;
+  int n[DSTACK];
+  float d[DSTACK];
+  __device__ __forceinline__ void put(int k, int node, float dd) {
+    n[k] = node;
+    d[k] = dd;
+  }
+  __device__ __forceinline__ int node(int k) const { return n[k]; }
+  __device__ __forceinline__ float dist(int k) const { return d[k]; }
+};
+__device__ inline void first_hit(const SceneView &sv, V3 o, V3 d, int retest, float *t_out, uint32_t *face_out) {
+  RayRegStack st;
+  first_hit_with(st, sv, o, d, retest, t_out, face_out);
 }
 
 // Crossings of the ray from o along +AXIS (RayScene::crossings, bvh.rs:160-178): every hit, from

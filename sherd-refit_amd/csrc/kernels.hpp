@@ -266,7 +266,8 @@ hipError_t launch_vertex_votes(const SceneView &sv, const double *V, int nv, con
                                hipStream_t s);
 hipError_t launch_iota(uint32_t *x, int n, hipStream_t s);
 hipError_t launch_cone(const SceneView &sv, const double *V, const uint32_t *F, int nf, const int64_t *faces,
-                       const double *ns, int n, const ConeConst &cc, int retest, uint8_t *good, hipStream_t s);
+                       const double *ns, int n, const ConeConst &cc, int retest, uint8_t *good, double *fn,
+                       hipStream_t s);
 
 // ---- R §3.4.4-3.4.8 (segment.hip) -----------------------------------------------------------------
 constexpr int SEG_BALL_CAP = 1024;  // faces within one representative's ball
