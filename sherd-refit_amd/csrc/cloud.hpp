@@ -31,8 +31,9 @@ __device__ __forceinline__ double leaf_d2(const Tri *tris, int slot, double q0, 
 // `seed` (a point index, or NONE) starts the search from that point's distance when it is inside
 // the radius: the answer is the same (every point at most as far is still examined, ties
 // included), the walk shorter when the seed is close, as last iteration's partner usually is.
-__device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, double q0, double q1, double q2,
-                                     double r2, uint32_t *out_j, double *out_d2, uint32_t seed = NONE) {
+template <class Stack>
+__device__ inline bool cloud_nearest_with(Stack &st, const SceneView &sv, const double *pts, double q0, double q1,
+                                          double q2, double r2, uint32_t *out_j, double *out_d2, uint32_t seed = NONE) {
   int nodes = 2 * sv.n - 1;
   int cur = sv.root;
   if (cur < 0 || cur >= nodes || !(r2 > 0.0)) return false;
@@ -54,8 +55,6 @@ __device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, dou
     if (bj == NONE) return false;
     cur = -1;  // (cannot happen with a seed: the root's box holds it) keep the seed
   }
-  int st_n[DSTACK];
-  float st_d[DSTACK];
   int sp = 0;
   bool overflow = false;
   for (int iter = 0; cur >= 0 && iter <= nodes; ++iter) {
@@ -77,9 +76,8 @@ __device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, dou
       bool hl = dl <= lim2, hr = dr <= lim2;
       if (hl && hr) {
         bool lf = dl <= dr;
-        if (sp < DSTACK) {
-          st_n[sp] = lf ? r : l;
-          st_d[sp] = lf ? dr : dl;
+        if (sp < Stack::CAP) {
+          st.put(sp, lf ? r : l, lf ? dr : dl);
           ++sp;
         } else {
           overflow = true;
@@ -97,7 +95,7 @@ __device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, dou
     if (next < 0) {
       while (next < 0 && sp > 0) {
         --sp;
-        if (st_d[sp] <= lim2) next = st_n[sp];
+        if (st.dist(sp) <= lim2) next = st.node(sp);
       }
       if (next < 0) break;
       nd = load_node(sv.nodes, next);
@@ -128,6 +126,12 @@ __device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, dou
   *out_j = bj;
   *out_d2 = best;
   return true;
+}
+
+__device__ inline bool cloud_nearest(const SceneView &sv, const double *pts, double q0, double q1, double q2,
+                                     double r2, uint32_t *out_j, double *out_d2, uint32_t seed = NONE) {
+  RayRegStack st;
+  return cloud_nearest_with(st, sv, pts, q0, q1, q2, r2, out_j, out_d2, seed);
 }
 
 

@@ -30,6 +30,7 @@ namespace sherd {
 namespace {
 
 constexpr int NV = ICP_NV;
+constexpr int ICP_LSTACK = 8;  // the nearest-point search's deferred nodes, in an LDS column per lane
 
 // Sums v[0..n) over the wave's lanes in a fixed shuffle tree; lane 0 holds the totals.
 template <int N>
@@ -119,8 +120,11 @@ struct IcpIter {
   int stop, iters, conv;
 };
 
-__device__ __forceinline__ void icp_iteration(const double *red, int ns, int it, int max_iter, bool plane, bool p2p,
-                                              IcpIter *s, double *U) {
+// ws: ICP_WS doubles and 6 ints of working space (the caller's LDS): W 16, the system 36 + 6 + 6,
+// the rotation 9, then the solve's 48 (or the Euler product's 36).
+constexpr int ICP_WS = 121;
+__device__ __forceinline__ void icp_iteration_ws(const double *red, int ns, int it, int max_iter, bool plane, bool p2p,
+                                                 IcpIter *s, double *U, double *ws, int *iws) {
   double cnt = red[0];
   double fit = cnt > 0.0 ? cnt / (double)ns : 0.0;
   double rmse = cnt > 0.0 ? sqrt(red[1] / cnt) : 0.0;
@@ -132,10 +136,9 @@ __device__ __forceinline__ void icp_iteration(const double *red, int ns, int it,
   s->iters = it;
   s->stop = (conv || it >= max_iter) ? 1 : 0;
   if (!s->stop && plane) {
-    double W[16];
+    double *W = ws, *a = ws + 16, *rhs = ws + 52, *x = ws + 58, *rot = ws + 64;  // (ws + 73: the solve's)
     eye4(W);
     if (cnt > 0.0) {
-      double a[36], rhs[6], x[6];
       int w = 2;
       for (int k = 0; k < 6; ++k)
         for (int l = 0; l <= k; ++l) {
@@ -144,9 +147,8 @@ __device__ __forceinline__ void icp_iteration(const double *red, int ns, int it,
           ++w;
         }
       for (int k = 0; k < 6; ++k) rhs[k] = -red[23 + k];
-      if (solve_ldlt(a, rhs, x)) {
-        double rot[9];
-        euler_zyx(x[0], x[1], x[2], rot);
+      if (solve_ldlt_ws(a, rhs, x, ws + 73, ws + 109, ws + 115, iws)) {
+        euler_zyx_ws(x[0], x[1], x[2], rot, ws + 73);
         for (int r = 0; r < 3; ++r) {
           for (int q = 0; q < 3; ++q) W[4 * r + q] = rot[3 * r + q];
           W[4 * r + 3] = x[3 + r];
@@ -215,6 +217,12 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
   __shared__ double red[NV];
   __shared__ IcpIter s_it;
   __shared__ int s_c;
+  // Lane 0's solves work in LDS: as private arrays their pivot-indexed entries went to scratch
+  // and their values held registers the whole kernel is allocated by.
+  __shared__ double s_ws[ICP_WS > 78 ? ICP_WS : 78];
+  __shared__ int s_iws[6];
+  __shared__ uint2 s_cstk[ICP_LSTACK][B];
+  LdsStack<B, ICP_LSTACK> cst{&s_cstk[0][threadIdx.x]};
   for (;;) {
     if (t == 0) {
       uint32_t k = atomicAdd(v.next, 1u);
@@ -238,9 +246,7 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
     if (t == 0) s_it = IcpIter{0.0, 0.0, 0, 0, 0};
     __syncthreads();
     for (int it = 0;; ++it) {
-      double M[12];
-#pragma unroll
-      for (int k = 0; k < 12; ++k) M[k] = sU[k];
+      const double *M = sU;  // read from LDS where used: no registers held across the searches
       for (int sp = wave; sp < nspan; sp += W) {
         const int end = ns < (sp + 1) * ICP_SPAN ? ns : (sp + 1) * ICP_SPAN;
         // The span's correspondences first, each lane its points l, l + 64, ... Nothing of the sums
@@ -257,7 +263,7 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
           uint32_t j = NONE;
           double d2;
           uint32_t seed = it > 0 ? corr[i] : NONE;  // last iteration's partner (this candidate's own)
-          if (!radius_ok || !cloud_nearest(pd.tree, pd.tp, p0, p1, p2, r2, &j, &d2, seed)) j = NONE;
+          if (!radius_ok || !cloud_nearest_with(cst, pd.tree, pd.tp, p0, p1, p2, r2, &j, &d2, seed)) j = NONE;
           corr[i] = j;
         }
         // Then the sums over the same points in the same order, from what the lane just stored (the
@@ -279,7 +285,7 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
       __syncthreads();
       if (wave == 0) spans_total<NS>(part, nspan, red);
       __syncthreads();
-      if (t == 0) icp_iteration(red, ns, it, v.max_iter, plane, p2p, &s_it, sU);
+      if (t == 0) icp_iteration_ws(red, ns, it, v.max_iter, plane, p2p, &s_it, sU, s_ws, s_iws);
       __syncthreads();
       if (s_it.stop) break;
       if (p2p) {
@@ -299,20 +305,17 @@ __global__ void __launch_bounds__(B) k_icp_batch(IcpBatchView v) {
           __syncthreads();
         }
         if (t == 0) {
-          double U[16];
-          eye4(U);
+          eye4(sU);
           if (cnt > 0.0) {
             double mp[3] = {m.mp0, m.mp1, m.mp2}, mq[3] = {m.mq0, m.mq1, m.mq2};
-            umeyama_pose(red + 2, mp, mq, U);
+            umeyama_pose_ws(red + 2, mp, mq, sU, s_ws, s_iws);
           }
-          for (int k = 0; k < 16; ++k) sU[k] = U[k];
         }
         __syncthreads();
       }
       if (t == 0) {
-        double nt[16];
-        mul_nn(sU, sT, nt, 4);
-        for (int k = 0; k < 16; ++k) sT[k] = nt[k];
+        mul_nn(sU, sT, s_ws, 4);
+        for (int k = 0; k < 16; ++k) sT[k] = s_ws[k];
       }
       __syncthreads();
     }
@@ -366,6 +369,7 @@ __global__ void __launch_bounds__(ICP_SPAN) k_icp_wide_corr(IcpWideView w, int i
   __shared__ double s_p[3 * ICP_SPAN];
   __shared__ double s_d2[ICP_SPAN];
   __shared__ uint32_t s_j[ICP_SPAN];
+  __shared__ uint2 s_cstk[ICP_LSTACK][ICP_SPAN];
   const int s = blockIdx.x;
   const int k = span_owner(w, s);
   const uint32_t c = w.b.order[k];
@@ -394,7 +398,8 @@ __global__ void __launch_bounds__(ICP_SPAN) k_icp_wide_corr(IcpWideView w, int i
     uint32_t j = NONE;
     double d2 = 0.0;
     uint32_t seed = it > 0 ? corr[i] : NONE;
-    if (!(pd.r > 0.0) || !cloud_nearest(pd.tree, pd.tp, p0, p1, p2, pd.r * pd.r, &j, &d2, seed)) j = NONE;
+    LdsStack<ICP_SPAN, ICP_LSTACK> cst{&s_cstk[0][threadIdx.x]};
+    if (!(pd.r > 0.0) || !cloud_nearest_with(cst, pd.tree, pd.tp, p0, p1, p2, pd.r * pd.r, &j, &d2, seed)) j = NONE;
     corr[i] = j;
     s_p[3 * t] = p0;
     s_p[3 * t + 1] = p1;
@@ -422,6 +427,8 @@ __global__ void __launch_bounds__(ICP_SPAN) k_icp_wide_corr(IcpWideView w, int i
 // (or no normals) the new pose.
 __global__ void __launch_bounds__(64) k_icp_wide_step(IcpWideView w, int it) {
   __shared__ double red[NV];
+  __shared__ double s_ws[ICP_WS + 32];
+  __shared__ int s_iws[6];
   const uint32_t c = w.b.order[blockIdx.x];
   IcpWideState &st = w.st[c];
   if (st.stop) return;
@@ -433,9 +440,9 @@ __global__ void __launch_bounds__(64) k_icp_wide_step(IcpWideView w, int it) {
   const bool plane = v.est == 1 && pd.tn != nullptr;
   const bool p2p = v.est == 0;
   IcpIter s{st.fit, st.rmse, st.stop, st.iters, st.conv};
-  double U[16];
+  double *U = s_ws + ICP_WS, *nt = s_ws + ICP_WS + 16;
   for (int k = 0; k < 16; ++k) U[k] = st.U[k];
-  icp_iteration(red, pd.ns, it, v.max_iter, plane, p2p, &s, U);
+  icp_iteration_ws(red, pd.ns, it, v.max_iter, plane, p2p, &s, U, s_ws, s_iws);
   st.fit = s.fit;
   st.rmse = s.rmse;
   st.iters = s.iters;
@@ -447,7 +454,6 @@ __global__ void __launch_bounds__(64) k_icp_wide_step(IcpWideView w, int it) {
     return;
   }
   if (!p2p) {
-    double nt[16];
     mul_nn(U, st.T, nt, 4);
     for (int k = 0; k < 16; ++k) {
       st.U[k] = U[k];
@@ -476,6 +482,8 @@ __global__ void __launch_bounds__(64) k_icp_wide_cov(IcpWideView w) {
 // Point-to-point's update (Umeyama over the covariance) and the new pose, one wave per candidate.
 __global__ void __launch_bounds__(64) k_icp_wide_umeyama(IcpWideView w) {
   __shared__ double sig[9];
+  __shared__ double s_ws[78 + 32];
+  __shared__ int s_iws[3];
   const uint32_t c = w.b.order[blockIdx.x];
   IcpWideState &st = w.st[c];
   if (st.stop) return;
@@ -483,14 +491,13 @@ __global__ void __launch_bounds__(64) k_icp_wide_umeyama(IcpWideView w) {
   const bool any = st.red[0] > 0.0;
   if (any) spans_total<9>(w.part + (size_t)w.span_first[blockIdx.x] * NV, (pd.ns + ICP_SPAN - 1) / ICP_SPAN, sig);
   if (threadIdx.x != 0) return;
-  double U[16];
+  double *U = s_ws + 78, *nt = s_ws + 94;
   eye4(U);
   if (any) {
     const P2pMeans m = p2p_means(st.red);
     double mp[3] = {m.mp0, m.mp1, m.mp2}, mq[3] = {m.mq0, m.mq1, m.mq2};
-    umeyama_pose(sig, mp, mq, U);
+    umeyama_pose_ws(sig, mp, mq, U, s_ws, s_iws);
   }
-  double nt[16];
   mul_nn(U, st.T, nt, 4);
   for (int k = 0; k < 16; ++k) {
     st.U[k] = U[k];

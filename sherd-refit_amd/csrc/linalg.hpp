@@ -39,44 +39,42 @@ SH_HD void eigen_pivots(const double *a, int *perm) {
   }
 }
 
-// icp.rs:692-753: LDL^T on the permuted matrix, pseudo-inverse of D below the smallest normal.
-SH_HD bool solve_ldlt(const double *a, const double *b, double *x) {
-  int perm[6];
+// icp.rs:692-753: LDL^T on the permuted matrix, pseudo-inverse of D below the smallest normal. The
+// working arrays come from the caller (m 36, y 6, temp 6, perm 6): a kernel passes LDS, where the
+// pivot-indexed accesses cost no private memory and the values hold no registers between steps.
+SH_HD bool solve_ldlt_ws(const double *a, const double *b, double *x, double *m, double *y, double *temp, int *perm) {
   eigen_pivots(a, perm);
-  double m[6][6];
   for (int k = 0; k < 6; ++k)
-    for (int l = 0; l < 6; ++l) m[k][l] = a[6 * perm[k] + perm[l]];
+    for (int l = 0; l < 6; ++l) m[6 * k + l] = a[6 * perm[k] + perm[l]];
   for (int k = 0; k < 6; ++k) {
     if (k > 0) {
-      double temp[6];
-      for (int j = 0; j < k; ++j) temp[j] = m[j][j] * m[k][j];
-      double diagonal = m[k][k];
-      for (int j = 0; j < k; ++j) diagonal -= m[k][j] * temp[j];
-      m[k][k] = diagonal;
+      for (int j = 0; j < k; ++j) temp[j] = m[6 * j + j] * m[6 * k + j];
+      double diagonal = m[6 * k + k];
+      for (int j = 0; j < k; ++j) diagonal -= m[6 * k + j] * temp[j];
+      m[6 * k + k] = diagonal;
       for (int r = k + 1; r < 6; ++r) {
-        double below = m[r][k];
-        for (int j = 0; j < k; ++j) below -= m[r][j] * temp[j];
-        m[r][k] = below;
+        double below = m[6 * r + k];
+        for (int j = 0; j < k; ++j) below -= m[6 * r + j] * temp[j];
+        m[6 * r + k] = below;
       }
     }
-    double pivot = m[k][k];
+    double pivot = m[6 * k + k];
     if (pivot != 0.0)
-      for (int r = k + 1; r < 6; ++r) m[r][k] /= pivot;
+      for (int r = k + 1; r < 6; ++r) m[6 * r + k] /= pivot;
   }
-  double y[6];
   for (int k = 0; k < 6; ++k) y[k] = b[perm[k]];
   for (int i = 0; i < 6; ++i) {
     double v = y[i];
-    for (int j = 0; j < i; ++j) v -= m[i][j] * y[j];
+    for (int j = 0; j < i; ++j) v -= m[6 * i + j] * y[j];
     y[i] = v;
   }
   for (int i = 0; i < 6; ++i) {
-    double pivot = m[i][i];
+    double pivot = m[6 * i + i];
     y[i] = fabs(pivot) > 2.2250738585072014e-308 ? y[i] / pivot : 0.0;
   }
   for (int i = 5; i >= 0; --i) {
     double v = y[i];
-    for (int j = i + 1; j < 6; ++j) v -= m[j][i] * y[j];
+    for (int j = i + 1; j < 6; ++j) v -= m[6 * j + i] * y[j];
     y[i] = v;
   }
   for (int k = 0; k < 6; ++k) x[perm[k]] = y[k];
@@ -84,23 +82,38 @@ SH_HD bool solve_ldlt(const double *a, const double *b, double *x) {
     if (!(x[k] - x[k] == 0.0)) return false;
   return true;
 }
+SH_HD bool solve_ldlt(const double *a, const double *b, double *x) {
+  double m[36], y[6], temp[6];
+  int perm[6];
+  return solve_ldlt_ws(a, b, x, m, y, temp, perm);
+}
 
 // icp.rs:770-778: Rz(gamma) Ry(beta) Rx(alpha).
-SH_HD void euler_zyx(double al, double be, double ga, double *r) {
+SH_HD void euler_zyx_ws(double al, double be, double ga, double *r, double *ws /* 36 */) {
   double sa = sin(al), ca = cos(al), sb = sin(be), cb = cos(be), sg = sin(ga), cg = cos(ga);
-  double rx[9] = {1, 0, 0, 0, ca, -sa, 0, sa, ca};
-  double ry[9] = {cb, 0, sb, 0, 1, 0, -sb, 0, cb};
-  double rz[9] = {cg, -sg, 0, sg, cg, 0, 0, 0, 1};
-  double t[9];
+  double *rx = ws, *ry = ws + 9, *rz = ws + 18, *t = ws + 27;
+  const double x[9] = {1, 0, 0, 0, ca, -sa, 0, sa, ca}, y[9] = {cb, 0, sb, 0, 1, 0, -sb, 0, cb},
+               z[9] = {cg, -sg, 0, sg, cg, 0, 0, 0, 1};
+  for (int i = 0; i < 9; ++i) {
+    rx[i] = x[i];
+    ry[i] = y[i];
+    rz[i] = z[i];
+  }
   mul_nn(rz, ry, t, 3);
   mul_nn(t, rx, r, 3);
+}
+SH_HD void euler_zyx(double al, double be, double ga, double *r) {
+  double ws[36];
+  euler_zyx_ws(al, be, ga, r, ws);
 }
 
 // One-sided Jacobi SVD of a 3x3: M = U diag(s) V^T, singular values descending, U completed by a
 // cross product where a singular value vanishes.
-SH_HD void svd3(const double *m, double *u, double *s, double *v) {
-  double a[9], vv[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-  for (int i = 0; i < 9; ++i) a[i] = m[i];
+SH_HD void svd3_ws(const double *m, double *u, double *s, double *v, double *a, double *vv, double *sig, int *ord) {
+  for (int i = 0; i < 9; ++i) {
+    a[i] = m[i];
+    vv[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  }
   for (int sweep = 0; sweep < 60; ++sweep) {
     double off = 0.0;
     for (int p = 0; p < 2; ++p)
@@ -128,9 +141,8 @@ SH_HD void svd3(const double *m, double *u, double *s, double *v) {
       }
     if (off < 1e-15) break;
   }
-  double sig[3];
   for (int j = 0; j < 3; ++j) sig[j] = sqrt(a[j] * a[j] + a[3 + j] * a[3 + j] + a[6 + j] * a[6 + j]);
-  int ord[3] = {0, 1, 2};
+  for (int j = 0; j < 3; ++j) ord[j] = j;
   for (int x = 0; x < 3; ++x)
     for (int y = x + 1; y < 3; ++y)
       if (sig[ord[y]] > sig[ord[x]]) {
@@ -169,15 +181,21 @@ SH_HD void svd3(const double *m, double *u, double *s, double *v) {
   }
 }
 
+SH_HD void svd3(const double *m, double *u, double *s, double *v) {
+  double a[9], vv[9], sig[3];
+  int ord[3];
+  svd3_ws(m, u, s, v, a, vv, sig, ord);
+}
+
 // Umeyama's rotation and translation from the means and the scaled covariance (icp.rs:578-591).
-SH_HD void umeyama_pose(const double *sigma, const double *mp, const double *mq, double *out) {
-  double u[9], s[3], v[9], vt[9];
-  svd3(sigma, u, s, v);
+// ws: 78 doubles and 3 ints of working space (LDS in a kernel, as solve_ldlt_ws's).
+SH_HD void umeyama_pose_ws(const double *sigma, const double *mp, const double *mq, double *out, double *ws, int *iws) {
+  double *u = ws, *s = ws + 9, *v = ws + 12, *vt = ws + 21, *d = ws + 30, *ud = ws + 39, *rot = ws + 48;
+  svd3_ws(sigma, u, s, v, ws + 57, ws + 66, ws + 75, iws);
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) vt[3 * i + j] = v[3 * j + i];
-  double d[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  for (int i = 0; i < 9; ++i) d[i] = (i % 4 == 0) ? 1.0 : 0.0;
   if (det3(u) * det3(vt) < 0.0) d[8] = -1.0;
-  double ud[9], rot[9];
   mul_nn(u, d, ud, 3);
   mul_nn(ud, vt, rot, 3);
   eye4(out);
@@ -185,6 +203,12 @@ SH_HD void umeyama_pose(const double *sigma, const double *mp, const double *mq,
     for (int j = 0; j < 3; ++j) out[4 * i + j] = rot[3 * i + j];
     out[4 * i + 3] = mq[i] - (rot[3 * i] * mp[0] + rot[3 * i + 1] * mp[1] + rot[3 * i + 2] * mp[2]);
   }
+}
+
+SH_HD void umeyama_pose(const double *sigma, const double *mp, const double *mq, double *out) {
+  double ws[78];
+  int iws[3];
+  umeyama_pose_ws(sigma, mp, mq, out, ws, iws);
 }
 
 // verify.rs:739-798 pose_inverse: LU with partial pivoting (first index on a tie), then one
