@@ -766,12 +766,14 @@ def collection_headline(ex, args):
     for _ in range(max(args.warmup, 2)):  # the first pass tunes the ICP workgroups, the second captures
         CL.run_batched(dev, coll, pairs, cp, cT, scales)
     ex.profile(True)
+    r0 = ex.stats("verify_rays")["items"]
     t0 = time.perf_counter()
     parts = np.zeros(3)
     for _ in range(args.steps):
         b = CL.run_batched(dev, coll, pairs, cp, cT, scales)
         parts += [b["icp_s"], b["lm_s"], b["verify_s"]]
     el = time.perf_counter() - t0
+    rays = (ex.stats("verify_rays")["items"] - r0) / args.steps  # parity rays per step
     prof = ex.profile_read()
     ex.profile(False)
     n_cand = len(cT)
@@ -782,19 +784,23 @@ def collection_headline(ex, args):
     # The plain schedule over its own timed steps, after one untimed pass (none with --plain-steps 0,
     # for counter collection).
     if args.plain_steps <= 0:
-        p, el_plain, plain = b, float("nan"), float("nan")
+        p, el_plain, plain, plain_rays = b, float("nan"), float("nan"), float("nan")
     else:
         CL.run_plain(dev, coll, pairs, cp, cT, scales)
+        r0 = ex.stats("verify_rays")["items"]
         t0 = time.perf_counter()
         for _ in range(args.plain_steps):
             p = CL.run_plain(dev, coll, pairs, cp, cT, scales)
         el_plain = time.perf_counter() - t0
         plain = n_cand * args.plain_steps / el_plain
+        plain_rays = (ex.stats("verify_rays")["items"] - r0) / args.plain_steps
     same = dict(icp=bool(np.array_equal(b["reg"], p["reg"])), lm=bool(np.array_equal(b["T_lm"], p["T_lm"])),
                 scores=bool(np.array_equal(b["scores"].view(np.uint64), p["scores"].view(np.uint64))),
                 accepted=bool(np.array_equal(b["accepted"], p["accepted"])))
     log(f"collection plain: {args.plain_steps} steps in {el_plain:.2f}s; batched/plain {value / plain:.1f}x; "
         f"identical {same}")
+    log(f"collection rays: {rays:.0f} parity rays per step; {rays / (el / args.steps) / 1e6:.1f} M/s batched, "
+        f"{plain_rays / (el_plain / args.plain_steps) / 1e6 if args.plain_steps > 0 else float('nan'):.2f} M/s plain")
     adj = np.array([tuple(pr) in coll["adjacent"] for pr in pairs])[cp]
     res = dict(
         value=value, ms_per_step=1e3 * el / args.steps, candidates=n_cand, prof=prof, coll=coll, pairs=pairs, cp=cp,
@@ -816,6 +822,11 @@ def collection_headline(ex, args):
             batched_vs_plain_with_setup=(round((el_plain / args.plain_steps + setup_s) / (el / args.steps + setup_s), 3)
                                          if args.plain_steps > 0 else None),
             target_3x_met=bool(value / plain >= 3.0) if args.plain_steps > 0 else None, identical_plain_batched=same,
+            rays=dict(unit="parity rays: 3 per surface sample past the mesh-box reject of R 6.4's inside test "
+                           "(the step casts no first-hit ray)", per_step=int(rays),
+                      batched_per_s=round(rays / (el / args.steps), 1),
+                      plain_per_s=round(plain_rays / (el_plain / args.plain_steps), 1) if args.plain_steps > 0 else None,
+                      plain_per_step=int(plain_rays) if args.plain_steps > 0 else None),
             accepted=int(b["accepted"].sum()), accepted_on_adjacent_pairs=int((b["accepted"] & adj).sum()),
             adjacent_candidates=int(adj.sum()), pen_unavailable=int(b["scores"][:, 16].sum())))
     cpu = None
@@ -884,15 +895,18 @@ def vessel_section(ex, synth, args):
     for _ in range(args.warmup):
         ex.verify_batch(vpairs, cand_pair, cand_T)
     ex.profile(True)
+    r0 = ex.stats("verify_rays")["items"]
     t0 = time.perf_counter()
     for _ in range(args.steps):
         scores, accepted = ex.verify_batch(vpairs, cand_pair, cand_T)
     el = time.perf_counter() - t0
+    prays = (ex.stats("verify_rays")["items"] - r0) / args.steps  # parity rays per step
     prof = ex.profile_read()
     ex.profile(False)
     n_cand = len(cand_T)
     value = n_cand * args.steps / el
     n_plain = max(1, min(args.plain_candidates, n_cand))
+    r0 = ex.stats("verify_rays")["items"]
     t0 = time.perf_counter()
     plain_acc = np.empty(n_plain, bool)
     plain_scores = np.empty((n_plain, 20))
@@ -901,6 +915,7 @@ def vessel_section(ex, synth, args):
         plain_scores[i], plain_acc[i] = s1[0], a1[0]
     el_plain = time.perf_counter() - t0
     plain = n_plain / el_plain
+    plain_prays = ex.stats("verify_rays")["items"] - r0
     same = bool(np.array_equal(plain_acc, accepted[:n_plain]) and
                 np.array_equal(plain_scores.view(np.uint64), scores[:n_plain].view(np.uint64)))
     log(f"vessel verify: {value:.0f} pair-verifications/s batched, {plain:.0f} plain; identical {same}")
@@ -918,7 +933,9 @@ def vessel_section(ex, synth, args):
         ex.nearest_below(frags[0]["margin_p"], q, scales[8])
     b1 = ex.stats("nearest")
     knn = (b1["items"] - b0["items"]) / ((b1["device_ns"] - b0["device_ns"]) * 1e-9)
-    log(f"vessel rays: {rays} cone rays at {rays_per_s / 1e6:.1f} M/s; kNN {knn / 1e6:.1f} M queries/s")
+    log(f"vessel rays: {rays} cone rays at {rays_per_s / 1e6:.1f} M/s; {prays:.0f} parity rays per verify step, "
+        f"{prays / (el / args.steps) / 1e6:.1f} M/s batched, {plain_prays / el_plain / 1e6:.2f} M/s plain; "
+        f"kNN {knn / 1e6:.1f} M queries/s")
     avg_ms = prof[ROOFLINE_KERNEL]["ms"] / prof[ROOFLINE_KERNEL]["launches"]
     nbytes = algorithmic_bytes(ROOFLINE_KERNEL, frags, pairs, cand_pair)
     overlapped = {}
@@ -935,6 +952,10 @@ def vessel_section(ex, synth, args):
                ms_per_step=round(1e3 * el / args.steps, 4), plain_pair_verifications_per_s=round(plain, 3),
                plain_candidates=n_plain, batched_vs_plain=round(value / plain, 3), accepted=int(accepted.sum()),
                identical_plain_batched=same, rays_per_s=round(rays_per_s, 1), rays=int(rays),
+               parity_rays=dict(unit="3 per surface sample past the mesh-box reject of R 6.4's inside test",
+                                per_step=int(prays), batched_per_s=round(prays / (el / args.steps), 1),
+                                plain_candidates=n_plain, plain_rays=int(plain_prays),
+                                plain_per_s=round(plain_prays / el_plain, 1)),
                knn_queries_per_s=round(knn, 1), lbvh_mtri_per_s=round(lbvh_mtri, 2), kernel_ms=prof, roofline=roof)
     if not args.no_cpu:
         cpu, oracle_scores = cpu_baseline(col, pairs, scales, cand_pair, cand_T)

@@ -60,8 +60,14 @@ typedef struct {
 
 enum {
   SHERD_M_FIRST_HIT, SHERD_M_BOUNDED_DISTANCE, SHERD_M_INSIDE, SHERD_M_COARSE, SHERD_M_ICP,
-  SHERD_M_NEAREST, SHERD_M_VERIFY, SHERD_M_SCENE_BUILD, SHERD_M_CONE, SHERD_M_HUBER, SHERD_M_SEGMENT, SHERD_M_COUNT
+  SHERD_M_NEAREST, SHERD_M_VERIFY, SHERD_M_SCENE_BUILD, SHERD_M_CONE, SHERD_M_HUBER, SHERD_M_SEGMENT,
+  SHERD_M_VERIFY_RAYS, SHERD_M_COUNT
 };
+/* SHERD_M_VERIFY_RAYS counts the parity rays of the inside tests that sherd_verify_batch runs (R
+ * §6.4 through RayScene::inside, bvh.rs:146-158): items = 3 per surface sample whose moved position
+ * passes the mesh-box reject, whichever two or three of its rays the majority vote actually needs;
+ * calls = sherd_verify_batch calls with full = 1. Its rays run inside the verification kernels, so
+ * device_ns stays 0 (their time is SHERD_M_VERIFY's). */
 
 /* ---- the executor ------------------------------------------------------------------------- */
 

@@ -314,6 +314,7 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
   // Candidates per chunk: a grid's y extent (65535) and x extent in int.
   const size_t MAX_TASK_CAND = std::min<size_t>(65535, (size_t)INT_MAX / (2 * per_task));
   size_t begin = 0;
+  uint64_t rays = 0;  // parity rays in the unit of SHERD_M_VERIFY_RAYS
   std::vector<uint32_t> task;
   std::vector<CandDesc> cd;
   while (begin < n_cand) {
@@ -407,11 +408,16 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
       if (hacc[i].flags & FLAG_VOXEL_RANGE) return fail(SHERD_E_LIMIT, "a seam voxel coordinate exceeds 2^20");
       if (hacc[i].flags & FLAG_GAVE_UP) return fail(SHERD_E_LIMIT, "a closest-approach query exceeded its step budget");
     }
+    for (size_t i = 0; i < nc; ++i) rays += 3 * ((uint64_t)hacc[i].in_box[0] + hacc[i].in_box[1]);
     std::memcpy(scores + VS_COUNT * begin, hbuf + dn_sc, sizeof(double) * VS_COUNT * nc);
     std::memcpy(accepted + begin, hbuf + dn_ac, nc);
     begin = end;
   }
   tm.stop();
+  if (full) {
+    c->stats[SHERD_M_VERIFY_RAYS].calls += 1;
+    c->stats[SHERD_M_VERIFY_RAYS].items += rays;
+  }
   return SHERD_OK;
 }
 

@@ -136,6 +136,7 @@ __global__ void k_prep(VerifyBatch vb) {
   for (int s = 0; s < 2; ++s) {
     a.facing[s] = a.tight[s] = a.touching[s] = 0;
     a.inside[s] = a.deeper[s] = 0;
+    a.in_box[s] = 0;
     a.deepest[s] = 0u;
     a.closest[s] = 0x7f7fffffu;  // f32::MAX, the window of the fallback (verify.rs:570-577)
   }
@@ -377,6 +378,7 @@ __global__ void __launch_bounds__(VT) k_pen(VerifyBatch vb, int per_task) {
     }
     __syncthreads();
     const uint32_t n = s_n;
+    if (threadIdx.x == 0 && n) atomicAdd(&acc.in_box[side], n);
     if (threadIdx.x < ((n + 63u) & ~63u)) {  // whole waves only
       bool in = false, deeper = false;
       float depth = 0.0f;

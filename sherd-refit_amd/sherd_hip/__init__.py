@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("SHERD_HIP_LIB") or os.path.join(HERE, "lib", "libsherd_hip.so")
 
 SHERD_M = dict(first_hit=0, bounded_distance=1, inside=2, coarse=3, icp=4, nearest=5, verify=6,
-               scene_build=7, cone=8, huber=9, segment=10)
+               scene_build=7, cone=8, huber=9, segment=10, verify_rays=11)
 
 _lib = None
 

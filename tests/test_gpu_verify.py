@@ -155,6 +155,30 @@ def test_known_answers_through_the_device(ex):
     assert (got[0][k("cont")], got[0][k("cont_n")]) == (1.0, -1.0)
 
 
+def moved_f32(T, P):
+    """T.p in f64, left to right, narrowed once to f32 (executor/cpu.rs:192-200)."""
+    return np.stack([((T[r, 0] * P[:, 0] + T[r, 1] * P[:, 1]) + T[r, 2] * P[:, 2]) + T[r, 3] for r in range(3)],
+                    axis=1).astype(np.float32)
+
+
+def parity_rays(frags, mesh_scenes, pairs, cp, cT):
+    """The unit of SHERD_M_VERIFY_RAYS on the host: 3 rays per surface sample whose moved position
+    lies in the other mesh's box (RayScene::inside's reject, bvh.rs:146-149), both sides of every
+    candidate whose meshes are watertight (verify.rs:490-493, 541)."""
+    boxes = [sc.info()[1:] for sc in mesh_scenes]
+    n = 0
+    for i in range(len(cT)):
+        a, b = pairs[cp[i]]
+        if not (frags[a]["watertight"] and frags[b]["watertight"]):
+            continue
+        T = np.asarray(cT[i], np.float64).reshape(4, 4)
+        for own, other, M in ((a, b, O.pose_inverse(T).reshape(4, 4)), (b, a, T)):
+            q = moved_f32(M, np.asarray(frags[own]["S"], np.float64).reshape(-1, 3))
+            lo, hi = boxes[other]
+            n += int(np.all((q >= lo) & (q <= hi), axis=1).sum())
+    return 3 * n
+
+
 def test_small_vessel_all_pairs_bit_exact(ex):
     """Six fragments of a small vessel, all fifteen pairs, the true relative poses and perturbations
     up to 90 degrees and 10 t: adjacent pairs touch, others stand apart, so the closest approach runs
@@ -176,7 +200,14 @@ def test_small_vessel_all_pairs_bit_exact(ex):
         dev.append(ex.fragment(fs, ms, *args))
         ora.append(O.SideData(ofs, oms, *args))
     for retest in (False, True):
+        before = ex.stats("verify_rays")
         scores, accepted = ex.verify_batch([(dev[a], dev[b], scales) for a, b in pairs], cp, cT, retest=retest)
+        after = ex.stats("verify_rays")
+        want_rays = parity_rays(frags, [ms for ms in keep[1::4]], pairs, cp, cT)
+        say(f"  parity rays (3 per sample past the mesh-box reject): {after['items'] - before['items']}, "
+            f"host count {want_rays}")
+        assert after["calls"] - before["calls"] == 1
+        assert after["items"] - before["items"] == want_rays > 0
         for i in range(len(cT)):
             a, b = pairs[cp[i]]
             want = O.verify(ora[a], ora[b], cT[i], scales, retest=retest)
