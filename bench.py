@@ -60,7 +60,7 @@ def algorithmic_bytes(kernel, frags, pairs, cand_pair):
     for c in cand_pair:
         a, b = pairs[c]
         fa, fb = frags[a], frags[b]
-        if kernel in ("k_pen", "k_pen_lb", "k_pen_min"):
+        if kernel in ("k_pen", "k_pen_close"):
             total += 24 * (len(fa["S"]) + len(fb["S"]))
         elif kernel == "k_frac":
             total += (24 + 8) * (len(fa["Pf"]) + len(fb["Pf"]))  # the point, its distance key
@@ -997,10 +997,11 @@ def kernel_roofline(kernel, nbytes, avg_ms, pmc_file):
 # Main-stream kernels of the collection step and the §8(d) unit each is counted in: bytes per point
 # processed (the f64 sample read, 24 B; their results are reduced in the kernel), over every
 # (candidate, side) task (k_pen) or over the tasks whose closest approach is needed (both sides
-# without a sample inside: k_pen_lb, k_pen_min). The meshes the points are tested against are not
-# counted: a launch reads only the parts near its samples (92 % of samples fail the box reject), and
-# the bytes it does read are the measured traffic beside it.
-POINT_KERNELS = ("k_pen", "k_pen_lb", "k_pen_min")
+# without a sample inside: k_pen_close, which reads every block's sphere but the samples of only the
+# blocks that can hold the minimum). The meshes the points are tested against are not counted: a
+# launch reads only the parts near its samples (92 % of samples fail the box reject), and the bytes
+# it does read are the measured traffic beside it.
+POINT_KERNELS = ("k_pen", "k_pen_close")
 
 
 def collection_roofline(head):
@@ -1013,8 +1014,7 @@ def collection_roofline(head):
     n_s = np.array([len(frags[a]["S"]) + len(frags[b]["S"]) for a, b in pairs])[cp]
     pen_ok = scores[:, 16] == 0.0
     apart = pen_ok & (scores[:, 15] < 0.0)  # pen_depth < 0: no sample inside on either side
-    points = {"k_pen": int(n_s[pen_ok].sum()), "k_pen_lb": int(n_s[apart].sum()),
-              "k_pen_min": int(n_s[apart].sum())}
+    points = {"k_pen": int(n_s[pen_ok].sum()), "k_pen_close": int(n_s[apart].sum())}
     cands = {k: v for k, v in prof.items() if k in POINT_KERNELS and v["launches"]}
     dom = max(cands, key=lambda k: cands[k]["ms"])
     avg_ms = prof[dom]["ms"] / prof[dom]["launches"]

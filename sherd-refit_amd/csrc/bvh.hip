@@ -368,6 +368,7 @@ void scene_free(DevScene *s) {
     g = AxisGrid{};
   }
   if (s->db.lb) (void)hipFree(s->db.lb);
+  if (s->db.cut) (void)hipFree(s->db.cut);
   s->db = DistBound{};
 }
 
@@ -460,6 +461,50 @@ __global__ void k_db_edt(float *d, int nx, int ny, int nz, int axis, int *vwork,
   }
 }
 
+// The tree's cut (DistBound::cut) by one wave: from the root, the frontier node holding the most
+// triangles (the lowest lane on a tie) is replaced by its two children until CUT_BOXES nodes are
+// reached or only leaves remain. A pre-order subtree from node i to its escape e (or the end)
+// holds (e - i + 1) / 2 triangles.
+__global__ void k_db_cut(const Node *nodes, int m, int root, float *cut, int *n_cut) {
+  const int lane = threadIdx.x;
+  const int total = 2 * m - 1;
+  int node = lane == 0 ? root : -1;
+  int n = 1;
+  while (n < CUT_BOXES) {
+    int tri = 0;
+    if (node >= 0) {
+      const Node nd = nodes[node];
+      if (nd.a >= 0) tri = ((nd.esc < 0 ? total : nd.esc) - node + 1) / 2;
+    }
+    int best = tri, who = lane;
+    for (int o = 32; o > 0; o >>= 1) {
+      int bt = __shfl_xor(best, o), bw = __shfl_xor(who, o);
+      if (bt > best || (bt == best && bw < who)) {
+        best = bt;
+        who = bw;
+      }
+    }
+    if (best < 2) break;  // (uniform) only leaves left
+    const int split = __shfl(node, who);
+    if (lane == who) node = split + 1;              // the left child is the next node
+    if (lane == n) node = nodes[split].a;           // the right child
+    ++n;
+  }
+  if (node >= 0) {
+    const Node nd = nodes[node];
+    float *o = cut + 8 * lane;
+    o[0] = nd.lo[0];
+    o[1] = nd.lo[1];
+    o[2] = nd.lo[2];
+    o[3] = __int_as_float(node);
+    o[4] = nd.hi[0];
+    o[5] = nd.hi[1];
+    o[6] = nd.hi[2];
+    o[7] = 0.0f;
+  }
+  if (lane == 0) *n_cut = n;
+}
+
 // Squared cell distances to the stored bounds.
 __global__ void k_db_final(float *d, size_t n, float h) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -546,8 +591,8 @@ hipError_t scene_distance_bounds(DevScene *sc, hipStream_t s) {
   hipError_t err = hipSuccess;
   do {
     if ((err = dalloc(&g.lb, ncell)) != hipSuccess) break;
-    if ((err = dalloc(&diag, 1)) != hipSuccess) break;
-    hipLaunchKernelGGL(k_db_fill, dim3(1), dim3(BT), 0, s, diag, (size_t)1, 0.0f);
+    if ((err = dalloc(&diag, 2)) != hipSuccess) break;
+    hipLaunchKernelGGL(k_db_fill, dim3(1), dim3(BT), 0, s, diag, (size_t)2, 0.0f);
     // Row r of a pass uses entries [r n, r n + n) and [r (n + 1), r (n + 1) + n + 1).
     if ((err = dalloc(&vwork, ncell)) != hipSuccess) break;
     if ((err = dalloc(&fwork, ncell)) != hipSuccess) break;
@@ -561,10 +606,17 @@ hipError_t scene_distance_bounds(DevScene *sc, hipStream_t s) {
                          vwork, fwork, zwork);
     }
     hipLaunchKernelGGL(k_db_final, dim3(blocks_for((long)ncell, BT)), dim3(BT), 0, s, g.lb, ncell, g.h);
+    if ((err = dalloc(&g.cut, 8 * CUT_BOXES)) != hipSuccess) break;
+    // diag's second word takes the cut's size.
+    if (sc->n > 0 && sc->root >= 0)
+      hipLaunchKernelGGL(k_db_cut, dim3(1), dim3(64), 0, s, sc->nodes, sc->n, sc->root, g.cut, (int *)diag + 1);
     if ((err = hipGetLastError()) != hipSuccess) break;
-    float hd = 0.0f;
-    if ((err = hipMemcpyAsync(&hd, diag, sizeof hd, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
+    uint32_t hb[2] = {0u, 0u};
+    if ((err = hipMemcpyAsync(hb, diag, sizeof hb, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
     if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
+    float hd;
+    std::memcpy(&hd, &hb[0], sizeof hd);
+    g.n_cut = (int)hb[1];
     g.ub_add = (float)((2.0 * 1.7320508075688772 * h + (double)hd + 2.0 * (double)pad) * 1.001);
   } while (0);
   if (diag) (void)hipFree(diag);
@@ -573,6 +625,7 @@ hipError_t scene_distance_bounds(DevScene *sc, hipStream_t s) {
   if (zwork) (void)hipFree(zwork);
   if (err != hipSuccess) {
     if (g.lb) (void)hipFree(g.lb);
+    if (g.cut) (void)hipFree(g.cut);
     return err;
   }
   sc->db = g;

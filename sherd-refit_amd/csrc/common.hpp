@@ -209,7 +209,14 @@ struct DistBound {
   float org[3], h;
   int n[3];
   float ub_add;  // what dist_upper adds back: two cell diagonals, the largest triangle box, the pads
+  // A cut of the tree: up to CUT_BOXES nodes whose subtrees together hold every triangle once, as
+  // 8 floats each (lo xyz, the node index's bits, hi xyz, 0). Far outside the grid, where its
+  // bound degrades to the distance to the grid's box, the nearest of these boxes bounds the
+  // distance far more tightly (a curved shell's boxes hug it; its box does not).
+  float *cut;
+  int n_cut;
 };
+constexpr int CUT_BOXES = 64;
 
 struct SceneView {
   const Node *nodes;

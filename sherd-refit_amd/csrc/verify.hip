@@ -14,9 +14,8 @@
 //              voxel set of the seam points
 //   k_cont     R §6.3: B's margin against A's margin grid, steps and normal agreements
 //   k_pen      R §6.4: surface samples inside the other mesh, deeper-than-pen count, deepest
-//   k_pen_lb   R §6.4's fallback when nothing is inside: each side's sample of smallest bound
-//   k_pen_seed the exact distance of that sample, one wave each: a first attained window
-//   k_pen_min  the closest approach
+//   k_pen_close R §6.4's fallback when nothing is inside: the closest approach, one workgroup
+//              per (candidate, side), its blocks of samples nearest first
 //   k_pen_hard the closest-approach queries too long for one lane, one wave each
 //   k_median   the four medians (numpy semantics, verify.rs:693-700)
 //   k_final    Scores and the accept verdict
@@ -98,6 +97,136 @@ __device__ __forceinline__ float block_bound(const SceneView &sv, const MovedBlo
   return b > 0.0 ? (float)b * 0.99999f : 0.0f;
 }
 
+// ---- the tree's cut as a distance bound --------------------------------------------------------
+// Far from a mesh the distance-bound grid only knows the distance to its own box; the nearest of
+// the cut's boxes (DistBound::cut) is a far tighter bound there. Two uses: a filter that only
+// skips what a traversal would prune anyway, so every result is unchanged, and an estimate that
+// orders work (which block or sample to seed with).
+
+// What a block of samples can reach through the cut within w: lane l tests box l (the wave's
+// lanes hold the cut, one box each, in *lo / *hi), and the ballot is the boxes some sample of
+// the block may find a triangle within w in. A box is dropped when its gap from the centre, less
+// the radius and three times the slack (which covers the growth of the box by a sample's slack,
+// the rounding of the moved samples and of these distances), exceeds w. Every lane calls it; all
+// ones without a cut.
+__device__ __forceinline__ uint64_t cut_near_block(const SceneView &sv, const MovedBlock &m, float w, float *lo,
+                                                   float *hi) {
+  const int lane = threadIdx.x & 63;
+  const int n = sv.db.n_cut;
+  if (n == 0) return ~0ull;
+  bool near = false;
+  if (lane < n) {
+    const float *b = sv.db.cut + 8 * lane;
+    float g2 = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = b[k];
+      hi[k] = b[4 + k];
+      const float x = (float)m.c[k];
+      const float g = fmaxf(fmaxf(lo[k] - x, x - hi[k]), 0.0f);
+      g2 += g * g;
+    }
+    const float margin = 3.0f * (slack_of(sv.absmax, centre_f32(m)) + 1e-5f * (float)m.r);
+    near = sqrtf(g2) - (float)m.r - margin <= w;
+  }
+  return __ballot(near);
+}
+
+// The smallest squared distance from q to the boxes of `mask` grown by pad, in box_d2's own
+// arithmetic: when it exceeds a traversal's (window + pad)^2 the traversal, which tests each of
+// these nodes on the way to any triangle below them, measures nothing. *far2 gets the smallest
+// squared distance from q to the farthest point of one of these boxes: every triangle of a box
+// lies within that of q, so q's distance to the mesh does too (both 0 / +inf without a cut).
+// Every lane calls it, with the boxes in lo / hi of the lanes (cut_near_block, cut_lanes).
+__device__ __forceinline__ float cut_d2(const SceneView &sv, uint64_t mask, const float *lo, const float *hi, V3 q,
+                                        float pad, float *far2 = nullptr) {
+  if (far2) *far2 = __builtin_inff();
+  if (sv.db.n_cut == 0) return 0.0f;
+  float best = __builtin_inff(), fbest = __builtin_inff();
+  while (mask) {
+    const int k = __ffsll((unsigned long long)mask) - 1;
+    mask &= mask - 1;
+    Node nd;
+    for (int j = 0; j < 3; ++j) {
+      nd.lo[j] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lo[j]), k));
+      nd.hi[j] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hi[j]), k));
+    }
+    const float d2 = box_d2(nd, q, pad);
+    best = d2 < best ? d2 : best;
+    if (far2) {
+      const float f0 = fmaxf(q.x - nd.lo[0], nd.hi[0] - q.x), f1 = fmaxf(q.y - nd.lo[1], nd.hi[1] - q.y),
+                  f2 = fmaxf(q.z - nd.lo[2], nd.hi[2] - q.z);
+      fbest = fminf(fbest, (f0 * f0 + f1 * f1) + f2 * f2);
+    }
+  }
+  if (far2) *far2 = fbest;
+  return best;
+}
+
+// A search radius that certainly reaches a triangle, from a far-corner distance of the cut
+// (cut_d2's far2), allowing for the f32 rounding of that distance and of the triangle's.
+__device__ __forceinline__ float cut_reach(float far2, float pad) { return sqrtf(far2) * 1.0001f + pad; }
+
+// The cut with one box per lane (lanes past its size hold an empty box). Every lane calls it.
+struct CutLanes {
+  float lo[3], hi[3];
+};
+__device__ __forceinline__ CutLanes cut_lanes(const SceneView &sv) {
+  const int lane = threadIdx.x & 63;
+  CutLanes c;
+  for (int k = 0; k < 3; ++k) {
+    c.lo[k] = lane < sv.db.n_cut ? sv.db.cut[8 * lane + k] : __builtin_inff();
+    c.hi[k] = lane < sv.db.n_cut ? sv.db.cut[8 * lane + 4 + k] : -__builtin_inff();
+  }
+  return c;
+}
+// The smallest squared distance from each lane's x (a point, or a block's centre) to the cut's
+// boxes, read from the lanes that hold them. Every lane calls it.
+__device__ __forceinline__ float cut_lanes_d2(const SceneView &sv, const CutLanes &c, V3 x) {
+  float best = __builtin_inff();
+  for (int j = 0; j < sv.db.n_cut; ++j) {
+    float g2 = 0.0f;
+    const float xs[3] = {x.x, x.y, x.z};
+    for (int k = 0; k < 3; ++k) {
+      const float lo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.lo[k]), j));
+      const float hi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.hi[k]), j));
+      const float g = fmaxf(fmaxf(lo - xs[k], xs[k] - hi), 0.0f);
+      g2 += g * g;
+    }
+    best = fminf(best, g2);
+  }
+  return best;
+}
+// The boxes that can be the nearest for some sample of a block: those whose gap from the block
+// (the centre's gap less the radius) is no more than the smallest far reach of any box (the
+// farthest point of the box from the centre, plus the radius). The nearest box of any sample of
+// the block is one of them, up to rounding, which an estimate may ignore. One box per lane (the
+// lanes of cut_lanes); every lane calls it.
+__device__ __forceinline__ uint64_t cut_seed_mask(const SceneView &sv, const CutLanes &c, const MovedBlock &m) {
+  const int lane = threadIdx.x & 63;
+  if (sv.db.n_cut == 0) return 0;
+  const V3 x = centre_f32(m);
+  const float xs[3] = {x.x, x.y, x.z}, r = (float)m.r;
+  float g2 = 0.0f, f2 = 0.0f;
+  for (int k = 0; k < 3; ++k) {
+    const float g = fmaxf(fmaxf(c.lo[k] - xs[k], xs[k] - c.hi[k]), 0.0f);
+    const float f = fmaxf(xs[k] - c.lo[k], c.hi[k] - xs[k]);
+    g2 += g * g;
+    f2 += f * f;
+  }
+  const bool real = lane < sv.db.n_cut;
+  const float ub = real ? sqrtf(f2) + r : __builtin_inff();
+  const float ub_min = __uint_as_float(wave_min(__float_as_uint(ub)));
+  return __ballot(real && sqrtf(g2) - r <= ub_min);
+}
+
+// The seed's estimate for a moved sample: the grid's bound and the distance to the nearest cut
+// box among `mask` (cut_seed_mask of its block), the larger.
+__device__ __forceinline__ float sample_key_mask(const SceneView &sv, const CutLanes &c, uint64_t mask, V3 q) {
+  const float b = dist_bound(sv.db, q);
+  if (sv.db.n_cut == 0) return b;
+  return fmaxf(b, sqrtf(cut_d2(sv, mask, c.lo, c.hi, q, 0.0f)));
+}
+
 __device__ __forceinline__ MovedBlock moved_sphere(const double *s, const double *T) {
   MovedBlock m;
   apply_f64(T, s, m.c);
@@ -118,15 +247,6 @@ __device__ __forceinline__ TaskGroup task_group(const VerifyBatch &vb, int per_t
   return TaskGroup{(int)(cs >> 1), (int)(cs & 1u), (int)(lb % (uint32_t)per_task) * CHUNKS};
 }
 
-struct TaskPos {
-  int c, side, i;
-};
-__device__ __forceinline__ TaskPos task_pos(const VerifyBatch &vb, int per_task) {
-  uint32_t lb = xcd_logical(blockIdx.x, gridDim.x);
-  uint32_t cs = vb.task[lb / (uint32_t)per_task];
-  return TaskPos{(int)(cs >> 1), (int)(cs & 1u), (int)((lb % (uint32_t)per_task) * VT + threadIdx.x)};
-}
-
 __global__ void k_prep(VerifyBatch vb) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= vb.n_cand) return;
@@ -144,7 +264,6 @@ __global__ void k_prep(VerifyBatch vb) {
   if (c == 0) vb.hard_n[0] = vb.hard_n[1] = 0;
   a.seam_cells = 0;
   a.flags = 0;
-  a.seed[0] = a.seed[1] = ~0ull;
 }
 
 // R §6.1 one_side inputs (verify.rs:318-349): side 0 is A's samples moved by T^-1 against B's
@@ -426,67 +545,160 @@ __device__ __forceinline__ float closest_window(CandAcc &acc, int side) {
 // min(sd) is then the closest approach, a running minimum over the samples whose window shrinks to
 // the best attained so far (DistReduce::Min, cpu.rs:100-111), from f32::MAX.
 
-// The seed's first half: each side's 64-sample block of smallest bound (the distance bound at the
-// moved centre less the radius), (bound bits << 32) | block, kept by an atomic minimum. One lane
-// per block.
-__global__ void __launch_bounds__(VT) k_pen_lb(VerifyBatch vb, int per_task) {
-  TaskPos tp = task_pos(vb, per_task);
-  const int c = tp.c, side = tp.side, b = tp.i;
-  const CandDesc &cd = vb.cand[c];
-  const PairDesc &pd = vb.pair[cd.pair];
-  if (!pd.pen_ok) return;
-  CandAcc &acc = vb.acc[c];
-  if (!closest_needed(acc, side)) return;
-  const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
-  const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
-  uint64_t key = ~0ull;
-  if (64 * (size_t)b < (size_t)own.n_s) {
-    const double *T = side == 0 ? cd.Ti : cd.T;
-    key = ((uint64_t)__float_as_uint(block_bound(other.mesh, moved_block(own.s_blk, 64 * b, T))) << 32) | (uint32_t)b;
+// The closest approach per task, one workgroup each. A seed sample's exact distance is the first
+// window (see below); then the task's 64-sample blocks are taken in order of a lower bound (the
+// larger of the grid's block bound and the cut's, with the margins cut_near_block uses), so the
+// nearest blocks shrink the window first, and once the least bound left exceeds the window every
+// remaining block is known to hold nothing below it. The waves share the ordered blocks, each
+// claiming the least one left; a task with more blocks than CLOSE_MAXB takes them in index order,
+// each tested the same way.
+//
+// In a block, one lane per sample: a sample whose grid bound, or distance to every cut box the
+// block can reach, already exceeds the window (by more than the slack, which covers the rounding
+// of the moved point) cannot lower the minimum; the others search (min_d2) no farther than the
+// window and their own upper bounds. A query that needs more than PEN_STEPS nodes is finished by
+// a whole wave (k_pen_hard); what it met so far is a distance it attains, so it may still lower
+// the window.
+constexpr int CLOSE_MAXB = 2048;
+constexpr int SEED_BLOCKS = 64;
+constexpr uint32_t CLAIMED = 0xffffffffu;  // above every key (non-negative f32 bits)
+
+__device__ __forceinline__ float close_key(const SceneView &sv, const CutLanes &c, const MovedBlock &m) {
+  float b = block_bound(sv, m);
+  if (sv.db.n_cut > 0) {
+    const float margin = 3.0f * (slack_of(sv.absmax, centre_f32(m)) + 1e-5f * (float)m.r);
+    b = fmaxf(b, sqrtf(cut_lanes_d2(sv, c, centre_f32(m))) - (float)m.r - margin);
   }
-  const uint32_t hi = wave_min((uint32_t)(key >> 32));
-  const uint32_t lo = wave_min((uint32_t)(key >> 32) == hi ? (uint32_t)key : 0xffffffffu);
-  if ((threadIdx.x & 63) == 0 && hi != 0xffffffffu)
-    atomicMin((unsigned long long *)&acc.seed[side], ((unsigned long long)hi << 32) | lo);
+  return fmaxf(b, 0.0f);
 }
 
-// The exact pass, one lane per sample: a sample whose distance bound already exceeds the window (by
-// more than the slack, which covers the rounding of the moved point) cannot lower the minimum; the
-// others search (min_d2). A query that needs more than PEN_STEPS nodes is finished by a whole wave
-// (k_pen_hard); what it met so far is a distance it attains, so it may still lower the window.
-__global__ void __launch_bounds__(VT) k_pen_min(VerifyBatch vb, int per_task) {
+__global__ void __launch_bounds__(VT) k_pen_close(VerifyBatch vb) {
   __shared__ uint2 s_stk[LSTACK][VT];
-  const TaskGroup g = task_group(vb, per_task);
-  const int c = g.c, side = g.side;
+  __shared__ uint32_t s_key[CLOSE_MAXB];
+  __shared__ uint32_t s_next;
+  const uint32_t cs = vb.task[xcd_logical(blockIdx.x, gridDim.x)];
+  const int c = (int)(cs >> 1), side = (int)(cs & 1u);
   const CandDesc &cd = vb.cand[c];
   const PairDesc &pd = vb.pair[cd.pair];
-  if (!pd.pen_ok) return;
+  if (!pd.pen_ok) return;  // (uniform: the workgroup is one task)
   CandAcc &acc = vb.acc[c];
   if (!closest_needed(acc, side)) return;
   const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
   const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
   const double *T = side == 0 ? cd.Ti : cd.T;
   if (own.n_s == 0) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nb = (own.n_s + 63) >> 6;
+  const bool ordered = nb <= CLOSE_MAXB;
+  const CutLanes cut = cut_lanes(other.mesh);
+  if (ordered) {
+    for (int base = 64 * w; base < nb; base += VT) {  // (uniform per wave)
+      const int b = base + lane;
+      const float k = close_key(other.mesh, cut, moved_block(own.s_blk, 64 * (b < nb ? b : 0), T));
+      if (b < nb) s_key[b] = __float_as_uint(k);
+    }
+  }
+  if (threadIdx.x == 0) s_next = 0;
+  __syncthreads();
+  // The seed (wave 0, before the others search): the sample of least estimate (the grid's bound or
+  // the nearest cut box, the larger) over the blocks in key order while a block's key, less the
+  // grid's cell diagonal by which a sample's bound may fall below its block's, does not exceed the
+  // best estimate so far (at most SEED_BLOCKS blocks); its exact distance, by the whole wave with
+  // the lane stacks' memory as its stack, is the first window. Which sample seeds changes the
+  // work, never the minimum.
+  if (ordered && w == 0) {
+    const float cell = 1.8f * other.mesh.db.h;
+    uint64_t last = 0;  // (key << 32 | block) + 1 of the last block visited
+    uint32_t khi = CLAIMED, seed = 0xffffffffu;
+    for (int it = 0; it < SEED_BLOCKS; ++it) {
+      uint64_t nxt = ~0ull;
+      for (int j = lane; j < nb; j += 64) {
+        const uint64_t v = ((uint64_t)s_key[j] << 32) | (uint32_t)j;
+        if (v >= last && v < nxt) nxt = v;
+      }
+      const uint32_t hi = wave_min((uint32_t)(nxt >> 32));
+      const uint32_t lo = wave_min((uint32_t)(nxt >> 32) == hi ? (uint32_t)nxt : 0xffffffffu);
+      if (lo == 0xffffffffu) break;  // (uniform) no block left
+      if (khi != CLAIMED && __uint_as_float(hi) - cell > __uint_as_float(khi)) break;
+      last = (((uint64_t)hi << 32) | lo) + 1;
+      const int blk = (int)lo, i = 64 * blk + lane;
+      const uint64_t mask = cut_seed_mask(other.mesh, cut, moved_block(own.s_blk, 64 * blk, T));
+      const double *pi = own.s + 3 * (size_t)(i < own.n_s ? i : 0);
+      const float k = sample_key_mask(other.mesh, cut, mask, moved_f32(T, pi[0], pi[1], pi[2]));
+      uint64_t k2 = i < own.n_s ? (((uint64_t)__float_as_uint(k) << 32) | (uint32_t)i) : ~0ull;
+      const uint64_t cur = ((uint64_t)khi << 32) | seed;
+      k2 = k2 < cur ? k2 : cur;
+      khi = wave_min((uint32_t)(k2 >> 32));
+      seed = wave_min((uint32_t)(k2 >> 32) == khi ? (uint32_t)k2 : 0xffffffffu);
+    }
+    if (seed != 0xffffffffu) {
+      const double *p = own.s + 3 * (size_t)seed;
+      const V3 q = moved_f32(T, p[0], p[1], p[2]);
+      const float window =
+          __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(closest_window(acc, side))));
+      float far2;
+      (void)cut_d2(other.mesh, other.mesh.db.n_cut ? ~0ull >> (64 - other.mesh.db.n_cut) : 0ull, cut.lo, cut.hi, q,
+                   0.0f, &far2);
+      const float radius =
+          fminf(fminf(window, dist_upper(other.mesh.db, q)), cut_reach(far2, slack_of(other.mesh.absmax, q)));
+      bool complete;
+      const float dist = sqrtf(min_d2_wave(other.mesh, q, radius, reinterpret_cast<int *>(&s_stk[0][0]),
+                                           (int)(sizeof s_stk / sizeof(int)), WAVE_STEPS, &complete));
+      if (lane == 0 && dist <= window) atomicMin(&acc.closest[side], __float_as_uint(dist));
+      if (lane == 0 && !complete) atomicOr(&acc.flags, FLAG_GAVE_UP);
+    }
+  }
+  __syncthreads();
   LdsStack<VT> st{&s_stk[0][threadIdx.x]};
-  for (int k = 0; k < CHUNKS; ++k) {
-    const int base = (g.chunk0 + k) * VT;
-    if (base >= own.n_s) break;  // (uniform)
-    const int i = base + threadIdx.x, i0 = i & ~63;
-    if (i0 >= own.n_s) break;
-    // A block every sample of which is farther than the window (read once for the wave) cannot
-    // lower the minimum: the wave goes on to its next block. Uniform across the wave.
-    const float w0 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(closest_window(acc, side))));
-    if (block_bound(other.mesh, moved_block(own.s_blk, i0, T)) > w0) continue;
+  for (;;) {
+    int b;
+    float w0 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(closest_window(acc, side))));
+    if (ordered) {
+      uint32_t kmin = CLAIMED, bmin = 0xffffffffu;
+      for (int j = lane; j < nb; j += 64) {
+        const uint32_t k = __hip_atomic_load(&s_key[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (k < kmin) {
+          kmin = k;
+          bmin = (uint32_t)j;
+        }
+      }
+      const uint32_t khi = wave_min(kmin);
+      if (khi == CLAIMED || __uint_as_float(khi) > w0) break;  // (uniform)
+      b = (int)wave_min(kmin == khi ? bmin : 0xffffffffu);
+      uint32_t old = 0;
+      if (lane == 0) old = atomicExch(&s_key[b], CLAIMED);
+      if (__builtin_amdgcn_readfirstlane(__shfl(old, 0)) == CLAIMED) continue;  // another wave took it
+    } else {
+      uint32_t nx = 0;
+      if (lane == 0) nx = atomicAdd(&s_next, 1u);
+      b = (int)__builtin_amdgcn_readfirstlane(__shfl(nx, 0));
+      if (b >= nb) break;
+    }
+    const int i0 = 64 * b, i = i0 + lane;
+    const MovedBlock mb = moved_block(own.s_blk, i0, T);
+    if (block_bound(other.mesh, mb) > w0) continue;
+    // The cut's boxes the block can reach within the window; none: the next block.
+    float clo[3], chi[3];
+    const uint64_t near = cut_near_block(other.mesh, mb, w0, clo, chi);
+    if (near == 0) continue;
+    const bool have = i < own.n_s;
+    V3 q = V3{0.0f, 0.0f, 0.0f};
+    if (have) {
+      const double *p = own.s + 3 * (size_t)i;
+      q = moved_f32(T, p[0], p[1], p[2]);
+    }
+    const float pad = slack_of(other.mesh.absmax, q);
+    float far2;
+    const float near_d2 = cut_d2(other.mesh, near, clo, chi, q, pad, &far2);
     uint32_t best = 0xffffffffu;
     bool hard = false;
-    if (i < own.n_s) {
-      const double *p = own.s + 3 * (size_t)i;
-      V3 q = moved_f32(T, p[0], p[1], p[2]);
+    if (have) {
       float window = closest_window(acc, side);
-      if (dist_bound(other.mesh.db, q) <= window + slack_of(other.mesh.absmax, q) && window > 0.0f) {
+      const float lim = window + pad;
+      if (dist_bound(other.mesh.db, q) <= lim && window > 0.0f && near_d2 <= lim * lim) {
         bool complete;
-        float dist = sqrtf(min_d2_with(st, other.mesh, q, fminf(window, dist_upper(other.mesh.db, q)), PEN_STEPS,
-                                       &complete));
+        const float radius = fminf(fminf(window, dist_upper(other.mesh.db, q)), cut_reach(far2, pad));
+        float dist = sqrtf(min_d2_with(st, other.mesh, q, radius, PEN_STEPS, &complete));
         if (dist <= window) best = __float_as_uint(dist);
         hard = !complete;
       }
@@ -494,7 +706,7 @@ __global__ void __launch_bounds__(VT) k_pen_min(VerifyBatch vb, int per_task) {
     uint32_t slot = wave_slot(&vb.hard_n[0], hard);
     if (hard) vb.hard[slot] = ((uint64_t)c << 33) | ((uint64_t)side << 32) | (uint32_t)i;
     best = wave_min(best);
-    if ((threadIdx.x & 63) == 0 && best != 0xffffffffu) atomicMin(&acc.closest[side], best);
+    if (lane == 0 && best != 0xffffffffu) atomicMin(&acc.closest[side], best);
   }
 }
 
@@ -551,80 +763,7 @@ __device__ K select_kth(const K *keys, int n, int k, uint32_t *hist, K *s_prefix
   return prefix;
 }
 
-// The closest approach is a minimum over every sample, and a search is cheap once a small
-// window is known. Before the exact pass, each task's sample of smallest distance bound is found
-// (k_pen_lb over blocks, then k_pen_seed over the samples of the blocks that can hold it), and the
-// same wave measures that sample's exact distance (min_d2_wave): a distance the samples really attain, so at least the true minimum,
-// and, the bound being within a couple of grid cells of the distance, close to it. The minimum that
-// comes out is the same whichever sample seeds it; only the work changes.
-__global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb) {
-  __shared__ int stk[VT / 64][HARD_STACK];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int t = (int)blockIdx.x * (VT / 64) + w;
-  if (t >= 2 * vb.n_cand) return;  // (uniform across the wave; the kernel has no block barrier)
-  const uint32_t cs = vb.task[t];
-  const int c = (int)(cs >> 1), side = (int)(cs & 1u);
-  const CandDesc &cd = vb.cand[c];
-  const PairDesc &pd = vb.pair[cd.pair];
-  if (!pd.pen_ok) return;
-  CandAcc &acc = vb.acc[c];
-  if (!closest_needed(acc, side)) return;
-  const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
-  const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
-  const double *T = side == 0 ? cd.Ti : cd.T;
-  const uint64_t key = acc.seed[side];
-  if (key == ~0ull) return;
-  // The task's sample of smallest distance bound, (bound bits, index) least: first over the block
-  // of smallest block bound, then over every other block whose block bound (less the bound grid's
-  // cell diagonal, by which a sample's bound may fall below its block's) does not exceed the best
-  // so far; the blocks are tested 64 at a time, each qualifying one by the whole wave.
-  const int bstar = (int)(uint32_t)key;
-  const int nb = (own.n_s + 63) >> 6;
-  const float cell = 1.8f * other.mesh.db.h;
-  uint32_t khi, seed;
-  {
-    const int i = 64 * bstar + lane;
-    uint64_t k2 = ~0ull;
-    if (i < own.n_s) {
-      const double *pi = own.s + 3 * (size_t)i;
-      k2 = ((uint64_t)__float_as_uint(dist_bound(other.mesh.db, moved_f32(T, pi[0], pi[1], pi[2]))) << 32) | (uint32_t)i;
-    }
-    khi = wave_min((uint32_t)(k2 >> 32));
-    seed = wave_min((uint32_t)(k2 >> 32) == khi ? (uint32_t)k2 : 0xffffffffu);
-  }
-  if (seed == 0xffffffffu) return;  // (uniform)
-  for (int base = 0; base < nb; base += 64) {
-    const int b = base + lane;
-    const bool take = b < nb && b != bstar &&
-                      block_bound(other.mesh, moved_block(own.s_blk, 64 * b, T)) - cell <= __uint_as_float(khi);
-    uint64_t m = __ballot(take);
-    while (m) {
-      const int bb = base + __ffsll((unsigned long long)m) - 1;
-      m &= m - 1;
-      const int i = 64 * bb + lane;
-      uint64_t k2 = ~0ull;
-      if (i < own.n_s) {
-        const double *pi = own.s + 3 * (size_t)i;
-        k2 = ((uint64_t)__float_as_uint(dist_bound(other.mesh.db, moved_f32(T, pi[0], pi[1], pi[2]))) << 32) | (uint32_t)i;
-      }
-      const uint64_t cur = ((uint64_t)khi << 32) | seed;
-      k2 = k2 < cur ? k2 : cur;
-      khi = wave_min((uint32_t)(k2 >> 32));
-      seed = wave_min((uint32_t)(k2 >> 32) == khi ? (uint32_t)k2 : 0xffffffffu);
-    }
-  }
-  const double *p = own.s + 3 * (size_t)seed;
-  V3 q = moved_f32(T, p[0], p[1], p[2]);
-  float window = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(closest_window(acc, side))));
-  // Searched no farther than the seed's upper bound: the exact distance is within it.
-  const float radius = fminf(window, dist_upper(other.mesh.db, q));
-  bool complete;
-  float dist = sqrtf(min_d2_wave(other.mesh, q, radius, stk[w], HARD_STACK, WAVE_STEPS, &complete));
-  if (lane == 0 && dist <= window) atomicMin(&acc.closest[side], __float_as_uint(dist));
-  if (lane == 0 && !complete) atomicOr(&acc.flags, FLAG_GAVE_UP);
-}
-
-// The closest-approach queries k_pen_min left unfinished, one per wave (min_d2_wave), taken from
+// The closest-approach queries k_pen_close left unfinished, one per wave (min_d2_wave), taken from
 // the queue until it is empty.
 __global__ void __launch_bounds__(VT) k_pen_hard(VerifyBatch vb) {
   __shared__ int stk[VT / 64][HARD_STACK];
@@ -646,11 +785,29 @@ __global__ void __launch_bounds__(VT) k_pen_hard(VerifyBatch vb) {
     const double *p = own.s + 3 * (size_t)i;
     V3 q = moved_f32(T, p[0], p[1], p[2]);
     float window = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(closest_window(acc, side))));
-    // The window may have shrunk since the query was queued: one beyond it is dropped unwalked.
-    if (!(dist_bound(other.mesh.db, q) <= window + slack_of(other.mesh.absmax, q))) continue;
-    // The search needs to reach no farther than the point's own upper bound (its distance is
-    // within it), which prunes from the first node on.
-    const float radius = fminf(window, dist_upper(other.mesh.db, q));
+    // The window may have shrunk since the query was queued: one beyond it, by the grid's bound
+    // or by every box of the cut (a lane each), is dropped unwalked.
+    const float pad = slack_of(other.mesh.absmax, q), lim = window + pad;
+    float bd2 = other.mesh.db.n_cut == 0 ? 0.0f : __builtin_inff(), fd2 = __builtin_inff();
+    if (lane < other.mesh.db.n_cut) {
+      const float *b = other.mesh.db.cut + 8 * lane;
+      Node nd;
+      for (int k = 0; k < 3; ++k) {
+        nd.lo[k] = b[k];
+        nd.hi[k] = b[4 + k];
+      }
+      bd2 = box_d2(nd, q, pad);
+      const float f0 = fmaxf(q.x - nd.lo[0], nd.hi[0] - q.x), f1 = fmaxf(q.y - nd.lo[1], nd.hi[1] - q.y),
+                  f2 = fmaxf(q.z - nd.lo[2], nd.hi[2] - q.z);
+      fd2 = (f0 * f0 + f1 * f1) + f2 * f2;
+    }
+    const float cut_min = __uint_as_float(wave_min(__float_as_uint(bd2)));
+    const float far2 = __uint_as_float(wave_min(__float_as_uint(fd2)));
+    if (!(dist_bound(other.mesh.db, q) <= lim) || cut_min > lim * lim) continue;
+    // The search needs to reach no farther than the point's own upper bounds (its distance is
+    // within them: the grid's, and the cut's nearest far corner), which prunes from the first
+    // node on.
+    const float radius = fminf(fminf(window, dist_upper(other.mesh.db, q)), cut_reach(far2, pad));
     bool complete;
     float dist = sqrtf(min_d2_wave(other.mesh, q, radius, stk[w], HARD_STACK, WAVE_STEPS, &complete));
     if (lane == 0 && dist <= window) atomicMin(&acc.closest[side], __float_as_uint(dist));
@@ -827,15 +984,8 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
     E(s3);
   }
   if (pen) {
-    B("k_pen_lb", s);
-    const int per_b = blocks_for(blocks_for(max_s, 64), VT);
-    hipLaunchKernelGGL(k_pen_lb, dim3(2 * nc * per_b), dim3(VT), 0, s, vb, per_b);
-    E(s);
-    B("k_pen_seed", s);
-    hipLaunchKernelGGL(k_pen_seed, dim3(blocks_for(2L * nc, VT / 64)), dim3(VT), 0, s, vb);
-    E(s);
-    B("k_pen_min", s);
-    hipLaunchKernelGGL(k_pen_min, dim3(2 * nc * per_sg), dim3(VT), 0, s, vb, per_sg);
+    B("k_pen_close", s);
+    hipLaunchKernelGGL(k_pen_close, dim3(2 * nc), dim3(VT), 0, s, vb);
     E(s);
     B("k_pen_hard", s);
     hipLaunchKernelGGL(k_pen_hard, dim3(HARD_BLOCKS), dim3(VT), 0, s, vb);

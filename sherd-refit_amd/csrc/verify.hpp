@@ -71,7 +71,6 @@ struct CandAcc {
   uint32_t in_box[2];  // samples that passed the mesh-box reject of the inside test, per side
   uint32_t pad;
   double med[4];
-  uint64_t seed[2];  // per side, (block bound bits << 32) | the 64-sample block of the smallest bound
 };
 
 struct VerifyBatch {
