@@ -291,6 +291,7 @@ void sherd_close(sherd_ctx *ctx) {
   for (auto &b : c->bufs)
     if (b.p) (void)hipFree(b.p);
   if (c->host.p) (void)hipHostFree(c->host.p);
+  if (c->huber_spare.p) (void)hipFree(c->huber_spare.p);
   if (c->receipt_dev) (void)hipFree(c->receipt_dev);
   if (c->receipt_host) (void)hipHostFree(c->receipt_host);
   if (c->ladder_exec) (void)hipGraphExecDestroy(c->ladder_exec);

@@ -18,6 +18,9 @@ struct sherd_huber {
   int k = 0;
   size_t points = 0;
   std::vector<size_t> off;
+  // Every device array below lives in one block (`block`, `block_n` bytes), carved in this order.
+  void *block = nullptr;
+  size_t block_n = 0;
   IcpPairDesc *pairs = nullptr;
   uint32_t *cand_pair = nullptr;
   size_t *doff = nullptr;
@@ -48,17 +51,37 @@ struct sherd_huber {
 
 namespace {
 
+// Returns a batch's block to its context as the spare (keeping the larger of the two blocks), or
+// frees it, and deletes the batch.
 void free_huber(sherd_huber *h) {
-  void *ptrs[] = {h->pairs, h->cand_pair, h->doff, h->T0, h->corr, h->lm_corr, h->xi, h->E, h->G, h->T_out, h->steps,
-                  h->lm};
-  for (void *p : ptrs)
-    if (p) (void)hipFree(p);
+  Ctx *c = h->ctx;
+  if (h->block) {
+    if (!c->huber_spare.p || c->huber_spare.n < h->block_n) {
+      if (c->huber_spare.p) (void)hipFree(c->huber_spare.p);
+      c->huber_spare.p = h->block;
+      c->huber_spare.n = h->block_n;
+    } else {
+      (void)hipFree(h->block);
+    }
+  }
   delete h;
 }
 
-template <class X>
-hipError_t dnew(X **p, size_t n) {
-  return hipMalloc((void **)p, sizeof(X) * (n ? n : 1));
+size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// The byte offsets of a batch's arrays in its block, in the struct's order; the last entry is the
+// block's size. The first four (pairs, candidate pairs, offsets, base poses) are the upload.
+void huber_layout(size_t n_pairs, size_t k, size_t points, size_t at[13]) {
+  const size_t sz[12] = {sizeof(IcpPairDesc) * n_pairs, sizeof(uint32_t) * k, sizeof(size_t) * k,
+                         sizeof(double) * 16 * k,       sizeof(int32_t) * points, sizeof(int32_t) * points,
+                         sizeof(double) * 6 * k,        sizeof(double) * k,   sizeof(double) * 6 * k,
+                         sizeof(double) * 16 * k,       sizeof(uint32_t) * k, sizeof(LmState) * k};
+  size_t b = 0;
+  for (int i = 0; i < 12; ++i) {
+    at[i] = b;
+    b += up256(sz[i] ? sz[i] : 1);
+  }
+  at[12] = b;
 }
 
 }  // namespace
@@ -104,29 +127,53 @@ int sherd_huber_create(sherd_ctx *ctx, const sherd_icp_pair *pairs, size_t n_pai
     h->points += (size_t)pd[cand_pair[i]].ns;
     h->max_ns = std::max(h->max_ns, pd[cand_pair[i]].ns);
   }
+  size_t at[13];
+  huber_layout(n_pairs, k, h->points, at);
   hipError_t e = hipSuccess;
   do {
-    if ((e = dnew(&h->pairs, n_pairs)) != hipSuccess) break;
-    if ((e = dnew(&h->cand_pair, k)) != hipSuccess) break;
-    if ((e = dnew(&h->doff, k)) != hipSuccess) break;
-    if ((e = dnew(&h->T0, 16 * k)) != hipSuccess) break;
-    if ((e = dnew(&h->corr, h->points)) != hipSuccess) break;
-    if ((e = dnew(&h->lm_corr, h->points)) != hipSuccess) break;
-    if ((e = dnew(&h->xi, 6 * k)) != hipSuccess) break;
-    if ((e = dnew(&h->E, k)) != hipSuccess) break;
-    if ((e = dnew(&h->G, 6 * k)) != hipSuccess) break;
-    if ((e = dnew(&h->T_out, 16 * k)) != hipSuccess) break;
-    if ((e = dnew(&h->steps, k)) != hipSuccess) break;
-    if ((e = dnew(&h->lm, k)) != hipSuccess) break;
-    if ((e = hipMemcpyAsync(h->pairs, pd.data(), sizeof(IcpPairDesc) * n_pairs, hipMemcpyHostToDevice, c->stream)))
-      break;
-    if ((e = hipMemcpyAsync(h->cand_pair, cand_pair, sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream))) break;
-    if ((e = hipMemcpyAsync(h->doff, h->off.data(), sizeof(size_t) * k, hipMemcpyHostToDevice, c->stream))) break;
-    if ((e = hipMemcpyAsync(h->T0, T0, sizeof(double) * 16 * k, hipMemcpyHostToDevice, c->stream))) break;
+    // The context's spare block when it is large enough, else a new one.
+    if (c->huber_spare.p && c->huber_spare.n >= at[12]) {
+      h->block = c->huber_spare.p;
+      h->block_n = c->huber_spare.n;
+      c->huber_spare.p = nullptr;
+      c->huber_spare.n = 0;
+    } else {
+      if (take_fault(c, SHERD_FAULT_NOMEM)) {
+        e = hipErrorOutOfMemory;
+        break;
+      }
+      if ((e = hipMalloc(&h->block, at[12])) != hipSuccess) {
+        h->block = nullptr;
+        break;
+      }
+      h->block_n = at[12];
+    }
+    char *b = (char *)h->block;
+    h->pairs = (IcpPairDesc *)(b + at[0]);
+    h->cand_pair = (uint32_t *)(b + at[1]);
+    h->doff = (size_t *)(b + at[2]);
+    h->T0 = (double *)(b + at[3]);
+    h->corr = (int32_t *)(b + at[4]);
+    h->lm_corr = (int32_t *)(b + at[5]);
+    h->xi = (double *)(b + at[6]);
+    h->E = (double *)(b + at[7]);
+    h->G = (double *)(b + at[8]);
+    h->T_out = (double *)(b + at[9]);
+    h->steps = (uint32_t *)(b + at[10]);
+    h->lm = (LmState *)(b + at[11]);
+    // The four inputs go up in one copy from the pinned staging buffer, laid out as in the block.
+    char *hp = (char *)c->pinned(at[4], &e);
+    if (!hp) break;
+    std::memcpy(hp + at[0], pd.data(), sizeof(IcpPairDesc) * n_pairs);
+    std::memcpy(hp + at[1], cand_pair, sizeof(uint32_t) * k);
+    std::memcpy(hp + at[2], h->off.data(), sizeof(size_t) * k);
+    std::memcpy(hp + at[3], T0, sizeof(double) * 16 * k);
+    if ((e = hipMemcpyAsync(b, hp, at[4], hipMemcpyHostToDevice, c->stream)) != hipSuccess) break;
     if ((e = launch_correspond(h->view(), h->T0, h->corr, c->stream)) != hipSuccess) break;
     e = hipStreamSynchronize(c->stream);
   } while (0);
   if (e != hipSuccess) {
+    (void)hipStreamSynchronize(c->stream);
     free_huber(h);
     return hip_fail(e, "huber batch");
   }

@@ -269,11 +269,16 @@ int grid_for(Ctx *c, int block, size_t n, int max_spans) {
 struct BatchDev {
   std::vector<IcpBatchView> v;
   std::vector<IcpWideView> w;  // the wide view of each rung (used by the wide ones)
-  std::vector<char> host;      // the upload's host copy, alive until the stream has drained
+  // The context's pinned staging buffer: the blob and the candidate pairs at its front (read by
+  // copies until the stream drains), then `extra` bytes from `pin_at` for the caller.
+  char *pin = nullptr;
+  size_t pin_at = 0;
 };
 
+size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
+
 int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand_pair, size_t n_cand, double *dT,
-                double *dout, BatchDev *bd) {
+                double *dout, size_t extra, BatchDev *bd) {
   hipError_t e;
   size_t nr = plans.size();
   size_t max_points = 1;
@@ -302,8 +307,13 @@ int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand
     at[F * r + 4] = bytes;
     bytes = al(bytes + sizeof(uint32_t) * plans[r].span_first.size());
   }
-  std::vector<char> &blob = bd->host;
-  blob.assign(bytes, 0);
+  const size_t at_cp = up256(bytes);
+  bd->pin_at = up256(at_cp + sizeof(uint32_t) * n_cand);
+  bd->pin = (char *)c->pinned(bd->pin_at + extra, &e);
+  if (!bd->pin) return hip_fail(e, "pinned staging");
+  char *blob = bd->pin;
+  std::memset(blob, 0, bytes);
+  std::memcpy(blob + at_cp, cand_pair, sizeof(uint32_t) * n_cand);
   for (size_t r = 0; r < nr; ++r) {
     const RungPlan &pl = plans[r];
     std::memcpy(&blob[at[F * r]], pl.pd.data(), sizeof(IcpPairDesc) * pl.pd.size());
@@ -324,8 +334,8 @@ int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand
   if (!dpart) return hip_fail(e, "scratch");
   IcpWideState *dst = (IcpWideState *)c->scratch(SL_WIDE, sizeof(IcpWideState) * n_cand, &e);
   if (!dst) return hip_fail(e, "scratch");
-  HIPCK(hipMemcpyAsync(dblob, blob.data(), bytes, hipMemcpyHostToDevice, c->stream), "upload");
-  HIPCK(hipMemcpyAsync(dcp, cand_pair, sizeof(uint32_t) * n_cand, hipMemcpyHostToDevice, c->stream), "upload");
+  HIPCK(hipMemcpyAsync(dblob, blob, bytes, hipMemcpyHostToDevice, c->stream), "upload");
+  HIPCK(hipMemcpyAsync(dcp, blob + at_cp, sizeof(uint32_t) * n_cand, hipMemcpyHostToDevice, c->stream), "upload");
   bd->v.assign(nr, IcpBatchView{});
   bd->w.assign(nr, IcpWideView{});
   for (size_t r = 0; r < nr; ++r) {
@@ -447,12 +457,20 @@ int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_
         any_skip = true;
       }
   BatchDev bd;
-  // Every return below leaves with the stream drained, so the copies queued from `skip` and
-  // `bd.host` have finished before those buffers go.
+  // Every return below leaves with the stream drained, so the copies queued from the pinned
+  // staging buffer have finished before another call reuses it.
   StreamFence fence(c->stream);
-  if ((rc = stage_batch(c, plans, cand_pair, n_cand, dT, dout, &bd))) return rc;
-  HIPCK(hipMemcpyAsync(dT, inits, sizeof(double) * 16 * n_cand, hipMemcpyHostToDevice, c->stream), "upload");
-  HIPCK(hipMemcpyAsync(dskip, skip.data(), skip.size(), hipMemcpyHostToDevice, c->stream), "upload");
+  // After the blob in the staging buffer: the initial poses and skip bytes going up, the outputs
+  // and poses coming down.
+  const size_t a_init = 0, a_skip = up256(sizeof(double) * 16 * n_cand), a_out = a_skip + up256(skip.size()),
+               a_T = a_out + up256(sizeof(double) * 21 * n_cand * n_rungs),
+               extra = a_T + sizeof(double) * 16 * n_cand * (n_rungs + 1);
+  if ((rc = stage_batch(c, plans, cand_pair, n_cand, dT, dout, extra, &bd))) return rc;
+  char *hp = bd.pin + bd.pin_at;
+  std::memcpy(hp + a_init, inits, sizeof(double) * 16 * n_cand);
+  std::memcpy(hp + a_skip, skip.data(), skip.size());
+  HIPCK(hipMemcpyAsync(dT, hp + a_init, sizeof(double) * 16 * n_cand, hipMemcpyHostToDevice, c->stream), "upload");
+  HIPCK(hipMemcpyAsync(dskip, hp + a_skip, skip.size(), hipMemcpyHostToDevice, c->stream), "upload");
   bool tuned = true;
   std::vector<int> blocks(n_rungs, 0), grids(n_rungs, 0);
   for (size_t r = 0; r < n_rungs; ++r) {
@@ -519,10 +537,11 @@ int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_
     HIPCK(hipGraphLaunch(c->ladder_exec, c->stream), "graph launch");
   }
   tm.stop();
-  std::vector<double> h(21 * n_cand * n_rungs);
-  HIPCK(hipMemcpyAsync(h.data(), dout, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream), "download");
-  std::vector<double> hT(16 * n_cand * (n_rungs + 1));
-  HIPCK(hipMemcpyAsync(hT.data(), dT, sizeof(double) * hT.size(), hipMemcpyDeviceToHost, c->stream), "download");
+  const double *h = (const double *)(hp + a_out), *hT = (const double *)(hp + a_T);
+  HIPCK(hipMemcpyAsync(hp + a_out, dout, sizeof(double) * 21 * n_cand * n_rungs, hipMemcpyDeviceToHost, c->stream),
+        "download");
+  HIPCK(hipMemcpyAsync(hp + a_T, dT, sizeof(double) * 16 * n_cand * (n_rungs + 1), hipMemcpyDeviceToHost, c->stream),
+        "download");
   SYNC_RECEIPT(c);
   c->prof.collect();
   for (size_t i = 0; i < n_cand; ++i)

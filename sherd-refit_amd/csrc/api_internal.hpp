@@ -43,6 +43,9 @@ struct Ctx {
   // staging; its previous contents are not kept when it grows.
   Buf host;
   void *pinned(size_t bytes, hipError_t *err);
+  // The device block of the last freed Huber batch, kept for the next one (a batch is created and
+  // freed per LM call; this spares a hipMalloc and a hipFree per call).
+  Buf huber_spare;
   // This device's copy of the file-scope `scratch` array of the no-allocation entry point (its
   // address from hipGetSymbolAddress, taken at open with the device current).
   void *noalloc = nullptr;

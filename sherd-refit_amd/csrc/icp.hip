@@ -462,39 +462,66 @@ __global__ void __launch_bounds__(64) k_icp_wide_step(IcpWideView w, int it) {
   }
 }
 
-// Point-to-point's covariance pass: one wave per span, the batched form's loop.
-__global__ void __launch_bounds__(64) k_icp_wide_cov(IcpWideView w) {
-  const int s = blockIdx.x;
-  const int k = span_owner(w, s);
-  const uint32_t c = w.b.order[k];
-  const IcpWideState &st = w.st[c];
-  if (st.stop || !(st.red[0] > 0.0)) return;
-  const IcpBatchView &v = w.b;
-  const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
-  double sg[9];
-  p2p_cov_span(v.moved + 3 * v.off[c], v.corr + v.off[c], pd.tp, pd.ns, s - (int)w.span_first[k], p2p_means(st.red),
-               sg);
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int k = 0; k < 9; ++k) w.part[(size_t)s * NV + k] = sg[k];
-}
-
-// Point-to-point's update (Umeyama over the covariance) and the new pose, one wave per candidate.
-__global__ void __launch_bounds__(64) k_icp_wide_umeyama(IcpWideView w) {
+// Point-to-point's iteration after a correspondence pass, one workgroup per candidate: the
+// bookkeeping of k_icp_wide_step, the covariance pass and the Umeyama update in one launch, with
+// the batched form's arithmetic in its order. Wave 0 takes the pass's totals and does the bookkeeping; unless the candidate
+// stopped, each wave takes spans w, w + UPD_WAVES, ... of the covariance (one wave per span, the
+// batched form's loop), and wave 0 adds the span sums and makes the Umeyama update.
+constexpr int UPD_WAVES = 4;
+__global__ void __launch_bounds__(64 * UPD_WAVES) k_icp_wide_p2p(IcpWideView w, int it) {
+  __shared__ double red[NV];
   __shared__ double sig[9];
-  __shared__ double s_ws[78 + 32];
-  __shared__ int s_iws[3];
+  __shared__ double s_ws[ICP_WS + 32];
+  __shared__ int s_iws[6];
+  __shared__ int s_go;
   const uint32_t c = w.b.order[blockIdx.x];
   IcpWideState &st = w.st[c];
-  if (st.stop) return;
-  const IcpPairDesc pd = w.b.pairs[w.b.cand_pair[c]];
-  const bool any = st.red[0] > 0.0;
-  if (any) spans_total<9>(w.part + (size_t)w.span_first[blockIdx.x] * NV, (pd.ns + ICP_SPAN - 1) / ICP_SPAN, sig);
-  if (threadIdx.x != 0) return;
+  if (st.stop) return;  // (uniform: one candidate per workgroup)
+  const IcpBatchView &v = w.b;
+  const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
+  const int nspan = (pd.ns + ICP_SPAN - 1) / ICP_SPAN;
+  double *part = w.part + (size_t)w.span_first[blockIdx.x] * NV;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (wave == 0) {
+    spans_total<NV>(part, nspan, red);
+    if (lane == 0) {
+      IcpIter s{st.fit, st.rmse, st.stop, st.iters, st.conv};
+      double *U = s_ws + ICP_WS;
+      for (int k = 0; k < 16; ++k) U[k] = st.U[k];
+      icp_iteration_ws(red, pd.ns, it, v.max_iter, false, true, &s, U, s_ws, s_iws);
+      st.fit = s.fit;
+      st.rmse = s.rmse;
+      st.iters = s.iters;
+      st.conv = s.conv;
+      for (int k = 0; k < NV; ++k) st.red[k] = red[k];
+      if (s.stop) {
+        write_result(v, (int)c, st.T, s, red[0]);
+        st.stop = 1;
+      }
+      s_go = s.stop ? 0 : (red[0] > 0.0 ? 2 : 1);  // 0 stopped, 1 identity update, 2 Umeyama
+    }
+  }
+  __syncthreads();
+  const int go = s_go;
+  if (go == 0) return;
+  if (go == 2) {
+    const P2pMeans m = p2p_means(red);
+    for (int sp = wave; sp < nspan; sp += UPD_WAVES) {
+      double sg[9];
+      p2p_cov_span(v.moved + 3 * v.off[c], v.corr + v.off[c], pd.tp, pd.ns, sp, m, sg);
+      if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) part[(size_t)sp * NV + k] = sg[k];
+    }
+  }
+  __syncthreads();
+  if (wave != 0) return;
+  if (go == 2) spans_total<9>(part, nspan, sig);
+  if (lane != 0) return;
   double *U = s_ws + 78, *nt = s_ws + 94;
   eye4(U);
-  if (any) {
-    const P2pMeans m = p2p_means(st.red);
+  if (go == 2) {
+    const P2pMeans m = p2p_means(red);
     double mp[3] = {m.mp0, m.mp1, m.mp2}, mq[3] = {m.mq0, m.mq1, m.mq2};
     umeyama_pose_ws(sig, mp, mq, U, s_ws, s_iws);
   }
@@ -551,11 +578,10 @@ hipError_t launch_icp_wide(const IcpWideView &w, hipStream_t s, KernelProfiler *
     else
       hipLaunchKernelGGL(k_icp_wide_corr<EST_PLANE>, dim3((unsigned)w.n_spans), dim3(ICP_SPAN), 0, s, w, it);
     if (prof) prof->end(s);
-    hipLaunchKernelGGL(k_icp_wide_step, dim3((unsigned)n), dim3(64), 0, s, w, it);
-    if (w.b.est == 0 && it < w.b.max_iter) {
-      hipLaunchKernelGGL(k_icp_wide_cov, dim3((unsigned)w.n_spans), dim3(64), 0, s, w);
-      hipLaunchKernelGGL(k_icp_wide_umeyama, dim3((unsigned)n), dim3(64), 0, s, w);
-    }
+    if (w.b.est == EST_P2P && it < w.b.max_iter)
+      hipLaunchKernelGGL(k_icp_wide_p2p, dim3((unsigned)n), dim3(64 * UPD_WAVES), 0, s, w, it);
+    else
+      hipLaunchKernelGGL(k_icp_wide_step, dim3((unsigned)n), dim3(64), 0, s, w, it);
   }
   return hipGetLastError();
 }

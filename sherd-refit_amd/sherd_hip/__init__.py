@@ -465,14 +465,46 @@ def _addr(h):
     return 0 if h is None else (h.ptr.value or 0)
 
 
+class IcpPairs:
+    """A table of (source Cloud, target Cloud, distance) in the ABI's layout (sherd_icp_pair), built
+    once and passed to the batch calls in place of a list, so that a batch over hundreds of pairs
+    crosses the boundary without a Python loop per call. Holds its clouds (they outlive it)."""
+
+    def __init__(self, pairs):
+        pairs = list(pairs)
+        self.clouds = [(s, t) for s, t, _ in pairs]
+        self.arr = np.zeros(max(len(pairs), 1), _ICP_PAIR_DT)
+        if pairs:
+            self.arr["source"] = [_addr(s) for s, _, _ in pairs]
+            self.arr["target"] = [_addr(t) for _, t, _ in pairs]
+            self.arr["distance"] = [float(d) for _, _, d in pairs]
+        # Source points that take part per pair (none when the target is empty).
+        self.ns = np.array([s.n if t.n else 0 for s, t in self.clouds], np.int64)
+
+    def __len__(self):
+        return len(self.clouds)
+
+
 def _icp_pairs(pairs):
-    """[(source, target, distance)] -> an IcpPair array (numpy, ABI layout)."""
-    arr = np.zeros(max(len(pairs), 1), _ICP_PAIR_DT)
-    if pairs:
-        arr["source"] = [_addr(s) for s, _, _ in pairs]
-        arr["target"] = [_addr(t) for _, t, _ in pairs]
-        arr["distance"] = [float(d) for _, _, d in pairs]
-    return arr
+    """[(source, target, distance)] or an IcpPairs -> an IcpPair array (numpy, ABI layout)."""
+    return (pairs if isinstance(pairs, IcpPairs) else IcpPairs(pairs)).arr
+
+
+class VerifyPairs:
+    """A table of (Fragment A, Fragment B, Scales as 12 f64) in the ABI's layout (sherd_verify_pair),
+    built once and passed to verify_batch in place of a list. Holds its fragments."""
+
+    def __init__(self, pairs):
+        pairs = list(pairs)
+        self.frags = [(a, b) for a, b, _ in pairs]
+        self.arr = np.zeros(max(len(pairs), 1), _VERIFY_PAIR_DT)
+        if pairs:
+            self.arr["a"] = [_addr(a) for a, _, _ in pairs]
+            self.arr["b"] = [_addr(b) for _, b, _ in pairs]
+            self.arr["scales"] = np.asarray([sc for _, _, sc in pairs], dtype=np.float64).reshape(len(pairs), 12)
+
+    def __len__(self):
+        return len(self.frags)
 
 
 def _registrations(out, k):
@@ -499,15 +531,16 @@ class HuberBatch:
 
     def __init__(self, ex, pairs, cand_pair, T0):
         self.ex = ex
-        self.clouds = [(s_, t_) for s_, t_, _ in pairs]
-        arr = _icp_pairs(pairs)
+        pairs = pairs if isinstance(pairs, IcpPairs) else IcpPairs(pairs)
+        self.clouds = pairs.clouds
         self.cand_pair = np.ascontiguousarray(cand_pair, dtype=np.uint32)
         T0 = np.ascontiguousarray(T0, dtype=np.float64).reshape(-1, 16)
         self.k = len(T0)
-        self.points = int(sum(pairs[int(p)][0].n if pairs[int(p)][1].n else 0 for p in self.cand_pair))
+        # (A pose naming a pair out of range is the C ABI's to refuse; clipped here only for the count.)
+        self.points = int(pairs.ns[np.minimum(self.cand_pair, len(pairs) - 1)].sum()) if len(pairs) else 0
         out = C.c_void_p()
-        _check(lib().sherd_huber_create(ex.ptr, _p(arr, IcpPair), len(pairs), _p(self.cand_pair, C.c_uint32), _p(T0, C.c_double),
-                                        self.k, C.byref(out)))
+        _check(lib().sherd_huber_create(ex.ptr, _p(pairs.arr, IcpPair), len(pairs), _p(self.cand_pair, C.c_uint32),
+                                        _p(T0, C.c_double), self.k, C.byref(out)))
         self.ptr = out
         ex._own(self)
 
@@ -858,7 +891,7 @@ class HipExecutor:
         nr, npairs = len(rungs), len(pairs[0])
         if any(len(pr) != npairs for pr in pairs):
             raise HipError(-1, "every rung needs the same number of pairs")
-        arr = _icp_pairs([x for pr in pairs for x in pr])
+        arr = np.concatenate([_icp_pairs(pr)[:npairs] for pr in pairs]) if npairs else _icp_pairs([])
         specs = (IcpRungSpec * nr)(*[IcpRungSpec(int(_EST.get(e, e)), int(it)) for e, it in rungs])
         cp = np.ascontiguousarray(cand_pair, dtype=np.uint32)
         inits = np.ascontiguousarray(inits, dtype=np.float64).reshape(-1, 16)
@@ -885,11 +918,7 @@ class HipExecutor:
     def verify_batch(self, pairs, cand_pair, cand_T, full=True, retest=False):
         """verify::verify + accept over candidates of many pairs. pairs: [(frag_a, frag_b, scales12)].
         Returns (scores (n, 20), accepted (n,) bool)."""
-        pv = np.zeros(max(len(pairs), 1), _VERIFY_PAIR_DT)
-        if pairs:
-            pv["a"] = [_addr(a) for a, _, _ in pairs]
-            pv["b"] = [_addr(b) for _, b, _ in pairs]
-            pv["scales"] = np.asarray([sc for _, _, sc in pairs], dtype=np.float64).reshape(len(pairs), 12)
+        pv = (pairs if isinstance(pairs, VerifyPairs) else VerifyPairs(pairs)).arr
         cp = np.ascontiguousarray(cand_pair, dtype=np.uint32)
         T = np.ascontiguousarray(cand_T, dtype=np.float64).reshape(-1, 16)
         n = len(T)

@@ -86,10 +86,12 @@ def pair_scales(coll, pairs):
 
 
 class Device:
-    """The collection resident on the device: scenes, fragments, breakline clouds."""
+    """The collection resident on the device: scenes, fragments, breakline clouds, and the pair
+    tables of the batch calls (built once per pair list, as a host keeps its pair set)."""
 
     def __init__(self, ex, coll):
         self.ex = ex
+        self._tables = {}
         self.scenes, self.frags, self.brk = [], [], []
         for fr in coll["fragments"]:
             fs, ms = ex.scene(fr["V"], fr["F"], fr["frac"]), ex.scene(fr["V"], fr["F"])
@@ -98,7 +100,22 @@ class Device:
                                           fr["brk_ns"], fr["margin_p"], fr["margin_n"]))
             self.brk.append(ex.cloud(fr["brk_P"], fr["brk_ns"]))
 
+    def tables(self, pairs, scales):
+        """The ladder's pair table per rung, the LM's and verification's, for these pairs and scales
+        (kept with the lists they were built from)."""
+        key = (id(pairs), id(scales))
+        if key not in self._tables:
+            import sherd_hip as H
+
+            ladder = [H.IcpPairs([(self.brk[b], self.brk[a], k * sc[0] * sc[11]) for (a, b), sc in zip(pairs, scales)])
+                      for k in RUNGS]
+            lm = H.IcpPairs([(self.brk[b], self.brk[a], RUNGS[-1] * sc[0] * sc[11]) for (a, b), sc in zip(pairs, scales)])
+            ver = H.VerifyPairs([(self.frags[a], self.frags[b], sc) for (a, b), sc in zip(pairs, scales)])
+            self._tables[key] = (pairs, scales, ladder, lm, ver)
+        return self._tables[key][2:]
+
     def close(self):
+        self._tables.clear()
         for c in self.brk:
             c.close()
         for f in self.frags:
@@ -116,16 +133,14 @@ LM_DELTA, LM_ITERATIONS = 0.04, 10
 def run_batched(dev, coll, pairs, cand_pair, cand_T, scales):
     ex = dev.ex
     t0 = time.perf_counter()
-    ladder_pairs = [[(dev.brk[b], dev.brk[a], k * sc[0] * sc[11]) for (a, b), sc in zip(pairs, scales)] for k in RUNGS]
+    ladder_pairs, lm_pairs, vpairs = dev.tables(pairs, scales)
     reg = ex.icp_batch_ladder(ladder_pairs, [("point_to_point", ITERATIONS)] * len(RUNGS), cand_pair, cand_T)
     T1 = reg[:, -1, :16].reshape(-1, 4, 4)
     t_icp = time.perf_counter() - t0
-    lm_pairs = [(dev.brk[b], dev.brk[a], RUNGS[-1] * sc[0] * sc[11]) for (a, b), sc in zip(pairs, scales)]
     hb = ex.huber(lm_pairs, cand_pair, T1)
     T2, E, steps = hb.lm(LM_DELTA, LM_ITERATIONS, 32)
     hb.close()
     t_lm = time.perf_counter() - t0 - t_icp
-    vpairs = [(dev.frags[a], dev.frags[b], sc) for (a, b), sc in zip(pairs, scales)]
     scores, accepted = ex.verify_batch(vpairs, cand_pair, T2, full=True, retest=True)
     el = time.perf_counter() - t0
     return dict(reg=reg, T_lm=T2, E=E, scores=scores, accepted=accepted, seconds=el, icp_s=t_icp, lm_s=t_lm,
@@ -155,7 +170,7 @@ def run_plain(dev, coll, pairs, cand_pair, cand_T, scales):
         T2[sel], E[sel], _ = hb.lm(LM_DELTA, LM_ITERATIONS, 32)
         hb.close()
     t_lm = time.perf_counter() - t0 - t_icp
-    vpairs = [(dev.frags[a], dev.frags[b], sc) for (a, b), sc in zip(pairs, scales)]
+    vpairs = dev.tables(pairs, scales)[2]
     scores = np.empty((n, 20))
     accepted = np.empty(n, bool)
     for i in range(n):
