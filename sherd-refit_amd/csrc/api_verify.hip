@@ -377,8 +377,13 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     if (!dtab) return hip_fail(e, "scratch seam tables");
     uint64_t *dhard = (uint64_t *)c->scratch(19, sizeof(uint64_t) * 2 * nc * (size_t)std::max(max_s, 1), &e);
     if (!dhard) return hip_fail(e, "scratch");
-    uint32_t *dhard_n = (uint32_t *)c->scratch(20, sizeof(uint32_t) * 2, &e);
+    uint32_t *dhard_n = (uint32_t *)c->scratch(20, sizeof(uint32_t) * 4, &e);
     if (!dhard_n) return hip_fail(e, "scratch");
+    // The inside samples' list: at most every sample of every (candidate, side) task.
+    uint64_t *ddeep = (uint64_t *)c->scratch(16, sizeof(uint64_t) * 2 * nc * (size_t)std::max(max_s, 1), &e);
+    if (!ddeep) return hip_fail(e, "scratch");
+    float *ddeep_t = (float *)c->scratch(17, sizeof(float) * 2 * nc * (size_t)std::max(max_s, 1), &e);
+    if (!ddeep_t) return hip_fail(e, "scratch");
     // The staging buffer is free: the previous chunk's download was waited for.
     std::memcpy(hbuf + up_cd, cd.data(), sizeof(CandDesc) * nc);
     std::memcpy(hbuf + up_task, task.data(), sizeof(uint32_t) * 2 * nc);
@@ -395,6 +400,8 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     vb.task = dtask;
     vb.hard = dhard;
     vb.hard_n = dhard_n;
+    vb.deep = ddeep;
+    vb.deep_t = ddeep_t;
     vb.n_cand = (int)nc;
     vb.full = full ? 1 : 0;
     vb.retest = retest ? 1 : 0;

@@ -523,10 +523,13 @@ __device__ __forceinline__ bool bounded_distance_with(Stack &st, const SceneView
   return true;
 }
 
-// CpuExecutor::inside's per-point body (cpu.rs:182-188, bvh.rs:146-158): the AABB reject, three
-// axis rays with a majority of odd counts, and the unbounded distance of an inside point.
-template <class Stack>
-__device__ __forceinline__ bool inside_depth_with(Stack &st, const SceneView &sv, V3 p, int retest, float *depth) {
+// CpuExecutor::inside's per-point body (cpu.rs:182-188, bvh.rs:146-158) in two halves: the AABB
+// reject and three axis rays with a majority of odd counts (inside_parity), then the unbounded
+// distance of an inside point (depth_within).
+//
+// The parity half: whether p is inside and, when it is, the nearest ray crossing's distance in *t
+// (+inf if no ray crossed anything).
+__device__ __forceinline__ bool inside_parity(const SceneView &sv, V3 p, int retest, float *t) {
   if (p.x < sv.lo[0] || p.x > sv.hi[0] || p.y < sv.lo[1] || p.y > sv.hi[1] || p.z < sv.lo[2] ||
       p.z > sv.hi[2])
     return false;
@@ -542,19 +545,38 @@ __device__ __forceinline__ bool inside_depth_with(Stack &st, const SceneView &sv
   int votes = v0 + v1;
   if (v0 != v1) votes += (int)(crossings_on(hi, sv, p, retest, &t2) & 1u);
   if (votes < 2) return false;
-  // The depth is the distance to the nearest triangle. A ray's nearest crossing is a point of the
-  // surface t away, so a search bounded a little beyond the smallest t finds the same minimum
-  // (any triangle within the bound is measured); only if that bound somehow missed does the
-  // unbounded search run.
-  float w = fminf(t0, fminf(t1, t2));
+  *t = fminf(t0, fminf(t1, t2));
+  return true;
+}
+
+// The search bound of an inside point's depth from its nearest crossing t: a ray's nearest crossing
+// is a point of the surface t away, so the nearest triangle is no farther than t, and t grown by
+// a relative 1e-4 and the slack (which exceed the f32 rounding of the crossing and of the
+// distance) bounds its distance as computed. +inf when t is.
+__device__ __forceinline__ float depth_bound(const SceneView &sv, V3 p, float t) {
+  return t < __builtin_inff() ? t * 1.0001f + slack_of(sv.absmax, p) : __builtin_inff();
+}
+
+// The depth of an inside point, the distance to its nearest triangle, searched within the bound
+// wb = depth_bound(t) (which finds the same minimum: every triangle within it is measured); only
+// if that bound somehow missed does the unbounded search run.
+template <class Stack>
+__device__ __forceinline__ float depth_within(Stack &st, const SceneView &sv, V3 p, float wb) {
   float d2 = __builtin_inff();
-  if (w < __builtin_inff()) {
-    w = w * 1.0001f + slack_of(sv.absmax, p);
-    d2 = min_d2_with(st, sv, p, w, 0, nullptr);
-    if (!(d2 <= w * w)) d2 = __builtin_inff();
+  if (wb < __builtin_inff()) {
+    d2 = min_d2_with(st, sv, p, wb, 0, nullptr);
+    if (!(d2 <= wb * wb)) d2 = __builtin_inff();
   }
   if (d2 == __builtin_inff()) d2 = min_d2_with(st, sv, p, __builtin_inff(), 0, nullptr);
-  *depth = sqrtf(d2);
+  return sqrtf(d2);
+}
+
+// Both halves: inside and, for an inside point, its depth.
+template <class Stack>
+__device__ __forceinline__ bool inside_depth_with(Stack &st, const SceneView &sv, V3 p, int retest, float *depth) {
+  float t;
+  if (!inside_parity(sv, p, retest, &t)) return false;
+  *depth = depth_within(st, sv, p, depth_bound(sv, p, t));
   return true;
 }
 

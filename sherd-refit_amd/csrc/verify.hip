@@ -41,10 +41,11 @@ constexpr int HARD_BLOCKS = 1024;  // workgroups of the cooperative pass
 // FLAG_GAVE_UP and the batch fails with a status (the caller answers it on the CPU), so the kernel
 // always ends in bounded time.
 constexpr int WAVE_STEPS = 1 << 19;
-// k_pen's depth searches start from an inside point with a window as wide as the nearest ray
-// crossing, so they defer more nodes than the windowed searches: a deeper LDS stack (27 KB of LDS
-// per workgroup, which keeps the five waves per SIMD its registers allow).
-constexpr int PEN_LSTACK = 12;
+constexpr int DEPTH_BLOCKS = 1024;  // workgroups of k_pen_depth, which loop over the listed samples
+// k_pen_depth's deferred nodes: its samples fill few waves, so a deep LDS stack (64 KiB per
+// workgroup) costs no occupancy they could use, and spares the overflow walk.
+constexpr int DEPTH_LSTACK = 32;
+
 
 __device__ __forceinline__ void apply_f64(const double *T, const double *p, double *o) {
   o[0] = T[0] * p[0] + T[1] * p[1] + T[2] * p[2] + T[3];
@@ -261,7 +262,7 @@ __global__ void k_prep(VerifyBatch vb) {
     a.closest[s] = 0x7f7fffffu;  // f32::MAX, the window of the fallback (verify.rs:570-577)
   }
   a.n_cont = 0;
-  if (c == 0) vb.hard_n[0] = vb.hard_n[1] = 0;
+  if (c == 0) vb.hard_n[0] = vb.hard_n[1] = vb.hard_n[2] = 0;
   a.seam_cells = 0;
   a.flags = 0;
 }
@@ -454,8 +455,9 @@ __global__ void __launch_bounds__(VT) k_cont(VerifyBatch vb) {
   }
 }
 
-// R §6.4 one_penetration's first pass (verify.rs:541-566): side 0 is A's surface samples moved by
-// T^-1 against B's mesh, side 1 is B's moved by T against A's mesh.
+// R §6.4 one_penetration's first pass (verify.rs:541-566), its parity half: side 0 is A's surface
+// samples moved by T^-1 against B's mesh, side 1 is B's moved by T against A's mesh. Inside samples
+// are counted and listed with their nearest ray crossing for k_pen_depth.
 __global__ void __launch_bounds__(VT) k_pen(VerifyBatch vb, int per_task) {
   const TaskGroup g = task_group(vb, per_task);
   const int c = g.c, side = g.side;
@@ -471,12 +473,11 @@ __global__ void __launch_bounds__(VT) k_pen(VerifyBatch vb, int per_task) {
   if (block_box_gap(moved_sphere(own.s_all, T), sv.lo, sv.hi, sv.absmax) > 0.0) return;
   // Most samples fall outside the other mesh's box (RayScene::inside's reject). Per chunk, the
   // workgroup gathers the ones inside it in LDS and its first lanes take them, one each, so the
-  // rays and depth searches run in as few waves as there are such samples.
+  // rays run in as few waves as there are such samples.
   __shared__ float s_q[3][VT];
+  __shared__ uint32_t s_i[VT];
   __shared__ uint32_t s_n;
-  __shared__ uint2 s_stk[PEN_LSTACK][VT];
   CandAcc &acc = vb.acc[c];
-  LdsStack<VT, PEN_LSTACK> st{&s_stk[0][threadIdx.x]};
   for (int k = 0; k < CHUNKS; ++k) {
     const int base = (g.chunk0 + k) * VT;
     if (base >= own.n_s) break;  // (uniform)
@@ -493,28 +494,55 @@ __global__ void __launch_bounds__(VT) k_pen(VerifyBatch vb, int per_task) {
         s_q[0][s] = q.x;
         s_q[1][s] = q.y;
         s_q[2][s] = q.z;
+        s_i[s] = (uint32_t)i;
       }
     }
     __syncthreads();
     const uint32_t n = s_n;
     if (threadIdx.x == 0 && n) atomicAdd(&acc.in_box[side], n);
     if (threadIdx.x < ((n + 63u) & ~63u)) {  // whole waves only
-      bool in = false, deeper = false;
-      float depth = 0.0f;
+      bool in = false;
+      float t = 0.0f;
       if (threadIdx.x < n) {
         V3 q = V3{s_q[0][threadIdx.x], s_q[1][threadIdx.x], s_q[2][threadIdx.x]};
-        in = inside_depth_with(st, sv, q, vb.retest, &depth);
-        deeper = in && (double)depth > pd.sc[S_PEN];
+        in = inside_parity(sv, q, vb.retest, &t);
       }
-      uint32_t deepest = wave_max(in ? __float_as_uint(depth) : 0u);
-      uint32_t ni = (uint32_t)__popcll(__ballot(in)), nd = (uint32_t)__popcll(__ballot(deeper));
-      if ((threadIdx.x & 63) == 0) {
-        if (ni) atomicAdd(&acc.inside[side], ni);
-        if (nd) atomicAdd(&acc.deeper[side], nd);
-        if (deepest) atomicMax(&acc.deepest[side], deepest);
+      const uint32_t ni = (uint32_t)__popcll(__ballot(in));
+      if ((threadIdx.x & 63) == 0 && ni) atomicAdd(&acc.inside[side], ni);
+      const uint32_t slot = wave_slot(&vb.hard_n[2], in);
+      if (in) {
+        vb.deep[slot] = ((uint64_t)c << 33) | ((uint64_t)side << 32) | s_i[threadIdx.x];
+        vb.deep_t[slot] = t;
       }
     }
     __syncthreads();  // the gathered samples are read before the next chunk writes them
+  }
+}
+
+// R §6.4's depths of the inside samples k_pen listed, one lane each, so the searches run in full
+// waves: the distance to the nearest triangle (the same search RayScene::inside's depth makes),
+// the count deeper than pen and the deepest per side.
+__global__ void __launch_bounds__(VT) k_pen_depth(VerifyBatch vb) {
+  __shared__ uint2 s_stk[DEPTH_LSTACK][VT];
+  LdsStack<VT, DEPTH_LSTACK> st{&s_stk[0][threadIdx.x]};
+  const uint32_t n = vb.hard_n[2];
+  for (uint32_t k = blockIdx.x * VT + threadIdx.x; k < n; k += gridDim.x * VT) {
+    const uint64_t e = vb.deep[k];
+    const int c = (int)(e >> 33), side = (int)((e >> 32) & 1u), i = (int)(uint32_t)e;
+    const CandDesc &cd = vb.cand[c];
+    const PairDesc &pd = vb.pair[cd.pair];
+    const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
+    const SceneView &sv = vb.frag[side == 0 ? pd.b : pd.a].mesh;
+    const double *T = side == 0 ? cd.Ti : cd.T;
+    const double *p = own.s + 3 * (size_t)i;
+    const V3 q = moved_f32(T, p[0], p[1], p[2]);
+    CandAcc &acc = vb.acc[c];
+    // The search bound: the nearest crossing's, or the distance-bound grid's upper bound when that
+    // is smaller (a ray along a wall can cross it far away).
+    const float depth = depth_within(st, sv, q, fminf(depth_bound(sv, q, vb.deep_t[k]), dist_upper(sv.db, q)));
+    const bool deeper = (double)depth > pd.sc[S_PEN];
+    if (deeper) atomicAdd(&acc.deeper[side], 1u);
+    if (__float_as_uint(depth)) atomicMax(&acc.deepest[side], __float_as_uint(depth));
   }
 }
 
@@ -950,6 +978,9 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
     // R §6.4's inside pass first and alone: it is the longest kernel and fills the device.
     B("k_pen", s);
     hipLaunchKernelGGL(k_pen, dim3(2 * nc * per_sg), dim3(VT), 0, s, vb, per_sg);
+    E(s);
+    B("k_pen_depth", s);
+    hipLaunchKernelGGL(k_pen_depth, dim3(DEPTH_BLOCKS), dim3(VT), 0, s, vb);
     E(s);
   }
   // The sections below read the poses k_prep completed and write disjoint accumulators and key

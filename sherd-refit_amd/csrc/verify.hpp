@@ -84,7 +84,9 @@ struct VerifyBatch {
   uint8_t *accepted;
   const uint32_t *task;  // 2 * n_cand (candidate << 1 | side), grouped by the fragment queried
   uint64_t *hard;        // closest-approach queries left to whole waves: candidate << 33 | side << 32 | sample
-  uint32_t *hard_n;      // [0] queued, [1] taken
+  uint32_t *hard_n;      // [0] queued, [1] taken, [2] inside samples listed in deep
+  uint64_t *deep;        // inside samples whose depth k_pen_depth measures, as in hard
+  float *deep_t;         // and their nearest ray crossings
   int n_cand;
   int full;
   int retest;
