@@ -234,12 +234,17 @@ int plan_rung(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const uint32_
     sum += d.ns;
   }
   pl->mean_ns = pl->order.empty() ? 0.0 : sum / (double)pl->order.size();
-  // Longest source first, so the persistent workgroups finish together.
-  std::stable_sort(pl->order.begin(), pl->order.end(),
-                   [&](uint32_t a, uint32_t b) { return pl->pd[cand_pair[a]].ns > pl->pd[cand_pair[b]].ns; });
+  // Longest source first, so the persistent workgroups finish together (sorted on a compact copy of
+  // the sizes: the descriptors are large).
+  std::vector<std::pair<int, uint32_t>> keyed(pl->order.size());
+  for (size_t k = 0; k < keyed.size(); ++k) keyed[k] = {-pl->pd[cand_pair[pl->order[k]]].ns, pl->order[k]};
+  std::stable_sort(keyed.begin(), keyed.end(),
+                   [](const std::pair<int, uint32_t> &a, const std::pair<int, uint32_t> &b) { return a.first < b.first; });
   pl->span_first.assign(1, 0u);
-  for (uint32_t i : pl->order)
-    pl->span_first.push_back(pl->span_first.back() + (uint32_t)((pl->pd[cand_pair[i]].ns + ICP_SPAN - 1) / ICP_SPAN));
+  for (size_t k = 0; k < keyed.size(); ++k) {
+    pl->order[k] = keyed[k].second;
+    pl->span_first.push_back(pl->span_first.back() + (uint32_t)((-keyed[k].first + ICP_SPAN - 1) / ICP_SPAN));
+  }
   return SHERD_OK;
 }
 
@@ -365,8 +370,9 @@ int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand
   return SHERD_OK;
 }
 
-hipError_t launch_rung(Ctx *c, const BatchDev &bd, size_t r, int block, int grid) {
-  if (block == WIDE) return launch_icp_wide(bd.w[r], c->stream, c->prof.on ? &c->prof : nullptr);
+// `timed`: each wide pass's correspondence kernel between profiler events (never in a capture).
+hipError_t launch_rung(Ctx *c, const BatchDev &bd, size_t r, int block, int grid, bool timed) {
+  if (block == WIDE) return launch_icp_wide(bd.w[r], c->stream, timed && c->prof.on ? &c->prof : nullptr);
   return launch_icp_batch(bd.v[r], block, grid, c->stream);
 }
 
@@ -387,7 +393,7 @@ int run_rung(Ctx *c, const BatchDev &bd, size_t r, const RungPlan &pl) {
   int block = c->icp_block ? c->icp_block : tuned;
   if (block) {
     if (c->prof.on && block != WIDE) c->prof.begin("k_icp_batch", c->stream);
-    HIPCK(launch_rung(c, bd, r, block, grid_for(c, block, pl.order.size(), pl.max_spans)), "icp kernel");
+    HIPCK(launch_rung(c, bd, r, block, grid_for(c, block, pl.order.size(), pl.max_spans), true), "icp kernel");
     if (c->prof.on && block != WIDE) c->prof.end(c->stream);
     return SHERD_OK;
   }
@@ -405,7 +411,7 @@ int run_rung(Ctx *c, const BatchDev &bd, size_t r, const RungPlan &pl) {
   for (int k = 0; k < n; ++k) {
     int b = sizes[k];
     HIPCK(hipEventRecord(c->ev0, c->stream), "event");
-    HIPCK(launch_rung(c, bd, r, b, b == WIDE ? 0 : grid_for(c, b, pl.order.size(), pl.max_spans)), "icp kernel");
+    HIPCK(launch_rung(c, bd, r, b, b == WIDE ? 0 : grid_for(c, b, pl.order.size(), pl.max_spans), false), "icp kernel");
     HIPCK(hipEventRecord(c->ev1, c->stream), "event");
     HIPCK(hipEventSynchronize(c->ev1), "event");
     float ms = 0.0f;
@@ -484,7 +490,7 @@ int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_
   // call also had is captured as a graph and replayed while it repeats; a one-off batch (the
   // refinement's walk issues one per depth) runs its launches directly.
   std::string sig;
-  if (tuned && !c->prof.on) {
+  if (tuned) {
     for (size_t r = 0; r < n_rungs; ++r) {
       sig.append((const char *)&bd.v[r], sizeof(IcpBatchView));
       sig.append((const char *)&bd.w[r], sizeof(IcpWideView));
@@ -494,7 +500,8 @@ int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_
     sig.append((const char *)&dskip, sizeof dskip);
     sig.append((const char *)&any_skip, sizeof any_skip);
   }
-  bool replay = tuned && !c->prof.on && ((c->ladder_exec && sig == c->ladder_sig) || sig == c->ladder_seen);
+  // (A replayed graph's kernels are not timed by the profiler's events; a kernel trace times them.)
+  bool replay = tuned && ((c->ladder_exec && sig == c->ladder_sig) || sig == c->ladder_seen);
   c->ladder_seen = sig;
   if (!replay) {
     for (size_t r = 0; r < n_rungs; ++r) {
@@ -514,7 +521,7 @@ int run_ladder(Ctx *c, const sherd_icp_pair *pairs, size_t n_pairs, const sherd_
       HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "graph capture");
       hipError_t le = hipSuccess;
       for (size_t r = 0; r < n_rungs && le == hipSuccess; ++r) {
-        if (bd.v[r].n_cand > 0) le = launch_rung(c, bd, r, blocks[r], grids[r]);
+        if (bd.v[r].n_cand > 0) le = launch_rung(c, bd, r, blocks[r], grids[r], false);
         if (le == hipSuccess && any_skip) {
           hipLaunchKernelGGL(k_carry, dim3(cb), dim3(256), 0, c->stream, bd.v[r].T_in, bd.v[r].T_out,
                              dskip + r * n_cand, (int)n_cand);

@@ -315,7 +315,7 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
   const size_t MAX_TASK_CAND = std::min<size_t>(65535, (size_t)INT_MAX / (2 * per_task));
   size_t begin = 0;
   uint64_t rays = 0;  // parity rays in the unit of SHERD_M_VERIFY_RAYS
-  std::vector<uint32_t> task;
+  std::vector<uint32_t> task, qfrag, first;
   std::vector<CandDesc> cd;
   while (begin < n_cand) {
     cd.clear();
@@ -345,14 +345,18 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
       ++end;
     }
     size_t nc = cd.size();
-    // (candidate, side) tasks grouped by the fragment they query: side 0 queries B, side 1 A.
+    // (candidate, side) tasks grouped by the fragment they query (side 0 queries B, side 1 A), in
+    // task order within a fragment: a counting sort over the fragment indices.
     task.resize(2 * nc);
-    for (size_t i = 0; i < 2 * nc; ++i) task[i] = (uint32_t)i;
-    auto queried = [&](uint32_t t) {
+    qfrag.resize(2 * nc);
+    first.assign(frags.size() + 1, 0);
+    for (size_t t = 0; t < 2 * nc; ++t) {
       const PairDesc &p = pd[cd[t >> 1].pair];
-      return (t & 1u) ? p.a : p.b;
-    };
-    std::stable_sort(task.begin(), task.end(), [&](uint32_t x, uint32_t y) { return queried(x) < queried(y); });
+      qfrag[t] = (uint32_t)((t & 1u) ? p.a : p.b);
+      ++first[qfrag[t] + 1];
+    }
+    for (size_t f = 0; f < frags.size(); ++f) first[f + 1] += first[f];
+    for (size_t t = 0; t < 2 * nc; ++t) task[first[qfrag[t]]++] = (uint32_t)t;
     // What goes up (descriptors, then tasks) and what comes back (accumulators, scores, verdicts)
     // each sit in one device piece, so one pinned copy moves each way.
     const size_t up_cd = 0, up_task = up256(sizeof(CandDesc) * nc), up_bytes = up_task + sizeof(uint32_t) * 2 * nc;
