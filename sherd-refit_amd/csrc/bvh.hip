@@ -717,47 +717,55 @@ namespace {
 
 __global__ void __launch_bounds__(256) k_first_hit(SceneView sv, const float *orig, const float *dir, int n,
                                                    int retest, float *t_out, uint32_t *prim_out) {
+  __shared__ uint2 s_stk[QSTACK][256];
+  LdsStack<256, QSTACK> st{&s_stk[0][threadIdx.x]};
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   V3 o = V3{orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]};
   V3 d = V3{dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
   float t;
   uint32_t f;
-  first_hit(sv, o, d, retest, &t, &f);
+  first_hit_with(st, sv, o, d, retest, &t, &f);
   t_out[i] = t;
   prim_out[i] = f;
 }
 
 __global__ void __launch_bounds__(256) k_inside(SceneView sv, const double *pts, int n, Pose T, int retest,
                                                 float *depth, uint32_t *inside) {
+  __shared__ uint2 s_stk[QSTACK][256];
+  LdsStack<256, QSTACK> st{&s_stk[0][threadIdx.x]};
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   V3 p = moved_f32(T.m, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
   float d = 0.0f;
-  bool in = inside_depth(sv, p, retest, &d);
+  bool in = inside_depth_with(st, sv, p, retest, &d);
   depth[i] = in ? d : 0.0f;
   inside[i] = in ? 1u : 0u;
 }
 
 __global__ void __launch_bounds__(256) k_bounded(SceneView sv, const double *pts, int n, Pose T, float window,
                                                  float *out) {
+  __shared__ uint2 s_stk[QSTACK][256];
+  LdsStack<256, QSTACK> st{&s_stk[0][threadIdx.x]};
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   V3 p = moved_f32(T.m, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
   float d;
-  out[i] = bounded_distance(sv, p, window, &d) ? d : __builtin_inff();
+  out[i] = bounded_distance_with(st, sv, p, window, &d) ? d : __builtin_inff();
 }
 
 // DistReduce::Min (cpu.rs:100-111): each point searches within the best distance found so far;
 // the bits of a non-negative float order as an unsigned integer, so atomicMin is the running min.
 __global__ void __launch_bounds__(256) k_bounded_min(SceneView sv, const double *pts, int n, Pose T,
                                                      unsigned int *best) {
+  __shared__ uint2 s_stk[QSTACK][256];
+  LdsStack<256, QSTACK> st{&s_stk[0][threadIdx.x]};
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   V3 p = moved_f32(T.m, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
   float window = __uint_as_float(__hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   float d;
-  if (bounded_distance(sv, p, window, &d)) atomicMin(best, __float_as_uint(d));
+  if (bounded_distance_with(st, sv, p, window, &d)) atomicMin(best, __float_as_uint(d));
 }
 
 }  // namespace

@@ -13,6 +13,7 @@ namespace sherd {
 constexpr int DSTACK = 24;  // deferred nodes of an ordered traversal
 constexpr int MSTACK = 16;  // the same for min_d2, whose overflow walk starts from the best found
 constexpr int LSTACK = 8;   // min_d2's deferred nodes when they live in LDS (see LdsStack)
+constexpr int QSTACK = 16;  // the same for the one-query-per-thread kernels of the executor's calls
 
 // Where min_d2 keeps its deferred nodes. A per-lane array indexed by the depth compiles to
 // private (scratch) memory, whose spills go out to HBM; LdsStack is the lane's column of a

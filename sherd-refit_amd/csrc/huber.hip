@@ -27,6 +27,7 @@ namespace sherd {
 namespace {
 
 constexpr int HT = 256;
+constexpr int H_LSTACK = 8;  // a correspondence search's deferred nodes, in an LDS column per lane
 
 __device__ __forceinline__ void rodrigues(const double *w, double *R) {
   double th = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
@@ -229,6 +230,8 @@ __global__ void __launch_bounds__(HT) k_correspond(HuberView v, const double *Tc
   const double *M = Tc + 16 * (size_t)c;
   int32_t *corr = corr_all + v.off[c];
   double r2 = pd.r * pd.r;
+  __shared__ uint2 s_cstk[H_LSTACK][HT];
+  LdsStack<HT, H_LSTACK> cst{&s_cstk[0][threadIdx.x]};
   for (int i = threadIdx.x; i < pd.ns; i += HT) {
     const double *s = pd.src + 3 * (size_t)i;
     double p0 = M[0] * s[0] + M[1] * s[1] + M[2] * s[2] + M[3];
@@ -236,7 +239,7 @@ __global__ void __launch_bounds__(HT) k_correspond(HuberView v, const double *Tc
     double p2 = M[8] * s[0] + M[9] * s[1] + M[10] * s[2] + M[11];
     uint32_t j;
     double d2;
-    corr[i] = (pd.r > 0.0 && cloud_nearest(pd.tree, pd.tp, p0, p1, p2, r2, &j, &d2)) ? (int32_t)j : -1;
+    corr[i] = (pd.r > 0.0 && cloud_nearest_with(cst, pd.tree, pd.tp, p0, p1, p2, r2, &j, &d2)) ? (int32_t)j : -1;
   }
 }
 
@@ -270,6 +273,8 @@ __global__ void __launch_bounds__(HT) k_lm_search(HuberView v, int it, int c0) {
   const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
   const int i = blockIdx.x * HT + threadIdx.x;
   if (i >= pd.ns) return;
+  __shared__ uint2 s_cstk[H_LSTACK][HT];
+  LdsStack<HT, H_LSTACK> cst{&s_cstk[0][threadIdx.x]};
   int32_t *corr = v.lm_corr + v.off[c];
   const int32_t *base = v.corr + v.off[c];
   const double *sT = st.T;
@@ -281,7 +286,8 @@ __global__ void __launch_bounds__(HT) k_lm_search(HuberView v, int it, int c0) {
   double d2;
   const int32_t prev = it > 0 ? corr[i] : base[i];
   const uint32_t seed = prev >= 0 ? (uint32_t)prev : NONE;
-  corr[i] = (pd.r > 0.0 && cloud_nearest(pd.tree, pd.tp, p0, p1, p2, pd.r * pd.r, &j, &d2, seed)) ? (int32_t)j : -1;
+  corr[i] =
+      (pd.r > 0.0 && cloud_nearest_with(cst, pd.tree, pd.tp, p0, p1, p2, pd.r * pd.r, &j, &d2, seed)) ? (int32_t)j : -1;
 }
 
 template <class T>
@@ -297,6 +303,10 @@ __global__ void __launch_bounds__(HT) k_lm_step(HuberView v, double delta, int i
   __shared__ T red[28];
   __shared__ double s_lambda, s_E, s_Ef;
   __shared__ int s_done, s_steps;
+  // Thread 0's 6x6 system and its solve work in LDS: as private arrays their pivot-indexed entries
+  // went to scratch.
+  __shared__ double s_H[36], s_g[6], s_x[6], s_m[36], s_y[6], s_tmp[6];
+  __shared__ int s_perm[6];
   if (threadIdx.x < 16) sT[threadIdx.x] = st.T[threadIdx.x];
   if (threadIdx.x == 0) {
     s_lambda = st.lambda;
@@ -337,7 +347,7 @@ __global__ void __launch_bounds__(HT) k_lm_step(HuberView v, double delta, int i
   if (threadIdx.x == 0) {
     s_E = (double)red[0];
     s_Ef = s_E;
-    double H[36], g[6], x[6];
+    double *H = s_H, *g = s_g, *x = s_x;
     int q = 1;
     for (int k = 0; k < 6; ++k)
       for (int l = 0; l <= k; ++l) {
@@ -350,7 +360,7 @@ __global__ void __launch_bounds__(HT) k_lm_step(HuberView v, double delta, int i
     double tr = 0.0;
     for (int k = 0; k < 6; ++k) tr += H[7 * k];
     for (int k = 0; k < 6; ++k) H[7 * k] += s_lambda * (fmax(H[7 * k], 0.0) + 1e-9 * tr / 6.0 + 1e-300);
-    bool ok = s_E > 0.0 && solve_ldlt(H, g, x);
+    bool ok = s_E > 0.0 && solve_ldlt_ws(H, g, x, s_m, s_y, s_tmp, s_perm);
     if (ok) {
       // A trust region: at most 0.05 rad of rotation and the correspondence radius of
       // translation per step.

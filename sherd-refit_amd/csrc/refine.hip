@@ -30,12 +30,15 @@ constexpr int RT = 256;
 __global__ void __launch_bounds__(RT) k_select(const double *V, int nv, SceneView tree, const double *C,
                                                const uint8_t *fracture, double radius, uint8_t *sel,
                                                uint32_t *pos) {
+  __shared__ uint2 s_stk[QSTACK][RT];
+  LdsStack<RT, QSTACK> st{&s_stk[0][threadIdx.x]};
   int i = blockIdx.x * RT + threadIdx.x;
   if (i >= nv) return;
   uint32_t j = NONE;
   double d2 = 0.0;
   bool keep = false;
-  if (cloud_nearest(tree, C, V[3 * (size_t)i], V[3 * (size_t)i + 1], V[3 * (size_t)i + 2], __builtin_inf(), &j, &d2))
+  if (cloud_nearest_with(st, tree, C, V[3 * (size_t)i], V[3 * (size_t)i + 1], V[3 * (size_t)i + 2], __builtin_inf(),
+                         &j, &d2))
     keep = fracture[j] != 0 && sqrt(d2) < radius;
   sel[i] = keep ? 1 : 0;
   pos[i] = keep ? 1u : 0u;

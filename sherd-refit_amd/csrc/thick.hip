@@ -53,6 +53,8 @@ __global__ void k_thick_normals(const double *V, int nv, const uint32_t *F, int 
 
 __global__ void __launch_bounds__(256) k_thick_rays(SceneView sv, const double *V, int nv, const uint32_t *F,
                                                     const double *N, int stride, int m, float *t_hit, uint32_t *prim) {
+  __shared__ uint2 s_stk[QSTACK][256];
+  LdsStack<256, QSTACK> st{&s_stk[0][threadIdx.x]};
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= m) return;
   uint32_t i = (uint32_t)r * (uint32_t)stride;
@@ -69,7 +71,7 @@ __global__ void __launch_bounds__(256) k_thick_rays(SceneView sv, const double *
   V3 d = V3{(float)d0, (float)d1, (float)d2};
   float t;
   uint32_t face;
-  first_hit(sv, o, d, 0, &t, &face);
+  first_hit_with(st, sv, o, d, 0, &t, &face);
   t_hit[r] = t;
   prim[r] = face;
 }
