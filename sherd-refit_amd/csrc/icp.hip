@@ -519,11 +519,13 @@ __global__ void __launch_bounds__(64 * UPD_WAVES) k_icp_wide_p2p(IcpWideView w, 
   if (go == 2) spans_total<9>(part, nspan, sig);
   if (lane != 0) return;
   double *U = s_ws + 78, *nt = s_ws + 94;
-  eye4(U);
   if (go == 2) {
     const P2pMeans m = p2p_means(red);
-    double mp[3] = {m.mp0, m.mp1, m.mp2}, mq[3] = {m.mq0, m.mq1, m.mq2};
-    umeyama_pose_ws(sig, mp, mq, U, s_ws, s_iws);
+    double mp[3] = {m.mp0, m.mp1, m.mp2}, mq[3] = {m.mq0, m.mq1, m.mq2}, ur[16];
+    umeyama_pose_reg(sig, mp, mq, ur);
+    for (int k = 0; k < 16; ++k) U[k] = ur[k];
+  } else {
+    eye4(U);
   }
   mul_nn(U, st.T, nt, 4);
   for (int k = 0; k < 16; ++k) {

@@ -211,6 +211,146 @@ SH_HD void umeyama_pose(const double *sigma, const double *mp, const double *mq,
   umeyama_pose_ws(sigma, mp, mq, out, ws, iws);
 }
 
+// The same Umeyama step for one lane with every array index fixed at compile time (the loops
+// unrolled, the singular values put in order by swapping whole columns rather than through an
+// index array), so it stays in registers: the operations of svd3_ws and umeyama_pose_ws in the same
+// order, hence the same bits, without the LDS round trip per access of the working-space form.
+__device__ __forceinline__ void mul3_reg(const double *a, const double *b, double *c) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double acc = a[i * 3] * b[j];
+      acc = acc + a[i * 3 + 1] * b[3 + j];
+      acc = acc + a[i * 3 + 2] * b[6 + j];
+      c[i * 3 + j] = acc;
+    }
+}
+
+__device__ __forceinline__ void svd3_reg(const double *m, double *u, double *s, double *v) {
+  double a[9], vv[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    a[i] = m[i];
+    vv[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  }
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int q = p + 1; q < 3; ++q) {
+        double al = 0, be = 0, ga = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          al += a[3 * i + p] * a[3 * i + p];
+          be += a[3 * i + q] * a[3 * i + q];
+          ga += a[3 * i + p] * a[3 * i + q];
+        }
+        if (fabs(ga) <= 1e-300) continue;
+        double o = fabs(ga) / sqrt(al * be > 0 ? al * be : 1e-300);
+        off = off < o ? o : off;
+        double zeta = (be - al) / (2.0 * ga);
+        double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          double ap = a[3 * i + p], aq = a[3 * i + q];
+          a[3 * i + p] = c * ap - sn * aq;
+          a[3 * i + q] = sn * ap + c * aq;
+          double vp = vv[3 * i + p], vq = vv[3 * i + q];
+          vv[3 * i + p] = c * vp - sn * vq;
+          vv[3 * i + q] = sn * vp + c * vq;
+        }
+      }
+    if (off < 1e-15) break;
+  }
+  // Columns with their singular values, ordered by the same exchanges as svd3_ws's index sort.
+  double sg[3], ac[3][3], vc[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    sg[j] = sqrt(a[j] * a[j] + a[3 + j] * a[3 + j] + a[6 + j] * a[6 + j]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      ac[j][i] = a[3 * i + j];
+      vc[j][i] = vv[3 * i + j];
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = x + 1; y < 3; ++y)
+      if (sg[y] > sg[x]) {
+        double t = sg[x];
+        sg[x] = sg[y];
+        sg[y] = t;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          t = ac[x][i];
+          ac[x][i] = ac[y][i];
+          ac[y][i] = t;
+          t = vc[x][i];
+          vc[x][i] = vc[y][i];
+          vc[y][i] = t;
+        }
+      }
+  const double smax = sg[0];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    s[c] = sg[c];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) v[3 * i + c] = vc[c][i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) u[3 * i + c] = (sg[c] > 1e-14 * smax && sg[c] > 0) ? ac[c][i] / sg[c] : 0.0;
+  }
+  double n0 = sqrt(u[0] * u[0] + u[3] * u[3] + u[6] * u[6]);
+  if (n0 == 0.0) {
+    u[0] = 1;
+    u[3] = 0;
+    u[6] = 0;
+  }
+  double n1 = sqrt(u[1] * u[1] + u[4] * u[4] + u[7] * u[7]);
+  if (n1 == 0.0) {
+    double x = u[0], y = u[3], z = u[6];
+    double w0 = fabs(x) < 0.9 ? 1.0 : 0.0, w1 = fabs(x) < 0.9 ? 0.0 : 1.0, w2 = 0.0;
+    double c0 = y * w2 - z * w1, c1 = z * w0 - x * w2, c2 = x * w1 - y * w0;
+    double l = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    u[1] = c0 / l;
+    u[4] = c1 / l;
+    u[7] = c2 / l;
+  }
+  double n2 = sqrt(u[2] * u[2] + u[5] * u[5] + u[8] * u[8]);
+  if (n2 == 0.0) {
+    u[2] = u[3] * u[7] - u[6] * u[4];
+    u[5] = u[6] * u[1] - u[0] * u[7];
+    u[8] = u[0] * u[4] - u[3] * u[1];
+  }
+}
+
+// umeyama_pose_ws in registers (sigma, mp, mq may be in LDS; out gets the 4x4).
+__device__ __forceinline__ void umeyama_pose_reg(const double *sigma, const double *mp, const double *mq,
+                                                 double *out) {
+  double u[9], s[3], v[9], vt[9], d[9], ud[9], rot[9];
+  svd3_reg(sigma, u, s, v);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) vt[3 * i + j] = v[3 * j + i];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) d[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  if (det3(u) * det3(vt) < 0.0) d[8] = -1.0;
+  mul3_reg(u, d, ud);
+  mul3_reg(ud, vt, rot);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) out[i] = (i % 5 == 0) ? 1.0 : 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) out[4 * i + j] = rot[3 * i + j];
+    out[4 * i + 3] = mq[i] - (rot[3 * i] * mp[0] + rot[3 * i + 1] * mp[1] + rot[3 * i + 2] * mp[2]);
+  }
+}
+
 // verify.rs:739-798 pose_inverse: LU with partial pivoting (first index on a tie), then one
 // forward and one back substitution per column.
 SH_HD void pose_inverse(const double *T, double *out) {
