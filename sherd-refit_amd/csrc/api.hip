@@ -263,6 +263,7 @@ int sherd_open(int device, sherd_ctx **out) {
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipStreamCreateWithFlags(&c->side.side[k], hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->side.early, hipEventDisableTiming);
   for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventCreateWithFlags(&c->side.join[k], hipEventDisableTiming);
   if (e != hipSuccess) {
     delete c;
@@ -305,6 +306,7 @@ void sherd_close(sherd_ctx *ctx) {
     if (c->side.join[k]) (void)hipEventDestroy(c->side.join[k]);
   }
   if (c->side.fork) (void)hipEventDestroy(c->side.fork);
+  if (c->side.early) (void)hipEventDestroy(c->side.early);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }

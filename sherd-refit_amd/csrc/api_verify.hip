@@ -381,7 +381,9 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     if (!dtab) return hip_fail(e, "scratch seam tables");
     uint64_t *dhard = (uint64_t *)c->scratch(19, sizeof(uint64_t) * 2 * nc * (size_t)std::max(max_s, 1), &e);
     if (!dhard) return hip_fail(e, "scratch");
-    uint32_t *dhard_n = (uint32_t *)c->scratch(20, sizeof(uint32_t) * 4, &e);
+    uint64_t *dhard_a = (uint64_t *)c->scratch(18, sizeof(uint64_t) * 2 * nc * (size_t)std::max(max_s, 1), &e);
+    if (!dhard_a) return hip_fail(e, "scratch");
+    uint32_t *dhard_n = (uint32_t *)c->scratch(20, sizeof(uint32_t) * 8, &e);
     if (!dhard_n) return hip_fail(e, "scratch");
     // The inside samples' list: at most every sample of every (candidate, side) task.
     uint64_t *ddeep = (uint64_t *)c->scratch(16, sizeof(uint64_t) * 2 * nc * (size_t)std::max(max_s, 1), &e);
@@ -403,6 +405,7 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     vb.accepted = dac;
     vb.task = dtask;
     vb.hard = dhard;
+    vb.hard_a = dhard_a;
     vb.hard_n = dhard_n;
     vb.deep = ddeep;
     vb.deep_t = ddeep_t;

@@ -437,6 +437,29 @@ __device__ inline float min_d2_subtree(const SceneView &sv, int c, V3 p, float w
   return best;
 }
 
+// A squared distance from p to one triangle of the scene, found by descending into the child of
+// nearer box at every node (+inf for an empty scene): an upper bound of the minimum, usually close
+// to it, for one tree depth of node fetches.
+__device__ inline float greedy_d2(const SceneView &sv, V3 p) {
+  const int nodes = 2 * sv.n - 1;
+  int cur = sv.root;
+  if (cur < 0 || cur >= nodes) return __builtin_inff();
+  Node nd = load_node(sv.nodes, cur);
+  for (int steps = 0; nd.a >= 0 && steps < nodes; ++steps) {
+    const int l = cur + 1, r = nd.a;
+    if (!(r > cur && r < nodes)) return __builtin_inff();
+    const Node ln = load_node(sv.nodes, l), rn = load_node(sv.nodes, r);
+    const bool left = box_d2(ln, p, 0.0f) <= box_d2(rn, p, 0.0f);
+    cur = left ? l : r;
+    nd = left ? ln : rn;
+  }
+  if (nd.a >= 0) return __builtin_inff();
+  V3 a, b, c;
+  uint32_t face;
+  load_tri(sv.tris, ~nd.a, &a, &b, &c, &face);
+  return tri_d2(p, a, b, c);
+}
+
 // min_d2 for one point by a whole wave: p and window are wave-uniform, and the wave keeps its
 // pending nodes in a stack in LDS (stk, cap entries, this wave's own), expanding up to 64 of them
 // per step, one per lane. Children are pushed when their box is within the current bound, leaves

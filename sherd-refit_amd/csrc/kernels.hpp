@@ -242,6 +242,7 @@ hipError_t launch_place(const double *p, const double *n, int cnt, const Pose &T
 struct SideStreams {
   hipStream_t side[3] = {};
   hipEvent_t fork = nullptr;
+  hipEvent_t early = nullptr;  // the fork of the apart candidates' closest approach, after k_prep
   hipEvent_t join[3] = {};
 };
 

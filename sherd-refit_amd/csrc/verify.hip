@@ -253,6 +253,15 @@ __global__ void k_prep(VerifyBatch vb) {
   if (c >= vb.n_cand) return;
   CandDesc &cd = vb.cand[c];
   pose_inverse(cd.T, cd.Ti);
+  // Apart: k_pen's whole-set tests dismiss both sides (no sample can be inside either mesh), so
+  // both closest approaches are needed whatever k_pen finds, and they can start before it.
+  {
+    const PairDesc &pd = vb.pair[cd.pair];
+    const FragDesc &A = vb.frag[pd.a], &B = vb.frag[pd.b];
+    cd.apart = pd.pen_ok &&
+               block_box_gap(moved_sphere(A.s_all, cd.Ti), B.mesh.lo, B.mesh.hi, B.mesh.absmax) > 0.0 &&
+               block_box_gap(moved_sphere(B.s_all, cd.T), A.mesh.lo, A.mesh.hi, A.mesh.absmax) > 0.0;
+  }
   CandAcc &a = vb.acc[c];
   for (int s = 0; s < 2; ++s) {
     a.facing[s] = a.tight[s] = a.touching[s] = 0;
@@ -262,7 +271,7 @@ __global__ void k_prep(VerifyBatch vb) {
     a.closest[s] = 0x7f7fffffu;  // f32::MAX, the window of the fallback (verify.rs:570-577)
   }
   a.n_cont = 0;
-  if (c == 0) vb.hard_n[0] = vb.hard_n[1] = vb.hard_n[2] = 0;
+  if (c == 0) vb.hard_n[0] = vb.hard_n[1] = vb.hard_n[2] = vb.hard_n[3] = vb.hard_n[4] = 0;
   a.seam_cells = 0;
   a.flags = 0;
 }
@@ -589,6 +598,7 @@ __device__ __forceinline__ float closest_window(CandAcc &acc, int side) {
 // the window.
 constexpr int CLOSE_MAXB = 2048;
 constexpr int SEED_BLOCKS = 64;
+constexpr int CLOSE_G = 8;  // workgroups per task of the block walk
 constexpr uint32_t CLAIMED = 0xffffffffu;  // above every key (non-negative f32 bits)
 
 __device__ __forceinline__ float close_key(const SceneView &sv, const CutLanes &c, const MovedBlock &m) {
@@ -600,82 +610,159 @@ __device__ __forceinline__ float close_key(const SceneView &sv, const CutLanes &
   return fmaxf(b, 0.0f);
 }
 
-__global__ void __launch_bounds__(VT) k_pen_close(VerifyBatch vb) {
-  __shared__ uint2 s_stk[LSTACK][VT];
-  __shared__ uint32_t s_key[CLOSE_MAXB];
-  __shared__ uint32_t s_next;
-  const uint32_t cs = vb.task[xcd_logical(blockIdx.x, gridDim.x)];
-  const int c = (int)(cs >> 1), side = (int)(cs & 1u);
-  const CandDesc &cd = vb.cand[c];
+// The (candidate, side) task of a closest-approach launch, or false when it has nothing to do: not
+// in this pass (apart or not), its closest approach not read, or no samples.
+struct CloseTask {
+  int c, side;
+  const FragDesc *own, *other;
+  const double *T;
+};
+__device__ __forceinline__ bool close_task(const VerifyBatch &vb, uint32_t cs, int apart, CloseTask *t) {
+  t->c = (int)(cs >> 1);
+  t->side = (int)(cs & 1u);
+  const CandDesc &cd = vb.cand[t->c];
   const PairDesc &pd = vb.pair[cd.pair];
-  if (!pd.pen_ok) return;  // (uniform: the workgroup is one task)
+  if (!pd.pen_ok || cd.apart != apart) return false;
+  if (!closest_needed(vb.acc[t->c], t->side)) return false;
+  t->own = &vb.frag[t->side == 0 ? pd.a : pd.b];
+  t->other = &vb.frag[t->side == 0 ? pd.b : pd.a];
+  t->T = t->side == 0 ? cd.Ti : cd.T;
+  return t->own->n_s > 0;
+}
+
+// The seed of a task, one workgroup: the sample of least estimate (the grid's bound or the nearest
+// cut box, the larger) over its blocks in key order while a block's key, less the grid's cell
+// diagonal by which a sample's bound may fall below its block's, does not exceed the best estimate
+// so far (at most SEED_BLOCKS blocks, four at a time, one per wave); its exact distance, by one whole
+// wave (min_d2_wave, its frontier in LDS), is the first window of the task's block walk. Which
+// sample seeds changes the work, never the minimum. A task with more than CLOSE_MAXB blocks has no
+// seed.
+// LDS (dynamic): the keys of the batch's largest sample set's blocks, their key order, then
+// SEED_FRONT ints of frontier.
+constexpr int SEED_FRONT = 1024;
+__global__ void __launch_bounds__(VT) k_pen_seed(VerifyBatch vb, int apart, int max_nb) {
+  extern __shared__ uint32_t s_dyn[];
+  uint32_t *s_key = s_dyn, *s_ord = s_dyn + max_nb;
+  int *s_front = reinterpret_cast<int *>(s_dyn + 2 * max_nb);
+  __shared__ unsigned long long s_best;  // (estimate bits << 32 | sample) of the best so far
+  __shared__ uint32_t s_bh;              // its estimate bits as the round began
+  __shared__ int s_stop;
+  CloseTask tk;
+  if (!close_task(vb, vb.task[xcd_logical(blockIdx.x, gridDim.x)], apart, &tk)) return;
+  const FragDesc &own = *tk.own, &other = *tk.other;
+  const double *T = tk.T;
+  CandAcc &acc = vb.acc[tk.c];
+  const int side = tk.side, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nb = (own.n_s + 63) >> 6;
+  if (nb > CLOSE_MAXB || nb > max_nb) return;
+  const CutLanes cut = cut_lanes(other.mesh);
+  for (int base = 64 * w; base < nb; base += VT) {  // (uniform per wave)
+    const int b = base + lane;
+    const float k = close_key(other.mesh, cut, moved_block(own.s_blk, 64 * (b < nb ? b : 0), T));
+    if (b < nb) s_key[b] = __float_as_uint(k);
+  }
+  if (threadIdx.x == 0) {
+    s_best = ~0ull;
+    s_bh = CLAIMED;
+  }
+  __syncthreads();
+  // Key order (ties by block index): each block's rank by counting the blocks before it.
+  for (int j = threadIdx.x; j < nb; j += VT) {
+    const uint32_t kj = s_key[j];
+    int r = 0;
+    for (int i = 0; i < nb; ++i) {
+      const uint32_t ki = s_key[i];
+      r += (ki < kj || (ki == kj && i < j)) ? 1 : 0;
+    }
+    s_ord[r] = (uint32_t)j;
+  }
+  __syncthreads();
+  const float cell = 1.8f * other.mesh.db.h;
+  const int last = nb < SEED_BLOCKS ? nb : SEED_BLOCKS;
+  for (int base = 0; base < last; base += VT / 64) {
+    const uint32_t bh = s_bh;
+    const int k = base + w;
+    bool take = false;
+    int blk = 0;
+    if (k < last) {
+      blk = (int)s_ord[k];
+      take = bh == CLAIMED || !(__uint_as_float(s_key[blk]) - cell > __uint_as_float(bh));
+    }
+    if (take) {  // (uniform per wave)
+      const int i = 64 * blk + lane;
+      const uint64_t mask = cut_seed_mask(other.mesh, cut, moved_block(own.s_blk, 64 * blk, T));
+      const double *pi = own.s + 3 * (size_t)(i < own.n_s ? i : 0);
+      const float e = sample_key_mask(other.mesh, cut, mask, moved_f32(T, pi[0], pi[1], pi[2]));
+      const uint64_t v = i < own.n_s ? (((uint64_t)__float_as_uint(e) << 32) | (uint32_t)i) : ~0ull;
+      const uint32_t hi = wave_min((uint32_t)(v >> 32));
+      const uint32_t lo = wave_min((uint32_t)(v >> 32) == hi ? (uint32_t)v : 0xffffffffu);
+      if (lane == 0) atomicMin(&s_best, ((unsigned long long)hi << 32) | lo);
+    }
+    __syncthreads();
+    // Blocks go by in key order: once one is past the best estimate, so is every later one. (One
+    // thread decides between two barriers, so every wave leaves the loop together.)
+    if (threadIdx.x == 0) {
+      const uint32_t nh = (uint32_t)(s_best >> 32);
+      const int kl = base + VT / 64;
+      s_bh = nh;
+      s_stop = kl >= last || (nh != CLAIMED && __uint_as_float(s_key[s_ord[kl]]) - cell > __uint_as_float(nh));
+    }
+    __syncthreads();
+    if (s_stop) break;
+  }
+  const uint32_t seed = (uint32_t)s_best;
+  if (w != 0 || seed == 0xffffffffu) return;
+  const double *p = own.s + 3 * (size_t)seed;
+  const V3 q = moved_f32(T, p[0], p[1], p[2]);
+  const float window = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(closest_window(acc, side))));
+  float far2;
+  (void)cut_d2(other.mesh, other.mesh.db.n_cut ? ~0ull >> (64 - other.mesh.db.n_cut) : 0ull, cut.lo, cut.hi, q, 0.0f,
+               &far2);
+  // The greedy descent's triangle is at a distance the seed attains: the search needs to reach no
+  // farther (a relative 2^-20 above covers the root's rounding).
+  const float radius = fminf(fminf(fminf(window, dist_upper(other.mesh.db, q)),
+                                   cut_reach(far2, slack_of(other.mesh.absmax, q))),
+                             sqrtf(greedy_d2(other.mesh, q)) * (1.0f + 0x1p-20f));
+  bool complete;
+  const float dist = sqrtf(min_d2_wave(other.mesh, q, radius, s_front, SEED_FRONT, WAVE_STEPS, &complete));
+  if (lane == 0 && dist <= window) atomicMin(&acc.closest[side], __float_as_uint(dist));
+  if (lane == 0 && !complete) atomicOr(&acc.flags, FLAG_GAVE_UP);
+}
+
+// The block walk, CLOSE_G workgroups per task after its seed: workgroup g takes blocks g,
+// g + CLOSE_G, ... (a spread of the fragment's Morton-ordered blocks), its waves claiming the one of
+// least key left, so a task's blocks are walked by CLOSE_G x 4 waves (a task far from the other
+// mesh has most of its blocks within the bound). Every workgroup stops once its least key exceeds
+// the task's window, which all of them lower.
+__global__ void __launch_bounds__(VT) k_pen_close(VerifyBatch vb, int apart) {
+  __shared__ uint2 s_stk[LSTACK][VT];
+  __shared__ uint32_t s_key[CLOSE_MAXB / CLOSE_G];
+  __shared__ uint32_t s_next;
+  const uint32_t t = xcd_logical(blockIdx.x, gridDim.x);
+  const int g = (int)(t % CLOSE_G);
+  CloseTask tk;
+  if (!close_task(vb, vb.task[t / CLOSE_G], apart, &tk)) return;  // (uniform: the workgroup is one task)
+  const FragDesc &own = *tk.own, &other = *tk.other;
+  const double *T = tk.T;
+  const int c = tk.c, side = tk.side;
   CandAcc &acc = vb.acc[c];
-  if (!closest_needed(acc, side)) return;
-  const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
-  const FragDesc &other = vb.frag[side == 0 ? pd.b : pd.a];
-  const double *T = side == 0 ? cd.Ti : cd.T;
-  if (own.n_s == 0) return;
+  // The queue of queries too long for one lane: the apart pass's own.
+  uint64_t *queue = apart ? vb.hard_a : vb.hard;
+  uint32_t *queued = vb.hard_n + (apart ? 3 : 0);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nb = (own.n_s + 63) >> 6;
+  const int nl = nb > g ? (nb - g + CLOSE_G - 1) / CLOSE_G : 0;  // this workgroup's blocks: g + CLOSE_G j
+  if (nl == 0) return;
   const bool ordered = nb <= CLOSE_MAXB;
   const CutLanes cut = cut_lanes(other.mesh);
   if (ordered) {
-    for (int base = 64 * w; base < nb; base += VT) {  // (uniform per wave)
-      const int b = base + lane;
-      const float k = close_key(other.mesh, cut, moved_block(own.s_blk, 64 * (b < nb ? b : 0), T));
-      if (b < nb) s_key[b] = __float_as_uint(k);
+    for (int base = 64 * w; base < nl; base += VT) {  // (uniform per wave)
+      const int j = base + lane, b = g + CLOSE_G * (j < nl ? j : 0);
+      const float k = close_key(other.mesh, cut, moved_block(own.s_blk, 64 * b, T));
+      if (j < nl) s_key[j] = __float_as_uint(k);
     }
   }
   if (threadIdx.x == 0) s_next = 0;
-  __syncthreads();
-  // The seed (wave 0, before the others search): the sample of least estimate (the grid's bound or
-  // the nearest cut box, the larger) over the blocks in key order while a block's key, less the
-  // grid's cell diagonal by which a sample's bound may fall below its block's, does not exceed the
-  // best estimate so far (at most SEED_BLOCKS blocks); its exact distance, by the whole wave with
-  // the lane stacks' memory as its stack, is the first window. Which sample seeds changes the
-  // work, never the minimum.
-  if (ordered && w == 0) {
-    const float cell = 1.8f * other.mesh.db.h;
-    uint64_t last = 0;  // (key << 32 | block) + 1 of the last block visited
-    uint32_t khi = CLAIMED, seed = 0xffffffffu;
-    for (int it = 0; it < SEED_BLOCKS; ++it) {
-      uint64_t nxt = ~0ull;
-      for (int j = lane; j < nb; j += 64) {
-        const uint64_t v = ((uint64_t)s_key[j] << 32) | (uint32_t)j;
-        if (v >= last && v < nxt) nxt = v;
-      }
-      const uint32_t hi = wave_min((uint32_t)(nxt >> 32));
-      const uint32_t lo = wave_min((uint32_t)(nxt >> 32) == hi ? (uint32_t)nxt : 0xffffffffu);
-      if (lo == 0xffffffffu) break;  // (uniform) no block left
-      if (khi != CLAIMED && __uint_as_float(hi) - cell > __uint_as_float(khi)) break;
-      last = (((uint64_t)hi << 32) | lo) + 1;
-      const int blk = (int)lo, i = 64 * blk + lane;
-      const uint64_t mask = cut_seed_mask(other.mesh, cut, moved_block(own.s_blk, 64 * blk, T));
-      const double *pi = own.s + 3 * (size_t)(i < own.n_s ? i : 0);
-      const float k = sample_key_mask(other.mesh, cut, mask, moved_f32(T, pi[0], pi[1], pi[2]));
-      uint64_t k2 = i < own.n_s ? (((uint64_t)__float_as_uint(k) << 32) | (uint32_t)i) : ~0ull;
-      const uint64_t cur = ((uint64_t)khi << 32) | seed;
-      k2 = k2 < cur ? k2 : cur;
-      khi = wave_min((uint32_t)(k2 >> 32));
-      seed = wave_min((uint32_t)(k2 >> 32) == khi ? (uint32_t)k2 : 0xffffffffu);
-    }
-    if (seed != 0xffffffffu) {
-      const double *p = own.s + 3 * (size_t)seed;
-      const V3 q = moved_f32(T, p[0], p[1], p[2]);
-      const float window =
-          __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(closest_window(acc, side))));
-      float far2;
-      (void)cut_d2(other.mesh, other.mesh.db.n_cut ? ~0ull >> (64 - other.mesh.db.n_cut) : 0ull, cut.lo, cut.hi, q,
-                   0.0f, &far2);
-      const float radius =
-          fminf(fminf(window, dist_upper(other.mesh.db, q)), cut_reach(far2, slack_of(other.mesh.absmax, q)));
-      bool complete;
-      const float dist = sqrtf(min_d2_wave(other.mesh, q, radius, reinterpret_cast<int *>(&s_stk[0][0]),
-                                           (int)(sizeof s_stk / sizeof(int)), WAVE_STEPS, &complete));
-      if (lane == 0 && dist <= window) atomicMin(&acc.closest[side], __float_as_uint(dist));
-      if (lane == 0 && !complete) atomicOr(&acc.flags, FLAG_GAVE_UP);
-    }
-  }
   __syncthreads();
   LdsStack<VT> st{&s_stk[0][threadIdx.x]};
   for (;;) {
@@ -683,7 +770,7 @@ __global__ void __launch_bounds__(VT) k_pen_close(VerifyBatch vb) {
     float w0 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(closest_window(acc, side))));
     if (ordered) {
       uint32_t kmin = CLAIMED, bmin = 0xffffffffu;
-      for (int j = lane; j < nb; j += 64) {
+      for (int j = lane; j < nl; j += 64) {
         const uint32_t k = __hip_atomic_load(&s_key[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (k < kmin) {
           kmin = k;
@@ -692,15 +779,17 @@ __global__ void __launch_bounds__(VT) k_pen_close(VerifyBatch vb) {
       }
       const uint32_t khi = wave_min(kmin);
       if (khi == CLAIMED || __uint_as_float(khi) > w0) break;  // (uniform)
-      b = (int)wave_min(kmin == khi ? bmin : 0xffffffffu);
+      const int j = (int)wave_min(kmin == khi ? bmin : 0xffffffffu);
       uint32_t old = 0;
-      if (lane == 0) old = atomicExch(&s_key[b], CLAIMED);
+      if (lane == 0) old = atomicExch(&s_key[j], CLAIMED);
       if (__builtin_amdgcn_readfirstlane(__shfl(old, 0)) == CLAIMED) continue;  // another wave took it
+      b = g + CLOSE_G * j;
     } else {
       uint32_t nx = 0;
       if (lane == 0) nx = atomicAdd(&s_next, 1u);
-      b = (int)__builtin_amdgcn_readfirstlane(__shfl(nx, 0));
-      if (b >= nb) break;
+      const int j = (int)__builtin_amdgcn_readfirstlane(__shfl(nx, 0));
+      if (j >= nl) break;
+      b = g + CLOSE_G * j;
     }
     const int i0 = 64 * b, i = i0 + lane;
     const MovedBlock mb = moved_block(own.s_blk, i0, T);
@@ -731,8 +820,8 @@ __global__ void __launch_bounds__(VT) k_pen_close(VerifyBatch vb) {
         hard = !complete;
       }
     }
-    uint32_t slot = wave_slot(&vb.hard_n[0], hard);
-    if (hard) vb.hard[slot] = ((uint64_t)c << 33) | ((uint64_t)side << 32) | (uint32_t)i;
+    uint32_t slot = wave_slot(queued, hard);
+    if (hard) queue[slot] = ((uint64_t)c << 33) | ((uint64_t)side << 32) | (uint32_t)i;
     best = wave_min(best);
     if (lane == 0 && best != 0xffffffffu) atomicMin(&acc.closest[side], best);
   }
@@ -793,16 +882,18 @@ __device__ K select_kth(const K *keys, int n, int k, uint32_t *hist, K *s_prefix
 
 // The closest-approach queries k_pen_close left unfinished, one per wave (min_d2_wave), taken from
 // the queue until it is empty.
-__global__ void __launch_bounds__(VT) k_pen_hard(VerifyBatch vb) {
+__global__ void __launch_bounds__(VT) k_pen_hard(VerifyBatch vb, int apart) {
   __shared__ int stk[VT / 64][HARD_STACK];
   int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint32_t n = vb.hard_n[0];
+  const uint64_t *queue = apart ? vb.hard_a : vb.hard;
+  uint32_t *counts = vb.hard_n + (apart ? 3 : 0);  // [0] queued, [1] taken
+  uint32_t n = counts[0];
   for (;;) {
     uint32_t idx = 0;
-    if (lane == 0) idx = atomicAdd(&vb.hard_n[1], 1u);
+    if (lane == 0) idx = atomicAdd(&counts[1], 1u);
     idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
     if (idx >= n) break;
-    uint64_t e = vb.hard[idx];
+    uint64_t e = queue[idx];
     int c = (int)(e >> 33), side = (int)((e >> 32) & 1u), i = (int)(uint32_t)e;
     const CandDesc &cd = vb.cand[c];
     const PairDesc &pd = vb.pair[cd.pair];
@@ -974,6 +1065,28 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
   const bool pen = vb.full && max_s > 0;
   const int per_s = blocks_for(max_s, VT);
   const int per_sg = blocks_for(per_s, CHUNKS);  // workgroups per task of the per-sample passes
+  // R §6.4's closest approach of the apart candidates needs nothing of k_pen: it starts at once on
+  // the first side stream (ahead of R §6.1 there), and fills what the inside passes leave idle.
+  const hipStream_t sa = ss ? ss->side[0] : s;
+  // The seed's LDS: keys for the largest sample set's blocks (none past CLOSE_MAXB), the frontier.
+  const int seed_nb = std::min(blocks_for(max_s, 64), CLOSE_MAXB);
+  const size_t seed_lds = sizeof(uint32_t) * 2 * (size_t)seed_nb + sizeof(int) * SEED_FRONT;
+  if (pen) {
+    if (ss) {
+      hipError_t e = hipEventRecord(ss->early, s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(sa, ss->early, 0);
+      if (e != hipSuccess) return e;
+    }
+    B("k_pen_seed_apart", sa);
+    hipLaunchKernelGGL(k_pen_seed, dim3(2 * nc), dim3(VT), seed_lds, sa, vb, 1, seed_nb);
+    E(sa);
+    B("k_pen_close_apart", sa);
+    hipLaunchKernelGGL(k_pen_close, dim3(2 * nc * CLOSE_G), dim3(VT), 0, sa, vb, 1);
+    E(sa);
+    B("k_pen_hard_apart", sa);
+    hipLaunchKernelGGL(k_pen_hard, dim3(HARD_BLOCKS), dim3(VT), 0, sa, vb, 1);
+    E(sa);
+  }
   if (pen) {
     // R §6.4's inside pass first and alone: it is the longest kernel and fills the device.
     B("k_pen", s);
@@ -1015,11 +1128,14 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
     E(s3);
   }
   if (pen) {
+    B("k_pen_seed", s);
+    hipLaunchKernelGGL(k_pen_seed, dim3(2 * nc), dim3(VT), seed_lds, s, vb, 0, seed_nb);
+    E(s);
     B("k_pen_close", s);
-    hipLaunchKernelGGL(k_pen_close, dim3(2 * nc), dim3(VT), 0, s, vb);
+    hipLaunchKernelGGL(k_pen_close, dim3(2 * nc * CLOSE_G), dim3(VT), 0, s, vb, 0);
     E(s);
     B("k_pen_hard", s);
-    hipLaunchKernelGGL(k_pen_hard, dim3(HARD_BLOCKS), dim3(VT), 0, s, vb);
+    hipLaunchKernelGGL(k_pen_hard, dim3(HARD_BLOCKS), dim3(VT), 0, s, vb, 0);
     E(s);
   }
   if (ss) {

@@ -53,6 +53,8 @@ struct PairDesc {
 
 struct CandDesc {
   int pair;
+  int apart;           // pen_ok, and on both sides the whole sample set's sphere misses the other mesh's box
+                       // (nothing can be inside: the closest approach is needed on both; set by k_prep)
   uint32_t seam_mask;  // seam voxel table capacity - 1
   size_t seam_off;     // into VerifyBatch::seam_table
   size_t seg[4];       // key segments: facing A, facing B, steps, agreements
@@ -84,7 +86,8 @@ struct VerifyBatch {
   uint8_t *accepted;
   const uint32_t *task;  // 2 * n_cand (candidate << 1 | side), grouped by the fragment queried
   uint64_t *hard;        // closest-approach queries left to whole waves: candidate << 33 | side << 32 | sample
-  uint32_t *hard_n;      // [0] queued, [1] taken, [2] inside samples listed in deep
+  uint64_t *hard_a;      // the same for the apart candidates' pass
+  uint32_t *hard_n;      // [0] queued, [1] taken, [2] inside samples listed in deep, [3] queued, [4] taken in hard_a
   uint64_t *deep;        // inside samples whose depth k_pen_depth measures, as in hard
   float *deep_t;         // and their nearest ray crossings
   int n_cand;
