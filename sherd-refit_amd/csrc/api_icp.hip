@@ -274,6 +274,7 @@ int grid_for(Ctx *c, int block, size_t n, int max_spans) {
 struct BatchDev {
   std::vector<IcpBatchView> v;
   std::vector<IcpWideView> w;  // the wide view of each rung (used by the wide ones)
+  std::vector<IcpWideSplit> split;  // where each rung's queue splits into two chains
   // The context's pinned staging buffer: the blob and the candidate pairs at its front (read by
   // copies until the stream drains), then `extra` bytes from `pin_at` for the caller.
   char *pin = nullptr;
@@ -343,6 +344,7 @@ int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand
   HIPCK(hipMemcpyAsync(dcp, blob + at_cp, sizeof(uint32_t) * n_cand, hipMemcpyHostToDevice, c->stream), "upload");
   bd->v.assign(nr, IcpBatchView{});
   bd->w.assign(nr, IcpWideView{});
+  bd->split.assign(nr, IcpWideSplit{});
   for (size_t r = 0; r < nr; ++r) {
     IcpBatchView &v = bd->v[r];
     v.pairs = (const IcpPairDesc *)(dblob + at[F * r]);
@@ -366,13 +368,36 @@ int stage_batch(Ctx *c, const std::vector<RungPlan> &plans, const uint32_t *cand
     w.n_spans = (int)plans[r].span_first.back();
     w.st = dst;
     w.part = dpart;
+    w.k0 = w.s0 = 0;
+    // Chains of about equal spans, as many as keep each at four spans per compute unit or more.
+    const std::vector<uint32_t> &sf = plans[r].span_first;
+    const size_t nq = sf.size() - 1;
+    const uint32_t total = sf.back();
+    IcpWideSplit &sp = bd->split[r];
+    sp.n = 1;
+    sp.k[1] = (int)nq;
+    sp.s[1] = (int)total;
+    const int parts = std::min<int>(ICP_CHAINS, (int)(total / (4u * (uint32_t)std::max(1, c->cus))));
+    if (parts > 1 && nq >= (size_t)parts) {
+      sp.n = parts;
+      for (int j = 1; j < parts; ++j) {
+        size_t k = std::lower_bound(sf.begin(), sf.end() - 1, (uint32_t)((uint64_t)total * j / parts)) - sf.begin();
+        k = std::max<size_t>(k, (size_t)sp.k[j - 1] + 1);
+        sp.k[j] = (int)std::min(k, nq - (size_t)(parts - j));
+        sp.s[j] = (int)sf[sp.k[j]];
+      }
+      sp.k[parts] = (int)nq;
+      sp.s[parts] = (int)total;
+    }
   }
   return SHERD_OK;
 }
 
 // `timed`: each wide pass's correspondence kernel between profiler events (never in a capture).
 hipError_t launch_rung(Ctx *c, const BatchDev &bd, size_t r, int block, int grid, bool timed) {
-  if (block == WIDE) return launch_icp_wide(bd.w[r], c->stream, timed && c->prof.on ? &c->prof : nullptr);
+  if (block == WIDE)
+    return launch_icp_wide(bd.w[r], c->stream, timed && c->prof.on ? &c->prof : nullptr, bd.split[r], c->side.side,
+                           c->side.fork, c->side.join);
   return launch_icp_batch(bd.v[r], block, grid, c->stream);
 }
 

@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(ICP_SPAN) k_icp_wide_corr(IcpWideView w, int i
   __shared__ double s_d2[ICP_SPAN];
   __shared__ uint32_t s_j[ICP_SPAN];
   __shared__ uint2 s_cstk[ICP_LSTACK][ICP_SPAN];
-  const int s = blockIdx.x;
+  const int s = w.s0 + (int)blockIdx.x;
   const int k = span_owner(w, s);
   const uint32_t c = w.b.order[k];
   const IcpWideState &st = w.st[c];
@@ -429,13 +429,14 @@ __global__ void __launch_bounds__(64) k_icp_wide_step(IcpWideView w, int it) {
   __shared__ double red[NV];
   __shared__ double s_ws[ICP_WS + 32];
   __shared__ int s_iws[6];
-  const uint32_t c = w.b.order[blockIdx.x];
+  const int kq = w.k0 + (int)blockIdx.x;
+  const uint32_t c = w.b.order[kq];
   IcpWideState &st = w.st[c];
   if (st.stop) return;
   const IcpBatchView &v = w.b;
   const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
   const int nspan = (pd.ns + ICP_SPAN - 1) / ICP_SPAN;
-  spans_total<NV>(w.part + (size_t)w.span_first[blockIdx.x] * NV, nspan, red);
+  spans_total<NV>(w.part + (size_t)w.span_first[kq] * NV, nspan, red);
   if (threadIdx.x != 0) return;
   const bool plane = v.est == 1 && pd.tn != nullptr;
   const bool p2p = v.est == 0;
@@ -474,13 +475,14 @@ __global__ void __launch_bounds__(64 * UPD_WAVES) k_icp_wide_p2p(IcpWideView w, 
   __shared__ double s_ws[ICP_WS + 32];
   __shared__ int s_iws[6];
   __shared__ int s_go;
-  const uint32_t c = w.b.order[blockIdx.x];
+  const int kq = w.k0 + (int)blockIdx.x;
+  const uint32_t c = w.b.order[kq];
   IcpWideState &st = w.st[c];
   if (st.stop) return;  // (uniform: one candidate per workgroup)
   const IcpBatchView &v = w.b;
   const IcpPairDesc pd = v.pairs[v.cand_pair[c]];
   const int nspan = (pd.ns + ICP_SPAN - 1) / ICP_SPAN;
-  double *part = w.part + (size_t)w.span_first[blockIdx.x] * NV;
+  double *part = w.part + (size_t)w.span_first[kq] * NV;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (wave == 0) {
     spans_total<NV>(part, nspan, red);
@@ -569,22 +571,47 @@ hipError_t launch_icp_batch(const IcpBatchView &v, int block, int grid, hipStrea
   return hipGetLastError();
 }
 
-hipError_t launch_icp_wide(const IcpWideView &w, hipStream_t s, KernelProfiler *prof) {
-  const int n = w.b.n_cand;
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_icp_wide_init, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, w);
+namespace {
+// One chain of a rung's passes over queue positions [k0, k1), spans [s0, s1).
+void wide_chain(IcpWideView w, int k0, int k1, int s0, int s1, hipStream_t s, KernelProfiler *prof) {
+  w.k0 = k0;
+  w.s0 = s0;
+  const unsigned nk = (unsigned)(k1 - k0), ns = (unsigned)(s1 - s0);
+  if (nk == 0) return;
   for (int it = 0; it <= w.b.max_iter; ++it) {
     if (prof) prof->begin("k_icp_wide_corr", s);
     if (w.b.est == EST_P2P)
-      hipLaunchKernelGGL(k_icp_wide_corr<EST_P2P>, dim3((unsigned)w.n_spans), dim3(ICP_SPAN), 0, s, w, it);
+      hipLaunchKernelGGL(k_icp_wide_corr<EST_P2P>, dim3(ns), dim3(ICP_SPAN), 0, s, w, it);
     else
-      hipLaunchKernelGGL(k_icp_wide_corr<EST_PLANE>, dim3((unsigned)w.n_spans), dim3(ICP_SPAN), 0, s, w, it);
+      hipLaunchKernelGGL(k_icp_wide_corr<EST_PLANE>, dim3(ns), dim3(ICP_SPAN), 0, s, w, it);
     if (prof) prof->end(s);
     if (w.b.est == EST_P2P && it < w.b.max_iter)
-      hipLaunchKernelGGL(k_icp_wide_p2p, dim3((unsigned)n), dim3(64 * UPD_WAVES), 0, s, w, it);
+      hipLaunchKernelGGL(k_icp_wide_p2p, dim3(nk), dim3(64 * UPD_WAVES), 0, s, w, it);
     else
-      hipLaunchKernelGGL(k_icp_wide_step, dim3((unsigned)n), dim3(64), 0, s, w, it);
+      hipLaunchKernelGGL(k_icp_wide_step, dim3(nk), dim3(64), 0, s, w, it);
   }
+}
+}  // namespace
+
+hipError_t launch_icp_wide(const IcpWideView &w, hipStream_t s, KernelProfiler *prof, const IcpWideSplit &split,
+                           const hipStream_t *side, hipEvent_t fork, const hipEvent_t *join) {
+  const int n = w.b.n_cand;
+  if (n == 0) return hipSuccess;
+  IcpWideView w0 = w;
+  w0.k0 = w0.s0 = 0;
+  hipLaunchKernelGGL(k_icp_wide_init, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, w0);
+  if (!side || split.n <= 1) {
+    wide_chain(w, 0, n, 0, w.n_spans, s, prof);
+    return hipGetLastError();
+  }
+  hipError_t e = hipEventRecord(fork, s);
+  for (int j = 1; j < split.n && e == hipSuccess; ++j) e = hipStreamWaitEvent(side[j - 1], fork, 0);
+  if (e != hipSuccess) return e;
+  for (int j = 0; j < split.n; ++j)
+    wide_chain(w, split.k[j], split.k[j + 1], split.s[j], split.s[j + 1], j == 0 ? s : side[j - 1], prof);
+  for (int j = 1; j < split.n && e == hipSuccess; ++j) e = hipEventRecord(join[j - 1], side[j - 1]);
+  for (int j = 1; j < split.n && e == hipSuccess; ++j) e = hipStreamWaitEvent(s, join[j - 1], 0);
+  if (e != hipSuccess) return e;
   return hipGetLastError();
 }
 

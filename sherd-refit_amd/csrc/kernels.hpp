@@ -172,9 +172,21 @@ struct IcpWideView {
   int n_spans;
   IcpWideState *st;             // per candidate
   double *part;                 // per span: ICP_NV sums
+  int k0, s0;                   // the first queue position and span of a launch (its grid's offset)
+};
+// How a rung's queue splits into chains: n parts, part j from queue position k[j] and span s[j]
+// (k[0] = s[0] = 0; k[n], s[n] the ends). One part when the rung is too small to split.
+constexpr int ICP_CHAINS = 2;
+struct IcpWideSplit {
+  int n = 1;
+  int k[ICP_CHAINS + 1] = {}, s[ICP_CHAINS + 1] = {};
 };
 // Queues the rung's max_iteration + 1 passes; results land in b.out / b.T_out as for the batched form.
-hipError_t launch_icp_wide(const IcpWideView &w, hipStream_t s, KernelProfiler *prof);
+// With a split, the parts of the queue run their passes as chains on s and on side[0..n-2] (forked by
+// `fork`, joined by join[0..n-2]), so one part's correspondence pass overlaps another's update;
+// every candidate's arithmetic is the same.
+hipError_t launch_icp_wide(const IcpWideView &w, hipStream_t s, KernelProfiler *prof, const IcpWideSplit &split,
+                           const hipStream_t *side, hipEvent_t fork, const hipEvent_t *join);
 
 // ---- R §5.1-5.5 over many pairs (stage.hip) -----------------------------------------------------
 // One fragment's breakline frames on the device (hypotheses.rs:47-68 `Frames`).

@@ -100,6 +100,29 @@ def test_workgroup_sizes_give_identical_results(ex):
         assert np.array_equal(o, outs[0])
 
 
+def test_wide_form_chains_equal_the_per_call_rung(ex):
+    say("case: wide form split into chains (72 candidates x 31 spans) == one candidate per call, and on replay")
+    # Over 2048 spans of 512 points the wide form runs the queue's halves as two chains on two
+    # streams; each candidate's arithmetic must not change (one candidate per call never splits).
+    raw = [synthetic_pair(150, 21), synthetic_pair(150, 22)]
+    assert min(len(r[0]) for r in raw) > 30 * 512
+    clouds = [(ex.cloud(src), ex.cloud(tp, tn), 0.5) for src, tp, tn in raw]
+    cand_pair = np.array([k % 2 for k in range(72)], dtype=np.uint32)
+    inits = perturbed(72, 23)
+    try:
+        ex.icp_block(1)
+        outs = [ex.icp_batch_rung(clouds, cand_pair, inits, "point_to_point", 20) for _ in range(3)]
+        for p, (src, tp, tn) in enumerate(raw):
+            sel = np.nonzero(cand_pair == p)[0]
+            for i in sel[:6]:
+                want = ex.icp_rung(src, tp, tn, inits[i:i + 1], "point_to_point", 0.5, 20)
+                assert np.array_equal(outs[0][i:i + 1], want), i
+    finally:
+        ex.icp_block(0)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
 def test_stage1_ladder_matches_chained_rungs_and_the_golden(ex):
     say("case: stage-1 ladder, 250 candidates x 2 rungs, graph replay")
     a, b, p, inits = stage1_setup()
