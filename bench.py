@@ -2,27 +2,24 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Workload ("plain method on GPU vs CPU"): the synthetic thin-wall vessel
-(height 200 t, spline radius 40-80 t, wall t = 1, rim 1.4 t) broken into 8 noisy Voronoi fragments at
-5.5 edges per t (0.9-1.6 M triangles each; the 17 edges/t of the configuration text would be ~87 M
-triangles), each under its own seeded rigid pose. All 28 pairs x 10 candidate poses
-(the true relative pose and nine perturbations of it) go through R §6's verification.
-
-One step = one batched verification of all 280 candidates (sherd_verify_batch), inputs resident on
-the device. `value` is pair-verifications per second over the timed steps. Beside it:
-  * the plain (unbatched, one candidate per call) HIP backend on the same candidates, and the
-    batched/plain ratio; accepted seams are checked identical between the two;
-  * rays/s of R §3.4.3's seven-ray cone vote over every face of every fragment;
-  * LBVH build rate, kNN queries/s;
-  * the roofline of the largest verification kernel that has the device to itself, its duration
-    measured with HIP events on the library's own stream around every launch (the sections that
-    run concurrently on side streams are listed beside it);
-  * the CPU baseline: the oracle (the C++ restatement of the CPU executor) on a bounded sample of
-    the same candidates, on this host's cores;
-  * the rows next to the path, each with its own oracle sample: the batched ICP ladder (`icp`),
-    the robust seam energy and its LM (`huber`), the 40-fragment collection (`collection`), R §9's
-    refinement (`refine`), R §5.1-5.5 over the vessel's pairs (`match`), R §3.2's wall thickness
-    (`thickness`) and R §3.4.4-3.4.8's segmentation clean-up (`segment`) over its fragments.
+Headline (the full collection): 40 fragments of three synthetic thin-wall
+vessels (walls t = 0.8 / 1.0 / 1.3 at 4 / 5 / 6 edges per t, 25.3 M triangles, two fragments made
+non-watertight), all 780 pairs x 4 candidate poses. One step = the batched pipeline over every
+candidate with the inputs resident on the device: the breakline ICP ladder (sherd_icp_batch_ladder),
+the Huber LM (sherd_huber_lm, f32 sums, FP64 6x6 solve) and R §6's verification with the FP64
+retest (sherd_verify_batch). `value` is pair-verifications per second over the timed steps (3120
+candidates per step); setup (scenes, parity grids, distance bounds, clouds) is timed apart. Beside it:
+  * the plain (per-pair / per-candidate) HIP schedule over its own timed steps, the batched/plain
+    ratio, and a check that every result is identical;
+  * the roofline of R §6.4's inside pass (k_pen), its launch time from HIP events on its stream over
+    the timed steps, with the measured HBM traffic of the committed counter profile beside it;
+  * the CPU baseline: the oracle's end-to-end pipeline (C++, `oracle/`) on a bounded sample of the
+    collection's candidates, on all host threads over candidates and on one, median of five;
+  * sections, each with its own oracle sample: configuration 2's vessel (`vessel`: verification,
+    cone-vote rays/s, kNN, LBVH build), configuration 1's slab plate (`slab`), configuration 3's
+    breakline ladder (`icp`), configuration 4's robust energy and LM (`huber`), R §9's refinement
+    (`refine`), R §5.1-5.5 (`match`), R §3.2's thickness (`thickness`) and R §3.4.4-3.4.8's
+    segmentation clean-up (`segment`).
 """
 
 import argparse
@@ -38,9 +35,8 @@ sys.path.insert(0, os.path.join(HERE, "sherd-refit_amd"))
 
 METRIC = "pair-verifications/s + rays/s, batched vs plain HIP, 1 MI355X, same accepted seams"
 HBM_PEAK_GBS = 8000.0
-# The verification kernel whose roofline the line reports: the largest with the device to itself
-# (k_prep, k_median and k_final, the others that run alone, are bookkeeping of under 0.1 ms per
-# step); and the three sections that run concurrently on side streams.
+# The vessel section's roofline kernel, R §6.4's inside pass (k_pen); and the three sections that run
+# concurrently on side streams after it.
 ROOFLINE_KERNEL = "k_pen"
 OVERLAPPED_KERNELS = ("k_frac", "k_seam", "k_cont")
 
@@ -60,7 +56,7 @@ def algorithmic_bytes(kernel, frags, pairs, cand_pair):
     for c in cand_pair:
         a, b = pairs[c]
         fa, fb = frags[a], frags[b]
-        if kernel in ("k_pen", "k_pen_close"):
+        if kernel == "k_pen":
             total += 24 * (len(fa["S"]) + len(fb["S"]))
         elif kernel == "k_frac":
             total += (24 + 8) * (len(fa["Pf"]) + len(fb["Pf"]))  # the point, its distance key
@@ -994,27 +990,24 @@ def kernel_roofline(kernel, nbytes, avg_ms, pmc_file):
     return out
 
 
-# Main-stream kernels of the collection step and the §8(d) unit each is counted in: bytes per point
-# processed (the f64 sample read, 24 B; their results are reduced in the kernel), over every
-# (candidate, side) task (k_pen) or over the tasks whose closest approach is needed (both sides
-# without a sample inside: k_pen_close, which reads every block's sphere but the samples of only the
-# blocks that can hold the minimum). The meshes the points are tested against are not counted: a
-# launch reads only the parts near its samples (92 % of samples fail the box reject), and the bytes
-# it does read are the measured traffic beside it.
-POINT_KERNELS = ("k_pen", "k_pen_close")
+# The collection step's roofline kernel and its §8(d) unit: k_pen, R §6.4's inside pass, the largest
+# kernel with a per-point unit, in bytes per point processed (the f64 sample read, 24 B; its results
+# are reduced in the kernel) over every (candidate, side) task of a watertight pair. The meshes the
+# points are tested against are not counted: a launch reads only the parts near its samples (most
+# samples fail the box reject), and the bytes it does read are the measured traffic beside it. It
+# shares the device with the apart candidates' closest approach (a side stream), so its launch time
+# is the time it takes beside that work.
+POINT_KERNELS = ("k_pen",)
 
 
 def collection_roofline(head):
-    """The roofline of the collection step's dominant kernel: the one with the most device time
-    among those that run alone on the library's stream and have a per-point unit (the verification
-    sections on side streams, the queue of long queries and the records of launch sequences are
-    listed beside it, not chosen)."""
+    """The roofline of the collection step's per-point kernel (POINT_KERNELS); every other kernel of
+    the step is listed beside it with its time per launch (HIP events on its own stream)."""
     prof, coll, pairs, cp, scores = head["prof"], head["coll"], head["pairs"], head["cp"], head["scores"]
     frags = coll["fragments"]
     n_s = np.array([len(frags[a]["S"]) + len(frags[b]["S"]) for a, b in pairs])[cp]
     pen_ok = scores[:, 16] == 0.0
-    apart = pen_ok & (scores[:, 15] < 0.0)  # pen_depth < 0: no sample inside on either side
-    points = {"k_pen": int(n_s[pen_ok].sum()), "k_pen_close": int(n_s[apart].sum())}
+    points = {"k_pen": int(n_s[pen_ok].sum())}
     cands = {k: v for k, v in prof.items() if k in POINT_KERNELS and v["launches"]}
     dom = max(cands, key=lambda k: cands[k]["ms"])
     avg_ms = prof[dom]["ms"] / prof[dom]["launches"]
