@@ -8,17 +8,22 @@
 // arithmetic (bvh_query.hpp, grid.hpp), so every score equals the CPU restatement's bit for bit.
 //
 // Kernels, each over all candidates of the batch:
-//   k_prep     pose inverse per candidate (verify.rs:739-798), counters cleared
-//   k_frac     R §6.1: both sides' fracture samples against the other's fracture scene
-//   k_seam     R §6.2: A's breakline against the moved B breakline (through B's breakline grid),
-//              voxel set of the seam points
-//   k_cont     R §6.3: B's margin against A's margin grid, steps and normal agreements
-//   k_pen      R §6.4: surface samples inside the other mesh, deeper-than-pen count, deepest
-//   k_pen_close R §6.4's fallback when nothing is inside: the closest approach, one workgroup
-//              per (candidate, side), its blocks of samples nearest first
-//   k_pen_hard the closest-approach queries too long for one lane, one wave each
-//   k_median   the four medians (numpy semantics, verify.rs:693-700)
-//   k_final    Scores and the accept verdict
+//   k_prep      pose inverse per candidate (verify.rs:739-798), counters cleared, the apart flag
+//   k_frac      R §6.1: both sides' fracture samples against the other's fracture scene
+//   k_seam      R §6.2: A's breakline against the moved B breakline (through B's breakline grid),
+//               voxel set of the seam points
+//   k_cont      R §6.3: B's margin against A's margin grid, steps and normal agreements
+//   k_pen       R §6.4: surface samples inside the other mesh (ray parity), listed with their
+//               nearest crossing
+//   k_pen_depth the listed samples' depths: deeper-than-pen count, deepest
+//   k_pen_seed  R §6.4's fallback when nothing is inside, the closest approach: per (candidate,
+//               side) the exact distance of the sample of least estimate, the first window
+//   k_pen_close the blocks of samples nearest first, eight workgroups per (candidate, side)
+//   k_pen_hard  the closest-approach queries too long for one lane, one wave each
+//   k_median    the four medians (numpy semantics, verify.rs:693-700)
+//   k_final     Scores and the accept verdict
+// The closest approach runs twice: for the candidates whose sample sets both miss the other mesh's
+// box (nothing can be inside) beside k_pen, for the others after k_pen_depth.
 #include <hip/hip_runtime.h>
 
 #include "bvh_query.hpp"
