@@ -20,7 +20,7 @@
 //               side) the exact distance of the sample of least estimate, the first window
 //   k_pen_close the blocks of samples nearest first, eight workgroups per (candidate, side)
 //   k_pen_hard  the closest-approach queries too long for one lane, one wave each
-//   k_median    the four medians (numpy semantics, verify.rs:693-700)
+//   k_median    the four medians (numpy semantics, verify.rs:693-700), each pair after its section
 //   k_final     Scores and the accept verdict
 // The closest approach runs twice: for the candidates whose sample sets both miss the other mesh's
 // box (nothing can be inside) beside k_pen, for the others after k_pen_depth.
@@ -951,8 +951,10 @@ __device__ double midpoint(double a, double b) {
 }
 
 // verify.rs:693-700 numpy's median over each candidate's four segments.
-__global__ void __launch_bounds__(VT) k_median(VerifyBatch vb) {
-  int c = blockIdx.x >> 2, seg = blockIdx.x & 3;
+// Statistics seg0 and seg0 + 1 of every candidate: the facing distances of R §6.1 (0, 1), or R
+// §6.3's steps and agreements (2, 3).
+__global__ void __launch_bounds__(VT) k_median(VerifyBatch vb, int seg0) {
+  int c = blockIdx.x >> 1, seg = seg0 + (int)(blockIdx.x & 1);
   CandAcc &acc = vb.acc[c];
   int n = seg < 2 ? (int)acc.facing[seg] : (int)acc.n_cont;
   __shared__ uint32_t hist[256];
@@ -1103,7 +1105,7 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
   }
   // The sections below read the poses k_prep completed and write disjoint accumulators and key
   // segments, so they may run side by side with the rest of R §6.4, whose closest-approach
-  // passes leave most of the device idle; k_median waits for all of them.
+  // passes leave most of the device idle; k_final waits for all of them.
   hipStream_t s1 = s, s2 = s, s3 = s;
   if (ss) {
     hipError_t e = hipEventRecord(ss->fork, s);
@@ -1119,6 +1121,10 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
     hipLaunchKernelGGL(k_frac, dim3(2 * nc * per), dim3(VT), 0, s1, vb, per);
     E(s1);
   }
+  // Each statistic's medians right after the section that fills its keys, on that section's stream.
+  B("k_median", s1);
+  hipLaunchKernelGGL(k_median, dim3(2 * nc), dim3(VT), 0, s1, vb, 0);
+  E(s1);
   B("k_seam_clear", s2);
   hipLaunchKernelGGL(k_seam_clear, dim3(nc), dim3(VT), 0, s2, vb);
   E(s2);
@@ -1132,6 +1138,9 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
     hipLaunchKernelGGL(k_cont, dim3(blocks_for(blocks_for(max_margin, VT), CHUNKS), nc), dim3(VT), 0, s3, vb);
     E(s3);
   }
+  B("k_median", s3);
+  hipLaunchKernelGGL(k_median, dim3(2 * nc), dim3(VT), 0, s3, vb, 2);
+  E(s3);
   if (pen) {
     B("k_pen_seed", s);
     hipLaunchKernelGGL(k_pen_seed, dim3(2 * nc), dim3(VT), seed_lds, s, vb, 0, seed_nb);
@@ -1149,9 +1158,6 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
     for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipStreamWaitEvent(s, ss->join[k], 0);
     if (e != hipSuccess) return e;
   }
-  B("k_median", s);
-  hipLaunchKernelGGL(k_median, dim3(4 * nc), dim3(VT), 0, s, vb);
-  E(s);
   B("k_final", s);
   hipLaunchKernelGGL(k_final, dim3(blocks_for(nc, 64)), dim3(64), 0, s, vb);
   E(s);
