@@ -23,7 +23,7 @@
 //   k_median    the four medians (numpy semantics, verify.rs:693-700), each pair after its section
 //   k_final     Scores and the accept verdict
 // The closest approach runs twice: for the candidates whose sample sets both miss the other mesh's
-// box (nothing can be inside) beside k_pen, for the others after k_pen_depth.
+// box (nothing can be inside) beside k_pen, for the others after it (beside k_pen_depth).
 #include <hip/hip_runtime.h>
 
 #include "bvh_query.hpp"
@@ -560,14 +560,13 @@ __global__ void __launch_bounds__(VT) k_pen_depth(VerifyBatch vb) {
   }
 }
 
-// Whether side `side`'s closest approach can change the scores. pen_depth is
-// max(-min_a, -min_b) / t (verify.rs:494), min being -deepest on a side with a sample inside and the
-// closest approach (at least 0) on a side without: when the other side has a sample inside at a
-// positive depth, this side's term is at most 0 < that depth and is never read (at depth 0 exactly
-// it still is, for the sign of a zero).
-__device__ __forceinline__ bool closest_needed(const CandAcc &acc, int side) {
-  return acc.inside[side] == 0 && !(acc.inside[1 - side] != 0 && acc.deepest[1 - side] != 0u);
-}
+// Whether side `side`'s closest approach is measured: when nothing of it is inside the other mesh.
+// pen_depth is max(-min_a, -min_b) / t (verify.rs:494), min being -deepest on a side with a sample
+// inside and the closest approach (at least 0) on a side without (k_final). When the other side has
+// a sample inside at a positive depth this side's term is never read, but it is measured anyway:
+// that needs the depths (k_pen_depth), and without them the closest approach starts as soon as
+// k_pen has counted the inside samples.
+__device__ __forceinline__ bool closest_needed(const CandAcc &acc, int side) { return acc.inside[side] == 0; }
 
 // The window a side's closest approach may prune with: the best distance attained so far on this
 // side, and on the other side too when that side has nothing inside. Only min(closest_a, closest_b)
@@ -1099,9 +1098,6 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
     B("k_pen", s);
     hipLaunchKernelGGL(k_pen, dim3(2 * nc * per_sg), dim3(VT), 0, s, vb, per_sg);
     E(s);
-    B("k_pen_depth", s);
-    hipLaunchKernelGGL(k_pen_depth, dim3(DEPTH_BLOCKS), dim3(VT), 0, s, vb);
-    E(s);
   }
   // The sections below read the poses k_prep completed and write disjoint accumulators and key
   // segments, so they may run side by side with the rest of R §6.4, whose closest-approach
@@ -1125,6 +1121,12 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
   B("k_median", s1);
   hipLaunchKernelGGL(k_median, dim3(2 * nc), dim3(VT), 0, s1, vb, 0);
   E(s1);
+  // The depths of the inside samples need only k_pen's list: ahead of R §6.2 on its stream.
+  if (pen) {
+    B("k_pen_depth", s2);
+    hipLaunchKernelGGL(k_pen_depth, dim3(DEPTH_BLOCKS), dim3(VT), 0, s2, vb);
+    E(s2);
+  }
   B("k_seam_clear", s2);
   hipLaunchKernelGGL(k_seam_clear, dim3(nc), dim3(VT), 0, s2, vb);
   E(s2);
