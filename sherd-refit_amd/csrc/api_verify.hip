@@ -385,7 +385,12 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     if (!dhard_a) return hip_fail(e, "scratch");
     uint32_t *dhard_n = (uint32_t *)c->scratch(20, sizeof(uint32_t) * 8, &e);
     if (!dhard_n) return hip_fail(e, "scratch");
-    // The inside samples' list: at most every sample of every (candidate, side) task.
+    // The in-box samples' list and the inside samples': at most every sample of every (candidate,
+    // side) task.
+    uint32_t *dboxq = (uint32_t *)c->scratch(9, sizeof(uint32_t) * 2 * nc * (size_t)std::max(max_s, 1), &e);
+    if (!dboxq) return hip_fail(e, "scratch");
+    uint64_t *dboxw = (uint64_t *)c->scratch(11, sizeof(uint64_t) * 2 * nc * (((size_t)std::max(max_s, 1) + 63) / 64), &e);
+    if (!dboxw) return hip_fail(e, "scratch");
     uint64_t *ddeep = (uint64_t *)c->scratch(16, sizeof(uint64_t) * 2 * nc * (size_t)std::max(max_s, 1), &e);
     if (!ddeep) return hip_fail(e, "scratch");
     float *ddeep_t = (float *)c->scratch(17, sizeof(float) * 2 * nc * (size_t)std::max(max_s, 1), &e);
@@ -407,6 +412,9 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     vb.hard = dhard;
     vb.hard_a = dhard_a;
     vb.hard_n = dhard_n;
+    vb.boxq = dboxq;
+    vb.boxw = dboxw;
+    vb.box_cap = std::max(max_s, 1);
     vb.deep = ddeep;
     vb.deep_t = ddeep_t;
     vb.n_cand = (int)nc;

@@ -13,8 +13,9 @@
 //   k_seam      R §6.2: A's breakline against the moved B breakline (through B's breakline grid),
 //               voxel set of the seam points
 //   k_cont      R §6.3: B's margin against A's margin grid, steps and normal agreements
-//   k_pen       R §6.4: surface samples inside the other mesh (ray parity), listed with their
-//               nearest crossing
+//   k_pen       R §6.4: surface samples inside the other mesh's box, listed
+//   k_pen_groups, k_pen_rays  those samples' parity rays in full waves: the inside ones counted and listed with
+//               their nearest crossing
 //   k_pen_depth the listed samples' depths: deeper-than-pen count, deepest
 //   k_pen_seed  R §6.4's fallback when nothing is inside, the closest approach: per (candidate,
 //               side) the exact distance of the sample of least estimate, the first window
@@ -276,7 +277,7 @@ __global__ void k_prep(VerifyBatch vb) {
     a.closest[s] = 0x7f7fffffu;  // f32::MAX, the window of the fallback (verify.rs:570-577)
   }
   a.n_cont = 0;
-  if (c == 0) vb.hard_n[0] = vb.hard_n[1] = vb.hard_n[2] = vb.hard_n[3] = vb.hard_n[4] = 0;
+  if (c == 0) vb.hard_n[0] = vb.hard_n[1] = vb.hard_n[2] = vb.hard_n[3] = vb.hard_n[4] = vb.hard_n[5] = 0;
   a.seam_cells = 0;
   a.flags = 0;
 }
@@ -469,9 +470,11 @@ __global__ void __launch_bounds__(VT) k_cont(VerifyBatch vb) {
   }
 }
 
-// R §6.4 one_penetration's first pass (verify.rs:541-566), its parity half: side 0 is A's surface
-// samples moved by T^-1 against B's mesh, side 1 is B's moved by T against A's mesh. Inside samples
-// are counted and listed with their nearest ray crossing for k_pen_depth.
+// R §6.4 one_penetration's first pass (verify.rs:541-566), RayScene::inside's AABB reject: side 0
+// is A's surface samples moved by T^-1 against B's mesh, side 1 is B's moved by T against A's mesh.
+// The samples inside the mesh's box (a tenth of those k_pen reads on the collection) are counted
+// and listed in the task's own segment of boxq (one counter per task, so the waves' atomics spread
+// over the tasks), for k_pen_rays, which casts their rays one group of 64 entries per wave.
 __global__ void __launch_bounds__(VT) k_pen(VerifyBatch vb, int per_task) {
   const TaskGroup g = task_group(vb, per_task);
   const int c = g.c, side = g.side;
@@ -485,51 +488,92 @@ __global__ void __launch_bounds__(VT) k_pen(VerifyBatch vb, int per_task) {
   // No sample can be inside when the whole set lies outside the mesh's box (by more than the
   // rounding), nor a block's when its sphere does: those skip the moves.
   if (block_box_gap(moved_sphere(own.s_all, T), sv.lo, sv.hi, sv.absmax) > 0.0) return;
-  // Most samples fall outside the other mesh's box (RayScene::inside's reject). Per chunk, the
-  // workgroup gathers the ones inside it in LDS and its first lanes take them, one each, so the
-  // rays run in as few waves as there are such samples.
-  __shared__ float s_q[3][VT];
-  __shared__ uint32_t s_i[VT];
-  __shared__ uint32_t s_n;
   CandAcc &acc = vb.acc[c];
+  uint32_t *list = vb.boxq + (size_t)(2 * c + side) * (size_t)vb.box_cap;
   for (int k = 0; k < CHUNKS; ++k) {
     const int base = (g.chunk0 + k) * VT;
     if (base >= own.n_s) break;  // (uniform)
-    if (threadIdx.x == 0) s_n = 0;
-    __syncthreads();
     const int i = base + threadIdx.x, i0 = i & ~63;
-    const bool skip = i0 >= own.n_s || block_box_gap(moved_block(own.s_blk, i0, T), sv.lo, sv.hi, sv.absmax) > 0.0;
-    if (i < own.n_s && !skip) {
+    if (i0 >= own.n_s || block_box_gap(moved_block(own.s_blk, i0, T), sv.lo, sv.hi, sv.absmax) > 0.0)
+      continue;  // (uniform per wave: a wave is one block)
+    bool in = false;
+    if (i < own.n_s) {
       const double *p = own.s + 3 * (size_t)i;
-      V3 q = moved_f32(T, p[0], p[1], p[2]);
-      if (!(q.x < sv.lo[0] || q.x > sv.hi[0] || q.y < sv.lo[1] || q.y > sv.hi[1] || q.z < sv.lo[2] ||
-            q.z > sv.hi[2])) {
-        uint32_t s = atomicAdd(&s_n, 1u);
-        s_q[0][s] = q.x;
-        s_q[1][s] = q.y;
-        s_q[2][s] = q.z;
-        s_i[s] = (uint32_t)i;
-      }
+      const V3 q = moved_f32(T, p[0], p[1], p[2]);
+      in = !(q.x < sv.lo[0] || q.x > sv.hi[0] || q.y < sv.lo[1] || q.y > sv.hi[1] || q.z < sv.lo[2] ||
+             q.z > sv.hi[2]);
     }
+    const uint32_t slot = wave_slot(&acc.in_box[side], in);
+    if (in) list[slot] = (uint32_t)i;
+  }
+}
+
+// The groups of 64 entries of every task's in-box list, in the tasks' order (by the fragment they
+// query), into boxw, and their number into hard_n[5]: one workgroup, each thread a run of tasks,
+// their offsets by a prefix sum in LDS. (A global counter that every group took a slot of cost more
+// than this launch: one address's atomics are serialised across the device.)
+constexpr int GROUPS_VT = 1024;
+__global__ void __launch_bounds__(GROUPS_VT) k_pen_groups(VerifyBatch vb) {
+  __shared__ uint32_t s_sum[GROUPS_VT];
+  const int nt = 2 * vb.n_cand, per = (nt + GROUPS_VT - 1) / GROUPS_VT;
+  const int t0 = (int)threadIdx.x * per, t1 = t0 + per < nt ? t0 + per : nt;
+  uint32_t mine = 0;
+  for (int j = t0; j < t1; ++j) {
+    const uint32_t cs = vb.task[j];
+    mine += (vb.acc[cs >> 1].in_box[cs & 1u] + 63u) >> 6;
+  }
+  s_sum[threadIdx.x] = mine;
+  __syncthreads();
+  for (int off = 1; off < GROUPS_VT; off <<= 1) {  // inclusive scan
+    const uint32_t v = (int)threadIdx.x >= off ? s_sum[threadIdx.x - off] : 0u;
     __syncthreads();
-    const uint32_t n = s_n;
-    if (threadIdx.x == 0 && n) atomicAdd(&acc.in_box[side], n);
-    if (threadIdx.x < ((n + 63u) & ~63u)) {  // whole waves only
-      bool in = false;
-      float t = 0.0f;
-      if (threadIdx.x < n) {
-        V3 q = V3{s_q[0][threadIdx.x], s_q[1][threadIdx.x], s_q[2][threadIdx.x]};
-        in = inside_parity(sv, q, vb.retest, &t);
-      }
-      const uint32_t ni = (uint32_t)__popcll(__ballot(in));
-      if ((threadIdx.x & 63) == 0 && ni) atomicAdd(&acc.inside[side], ni);
-      const uint32_t slot = wave_slot(&vb.hard_n[2], in);
-      if (in) {
-        vb.deep[slot] = ((uint64_t)c << 33) | ((uint64_t)side << 32) | s_i[threadIdx.x];
-        vb.deep_t[slot] = t;
-      }
+    s_sum[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t at = s_sum[threadIdx.x] - mine;
+  if (threadIdx.x == GROUPS_VT - 1) vb.hard_n[5] = s_sum[threadIdx.x];
+  for (int j = t0; j < t1; ++j) {
+    const uint32_t cs = vb.task[j];
+    const uint32_t g = (vb.acc[cs >> 1].in_box[cs & 1u] + 63u) >> 6;
+    for (uint32_t q = 0; q < g; ++q) vb.boxw[at++] = ((uint64_t)cs << 32) | q;
+  }
+}
+
+// R §6.4's parity half for the samples k_pen listed, one group of 64 entries per wave (so the rays
+// run in full waves whatever share of a task's samples is in the box), the waves of the grid
+// taking the groups in turn (a logical workgroup's waves take consecutive groups, on one XCD):
+// each lane's sample moved again, three axis rays with a majority (inside_parity); inside samples
+// are counted and listed with their nearest ray crossing for k_pen_depth.
+constexpr int RAY_BLOCKS = 2048;
+__global__ void __launch_bounds__(VT) k_pen_rays(VerifyBatch vb) {
+  const uint32_t n = vb.hard_n[5];
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t lb = xcd_logical(blockIdx.x, gridDim.x);
+  for (uint32_t w = (lb * VT + threadIdx.x) >> 6; w < n; w += (gridDim.x * VT) >> 6) {
+    const uint64_t item = vb.boxw[w];
+    const uint32_t task = (uint32_t)(item >> 32), k = 64u * (uint32_t)item + lane;
+    const int c = (int)(task >> 1), side = (int)(task & 1u);
+    CandAcc &acc = vb.acc[c];
+    bool in = false;
+    float t = 0.0f;
+    uint32_t i = 0;
+    if (k < acc.in_box[side]) {
+      const CandDesc &cd = vb.cand[c];
+      const PairDesc &pd = vb.pair[cd.pair];
+      const FragDesc &own = vb.frag[side == 0 ? pd.a : pd.b];
+      const SceneView &sv = vb.frag[side == 0 ? pd.b : pd.a].mesh;
+      const double *T = side == 0 ? cd.Ti : cd.T;
+      i = vb.boxq[(size_t)task * (size_t)vb.box_cap + k];
+      const double *p = own.s + 3 * (size_t)i;
+      in = inside_parity(sv, moved_f32(T, p[0], p[1], p[2]), vb.retest, &t);
     }
-    __syncthreads();  // the gathered samples are read before the next chunk writes them
+    const uint32_t ni = (uint32_t)__popcll(__ballot(in));
+    if (lane == 0 && ni) atomicAdd(&acc.inside[side], ni);
+    const uint32_t slot = wave_slot(&vb.hard_n[2], in);
+    if (in) {
+      vb.deep[slot] = ((uint64_t)task << 32) | i;
+      vb.deep_t[slot] = t;
+    }
   }
 }
 
@@ -1094,9 +1138,13 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
     E(sa);
   }
   if (pen) {
-    // R §6.4's inside pass first and alone: it is the longest kernel and fills the device.
+    // R §6.4's inside pass first and alone: the box reject, then the rays of the samples it kept.
     B("k_pen", s);
     hipLaunchKernelGGL(k_pen, dim3(2 * nc * per_sg), dim3(VT), 0, s, vb, per_sg);
+    E(s);
+    hipLaunchKernelGGL(k_pen_groups, dim3(1), dim3(GROUPS_VT), 0, s, vb);
+    B("k_pen_rays", s);
+    hipLaunchKernelGGL(k_pen_rays, dim3(RAY_BLOCKS), dim3(VT), 0, s, vb);
     E(s);
   }
   // The sections below read the poses k_prep completed and write disjoint accumulators and key

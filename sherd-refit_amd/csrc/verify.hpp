@@ -87,7 +87,11 @@ struct VerifyBatch {
   const uint32_t *task;  // 2 * n_cand (candidate << 1 | side), grouped by the fragment queried
   uint64_t *hard;        // closest-approach queries left to whole waves: candidate << 33 | side << 32 | sample
   uint64_t *hard_a;      // the same for the apart candidates' pass
-  uint32_t *hard_n;      // [0] queued, [1] taken, [2] inside samples listed in deep, [3] queued, [4] taken in hard_a
+  uint32_t *hard_n;      // [0] queued, [1] taken, [2] inside samples listed in deep, [3] queued, [4] taken in hard_a,
+                         // [5] groups listed in boxw
+  uint32_t *boxq;        // per (candidate, side) task, box_cap entries from (2 candidate + side) box_cap: the
+  int box_cap;           // samples inside the other mesh's box (CandAcc::in_box of them), whose rays k_pen_rays casts
+  uint64_t *boxw;        // their groups of 64 entries: (2 candidate + side) << 32 | group
   uint64_t *deep;        // inside samples whose depth k_pen_depth measures, as in hard
   float *deep_t;         // and their nearest ray crossings
   int n_cand;
