@@ -11,7 +11,7 @@ retest (sherd_verify_batch). `value` is pair-verifications per second over the t
 candidates per step); setup (scenes, parity grids, distance bounds, clouds) is timed apart. Beside it:
   * the plain (per-pair / per-candidate) HIP schedule over its own timed steps, the batched/plain
     ratio, and a check that every result is identical;
-  * the roofline of R §6.4's inside pass (k_pen), its launch time from HIP events on its stream over
+  * the roofline of R §6.4's inside pass's box reject (k_pen), its launch time from HIP events on its stream over
     the timed steps, with the measured HBM traffic of the committed counter profile beside it;
   * the CPU baseline: the oracle's end-to-end pipeline (C++, `oracle/`) on a bounded sample of the
     collection's candidates, on all host threads over candidates and on one, median of five;
@@ -990,13 +990,12 @@ def kernel_roofline(kernel, nbytes, avg_ms, pmc_file):
     return out
 
 
-# The collection step's roofline kernel and its §8(d) unit: k_pen, R §6.4's inside pass, the largest
-# kernel with a per-point unit, in bytes per point processed (the f64 sample read, 24 B; its results
-# are reduced in the kernel) over every (candidate, side) task of a watertight pair. The meshes the
-# points are tested against are not counted: a launch reads only the parts near its samples (most
-# samples fail the box reject), and the bytes it does read are the measured traffic beside it. It
-# shares the device with the apart candidates' closest approach (a side stream), so its launch time
-# is the time it takes beside that work.
+# The collection step's roofline kernel and its unit: k_pen, the box reject of R §6.4's inside pass
+# (the largest kernel with a per-point unit; the parity rays of the tenth of samples inside the
+# mesh's box are k_pen_rays', listed with the other kernels), in bytes per point processed (the f64
+# sample read, 24 B; what it lists is a few percent of that) over every (candidate, side) task of a
+# watertight pair. It shares the device with the apart candidates' closest approach (a side
+# stream), so its launch time is the time it takes beside that work.
 POINT_KERNELS = ("k_pen",)
 
 
@@ -1014,7 +1013,7 @@ def collection_roofline(head):
     nbytes = 24 * points[dom]
     roof = kernel_roofline(dom, nbytes, avg_ms, "collection_pmc.json")
     roof["points_per_launch"] = points[dom]
-    roof["unit_bytes"] = "24 B per point processed (the f64 sample), meshes excluded (read only near the samples)"
+    roof["unit_bytes"] = "24 B per point processed (the f64 sample); k_pen reads the meshes' boxes only (k_pen_rays casts the rays)"
     main = [k for k in prof if k not in OVERLAPPED_KERNELS]
     roof["share_of_main_stream_ms"] = round(prof[dom]["ms"] / sum(prof[k]["ms"] for k in main), 4)
     roof["others_ms_per_launch"] = {k: round(v["ms"] / v["launches"], 4) for k, v in prof.items() if k != dom}
@@ -1052,7 +1051,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the pair list would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(the pair list would shard across GPUs by pair, each GPU holding the fragments its "
+              "pairs name; this benchmark times one GPU's share)", file=sys.stderr)
         sys.exit(2)
 
     import sherd_hip
