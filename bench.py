@@ -11,8 +11,9 @@ retest (sherd_verify_batch). `value` is pair-verifications per second over the t
 candidates per step); setup (scenes, parity grids, distance bounds, clouds) is timed apart. Beside it:
   * the plain (per-pair / per-candidate) HIP schedule over its own timed steps, the batched/plain
     ratio, and a check that every result is identical;
-  * the roofline of R §6.4's inside pass's box reject (k_pen), its launch time from HIP events on its stream over
-    the timed steps, with the measured HBM traffic of the committed counter profile beside it;
+  * the roofline of R §6.4's inside pass (k_pen's box reject, k_pen_groups, k_pen_rays' parity rays), its launch
+    times from HIP events on its stream over the timed steps, with the measured HBM traffic of the committed
+    counter profile beside it;
   * the CPU baseline: the oracle's end-to-end pipeline (C++, `oracle/`) on a bounded sample of the
     collection's candidates, on all host threads over candidates and on one, median of five;
   * sections, each with its own oracle sample: configuration 2's vessel (`vessel`: verification,
@@ -35,9 +36,7 @@ sys.path.insert(0, os.path.join(HERE, "sherd-refit_amd"))
 
 METRIC = "pair-verifications/s + rays/s, batched vs plain HIP, 1 MI355X, same accepted seams"
 HBM_PEAK_GBS = 8000.0
-# The vessel section's roofline kernel, R §6.4's inside pass (k_pen); and the three sections that run
-# concurrently on side streams after it.
-ROOFLINE_KERNEL = "k_pen"
+# The three sections that run concurrently on side streams after R §6.4's inside pass.
 OVERLAPPED_KERNELS = ("k_frac", "k_seam", "k_cont")
 
 
@@ -932,8 +931,7 @@ def vessel_section(ex, synth, args):
     log(f"vessel rays: {rays} cone rays at {rays_per_s / 1e6:.1f} M/s; {prays:.0f} parity rays per verify step, "
         f"{prays / (el / args.steps) / 1e6:.1f} M/s batched, {plain_prays / el_plain / 1e6:.2f} M/s plain; "
         f"kNN {knn / 1e6:.1f} M queries/s")
-    avg_ms = prof[ROOFLINE_KERNEL]["ms"] / prof[ROOFLINE_KERNEL]["launches"]
-    nbytes = algorithmic_bytes(ROOFLINE_KERNEL, frags, pairs, cand_pair)
+    nbytes = algorithmic_bytes("k_pen", frags, pairs, cand_pair)
     overlapped = {}
     for k in OVERLAPPED_KERNELS:
         if k in prof:
@@ -941,7 +939,7 @@ def vessel_section(ex, synth, args):
             b_k = algorithmic_bytes(k, frags, pairs, cand_pair)
             overlapped[k] = dict(avg_launch_ms=round(ms_k, 4), algorithmic_bytes_per_launch=b_k,
                                  achieved_gbs=round(b_k / (ms_k * 1e-3) / 1e9, 3))
-    roof = kernel_roofline(ROOFLINE_KERNEL, nbytes, avg_ms, "verify_vessel_pmc.json")
+    roof = kernel_roofline(INSIDE_PASS, nbytes, prof, "verify_vessel_pmc.json")
     roof["overlapped"] = overlapped
     res = dict(workload=f"vessel: 8 fragments, {args.edges_per_t} edges/t, {n_tri} triangles, 28 pairs x 10 "
                         f"candidates", pair_verifications_per_s=round(value, 3), steps=args.steps,
@@ -967,56 +965,62 @@ def vessel_section(ex, synth, args):
     return res, col
 
 
-def kernel_roofline(kernel, nbytes, avg_ms, pmc_file):
-    """The roofline object of one kernel: algorithmic bytes per launch over its average launch time
-    (HIP events on the library's stream over the timed steps) against the HBM peak; beside it the
-    HBM bytes per launch read from a committed rocprofv3 counter summary (FETCH_SIZE, WRITE_SIZE,
-    scaled by the copy-kernel calibration) when one exists for this kernel, and the bandwidth those
+def kernel_roofline(kernels, nbytes, prof, pmc_file):
+    """The roofline object of one pass of consecutive launches on one stream (a tuple of kernel
+    names): algorithmic bytes per pass over the sum of the launches' average times (HIP events on
+    the library's stream over the timed steps) against the HBM peak; beside it the HBM bytes per
+    pass read from a committed rocprofv3 counter summary (FETCH_SIZE, WRITE_SIZE, scaled by the
+    copy-kernel calibration) when it holds every launch of the pass, and the bandwidth those
     measured bytes give."""
+    launch_ms = {k: prof[k]["ms"] / prof[k]["launches"] for k in kernels}
+    avg_ms = sum(launch_ms.values())
     achieved = nbytes / (avg_ms * 1e-3) / 1e9
-    traffic, src = None, None
+    traffic, src, per = None, None, {}
     path = os.path.join(HERE, "profiles", pmc_file)
     if os.path.exists(path):
         with open(path) as fh:
-            traffic = json.load(fh).get("kernels", {}).get(kernel, {}).get("bytes_per_launch")
-        src = f"profiles/{pmc_file} (rocprofv3 --pmc of the same workload, committed)" if traffic else None
+            table = json.load(fh).get("kernels", {})
+        per = {k: table[k]["bytes_per_launch"] for k in kernels if k in table}
+        if len(per) == len(kernels):
+            traffic = int(sum(per.values()))
+            src = f"profiles/{pmc_file} (rocprofv3 --pmc of the same workload, committed)"
     out = dict(bound="hbm", achieved=round(achieved, 3), peak=HBM_PEAK_GBS, unit="GB/s",
-               frac=round(achieved / HBM_PEAK_GBS, 6), traffic=traffic, kernel=kernel,
-               algorithmic_bytes_per_launch=int(nbytes), avg_launch_ms=round(avg_ms, 4))
+               frac=round(achieved / HBM_PEAK_GBS, 6), traffic=traffic, kernel=" + ".join(kernels),
+               algorithmic_bytes_per_launch=int(nbytes), avg_launch_ms=round(avg_ms, 4),
+               launch_ms={k: round(v, 4) for k, v in launch_ms.items()})
     if traffic:
         out["traffic_source"] = src
+        out["traffic_per_launch"] = per
         out["measured_gbs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 3)
         out["measured_frac"] = round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 6)
     return out
 
 
-# The collection step's roofline kernel and its unit: k_pen, the box reject of R §6.4's inside pass
-# (the largest kernel with a per-point unit; the parity rays of the tenth of samples inside the
-# mesh's box are k_pen_rays', listed with the other kernels), in bytes per point processed (the f64
-# sample read, 24 B; what it lists is a few percent of that) over every (candidate, side) task of a
-# watertight pair. It shares the device with the apart candidates' closest approach (a side
-# stream), so its launch time is the time it takes beside that work.
-POINT_KERNELS = ("k_pen",)
+# The roofline of both verification sections is R §6.4's inside pass (the unit of the parity rays):
+# k_pen's box reject, k_pen_groups' prefix sum over its lists and k_pen_rays' parity rays, three
+# consecutive launches on the main stream, in bytes per point processed (the f64 sample, 24 B) over
+# every (candidate, side) task of a watertight pair. k_pen shares the device with the apart
+# candidates' closest approach (a side stream), so its launch time is the time it takes beside it.
+INSIDE_PASS = ("k_pen", "k_pen_groups", "k_pen_rays")
 
 
 def collection_roofline(head):
-    """The roofline of the collection step's per-point kernel (POINT_KERNELS); every other kernel of
-    the step is listed beside it with its time per launch (HIP events on its own stream)."""
+    """The roofline of the collection step's inside pass (INSIDE_PASS); every other kernel of the
+    step is listed beside it with its time per launch (HIP events on its own stream)."""
     prof, coll, pairs, cp, scores = head["prof"], head["coll"], head["pairs"], head["cp"], head["scores"]
     frags = coll["fragments"]
     n_s = np.array([len(frags[a]["S"]) + len(frags[b]["S"]) for a, b in pairs])[cp]
     pen_ok = scores[:, 16] == 0.0
-    points = {"k_pen": int(n_s[pen_ok].sum())}
-    cands = {k: v for k, v in prof.items() if k in POINT_KERNELS and v["launches"]}
-    dom = max(cands, key=lambda k: cands[k]["ms"])
-    avg_ms = prof[dom]["ms"] / prof[dom]["launches"]
-    nbytes = 24 * points[dom]
-    roof = kernel_roofline(dom, nbytes, avg_ms, "collection_pmc.json")
-    roof["points_per_launch"] = points[dom]
-    roof["unit_bytes"] = "24 B per point processed (the f64 sample); k_pen reads the meshes' boxes only (k_pen_rays casts the rays)"
+    points = int(n_s[pen_ok].sum())
+    roof = kernel_roofline(INSIDE_PASS, 24 * points, prof, "collection_pmc.json")
+    roof["points_per_launch"] = points
+    roof["unit_bytes"] = ("24 B per point processed (the f64 sample) over the inside pass's three launches; "
+                          "the meshes the in-box samples' rays scan are in the measured traffic, not the unit")
     main = [k for k in prof if k not in OVERLAPPED_KERNELS]
-    roof["share_of_main_stream_ms"] = round(prof[dom]["ms"] / sum(prof[k]["ms"] for k in main), 4)
-    roof["others_ms_per_launch"] = {k: round(v["ms"] / v["launches"], 4) for k, v in prof.items() if k != dom}
+    roof["share_of_main_stream_ms"] = round(sum(prof[k]["ms"] for k in INSIDE_PASS) /
+                                            sum(prof[k]["ms"] for k in main), 4)
+    roof["others_ms_per_launch"] = {k: round(v["ms"] / v["launches"], 4) for k, v in prof.items()
+                                    if k not in INSIDE_PASS and v["launches"]}
     return roof
 
 

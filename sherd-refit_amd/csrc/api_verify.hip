@@ -391,6 +391,8 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     if (!dboxq) return hip_fail(e, "scratch");
     uint64_t *dboxw = (uint64_t *)c->scratch(11, sizeof(uint64_t) * 2 * nc * (((size_t)std::max(max_s, 1) + 63) / 64), &e);
     if (!dboxw) return hip_fail(e, "scratch");
+    uint32_t *dgstart = (uint32_t *)c->scratch(21, sizeof(uint32_t) * (2 * (size_t)nc + 1), &e);
+    if (!dgstart) return hip_fail(e, "scratch");
     uint64_t *ddeep = (uint64_t *)c->scratch(16, sizeof(uint64_t) * 2 * nc * (size_t)std::max(max_s, 1), &e);
     if (!ddeep) return hip_fail(e, "scratch");
     float *ddeep_t = (float *)c->scratch(17, sizeof(float) * 2 * nc * (size_t)std::max(max_s, 1), &e);
@@ -414,6 +416,7 @@ int sherd_verify_batch(sherd_ctx *ctx, const sherd_verify_pair *pairs, size_t n_
     vb.hard_n = dhard_n;
     vb.boxq = dboxq;
     vb.boxw = dboxw;
+    vb.gstart = dgstart;
     vb.box_cap = std::max(max_s, 1);
     vb.deep = ddeep;
     vb.deep_t = ddeep_t;

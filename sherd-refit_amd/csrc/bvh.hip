@@ -364,7 +364,9 @@ void scene_free(DevScene *s) {
   s->tris = nullptr;
   for (AxisGrid &g : s->pg) {
     if (g.start) (void)hipFree(g.start);
-    if (g.item) (void)hipFree(g.item);
+    if (g.ix) (void)hipFree(g.ix);
+    if (g.iy) (void)hipFree(g.iy);
+    if (g.iz) (void)hipFree(g.iz);
     g = AxisGrid{};
   }
   if (s->db.lb) (void)hipFree(s->db.lb);
@@ -553,9 +555,16 @@ __global__ void k_pg_fill(const Tri *tris, int m, AxisGrid g, int axis, float pa
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
   int u0, u1, v0, v1;
-  pg_range(tris[j], axis, g, pad, &u0, &u1, &v0, &v1);
+  const Tri t = tris[j];
+  pg_range(t, axis, g, pad, &u0, &u1, &v0, &v1);
+  const float4 x = make_float4(t.v[0], t.v[1], t.v[2], t.v[3]), y = make_float4(t.v[4], t.v[5], t.v[6], t.v[7]);
   for (int cu = u0; cu <= u1; ++cu)
-    for (int cv = v0; cv <= v1; ++cv) g.item[atomicAdd(&cursor[(size_t)cu * g.nv + cv], 1u)] = (uint32_t)j;
+    for (int cv = v0; cv <= v1; ++cv) {
+      uint32_t k = atomicAdd(&cursor[(size_t)cu * g.nv + cv], 1u);
+      g.ix[k] = x;
+      g.iy[k] = y;
+      g.iz[k] = t.v[8];
+    }
 }
 
 }  // namespace
@@ -674,7 +683,9 @@ hipError_t scene_parity_grids(DevScene *sc, hipStream_t s) {
         err = hipErrorInvalidValue;
         break;
       }
-      if ((err = dalloc(&g.item, total)) != hipSuccess) break;
+      if ((err = dalloc(&g.ix, total)) != hipSuccess) break;
+      if ((err = dalloc(&g.iy, total)) != hipSuccess) break;
+      if ((err = dalloc(&g.iz, total)) != hipSuccess) break;
       if ((err = hipMemcpyAsync(cursor, g.start, sizeof(uint32_t) * ncell, hipMemcpyDeviceToDevice, s)) != hipSuccess)
         break;
       hipLaunchKernelGGL(k_pg_fill, dim3(blocks_for(m, BT)), dim3(BT), 0, s, sc->tris, m, g, axis, pad, cursor);
@@ -688,7 +699,9 @@ hipError_t scene_parity_grids(DevScene *sc, hipStream_t s) {
   if (err != hipSuccess) {
     for (AxisGrid &g : sc->pg) {
       if (g.start) (void)hipFree(g.start);
-      if (g.item) (void)hipFree(g.item);
+      if (g.ix) (void)hipFree(g.ix);
+      if (g.iy) (void)hipFree(g.iy);
+      if (g.iz) (void)hipFree(g.iz);
       g = AxisGrid{};
     }
   }

@@ -217,7 +217,9 @@ __device__ inline void first_hit(const SceneView &sv, V3 o, V3 d, int retest, fl
 //
 // A cell's items are independent of one another, so they are taken XCH at a time with every load
 // of the group issued before any test: a long cell (a wall seen edge-on lists hundreds) then costs
-// a few memory round trips per group rather than two per triangle. The count and the minimum do
+// a few memory round trips per group. The grid keeps each item's coordinates in its cell's run, so
+// a ray reads its cell as three contiguous arrays with no index to follow (scattered 48-byte
+// records behind an index list moved 3.6 GB per collection step). The count and the minimum do
 // not depend on the order.
 constexpr int XCH = 4;
 
@@ -250,26 +252,18 @@ __device__ inline uint32_t crossings_axis(const SceneView &sv, V3 o, int retest,
   float tmin = __builtin_inff();
   uint32_t k = b;
   for (; k + XCH <= e; k += XCH) {
-    uint32_t idx[XCH];
-#pragma unroll
-    for (int u = 0; u < XCH; ++u) idx[u] = gldu(g.item + k + u);
     float4 x[XCH], y[XCH];
     float zx[XCH];
 #pragma unroll
     for (int u = 0; u < XCH; ++u) {
-      const float4 *p = reinterpret_cast<const float4 *>(sv.tris + idx[u]);
-      x[u] = gld4(p);
-      y[u] = gld4(p + 1);
-      zx[u] = gldf(reinterpret_cast<const float *>(p) + 8);
+      x[u] = gld4(g.ix + k + u);
+      y[u] = gld4(g.iy + k + u);
+      zx[u] = gldf(g.iz + k + u);
     }
 #pragma unroll
     for (int u = 0; u < XCH; ++u) crossing_test<AXIS>(o, retest, pad, x[u], y[u], zx[u], &hits, &tmin);
   }
-  for (; k < e; ++k) {
-    const float4 *p = reinterpret_cast<const float4 *>(sv.tris + gldu(g.item + k));
-    crossing_test<AXIS>(o, retest, pad, gld4(p), gld4(p + 1), gldf(reinterpret_cast<const float *>(p) + 8), &hits,
-                        &tmin);
-  }
+  for (; k < e; ++k) crossing_test<AXIS>(o, retest, pad, gld4(g.ix + k), gld4(g.iy + k), gldf(g.iz + k), &hits, &tmin);
   *nearest = tmin;
   return hits;
 }

@@ -195,8 +195,11 @@ struct Tri {
 // cell (cu, cv) of a uniform grid over the scene's box lists every triangle whose box, grown by
 // the pruning slack, overlaps it. A +axis ray's crossings are the hits among its cell's triangles.
 struct AxisGrid {
-  uint32_t *start;  // nu * nv + 1 offsets into item
-  uint32_t *item;   // triangle slots
+  uint32_t *start;  // nu * nv + 1 offsets into the item arrays
+  // Each listed triangle's coordinates, copied into its cell's run so that a ray reads its cell
+  // contiguously: v0.xyz v1.x | v1.yz v2.xy | v2.z (the Tri record's v[0..8]).
+  float4 *ix, *iy;
+  float *iz;
   float org_u, org_v, inv;
   int nu, nv;
 };

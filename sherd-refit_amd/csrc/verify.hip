@@ -510,18 +510,20 @@ __global__ void __launch_bounds__(VT) k_pen(VerifyBatch vb, int per_task) {
 
 // The groups of 64 entries of every task's in-box list, in the tasks' order (by the fragment they
 // query), into boxw, and their number into hard_n[5]: one workgroup, each thread a run of tasks,
-// their offsets by a prefix sum in LDS. (A global counter that every group took a slot of cost more
-// than this launch: one address's atomics are serialised across the device.)
+// their offsets by a prefix sum in LDS into gstart, then the groups written by whole waves. (A
+// global counter that every group took a slot of cost more than this launch: one address's atomics
+// are serialised across the device.)
 constexpr int GROUPS_VT = 1024;
 __global__ void __launch_bounds__(GROUPS_VT) k_pen_groups(VerifyBatch vb) {
   __shared__ uint32_t s_sum[GROUPS_VT];
   const int nt = 2 * vb.n_cand, per = (nt + GROUPS_VT - 1) / GROUPS_VT;
   const int t0 = (int)threadIdx.x * per, t1 = t0 + per < nt ? t0 + per : nt;
-  uint32_t mine = 0;
-  for (int j = t0; j < t1; ++j) {
+  auto groups = [&](int j) {
     const uint32_t cs = vb.task[j];
-    mine += (vb.acc[cs >> 1].in_box[cs & 1u] + 63u) >> 6;
-  }
+    return (vb.acc[cs >> 1].in_box[cs & 1u] + 63u) >> 6;
+  };
+  uint32_t mine = 0;
+  for (int j = t0; j < t1; ++j) mine += groups(j);
   s_sum[threadIdx.x] = mine;
   __syncthreads();
   for (int off = 1; off < GROUPS_VT; off <<= 1) {  // inclusive scan
@@ -532,10 +534,27 @@ __global__ void __launch_bounds__(GROUPS_VT) k_pen_groups(VerifyBatch vb) {
   }
   uint32_t at = s_sum[threadIdx.x] - mine;
   if (threadIdx.x == GROUPS_VT - 1) vb.hard_n[5] = s_sum[threadIdx.x];
-  for (int j = t0; j < t1; ++j) {
-    const uint32_t cs = vb.task[j];
-    const uint32_t g = (vb.acc[cs >> 1].in_box[cs & 1u] + 63u) >> 6;
-    for (uint32_t q = 0; q < g; ++q) vb.boxw[at++] = ((uint64_t)cs << 32) | q;
+  for (int j = t0; j < t1; ++j) {  // each task's first group
+    vb.gstart[j] = at;
+    at += groups(j);
+  }
+  if (t0 < t1 && t1 == nt) vb.gstart[nt] = at;
+  __syncthreads();
+  // The groups written by whole waves: a wave takes 64 tasks at a time, one per lane, and its lanes
+  // then write each of those tasks' groups as one contiguous run (in-box counts are skewed: one
+  // thread per task range, writing its groups in turn, waited on its largest tasks for 0.19 ms).
+  const uint32_t lane = threadIdx.x & 63;
+  for (int j0 = (int)(threadIdx.x >> 6) * 64; j0 < nt; j0 += GROUPS_VT) {
+    const int j = j0 + (int)lane;
+    const uint32_t st = j < nt ? vb.gstart[j] : 0u, g = j < nt ? vb.gstart[j + 1] - st : 0u;
+    const uint32_t cs = j < nt ? vb.task[j] : 0u;
+    uint64_t todo = __ballot(g != 0u);
+    while (todo) {
+      const int l = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const uint32_t sl = __shfl(st, l), gl = __shfl(g, l), cl = __shfl(cs, l);
+      for (uint32_t q = lane; q < gl; q += 64) vb.boxw[sl + q] = ((uint64_t)cl << 32) | q;
+    }
   }
 }
 
@@ -875,9 +894,13 @@ __global__ void __launch_bounds__(VT) k_pen_close(VerifyBatch vb, int apart) {
   }
 }
 
-// The k-th smallest key (0-based) of n by an 8-pass MSD radix select.
+// The k-th smallest key (0-based) of n by an MSD radix select over 8-bit digits, which stops as soon
+// as the chosen bucket holds a single key (that key is then found by one more scan). *rank gets the
+// k-th key's rank among the keys equal to it: 0 exactly when fewer than k + 1 keys are at most it
+// and k keys are below it.
 template <class K>
-__device__ K select_kth(const K *keys, int n, int k, uint32_t *hist, K *s_prefix, int *s_k) {
+__device__ K select_kth(const K *keys, int n, int k, uint32_t *hist, K *s_prefix, int *s_k, uint32_t *s_cnt,
+                        int *rank) {
   constexpr int PASSES = (int)sizeof(K);
   K prefix = 0, pmask = 0;
   if (threadIdx.x == 0) *s_k = k;
@@ -895,36 +918,41 @@ __device__ K select_kth(const K *keys, int n, int k, uint32_t *hist, K *s_prefix
       // over the lanes' totals finds the lane, then that lane walks its four bins.
       int lane = threadIdx.x;
       int kk = *s_k;
-      uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2], h3 = hist[4 * lane + 3];
-      uint32_t tot = h0 + h1 + h2 + h3, incl = tot;
+      uint32_t h[4] = {hist[4 * lane], hist[4 * lane + 1], hist[4 * lane + 2], hist[4 * lane + 3]};
+      uint32_t tot = h[0] + h[1] + h[2] + h[3], incl = tot;
       for (int o = 1; o < 64; o <<= 1) {
         uint32_t v = (uint32_t)__shfl_up((int)incl, o);
         if (lane >= o) incl += v;
       }
       uint32_t excl = incl - tot;
       if ((uint32_t)kk >= excl && (uint32_t)kk < incl) {
-        int r = kk - (int)excl, d = 4 * lane;
-        if (r >= (int)h0) {
-          r -= (int)h0;
+        int r = kk - (int)excl, d = 0;
+        while (r >= (int)h[d]) {  // ends within the four bins: r < tot
+          r -= (int)h[d];
           ++d;
-          if (r >= (int)h1) {
-            r -= (int)h1;
-            ++d;
-            if (r >= (int)h2) {
-              r -= (int)h2;
-              ++d;
-            }
-          }
         }
         *s_k = r;
-        *s_prefix = prefix | ((K)d << shift);
+        *s_cnt = h[d];
+        *s_prefix = prefix | ((K)(4 * lane + d) << shift);
       }
     }
     __syncthreads();
     prefix = *s_prefix;
     pmask |= ((K)255 << shift);
+    const uint32_t cnt = *s_cnt;
     __syncthreads();
+    if (cnt == 1u && pass + 1 < PASSES) {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        K key = keys[i];
+        if ((key & pmask) == prefix) *s_prefix = key;
+      }
+      __syncthreads();
+      prefix = *s_prefix;
+      break;
+    }
   }
+  *rank = *s_k;
+  __syncthreads();
   return prefix;
 }
 
@@ -996,13 +1024,16 @@ __device__ double midpoint(double a, double b) {
 // verify.rs:693-700 numpy's median over each candidate's four segments.
 // Statistics seg0 and seg0 + 1 of every candidate: the facing distances of R §6.1 (0, 1), or R
 // §6.3's steps and agreements (2, 3).
-__global__ void __launch_bounds__(VT) k_median(VerifyBatch vb, int seg0) {
+constexpr int MEDIAN_VT = 512;
+__global__ void __launch_bounds__(MEDIAN_VT) k_median(VerifyBatch vb, int seg0) {
   int c = blockIdx.x >> 1, seg = seg0 + (int)(blockIdx.x & 1);
   CandAcc &acc = vb.acc[c];
   int n = seg < 2 ? (int)acc.facing[seg] : (int)acc.n_cont;
   __shared__ uint32_t hist[256];
   __shared__ uint64_t s_prefix;
+  __shared__ unsigned long long s_max;
   __shared__ int s_k;
+  __shared__ uint32_t s_cnt;
   __shared__ uint64_t lds_keys[MEDIAN_LDS];
   if (n == 0) {
     if (threadIdx.x == 0) acc.med[seg] = __builtin_nan("");
@@ -1010,17 +1041,32 @@ __global__ void __launch_bounds__(VT) k_median(VerifyBatch vb, int seg0) {
   }
   const uint64_t *keys = vb.keys + vb.cand[c].seg[seg];
   if (n <= MEDIAN_LDS) {
-    for (int i = threadIdx.x; i < n; i += VT) lds_keys[i] = keys[i];
+    for (int i = threadIdx.x; i < n; i += MEDIAN_VT) lds_keys[i] = keys[i];
     __syncthreads();
     keys = lds_keys;
   }
+  int rank;
+  const uint64_t hi = select_kth(keys, n, n / 2, hist, &s_prefix, &s_k, &s_cnt, &rank);
   double med;
   if (n % 2 == 1) {
-    med = from_total_key(select_kth(keys, n, n / 2, hist, &s_prefix, &s_k));
+    med = from_total_key(hi);
   } else {
-    double lo = from_total_key(select_kth(keys, n, n / 2 - 1, hist, &s_prefix, &s_k));
-    double hi = from_total_key(select_kth(keys, n, n / 2, hist, &s_prefix, &s_k));
-    med = midpoint(lo, hi);
+    // The lower middle key: the upper one when a key equal to it comes before it, else the
+    // largest key below it (exactly n / 2 keys are below it then).
+    uint64_t lo = hi;
+    if (rank == 0) {
+      if (threadIdx.x == 0) s_max = 0ull;
+      __syncthreads();
+      unsigned long long m = 0ull;
+      for (int i = threadIdx.x; i < n; i += MEDIAN_VT) {
+        const uint64_t key = keys[i];
+        if (key < hi && key > m) m = key;
+      }
+      atomicMax(&s_max, m);
+      __syncthreads();
+      lo = s_max;
+    }
+    med = midpoint(from_total_key(lo), from_total_key(hi));
   }
   if (threadIdx.x == 0) acc.med[seg] = med;
 }
@@ -1142,7 +1188,9 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
     B("k_pen", s);
     hipLaunchKernelGGL(k_pen, dim3(2 * nc * per_sg), dim3(VT), 0, s, vb, per_sg);
     E(s);
+    B("k_pen_groups", s);
     hipLaunchKernelGGL(k_pen_groups, dim3(1), dim3(GROUPS_VT), 0, s, vb);
+    E(s);
     B("k_pen_rays", s);
     hipLaunchKernelGGL(k_pen_rays, dim3(RAY_BLOCKS), dim3(VT), 0, s, vb);
     E(s);
@@ -1167,7 +1215,7 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
   }
   // Each statistic's medians right after the section that fills its keys, on that section's stream.
   B("k_median", s1);
-  hipLaunchKernelGGL(k_median, dim3(2 * nc), dim3(VT), 0, s1, vb, 0);
+  hipLaunchKernelGGL(k_median, dim3(2 * nc), dim3(MEDIAN_VT), 0, s1, vb, 0);
   E(s1);
   // The depths of the inside samples need only k_pen's list: ahead of R §6.2 on its stream.
   if (pen) {
@@ -1189,7 +1237,7 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
     E(s3);
   }
   B("k_median", s3);
-  hipLaunchKernelGGL(k_median, dim3(2 * nc), dim3(VT), 0, s3, vb, 2);
+  hipLaunchKernelGGL(k_median, dim3(2 * nc), dim3(MEDIAN_VT), 0, s3, vb, 2);
   E(s3);
   if (pen) {
     B("k_pen_seed", s);

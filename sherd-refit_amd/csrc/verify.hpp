@@ -14,7 +14,9 @@ enum {
   VS_BRK_BEST, VS_COUNT
 };
 
-constexpr int MEDIAN_LDS = 12288;   // a median segment up to this many keys is selected from LDS
+constexpr int MEDIAN_LDS = 2048;  // a median segment up to this many keys is selected from LDS (16 KB, so
+                                  // the median's workgroups fit beside the side streams' kernels on a
+                                  // CU; a larger segment is read from L2 on each pass)
 constexpr uint32_t FLAG_VOXEL_RANGE = 1u;  // a seam voxel coordinate beyond +-2^20
 constexpr uint32_t FLAG_GAVE_UP = 2u;      // a closest-approach query ran out of its step budget
 
@@ -92,6 +94,7 @@ struct VerifyBatch {
   uint32_t *boxq;        // per (candidate, side) task, box_cap entries from (2 candidate + side) box_cap: the
   int box_cap;           // samples inside the other mesh's box (CandAcc::in_box of them), whose rays k_pen_rays casts
   uint64_t *boxw;        // their groups of 64 entries: (2 candidate + side) << 32 | group
+  uint32_t *gstart;      // per task (in task order) the first of its groups in boxw, and the total
   uint64_t *deep;        // inside samples whose depth k_pen_depth measures, as in hard
   float *deep_t;         // and their nearest ray crossings
   int n_cand;
