@@ -28,13 +28,13 @@ __global__ void __launch_bounds__(256) k_face_normals(const double *V, const uin
 
 constexpr int CONE_RAYS = 7;
 constexpr int CONE_FACES_PER_WAVE = 64 / CONE_RAYS;  // 9 faces, 63 lanes; the last lane idles
-constexpr int CONE_LSTACK = 12;                      // deferred nodes per ray, in LDS
+constexpr int CONE_LSTACK = 11;                      // deferred nodes per ray, in LDS (22 KB: 7 workgroups per CU)
 
 // One lane per ray: a wave takes nine faces, its lanes 7j..7j+6 cast face j's seven rays (straight
 // down -NS, then the six of the 15 degree cone) from the same origin, so a face's traversals run
 // side by side through the same nodes. Each lane builds the face's basis itself (the same doubles
 // on every lane); a face's votes are a popcount of its seven bits of the wave's ballot.
-__global__ void __launch_bounds__(256) k_cone(SceneView sv, const double *V, const uint32_t *F, const double *fnv,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k_cone(SceneView sv, const double *V, const uint32_t *F, const double *fnv,
                                               int nf, const int64_t *faces, const double *ns, int n, ConeConst cc,
                                               int retest, uint8_t *good) {
   __shared__ uint2 s_stk[CONE_LSTACK][256];
