@@ -47,7 +47,9 @@ constexpr int HARD_BLOCKS = 1024;  // workgroups of the cooperative pass
 // FLAG_GAVE_UP and the batch fails with a status (the caller answers it on the CPU), so the kernel
 // always ends in bounded time.
 constexpr int WAVE_STEPS = 1 << 19;
-constexpr int DEPTH_BLOCKS = 1024;  // workgroups of k_pen_depth, which loop over the listed samples
+constexpr int DEPTH_VT = 64;        // one wave per workgroup of k_pen_depth: its 16 KB of LDS stacks fit beside
+                                    // the side streams' kernels on a CU (256 lanes held 64 KB, 2 waves per SIMD)
+constexpr int DEPTH_BLOCKS = 4096;  // workgroups of k_pen_depth, which loop over the listed samples
 // k_pen_depth's deferred nodes: its samples fill few waves, so a deep LDS stack (64 KiB per
 // workgroup) costs no occupancy they could use, and spares the overflow walk.
 constexpr int DEPTH_LSTACK = 32;
@@ -599,11 +601,11 @@ __global__ void __launch_bounds__(VT) k_pen_rays(VerifyBatch vb) {
 // R §6.4's depths of the inside samples k_pen listed, one lane each, so the searches run in full
 // waves: the distance to the nearest triangle (the same search RayScene::inside's depth makes),
 // the count deeper than pen and the deepest per side.
-__global__ void __launch_bounds__(VT) k_pen_depth(VerifyBatch vb) {
-  __shared__ uint2 s_stk[DEPTH_LSTACK][VT];
-  LdsStack<VT, DEPTH_LSTACK> st{&s_stk[0][threadIdx.x]};
+__global__ void __launch_bounds__(DEPTH_VT) k_pen_depth(VerifyBatch vb) {
+  __shared__ uint2 s_stk[DEPTH_LSTACK][DEPTH_VT];
+  LdsStack<DEPTH_VT, DEPTH_LSTACK> st{&s_stk[0][threadIdx.x]};
   const uint32_t n = vb.hard_n[2];
-  for (uint32_t k = blockIdx.x * VT + threadIdx.x; k < n; k += gridDim.x * VT) {
+  for (uint32_t k = blockIdx.x * DEPTH_VT + threadIdx.x; k < n; k += gridDim.x * DEPTH_VT) {
     const uint64_t e = vb.deep[k];
     const int c = (int)(e >> 33), side = (int)((e >> 32) & 1u), i = (int)(uint32_t)e;
     const CandDesc &cd = vb.cand[c];
@@ -1220,7 +1222,7 @@ hipError_t launch_verify(const VerifyBatch &vb, int max_pf, int max_s, int max_m
   // The depths of the inside samples need only k_pen's list: ahead of R §6.2 on its stream.
   if (pen) {
     B("k_pen_depth", s2);
-    hipLaunchKernelGGL(k_pen_depth, dim3(DEPTH_BLOCKS), dim3(VT), 0, s2, vb);
+    hipLaunchKernelGGL(k_pen_depth, dim3(DEPTH_BLOCKS), dim3(DEPTH_VT), 0, s2, vb);
     E(s2);
   }
   B("k_seam_clear", s2);
